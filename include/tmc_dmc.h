@@ -1,0 +1,146 @@
+/*
+ * tmc_dmc.h -- C ABI of the MI355X Dual Marching Cubes engine (libtmc_dmc.so).
+ *
+ * This is the drop-in boundary for the reference's extraction path.  The
+ * reference has no library boundary of its own; its two call surfaces are
+ *   cpp_dmc/DualMarchingCubes.h:163-165
+ *     void dmc::DualMarchingCubes::dualMC(const double i0, UGrid& ugrid,
+ *          std::vector<Vertex>& v, std::vector<Normal>& n,
+ *          std::vector<int>& tris, std::vector<int>& quads, const bool sFlag);
+ *   dmc/DualMarchingCubes.h:74-77
+ *     void p_mc::DualMarchingCubes::dualMC(const float i0, v, n, t, q,
+ *          he, hef, hev, std::map<std::string,int>& config);
+ * plus the grid set-up of cpp_dmc/UniformGrid.h (init/scalar/gradient) and
+ * getBndVertices (cpp_dmc/DualMarchingCubes.h:171-182).  Each entry point
+ * below names the reference interface it replaces.  include/tmc_dmc.hpp
+ * rebuilds the reference's C++ classes on top of this ABI.
+ *
+ * All entry points return a dmc_status; none of them exits the process
+ * (the reference exits on CUDA and file errors, dmc/helper_cuda.h:88-98,
+ * cpp_dmc/UniformGrid.cpp:7-9).  A context owns one HIP stream and its
+ * device buffers; calls on one context must not overlap.
+ */
+#ifndef TMC_DMC_H
+#define TMC_DMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DMC_OK = 0,
+    DMC_ERR_INVALID_ARG = 1,   /* bad pointer, size or flag */
+    DMC_ERR_OUT_OF_MEMORY = 2, /* device allocation failed */
+    DMC_ERR_CAPACITY = 3,      /* a count exceeds the 32-bit index range */
+    DMC_ERR_HIP = 4,           /* a HIP runtime call failed */
+    DMC_ERR_NO_DEVICE = 5,     /* no usable GPU */
+    DMC_ERR_BUDGET = 6,        /* a request does not fit the memory budget */
+    DMC_ERR_STATE = 7          /* call out of order (e.g. download before extract) */
+} dmc_status;
+
+typedef struct dmc_context dmc_context;
+
+/* Uniform grid description: nx*ny*nz points, x fastest, point (i,j,k) at
+ * (x0 + i*dx, y0 + j*dy, z0 + k*dz).  Replaces the geometry part of
+ * dmc::UniformGrid (cpp_dmc/UniformGrid.h:18-354). */
+typedef struct {
+    int32_t nx, ny, nz;
+    double x0, y0, z0;
+    double dx, dy, dz;
+} dmc_grid;
+
+/* field element types */
+#define DMC_FIELD_F32 0
+#define DMC_FIELD_F64 1
+
+/* extraction flags (the CUDA reference's config keys, dmc/kernel.cu:25-32) */
+#define DMC_SIMPLIFY_3X3Y_COLOR    (1u << 0)  /* "p3X3YColor" */
+#define DMC_SIMPLIFY_3X3Y_ISOLATED (1u << 1)  /* "p3X3YOld"   */
+#define DMC_SIMPLIFY_3333          (1u << 2)  /* "p3333"      */
+#define DMC_TRIANGLES              (1u << 3)  /* collectTriangles */
+#define DMC_GRADIENTS_SUPPLIED     (1u << 4)  /* normals from caller gradients */
+#define DMC_FIELD_ON_DEVICE        (1u << 5)  /* field/gradients are device pointers */
+#define DMC_STAGE_TIMING           (1u << 6)  /* record per-stage HIP events */
+/* cpp sFlag == true (cpp_dmc/DualMarchingCubes.cpp:62-76) */
+#define DMC_SIMPLIFY_ALL (DMC_SIMPLIFY_3X3Y_COLOR | DMC_SIMPLIFY_3X3Y_ISOLATED | DMC_SIMPLIFY_3333)
+
+/* Result of one extraction.  Device pointers are owned by the context and
+ * stay valid until the next dmc_extract or dmc_destroy on it. */
+typedef struct {
+    int64_t nv;                 /* vertices */
+    int64_t nq;                 /* quadrilaterals */
+    int64_t nt;                 /* triangles (DMC_TRIANGLES) */
+    const double* d_positions;  /* 3*nv, FP64 */
+    const double* d_normals;    /* 3*nv, FP64, unit length */
+    const int32_t* d_quads;     /* 4*nq vertex ids */
+    const int32_t* d_tris;      /* 3*nt vertex ids */
+    const int32_t* d_twins;     /* 4*nq: twin of halfedge 4q+k, -1 on the boundary */
+    const uint8_t* d_he_flags;  /* 4*nq: 0 manifold, 1 non-manifold, 2 multiple boundary edge */
+    const int32_t* d_he_v;      /* nv: a halfedge leaving each vertex */
+    int64_t nonmanifold_edges;  /* edges with 3 or 4 faces in the final mesh */
+    int64_t nbnd;               /* entries of getBndVertices() */
+    int64_t edge_errors;        /* edges with more than 4 faces (left unpaired) */
+    int64_t invalid_refs;       /* quad vertex ids out of range after simplification */
+    int64_t valence_zero;       /* vertices without faces at the last valence pass */
+    uint32_t failed_projections_l1;  /* nrProj1 (cpp_dmc/DualMarchingCubes.cpp:51) */
+    uint32_t failed_projections_l2;  /* nrProj2 (:52) */
+    int64_t active_cells;       /* cells that are neither all-in nor all-out */
+    int64_t nv_emitted;         /* vertices produced by the cell sweep */
+    int64_t nv_cells;           /* vertices after removeUnusedVertices */
+    int64_t nq_cells;           /* quadrilaterals before simplification */
+} dmc_mesh;
+
+/* Create a context on HIP device `device`. */
+dmc_status dmc_create(dmc_context** ctx, int device);
+/* Release a context and all of its device memory. */
+dmc_status dmc_destroy(dmc_context* ctx);
+
+/* Extract the iso-surface of `field` at `iso`.  Replaces
+ * cpp_dmc/DualMarchingCubes.h:163-165 (dualMC, sFlag -> DMC_SIMPLIFY_ALL)
+ * and dmc/DualMarchingCubes.h:74-77 (dualMC with its config map).
+ * field_type is DMC_FIELD_F32 or DMC_FIELD_F64.  Without
+ * DMC_GRADIENTS_SUPPLIED the normals use the gradient of
+ * UniformGrid::estimateGradient + flip_gradient (cpp_dmc/UniformGrid.cpp:
+ * 156-197), computed on the fly; with it, `gradients` holds 3 doubles per
+ * grid point as cpp_dmc's UniformGrid stores them.  The mesh stays on the
+ * device; `out` receives counts and device pointers. */
+dmc_status dmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type,
+                       double iso, uint32_t flags, const double* gradients, dmc_mesh* out);
+
+/* Copy the last mesh to host memory; any pointer may be NULL.
+ * positions/normals: 3*nv doubles; quads 4*nq; tris 3*nt;
+ * halfedges: 5 ints per halfedge {origin, face, next, twin, flag} as in
+ * cpp_dmc/DualMarchingCubes.h:115-120 and dmc/DualMarchingCubes.cu:1098-1119;
+ * he_v nv; he_f nq; bnd nbnd (cpp_dmc/DualMarchingCubes.h:171-182). */
+dmc_status dmc_mesh_download(dmc_context* ctx, double* positions, double* normals, int32_t* quads,
+                             int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f,
+                             int32_t* bnd);
+
+/* FP32 export as the CUDA reference returns it (dmc/DualMarchingCubes.cu:
+ * 1076-1090): positions and normals as float triples; when negate_normals
+ * is non-zero the normals are negated like the CUDA path does. */
+dmc_status dmc_mesh_download_f32(dmc_context* ctx, float* positions, float* normals, int negate_normals);
+
+/* Per-stage device times (ms) of the last extraction run with
+ * DMC_STAGE_TIMING.  names[i] point to static strings.  *count is in/out:
+ * capacity in, number of stages out. */
+dmc_status dmc_stage_times(dmc_context* ctx, const char** names, float* ms, int* count);
+
+/* Device memory helpers for callers without their own HIP runtime. */
+dmc_status dmc_device_alloc(dmc_context* ctx, size_t bytes, void** ptr);
+dmc_status dmc_device_free(dmc_context* ctx, void* ptr);
+dmc_status dmc_copy_to_device(dmc_context* ctx, void* dst, const void* src, size_t bytes);
+dmc_status dmc_copy_to_host(dmc_context* ctx, void* dst, const void* src, size_t bytes);
+dmc_status dmc_synchronize(dmc_context* ctx);
+
+const char* dmc_status_str(dmc_status s);
+/* Text of the last error on this context (empty string if none). */
+const char* dmc_last_error(dmc_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -47,6 +47,13 @@ class _HE(C.Structure):
                 ("errors", C.c_int64)]
 
 
+def _g(grid):
+    """Accept any object with the grid fields (e.g. tmc_amd.Grid)."""
+    if isinstance(grid, Grid):
+        return grid
+    return Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+
+
 def _arr(ptr, n, dtype):
     if n == 0:
         return np.zeros(0, dtype)
@@ -99,6 +106,8 @@ class Oracle:
         L.dmco_triangles.argtypes = [C.POINTER(_Mesh)]
         L.dmco_dualmc.argtypes = [C.POINTER(Grid), f64p, f64p, C.c_double, C.c_int,
                                   C.POINTER(_Mesh), C.POINTER(_HE)]
+        L.dmco_dualmc_flags.argtypes = [C.POINTER(Grid), f64p, f64p, C.c_double, C.c_int,
+                                        C.POINTER(_Mesh), C.POINTER(_HE)]
         L.dmco_cell_polygons.argtypes = [f64p, C.c_double, i32p, i32p]
         L.dmco_cell_vertices.argtypes = [f64p, C.c_double, f64p, u32p, u32p]
 
@@ -106,14 +115,14 @@ class Oracle:
     def gradient(self, grid, scalars, flip=True):
         s = np.ascontiguousarray(scalars, dtype=np.float64)
         g = np.zeros(s.size * 3, np.float64)
-        self.lib.dmco_estimate_gradient(C.byref(grid), _p(s, f64p), _p(g, f64p), int(flip))
+        self.lib.dmco_estimate_gradient(C.byref(_g(grid)), _p(s, f64p), _p(g, f64p), int(flip))
         return g
 
     # -- stages -------------------------------------------------------------
     def cells(self, grid, scalars, grad, iso):
         s = np.ascontiguousarray(scalars, dtype=np.float64)
         m = _Mesh()
-        self.lib.dmco_cells(C.byref(grid), _p(s, f64p), _p(grad, f64p), float(iso), C.byref(m))
+        self.lib.dmco_cells(C.byref(_g(grid)), _p(s, f64p), _p(grad, f64p), float(iso), C.byref(m))
         out = _mesh_from(m)
         self.lib.dmco_mesh_free(C.byref(m))
         return out
@@ -227,12 +236,16 @@ class Oracle:
         self.lib.dmco_mesh_free(C.byref(m))
         return out["tris"] if "tris" in out else np.zeros((0, 3), np.int32)
 
-    def dualmc(self, grid, scalars, grad, iso, sflag=True):
+    def dualmc(self, grid, scalars, grad, iso, sflag=True, flags=None):
+        """Whole path.  flags (engine bits: 1 colored 3X3Y, 2 isolated 3X3Y,
+        4 3333, 8 triangles) overrides sflag."""
         s = np.ascontiguousarray(scalars, dtype=np.float64)
         m = _Mesh()
         h = _HE()
-        self.lib.dmco_dualmc(C.byref(grid), _p(s, f64p), _p(grad, f64p), float(iso), int(sflag),
-                             C.byref(m), C.byref(h))
+        if flags is None:
+            flags = (7 if sflag else 0) | 8
+        self.lib.dmco_dualmc_flags(C.byref(_g(grid)), _p(s, f64p), _p(grad, f64p), float(iso), int(flags),
+                                   C.byref(m), C.byref(h))
         out = _mesh_from(m)
         out["he"] = _arr(h.he, 5 * h.nhe, np.int32).reshape(-1, 5)
         out["he_v"] = _arr(h.he_v, out["v"].shape[0], np.int32)
@@ -303,7 +316,7 @@ class Reference:
     def gradient(self, grid, scalars, flip=True):
         s = np.ascontiguousarray(scalars, dtype=np.float64)
         g = np.zeros(s.size * 3, np.float64)
-        self.lib.dmcref_estimate_gradient(C.byref(grid), _p(s, f64p), _p(g, f64p), int(flip))
+        self.lib.dmcref_estimate_gradient(C.byref(_g(grid)), _p(s, f64p), _p(g, f64p), int(flip))
         return g
 
     def cells(self, grid, scalars, grad, iso):
@@ -312,7 +325,7 @@ class Reference:
         v, n = f64p(), f64p()
         q, c = i32p(), i32p()
         p1, p2 = C.c_uint32(), C.c_uint32()
-        self.lib.dmcref_cells(C.byref(grid), _p(s, f64p), _p(grad, f64p), float(iso), C.byref(nv),
+        self.lib.dmcref_cells(C.byref(_g(grid)), _p(s, f64p), _p(grad, f64p), float(iso), C.byref(nv),
                               C.byref(v), C.byref(n), C.byref(nq), C.byref(q), C.byref(c),
                               C.byref(p1), C.byref(p2), C.byref(nvb))
         out = Mesh()
@@ -420,7 +433,7 @@ class Reference:
         nv, nq, nt = C.c_int64(), C.c_int64(), C.c_int64()
         v, n = f64p(), f64p()
         q, t = i32p(), i32p()
-        self.lib.dmcref_dualmc(C.byref(grid), _p(s, f64p), _p(grad, f64p), float(iso), int(sflag),
+        self.lib.dmcref_dualmc(C.byref(_g(grid)), _p(s, f64p), _p(grad, f64p), float(iso), int(sflag),
                                C.byref(nv), C.byref(v), C.byref(n), C.byref(nq), C.byref(q),
                                C.byref(nt), C.byref(t))
         out = Mesh()
@@ -433,7 +446,7 @@ class Reference:
     def run_float(self, grid, field32, iso, sflag=True):
         f = np.ascontiguousarray(field32, dtype=np.float32)
         nv = C.c_int64()
-        nq = self.lib.dmcref_run_float(C.byref(grid), f.ctypes.data_as(C.POINTER(C.c_float)),
+        nq = self.lib.dmcref_run_float(C.byref(_g(grid)), f.ctypes.data_as(C.POINTER(C.c_float)),
                                        float(iso), int(sflag), C.byref(nv))
         return nv.value, nq
 

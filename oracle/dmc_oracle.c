@@ -737,25 +737,30 @@ static void remove3x3y(dmco_mesh* m, uint8_t* vm_first, int32_t* vm_second, cons
     m->nq = nq;
 }
 
+/* One round of simplify3X3Y (DualMarchingCubes.cpp:1152-1158 colored,
+ * :1172-1178 isolated). */
+int dmco_simplify3x3y_round(dmco_mesh* m, int colored) {
+    dmco_halfedges H;
+    dmco_halfedges_build(m->nv, m->nq, m->quads, &H);
+    if (colored) dmco_color_faces(m->nq, H.he, H.he_f, m->colors);
+    int32_t* vV = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)m->nv);
+    dmco_vertex_valence(m->nv, H.nhe, H.he, vV);
+    uint8_t* p = (uint8_t*)xmalloc((size_t)m->nq);
+    dmco_mark3x3y(m->nq, m->quads, vV, p);
+    uint8_t* vf = (uint8_t*)xmalloc((size_t)m->nv);
+    int32_t* vs = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)m->nv);
+    uint8_t* em = (uint8_t*)xmalloc((size_t)m->nq);
+    dmco_merge3x3y(m->nv, m->nq, m->v, m->n, p, vV, colored ? m->colors : NULL, H.he, H.he_f, vf, vs, em);
+    remove3x3y(m, vf, vs, em);
+    free(vV); free(p); free(vf); free(vs); free(em);
+    dmco_halfedges_free(&H);
+    return 0;
+}
+
 /* DualMarchingCubes.cpp:1139-1183 */
 int dmco_simplify3x3y(dmco_mesh* m) {
-    for (int round = 0; round < 2; round++) {
-        dmco_halfedges H;
-        dmco_halfedges_build(m->nv, m->nq, m->quads, &H);
-        if (round == 0) dmco_color_faces(m->nq, H.he, H.he_f, m->colors);
-        int32_t* vV = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)m->nv);
-        dmco_vertex_valence(m->nv, H.nhe, H.he, vV);
-        uint8_t* p = (uint8_t*)xmalloc((size_t)m->nq);
-        dmco_mark3x3y(m->nq, m->quads, vV, p);
-        uint8_t* vf = (uint8_t*)xmalloc((size_t)m->nv);
-        int32_t* vs = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)m->nv);
-        uint8_t* em = (uint8_t*)xmalloc((size_t)m->nq);
-        dmco_merge3x3y(m->nv, m->nq, m->v, m->n, p, vV, round == 0 ? m->colors : NULL,
-                       H.he, H.he_f, vf, vs, em);
-        remove3x3y(m, vf, vs, em);
-        free(vV); free(p); free(vf); free(vs); free(em);
-        dmco_halfedges_free(&H);
-    }
+    dmco_simplify3x3y_round(m, 1);
+    dmco_simplify3x3y_round(m, 0);
     return 0;
 }
 
@@ -878,20 +883,25 @@ int dmco_triangles(dmco_mesh* m) {
     return 0;
 }
 
-/* DualMarchingCubes.cpp:5-88 */
-int dmco_dualmc(const dmco_grid* g, const double* s, const double* grad, double iso,
-                int sflag, dmco_mesh* out, dmco_halfedges* he_out) {
+/* DualMarchingCubes.cpp:5-88.  flags: 1 = 3X3Y colored round, 2 = 3X3Y
+ * isolated round, 4 = 3333 (sFlag == true is all three); 8 = triangles. */
+int dmco_dualmc_flags(const dmco_grid* g, const double* s, const double* grad, double iso,
+                      int flags, dmco_mesh* out, dmco_halfedges* he_out) {
     dmco_cells(g, s, grad, iso, out);
-    if (sflag) {
-        dmco_simplify3x3y(out);
-        dmco_simplify3333(out);
-    }
+    if (flags & 1) dmco_simplify3x3y_round(out, 1);
+    if (flags & 2) dmco_simplify3x3y_round(out, 0);
+    if (flags & 4) dmco_simplify3333(out);
     dmco_halfedges H;
     dmco_halfedges_build(out->nv, out->nq, out->quads, &H);
     if (he_out) *he_out = H;
     else dmco_halfedges_free(&H);
-    dmco_triangles(out);
+    if (flags & 8) dmco_triangles(out);
     return 0;
+}
+
+int dmco_dualmc(const dmco_grid* g, const double* s, const double* grad, double iso,
+                int sflag, dmco_mesh* out, dmco_halfedges* he_out) {
+    return dmco_dualmc_flags(g, s, grad, iso, (sflag ? 7 : 0) | 8, out, he_out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -84,6 +84,7 @@ void dmco_merge3x3y(int64_t nv, int64_t nq, double* v, double* n, const uint8_t*
                     const int32_t* he_f, uint8_t* vm_first, int32_t* vm_second, uint8_t* em);
 
 int dmco_simplify3x3y(dmco_mesh* m);
+int dmco_simplify3x3y_round(dmco_mesh* m, int colored);
 int dmco_simplify3333(dmco_mesh* m);
 void dmco_mark3333(int64_t nv, int64_t nq, const int32_t* he, const int32_t* he_f,
                    const int32_t* valence, uint8_t* vm_flag, int32_t* vm_twin,
@@ -94,6 +95,10 @@ int dmco_triangles(dmco_mesh* m);
  * final halfedges (returned in he_out when non-NULL), triangles. */
 int dmco_dualmc(const dmco_grid* g, const double* scalars, const double* grad, double iso,
                 int sflag, dmco_mesh* out, dmco_halfedges* he_out);
+/* Same with the engine's flag bits: 1 = 3X3Y colored, 2 = 3X3Y isolated,
+ * 4 = 3333, 8 = triangles. */
+int dmco_dualmc_flags(const dmco_grid* g, const double* scalars, const double* grad, double iso,
+                      int flags, dmco_mesh* out, dmco_halfedges* he_out);
 
 /* Single-cell helpers used by the known-answer tests. */
 int dmco_cell_polygons(const double f[8], double iso, int32_t sizes[4], int32_t edges[12]);

@@ -1,0 +1,233 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle and with the
+golden fixtures of the compiled reference.
+
+Integer and topological outputs (quads, halfedges {origin, face, next,
+twin, flag}, he_v, boundary vertices, counts) must be bit-exact.  Positions
+and normals are FP64 and must be bit-exact as well (the engine reproduces
+the reference's rounding points; the stated tolerance of 1e-6 grid units is
+therefore met with margin 0).  Triangles may differ only where the two
+diagonal splits tie to 1e-5 rad in minimum angle (float acos; see
+meshcheck.triangle_mismatches).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden_files, grid_from, load_golden
+from meshcheck import assert_same, euler, halfedge_invariants, triangle_mismatches
+
+pytestmark = pytest.mark.gpu
+
+import tmc_amd  # noqa: E402
+from tmc_amd import SIMPLIFY_3333, SIMPLIFY_3X3Y_COLOR, SIMPLIFY_3X3Y_ISOLATED, SIMPLIFY_ALL, TRIANGLES  # noqa: E402
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def engine_mesh(engine, field, grid, iso, flags, grad=None):
+    engine.extract(field, iso, grid, flags=flags, gradients=grad)
+    return engine.download()
+
+
+def check_mesh(got, exp, what, tris=True):
+    for k in ("v", "n", "quads", "he", "he_v", "bnd"):
+        assert_same(got[k], exp[k], f"{what}:{k}")
+    if tris:
+        bad = triangle_mismatches(exp["tris"], got["tris"], exp["quads"], exp["v"])
+        assert len(bad) == 0, f"{what}: triangles differ off ties at quads {bad[:10]}"
+
+
+@pytest.mark.parametrize("name", golden_files("dualmc_"))
+def test_engine_matches_reference_fixture(engine, name):
+    _say(f"[gpu] fixture {name}")
+    z = load_golden(name)
+    g = grid_from(z["grid"])
+    flags = TRIANGLES | (SIMPLIFY_ALL if int(z["sflag"]) else 0)
+    grad = z["grad"] if z["grad"].size else None
+    got = engine_mesh(engine, z["field"], g, float(z["iso"]), flags, grad)
+    check_mesh(got, z, name)
+
+
+def _fields(seed, count, max_n=14):
+    rng = np.random.default_rng(seed)
+    for it in range(count):
+        kind = it % 5
+        sh = tuple(int(x) for x in rng.integers(3, max_n, size=3))
+        if kind == 0:
+            f, iso = rng.integers(-2, 3, size=sh).astype(np.float32), float(rng.integers(-1, 2))
+        elif kind == 1:
+            f, iso = rng.standard_normal(sh).astype(np.float32), float(rng.normal() * 0.3)
+        elif kind == 2:
+            f, iso = (rng.random(sh) < 0.5).astype(np.float32), 0.5
+        elif kind == 3:
+            f, iso = rng.standard_normal(sh), 0.3
+        else:
+            f, iso = rng.standard_normal(sh).astype(np.float32), 0.3
+        g = tmc_amd.Grid.make(sh, tuple(rng.normal(size=3)), tuple(rng.uniform(0.1, 2, size=3)))
+        yield it, f, g, iso
+
+
+FLAG_SETS = [0, SIMPLIFY_3X3Y_COLOR, SIMPLIFY_3X3Y_ISOLATED, SIMPLIFY_3333,
+             SIMPLIFY_3X3Y_COLOR | SIMPLIFY_3X3Y_ISOLATED, SIMPLIFY_3X3Y_COLOR | SIMPLIFY_3333,
+             SIMPLIFY_3X3Y_ISOLATED | SIMPLIFY_3333, SIMPLIFY_ALL]
+
+
+def test_engine_matches_oracle_random_fields(engine, oracle):
+    """Random, tie-heavy, binary, FP64 and non-float-iso fields, every
+    combination of simplification passes."""
+    nfail = nnm = nbnd = 0
+    for it, f, g, iso in _fields(3, 60):
+        s = np.asarray(f, np.float64).ravel()
+        grad = oracle.gradient(g, s)
+        fl = FLAG_SETS[it % len(FLAG_SETS)]
+        _say(f"[gpu] random field {it} shape {f.shape} dtype {f.dtype} iso {iso} flags {fl}")
+        exp = oracle.dualmc(g, s, grad, iso, flags=fl | 8)
+        got = engine_mesh(engine, f, g, iso, fl | TRIANGLES)
+        check_mesh(got, exp, f"field {it}")
+        assert got["nonmanifold_edges"] == exp["nonmanifold"]
+        c = oracle.cells(g, s, grad, iso)
+        assert (got["failed_projections_l1"], got["failed_projections_l2"]) == (c["nr_proj1"], c["nr_proj2"])
+        assert got["nv_emitted"] == c["nv_before_unused"]
+        assert (got["nv_cells"], got["nq_cells"]) == (c["v"].shape[0], c["quads"].shape[0])
+        nfail += c["nr_proj1"]
+        nnm += exp["nonmanifold"]
+        nbnd += len(exp["bnd"])
+    assert nfail > 0 and nnm > 0 and nbnd > 0, "sample must reach failed projections and non-manifold edges"
+
+
+def test_engine_supplied_gradients(engine, oracle):
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((7, 9, 11)).astype(np.float32)
+    g = tmc_amd.Grid.make(f.shape, (0, 0, 0), (1, 1, 1))
+    grad = rng.standard_normal(f.size * 3)
+    exp = oracle.dualmc(g, f.astype(np.float64).ravel(), grad, 0.0, True)
+    got = engine_mesh(engine, f, g, 0.0, SIMPLIFY_ALL | TRIANGLES, grad=grad)
+    check_mesh(got, exp, "grad")
+
+
+def test_config1_genus2_128(engine, oracle):
+    """Config 1: genus-2, 128^3 cell-centred on [-1.5,1.5]^3, iso 0."""
+    from tmc_amd import workloads
+    _say("[gpu] genus2 128")
+    f, g, iso = workloads.genus2(128)
+    s = f.astype(np.float64).ravel()
+    exp = oracle.dualmc(g, s, oracle.gradient(g, s), iso, True)
+    got = engine_mesh(engine, f, g, iso, SIMPLIFY_ALL | TRIANGLES)
+    check_mesh(got, exp, "genus2_128")
+    assert (got["nv_cells"], got["nq_cells"]) == (65965, 65752)
+    assert (len(got["v"]), len(got["quads"])) == (59177, 58964)
+    E, chi = euler(len(got["v"]), got["quads"], got["he"])
+    assert (E, chi) == (118146, -5)
+
+
+@pytest.mark.parametrize("which", ["blobs_160", "schwarz_161"])
+def test_benchmark_fields_sampled(engine, oracle, which):
+    """The blob and Schwarz-P generators at a size the oracle finishes in
+    seconds (same parameters as the 512^3 and 1024^3 workloads)."""
+    from tmc_amd import workloads
+    _say(f"[gpu] {which}")
+    if which == "blobs_160":
+        f, g, iso = workloads.blobs(160, seed=2)
+    else:
+        f, g, iso = workloads.schwarz(161, seed=3)
+    s = f.astype(np.float64).ravel()
+    exp = oracle.dualmc(g, s, oracle.gradient(g, s), iso, True)
+    got = engine_mesh(engine, f, g, iso, SIMPLIFY_ALL | TRIANGLES)
+    check_mesh(got, exp, which)
+
+
+def test_repeat_and_reuse_is_deterministic(engine):
+    """Same input twice, with a different extraction in between, gives the
+    same bytes (no state leaks through the reused buffers)."""
+    from tmc_amd import workloads
+    f, g, iso = workloads.genus2(48)
+    a = engine_mesh(engine, f, g, iso, SIMPLIFY_ALL | TRIANGLES)
+    f2, g2, iso2 = workloads.blobs(40, seed=9)
+    engine_mesh(engine, f2, g2, iso2, SIMPLIFY_ALL)
+    b = engine_mesh(engine, f, g, iso, SIMPLIFY_ALL | TRIANGLES)
+    for k in ("v", "n", "quads", "tris", "he", "he_v"):
+        assert_same(a[k], b[k], k)
+
+
+def test_empty_and_degenerate_inputs(engine):
+    f = np.zeros((4, 5, 6), np.float32)
+    got = engine_mesh(engine, f, tmc_amd.Grid.make(f.shape), 1.0, SIMPLIFY_ALL | TRIANGLES)
+    assert len(got["v"]) == 0 and len(got["quads"]) == 0
+    f = np.zeros((2, 2, 2), np.float32)
+    f[0, 0, 0] = 1
+    got = engine_mesh(engine, f, tmc_amd.Grid.make(f.shape), 0.5, SIMPLIFY_ALL | TRIANGLES)
+    assert len(got["quads"]) == 0 and got["nv_emitted"] == 1 and len(got["v"]) == 0
+    with pytest.raises(tmc_amd.DMCError):
+        engine.extract(np.zeros((1, 4, 4), np.float32), 0.0)
+
+
+def test_device_resident_field(engine, oracle):
+    from tmc_amd import workloads
+    f, g, iso = workloads.genus2(40)
+    buf = engine.alloc(f.nbytes).upload(f)
+    try:
+        engine.extract(buf, iso, g, flags=SIMPLIFY_ALL | TRIANGLES, shape=f.shape)
+        got = engine.download()
+    finally:
+        buf.free()
+    s = f.astype(np.float64).ravel()
+    exp = oracle.dualmc(g, s, oracle.gradient(g, s), iso, True)
+    check_mesh(got, exp, "device field")
+
+
+def test_fp32_export(engine):
+    from tmc_amd import workloads
+    f, g, iso = workloads.genus2(32)
+    engine.extract(f, iso, g)
+    m = engine.download(fp32=True)
+    assert np.array_equal(m["v32"], m["v"].astype(np.float32))
+    assert np.array_equal(m["n32"], m["n"].astype(np.float32))
+
+
+def test_halfedge_invariants_on_gpu_mesh(engine):
+    from tmc_amd import workloads
+    f, g, iso = workloads.blobs(96, seed=2)
+    m = engine_mesh(engine, f, g, iso, SIMPLIFY_ALL)
+    halfedge_invariants(len(m["v"]), m["quads"], m["he"])
+
+
+def test_cpp_dropin_programs(oracle, tmp_path):
+    """The reference's own driver programs (cpp_dmc/main.cpp, dmc/kernel.cu)
+    written against include/tmc_dmc.hpp: cpp_dmc/main.cpp's 64^3 GenusTwo
+    gives V 8,825 / Q 8,827 in the reference; the C++ surface must return
+    the oracle's mesh bit for bit, also through the binary-file loader."""
+    import json
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = str(tmp_path / "dropin")
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp")
+    libdir = os.path.join(ROOT, "tmc_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), src, "-L" + libdir,
+                        "-ltmc_dmc", "-Wl,-rpath," + libdir, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    _say("[gpu] running C++ drop-in program")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = {d["case"]: d for d in (json.loads(l) for l in r.stdout.splitlines() if l.startswith("{"))}
+    assert (res["cpp_genus2_64"]["nv"], res["cpp_genus2_64"]["nq"]) == (8825, 8827)
+    assert res["cpp_file_64"]["on_device_gradients"] == 1
+    assert res["cuda_genus2_64"]["twins_ok"] == 1
+    rd = lambda n, t: np.fromfile(str(tmp_path / n), dtype=t)
+    geo = rd("geo.f64", np.float64)
+    from oracle import Grid
+    g = Grid(64, 64, 64, *geo)
+    s = rd("scalars.f64", np.float64)
+    exp = oracle.dualmc(g, s, rd("grads.f64", np.float64), 0.0, True)
+    assert_same(rd("v.f64", np.float64).reshape(-1, 3), exp["v"], "cpp v")
+    assert_same(rd("n.f64", np.float64).reshape(-1, 3), exp["n"], "cpp n")
+    assert_same(rd("quads.i32", np.int32).reshape(-1, 4), exp["quads"], "cpp quads")
+    # file path: float field, gradients estimated + flipped
+    s32 = s.astype(np.float32).astype(np.float64)
+    exp2 = oracle.dualmc(g, s32, oracle.gradient(g, s32), 0.0, True)
+    assert_same(rd("file_v.f64", np.float64).reshape(-1, 3), exp2["v"], "file v")
+    assert_same(rd("file_quads.i32", np.int32).reshape(-1, 4), exp2["quads"], "file quads")

@@ -1,0 +1,275 @@
+"""tmc_amd -- MI355X Dual Marching Cubes engine (Python binding over the C ABI).
+
+The engine is libtmc_dmc.so (HIP, gfx950), declared in include/tmc_dmc.h.
+This module loads it with ctypes and exposes:
+
+* ``Engine`` -- one device context: ``extract`` runs the whole DMC path on
+  the GPU, ``download`` copies the mesh back (positions/normals FP64, quads,
+  triangles, halfedges {origin, face, next, twin, flag}, he_v, he_f and the
+  boundary vertices of getBndVertices).
+* ``DualMarchingCubes`` -- the reference call surface
+  ``dualMC(i0, ugrid, sFlag)`` (cpp_dmc/DualMarchingCubes.h:163-165).
+
+There is no CPU fallback: if the library is missing or no GPU is present,
+the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import ENGINE_SO
+
+__all__ = ["Engine", "DualMarchingCubes", "UniformGrid", "DMCError", "SIMPLIFY_3X3Y_COLOR",
+           "SIMPLIFY_3X3Y_ISOLATED", "SIMPLIFY_3333", "SIMPLIFY_ALL", "TRIANGLES", "GRADIENTS_SUPPLIED",
+           "FIELD_ON_DEVICE", "STAGE_TIMING", "load_library"]
+
+SIMPLIFY_3X3Y_COLOR = 1 << 0
+SIMPLIFY_3X3Y_ISOLATED = 1 << 1
+SIMPLIFY_3333 = 1 << 2
+TRIANGLES = 1 << 3
+GRADIENTS_SUPPLIED = 1 << 4
+FIELD_ON_DEVICE = 1 << 5
+STAGE_TIMING = 1 << 6
+SIMPLIFY_ALL = SIMPLIFY_3X3Y_COLOR | SIMPLIFY_3X3Y_ISOLATED | SIMPLIFY_3333
+
+FIELD_F32 = 0
+FIELD_F64 = 1
+
+# every symbol include/tmc_dmc.h declares
+EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_mesh_download", "dmc_mesh_download_f32",
+           "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host",
+           "dmc_synchronize", "dmc_status_str", "dmc_last_error"]
+
+
+class DMCError(RuntimeError):
+    pass
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+    @classmethod
+    def make(cls, shape, origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0)):
+        """shape is (nz, ny, nx), the numpy order of an x-fastest field."""
+        nz, ny, nx = (int(s) for s in shape)
+        return cls(nx, ny, nz, *[float(o) for o in origin], *[float(s) for s in spacing])
+
+
+class Mesh(C.Structure):
+    _fields_ = [("nv", C.c_int64), ("nq", C.c_int64), ("nt", C.c_int64),
+                ("d_positions", C.c_void_p), ("d_normals", C.c_void_p), ("d_quads", C.c_void_p),
+                ("d_tris", C.c_void_p), ("d_twins", C.c_void_p), ("d_he_flags", C.c_void_p),
+                ("d_he_v", C.c_void_p),
+                ("nonmanifold_edges", C.c_int64), ("nbnd", C.c_int64), ("edge_errors", C.c_int64),
+                ("invalid_refs", C.c_int64), ("valence_zero", C.c_int64),
+                ("failed_projections_l1", C.c_uint32), ("failed_projections_l2", C.c_uint32),
+                ("active_cells", C.c_int64), ("nv_emitted", C.c_int64), ("nv_cells", C.c_int64),
+                ("nq_cells", C.c_int64)]
+
+    def summary(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("d_")}
+
+
+_lib = None
+
+
+def load_library(path=ENGINE_SO):
+    """Load libtmc_dmc.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise DMCError(f"{path} not built: run `python -m tmc_amd.build` (or __graft_entry__.build())")
+    L = C.CDLL(path)
+    vp, i32p, f64p, f32p = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_float)
+    L.dmc_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.dmc_destroy.argtypes = [vp]
+    L.dmc_extract.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, vp, C.POINTER(Mesh)]
+    L.dmc_mesh_download.argtypes = [vp, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
+    L.dmc_mesh_download_f32.argtypes = [vp, f32p, f32p, C.c_int]
+    L.dmc_stage_times.argtypes = [vp, C.POINTER(C.c_char_p), f32p, C.POINTER(C.c_int)]
+    L.dmc_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.dmc_device_free.argtypes = [vp, vp]
+    L.dmc_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
+    L.dmc_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+    L.dmc_synchronize.argtypes = [vp]
+    L.dmc_status_str.argtypes = [C.c_int]
+    L.dmc_status_str.restype = C.c_char_p
+    L.dmc_last_error.argtypes = [vp]
+    L.dmc_last_error.restype = C.c_char_p
+    for name in ("dmc_create", "dmc_destroy", "dmc_extract", "dmc_mesh_download", "dmc_mesh_download_f32",
+                 "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
+                 "dmc_copy_to_host", "dmc_synchronize"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class DeviceBuffer:
+    """Device allocation owned by an Engine (freed on close/GC of the engine)."""
+
+    def __init__(self, engine, nbytes):
+        self.engine = engine
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        engine._check(engine.lib.dmc_device_alloc(engine.ctx, self.nbytes, C.byref(self.ptr)))
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.engine._check(self.engine.lib.dmc_copy_to_device(self.engine.ctx, self.ptr, arr.ctypes.data, arr.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            self.engine.lib.dmc_device_free(self.engine.ctx, self.ptr)
+            self.ptr = C.c_void_p()
+
+
+class Engine:
+    """One device context (one HIP stream + reusable device buffers)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.ctx = C.c_void_p()
+        s = self.lib.dmc_create(C.byref(self.ctx), int(device))
+        if s != 0:
+            raise DMCError(f"dmc_create: {self.lib.dmc_status_str(s).decode()}")
+        self.mesh = None
+
+    def _check(self, s):
+        if s != 0:
+            msg = self.lib.dmc_last_error(self.ctx).decode() if self.ctx else ""
+            raise DMCError(f"{self.lib.dmc_status_str(s).decode()}: {msg}")
+
+    def close(self):
+        if self.ctx and self.ctx.value:
+            self.lib.dmc_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def extract(self, field, iso, grid=None, flags=SIMPLIFY_ALL | TRIANGLES, gradients=None, shape=None,
+                dtype=np.float32):
+        """Run the DMC path.  `field` is a numpy array (nz, ny, nx) or a
+        DeviceBuffer holding one (then pass `shape` and `dtype`)."""
+        if isinstance(field, DeviceBuffer):
+            fptr = field.ptr
+            flags |= FIELD_ON_DEVICE
+            if grid is None:
+                grid = Grid.make(shape)
+            ftype = FIELD_F32 if np.dtype(dtype) == np.float32 else FIELD_F64
+            gptr = gradients.ptr if isinstance(gradients, DeviceBuffer) else None
+            if gradients is not None and not isinstance(gradients, DeviceBuffer):
+                raise DMCError("device field needs device gradients")
+            keep = None
+        else:
+            arr = np.ascontiguousarray(field)
+            if arr.dtype not in (np.float32, np.float64):
+                arr = arr.astype(np.float32)
+            ftype = FIELD_F32 if arr.dtype == np.float32 else FIELD_F64
+            if grid is None:
+                grid = Grid.make(arr.shape)
+            fptr = C.c_void_p(arr.ctypes.data)
+            keep = [arr]
+            gptr = None
+            if gradients is not None:
+                g = np.ascontiguousarray(gradients, dtype=np.float64)
+                keep.append(g)
+                gptr = C.c_void_p(g.ctypes.data)
+        if gradients is not None:
+            flags |= GRADIENTS_SUPPLIED
+        m = Mesh()
+        self._check(self.lib.dmc_extract(self.ctx, C.byref(grid), fptr, ftype, float(iso), int(flags), gptr,
+                                         C.byref(m)))
+        del keep
+        self.mesh = m
+        return m
+
+    def download(self, halfedges=True, fp32=False):
+        """Copy the last mesh to host numpy arrays."""
+        m = self.mesh
+        if m is None:
+            raise DMCError("no mesh: call extract first")
+        out = {}
+        v = np.empty((m.nv, 3), np.float64)
+        n = np.empty((m.nv, 3), np.float64)
+        q = np.empty((m.nq, 4), np.int32)
+        t = np.empty((m.nt, 3), np.int32)
+        he = np.empty((4 * m.nq, 5), np.int32) if halfedges else None
+        hv = np.empty(m.nv, np.int32)
+        hf = np.empty(m.nq, np.int32)
+        bnd = np.empty(m.nbnd, np.int32)
+        i32 = C.c_int32
+        self._check(self.lib.dmc_mesh_download(
+            self.ctx, _p(v, C.c_double), _p(n, C.c_double), _p(q, i32), _p(t, i32) if m.nt else None,
+            _p(he, i32) if he is not None else None, _p(hv, i32), _p(hf, i32), _p(bnd, i32) if m.nbnd else None))
+        out.update(v=v, n=n, quads=q, tris=t, he_v=hv, he_f=hf, bnd=bnd)
+        if he is not None:
+            out["he"] = he
+        if fp32:
+            v32 = np.empty((m.nv, 3), np.float32)
+            n32 = np.empty((m.nv, 3), np.float32)
+            self._check(self.lib.dmc_mesh_download_f32(self.ctx, _p(v32, C.c_float), _p(n32, C.c_float), 0))
+            out.update(v32=v32, n32=n32)
+        out.update(m.summary())
+        return out
+
+    def stage_times(self):
+        n = C.c_int(64)
+        names = (C.c_char_p * 64)()
+        ms = (C.c_float * 64)()
+        self._check(self.lib.dmc_stage_times(self.ctx, names, ms, C.byref(n)))
+        return {names[i].decode(): float(ms[i]) for i in range(min(n.value, 64))}
+
+    def synchronize(self):
+        self._check(self.lib.dmc_synchronize(self.ctx))
+
+
+class UniformGrid:
+    """The reference's grid description (cpp_dmc/UniformGrid.h:18-354):
+    nx*ny*nz scalars, x fastest, bounding box origin and spacing."""
+
+    def __init__(self, scalars, origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0), gradients=None):
+        self.scalars = np.ascontiguousarray(scalars)
+        self.origin = tuple(float(o) for o in origin)
+        self.spacing = tuple(float(s) for s in spacing)
+        self.gradients = gradients
+
+    def grid(self):
+        return Grid.make(self.scalars.shape, self.origin, self.spacing)
+
+
+class DualMarchingCubes:
+    """Reference call surface: dualMC(i0, ugrid, sFlag) -> (v, n, tris, quads)
+    as cpp_dmc/DualMarchingCubes.h:163-165 returns them (flat lists become
+    (n,3)/(n,4) arrays)."""
+
+    def __init__(self, device=0):
+        self.engine = Engine(device)
+        self.bnd = np.zeros(0, np.int32)
+
+    def dualMC(self, i0, ugrid, sFlag=True):
+        flags = TRIANGLES | (SIMPLIFY_ALL if sFlag else 0)
+        self.engine.extract(ugrid.scalars, i0, ugrid.grid(), flags=flags, gradients=ugrid.gradients)
+        m = self.engine.download(halfedges=False)
+        self.bnd = m["bnd"]
+        return m["v"], m["n"], m["tris"], m["quads"]
+
+    def getBndVertices(self):
+        """cpp_dmc/DualMarchingCubes.h:171-182"""
+        return self.bnd.copy()

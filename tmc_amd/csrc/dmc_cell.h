@@ -1,0 +1,409 @@
+// dmc_cell.h -- per-cell device math of the DMC engine (gfx950).
+//
+// Cell corners are numbered c = x | y<<1 | z<<2.  The functions here follow
+// the reference cpp_dmc cell path (cpp_dmc/DualMarchingCubes.cpp:97-481,
+// DualMarchingCubes.h:301-597, UniformGrid.h:286-305) including where it
+// rounds to float; this file must be compiled with -ffp-contract=off so that
+// no multiply-add is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dmc_tables.h"
+
+namespace tmc {
+
+// ---------------------------------------------------------------------------
+// cell geometry
+// ---------------------------------------------------------------------------
+// Edge end points: v0 is always the lower-coordinate corner
+// (cpp_dmc/DualMarchingCubes.cpp:110).
+constexpr int kEdgeV0[12] = {0, 1, 2, 0, 4, 5, 6, 4, 0, 1, 3, 2};
+constexpr int kEdgeV1[12] = {1, 3, 3, 2, 5, 7, 7, 6, 4, 5, 7, 6};
+// Axis of each edge: 0 = x, 1 = y, 2 = z.
+constexpr int kEdgeAxis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+// Slot of a cell's vertex in the quadrilateral around edge e, for the two
+// orientations (bit = field at v0 below the iso-value); DualMarchingCubes.h:226-230.
+constexpr int kEdgeSlot[12][2] = {{0, 0}, {0, 0}, {3, 1}, {3, 1}, {1, 3}, {1, 3},
+                                  {2, 2}, {2, 2}, {0, 0}, {3, 1}, {2, 2}, {1, 3}};
+// Oriented faces seen from outside: corners and edges in face order
+// (DualMarchingCubes.h:301-303).
+constexpr int kFaceV[6][4] = {{0, 2, 1, 3}, {5, 7, 4, 6}, {4, 0, 5, 1},
+                              {2, 6, 3, 7}, {4, 6, 0, 2}, {1, 3, 5, 7}};
+constexpr int kFaceE[6][4] = {{3, 2, 1, 0}, {5, 6, 7, 4}, {8, 0, 9, 4},
+                              {11, 6, 10, 2}, {7, 11, 3, 8}, {1, 10, 5, 9}};
+
+constexpr uint64_t pack4(const int (&a)[12]) {
+    uint64_t w = 0;
+    for (int i = 0; i < 12; i++) w |= uint64_t(a[i]) << (4 * i);
+    return w;
+}
+constexpr uint64_t kEdgeV0Packed = pack4(kEdgeV0);
+constexpr uint64_t kEdgeV1Packed = pack4(kEdgeV1);
+constexpr uint64_t kEdgeAxisPacked = pack4(kEdgeAxis);
+constexpr uint64_t pack_slots() {
+    uint64_t w = 0;
+    for (int e = 0; e < 12; e++) w |= uint64_t(kEdgeSlot[e][0] | kEdgeSlot[e][1] << 2) << (4 * e);
+    return w;
+}
+constexpr uint64_t kEdgeSlotPacked = pack_slots();
+
+__device__ __forceinline__ int edge_v0(int e) { return (int)((kEdgeV0Packed >> (4 * e)) & 0xF); }
+__device__ __forceinline__ int edge_v1(int e) { return (int)((kEdgeV1Packed >> (4 * e)) & 0xF); }
+__device__ __forceinline__ int edge_axis(int e) { return (int)((kEdgeAxisPacked >> (4 * e)) & 0xF); }
+__device__ __forceinline__ int edge_slot(int e, int bit) {
+    return (int)((kEdgeSlotPacked >> (4 * e + 2 * bit)) & 3);
+}
+// Does the iso-surface cross edge e of a cell with case `icase`?
+__device__ __forceinline__ bool edge_cut(unsigned icase, int e) {
+    return ((icase >> edge_v0(e)) ^ (icase >> edge_v1(e))) & 1u;
+}
+
+// Is the grid edge e of cell (i,j,k) shared by four existing cells?  Only
+// such edges yield complete quadrilaterals (DualMarchingCubes.h:663-676).
+__device__ __forceinline__ bool edge_interior(int e, int i, int j, int k, int nx, int ny, int nz) {
+    const int v0 = edge_v0(e);
+    const int pi = i + (v0 & 1), pj = j + ((v0 >> 1) & 1), pk = k + ((v0 >> 2) & 1);
+    const int ax = edge_axis(e);
+    const bool okx = pi >= 1 && pi <= nx - 2;
+    const bool oky = pj >= 1 && pj <= ny - 2;
+    const bool okz = pk >= 1 && pk <= nz - 2;
+    return ax == 0 ? (oky && okz) : ax == 1 ? (okx && okz) : (okx && oky);
+}
+
+// ---------------------------------------------------------------------------
+// MC polygons, packed: bits 0..15 = four 4-bit contour sizes (0 = absent),
+// bits 16..63 = edge ids, contour after contour.
+// ---------------------------------------------------------------------------
+__device__ __constant__ uint64_t c_case_contours[256] = DMC_CASE_CONTOURS_INIT;
+__device__ __constant__ uint64_t c_ambiguous[4] = DMC_AMBIGUOUS_MASK_INIT;
+
+__device__ __forceinline__ bool case_ambiguous(unsigned icase) {
+    return (c_ambiguous[icase >> 6] >> (icase & 63)) & 1ull;
+}
+__device__ __forceinline__ int poly_size(uint64_t w, int t) { return (int)((w >> (4 * t)) & 0xF); }
+__device__ __forceinline__ int poly_count(uint64_t w) {
+    return (poly_size(w, 0) != 0) + (poly_size(w, 1) != 0) + (poly_size(w, 2) != 0) + (poly_size(w, 3) != 0);
+}
+__device__ __forceinline__ int poly_start(uint64_t w, int t) {
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) s += (i < t) ? poly_size(w, i) : 0;
+    return s;
+}
+__device__ __forceinline__ int poly_edge_at(uint64_t w, int pos) { return (int)((w >> (16 + 4 * pos)) & 0xF); }
+
+// cpp_dmc/DualMarchingCubes.h:330-332
+__device__ __forceinline__ double asymptotic_decider(double f0, double f1, double f2, double f3) {
+    return (f0 * f3 - f1 * f2) / (f0 + f3 - f1 - f2);
+}
+
+// Polygons of an ambiguous case (cpp_dmc/DualMarchingCubes.cpp:328-461):
+// each face contributes oriented segments, faces with two opposite positive
+// corners are split by the asymptotic decider, and segments are chained into
+// contours starting from the lowest intersected edge.
+__device__ inline uint64_t polygons_decider(double iso, const double F[8]) {
+    uint64_t outs = 0;   // 4-bit successor per edge
+    unsigned set = 0;    // edges touched by a segment
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        const double f0 = F[kFaceV[f][0]], f1 = F[kFaceV[f][1]], f2 = F[kFaceV[f][2]], f3 = F[kFaceV[f][3]];
+        const int e0 = kFaceE[f][0], e1 = kFaceE[f][1], e2 = kFaceE[f][2], e3 = kFaceE[f][3];
+        const unsigned fc = (unsigned)(f0 >= iso) | (unsigned)(f1 >= iso) << 1 |
+                            (unsigned)(f2 >= iso) << 2 | (unsigned)(f3 >= iso) << 3;
+        int a0 = -1, b0 = -1, a1 = -1, b1 = -1;   // up to two segments a->b
+        switch (fc) {
+        case 1: a0 = e0; b0 = e3; break;
+        case 2: a0 = e1; b0 = e0; break;
+        case 3: a0 = e1; b0 = e3; break;
+        case 4: a0 = e3; b0 = e2; break;
+        case 5: a0 = e0; b0 = e2; break;
+        case 6: {
+            const double val = asymptotic_decider(f0, f1, f2, f3);
+            if (val >= iso) { a0 = e3; b0 = e0; a1 = e1; b1 = e2; }
+            else if (val < iso) { a0 = e1; b0 = e0; a1 = e3; b1 = e2; }
+        } break;
+        case 7: a0 = e1; b0 = e2; break;
+        case 8: a0 = e2; b0 = e1; break;
+        case 9: {
+            const double val = asymptotic_decider(f0, f1, f2, f3);
+            if (val >= iso) { a0 = e0; b0 = e1; a1 = e2; b1 = e3; }
+            else if (val < iso) { a0 = e0; b0 = e3; a1 = e2; b1 = e1; }
+        } break;
+        case 10: a0 = e2; b0 = e0; break;
+        case 11: a0 = e2; b0 = e3; break;
+        case 12: a0 = e3; b0 = e1; break;
+        case 13: a0 = e0; b0 = e1; break;
+        case 14: a0 = e3; b0 = e0; break;
+        default: break;
+        }
+        if (a0 >= 0) {
+            outs = (outs & ~(0xFull << (4 * a0))) | ((uint64_t)b0 << (4 * a0));
+            set |= (1u << a0) | (1u << b0);
+        }
+        if (a1 >= 0) {
+            outs = (outs & ~(0xFull << (4 * a1))) | ((uint64_t)b1 << (4 * a1));
+            set |= (1u << a1) | (1u << b1);
+        }
+    }
+    uint64_t w = 0;
+    int cnt = 0, pos = 0;
+    for (int e = 0; e < 12; e++) {
+        if (!((set >> e) & 1u)) continue;
+        if (cnt >= 4 || pos >= 12) break;
+        int size = 1;
+        w |= (uint64_t)e << (16 + 4 * pos);
+        pos++;
+        int to = (int)((outs >> (4 * e)) & 0xF);
+        while (to != e && to < 12 && pos < 12) {
+            w |= (uint64_t)to << (16 + 4 * pos);
+            pos++;
+            size++;
+            const int cur = to;
+            to = (int)((outs >> (4 * cur)) & 0xF);
+            set &= ~(1u << cur);
+        }
+        w |= (uint64_t)size << (4 * cnt);
+        cnt++;
+    }
+    return w;
+}
+
+__device__ __forceinline__ uint64_t cell_polygons(unsigned icase, double iso, const double F[8]) {
+    return case_ambiguous(icase) ? polygons_decider(iso, F) : c_case_contours[icase];
+}
+
+// Pick element t of a 4-array without dynamic register indexing.
+template <class T>
+__device__ __forceinline__ T pick4(const T (&a)[4], int t) {
+    return t == 0 ? a[0] : t == 1 ? a[1] : t == 2 ? a[2] : a[3];
+}
+template <class T>
+__device__ __forceinline__ T pick8(const T* a, int t) {
+    const T lo = (t & 2) ? ((t & 1) ? a[3] : a[2]) : ((t & 1) ? a[1] : a[0]);
+    const T hi = (t & 2) ? ((t & 1) ? a[7] : a[6]) : ((t & 1) ? a[5] : a[4]);
+    return (t & 4) ? hi : lo;
+}
+
+// Edge parameter rounded to float (cpp_dmc/DualMarchingCubes.h:391-407): the
+// coordinate along the edge axis is l, the others 0 or 1 by the position of v0.
+__device__ __forceinline__ float edge_param(int e, const double F[8], double iso) {
+    const double f0 = pick8(F, edge_v0(e)), f1 = pick8(F, edge_v1(e));
+    const float l = (iso - f0) / (f1 - f0);
+    return l;
+}
+__device__ __forceinline__ void local_coords(int e, float l, double u[3]) {
+    const int v0 = edge_v0(e), ax = edge_axis(e);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float val = (ax == c) ? 1.f : 0.f;
+        const float con = (float)((v0 >> c) & 1);
+        u[c] = l * val + con;
+    }
+}
+
+// Per-contour bounding boxes.  All entries are floats widened to double in
+// the reference, so float storage is exact.
+struct BBoxes {
+    float lo[3][4];
+    float hi[3][4];
+};
+
+// cpp_dmc/DualMarchingCubes.h:590-597
+__device__ __forceinline__ void trilinear_gradient(const double f[8], double u, double v, double w, double g[3]) {
+    g[0] = (1 - w) * ((1 - v) * (f[1] - f[0]) + v * (f[3] - f[2])) + w * ((1 - v) * (f[5] - f[4]) + v * (f[7] - f[6]));
+    g[1] = (1 - w) * ((1 - u) * (f[2] - f[0]) + u * (f[3] - f[1])) + w * ((1 - u) * (f[6] - f[4]) + u * (f[7] - f[5]));
+    g[2] = (1 - v) * ((1 - u) * (f[4] - f[0]) + u * (f[5] - f[1])) + v * ((1 - u) * (f[6] - f[2]) + u * (f[7] - f[3]));
+}
+
+// Projection sampling (cpp_dmc/DualMarchingCubes.cpp:274-326).  The caller
+// passes the eight corner values already permuted for the projection axis
+// (g[0..3] = bilinear face at w = 0, g[4..7] at w = 1), the bounds of the
+// contour's box in the projected frame and, for the neighbour-box test, the
+// boxes of the other size-3 contours in the same frame.
+struct ProjFrame {
+    double g[8];
+    float alo[4], ahi[4], blo[4], bhi[4], clo[4], chi[4];   // boxes per contour (u, v, w)
+};
+
+__device__ inline bool projection(const ProjFrame& P, double iso, int t, int cnt, uint64_t w,
+                                  double p[3], int ns) {
+    const double umin = pick4(P.alo, t), umax = pick4(P.ahi, t);
+    const double vmin = pick4(P.blo, t), vmax = pick4(P.bhi, t);
+    const double du = (umax - umin) / (ns - 1);
+    const double dv = (vmax - vmin) / (ns - 1);
+    double minDistance = 100;
+    float bu = 0.f, bv = 0.f, bw = -1.f;
+    bool found = false;
+    double wmin = pick4(P.clo, t), wmax = pick4(P.chi, t);
+    const double eps = 1e-5;
+    if (fabs(wmax - wmin) < eps) {
+        wmin -= eps;
+        wmax += eps;
+    }
+    const int cnt_size = poly_size(w, t);
+    // Neighbour boxes of size-3 contours other than t (DualMarchingCubes.h:500-519).
+    bool nb[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) nb[q] = q < cnt && q != t && poly_size(w, q) == 3;
+    const bool test_nb = !(cnt == 1 || cnt_size == 3);
+    const double f0 = P.g[0], f1 = P.g[1], f2 = P.g[2], f3 = P.g[3];
+    const double f4 = P.g[4], f5 = P.g[5], f6 = P.g[6], f7 = P.g[7];
+    for (int i = 1; i < (ns - 1); i++) {
+        const float u = umin + i * du;
+        for (int j = 1; j < (ns - 1); j++) {
+            const float v = vmin + j * dv;
+            const float g1 = (1 - v) * ((1 - u) * f0 + u * f1) + v * ((1 - u) * f2 + u * f3);
+            const float g2 = (1 - v) * ((1 - u) * f4 + u * f5) + v * ((1 - u) * f6 + u * f7);
+            if (g1 == g2) continue;
+            const float ww = (iso - g1) / (g2 - g1);
+            if (wmin <= ww && ww <= wmax) {
+                bool inside = false;
+                if (test_nb) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (!nb[q] || inside) continue;
+                        const double du_ = u, dv_ = v, dw_ = ww;
+                        if (du_ < (double)P.alo[q] || (double)P.ahi[q] < du_) continue;
+                        if (dv_ < (double)P.blo[q] || (double)P.bhi[q] < dv_) continue;
+                        if (dw_ < (double)P.clo[q] || (double)P.chi[q] < dw_) continue;
+                        inside = true;
+                    }
+                }
+                if (!inside) {
+                    const float d = (p[0] - u) * (p[0] - u) + (p[1] - v) * (p[1] - v) + (p[2] - ww) * (p[2] - ww);
+                    if (minDistance > d) {
+                        minDistance = d;
+                        bu = u;
+                        bv = v;
+                        bw = ww;
+                        found = true;
+                    }
+                }
+            }
+        }
+    }
+    if (found) {
+        p[0] = bu;
+        p[1] = bv;
+        p[2] = bw;
+    }
+    return found;
+}
+
+// Representative of contour t in local cell coordinates
+// (cpp_dmc/DualMarchingCubes.cpp:141-156 and :194-272).  Returns the
+// number of failed projection levels (0, 1 or 2).
+__device__ inline int contour_representative(const double F[8], double iso, uint64_t w, int cnt, int t,
+                                             const BBoxes& B, const float (&lpar)[12], double p[3]) {
+    // mean of the edge intersections, summed in contour order
+    const int start = poly_start(w, t), size = poly_size(w, t);
+    p[0] = 0; p[1] = 0; p[2] = 0;
+    for (int m = 0; m < size; m++) {
+        const int e = poly_edge_at(w, start + m);
+        double ui[3];
+        float l = 0.f;
+#pragma unroll
+        for (int q = 0; q < 12; q++) l = (q == e) ? lpar[q] : l;
+        local_coords(e, l, ui);
+        p[0] += ui[0];
+        p[1] += ui[1];
+        p[2] += ui[2];
+    }
+    p[0] /= size;
+    p[1] /= size;
+    p[2] /= size;
+    // projection axis
+    int prj;
+    if (size >= 6) {
+        double g[3];
+        trilinear_gradient(F, p[0], p[1], p[2], g);
+        const double dx = fabs(g[0]), dy = fabs(g[1]), dz = fabs(g[2]);
+        if (dz >= dx && dz >= dy) prj = 1;
+        else if (dy >= dx && dy >= dz) prj = 2;
+        else if (dx >= dy && dx >= dz) prj = 3;
+        else prj = 1;
+    } else {
+        const double eu = (double)pick4(B.hi[0], t) - (double)pick4(B.lo[0], t);
+        const double ev = (double)pick4(B.hi[1], t) - (double)pick4(B.lo[1], t);
+        const double ew = (double)pick4(B.hi[2], t) - (double)pick4(B.lo[2], t);
+        if (ew <= eu && ew <= ev) prj = 1;
+        else if (ev <= eu && ev <= ew) prj = 2;
+        else if (eu <= ev && eu <= ew) prj = 3;
+        else prj = 1;
+    }
+    // frame: (a, b, c) = projected (u, v, w) axes as permutation of (x, y, z)
+    ProjFrame P;
+    int ax_a, ax_b, ax_c;
+    if (prj == 1) { ax_a = 0; ax_b = 1; ax_c = 2; }
+    else if (prj == 2) { ax_a = 0; ax_b = 2; ax_c = 1; }
+    else { ax_a = 1; ax_b = 2; ax_c = 0; }
+    // corner (a, b, c) in the frame -> cell corner
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+        const int corner = (a << ax_a) | (b << ax_b) | (c << ax_c);
+        P.g[q] = pick8(F, corner);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        P.alo[q] = ax_a == 0 ? B.lo[0][q] : B.lo[1][q];
+        P.ahi[q] = ax_a == 0 ? B.hi[0][q] : B.hi[1][q];
+        P.blo[q] = ax_b == 1 ? B.lo[1][q] : B.lo[2][q];
+        P.bhi[q] = ax_b == 1 ? B.hi[1][q] : B.hi[2][q];
+        P.clo[q] = ax_c == 2 ? B.lo[2][q] : ax_c == 1 ? B.lo[1][q] : B.lo[0][q];
+        P.chi[q] = ax_c == 2 ? B.hi[2][q] : ax_c == 1 ? B.hi[1][q] : B.hi[0][q];
+    }
+    double pt[3];
+    pt[0] = ax_a == 0 ? p[0] : p[1];
+    pt[1] = ax_b == 1 ? p[1] : p[2];
+    pt[2] = ax_c == 2 ? p[2] : ax_c == 1 ? p[1] : p[0];
+    int fails = 0;
+    if (!projection(P, iso, t, cnt, w, pt, 9)) {
+        fails = 1;
+        if (!projection(P, iso, t, cnt, w, pt, 28)) fails = 2;
+    }
+    // back to (x, y, z)
+    double q0 = p[0], q1 = p[1], q2 = p[2];
+    if (ax_a == 0) q0 = pt[0]; else q1 = pt[0];
+    if (ax_b == 1) q1 = pt[1]; else q2 = pt[1];
+    if (ax_c == 2) q2 = pt[2]; else if (ax_c == 1) q1 = pt[2]; else q0 = pt[2];
+    p[0] = q0;
+    p[1] = q1;
+    p[2] = q2;
+    return fails;
+}
+
+// Bounding boxes of all contours and the float edge parameters
+// (cpp_dmc/DualMarchingCubes.cpp:113-139).
+__device__ inline void cell_boxes(const double F[8], double iso, unsigned icase, uint64_t w, int cnt,
+                                  BBoxes& B, float (&lpar)[12]) {
+#pragma unroll
+    for (int e = 0; e < 12; e++) lpar[e] = edge_cut(icase, e) ? edge_param(e, F, iso) : 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        B.lo[0][t] = B.lo[1][t] = B.lo[2][t] = 1.f;
+        B.hi[0][t] = B.hi[1][t] = B.hi[2][t] = 0.f;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (t >= cnt) continue;
+        const int start = poly_start(w, t), size = poly_size(w, t);
+        for (int m = 0; m < size; m++) {
+            const int e = poly_edge_at(w, start + m);
+            float l = 0.f;
+#pragma unroll
+            for (int q = 0; q < 12; q++) l = (q == e) ? lpar[q] : l;
+            double ui[3];
+            local_coords(e, l, ui);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float x = (float)ui[c];
+                B.lo[c][t] = B.lo[c][t] > x ? x : B.lo[c][t];
+                B.hi[c][t] = B.hi[c][t] < x ? x : B.hi[c][t];
+            }
+        }
+    }
+}
+
+}  // namespace tmc

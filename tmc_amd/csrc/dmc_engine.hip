@@ -1,0 +1,772 @@
+// dmc_engine.hip -- host runtime and C ABI of the MI355X DMC engine.
+//
+// One context = one HIP stream + a pool of device buffers that grow on
+// demand and are reused across extractions.  dmc_extract runs
+//   classify -> compact -> cell info -> scans -> vertices -> quads
+//   [-> 3X3Y colour] [-> 3X3Y isolated] [-> 3333] -> halfedges [-> triangles]
+// entirely on the device; the host only reads back the counts that size the
+// next launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tmc_dmc.h"
+#include "dmc_extract_kernels.h"
+#include "dmc_mesh_kernels.h"
+
+using namespace tmc;
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct StageMark {
+    const char* name;
+    hipEvent_t ev;
+};
+
+}  // namespace
+
+struct dmc_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<Buf*> bufs;
+    // cell sweep
+    Buf rowCount, rowBase, rowData, acell, acase, acode, anv, avbase, anq, aqbase;
+    Buf field, grads, fails, scanPartial, scanTotal;
+    // mesh (double-buffered where compacted)
+    Buf pos[2], nrm[2], quads[2], colors;
+    int cur = 0;  // which of the double buffers holds the live mesh
+    // halfedges / simplification scratch
+    Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
+        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats;
+    // results
+    dmc_mesh mesh{};
+    bool have_mesh = false;
+    std::vector<int32_t> bnd;  // sorted boundary vertices of the last final halfedge build
+    // stage timing
+    bool timing = false;
+    std::vector<StageMark> marks;
+    std::vector<std::pair<const char*, float>> stage_ms;
+
+    dmc_context() {
+        Buf* all[] = {&rowCount, &rowBase, &rowData, &acell, &acase, &acode, &anv, &avbase, &anq, &aqbase,
+                      &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
+                      &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
+                      &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
+                      &keepQ, &newQ, &clist, &ccnt, &tris, &stats};
+        for (Buf* b : all) bufs.push_back(b);
+    }
+};
+
+namespace {
+
+struct Fail {
+    dmc_status s;
+};
+
+void fail(dmc_context* ctx, dmc_status s, const char* fmt, ...) {
+    char msg[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(msg, sizeof(msg), fmt, ap);
+    va_end(ap);
+    ctx->err = msg;
+    throw Fail{s};
+}
+
+#define HIPC(ctx, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) fail(ctx, DMC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+T* ensure(dmc_context* ctx, Buf& b, size_t n) {
+    const size_t bytes = std::max<size_t>(n * sizeof(T), 256);
+    if (b.cap < bytes) {
+        if (b.p) {
+            HIPC(ctx, hipStreamSynchronize(ctx->stream));
+            HIPC(ctx, hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        const size_t want = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            fail(ctx, DMC_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        }
+        b.cap = want;
+    }
+    return static_cast<T*>(b.p);
+}
+
+int grid_for(int64_t n, int per_block = kBlock, int64_t cap = 1 << 20) {
+    int64_t g = (n + per_block - 1) / per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+void mark(dmc_context* ctx, const char* name) {
+    if (!ctx->timing) return;
+    StageMark m{name, nullptr};
+    HIPC(ctx, hipEventCreate(&m.ev));
+    HIPC(ctx, hipEventRecord(m.ev, ctx->stream));
+    ctx->marks.push_back(m);
+}
+
+void check_launch(dmc_context* ctx, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(ctx, DMC_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+}
+
+// Exclusive scan of n uint32 counts into out; returns the 64-bit total.
+uint64_t scan(dmc_context* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
+    if (n <= 0) return 0;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
+    uint64_t* tot = ensure<uint64_t>(ctx, ctx->scanTotal, 1);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, n, part);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, tiles, tot);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
+    check_launch(ctx, "scan");
+    uint64_t total = 0;
+    HIPC(ctx, hipMemcpyAsync(&total, tot, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return total;
+}
+
+uint64_t read_u64(dmc_context* ctx, const void* dptr) {
+    uint64_t v = 0;
+    HIPC(ctx, hipMemcpyAsync(&v, dptr, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// cell stage
+// ---------------------------------------------------------------------------
+template <class T>
+void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, double iso, const double* grads) {
+    Dims d{g.nx, g.ny, g.nz, (int64_t)g.nx - 1, (int64_t)(g.ny - 1) * (g.nz - 1)};
+    const int64_t R = d.R;
+    uint32_t* rowCount = ensure<uint32_t>(ctx, ctx->rowCount, R);
+    uint32_t* rowBase = ensure<uint32_t>(ctx, ctx->rowBase, R);
+    uint32_t* rowData = ensure<uint32_t>(ctx, ctx->rowData, (size_t)(R * d.C));
+    mark(ctx, "classify");
+    hipLaunchKernelGGL(k_classify<T>, dim3(grid_for(R, 1, (int64_t)1 << 30)), dim3(kBlock), 0, ctx->stream, field, d,
+                       iso_t, rowCount, rowData);
+    check_launch(ctx, "k_classify");
+    mark(ctx, "compact");
+    const uint64_t A = scan(ctx, rowCount, rowBase, R);
+    if (A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)A);
+    ctx->mesh.active_cells = (int64_t)A;
+    uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
+    uint8_t* acase = ensure<uint8_t>(ctx, ctx->acase, A);
+    uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
+    uint32_t* anv = ensure<uint32_t>(ctx, ctx->anv, A);
+    uint32_t* avbase = ensure<uint32_t>(ctx, ctx->avbase, A);
+    uint32_t* anq = ensure<uint32_t>(ctx, ctx->anq, A);
+    uint32_t* aqbase = ensure<uint32_t>(ctx, ctx->aqbase, A);
+    // fails[0..1]: failed projections; fails[2..3]: 64-bit contour count;
+    // fails[4]: vertices of valence 0 (all valence passes); fails[5]: quads
+    // with an unresolved neighbour cell
+    uint32_t* fails = ensure<uint32_t>(ctx, ctx->fails, 6);
+    unsigned long long* ncont = reinterpret_cast<unsigned long long*>(fails + 2);
+    HIPC(ctx, hipMemsetAsync(fails, 0, 6 * sizeof(uint32_t), ctx->stream));
+    if (A > 0) {
+        hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, rowCount,
+                           rowBase, rowData, d, acell, acase);
+        mark(ctx, "cell_info");
+        hipLaunchKernelGGL(k_cell_info<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, field, d, iso, acell,
+                           acase, (int64_t)A, acode, anv, anq, ncont);
+        check_launch(ctx, "k_cell_info");
+    }
+    const uint64_t V = scan(ctx, anv, avbase, A);
+    const uint64_t Q = scan(ctx, anq, aqbase, A);
+    if (V > 0x7FFFFFFFull || Q > 0x1FFFFFFFull)
+        fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)V, (unsigned long long)Q);
+    ctx->cur = 0;
+    double* pos = ensure<double>(ctx, ctx->pos[0], 3 * V);
+    double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * V);
+    int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * Q);
+    int32_t* colors = ensure<int32_t>(ctx, ctx->colors, Q);
+    GridGeom geo{g.x0, g.y0, g.z0, g.dx, g.dy, g.dz};
+    mark(ctx, "vertices");
+    if (A > 0) {
+        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, field, grads, d, geo, iso,
+                           acell, acode, avbase, (int64_t)A, pos, nrm, fails);
+        check_launch(ctx, "k_vertices");
+        mark(ctx, "quads");
+        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, d, acell, acode, avbase, aqbase,
+                           (int64_t)A, rowData, rowCount, rowBase, quads, colors, fails + 5);
+        check_launch(ctx, "k_quads");
+    }
+    uint32_t f[4] = {0, 0, 0, 0};
+    HIPC(ctx, hipMemcpyAsync(f, fails, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mesh.failed_projections_l1 = f[0];
+    ctx->mesh.failed_projections_l2 = f[1];
+    ctx->mesh.nv = (int64_t)V;
+    ctx->mesh.nq = (int64_t)Q;
+    ctx->mesh.nv_cells = (int64_t)V;
+    ctx->mesh.nq_cells = (int64_t)Q;
+    // vertices the reference emits before removeUnusedVertices: every contour
+    unsigned long long nc;
+    memcpy(&nc, f + 2, sizeof(nc));
+    ctx->mesh.nv_emitted = (int64_t)nc;
+}
+
+// ---------------------------------------------------------------------------
+// halfedges
+// ---------------------------------------------------------------------------
+HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
+    const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq, nhe = 4 * nq;
+    const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
+    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, nv);
+    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, nv);
+    uint32_t* cursor = ensure<uint32_t>(ctx, ctx->cursor, nv);
+    int32_t* adj = ensure<int32_t>(ctx, ctx->adj, nhe);
+    int32_t* twin = ensure<int32_t>(ctx, ctx->twin, nhe);
+    uint8_t* flag = ensure<uint8_t>(ctx, ctx->flag, nhe);
+    HeStats* st = ensure<HeStats>(ctx, ctx->stats, 1);
+    HIPC(ctx, hipMemsetAsync(st, 0, sizeof(HeStats), ctx->stream));
+    HeStats hs{};
+    if (nv > 0) {
+        HIPC(ctx, hipMemsetAsync(deg, 0, nv * sizeof(uint32_t), ctx->stream));
+        HIPC(ctx, hipMemsetAsync(cursor, 0, nv * sizeof(uint32_t), ctx->stream));
+    }
+    if (nhe > 0) {
+        hipLaunchKernelGGL(k_he_degree, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, deg,
+                           &st->bad);
+        check_launch(ctx, "k_he_degree");
+    }
+    scan(ctx, deg, off, nv);
+    if (nhe > 0) {
+        hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
+                           cursor, adj);
+        hipLaunchKernelGGL(k_he_sort, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, deg, nv, adj);
+        hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, deg,
+                           adj, twin, flag, st);
+        check_launch(ctx, "k_he_twins");
+    }
+    HIPC(ctx, hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    if (final_pass) {
+        int32_t* hev = ensure<int32_t>(ctx, ctx->hev, nv);
+        if (nv > 0)
+            hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, deg, adj, nv, hev);
+        ctx->bnd.clear();
+        if (hs.three > 0) {
+            unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, nv);
+            BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, hs.three);
+            unsigned int* nrec = ensure<unsigned int>(ctx, ctx->nbnd, 1);
+            HIPC(ctx, hipMemsetAsync(ovr, 0, nv * sizeof(unsigned long long), ctx->stream));
+            HIPC(ctx, hipMemsetAsync(nrec, 0, sizeof(unsigned int), ctx->stream));
+            hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
+                               deg, adj, ovr, recs, nrec, 0, hev);
+            hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
+                               deg, adj, ovr, recs, nrec, 1, hev);
+            check_launch(ctx, "k_he_three");
+            std::vector<BndRec> h(hs.three);
+            HIPC(ctx, hipMemcpyAsync(h.data(), recs, hs.three * sizeof(BndRec), hipMemcpyDeviceToHost, ctx->stream));
+            HIPC(ctx, hipStreamSynchronize(ctx->stream));
+            std::sort(h.begin(), h.end(), [](const BndRec& a, const BndRec& b) { return a.key < b.key; });
+            for (const BndRec& r : h) {
+                ctx->bnd.push_back(r.a);
+                ctx->bnd.push_back(r.b);
+            }
+        }
+    }
+    return hs;
+}
+
+int32_t* valence(dmc_context* ctx) {
+    const int64_t nv = ctx->mesh.nv;
+    int32_t* val = ensure<int32_t>(ctx, ctx->val, nv);
+    unsigned int* zero = static_cast<unsigned int*>(ctx->fails.p) + 4;
+    if (nv > 0)
+        hipLaunchKernelGGL(k_valence, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<uint32_t*>(ctx->off.p), static_cast<uint32_t*>(ctx->deg.p),
+                           static_cast<int32_t*>(ctx->adj.p), static_cast<int32_t*>(ctx->twin.p), nv, val, zero);
+    check_launch(ctx, "k_valence");
+    return val;
+}
+
+void color_faces(dmc_context* ctx) {
+    const int64_t nq = ctx->mesh.nq;
+    if (nq == 0) return;
+    int32_t* colors = static_cast<int32_t*>(ctx->colors.p);
+    uint32_t* cnt = ensure<uint32_t>(ctx, ctx->ccnt, 48);
+    int32_t* list = ensure<int32_t>(ctx, ctx->clist, nq);
+    HIPC(ctx, hipMemsetAsync(cnt, 0, 48 * sizeof(uint32_t), ctx->stream));
+    hipLaunchKernelGGL(k_color_count, dim3(grid_for(nq, kBlock, 4096)), dim3(kBlock), 0, ctx->stream, colors, nq, cnt);
+    uint32_t h[24];
+    HIPC(ctx, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t offs[24], o = 0;
+    for (int c = 0; c < 24; c++) {
+        offs[c] = o;
+        o += h[c];
+    }
+    HIPC(ctx, hipMemcpyAsync(cnt + 24, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_color_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, colors, nq, cnt + 24, list);
+    const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    for (int c = 5; c < 24; c++)
+        if (h[c])
+            hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[c])), dim3(kBlock), 0, ctx->stream, list + offs[c],
+                               (int64_t)h[c], twin, colors, 5);
+    if (h[4])
+        hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[4])), dim3(kBlock), 0, ctx->stream, list + offs[4],
+                           (int64_t)h[4], twin, colors, 4);
+    check_launch(ctx, "colour");
+}
+
+// Compact vertices by `removed`; vsecond (if non-null) receives new indices of
+// kept vertices.  Returns the new count; swaps the live buffers.
+int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond, uint32_t** newidx_out) {
+    const int64_t nv = ctx->mesh.nv;
+    uint32_t* keep = ensure<uint32_t>(ctx, ctx->keepV, nv);
+    uint32_t* newidx = ensure<uint32_t>(ctx, ctx->newV, nv);
+    if (nv > 0)
+        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, nv, keep);
+    const int64_t nn = (int64_t)scan(ctx, keep, newidx, nv);
+    const int src = ctx->cur, dst = 1 - ctx->cur;
+    double* pos2 = ensure<double>(ctx, ctx->pos[dst], 3 * nn);
+    double* nrm2 = ensure<double>(ctx, ctx->nrm[dst], 3 * nn);
+    if (nv > 0)
+        hipLaunchKernelGGL(k_compact_vertices, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv,
+                           static_cast<double*>(ctx->pos[src].p), static_cast<double*>(ctx->nrm[src].p), pos2, nrm2,
+                           vsecond);
+    check_launch(ctx, "k_compact_vertices");
+    if (newidx_out) *newidx_out = newidx;
+    return nn;
+}
+
+void simplify3x3y(dmc_context* ctx, bool colored) {
+    build_halfedges(ctx, false);
+    if (colored) color_faces(ctx);
+    int32_t* val = valence(ctx);
+    const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq;
+    const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
+    uint8_t* p = ensure<uint8_t>(ctx, ctx->p, nq);
+    uint8_t* em = ensure<uint8_t>(ctx, ctx->em, nq);
+    uint8_t* vfirst = ensure<uint8_t>(ctx, ctx->vfirst, nv);
+    int32_t* vsecond = ensure<int32_t>(ctx, ctx->vsecond, nv);
+    int32_t* lastT = ensure<int32_t>(ctx, ctx->lastT, nv);
+    int32_t* lastR = ensure<int32_t>(ctx, ctx->lastR, nv);
+    if (nv > 0) {
+        HIPC(ctx, hipMemsetAsync(vfirst, 0, nv, ctx->stream));
+        HIPC(ctx, hipMemsetAsync(vsecond, 0xFF, nv * sizeof(int32_t), ctx->stream));
+        HIPC(ctx, hipMemsetAsync(lastT, 0xFF, nv * sizeof(int32_t), ctx->stream));
+        HIPC(ctx, hipMemsetAsync(lastR, 0xFF, nv * sizeof(int32_t), ctx->stream));
+    }
+    const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
+    const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
+    if (nq > 0) {
+        hipLaunchKernelGGL(k_mark3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p);
+        hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
+                           colors, twin, flag, lastT, lastR, em, vfirst);
+        hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, val,
+                           lastT, lastR, static_cast<double*>(ctx->pos[ctx->cur].p),
+                           static_cast<double*>(ctx->nrm[ctx->cur].p), vsecond);
+        check_launch(ctx, "merge3x3y");
+    }
+    const int64_t nn = compact_vertices(ctx, vfirst, vsecond, nullptr);
+    // quads
+    uint32_t* keepQ = ensure<uint32_t>(ctx, ctx->keepQ, nq);
+    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, nq, keepQ);
+    const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
+    const int dst = 1 - ctx->cur;
+    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
+    HeStats* st = static_cast<HeStats*>(ctx->stats.p);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
+                           vfirst, vsecond, nv, q2, &st->bad);
+    check_launch(ctx, "k_compact_quads3x3y");
+    ctx->cur = dst;
+    ctx->mesh.nv = nn;
+    ctx->mesh.nq = qn;
+}
+
+void simplify3333(dmc_context* ctx) {
+    build_halfedges(ctx, false);
+    int32_t* val = valence(ctx);
+    const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq;
+    const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
+    uint8_t* vflag = ensure<uint8_t>(ctx, ctx->vfirst, nv);
+    int32_t* vtwin = ensure<int32_t>(ctx, ctx->vsecond, nv);
+    int32_t* lastF = ensure<int32_t>(ctx, ctx->lastT, nv);
+    uint8_t* p3333 = ensure<uint8_t>(ctx, ctx->p, nq);
+    uint8_t* rflag = ensure<uint8_t>(ctx, ctx->em, nq);
+    if (nv > 0) {
+        HIPC(ctx, hipMemsetAsync(vflag, 0, nv, ctx->stream));
+        HIPC(ctx, hipMemsetAsync(vtwin, 0xFF, nv * sizeof(int32_t), ctx->stream));
+        HIPC(ctx, hipMemsetAsync(lastF, 0xFF, nv * sizeof(int32_t), ctx->stream));
+    }
+    if (nq > 0) {
+        HIPC(ctx, hipMemsetAsync(p3333, 0, nq, ctx->stream));
+        HIPC(ctx, hipMemsetAsync(rflag, 0, nq, ctx->stream));
+    }
+    const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
+    if (nq > 0) {
+        hipLaunchKernelGGL(k_mark3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag, val,
+                           lastF, vflag, p3333, rflag);
+        hipLaunchKernelGGL(k_twin3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag,
+                           val, p3333, lastF, vtwin);
+        check_launch(ctx, "mark3333");
+    }
+    uint32_t* newv = nullptr;
+    const int64_t nn = compact_vertices(ctx, vflag, nullptr, &newv);
+    uint32_t* keepQ = ensure<uint32_t>(ctx, ctx->keepQ, nq);
+    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, rflag, nq, keepQ);
+    const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
+    const int dst = 1 - ctx->cur;
+    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
+    HeStats* st = static_cast<HeStats*>(ctx->stats.p);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
+                           newQ, p3333, vtwin, vflag, newv, nv, q2, &st->bad);
+    check_launch(ctx, "k_compact_quads3333");
+    ctx->cur = dst;
+    ctx->mesh.nv = nn;
+    ctx->mesh.nq = qn;
+}
+
+float smallest_float_not_below(double iso) {
+    float f = (float)iso;
+    if (std::isnan(iso)) return f;
+    if ((double)f < iso) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+void finish_timing(dmc_context* ctx) {
+    ctx->stage_ms.clear();
+    if (!ctx->timing || ctx->marks.empty()) return;
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i + 1 < ctx->marks.size(); i++) {
+        float ms = 0;
+        HIPC(ctx, hipEventElapsedTime(&ms, ctx->marks[i].ev, ctx->marks[i + 1].ev));
+        ctx->stage_ms.emplace_back(ctx->marks[i].name, ms);
+    }
+    for (auto& m : ctx->marks) (void)hipEventDestroy(m.ev);
+    ctx->marks.clear();
+}
+
+dmc_status guard(dmc_context* ctx, void (*fn)(dmc_context*, void*), void* arg) {
+    if (!ctx) return DMC_ERR_INVALID_ARG;
+    ctx->err.clear();
+    try {
+        HIPC(ctx, hipSetDevice(ctx->device));
+        fn(ctx, arg);
+        return DMC_OK;
+    } catch (const Fail& f) {
+        for (auto& m : ctx->marks) (void)hipEventDestroy(m.ev);
+        ctx->marks.clear();
+        return f.s;
+    }
+}
+
+struct ExtractArgs {
+    const dmc_grid* grid;
+    const void* field;
+    int field_type;
+    double iso;
+    uint32_t flags;
+    const double* grads;
+    dmc_mesh* out;
+};
+
+void do_extract(dmc_context* ctx, void* a_) {
+    ExtractArgs& a = *static_cast<ExtractArgs*>(a_);
+    const dmc_grid& g = *a.grid;
+    if (g.nx < 2 || g.ny < 2 || g.nz < 2) fail(ctx, DMC_ERR_INVALID_ARG, "grid needs at least 2 points per axis");
+    if (g.nx > (1 << 24)) fail(ctx, DMC_ERR_INVALID_ARG, "nx above 16777216");
+    const uint64_t npts = (uint64_t)g.nx * g.ny * g.nz;
+    const uint64_t ncells = (uint64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+    if (ncells > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "more than 2^32 cells");
+    if (a.field_type != DMC_FIELD_F32 && a.field_type != DMC_FIELD_F64)
+        fail(ctx, DMC_ERR_INVALID_ARG, "unknown field type %d", a.field_type);
+    if (!a.field) fail(ctx, DMC_ERR_INVALID_ARG, "field is NULL");
+    if ((a.flags & DMC_GRADIENTS_SUPPLIED) && !a.grads) fail(ctx, DMC_ERR_INVALID_ARG, "gradients are NULL");
+    const size_t esz = a.field_type == DMC_FIELD_F32 ? 4 : 8;
+    ctx->timing = (a.flags & DMC_STAGE_TIMING) != 0;
+    ctx->have_mesh = false;
+    ctx->mesh = dmc_mesh{};
+    mark(ctx, "upload");
+    const void* dfield = a.field;
+    const double* dgrads = (a.flags & DMC_GRADIENTS_SUPPLIED) ? a.grads : nullptr;
+    if (!(a.flags & DMC_FIELD_ON_DEVICE)) {
+        void* f = ensure<uint8_t>(ctx, ctx->field, npts * esz);
+        HIPC(ctx, hipMemcpyAsync(f, a.field, npts * esz, hipMemcpyHostToDevice, ctx->stream));
+        dfield = f;
+        if (dgrads) {
+            double* gd = ensure<double>(ctx, ctx->grads, 3 * npts);
+            HIPC(ctx, hipMemcpyAsync(gd, a.grads, 3 * npts * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            dgrads = gd;
+        }
+    }
+    if (a.field_type == DMC_FIELD_F32)
+        run_cells<float>(ctx, g, static_cast<const float*>(dfield), smallest_float_not_below(a.iso), a.iso, dgrads);
+    else
+        run_cells<double>(ctx, g, static_cast<const double*>(dfield), a.iso, a.iso, dgrads);
+    if (a.flags & DMC_SIMPLIFY_3X3Y_COLOR) {
+        mark(ctx, "p3x3y_color");
+        simplify3x3y(ctx, true);
+    }
+    if (a.flags & DMC_SIMPLIFY_3X3Y_ISOLATED) {
+        mark(ctx, "p3x3y_isolated");
+        simplify3x3y(ctx, false);
+    }
+    if (a.flags & DMC_SIMPLIFY_3333) {
+        mark(ctx, "p3333");
+        simplify3333(ctx);
+    }
+    mark(ctx, "halfedges");
+    const HeStats hs = build_halfedges(ctx, true);
+    ctx->mesh.nonmanifold_edges = (int64_t)hs.nonmanifold;
+    ctx->mesh.edge_errors = (int64_t)hs.errors;
+    ctx->mesh.invalid_refs = (int64_t)hs.bad;
+    ctx->mesh.nbnd = (int64_t)ctx->bnd.size();
+    ctx->mesh.nt = 0;
+    if (a.flags & DMC_TRIANGLES) {
+        mark(ctx, "triangles");
+        const int64_t nq = ctx->mesh.nq;
+        int32_t* tris = ensure<int32_t>(ctx, ctx->tris, 6 * nq);
+        if (nq > 0)
+            hipLaunchKernelGGL(k_triangles, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream,
+                               static_cast<int32_t*>(ctx->quads[ctx->cur].p), nq, ctx->mesh.nv,
+                               static_cast<double*>(ctx->pos[ctx->cur].p), tris);
+        check_launch(ctx, "k_triangles");
+        ctx->mesh.nt = 2 * nq;
+        ctx->mesh.d_tris = tris;
+    }
+    mark(ctx, "end");
+    uint32_t diag[6];
+    HIPC(ctx, hipMemcpyAsync(diag, ctx->fails.p, sizeof(diag), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mesh.valence_zero = diag[4];
+    ctx->mesh.invalid_refs += diag[5];
+    finish_timing(ctx);
+    ctx->mesh.d_positions = static_cast<double*>(ctx->pos[ctx->cur].p);
+    ctx->mesh.d_normals = static_cast<double*>(ctx->nrm[ctx->cur].p);
+    ctx->mesh.d_quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
+    ctx->mesh.d_twins = static_cast<int32_t*>(ctx->twin.p);
+    ctx->mesh.d_he_flags = static_cast<uint8_t*>(ctx->flag.p);
+    ctx->mesh.d_he_v = static_cast<int32_t*>(ctx->hev.p);
+    ctx->have_mesh = true;
+    if (a.out) *a.out = ctx->mesh;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dmc_status_str(dmc_status s) {
+    switch (s) {
+    case DMC_OK: return "ok";
+    case DMC_ERR_INVALID_ARG: return "invalid argument";
+    case DMC_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case DMC_ERR_CAPACITY: return "capacity exceeded";
+    case DMC_ERR_HIP: return "HIP error";
+    case DMC_ERR_NO_DEVICE: return "no device";
+    case DMC_ERR_BUDGET: return "memory budget exceeded";
+    case DMC_ERR_STATE: return "invalid state";
+    }
+    return "unknown status";
+}
+
+const char* dmc_last_error(dmc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+dmc_status dmc_create(dmc_context** out, int device) {
+    if (!out) return DMC_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return DMC_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return DMC_ERR_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return DMC_ERR_HIP;
+    dmc_context* ctx = new dmc_context();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return DMC_ERR_HIP;
+    }
+    *out = ctx;
+    return DMC_OK;
+}
+
+dmc_status dmc_destroy(dmc_context* ctx) {
+    if (!ctx) return DMC_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (Buf* b : ctx->bufs)
+        if (b->p) (void)hipFree(b->p);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DMC_OK;
+}
+
+dmc_status dmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type, double iso,
+                       uint32_t flags, const double* gradients, dmc_mesh* out) {
+    if (!ctx || !grid) return DMC_ERR_INVALID_ARG;
+    ExtractArgs a{grid, field, field_type, iso, flags, gradients, out};
+    return guard(ctx, do_extract, &a);
+}
+
+struct DownloadArgs {
+    double *pos, *nrm;
+    int32_t *quads, *tris, *he, *he_v, *he_f, *bnd;
+};
+
+static void do_download(dmc_context* ctx, void* a_) {
+    DownloadArgs& a = *static_cast<DownloadArgs*>(a_);
+    if (!ctx->have_mesh) fail(ctx, DMC_ERR_STATE, "no mesh: call dmc_extract first");
+    const dmc_mesh& m = ctx->mesh;
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (dst && bytes) HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    };
+    d2h(a.pos, m.d_positions, 3 * m.nv * sizeof(double));
+    d2h(a.nrm, m.d_normals, 3 * m.nv * sizeof(double));
+    d2h(a.quads, m.d_quads, 4 * m.nq * sizeof(int32_t));
+    d2h(a.tris, m.d_tris, 3 * m.nt * sizeof(int32_t));
+    d2h(a.he_v, m.d_he_v, m.nv * sizeof(int32_t));
+    std::vector<int32_t> q, tw;
+    std::vector<uint8_t> fl;
+    if (a.he) {
+        q.resize(4 * m.nq);
+        tw.resize(4 * m.nq);
+        fl.resize(4 * m.nq);
+        d2h(q.data(), m.d_quads, 4 * m.nq * sizeof(int32_t));
+        d2h(tw.data(), m.d_twins, 4 * m.nq * sizeof(int32_t));
+        d2h(fl.data(), m.d_he_flags, 4 * m.nq);
+    }
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    if (a.he) {
+        for (int64_t h = 0; h < 4 * m.nq; h++) {
+            int32_t* o = a.he + 5 * h;
+            o[0] = q[h];
+            o[1] = (int32_t)(h / 4);
+            o[2] = (int32_t)((h & ~3) | ((h + 1) & 3));
+            o[3] = tw[h];
+            o[4] = fl[h];
+        }
+    }
+    if (a.he_f)
+        for (int64_t f = 0; f < m.nq; f++) a.he_f[f] = (int32_t)(4 * f);
+    if (a.bnd && !ctx->bnd.empty()) memcpy(a.bnd, ctx->bnd.data(), ctx->bnd.size() * sizeof(int32_t));
+}
+
+dmc_status dmc_mesh_download(dmc_context* ctx, double* positions, double* normals, int32_t* quads, int32_t* tris,
+                             int32_t* halfedges, int32_t* he_v, int32_t* he_f, int32_t* bnd) {
+    DownloadArgs a{positions, normals, quads, tris, halfedges, he_v, he_f, bnd};
+    return guard(ctx, do_download, &a);
+}
+
+struct F32Args {
+    float *pos, *nrm;
+    int negate;
+};
+
+static void do_download_f32(dmc_context* ctx, void* a_) {
+    F32Args& a = *static_cast<F32Args*>(a_);
+    if (!ctx->have_mesh) fail(ctx, DMC_ERR_STATE, "no mesh: call dmc_extract first");
+    const int64_t n = 3 * ctx->mesh.nv;
+    std::vector<double> tmp(n);
+    if (a.pos) {
+        HIPC(ctx, hipMemcpy(tmp.data(), ctx->mesh.d_positions, n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) a.pos[i] = (float)tmp[i];
+    }
+    if (a.nrm) {
+        HIPC(ctx, hipMemcpy(tmp.data(), ctx->mesh.d_normals, n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) a.nrm[i] = a.negate ? -(float)tmp[i] : (float)tmp[i];
+    }
+}
+
+dmc_status dmc_mesh_download_f32(dmc_context* ctx, float* positions, float* normals, int negate_normals) {
+    F32Args a{positions, normals, negate_normals};
+    return guard(ctx, do_download_f32, &a);
+}
+
+dmc_status dmc_stage_times(dmc_context* ctx, const char** names, float* ms, int* count) {
+    if (!ctx || !count) return DMC_ERR_INVALID_ARG;
+    const int n = (int)ctx->stage_ms.size();
+    const int m = std::min(n, *count);
+    for (int i = 0; i < m; i++) {
+        if (names) names[i] = ctx->stage_ms[i].first;
+        if (ms) ms[i] = ctx->stage_ms[i].second;
+    }
+    *count = n;
+    return DMC_OK;
+}
+
+struct MemArgs {
+    void* dst;
+    const void* src;
+    size_t bytes;
+    void** out;
+    int kind;
+};
+
+static void do_mem(dmc_context* ctx, void* a_) {
+    MemArgs& a = *static_cast<MemArgs*>(a_);
+    switch (a.kind) {
+    case 0: {
+        hipError_t e = hipMalloc(a.out, a.bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(ctx, DMC_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", a.bytes, hipGetErrorString(e));
+        }
+        break;
+    }
+    case 1: HIPC(ctx, hipFree(a.dst)); break;
+    case 2: HIPC(ctx, hipMemcpy(a.dst, a.src, a.bytes, hipMemcpyHostToDevice)); break;
+    case 3: HIPC(ctx, hipMemcpy(a.dst, a.src, a.bytes, hipMemcpyDeviceToHost)); break;
+    case 4: HIPC(ctx, hipStreamSynchronize(ctx->stream)); break;
+    }
+}
+
+dmc_status dmc_device_alloc(dmc_context* ctx, size_t bytes, void** ptr) {
+    if (!ptr) return DMC_ERR_INVALID_ARG;
+    MemArgs a{nullptr, nullptr, bytes, ptr, 0};
+    return guard(ctx, do_mem, &a);
+}
+dmc_status dmc_device_free(dmc_context* ctx, void* ptr) {
+    MemArgs a{ptr, nullptr, 0, nullptr, 1};
+    return guard(ctx, do_mem, &a);
+}
+dmc_status dmc_copy_to_device(dmc_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return DMC_ERR_INVALID_ARG;
+    MemArgs a{dst, src, bytes, nullptr, 2};
+    return guard(ctx, do_mem, &a);
+}
+dmc_status dmc_copy_to_host(dmc_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return DMC_ERR_INVALID_ARG;
+    MemArgs a{dst, src, bytes, nullptr, 3};
+    return guard(ctx, do_mem, &a);
+}
+dmc_status dmc_synchronize(dmc_context* ctx) {
+    MemArgs a{nullptr, nullptr, 0, nullptr, 4};
+    return guard(ctx, do_mem, &a);
+}
+
+}  // extern "C"

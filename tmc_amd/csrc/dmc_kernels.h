@@ -1,0 +1,119 @@
+// dmc_kernels.h -- generic device building blocks: wave/block scans and the
+// device-wide exclusive scan used to turn per-item counts into offsets.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmc {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+// Block-wide exclusive scan; smem needs BS/64 + 1 entries.  Returns the
+// exclusive prefix of x and the block total.
+template <class T, int BS>
+__device__ __forceinline__ T block_exclusive_scan(T x, T* smem, T& total) {
+    constexpr int NW = BS / kWave;
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const T inc = wave_inclusive_scan(x);
+    if (lane == kWave - 1) smem[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        const T s = lane < NW ? smem[lane] : T(0);
+        const T si = wave_inclusive_scan(s);
+        if (lane < NW) smem[lane] = si - s;
+        if (lane == NW - 1) smem[NW] = si;
+    }
+    __syncthreads();
+    const T res = inc - x + smem[wid];
+    total = smem[NW];
+    __syncthreads();
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of uint32 counts (three passes over tiles of
+// kScanTile items).  Totals are accumulated in 64 bits so that overflow of
+// the 32-bit offsets can be detected by the host.
+// ---------------------------------------------------------------------------
+constexpr int kScanIPT = 16;
+constexpr int kScanTile = kBlock * kScanIPT;
+
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                         uint64_t* __restrict__ partial) {
+    __shared__ uint64_t sm[kBlock / kWave + 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanIPT; q++) {
+        const int64_t idx = base + (int64_t)q * kBlock + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    s = wave_sum(s);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kBlock / kWave; w++) t += sm[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// Single block: exclusive scan of the tile sums in place, total to *total.
+__global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ partial, int64_t ntiles,
+                                                         uint64_t* __restrict__ total) {
+    __shared__ uint64_t sm[1024 / kWave + 1];
+    uint64_t carry = 0;
+    for (int64_t b = 0; b < ntiles; b += 1024) {
+        const int64_t idx = b + threadIdx.x;
+        const uint64_t x = idx < ntiles ? partial[idx] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_exclusive_scan<uint64_t, 1024>(x, sm, tot);
+        if (idx < ntiles) partial[idx] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       int64_t n, const uint64_t* __restrict__ partial) {
+    __shared__ uint32_t sm[kBlock / kWave + 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    uint32_t v[kScanIPT];
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanIPT; q++) {
+        const int64_t idx = base + q;
+        v[q] = idx < n ? in[idx] : 0u;
+        s += v[q];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan<uint32_t, kBlock>(s, sm, tot) + (uint32_t)partial[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kScanIPT; q++) {
+        const int64_t idx = base + q;
+        if (idx < n) out[idx] = run;
+        run += v[q];
+    }
+}
+
+}  // namespace tmc

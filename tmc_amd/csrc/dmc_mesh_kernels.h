@@ -1,0 +1,583 @@
+// dmc_mesh_kernels.h -- quad-mesh stages of the DMC engine: halfedges,
+// vertex valence, face colouring, 3X3Y and 3333 simplification, triangles.
+//
+// Halfedge h = 4q + k runs from quads[4q+k] to quads[4q+(k+1)%4]; face and
+// next are implicit, twins and flags are stored (4 B + 1 B per halfedge).
+// Twins are found through a per-vertex CSR of outgoing halfedges instead of
+// the reference's map keyed by the vertex pair
+// (cpp_dmc/DualMarchingCubes.cpp:488-671): the halfedges sharing an edge
+// {a, b} are the out-halfedges of a ending at b plus those of b ending at a,
+// taken in ascending halfedge order, which is the map's insertion order.
+#pragma once
+#include "dmc_kernels.h"
+
+namespace tmc {
+
+__device__ __forceinline__ int32_t he_next(int32_t h) { return (h & ~3) | ((h + 1) & 3); }
+__device__ __forceinline__ int32_t he_prev(int32_t h) { return (h & ~3) | ((h + 3) & 3); }
+
+#define TMC_GRID_STRIDE(idx, n) \
+    for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < (n); idx += (int64_t)gridDim.x * kBlock)
+
+// Quad vertex ids must be in [0, nv); anything else (possible only after a
+// degenerate simplification) is counted and skipped.
+__device__ __forceinline__ bool vid_ok(int32_t v, int64_t nv) { return v >= 0 && (int64_t)v < nv; }
+
+__global__ void __launch_bounds__(kBlock) k_he_degree(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+                                                       uint32_t* __restrict__ deg, uint32_t* __restrict__ bad) {
+    TMC_GRID_STRIDE(h, nhe) {
+        const int32_t v = quads[h];
+        if (vid_ok(v, nv)) atomicAdd(&deg[v], 1u);
+        else atomicAdd(bad, 1u);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_he_scatter(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+                                                        const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+                                                        int32_t* __restrict__ adj) {
+    TMC_GRID_STRIDE(h, nhe) {
+        const int32_t v = quads[h];
+        if (!vid_ok(v, nv)) continue;
+        const uint32_t p = off[v] + atomicAdd(&cursor[v], 1u);
+        adj[p] = (int32_t)h;
+    }
+}
+
+// Sort each vertex's outgoing halfedges ascending (insertion sort; buckets
+// are small).
+__global__ void __launch_bounds__(kBlock) k_he_sort(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                                                     int64_t nv, int32_t* __restrict__ adj) {
+    TMC_GRID_STRIDE(v, nv) {
+        int32_t* b = adj + off[v];
+        const int n = (int)deg[v];
+        for (int x = 1; x < n; x++) {
+            const int32_t key = b[x];
+            int y = x - 1;
+            while (y >= 0 && b[y] > key) {
+                b[y + 1] = b[y];
+                y--;
+            }
+            b[y + 1] = key;
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t he_dest(const int32_t* __restrict__ quads, int32_t h) { return quads[he_next(h)]; }
+
+// Members of the edge group of halfedge h in ascending order (up to 5 kept,
+// the true count returned).
+__device__ __forceinline__ int he_group(const int32_t* __restrict__ quads, const uint32_t* __restrict__ off,
+                                        const uint32_t* __restrict__ deg, const int32_t* __restrict__ adj, int32_t h,
+                                        int64_t nv, int32_t g[5]) {
+    const int32_t a = quads[h], b = he_dest(quads, h);
+    if (!vid_ok(a, nv) || !vid_ok(b, nv)) return 0;
+    int32_t l1[8], l2[8];
+    int n1 = 0, n2 = 0, c1 = 0, c2 = 0;
+    {
+        const int32_t* p = adj + off[a];
+        const int n = (int)deg[a];
+        for (int x = 0; x < n; x++) {
+            const int32_t o = p[x];
+            if (he_dest(quads, o) == b) {
+                if (n1 < 8) l1[n1++] = o;
+                c1++;
+            }
+        }
+    }
+    if (b != a) {
+        const int32_t* p = adj + off[b];
+        const int n = (int)deg[b];
+        for (int x = 0; x < n; x++) {
+            const int32_t o = p[x];
+            if (he_dest(quads, o) == a) {
+                if (n2 < 8) l2[n2++] = o;
+                c2++;
+            }
+        }
+    }
+    // merge the two ascending lists
+    int x = 0, y = 0, m = 0;
+    while ((x < n1 || y < n2) && m < 5) {
+        if (y >= n2 || (x < n1 && l1[x] < l2[y])) g[m++] = l1[x++];
+        else g[m++] = l2[y++];
+    }
+    return c1 + c2;
+}
+
+// Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669).
+// The group's first member also counts the edge (non-manifold / error) and,
+// for three faces, records the override of he_v and the boundary vertices.
+struct HeStats {
+    unsigned long long nonmanifold;
+    unsigned long long errors;
+    unsigned long long three;    // number of 3-face edges
+    unsigned int bad;            // invalid vertex references
+    unsigned int pad;
+};
+
+__global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+                                                      const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                                                      const int32_t* __restrict__ adj, int32_t* __restrict__ twin,
+                                                      uint8_t* __restrict__ flag, HeStats* __restrict__ st) {
+    TMC_GRID_STRIDE(h64, nhe) {
+        const int32_t h = (int32_t)h64;
+        int32_t g[5];
+        const int c = he_group(quads, off, deg, adj, h, nv, g);
+        int32_t tw = -1;
+        uint8_t fl = 0;
+        if (c == 2) {
+            tw = (g[0] == h) ? g[1] : g[0];
+        } else if (c == 3) {
+            fl = 2;
+            const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
+            int32_t pa, pb;
+            if (v0 != v1) { pa = g[0]; pb = g[1]; }
+            else if (v0 != v2) { pa = g[0]; pb = g[2]; }
+            else { pa = g[1]; pb = g[2]; }
+            tw = (h == pa) ? pb : (h == pb) ? pa : -1;
+        } else if (c == 4) {
+            fl = 1;
+            if (quads[g[0]] != quads[g[1]]) {
+                tw = (h == g[0]) ? g[1] : (h == g[1]) ? g[0] : (h == g[2]) ? g[3] : g[2];
+            } else {
+                tw = (h == g[0]) ? g[2] : (h == g[2]) ? g[0] : (h == g[1]) ? g[3] : g[1];
+            }
+        }
+        twin[h] = tw;
+        flag[h] = fl;
+        if (c > 0 && g[0] == h) {
+            if (c == 3 || c == 4) atomicAdd(&st->nonmanifold, 1ull);
+            if (c == 3) atomicAdd(&st->three, 1ull);
+            if (c > 4) atomicAdd(&st->errors, 1ull);
+        }
+    }
+}
+
+// he_v: the last halfedge (in face order) leaving each vertex
+// (cpp_dmc/DualMarchingCubes.cpp:561-564).
+__global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                                                  const int32_t* __restrict__ adj, int64_t nv,
+                                                  int32_t* __restrict__ he_v) {
+    TMC_GRID_STRIDE(v, nv) {
+        const uint32_t n = deg[v];
+        he_v[v] = n ? adj[off[v] + n - 1] : -1;
+    }
+}
+
+__device__ __forceinline__ uint64_t he_key(int32_t v0, int32_t v1) {
+    return v0 < v1 ? ((uint64_t)(uint32_t)v0 << 32) | (uint32_t)v1 : ((uint64_t)(uint32_t)v1 << 32) | (uint32_t)v0;
+}
+
+// 3-face edges: the unpaired halfedge overrides he_v of its origin; the map
+// is walked in key order, so the largest key wins.  Records (key, two
+// boundary vertices) for getBndVertices.
+struct BndRec {
+    unsigned long long key;
+    int32_t a, b;
+};
+
+__global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+                                                      const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                                                      const int32_t* __restrict__ adj,
+                                                      unsigned long long* __restrict__ ovr, BndRec* __restrict__ recs,
+                                                      unsigned int* __restrict__ nrec, int pass,
+                                                      int32_t* __restrict__ he_v) {
+    TMC_GRID_STRIDE(h64, nhe) {
+        const int32_t h = (int32_t)h64;
+        int32_t g[5];
+        const int c = he_group(quads, off, deg, adj, h, nv, g);
+        if (c != 3 || g[0] != h) continue;
+        const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
+        int32_t ba, bb, lone;
+        if (v0 != v1) { ba = v0; bb = v1; lone = g[2]; }
+        else if (v0 != v2) { ba = v0; bb = v2; lone = g[1]; }
+        else { ba = v1; bb = v2; lone = g[0]; }
+        const int32_t vl = quads[lone];
+        const uint64_t key = he_key(quads[h], he_dest(quads, h));
+        if (pass == 0) {
+            atomicMax(&ovr[vl], (unsigned long long)key + 1ull);
+            const unsigned int p = atomicAdd(nrec, 1u);
+            recs[p].key = key;
+            recs[p].a = ba;
+            recs[p].b = bb;
+        } else if (ovr[vl] == (unsigned long long)key + 1ull) {
+            he_v[vl] = lone;
+        }
+    }
+}
+
+// Valence: outgoing halfedges plus boundary halfedges ending at the vertex
+// (cpp_dmc/DualMarchingCubes.cpp:847-871).  A halfedge ends at v exactly
+// when it is the predecessor of one leaving v.
+__global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                                                     const int32_t* __restrict__ adj, const int32_t* __restrict__ twin,
+                                                     int64_t nv, int32_t* __restrict__ val,
+                                                     unsigned int* __restrict__ zero) {
+    TMC_GRID_STRIDE(v, nv) {
+        const int32_t* p = adj + off[v];
+        const int n = (int)deg[v];
+        int s = 0;
+        for (int x = 0; x < n; x++) s += 1 + (twin[he_prev(p[x])] == -1);
+        val[v] = s;
+        if (s == 0) atomicAdd(zero, 1u);
+    }
+}
+
+__device__ __forceinline__ void face_neighbors(const int32_t* __restrict__ twin, int64_t f, int32_t nb[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int32_t t = twin[4 * f + k];
+        nb[k] = t >= 0 ? (t >> 2) : -1;
+    }
+}
+
+// Face colouring (cpp_dmc/DualMarchingCubes.cpp:714-844).  Faces are
+// bucketed by initial colour; classes 5..23 are processed in order, each
+// face taking the first colour in 0..4 unused by its edge neighbours, then
+// the initially colour-4 faces try 0..3.  Faces of one initial colour are
+// never edge neighbours (their grid edges are parallel edges of different
+// parity), so each class is processed in parallel.
+__global__ void __launch_bounds__(kBlock) k_color_count(const int32_t* __restrict__ colors, int64_t nq,
+                                                         uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t sm[32];
+    if (threadIdx.x < 32) sm[threadIdx.x] = 0;
+    __syncthreads();
+    TMC_GRID_STRIDE(f, nq) {
+        const int32_t c = colors[f];
+        if (c >= 0 && c < 24) atomicAdd(&sm[c], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 24 && sm[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], sm[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restrict__ colors, int64_t nq,
+                                                           uint32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+    TMC_GRID_STRIDE(f, nq) {
+        const int32_t c = colors[f];
+        if (c >= 0 && c < 24) list[atomicAdd(&cursor[c], 1u)] = (int32_t)f;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restrict__ list, int64_t n,
+                                                         const int32_t* __restrict__ twin, int32_t* __restrict__ colors,
+                                                         int ncol) {
+    TMC_GRID_STRIDE(x, n) {
+        const int32_t f = list[x];
+        int32_t nb[4];
+        face_neighbors(twin, f, nb);
+        int32_t c[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[k] = nb[k] >= 0 ? colors[nb[k]] : -1;
+        for (int col = 0; col < ncol; col++) {
+            if (c[0] != col && c[1] != col && c[2] != col && c[3] != col) {
+                colors[f] = col;
+                break;
+            }
+        }
+    }
+}
+
+// 3X3Y pattern (cpp_dmc/DualMarchingCubes.cpp:889-913)
+__global__ void __launch_bounds__(kBlock) k_mark3x3y(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+                                                      const int32_t* __restrict__ val, uint8_t* __restrict__ p) {
+    TMC_GRID_STRIDE(f, nq) {
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        bool ok = vid_ok(q.x, nv) && vid_ok(q.y, nv) && vid_ok(q.z, nv) && vid_ok(q.w, nv);
+        uint8_t r = 0;
+        if (ok) {
+            const int a = val[q.x], b = val[q.y], c = val[q.z], d = val[q.w];
+            r = (a == 3 && b >= 5 && c == 3 && d >= 5) || (a >= 5 && b == 3 && c >= 5 && d == 3);
+        }
+        p[f] = r;
+    }
+}
+
+__device__ __forceinline__ bool face_nonmanifold(const uint8_t* __restrict__ flag, int32_t f) {
+    return flag[4 * f] == 1 || flag[4 * f + 1] == 1 || flag[4 * f + 2] == 1 || flag[4 * f + 3] == 1;
+}
+
+// Decision of mergeVertices3X3Y for face f (colored: :916-1012, isolated:
+// :1014-1094).  Depends only on data fixed before the merge, so all faces
+// decide in parallel.  Returns 0 (keep), 1 (move v0, drop v2), 2 (move v1,
+// drop v3).
+__device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restrict__ quads,
+                                              const int32_t* __restrict__ val, const uint8_t* __restrict__ p,
+                                              const int32_t* __restrict__ colors, const int32_t* __restrict__ twin,
+                                              const uint8_t* __restrict__ flag, int64_t nv) {
+    if (!p[f]) return 0;
+    int32_t nb[4];
+    face_neighbors(twin, f, nb);
+    if (colors) {
+        const int32_t c = colors[f];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (nb[k] < 0 || face_nonmanifold(flag, nb[k])) return 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (p[nb[k]] && colors[nb[k]] <= c) return 0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (nb[k] >= 0 && face_nonmanifold(flag, nb[k])) return 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (nb[k] >= 0 && p[nb[k]]) return 0;
+    }
+    const int4 q = reinterpret_cast<const int4*>(quads)[f];
+    const int a = val[q.x], b = val[q.y], c = val[q.z], d = val[q.w];
+    if (a == 3 && b >= 5 && c == 3 && d >= 5) return 1;
+    if (a >= 5 && b == 3 && c >= 5 && d == 3) return 2;
+    return 0;
+}
+
+// Pass A: every merging face claims its moved and dropped vertex; the
+// reference loops over faces in order, so the largest face index wins.
+__global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+                                                         const int32_t* __restrict__ val, const uint8_t* __restrict__ p,
+                                                         const int32_t* __restrict__ colors,
+                                                         const int32_t* __restrict__ twin,
+                                                         const uint8_t* __restrict__ flag, int32_t* __restrict__ lastT,
+                                                         int32_t* __restrict__ lastR, uint8_t* __restrict__ em,
+                                                         uint8_t* __restrict__ vfirst) {
+    TMC_GRID_STRIDE(f, nq) {
+        const int m = merge_decision(f, quads, val, p, colors, twin, flag, nv);
+        em[f] = m != 0;
+        if (!m) continue;
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        const int32_t t = m == 1 ? q.x : q.y;
+        const int32_t r = m == 1 ? q.z : q.w;
+        atomicMax(&lastT[t], (int32_t)f);
+        atomicMax(&lastR[r], (int32_t)f);
+        vfirst[r] = 1;
+    }
+}
+
+// Pass B: the winning face writes the midpoint position and the normalised
+// mean normal (reads only valence >= 5 vertices, which are never written)
+// and the drop map.
+__global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restrict__ quads, int64_t nq,
+                                                         const uint8_t* __restrict__ em,
+                                                         const int32_t* __restrict__ val,
+                                                         const int32_t* __restrict__ lastT,
+                                                         const int32_t* __restrict__ lastR, double* __restrict__ pos,
+                                                         double* __restrict__ nrm, int32_t* __restrict__ vsecond) {
+    TMC_GRID_STRIDE(f, nq) {
+        if (!em[f]) continue;
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        const bool first = val[q.x] == 3;     // pattern (3, >=5, 3, >=5)
+        const int32_t t = first ? q.x : q.y;
+        const int32_t r = first ? q.z : q.w;
+        const int32_t s0 = first ? q.y : q.x;
+        const int32_t s1 = first ? q.w : q.z;
+        if (lastT[t] == (int32_t)f) {
+            double np[3], nn[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                np[c] = pos[3 * (int64_t)s0 + c] + 0.5 * (pos[3 * (int64_t)s1 + c] - pos[3 * (int64_t)s0 + c]);
+                nn[c] = nrm[3 * (int64_t)s0 + c] + 0.5 * (nrm[3 * (int64_t)s1 + c] - nrm[3 * (int64_t)s0 + c]);
+            }
+            const double s = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            nn[0] /= s;
+            nn[1] /= s;
+            nn[2] /= s;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                pos[3 * (int64_t)t + c] = np[c];
+                nrm[3 * (int64_t)t + c] = nn[c];
+            }
+        }
+        if (lastR[r] == (int32_t)f) vsecond[r] = t;
+    }
+}
+
+// keep flag (1 - removed) as uint32 for the scan
+__global__ void __launch_bounds__(kBlock) k_keep_from_flag(const uint8_t* __restrict__ removed, int64_t n,
+                                                            uint32_t* __restrict__ keep) {
+    TMC_GRID_STRIDE(x, n) keep[x] = removed[x] ? 0u : 1u;
+}
+
+// Kept vertices: record their new index in vsecond and move their data.
+__global__ void __launch_bounds__(kBlock) k_compact_vertices(const uint8_t* __restrict__ removed,
+                                                              const uint32_t* __restrict__ newidx, int64_t nv,
+                                                              const double* __restrict__ pos,
+                                                              const double* __restrict__ nrm,
+                                                              double* __restrict__ pos2, double* __restrict__ nrm2,
+                                                              int32_t* __restrict__ vsecond) {
+    TMC_GRID_STRIDE(v, nv) {
+        if (removed[v]) continue;
+        const int64_t o = newidx[v];
+        if (vsecond) vsecond[v] = (int32_t)o;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            pos2[3 * o + c] = pos[3 * v + c];
+            nrm2[3 * o + c] = nrm[3 * v + c];
+        }
+    }
+}
+
+// removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137)
+__global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, int64_t nq,
+                                                               const uint8_t* __restrict__ em,
+                                                               const uint32_t* __restrict__ newq,
+                                                               const uint8_t* __restrict__ vfirst,
+                                                               const int32_t* __restrict__ vsecond, int64_t nv,
+                                                               int32_t* __restrict__ quads2,
+                                                               unsigned int* __restrict__ bad) {
+    TMC_GRID_STRIDE(f, nq) {
+        if (em[f]) continue;
+        int32_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int32_t x = quads[4 * f + k];
+            int32_t y = -1;
+            if (vid_ok(x, nv)) {
+                if (vfirst[x]) {
+                    const int32_t t = vsecond[x];
+                    y = vid_ok(t, nv) ? vsecond[t] : -1;
+                } else {
+                    y = vsecond[x];
+                }
+            }
+            if (y < 0) atomicAdd(bad, 1u);
+            r[k] = y;
+        }
+        reinterpret_cast<int4*>(quads2)[newq[f]] = make_int4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// mark3333 (cpp_dmc/DualMarchingCubes.cpp:1190-1301).  Returns true with
+// the face's four vertices, outer vertices and neighbour faces if f is a
+// removable all-valence-3 face.
+__device__ __forceinline__ bool pattern3333(int64_t f, const int32_t* __restrict__ quads,
+                                            const int32_t* __restrict__ twin, const uint8_t* __restrict__ flag,
+                                            const int32_t* __restrict__ val, int64_t nv, int32_t vq[4],
+                                            int32_t vo[4], int32_t fn[4]) {
+    if (face_nonmanifold(flag, (int32_t)f)) return false;
+    const int4 q = reinterpret_cast<const int4*>(quads)[f];
+    vq[0] = q.x; vq[1] = q.y; vq[2] = q.z; vq[3] = q.w;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (!vid_ok(vq[k], nv) || val[vq[k]] != 3) return false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int32_t tw = twin[4 * f + k];
+        if (tw == -1) return false;
+        fn[k] = tw >> 2;
+        const int32_t n2 = he_next(he_next(tw));
+        vo[k] = quads[n2];
+        if (!vid_ok(vo[k], nv)) return false;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (face_nonmanifold(flag, fn[k])) return false;
+    const int a4 = val[vo[0]], a5 = val[vo[1]], a6 = val[vo[2]], a7 = val[vo[3]];
+    if ((a4 == 3 && a5 == 3) || (a5 == 3 && a6 == 3) || (a6 == 3 && a7 == 3) || (a7 == 3 && a4 == 3)) return false;
+    if (a4 == 3 || a5 == 3 || a6 == 3 || a7 == 3) return false;
+    if (a4 == 4 && a5 == 4 && a6 == 4 && a7 == 4) return false;
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mark3333(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+                                                      const int32_t* __restrict__ twin, const uint8_t* __restrict__ flag,
+                                                      const int32_t* __restrict__ val, int32_t* __restrict__ lastF,
+                                                      uint8_t* __restrict__ vflag, uint8_t* __restrict__ p3333,
+                                                      uint8_t* __restrict__ rflag) {
+    TMC_GRID_STRIDE(f, nq) {
+        int32_t vq[4], vo[4], fn[4];
+        const bool m = pattern3333(f, quads, twin, flag, val, nv, vq, vo, fn);
+        if (!m) continue;
+        p3333[f] = 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            vflag[vq[k]] = 1;
+            atomicMax(&lastF[vq[k]], (int32_t)f);
+            rflag[fn[k]] = 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+                                                      const int32_t* __restrict__ twin, const uint8_t* __restrict__ flag,
+                                                      const int32_t* __restrict__ val,
+                                                      const uint8_t* __restrict__ p3333,
+                                                      const int32_t* __restrict__ lastF, int32_t* __restrict__ vtwin) {
+    TMC_GRID_STRIDE(f, nq) {
+        if (!p3333[f]) continue;
+        int32_t vq[4], vo[4], fn[4];
+        pattern3333(f, quads, twin, flag, val, nv, vq, vo, fn);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (lastF[vq[k]] == (int32_t)f) vtwin[vq[k]] = vo[k];
+    }
+}
+
+// removeQuadrilaterals3333 (cpp_dmc/DualMarchingCubes.cpp:1320-1360)
+__global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __restrict__ quads, int64_t nq,
+                                                               const uint8_t* __restrict__ rflag,
+                                                               const uint32_t* __restrict__ newq,
+                                                               const uint8_t* __restrict__ p3333,
+                                                               const int32_t* __restrict__ vtwin,
+                                                               const uint8_t* __restrict__ vflag,
+                                                               const uint32_t* __restrict__ newv, int64_t nv,
+                                                               int32_t* __restrict__ quads2,
+                                                               unsigned int* __restrict__ bad) {
+    TMC_GRID_STRIDE(f, nq) {
+        if (rflag[f]) continue;
+        int32_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int32_t x = quads[4 * f + k];
+            if (p3333[f] && vid_ok(x, nv)) x = vtwin[x];
+            int32_t y = -1;
+            if (vid_ok(x, nv) && !vflag[x]) y = (int32_t)newv[x];
+            if (y < 0) atomicAdd(bad, 1u);
+            r[k] = y;
+        }
+        reinterpret_cast<int4*>(quads2)[newq[f]] = make_int4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// collectTriangles (cpp_dmc/DualMarchingCubes.cpp:1387-1460) with minAngle
+// (DualMarchingCubes.h:746-756): split along the diagonal whose two
+// triangles have the larger minimum angle.
+__device__ __forceinline__ double tri_min_angle(const double* __restrict__ pos, int32_t i0, int32_t i1, int32_t i2) {
+    auto dist = [&](int32_t p, int32_t q) {
+        const double* a = pos + 3 * (int64_t)p;
+        const double* b = pos + 3 * (int64_t)q;
+        const double d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+        return sqrt(d);
+    };
+    const float a = dist(i0, i1);
+    const float b = dist(i1, i2);
+    const float c = dist(i2, i0);
+    const float A = acosf((b * b + c * c - a * a) / (2 * b * c));
+    const float B = acosf((a * a + c * c - b * b) / (2 * a * c));
+    const float C = acosf((b * b + a * a - c * c) / (2 * b * a));
+    const float m = (B < A) ? B : A;
+    return (C < m) ? C : m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_triangles(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+                                                       const double* __restrict__ pos, int32_t* __restrict__ tris) {
+    TMC_GRID_STRIDE(f, nq) {
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        int32_t* t = tris + 6 * f;
+        if (!(vid_ok(q.x, nv) && vid_ok(q.y, nv) && vid_ok(q.z, nv) && vid_ok(q.w, nv))) {
+            t[0] = q.x; t[1] = q.y; t[2] = q.z; t[3] = q.x; t[4] = q.z; t[5] = q.w;
+            continue;
+        }
+        double a1 = tri_min_angle(pos, q.x, q.y, q.z);
+        double a2 = tri_min_angle(pos, q.x, q.z, q.w);
+        const double b1 = (a2 < a1) ? a2 : a1, b2 = (a1 < a2) ? a2 : a1;
+        a1 = tri_min_angle(pos, q.y, q.w, q.x);
+        a2 = tri_min_angle(pos, q.y, q.z, q.w);
+        const double c1 = (a2 < a1) ? a2 : a1, c2 = (a1 < a2) ? a2 : a1;
+        if (b1 < c1 || (b1 == c1 && b2 <= c2)) {
+            t[0] = q.y; t[1] = q.w; t[2] = q.x; t[3] = q.y; t[4] = q.z; t[5] = q.w;
+        } else {
+            t[0] = q.x; t[1] = q.y; t[2] = q.z; t[3] = q.x; t[4] = q.z; t[5] = q.w;
+        }
+    }
+}
+
+}  // namespace tmc

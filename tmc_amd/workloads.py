@@ -1,0 +1,90 @@
+"""Synthetic workloads of the benchmark configurations (libtmc_workloads.so).
+
+Each generator returns (field float32 array (nz, ny, nx), Grid, iso):
+
+* ``genus2(n)``        -- genus-2 implicit on [-1.5,1.5]^3, cell-centred, iso 0
+* ``blobs(n, seed)``   -- 64 Gaussian blobs + N(0, 0.05^2) noise, cell-centred
+                          on [0,1]^3, iso 0.5
+* ``schwarz(n, seed)`` -- cos x + cos y + cos z on [-8pi,8pi]^3, node-aligned
+                          (x_i = -8pi + i*16pi/(n-1)), + N(0, 0.01^2), iso 0.3
+* ``batch_specs``      -- the mixed batch of small volumes
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import Grid
+from .build import WORKLOADS_SO
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(WORKLOADS_SO):
+            raise RuntimeError(f"{WORKLOADS_SO} not built: run `python -m tmc_amd.build`")
+        L = C.CDLL(WORKLOADS_SO)
+        fp = C.POINTER(C.c_float)
+        L.wl_genus2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, fp, C.c_int]
+        L.wl_schwarz.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                 fp, C.c_int]
+        L.wl_blobs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_uint64, fp, C.c_int]
+        L.wl_uniform.argtypes = [C.c_uint64, C.c_uint64]
+        L.wl_uniform.restype = C.c_double
+        L.wl_normal.argtypes = [C.c_uint64, C.c_uint64]
+        L.wl_normal.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _shape3(n):
+    return (n, n, n) if np.isscalar(n) else tuple(int(x) for x in n)   # (nz, ny, nx)
+
+
+def _threads():
+    return int(os.environ.get("TMC_WORKLOAD_THREADS", "0"))
+
+
+def genus2(n=128, lo=-1.5, hi=1.5):
+    nz, ny, nx = _shape3(n)
+    f = np.empty((nz, ny, nx), np.float32)
+    _load().wl_genus2(nx, ny, nz, lo, hi, f.ctypes.data_as(C.POINTER(C.c_float)), _threads())
+    h = [(hi - lo) / m for m in (nx, ny, nz)]
+    g = Grid(nx, ny, nz, lo + 0.5 * h[0], lo + 0.5 * h[1], lo + 0.5 * h[2], h[0], h[1], h[2])
+    return f, g, 0.0
+
+
+def schwarz(n=1024, seed=3, sigma=0.01, lo=-8 * math.pi, hi=8 * math.pi, iso=0.3):
+    nz, ny, nx = _shape3(n)
+    f = np.empty((nz, ny, nx), np.float32)
+    _load().wl_schwarz(nx, ny, nz, lo, hi, sigma, seed, f.ctypes.data_as(C.POINTER(C.c_float)), _threads())
+    g = Grid(nx, ny, nz, lo, lo, lo, (hi - lo) / (nx - 1), (hi - lo) / (ny - 1), (hi - lo) / (nz - 1))
+    return f, g, iso
+
+
+def blobs(n=512, seed=2, nblobs=64, sigma=0.05, iso=0.5):
+    nz, ny, nx = _shape3(n)
+    f = np.empty((nz, ny, nx), np.float32)
+    _load().wl_blobs(nx, ny, nz, nblobs, 0.02, 0.08, 0.5, 1.5, sigma, seed,
+                     f.ctypes.data_as(C.POINTER(C.c_float)), _threads())
+    g = Grid(nx, ny, nz, 0.5 / nx, 0.5 / ny, 0.5 / nz, 1.0 / nx, 1.0 / ny, 1.0 / nz)
+    return f, g, iso
+
+
+CONFIGS = {
+    "genus2_128": lambda: genus2(128),
+    "blobs_512": lambda: blobs(512),
+    "schwarz_1024": lambda: schwarz(1024),
+}
+
+
+def uniform(seed, k):
+    return _load().wl_uniform(seed, k)
+
+
+def normal(seed, idx):
+    return _load().wl_normal(seed, idx)
