@@ -226,8 +226,10 @@ def test_cpp_dropin_programs(oracle, tmp_path):
     assert_same(rd("v.f64", np.float64).reshape(-1, 3), exp["v"], "cpp v")
     assert_same(rd("n.f64", np.float64).reshape(-1, 3), exp["n"], "cpp n")
     assert_same(rd("quads.i32", np.int32).reshape(-1, 4), exp["quads"], "cpp quads")
-    # file path: float field, gradients estimated + flipped
+    # file path: float field, gradients estimated + flipped; the reference's
+    # loader puts the grid origin at 0 (cpp_dmc/UniformGrid.cpp:21-29)
     s32 = s.astype(np.float32).astype(np.float64)
-    exp2 = oracle.dualmc(g, s32, oracle.gradient(g, s32), 0.0, True)
+    g0 = Grid(64, 64, 64, 0.0, 0.0, 0.0, *geo[3:])
+    exp2 = oracle.dualmc(g0, s32, oracle.gradient(g0, s32), 0.0, True)
     assert_same(rd("file_v.f64", np.float64).reshape(-1, 3), exp2["v"], "file v")
     assert_same(rd("file_quads.i32", np.int32).reshape(-1, 4), exp2["quads"], "file quads")

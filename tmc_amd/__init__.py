@@ -193,6 +193,8 @@ class Engine:
                 gptr = C.c_void_p(g.ctypes.data)
         if gradients is not None:
             flags |= GRADIENTS_SUPPLIED
+        if not isinstance(grid, Grid):
+            grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
         m = Mesh()
         self._check(self.lib.dmc_extract(self.ctx, C.byref(grid), fptr, ftype, float(iso), int(flags), gptr,
                                          C.byref(m)))
