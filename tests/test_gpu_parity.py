@@ -233,3 +233,4 @@ def test_cpp_dropin_programs(oracle, tmp_path):
     exp2 = oracle.dualmc(g0, s32, oracle.gradient(g0, s32), 0.0, True)
     assert_same(rd("file_v.f64", np.float64).reshape(-1, 3), exp2["v"], "file v")
     assert_same(rd("file_quads.i32", np.int32).reshape(-1, 4), exp2["quads"], "file quads")
+

@@ -185,10 +185,23 @@ __device__ __forceinline__ T pick8(const T* a, int t) {
     return (t & 4) ? hi : lo;
 }
 
+// Corner values of one cell kept in LDS (one column per thread) for reads
+// with a run-time corner index; register arrays indexed that way end up in
+// scratch memory.
+constexpr int kCornerStride = 256;
+struct CornerRef {
+    double* p;   // &corners[0][threadIdx.x] of a [8][kCornerStride] LDS array
+    __device__ __forceinline__ double operator()(int c) const { return p[c * kCornerStride]; }
+    __device__ __forceinline__ void store(const double F[8]) const {
+#pragma unroll
+        for (int c = 0; c < 8; c++) p[c * kCornerStride] = F[c];
+    }
+};
+
 // Edge parameter rounded to float (cpp_dmc/DualMarchingCubes.h:391-407): the
 // coordinate along the edge axis is l, the others 0 or 1 by the position of v0.
-__device__ __forceinline__ float edge_param(int e, const double F[8], double iso) {
-    const double f0 = pick8(F, edge_v0(e)), f1 = pick8(F, edge_v1(e));
+__device__ __forceinline__ float edge_param(int e, const CornerRef& Fc, double iso) {
+    const double f0 = Fc(edge_v0(e)), f1 = Fc(edge_v1(e));
     const float l = (iso - f0) / (f1 - f0);
     return l;
 }
@@ -202,11 +215,21 @@ __device__ __forceinline__ void local_coords(int e, float l, double u[3]) {
     }
 }
 
-// Per-contour bounding boxes.  All entries are floats widened to double in
-// the reference, so float storage is exact.
-struct BBoxes {
-    float lo[3][4];
-    float hi[3][4];
+// Per-contour bounding boxes live in LDS, one column per thread: entry
+// (axis, lo/hi, contour) of thread x is at base[((axis*2 + hi)*4 + t) * 256 + x].
+// All entries are floats widened to double in the reference, so float
+// storage is exact.
+constexpr int kBoxStride = 256;
+constexpr int kBoxFloats = 24;
+
+struct BoxRef {
+    float* p;   // &box[0][threadIdx.x] of a [24][kBoxStride] LDS array
+    __device__ __forceinline__ float lo(int ax, int t) const { return p[((ax * 2) * 4 + t) * kBoxStride]; }
+    __device__ __forceinline__ float hi(int ax, int t) const { return p[((ax * 2 + 1) * 4 + t) * kBoxStride]; }
+    __device__ __forceinline__ void set(int ax, int t, float l, float h) const {
+        p[((ax * 2) * 4 + t) * kBoxStride] = l;
+        p[((ax * 2 + 1) * 4 + t) * kBoxStride] = h;
+    }
 };
 
 // cpp_dmc/DualMarchingCubes.h:590-597
@@ -216,41 +239,37 @@ __device__ __forceinline__ void trilinear_gradient(const double f[8], double u, 
     g[2] = (1 - v) * ((1 - u) * (f[4] - f[0]) + u * (f[5] - f[1])) + v * ((1 - u) * (f[6] - f[2]) + u * (f[7] - f[3]));
 }
 
-// Projection sampling (cpp_dmc/DualMarchingCubes.cpp:274-326).  The caller
-// passes the eight corner values already permuted for the projection axis
-// (g[0..3] = bilinear face at w = 0, g[4..7] at w = 1), the bounds of the
-// contour's box in the projected frame and, for the neighbour-box test, the
-// boxes of the other size-3 contours in the same frame.
-struct ProjFrame {
-    double g[8];
-    float alo[4], ahi[4], blo[4], bhi[4], clo[4], chi[4];   // boxes per contour (u, v, w)
-};
-
-__device__ inline bool projection(const ProjFrame& P, double iso, int t, int cnt, uint64_t w,
-                                  double p[3], int ns) {
-    const double umin = pick4(P.alo, t), umax = pick4(P.ahi, t);
-    const double vmin = pick4(P.blo, t), vmax = pick4(P.bhi, t);
+// Projection sampling (cpp_dmc/DualMarchingCubes.cpp:274-326).  The frame
+// (a, b, c) maps the projected (u, v, w) axes to cell axes; g holds the
+// eight corner values in frame order (g[0..3] = bilinear face at w = 0,
+// g[4..7] at w = 1).  Boxes of other size-3 contours are only consulted for
+// multi-contour cells (DualMarchingCubes.h:500-519).
+__device__ inline bool projection(const double g[8], const BoxRef& B, int axa, int axb, int axc, double iso, int t,
+                                  int cnt, uint64_t w, double p[3], int ns) {
+    const double umin = B.lo(axa, t), umax = B.hi(axa, t);
+    const double vmin = B.lo(axb, t), vmax = B.hi(axb, t);
     const double du = (umax - umin) / (ns - 1);
     const double dv = (vmax - vmin) / (ns - 1);
     double minDistance = 100;
     float bu = 0.f, bv = 0.f, bw = -1.f;
     bool found = false;
-    double wmin = pick4(P.clo, t), wmax = pick4(P.chi, t);
+    double wmin = B.lo(axc, t), wmax = B.hi(axc, t);
     const double eps = 1e-5;
     if (fabs(wmax - wmin) < eps) {
         wmin -= eps;
         wmax += eps;
     }
     const int cnt_size = poly_size(w, t);
-    // Neighbour boxes of size-3 contours other than t (DualMarchingCubes.h:500-519).
-    bool nb[4];
+    unsigned nbmask = 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) nb[q] = q < cnt && q != t && poly_size(w, q) == 3;
-    const bool test_nb = !(cnt == 1 || cnt_size == 3);
-    const double f0 = P.g[0], f1 = P.g[1], f2 = P.g[2], f3 = P.g[3];
-    const double f4 = P.g[4], f5 = P.g[5], f6 = P.g[6], f7 = P.g[7];
+    for (int q = 0; q < 4; q++) nbmask |= (q < cnt && q != t && poly_size(w, q) == 3) ? (1u << q) : 0u;
+    if (cnt == 1 || cnt_size == 3) nbmask = 0;
+    const double f0 = g[0], f1 = g[1], f2 = g[2], f3 = g[3];
+    const double f4 = g[4], f5 = g[5], f6 = g[6], f7 = g[7];
+#pragma unroll 1
     for (int i = 1; i < (ns - 1); i++) {
         const float u = umin + i * du;
+#pragma unroll 1
         for (int j = 1; j < (ns - 1); j++) {
             const float v = vmin + j * dv;
             const float g1 = (1 - v) * ((1 - u) * f0 + u * f1) + v * ((1 - u) * f2 + u * f3);
@@ -259,14 +278,13 @@ __device__ inline bool projection(const ProjFrame& P, double iso, int t, int cnt
             const float ww = (iso - g1) / (g2 - g1);
             if (wmin <= ww && ww <= wmax) {
                 bool inside = false;
-                if (test_nb) {
-#pragma unroll
+                if (nbmask) {
+                    const double du_ = u, dv_ = v, dw_ = ww;
                     for (int q = 0; q < 4; q++) {
-                        if (!nb[q] || inside) continue;
-                        const double du_ = u, dv_ = v, dw_ = ww;
-                        if (du_ < (double)P.alo[q] || (double)P.ahi[q] < du_) continue;
-                        if (dv_ < (double)P.blo[q] || (double)P.bhi[q] < dv_) continue;
-                        if (dw_ < (double)P.clo[q] || (double)P.chi[q] < dw_) continue;
+                        if (!((nbmask >> q) & 1u) || inside) continue;
+                        if (du_ < (double)B.lo(axa, q) || (double)B.hi(axa, q) < du_) continue;
+                        if (dv_ < (double)B.lo(axb, q) || (double)B.hi(axb, q) < dv_) continue;
+                        if (dw_ < (double)B.lo(axc, q) || (double)B.hi(axc, q) < dw_) continue;
                         inside = true;
                     }
                 }
@@ -294,18 +312,17 @@ __device__ inline bool projection(const ProjFrame& P, double iso, int t, int cnt
 // Representative of contour t in local cell coordinates
 // (cpp_dmc/DualMarchingCubes.cpp:141-156 and :194-272).  Returns the
 // number of failed projection levels (0, 1 or 2).
-__device__ inline int contour_representative(const double F[8], double iso, uint64_t w, int cnt, int t,
-                                             const BBoxes& B, const float (&lpar)[12], double p[3]) {
+__device__ inline int contour_representative(const double F[8], const CornerRef& Fc, double iso, uint64_t w,
+                                             int cnt, int t, const BoxRef& B, double p[3]) {
     // mean of the edge intersections, summed in contour order
     const int start = poly_start(w, t), size = poly_size(w, t);
-    p[0] = 0; p[1] = 0; p[2] = 0;
+    p[0] = 0;
+    p[1] = 0;
+    p[2] = 0;
     for (int m = 0; m < size; m++) {
         const int e = poly_edge_at(w, start + m);
         double ui[3];
-        float l = 0.f;
-#pragma unroll
-        for (int q = 0; q < 12; q++) l = (q == e) ? lpar[q] : l;
-        local_coords(e, l, ui);
+        local_coords(e, edge_param(e, Fc, iso), ui);
         p[0] += ui[0];
         p[1] += ui[1];
         p[2] += ui[2];
@@ -313,7 +330,7 @@ __device__ inline int contour_representative(const double F[8], double iso, uint
     p[0] /= size;
     p[1] /= size;
     p[2] /= size;
-    // projection axis
+    // projection axis: W = 1, V = 2, U = 3
     int prj;
     if (size >= 6) {
         double g[3];
@@ -324,85 +341,61 @@ __device__ inline int contour_representative(const double F[8], double iso, uint
         else if (dx >= dy && dx >= dz) prj = 3;
         else prj = 1;
     } else {
-        const double eu = (double)pick4(B.hi[0], t) - (double)pick4(B.lo[0], t);
-        const double ev = (double)pick4(B.hi[1], t) - (double)pick4(B.lo[1], t);
-        const double ew = (double)pick4(B.hi[2], t) - (double)pick4(B.lo[2], t);
+        const double eu = (double)B.hi(0, t) - (double)B.lo(0, t);
+        const double ev = (double)B.hi(1, t) - (double)B.lo(1, t);
+        const double ew = (double)B.hi(2, t) - (double)B.lo(2, t);
         if (ew <= eu && ew <= ev) prj = 1;
         else if (ev <= eu && ev <= ew) prj = 2;
         else if (eu <= ev && eu <= ew) prj = 3;
         else prj = 1;
     }
-    // frame: (a, b, c) = projected (u, v, w) axes as permutation of (x, y, z)
-    ProjFrame P;
-    int ax_a, ax_b, ax_c;
-    if (prj == 1) { ax_a = 0; ax_b = 1; ax_c = 2; }
-    else if (prj == 2) { ax_a = 0; ax_b = 2; ax_c = 1; }
-    else { ax_a = 1; ax_b = 2; ax_c = 0; }
-    // corner (a, b, c) in the frame -> cell corner
+    // frame (a, b, c): W = (x, y, z), V = (x, z, y), U = (y, z, x)
+    const int axa = prj == 3 ? 1 : 0;
+    const int axb = prj == 1 ? 1 : 2;
+    const int axc = prj == 1 ? 2 : prj == 2 ? 1 : 0;
+    double g[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
-        const int corner = (a << ax_a) | (b << ax_b) | (c << ax_c);
-        P.g[q] = pick8(F, corner);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        P.alo[q] = ax_a == 0 ? B.lo[0][q] : B.lo[1][q];
-        P.ahi[q] = ax_a == 0 ? B.hi[0][q] : B.hi[1][q];
-        P.blo[q] = ax_b == 1 ? B.lo[1][q] : B.lo[2][q];
-        P.bhi[q] = ax_b == 1 ? B.hi[1][q] : B.hi[2][q];
-        P.clo[q] = ax_c == 2 ? B.lo[2][q] : ax_c == 1 ? B.lo[1][q] : B.lo[0][q];
-        P.chi[q] = ax_c == 2 ? B.hi[2][q] : ax_c == 1 ? B.hi[1][q] : B.hi[0][q];
+        const int corner = ((q & 1) << axa) | (((q >> 1) & 1) << axb) | (((q >> 2) & 1) << axc);
+        g[q] = Fc(corner);
     }
     double pt[3];
-    pt[0] = ax_a == 0 ? p[0] : p[1];
-    pt[1] = ax_b == 1 ? p[1] : p[2];
-    pt[2] = ax_c == 2 ? p[2] : ax_c == 1 ? p[1] : p[0];
+    pt[0] = axa == 0 ? p[0] : p[1];
+    pt[1] = axb == 1 ? p[1] : p[2];
+    pt[2] = axc == 2 ? p[2] : axc == 1 ? p[1] : p[0];
     int fails = 0;
-    if (!projection(P, iso, t, cnt, w, pt, 9)) {
+    if (!projection(g, B, axa, axb, axc, iso, t, cnt, w, pt, 9)) {
         fails = 1;
-        if (!projection(P, iso, t, cnt, w, pt, 28)) fails = 2;
+        if (!projection(g, B, axa, axb, axc, iso, t, cnt, w, pt, 28)) fails = 2;
     }
-    // back to (x, y, z)
     double q0 = p[0], q1 = p[1], q2 = p[2];
-    if (ax_a == 0) q0 = pt[0]; else q1 = pt[0];
-    if (ax_b == 1) q1 = pt[1]; else q2 = pt[1];
-    if (ax_c == 2) q2 = pt[2]; else if (ax_c == 1) q1 = pt[2]; else q0 = pt[2];
+    if (axa == 0) q0 = pt[0]; else q1 = pt[0];
+    if (axb == 1) q1 = pt[1]; else q2 = pt[1];
+    if (axc == 2) q2 = pt[2]; else if (axc == 1) q1 = pt[2]; else q0 = pt[2];
     p[0] = q0;
     p[1] = q1;
     p[2] = q2;
     return fails;
 }
 
-// Bounding boxes of all contours and the float edge parameters
-// (cpp_dmc/DualMarchingCubes.cpp:113-139).
-__device__ inline void cell_boxes(const double F[8], double iso, unsigned icase, uint64_t w, int cnt,
-                                  BBoxes& B, float (&lpar)[12]) {
-#pragma unroll
-    for (int e = 0; e < 12; e++) lpar[e] = edge_cut(icase, e) ? edge_param(e, F, iso) : 0.f;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        B.lo[0][t] = B.lo[1][t] = B.lo[2][t] = 1.f;
-        B.hi[0][t] = B.hi[1][t] = B.hi[2][t] = 0.f;
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        if (t >= cnt) continue;
+// Bounding boxes of all contours (cpp_dmc/DualMarchingCubes.cpp:113-139).
+__device__ inline void cell_boxes(const CornerRef& Fc, double iso, uint64_t w, int cnt, const BoxRef& B) {
+    for (int t = 0; t < cnt; t++) {
         const int start = poly_start(w, t), size = poly_size(w, t);
+        float lo[3] = {1.f, 1.f, 1.f}, hi[3] = {0.f, 0.f, 0.f};
         for (int m = 0; m < size; m++) {
             const int e = poly_edge_at(w, start + m);
-            float l = 0.f;
-#pragma unroll
-            for (int q = 0; q < 12; q++) l = (q == e) ? lpar[q] : l;
             double ui[3];
-            local_coords(e, l, ui);
+            local_coords(e, edge_param(e, Fc, iso), ui);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const float x = (float)ui[c];
-                B.lo[c][t] = B.lo[c][t] > x ? x : B.lo[c][t];
-                B.hi[c][t] = B.hi[c][t] < x ? x : B.hi[c][t];
+                lo[c] = lo[c] > x ? x : lo[c];
+                hi[c] = hi[c] < x ? x : hi[c];
             }
         }
+#pragma unroll
+        for (int c = 0; c < 3; c++) B.set(c, t, lo[c], hi[c]);
     }
 }
 

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -29,6 +30,11 @@ struct Buf {
     size_t cap = 0;
 };
 
+// Grid-stride kernels run on at most this many blocks (persistent style):
+// enough to fill every CU several times over, few enough that waves live
+// long and keep many loads in flight.
+int kGridCap = 16384;
+
 struct StageMark {
     const char* name;
     hipEvent_t ev;
@@ -42,7 +48,7 @@ struct dmc_context {
     std::string err;
     std::vector<Buf*> bufs;
     // cell sweep
-    Buf rowCount, rowBase, rowData, acell, acase, acode, anv, avbase, anq, aqbase;
+    Buf segCount, segBase, lists, acell, acode, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
     Buf field, grads, fails, scanPartial, scanTotal;
     // mesh (double-buffered where compacted)
     Buf pos[2], nrm[2], quads[2], colors;
@@ -60,7 +66,7 @@ struct dmc_context {
     std::vector<std::pair<const char*, float>> stage_ms;
 
     dmc_context() {
-        Buf* all[] = {&rowCount, &rowBase, &rowData, &acell, &acase, &acode, &anv, &avbase, &anq, &aqbase,
+        Buf* all[] = {&segCount, &segBase, &lists, &acell, &acode, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
                       &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
@@ -113,7 +119,7 @@ T* ensure(dmc_context* ctx, Buf& b, size_t n) {
     return static_cast<T*>(b.p);
 }
 
-int grid_for(int64_t n, int per_block = kBlock, int64_t cap = 1 << 20) {
+int grid_for(int64_t n, int per_block = kBlock, int64_t cap = kGridCap) {
     int64_t g = (n + per_block - 1) / per_block;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
@@ -161,40 +167,52 @@ template <class T>
 void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, double iso, const double* grads) {
     Dims d{g.nx, g.ny, g.nz, (int64_t)g.nx - 1, (int64_t)(g.ny - 1) * (g.nz - 1)};
     const int64_t R = d.R;
-    uint32_t* rowCount = ensure<uint32_t>(ctx, ctx->rowCount, R);
-    uint32_t* rowBase = ensure<uint32_t>(ctx, ctx->rowBase, R);
-    uint32_t* rowData = ensure<uint32_t>(ctx, ctx->rowData, (size_t)(R * d.C));
-    mark(ctx, "classify");
-    hipLaunchKernelGGL(k_classify<T>, dim3(grid_for(R, 1, (int64_t)1 << 30)), dim3(kBlock), 0, ctx->stream, field, d,
-                       iso_t, rowCount, rowData);
-    check_launch(ctx, "k_classify");
-    mark(ctx, "compact");
-    const uint64_t A = scan(ctx, rowCount, rowBase, R);
+    const int S = (int)((d.C + kSeg - 1) / kSeg);
+    const int64_t RS = R * S;
+    uint32_t* segCount = ensure<uint32_t>(ctx, ctx->segCount, RS);
+    uint32_t* segBase = ensure<uint32_t>(ctx, ctx->segBase, RS);
+    uint32_t* lists = ensure<uint32_t>(ctx, ctx->lists, (size_t)RS * kSeg);
+    // sweep tiling: J rows per wave, K planes per wave, enough waves to fill
+    // the chip (about 32 per CU)
+    const int ntj = (g.ny - 1 + kSweepRows - 1) / kSweepRows;
+    const int64_t per_plane = (int64_t)S * ntj;
+    int K = (int)std::max<int64_t>(8, ((int64_t)(g.nz - 1) * per_plane + 8191) / 8192);
+    K = std::min(K, g.nz - 1);
+    const int ntk = (g.nz - 1 + K - 1) / K;
+    const int64_t waves = per_plane * ntk;
+    const bool vec = (g.nx % 4 == 0) && ((uintptr_t)field % 16 == 0);
+    mark(ctx, "sweep");
+    const dim3 sgrid((unsigned)((waves + kBlock / kWave - 1) / (kBlock / kWave)));
+    if (vec)
+        hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, field, d, iso_t, S, ntj, K, ntk,
+                           lists, segCount);
+    else
+        hipLaunchKernelGGL((k_sweep<T, false>), sgrid, dim3(kBlock), 0, ctx->stream, field, d, iso_t, S, ntj, K, ntk,
+                           lists, segCount);
+    check_launch(ctx, "k_sweep");
+    mark(ctx, "rows");
+    const uint64_t A = scan(ctx, segCount, segBase, RS);
     if (A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)A);
     ctx->mesh.active_cells = (int64_t)A;
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
-    uint8_t* acase = ensure<uint8_t>(ctx, ctx->acase, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
-    uint32_t* anv = ensure<uint32_t>(ctx, ctx->anv, A);
-    uint32_t* avbase = ensure<uint32_t>(ctx, ctx->avbase, A);
-    uint32_t* anq = ensure<uint32_t>(ctx, ctx->anq, A);
-    uint32_t* aqbase = ensure<uint32_t>(ctx, ctx->aqbase, A);
+    uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, A);
+    uint32_t* aqloc = ensure<uint32_t>(ctx, ctx->aqloc, A);
+    uint32_t* rowV = ensure<uint32_t>(ctx, ctx->rowV, R);
+    uint32_t* rowQ = ensure<uint32_t>(ctx, ctx->rowQ, R);
+    uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
+    uint32_t* rowBaseQ = ensure<uint32_t>(ctx, ctx->rowBaseQ, R);
     // fails[0..1]: failed projections; fails[2..3]: 64-bit contour count;
     // fails[4]: vertices of valence 0 (all valence passes); fails[5]: quads
     // with an unresolved neighbour cell
     uint32_t* fails = ensure<uint32_t>(ctx, ctx->fails, 6);
     unsigned long long* ncont = reinterpret_cast<unsigned long long*>(fails + 2);
     HIPC(ctx, hipMemsetAsync(fails, 0, 6 * sizeof(uint32_t), ctx->stream));
-    if (A > 0) {
-        hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, rowCount,
-                           rowBase, rowData, d, acell, acase);
-        mark(ctx, "cell_info");
-        hipLaunchKernelGGL(k_cell_info<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, field, d, iso, acell,
-                           acase, (int64_t)A, acode, anv, anq, ncont);
-        check_launch(ctx, "k_cell_info");
-    }
-    const uint64_t V = scan(ctx, anv, avbase, A);
-    const uint64_t Q = scan(ctx, anq, aqbase, A);
+    hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, field, d, iso, S,
+                       lists, segCount, segBase, acell, acode, avloc, aqloc, rowV, rowQ, ncont);
+    check_launch(ctx, "k_rows");
+    const uint64_t V = scan(ctx, rowV, rowBaseV, R);
+    const uint64_t Q = scan(ctx, rowQ, rowBaseQ, R);
     if (V > 0x7FFFFFFFull || Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)V, (unsigned long long)Q);
     ctx->cur = 0;
@@ -206,11 +224,12 @@ void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, dou
     mark(ctx, "vertices");
     if (A > 0) {
         hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, field, grads, d, geo, iso,
-                           acell, acode, avbase, (int64_t)A, pos, nrm, fails);
+                           acell, acode, avloc, rowBaseV, (int64_t)A, pos, nrm, fails);
         check_launch(ctx, "k_vertices");
         mark(ctx, "quads");
-        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, d, acell, acode, avbase, aqbase,
-                           (int64_t)A, rowData, rowCount, rowBase, quads, colors, fails + 5);
+        SegLists L{lists, segCount, segBase, S};
+        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, d, acell, acode, avloc, aqloc,
+                           rowBaseV, rowBaseQ, (int64_t)A, L, quads, colors, fails + 5);
         check_launch(ctx, "k_quads");
     }
     uint32_t f[4] = {0, 0, 0, 0};
@@ -231,13 +250,38 @@ void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, dou
 // ---------------------------------------------------------------------------
 // halfedges
 // ---------------------------------------------------------------------------
+// Final pass extras shared by both halfedge paths: 3-face edges override
+// he_v and give the boundary vertices (sorted by key on the host).
+template <class ThreeLaunch>
+void three_face_edges(dmc_context* ctx, const HeStats& hs, int64_t nv, ThreeLaunch launch) {
+    ctx->bnd.clear();
+    if (hs.three == 0) return;
+    unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, nv);
+    BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, hs.three);
+    unsigned int* nrec = ensure<unsigned int>(ctx, ctx->nbnd, 1);
+    HIPC(ctx, hipMemsetAsync(ovr, 0, nv * sizeof(unsigned long long), ctx->stream));
+    HIPC(ctx, hipMemsetAsync(nrec, 0, sizeof(unsigned int), ctx->stream));
+    launch(ovr, recs, nrec, 0);
+    launch(ovr, recs, nrec, 1);
+    check_launch(ctx, "three-face edges");
+    std::vector<BndRec> h(hs.three);
+    HIPC(ctx, hipMemcpyAsync(h.data(), recs, hs.three * sizeof(BndRec), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    std::sort(h.begin(), h.end(), [](const BndRec& a, const BndRec& b) { return a.key < b.key; });
+    for (const BndRec& r : h) {
+        ctx->bnd.push_back(r.a);
+        ctx->bnd.push_back(r.b);
+    }
+}
+
+// Halfedges through a CSR of outgoing (halfedge, destination) pairs.
 HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq, nhe = 4 * nq;
     const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
     uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, nv);
     uint32_t* off = ensure<uint32_t>(ctx, ctx->off, nv);
     uint32_t* cursor = ensure<uint32_t>(ctx, ctx->cursor, nv);
-    int32_t* adj = ensure<int32_t>(ctx, ctx->adj, nhe);
+    int2* adj = ensure<int2>(ctx, ctx->adj, nhe);
     int32_t* twin = ensure<int32_t>(ctx, ctx->twin, nhe);
     uint8_t* flag = ensure<uint8_t>(ctx, ctx->flag, nhe);
     HeStats* st = ensure<HeStats>(ctx, ctx->stats, 1);
@@ -254,9 +298,8 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     }
     scan(ctx, deg, off, nv);
     if (nhe > 0) {
-        hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
+        hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
                            cursor, adj);
-        hipLaunchKernelGGL(k_he_sort, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, deg, nv, adj);
         hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, deg,
                            adj, twin, flag, st);
         check_launch(ctx, "k_he_twins");
@@ -267,27 +310,10 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
         int32_t* hev = ensure<int32_t>(ctx, ctx->hev, nv);
         if (nv > 0)
             hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, deg, adj, nv, hev);
-        ctx->bnd.clear();
-        if (hs.three > 0) {
-            unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, nv);
-            BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, hs.three);
-            unsigned int* nrec = ensure<unsigned int>(ctx, ctx->nbnd, 1);
-            HIPC(ctx, hipMemsetAsync(ovr, 0, nv * sizeof(unsigned long long), ctx->stream));
-            HIPC(ctx, hipMemsetAsync(nrec, 0, sizeof(unsigned int), ctx->stream));
+        three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
             hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
-                               deg, adj, ovr, recs, nrec, 0, hev);
-            hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
-                               deg, adj, ovr, recs, nrec, 1, hev);
-            check_launch(ctx, "k_he_three");
-            std::vector<BndRec> h(hs.three);
-            HIPC(ctx, hipMemcpyAsync(h.data(), recs, hs.three * sizeof(BndRec), hipMemcpyDeviceToHost, ctx->stream));
-            HIPC(ctx, hipStreamSynchronize(ctx->stream));
-            std::sort(h.begin(), h.end(), [](const BndRec& a, const BndRec& b) { return a.key < b.key; });
-            for (const BndRec& r : h) {
-                ctx->bnd.push_back(r.a);
-                ctx->bnd.push_back(r.b);
-            }
-        }
+                               deg, adj, ovr, recs, nrec, pass, hev);
+        });
     }
     return hs;
 }
@@ -296,11 +322,12 @@ int32_t* valence(dmc_context* ctx) {
     const int64_t nv = ctx->mesh.nv;
     int32_t* val = ensure<int32_t>(ctx, ctx->val, nv);
     unsigned int* zero = static_cast<unsigned int*>(ctx->fails.p) + 4;
-    if (nv > 0)
+    if (nv > 0) {
         hipLaunchKernelGGL(k_valence, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
                            static_cast<uint32_t*>(ctx->off.p), static_cast<uint32_t*>(ctx->deg.p),
-                           static_cast<int32_t*>(ctx->adj.p), static_cast<int32_t*>(ctx->twin.p), nv, val, zero);
-    check_launch(ctx, "k_valence");
+                           static_cast<int2*>(ctx->adj.p), static_cast<int32_t*>(ctx->twin.p), nv, val, zero);
+    }
+    check_launch(ctx, "valence");
     return val;
 }
 
@@ -308,27 +335,23 @@ void color_faces(dmc_context* ctx) {
     const int64_t nq = ctx->mesh.nq;
     if (nq == 0) return;
     int32_t* colors = static_cast<int32_t*>(ctx->colors.p);
-    uint32_t* cnt = ensure<uint32_t>(ctx, ctx->ccnt, 48);
+    const int nb = (int)std::min<int64_t>(kColorBlocks, (nq + kBlock - 1) / kBlock);
+    uint32_t* hist = ensure<uint32_t>(ctx, ctx->ccnt, (size_t)kColorBlocks * 24 + 48);
+    uint32_t* totals = hist + (size_t)kColorBlocks * 24;
     int32_t* list = ensure<int32_t>(ctx, ctx->clist, nq);
-    HIPC(ctx, hipMemsetAsync(cnt, 0, 48 * sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_color_count, dim3(grid_for(nq, kBlock, 4096)), dim3(kBlock), 0, ctx->stream, colors, nq, cnt);
-    uint32_t h[24];
-    HIPC(ctx, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist);
+    hipLaunchKernelGGL(k_color_offsets, dim3(1), dim3(64), 0, ctx->stream, hist, nb, totals);
+    hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist, list);
+    uint32_t h[48];
+    HIPC(ctx, hipMemcpyAsync(h, totals, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    uint32_t offs[24], o = 0;
-    for (int c = 0; c < 24; c++) {
-        offs[c] = o;
-        o += h[c];
-    }
-    HIPC(ctx, hipMemcpyAsync(cnt + 24, offs, sizeof(offs), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_color_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, colors, nq, cnt + 24, list);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     for (int c = 5; c < 24; c++)
         if (h[c])
-            hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[c])), dim3(kBlock), 0, ctx->stream, list + offs[c],
+            hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[c])), dim3(kBlock), 0, ctx->stream, list + h[24 + c],
                                (int64_t)h[c], twin, colors, 5);
     if (h[4])
-        hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[4])), dim3(kBlock), 0, ctx->stream, list + offs[4],
+        hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[4])), dim3(kBlock), 0, ctx->stream, list + h[24 + 4],
                            (int64_t)h[4], twin, colors, 4);
     check_launch(ctx, "colour");
 }
@@ -605,6 +628,10 @@ dmc_status dmc_create(dmc_context** out, int device) {
     }
     if (device < 0 || device >= n) return DMC_ERR_INVALID_ARG;
     if (hipSetDevice(device) != hipSuccess) return DMC_ERR_HIP;
+    if (const char* cap = getenv("TMC_GRID_CAP")) {
+        const long v = strtol(cap, nullptr, 10);
+        if (v >= 64 && v <= (1l << 24)) kGridCap = (int)v;
+    }
     dmc_context* ctx = new dmc_context();
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

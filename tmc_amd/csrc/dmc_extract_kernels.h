@@ -1,14 +1,20 @@
-// dmc_extract_kernels.h -- cell-sweep kernels of the DMC engine: classify,
-// compact, per-cell polygons, vertex representatives and quadrilaterals.
+// dmc_extract_kernels.h -- cell-sweep kernels of the DMC engine: the field
+// sweep, per-row polygon codes, vertex representatives and quadrilaterals.
 //
 // Data layout in HBM:
-//   field      nx*ny*nz scalars, x fastest (T = float or double)
-//   cell rows  r = k*(ny-1) + j, C = nx-1 cells per row
-//   row lists  rowData[r*C + m]  = i | icase << 24 for the m-th active cell of
-//              row r (ascending i); rowCount[r]; rowBase[r] = exclusive scan
-//   active     a = rowBase[r] + m enumerates active cells in (k, j, i) order,
-//              which is the reference's vertex order
-//              (cpp_dmc/DualMarchingCubes.cpp:16-50 with :167-171).
+//   field     nx*ny*nz scalars, x fastest (T = float or double)
+//   rows      r = k*(ny-1) + j holds C = nx-1 cells, cut into S segments of
+//             256 cells along x
+//   lists     lists[(r*S + s)*256 + m] = i | case << 24: the m-th active
+//             cell of segment s of row r (ascending i); segCount[r*S + s];
+//             segBase = exclusive scan of segCount
+//   active    a = segBase[r*S + s] + m enumerates the active cells in
+//             (k, j, i) order, the reference's vertex order
+//             (cpp_dmc/DualMarchingCubes.cpp:16-50 with :167-171):
+//             acell[a] (linear cell index), acode[a] (case | edge-to-contour
+//             map << 8), avloc[a] / aqloc[a] (first vertex / quadrilateral
+//             of the cell relative to its row); rowBaseV / rowBaseQ are the
+//             scans of the row totals.
 #pragma once
 #include "dmc_cell.h"
 #include "dmc_kernels.h"
@@ -25,80 +31,104 @@ template <class T>
 __device__ __forceinline__ bool at_or_above(T f, T iso_t) { return f >= iso_t; }
 
 // ---------------------------------------------------------------------------
-// K1: classify every cell of a row and write the row's active list.
-// One 256-thread block per row, 4 consecutive cells per thread.  For F32
-// fields the comparison (double)f >= iso is done as f >= iso_up, iso_up the
-// smallest float not below iso, which is the same predicate.
+// K1: the field sweep.  A wave owns one 256-cell segment of J consecutive
+// cell rows and marches over K planes, keeping only the comparison bits
+// (field >= iso) of the previous point plane in registers, so every field
+// value is read once (plus one halo row per J rows and one halo plane per K
+// planes).  Each lane reads 4 consecutive values with one vector load; the
+// fifth (x+4) comes from the next lane.  Active cells (case not 0/255) of a
+// (row, segment) are written in ascending x using wave ballots.  For F32
+// fields (double)f >= iso is evaluated as f >= iso_up, iso_up the smallest
+// float not below iso: the same predicate.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_classify(const T* __restrict__ field, Dims d, T iso_t,
-                                                      uint32_t* __restrict__ rowCount,
-                                                      uint32_t* __restrict__ rowData) {
-    __shared__ uint32_t sm[kBlock / kWave + 1];
-    for (int64_t r = blockIdx.x; r < d.R; r += gridDim.x) {
-        const int j = (int)(r % (d.ny - 1));
-        const int k = (int)(r / (d.ny - 1));
-        const int64_t sz = (int64_t)d.nx * d.ny;
-        const T* p00 = field + (int64_t)k * sz + (int64_t)j * d.nx;
-        const T* p10 = p00 + d.nx;
-        const T* p01 = p00 + sz;
-        const T* p11 = p01 + d.nx;
-        uint32_t* out = rowData + r * d.C;
-        uint32_t base = 0;
-        for (int i0 = 0; i0 < d.nx - 1; i0 += kBlock * 4) {
-            const int i = i0 + threadIdx.x * 4;
-            unsigned b00 = 0, b10 = 0, b01 = 0, b11 = 0;
-#pragma unroll
-            for (int q = 0; q < 5; q++) {
-                if (i + q < d.nx) {
-                    b00 |= (unsigned)at_or_above(p00[i + q], iso_t) << q;
-                    b10 |= (unsigned)at_or_above(p10[i + q], iso_t) << q;
-                    b01 |= (unsigned)at_or_above(p01[i + q], iso_t) << q;
-                    b11 |= (unsigned)at_or_above(p11[i + q], iso_t) << q;
-                }
+constexpr int kSeg = 256;           // cells per segment (one wave)
+constexpr int kSweepRows = 16;      // J
+
+template <class T, bool VEC>
+__device__ __forceinline__ uint32_t row_bits(const T* __restrict__ field, const Dims& d, int i0, int j, int k,
+                                             T iso_t, int lane) {
+    const T* row = field + ((int64_t)k * d.ny + j) * d.nx;
+    uint32_t b = 0;
+    if (VEC) {
+        if (i0 < d.nx) {
+            if constexpr (sizeof(T) == 4) {
+                const float4 v = *reinterpret_cast<const float4*>(row + i0);
+                b = (uint32_t)at_or_above<T>(v.x, iso_t) | (uint32_t)at_or_above<T>(v.y, iso_t) << 1 |
+                    (uint32_t)at_or_above<T>(v.z, iso_t) << 2 | (uint32_t)at_or_above<T>(v.w, iso_t) << 3;
+            } else {
+                const double2 v0 = *reinterpret_cast<const double2*>(row + i0);
+                const double2 v1 = *reinterpret_cast<const double2*>(row + i0 + 2);
+                b = (uint32_t)at_or_above<T>(v0.x, iso_t) | (uint32_t)at_or_above<T>(v0.y, iso_t) << 1 |
+                    (uint32_t)at_or_above<T>(v1.x, iso_t) << 2 | (uint32_t)at_or_above<T>(v1.y, iso_t) << 3;
             }
-            unsigned cases[4];
-            uint32_t cnt = 0;
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const unsigned ic = ((b00 >> s) & 1) | ((b00 >> (s + 1)) & 1) << 1 | ((b10 >> s) & 1) << 2 |
-                                    ((b10 >> (s + 1)) & 1) << 3 | ((b01 >> s) & 1) << 4 |
-                                    ((b01 >> (s + 1)) & 1) << 5 | ((b11 >> s) & 1) << 6 |
-                                    ((b11 >> (s + 1)) & 1) << 7;
-                const bool act = (i + s < d.nx - 1) && ic != 0 && ic != 255;
-                cases[s] = act ? ic : 256u;
-                cnt += act;
-            }
-            uint32_t tot;
-            uint32_t pos = base + block_exclusive_scan<uint32_t, kBlock>(cnt, sm, tot);
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                if (cases[s] < 256u) out[pos++] = (uint32_t)(i + s) | (cases[s] << 24);
-            }
-            base += tot;
         }
-        if (threadIdx.x == 0) rowCount[r] = base;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (i0 + q < d.nx) b |= (uint32_t)at_or_above<T>(row[i0 + q], iso_t) << q;
     }
+    // point x+4: first bit of the next lane; lane 63 reads it itself
+    uint32_t nb = (uint32_t)__shfl_down((int)(b & 1u), 1, kWave);
+    if (lane == kWave - 1) nb = (i0 + 4 < d.nx) ? (uint32_t)at_or_above<T>(row[i0 + 4], iso_t) : 0u;
+    return b | (nb << 4);
 }
 
-// ---------------------------------------------------------------------------
-// K2: gather the row lists into the flat active-cell arrays.  One wave per row.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_compact_rows(const uint32_t* __restrict__ rowCount,
-                                                          const uint32_t* __restrict__ rowBase,
-                                                          const uint32_t* __restrict__ rowData, Dims d,
-                                                          uint32_t* __restrict__ acell,
-                                                          uint8_t* __restrict__ acase) {
+template <class T, bool VEC>
+__global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, Dims d, T iso_t, int S, int ntj,
+                                                   int K, int ntk, uint32_t* __restrict__ lists,
+                                                   uint32_t* __restrict__ segCount) {
+    constexpr int J = kSweepRows;
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wpb = kBlock / kWave;
-    for (int64_t r = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; r < d.R; r += (int64_t)gridDim.x * wpb) {
-        const uint32_t cnt = rowCount[r], base = rowBase[r];
-        const uint32_t* src = rowData + r * d.C;
-        for (uint32_t m = lane; m < cnt; m += kWave) {
-            const uint32_t x = src[m];
-            acell[base + m] = (uint32_t)(r * d.C) + (x & 0xFFFFFFu);
-            acase[base + m] = (uint8_t)(x >> 24);
+    const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (wave >= (int64_t)S * ntj * ntk) return;
+    const int s = (int)(wave % S);
+    const int64_t rest = wave / S;
+    const int tj = (int)(rest % ntj), tk = (int)(rest / ntj);
+    const int j0 = tj * J, k0 = tk * K;
+    const int nj = min(J, d.ny - 1 - j0);
+    const int k1 = min(k0 + K, d.nz - 1);
+    const int i0 = s * kSeg + lane * 4;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t prev[J + 1], cur[J + 1];
+#pragma unroll
+    for (int jr = 0; jr <= J; jr++)
+        prev[jr] = jr <= nj ? row_bits<T, VEC>(field, d, i0, j0 + jr, k0, iso_t, lane) : 0u;
+    for (int k = k0; k < k1; k++) {
+#pragma unroll
+        for (int jr = 0; jr <= J; jr++)
+            cur[jr] = jr <= nj ? row_bits<T, VEC>(field, d, i0, j0 + jr, k + 1, iso_t, lane) : 0u;
+#pragma unroll
+        for (int jr = 0; jr < J; jr++) {
+            if (jr < nj) {
+                const uint32_t A = prev[jr], B = prev[jr + 1], Cm = cur[jr], D = cur[jr + 1];
+                uint32_t cs[4];
+                uint64_t bal[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t ic = ((A >> q) & 1u) | ((A >> (q + 1)) & 1u) << 1 | ((B >> q) & 1u) << 2 |
+                                        ((B >> (q + 1)) & 1u) << 3 | ((Cm >> q) & 1u) << 4 |
+                                        ((Cm >> (q + 1)) & 1u) << 5 | ((D >> q) & 1u) << 6 |
+                                        ((D >> (q + 1)) & 1u) << 7;
+                    const bool act = (i0 + q < d.nx - 1) && ic != 0u && ic != 255u;
+                    cs[q] = act ? ic : 256u;
+                    bal[q] = __ballot(act);
+                }
+                const int64_t r = (int64_t)k * (d.ny - 1) + (j0 + jr);
+                const uint32_t total =
+                    (uint32_t)(__popcll(bal[0]) + __popcll(bal[1]) + __popcll(bal[2]) + __popcll(bal[3]));
+                if (total) {
+                    uint32_t pos = (uint32_t)(__popcll(bal[0] & lt) + __popcll(bal[1] & lt) + __popcll(bal[2] & lt) +
+                                              __popcll(bal[3] & lt));
+                    uint32_t* out = lists + (r * S + s) * kSeg;
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if (cs[q] < 256u) out[pos++] = (uint32_t)(i0 + q) | (cs[q] << 24);
+                }
+                if (lane == 0) segCount[r * S + s] = total;
+            }
         }
+#pragma unroll
+        for (int jr = 0; jr <= J; jr++) prev[jr] = cur[jr];
     }
 }
 
@@ -117,8 +147,8 @@ __device__ __forceinline__ void load_corners(const T* __restrict__ field, const 
     F[7] = (double)p[sz + d.nx + 1];
 }
 
-__device__ __forceinline__ void cell_ijk(uint32_t c, const Dims& d, int& i, int& j, int& k) {
-    const uint32_t r = c / (uint32_t)d.C;
+__device__ __forceinline__ void cell_ijk(uint32_t c, const Dims& d, int& i, int& j, int& k, uint32_t& r) {
+    r = c / (uint32_t)d.C;
     i = (int)(c - r * (uint32_t)d.C);
     j = (int)(r % (uint32_t)(d.ny - 1));
     k = (int)(r / (uint32_t)(d.ny - 1));
@@ -136,52 +166,89 @@ __device__ __forceinline__ unsigned used_contours(unsigned icase, uint32_t map, 
 }
 
 // ---------------------------------------------------------------------------
-// K3: polygons of every active cell -> code (icase | edge-to-contour map),
-// number of kept vertices and of owned quadrilaterals.
+// K2: one wave per row, cells in ascending x: polygons of every active cell
+// -> code, kept vertices and owned quadrilaterals (edges e0, e3, e8 of the
+// cell with all four neighbour cells present) and their prefix inside the
+// row; the flat active arrays; row totals for the row scans.  Counts every
+// contour for the reference's pre-removal vertex count.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_cell_info(const T* __restrict__ field, Dims d, double iso,
-                                                       const uint32_t* __restrict__ acell,
-                                                       const uint8_t* __restrict__ acase, int64_t A,
-                                                       uint32_t* __restrict__ acode, uint32_t* __restrict__ anv,
-                                                       uint32_t* __restrict__ anq,
-                                                       unsigned long long* __restrict__ ncontours) {
+__global__ void __launch_bounds__(kBlock) k_rows(const T* __restrict__ field, Dims d, double iso, int S,
+                                                  const uint32_t* __restrict__ lists,
+                                                  const uint32_t* __restrict__ segCount,
+                                                  const uint32_t* __restrict__ segBase,
+                                                  uint32_t* __restrict__ acell, uint32_t* __restrict__ acode,
+                                                  uint32_t* __restrict__ avloc, uint32_t* __restrict__ aqloc,
+                                                  uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowQ,
+                                                  unsigned long long* __restrict__ ncontours) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wpb = kBlock / kWave;
     unsigned long long total = 0;
-    for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
-        int i, j, k;
-        cell_ijk(acell[a], d, i, j, k);
-        const unsigned icase = acase[a];
-        uint64_t w;
-        if (case_ambiguous(icase)) {
-            double F[8];
-            load_corners(field, d, i, j, k, F);
-            w = polygons_decider(iso, F);
-        } else {
-            w = c_case_contours[icase];
-        }
-        uint32_t map = 0;
-        int pos = 0;
+    for (int64_t r = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; r < d.R; r += (int64_t)gridDim.x * wpb) {
+        const int j = (int)(r % (d.ny - 1));
+        const int k = (int)(r / (d.ny - 1));
+        uint32_t cv = 0, cq = 0;
+        // the row's active cells are contiguous in the flat order: process
+        // them 64 at a time across segment boundaries
+        const int64_t rs0 = r * S;
+        const uint32_t first = segBase[rs0];
+        const uint32_t last = segBase[rs0 + S - 1] + segCount[rs0 + S - 1];
+        for (uint32_t a0 = first; a0 < last; a0 += kWave) {
+            const uint32_t a = a0 + lane;
+            const bool act = a < last;
+            uint32_t nv = 0, nq = 0, code = 0;
+            int i = 0;
+            if (act) {
+                // segment of a: the last s with segBase[rs0 + s] <= a
+                int lo = 0, hi = S - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (segBase[rs0 + mid] <= a) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t x = lists[(rs0 + lo) * kSeg + (a - segBase[rs0 + lo])];
+                i = (int)(x & 0xFFFFFFu);
+                const unsigned icase = x >> 24;
+                uint64_t w;
+                if (case_ambiguous(icase)) {
+                    double F[8];
+                    load_corners(field, d, i, j, k, F);
+                    w = polygons_decider(iso, F);
+                } else {
+                    w = c_case_contours[icase];
+                }
+                uint32_t map = 0;
+                int pos = 0;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int s = poly_size(w, t);
-            for (int m = 0; m < s; m++) {
-                const int e = poly_edge_at(w, pos + m);
-                map |= (uint32_t)t << (2 * e);
+                for (int t = 0; t < 4; t++) {
+                    const int sz = poly_size(w, t);
+                    for (int q = 0; q < sz; q++) map |= (uint32_t)t << (2 * poly_edge_at(w, pos + q));
+                    pos += sz;
+                }
+                total += (unsigned long long)poly_count(w);
+                nv = (uint32_t)__popc(used_contours(icase, map, i, j, k, d));
+                nq = (uint32_t)(edge_cut(icase, 0) && j >= 1 && k >= 1) +
+                     (uint32_t)(edge_cut(icase, 3) && i >= 1 && k >= 1) +
+                     (uint32_t)(edge_cut(icase, 8) && i >= 1 && j >= 1);
+                code = icase | (map << 8);
             }
-            pos += s;
+            const uint32_t iv = wave_inclusive_scan(nv), iq = wave_inclusive_scan(nq);
+            if (act) {
+                acell[a] = (uint32_t)(r * d.C) + (uint32_t)i;
+                acode[a] = code;
+                avloc[a] = cv + iv - nv;
+                aqloc[a] = cq + iq - nq;
+            }
+            cv += (uint32_t)__shfl((int)iv, kWave - 1, kWave);
+            cq += (uint32_t)__shfl((int)iq, kWave - 1, kWave);
         }
-        const unsigned used = used_contours(icase, map, i, j, k, d);
-        total += poly_count(w);
-        acode[a] = icase | (map << 8);
-        anv[a] = __popc(used);
-        uint32_t nq = 0;
-        nq += (edge_cut(icase, 0) && j >= 1 && k >= 1);
-        nq += (edge_cut(icase, 3) && i >= 1 && k >= 1);
-        nq += (edge_cut(icase, 8) && i >= 1 && j >= 1);
-        anq[a] = nq;
+        if (lane == 0) {
+            rowV[r] = cv;
+            rowQ[r] = cq;
+        }
     }
     total = wave_sum(total);
-    if ((threadIdx.x & (kWave - 1)) == 0 && total) atomicAdd(ncontours, total);
+    if (lane == 0 && total) atomicAdd(ncontours, total);
 }
 
 // Gradient of UniformGrid::estimateGradient followed by flip_gradient
@@ -206,10 +273,11 @@ __device__ __forceinline__ void point_gradient(const T* __restrict__ field, cons
 }
 
 // ---------------------------------------------------------------------------
-// K4: vertex representatives, normals and positions
+// K3: vertex representatives, normals and positions
 // (cpp_dmc/DualMarchingCubes.cpp:141-171, UniformGrid.h:286-305).
-// One thread per active cell; vertices of used contours are written at
-// avbase[a] + rank.  Failed projections are counted for every contour.
+// One thread per active cell; vertices of used contours are written from
+// rowBaseV[row] + avloc[a] on.  Failed projections are counted for every
+// contour, as the reference does.
 // ---------------------------------------------------------------------------
 struct GridGeom {
     double x0, y0, z0, dx, dy, dz;
@@ -220,59 +288,72 @@ __global__ void __launch_bounds__(kBlock) k_vertices(const T* __restrict__ field
                                                       Dims d, GridGeom geo, double iso,
                                                       const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
-                                                      const uint32_t* __restrict__ avbase, int64_t A,
+                                                      const uint32_t* __restrict__ avloc,
+                                                      const uint32_t* __restrict__ rowBaseV, int64_t A,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
                                                       uint32_t* __restrict__ fails) {
+    static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
+    __shared__ float boxes[kBoxFloats * kBoxStride];
+    __shared__ double corners[8 * kCornerStride];
+    const BoxRef B{boxes + threadIdx.x};
+    const CornerRef Fc{corners + threadIdx.x};
     uint32_t f1 = 0, f2 = 0;
     const double dx2 = 2 * geo.dx, dy2 = 2 * geo.dy, dz2 = 2 * geo.dz;
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         int i, j, k;
-        cell_ijk(acell[a], d, i, j, k);
+        uint32_t r;
+        cell_ijk(acell[a], d, i, j, k, r);
         const uint32_t code = acode[a];
         const unsigned icase = code & 0xFFu;
         double F[8];
         load_corners(field, d, i, j, k, F);
+        Fc.store(F);
         const uint64_t w = cell_polygons(icase, iso, F);
         const int cnt = poly_count(w);
-        BBoxes B;
-        float lpar[12];
-        cell_boxes(F, iso, icase, w, cnt, B, lpar);
+        cell_boxes(Fc, iso, w, cnt, B);
         const unsigned used = used_contours(icase, code >> 8, i, j, k, d);
-        uint32_t out = avbase[a];
-        bool have_n = false;
-        double G[8][3];
+        uint32_t out = rowBaseV[r] + avloc[a];
         for (int t = 0; t < cnt; t++) {
             double p[3];
-            const int fl = contour_representative(F, iso, w, cnt, t, B, lpar, p);
+            const int fl = contour_representative(F, Fc, iso, w, cnt, t, B, p);
             f1 += fl >= 1;
             f2 += fl >= 2;
             if (!((used >> t) & 1u)) continue;
-            if (!have_n) {
+            // normal: trilinear blend of the corner gradients, normalised
+            const double u = p[0], v = p[1], ww = p[2];
+            double n[3] = {0, 0, 0}, lo[3], hi[3], a0[3], a1[3];
+#pragma unroll 1
+            for (int c = 0; c < 8; c++) {
+                const int ci = i + (c & 1), cj = j + ((c >> 1) & 1), ck = k + ((c >> 2) & 1);
+                double G[3];
+                if (grads) {
+                    const int64_t q = ((int64_t)ck * d.ny + cj) * d.nx + ci;
+                    G[0] = grads[3 * q];
+                    G[1] = grads[3 * q + 1];
+                    G[2] = grads[3 * q + 2];
+                } else {
+                    point_gradient(field, d, ci, cj, ck, dx2, dy2, dz2, G);
+                }
+                // (1-u) n_even + u n_odd, then rows by v, then planes by w
 #pragma unroll
-                for (int c = 0; c < 8; c++) {
-                    const int ci = i + (c & 1), cj = j + ((c >> 1) & 1), ck = k + ((c >> 2) & 1);
-                    if (grads) {
-                        const int64_t q = ((int64_t)ck * d.ny + cj) * d.nx + ci;
-                        G[c][0] = grads[3 * q];
-                        G[c][1] = grads[3 * q + 1];
-                        G[c][2] = grads[3 * q + 2];
-                    } else {
-                        point_gradient(field, d, ci, cj, ck, dx2, dy2, dz2, G[c]);
+                for (int q = 0; q < 3; q++) {
+                    if ((c & 1) == 0) a0[q] = G[q] * (1 - u);
+                    else a1[q] = a0[q] + G[q] * u;
+                }
+                if (c & 1) {
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        if ((c & 2) == 0) lo[q] = a1[q] * (1 - v);
+                        else hi[q] = lo[q] + a1[q] * v;
+                    }
+                    if (c & 2) {
+#pragma unroll
+                        for (int q = 0; q < 3; q++) {
+                            if ((c & 4) == 0) n[q] = hi[q] * (1 - ww);
+                            else n[q] = n[q] + hi[q] * ww;
+                        }
                     }
                 }
-                have_n = true;
-            }
-            const double u = p[0], v = p[1], ww = p[2];
-            double n[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double a0 = G[0][c] * (1 - u) + G[1][c] * u;
-                const double a1 = G[2][c] * (1 - u) + G[3][c] * u;
-                const double a2 = G[4][c] * (1 - u) + G[5][c] * u;
-                const double a3 = G[6][c] * (1 - u) + G[7][c] * u;
-                const double lo = a0 * (1 - v) + a1 * v;
-                const double hi = a2 * (1 - v) + a3 * v;
-                n[c] = lo * (1 - ww) + hi * ww;
             }
             const double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
             n[0] /= s;
@@ -295,16 +376,20 @@ __global__ void __launch_bounds__(kBlock) k_vertices(const T* __restrict__ field
     }
 }
 
-// Position of cell (i, j, k) in the active arrays: binary search in its
-// row's list.  The cell is active whenever one of its edges is cut; -1 is
-// returned (and counted by the caller) if it is not found.
-__device__ __forceinline__ int64_t find_active(const uint32_t* __restrict__ rowData,
-                                               const uint32_t* __restrict__ rowCount,
-                                               const uint32_t* __restrict__ rowBase, const Dims& d, int i, int j,
-                                               int k) {
-    const int64_t r = (int64_t)k * (d.ny - 1) + j;
-    const uint32_t* list = rowData + r * d.C;
-    const uint32_t n = rowCount[r];
+// Position of cell (i, j, k) in the active arrays: binary search in the
+// list of its (row, segment).  The cell is active whenever one of its edges
+// is cut; -1 is returned (and counted by the caller) if it is not found.
+struct SegLists {
+    const uint32_t* lists;
+    const uint32_t* count;
+    const uint32_t* base;
+    int S;
+};
+
+__device__ __forceinline__ int64_t find_active(const SegLists& L, const Dims& d, int i, int j, int k) {
+    const int64_t rs = ((int64_t)k * (d.ny - 1) + j) * L.S + (i / kSeg);
+    const uint32_t* list = L.lists + rs * kSeg;
+    const uint32_t n = L.count[rs];
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -312,23 +397,26 @@ __device__ __forceinline__ int64_t find_active(const uint32_t* __restrict__ rowD
         else hi = mid;
     }
     if (lo >= n || (list[lo] & 0xFFFFFFu) != (uint32_t)i) return -1;
-    return (int64_t)rowBase[r] + lo;
+    return (int64_t)L.base[rs] + lo;
 }
 
-// Final id of the vertex of cell `a` whose contour contains edge e.
-__device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode, const uint32_t* __restrict__ avbase,
-                                             int64_t a, int e, int i, int j, int k, const Dims& d) {
+// Final id of the vertex of active cell `a` = (i, j, k) whose contour
+// contains edge e.
+__device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode, const uint32_t* __restrict__ avloc,
+                                             const uint32_t* __restrict__ rowBaseV, int64_t a, int e, int i, int j,
+                                             int k, const Dims& d) {
     if (a < 0) return -1;
     const uint32_t code = acode[a];
     const unsigned icase = code & 0xFFu;
     const uint32_t map = code >> 8;
     const unsigned t = (map >> (2 * e)) & 3u;
     const unsigned used = used_contours(icase, map, i, j, k, d);
-    return (int32_t)(avbase[a] + __popc(used & ((1u << t) - 1u)));
+    const uint32_t vb = rowBaseV[(int64_t)k * (d.ny - 1) + j] + avloc[a];
+    return (int32_t)(vb + __popc(used & ((1u << t) - 1u)));
 }
 
 // ---------------------------------------------------------------------------
-// K5: quadrilaterals.  Each complete quadrilateral belongs to the grid edge
+// K4: quadrilaterals.  Each complete quadrilateral belongs to the grid edge
 // e0 (x), e3 (y) or e8 (z) of the cell at its lower corner; ascending
 // (cell, direction) is the reference's map order
 // (cpp_dmc/DualMarchingCubes.h:215-223, :663-676).  Colour: 3*parity + dir
@@ -336,60 +424,62 @@ __device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_quads(Dims d, const uint32_t* __restrict__ acell,
                                                    const uint32_t* __restrict__ acode,
-                                                   const uint32_t* __restrict__ avbase,
-                                                   const uint32_t* __restrict__ aqbase, int64_t A,
-                                                   const uint32_t* __restrict__ rowData,
-                                                   const uint32_t* __restrict__ rowCount,
-                                                   const uint32_t* __restrict__ rowBase,
+                                                   const uint32_t* __restrict__ avloc,
+                                                   const uint32_t* __restrict__ aqloc,
+                                                   const uint32_t* __restrict__ rowBaseV,
+                                                   const uint32_t* __restrict__ rowBaseQ, int64_t A, SegLists L,
                                                    int32_t* __restrict__ quads, int32_t* __restrict__ colors,
                                                    uint32_t* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         int i, j, k;
-        cell_ijk(acell[a], d, i, j, k);
+        uint32_t r;
+        cell_ijk(acell[a], d, i, j, k, r);
         const unsigned icase = acode[a] & 0xFFu;
         const int bit = (icase & 1u) ? 0 : 1;     // field at the edge origin below iso
         const int parity = (i & 1) | (j & 1) << 1 | (k & 1) << 2;
-        int64_t q = aqbase[a];
+        int64_t q = (int64_t)rowBaseQ[r] + aqloc[a];
+        const bool ex = edge_cut(icase, 0) && j >= 1 && k >= 1;
+        const bool ey = edge_cut(icase, 3) && i >= 1 && k >= 1;
+        const bool ez = edge_cut(icase, 8) && i >= 1 && j >= 1;
+        if (!(ex || ey || ez)) continue;
+        const int32_t own0 = vertex_of(acode, avloc, rowBaseV, a, 0, i, j, k, d);
+        const int32_t own3 = vertex_of(acode, avloc, rowBaseV, a, 3, i, j, k, d);
+        const int32_t own8 = vertex_of(acode, avloc, rowBaseV, a, 8, i, j, k, d);
+        // cells of the previous row (j-1) and plane (k-1) found once
+        const int64_t cjm = (ex || ez) ? find_active(L, d, i, j - 1, k) : -1;
+        const int64_t ckm = (ex || ey) ? find_active(L, d, i, j, k - 1) : -1;
+        int32_t out[4];
         // x edge: cells (i,j,k) e0, (i,j-1,k) e2, (i,j,k-1) e4, (i,j-1,k-1) e6
-        if (edge_cut(icase, 0) && j >= 1 && k >= 1) {
-            int32_t out[4];
-            const int64_t b = find_active(rowData, rowCount, rowBase, d, i, j - 1, k);
-            const int64_t c = find_active(rowData, rowCount, rowBase, d, i, j, k - 1);
-            const int64_t e = find_active(rowData, rowCount, rowBase, d, i, j - 1, k - 1);
-            out[edge_slot(0, bit)] = vertex_of(acode, avbase, a, 0, i, j, k, d);
-            out[edge_slot(2, bit)] = vertex_of(acode, avbase, b, 2, i, j - 1, k, d);
-            out[edge_slot(4, bit)] = vertex_of(acode, avbase, c, 4, i, j, k - 1, d);
-            out[edge_slot(6, bit)] = vertex_of(acode, avbase, e, 6, i, j - 1, k - 1, d);
+        if (ex) {
+            const int64_t e = find_active(L, d, i, j - 1, k - 1);
+            out[edge_slot(0, bit)] = own0;
+            out[edge_slot(2, bit)] = vertex_of(acode, avloc, rowBaseV, cjm, 2, i, j - 1, k, d);
+            out[edge_slot(4, bit)] = vertex_of(acode, avloc, rowBaseV, ckm, 4, i, j, k - 1, d);
+            out[edge_slot(6, bit)] = vertex_of(acode, avloc, rowBaseV, e, 6, i, j - 1, k - 1, d);
             if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) atomicAdd(unresolved, 1u);
             reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
             colors[q] = 3 * parity;
             q++;
         }
         // y edge: cells (i,j,k) e3, (i-1,j,k) e1, (i,j,k-1) e7, (i-1,j,k-1) e5
-        if (edge_cut(icase, 3) && i >= 1 && k >= 1) {
-            int32_t out[4];
-            const int64_t b = a - 1;
-            const int64_t c = find_active(rowData, rowCount, rowBase, d, i, j, k - 1);
-            const int64_t e = c > 0 ? c - 1 : -1;
-            out[edge_slot(3, bit)] = vertex_of(acode, avbase, a, 3, i, j, k, d);
-            out[edge_slot(1, bit)] = vertex_of(acode, avbase, b, 1, i - 1, j, k, d);
-            out[edge_slot(7, bit)] = vertex_of(acode, avbase, c, 7, i, j, k - 1, d);
-            out[edge_slot(5, bit)] = vertex_of(acode, avbase, e, 5, i - 1, j, k - 1, d);
+        if (ey) {
+            const int64_t e = ckm > 0 ? ckm - 1 : -1;
+            out[edge_slot(3, bit)] = own3;
+            out[edge_slot(1, bit)] = vertex_of(acode, avloc, rowBaseV, a - 1, 1, i - 1, j, k, d);
+            out[edge_slot(7, bit)] = vertex_of(acode, avloc, rowBaseV, ckm, 7, i, j, k - 1, d);
+            out[edge_slot(5, bit)] = vertex_of(acode, avloc, rowBaseV, e, 5, i - 1, j, k - 1, d);
             if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) atomicAdd(unresolved, 1u);
             reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
             colors[q] = 3 * parity + 1;
             q++;
         }
         // z edge: cells (i,j,k) e8, (i-1,j,k) e9, (i,j-1,k) e11, (i-1,j-1,k) e10
-        if (edge_cut(icase, 8) && i >= 1 && j >= 1) {
-            int32_t out[4];
-            const int64_t b = a - 1;
-            const int64_t c = find_active(rowData, rowCount, rowBase, d, i, j - 1, k);
-            const int64_t e = c > 0 ? c - 1 : -1;
-            out[edge_slot(8, bit)] = vertex_of(acode, avbase, a, 8, i, j, k, d);
-            out[edge_slot(9, bit)] = vertex_of(acode, avbase, b, 9, i - 1, j, k, d);
-            out[edge_slot(11, bit)] = vertex_of(acode, avbase, c, 11, i, j - 1, k, d);
-            out[edge_slot(10, bit)] = vertex_of(acode, avbase, e, 10, i - 1, j - 1, k, d);
+        if (ez) {
+            const int64_t e = cjm > 0 ? cjm - 1 : -1;
+            out[edge_slot(8, bit)] = own8;
+            out[edge_slot(9, bit)] = vertex_of(acode, avloc, rowBaseV, a - 1, 9, i - 1, j, k, d);
+            out[edge_slot(11, bit)] = vertex_of(acode, avloc, rowBaseV, cjm, 11, i, j - 1, k, d);
+            out[edge_slot(10, bit)] = vertex_of(acode, avloc, rowBaseV, e, 10, i - 1, j - 1, k, d);
             if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) atomicAdd(unresolved, 1u);
             reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
             colors[q] = 3 * parity + 2;

@@ -3,8 +3,8 @@
 //
 // Halfedge h = 4q + k runs from quads[4q+k] to quads[4q+(k+1)%4]; face and
 // next are implicit, twins and flags are stored (4 B + 1 B per halfedge).
-// Twins are found through a per-vertex CSR of outgoing halfedges instead of
-// the reference's map keyed by the vertex pair
+// Twins are found through a per-vertex CSR of outgoing (halfedge,
+// destination) pairs instead of the reference's map keyed by the vertex pair
 // (cpp_dmc/DualMarchingCubes.cpp:488-671): the halfedges sharing an edge
 // {a, b} are the out-halfedges of a ending at b plus those of b ending at a,
 // taken in ascending halfedge order, which is the map's insertion order.
@@ -32,76 +32,94 @@ __global__ void __launch_bounds__(kBlock) k_he_degree(const int32_t* __restrict_
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_he_scatter(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+// Scatter (halfedge, destination) pairs into the CSR rows; the order
+// inside a row is arbitrary (the group search sorts what it needs).
+__global__ void __launch_bounds__(kBlock) k_he_scatter(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
                                                         const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-                                                        int32_t* __restrict__ adj) {
-    TMC_GRID_STRIDE(h, nhe) {
-        const int32_t v = quads[h];
-        if (!vid_ok(v, nv)) continue;
-        const uint32_t p = off[v] + atomicAdd(&cursor[v], 1u);
-        adj[p] = (int32_t)h;
-    }
-}
-
-// Sort each vertex's outgoing halfedges ascending (insertion sort; buckets
-// are small).
-__global__ void __launch_bounds__(kBlock) k_he_sort(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                     int64_t nv, int32_t* __restrict__ adj) {
-    TMC_GRID_STRIDE(v, nv) {
-        int32_t* b = adj + off[v];
-        const int n = (int)deg[v];
-        for (int x = 1; x < n; x++) {
-            const int32_t key = b[x];
-            int y = x - 1;
-            while (y >= 0 && b[y] > key) {
-                b[y + 1] = b[y];
-                y--;
-            }
-            b[y + 1] = key;
+                                                        int2* __restrict__ adj) {
+    TMC_GRID_STRIDE(f, nq) {
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        const int32_t v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int32_t a = v[k];
+            if (!vid_ok(a, nv)) continue;
+            const uint32_t p = off[a] + atomicAdd(&cursor[a], 1u);
+            adj[p] = make_int2((int32_t)(4 * f + k), v[(k + 1) & 3]);
         }
     }
 }
 
 __device__ __forceinline__ int32_t he_dest(const int32_t* __restrict__ quads, int32_t h) { return quads[he_next(h)]; }
 
-// Members of the edge group of halfedge h in ascending order (up to 5 kept,
-// the true count returned).
+// Keep the 5 smallest members in ascending order (unrolled insertion).
+__device__ __forceinline__ void group_insert(int32_t g[5], int32_t h) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+        if (h < g[p]) {
+            const int32_t t = g[p];
+            g[p] = h;
+            h = t;
+        }
+    }
+}
+
+// Members of the edge group of halfedge h = (a -> b): out-halfedges of a
+// ending at b and of b ending at a, in ascending order (5 smallest kept);
+// the true count is returned.
 __device__ __forceinline__ int he_group(const int32_t* __restrict__ quads, const uint32_t* __restrict__ off,
-                                        const uint32_t* __restrict__ deg, const int32_t* __restrict__ adj, int32_t h,
+                                        const uint32_t* __restrict__ deg, const int2* __restrict__ adj, int32_t h,
                                         int64_t nv, int32_t g[5]) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) g[p] = 0x7FFFFFFF;
     const int32_t a = quads[h], b = he_dest(quads, h);
     if (!vid_ok(a, nv) || !vid_ok(b, nv)) return 0;
-    int32_t l1[8], l2[8];
-    int n1 = 0, n2 = 0, c1 = 0, c2 = 0;
+    int c = 0;
     {
-        const int32_t* p = adj + off[a];
+        const int2* p = adj + off[a];
         const int n = (int)deg[a];
         for (int x = 0; x < n; x++) {
-            const int32_t o = p[x];
-            if (he_dest(quads, o) == b) {
-                if (n1 < 8) l1[n1++] = o;
-                c1++;
+            const int2 e = p[x];
+            if (e.y == b) {
+                group_insert(g, e.x);
+                c++;
             }
         }
     }
     if (b != a) {
-        const int32_t* p = adj + off[b];
+        const int2* p = adj + off[b];
         const int n = (int)deg[b];
         for (int x = 0; x < n; x++) {
-            const int32_t o = p[x];
-            if (he_dest(quads, o) == a) {
-                if (n2 < 8) l2[n2++] = o;
-                c2++;
+            const int2 e = p[x];
+            if (e.y == a) {
+                group_insert(g, e.x);
+                c++;
             }
         }
     }
-    // merge the two ascending lists
-    int x = 0, y = 0, m = 0;
-    while ((x < n1 || y < n2) && m < 5) {
-        if (y >= n2 || (x < n1 && l1[x] < l2[y])) g[m++] = l1[x++];
-        else g[m++] = l2[y++];
+    return c;
+}
+
+// Twin and flag from a sorted group (cpp_dmc/DualMarchingCubes.cpp:574-669).
+__device__ __forceinline__ void twin_rule(const int32_t* __restrict__ quads, int c, const int32_t g[5], int32_t h,
+                                          int32_t& tw, uint8_t& fl) {
+    tw = -1;
+    fl = 0;
+    if (c == 2) {
+        tw = (g[0] == h) ? g[1] : g[0];
+    } else if (c == 3) {
+        fl = 2;
+        const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
+        int32_t pa, pb;
+        if (v0 != v1) { pa = g[0]; pb = g[1]; }
+        else if (v0 != v2) { pa = g[0]; pb = g[2]; }
+        else { pa = g[1]; pb = g[2]; }
+        tw = (h == pa) ? pb : (h == pb) ? pa : -1;
+    } else if (c == 4) {
+        fl = 1;
+        if (quads[g[0]] != quads[g[1]]) tw = (h == g[0]) ? g[1] : (h == g[1]) ? g[0] : (h == g[2]) ? g[3] : g[2];
+        else tw = (h == g[0]) ? g[2] : (h == g[2]) ? g[0] : (h == g[1]) ? g[3] : g[1];
     }
-    return c1 + c2;
 }
 
 // Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669).
@@ -117,32 +135,15 @@ struct HeStats {
 
 __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
                                                       const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                      const int32_t* __restrict__ adj, int32_t* __restrict__ twin,
+                                                      const int2* __restrict__ adj, int32_t* __restrict__ twin,
                                                       uint8_t* __restrict__ flag, HeStats* __restrict__ st) {
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         int32_t g[5];
         const int c = he_group(quads, off, deg, adj, h, nv, g);
-        int32_t tw = -1;
-        uint8_t fl = 0;
-        if (c == 2) {
-            tw = (g[0] == h) ? g[1] : g[0];
-        } else if (c == 3) {
-            fl = 2;
-            const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
-            int32_t pa, pb;
-            if (v0 != v1) { pa = g[0]; pb = g[1]; }
-            else if (v0 != v2) { pa = g[0]; pb = g[2]; }
-            else { pa = g[1]; pb = g[2]; }
-            tw = (h == pa) ? pb : (h == pb) ? pa : -1;
-        } else if (c == 4) {
-            fl = 1;
-            if (quads[g[0]] != quads[g[1]]) {
-                tw = (h == g[0]) ? g[1] : (h == g[1]) ? g[0] : (h == g[2]) ? g[3] : g[2];
-            } else {
-                tw = (h == g[0]) ? g[2] : (h == g[2]) ? g[0] : (h == g[1]) ? g[3] : g[1];
-            }
-        }
+        int32_t tw;
+        uint8_t fl;
+        twin_rule(quads, c, g, h, tw, fl);
         twin[h] = tw;
         flag[h] = fl;
         if (c > 0 && g[0] == h) {
@@ -156,11 +157,14 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
 // he_v: the last halfedge (in face order) leaving each vertex
 // (cpp_dmc/DualMarchingCubes.cpp:561-564).
 __global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                  const int32_t* __restrict__ adj, int64_t nv,
+                                                  const int2* __restrict__ adj, int64_t nv,
                                                   int32_t* __restrict__ he_v) {
     TMC_GRID_STRIDE(v, nv) {
-        const uint32_t n = deg[v];
-        he_v[v] = n ? adj[off[v] + n - 1] : -1;
+        const int2* p = adj + off[v];
+        const int n = (int)deg[v];
+        int32_t last = -1;
+        for (int x = 0; x < n; x++) last = p[x].x > last ? p[x].x : last;
+        he_v[v] = last;
     }
 }
 
@@ -178,7 +182,7 @@ struct BndRec {
 
 __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
                                                       const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                      const int32_t* __restrict__ adj,
+                                                      const int2* __restrict__ adj,
                                                       unsigned long long* __restrict__ ovr, BndRec* __restrict__ recs,
                                                       unsigned int* __restrict__ nrec, int pass,
                                                       int32_t* __restrict__ he_v) {
@@ -210,14 +214,14 @@ __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__
 // (cpp_dmc/DualMarchingCubes.cpp:847-871).  A halfedge ends at v exactly
 // when it is the predecessor of one leaving v.
 __global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                     const int32_t* __restrict__ adj, const int32_t* __restrict__ twin,
+                                                     const int2* __restrict__ adj, const int32_t* __restrict__ twin,
                                                      int64_t nv, int32_t* __restrict__ val,
                                                      unsigned int* __restrict__ zero) {
     TMC_GRID_STRIDE(v, nv) {
-        const int32_t* p = adj + off[v];
+        const int2* p = adj + off[v];
         const int n = (int)deg[v];
         int s = 0;
-        for (int x = 0; x < n; x++) s += 1 + (twin[he_prev(p[x])] == -1);
+        for (int x = 0; x < n; x++) s += 1 + (twin[he_prev(p[x].x)] == -1);
         val[v] = s;
         if (s == 0) atomicAdd(zero, 1u);
     }
@@ -237,24 +241,66 @@ __device__ __forceinline__ void face_neighbors(const int32_t* __restrict__ twin,
 // the initially colour-4 faces try 0..3.  Faces of one initial colour are
 // never edge neighbours (their grid edges are parallel edges of different
 // parity), so each class is processed in parallel.
-__global__ void __launch_bounds__(kBlock) k_color_count(const int32_t* __restrict__ colors, int64_t nq,
-                                                         uint32_t* __restrict__ cnt) {
-    __shared__ uint32_t sm[32];
-    if (threadIdx.x < 32) sm[threadIdx.x] = 0;
+// Bucketing by initial colour without hot global counters: each block keeps
+// an LDS histogram of its grid-stride range, one block turns the histograms
+// into per-(block, colour) offsets, and the same blocks scatter again.  The
+// order inside a class does not matter (its faces are independent).
+constexpr int kColorBlocks = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_color_hist(const int32_t* __restrict__ colors, int64_t nq,
+                                                        uint32_t* __restrict__ hist) {
+    __shared__ uint32_t sm[24];
+    if (threadIdx.x < 24) sm[threadIdx.x] = 0;
     __syncthreads();
     TMC_GRID_STRIDE(f, nq) {
         const int32_t c = colors[f];
         if (c >= 0 && c < 24) atomicAdd(&sm[c], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < 24 && sm[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], sm[threadIdx.x]);
+    if (threadIdx.x < 24) hist[blockIdx.x * 24 + threadIdx.x] = sm[threadIdx.x];
+}
+
+// One block: totals per colour, class starts, and per-(block, colour)
+// offsets in place of the histogram.
+__global__ void __launch_bounds__(64) k_color_offsets(uint32_t* __restrict__ hist, int nblocks,
+                                                       uint32_t* __restrict__ totals) {
+    __shared__ uint32_t tot[24], start[24];
+    const int c = threadIdx.x;
+    if (c < 24) {
+        uint32_t t = 0;
+        for (int b = 0; b < nblocks; b++) t += hist[b * 24 + c];
+        tot[c] = t;
+    }
+    __syncthreads();
+    if (c == 0) {
+        uint32_t o = 0;
+        for (int x = 0; x < 24; x++) {
+            start[x] = o;
+            o += tot[x];
+        }
+    }
+    __syncthreads();
+    if (c < 24) {
+        uint32_t o = start[c];
+        for (int b = 0; b < nblocks; b++) {
+            const uint32_t h = hist[b * 24 + c];
+            hist[b * 24 + c] = o;
+            o += h;
+        }
+        totals[c] = tot[c];
+        totals[24 + c] = start[c];
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restrict__ colors, int64_t nq,
-                                                           uint32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+                                                           const uint32_t* __restrict__ offs,
+                                                           int32_t* __restrict__ list) {
+    __shared__ uint32_t cur[24];
+    if (threadIdx.x < 24) cur[threadIdx.x] = offs[blockIdx.x * 24 + threadIdx.x];
+    __syncthreads();
     TMC_GRID_STRIDE(f, nq) {
         const int32_t c = colors[f];
-        if (c >= 0 && c < 24) list[atomicAdd(&cursor[c], 1u)] = (int32_t)f;
+        if (c >= 0 && c < 24) list[atomicAdd(&cur[c], 1u)] = (int32_t)f;
     }
 }
 
