@@ -7,7 +7,7 @@ A step is one full dualMC of the workload's field (already resident in
 HBM): classification, compaction, vertices, quadrilaterals, 3X3Y (colour and
 isolated), 3333, halfedges and triangles; the mesh stays in HBM.  `value` is
 Gvoxels/s = nx*ny*nz / (time per step).  The roofline object is for the
-field sweep (k_classify), timed with HIP events on the engine's stream over
+field sweep (k_sweep), timed with HIP events on the engine's stream over
 the timed steps.  The CPU baseline is the reference cpp_dmc compiled from its
 sources (oracle/_ref) -- or the C restatement if that is absent -- timed on a
 sub-block of the same field on one host core.
@@ -25,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+# scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024.md)
+TRAFFIC = {}
 
 
 def log(msg):
@@ -73,7 +76,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="schwarz_1024", choices=["schwarz_1024", "blobs_512", "genus2_128"])
-    ap.add_argument("--cpu-block", type=int, default=256)
+    ap.add_argument("--cpu-block", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -122,18 +125,20 @@ def main():
     log(f"[bench] {args.steps} steps, {ms:.3f} ms/step; stages {stages}")
     summ = m.summary()
     V, Q, A = summ["nv_cells"], summ["nq_cells"], summ["active_cells"]
-    # field sweep: one read of the field plus the row lists it writes
-    classify_ms = stages.get("classify", float("nan"))
+    # field sweep: one read of the field (4 B per point) plus what it writes:
+    # 4 B per active cell into the segment lists and 4 B per segment count
+    sweep_ms = stages.get("sweep", float("nan"))
     R = (grid.ny - 1) * (grid.nz - 1)
-    sweep_bytes = 4 * nvox + 4 * A + 4 * R
-    achieved = sweep_bytes / (classify_ms * 1e-3) / 1e9
+    S = (grid.nx - 1 + 255) // 256
+    sweep_bytes = 4 * nvox + 4 * A + 4 * R * S
+    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_classify", "kernel_ms": classify_ms,
-                "algorithmic_bytes": sweep_bytes}
-    # extraction stages as a whole (B = 4N + 24V + 17Q over classify..quads)
-    ext_ms = sum(stages.get(k, 0.0) for k in ("classify", "compact", "cell_info", "vertices", "quads"))
-    ext_bytes = 4 * nvox + 24 * V + 17 * Q
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(args.workload),
+                "kernel": "k_sweep", "kernel_ms": sweep_ms, "algorithmic_bytes": sweep_bytes}
+    # cell stage as a whole (sweep, rows, vertices, quads): field read once,
+    # FP64 position + normal per vertex (48 B), 4 ids + 1 colour per quad
+    ext_ms = sum(stages.get(k, 0.0) for k in ("sweep", "rows", "vertices", "quads"))
+    ext_bytes = 4 * nvox + 48 * V + 17 * Q
     out = {
         "metric": "DMC Gvoxels/s at 1024^3 f32 (%HBM roofline)",
         "value": round(nvox / (ms * 1e-3) / 1e9, 3),

@@ -89,6 +89,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, D
     const int k1 = min(k0 + K, d.nz - 1);
     const int i0 = s * kSeg + lane * 4;
     const uint64_t lt = (1ull << lane) - 1ull;
+    // cells i0+q with q < 4 that exist (i < nx-1)
+    const int ncell = d.nx - 1 - i0;
+    const uint32_t valid = ncell >= 4 ? 0xFu : ncell > 0 ? (1u << ncell) - 1u : 0u;
     uint32_t prev[J + 1], cur[J + 1];
 #pragma unroll
     for (int jr = 0; jr <= J; jr++)
@@ -101,28 +104,25 @@ __global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, D
         for (int jr = 0; jr < J; jr++) {
             if (jr < nj) {
                 const uint32_t A = prev[jr], B = prev[jr + 1], Cm = cur[jr], D = cur[jr + 1];
-                uint32_t cs[4];
-                uint64_t bal[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t ic = ((A >> q) & 1u) | ((A >> (q + 1)) & 1u) << 1 | ((B >> q) & 1u) << 2 |
-                                        ((B >> (q + 1)) & 1u) << 3 | ((Cm >> q) & 1u) << 4 |
-                                        ((Cm >> (q + 1)) & 1u) << 5 | ((D >> q) & 1u) << 6 |
-                                        ((D >> (q + 1)) & 1u) << 7;
-                    const bool act = (i0 + q < d.nx - 1) && ic != 0u && ic != 255u;
-                    cs[q] = act ? ic : 256u;
-                    bal[q] = __ballot(act);
-                }
+                // bit q: cell i0+q has all 8 corners in / any corner in
+                const uint32_t all_in = A & (A >> 1) & B & (B >> 1) & Cm & (Cm >> 1) & D & (D >> 1);
+                const uint32_t any_in = A | (A >> 1) | B | (B >> 1) | Cm | (Cm >> 1) | D | (D >> 1);
+                uint32_t act = any_in & ~all_in & valid;
+                const uint64_t b0 = __ballot(act & 1u), b1 = __ballot(act & 2u), b2 = __ballot(act & 4u),
+                               b3 = __ballot(act & 8u);
                 const int64_t r = (int64_t)k * (d.ny - 1) + (j0 + jr);
-                const uint32_t total =
-                    (uint32_t)(__popcll(bal[0]) + __popcll(bal[1]) + __popcll(bal[2]) + __popcll(bal[3]));
+                const uint32_t total = (uint32_t)(__popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3));
                 if (total) {
-                    uint32_t pos = (uint32_t)(__popcll(bal[0] & lt) + __popcll(bal[1] & lt) + __popcll(bal[2] & lt) +
-                                              __popcll(bal[3] & lt));
+                    uint32_t pos = (uint32_t)(__popcll(b0 & lt) + __popcll(b1 & lt) + __popcll(b2 & lt) +
+                                              __popcll(b3 & lt));
                     uint32_t* out = lists + (r * S + s) * kSeg;
-#pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        if (cs[q] < 256u) out[pos++] = (uint32_t)(i0 + q) | (cs[q] << 24);
+                    while (act) {
+                        const int q = __builtin_ctz(act);
+                        act &= act - 1u;
+                        const uint32_t ic = ((A >> q) & 3u) | ((B >> q) & 3u) << 2 | ((Cm >> q) & 3u) << 4 |
+                                            ((D >> q) & 3u) << 6;
+                        out[pos++] = (uint32_t)(i0 + q) | (ic << 24);
+                    }
                 }
                 if (lane == 0) segCount[r * S + s] = total;
             }
