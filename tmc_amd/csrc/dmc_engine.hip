@@ -34,6 +34,8 @@ struct Buf {
 // enough to fill every CU several times over, few enough that waves live
 // long and keep many loads in flight.
 int kGridCap = 16384;
+// field sweep: waves to aim for (sets the planes per wave)
+int kSweepWaves = 8192;
 
 struct StageMark {
     const char* name;
@@ -176,7 +178,7 @@ void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, dou
     // the chip (about 32 per CU)
     const int ntj = (g.ny - 1 + kSweepRows - 1) / kSweepRows;
     const int64_t per_plane = (int64_t)S * ntj;
-    int K = (int)std::max<int64_t>(8, ((int64_t)(g.nz - 1) * per_plane + 8191) / 8192);
+    int K = (int)std::max<int64_t>(8, ((int64_t)(g.nz - 1) * per_plane + kSweepWaves - 1) / kSweepWaves);
     K = std::min(K, g.nz - 1);
     const int ntk = (g.nz - 1 + K - 1) / K;
     const int64_t waves = per_plane * ntk;
@@ -631,6 +633,10 @@ dmc_status dmc_create(dmc_context** out, int device) {
     if (const char* cap = getenv("TMC_GRID_CAP")) {
         const long v = strtol(cap, nullptr, 10);
         if (v >= 64 && v <= (1l << 24)) kGridCap = (int)v;
+    }
+    if (const char* w = getenv("TMC_SWEEP_WAVES")) {
+        const long v = strtol(w, nullptr, 10);
+        if (v >= 256 && v <= (1l << 24)) kSweepWaves = (int)v;
     }
     dmc_context* ctx = new dmc_context();
     ctx->device = device;

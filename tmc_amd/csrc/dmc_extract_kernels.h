@@ -27,49 +27,49 @@ struct Dims {
     int64_t R;      // rows (ny-1)*(nz-1)
 };
 
-template <class T>
-__device__ __forceinline__ bool at_or_above(T f, T iso_t) { return f >= iso_t; }
-
 // ---------------------------------------------------------------------------
-// K1: the field sweep.  A wave owns one 256-cell segment of J consecutive
-// cell rows and marches over K planes, keeping only the comparison bits
-// (field >= iso) of the previous point plane in registers, so every field
-// value is read once (plus one halo row per J rows and one halo plane per K
-// planes).  Each lane reads 4 consecutive values with one vector load; the
-// fifth (x+4) comes from the next lane.  Active cells (case not 0/255) of a
+// K1: the field sweep.  A wave owns one x-segment of 252 cells for J
+// consecutive cell rows and marches over K planes, keeping only the
+// comparison bits (field >= iso) of the previous point plane in registers,
+// so every field value is read once (plus one halo row per J rows and one
+// halo plane per K planes).  Lane L reads the 4 points x0+4L..x0+4L+3 with
+// one vector load and owns the cells starting at them; the fifth point
+// comes from the next lane, and for lane 63 from a wave-uniform scalar load
+// of point x0+256.  All loads of a plane are issued before any comparison
+// (clamped addresses, no branches), so a wave keeps J+1 loads in flight.  Active cells (case not 0/255) of a
 // (row, segment) are written in ascending x using wave ballots.  For F32
 // fields (double)f >= iso is evaluated as f >= iso_up, iso_up the smallest
 // float not below iso: the same predicate.
 // ---------------------------------------------------------------------------
-constexpr int kSeg = 256;           // cells per segment (one wave)
-constexpr int kSweepRows = 16;      // J
+constexpr int kSeg = 256;           // cells per segment (one wave) and list slots per (row, segment)
+#ifndef TMC_SWEEP_ROWS
+#define TMC_SWEEP_ROWS 16
+#endif
+constexpr int kSweepRows = TMC_SWEEP_ROWS;  // J
 
 template <class T, bool VEC>
-__device__ __forceinline__ uint32_t row_bits(const T* __restrict__ field, const Dims& d, int i0, int j, int k,
-                                             T iso_t, int lane) {
-    const T* row = field + ((int64_t)k * d.ny + j) * d.nx;
-    uint32_t b = 0;
-    if (VEC) {
-        if (i0 < d.nx) {
-            if constexpr (sizeof(T) == 4) {
-                const float4 v = *reinterpret_cast<const float4*>(row + i0);
-                b = (uint32_t)at_or_above<T>(v.x, iso_t) | (uint32_t)at_or_above<T>(v.y, iso_t) << 1 |
-                    (uint32_t)at_or_above<T>(v.z, iso_t) << 2 | (uint32_t)at_or_above<T>(v.w, iso_t) << 3;
-            } else {
-                const double2 v0 = *reinterpret_cast<const double2*>(row + i0);
-                const double2 v1 = *reinterpret_cast<const double2*>(row + i0 + 2);
-                b = (uint32_t)at_or_above<T>(v0.x, iso_t) | (uint32_t)at_or_above<T>(v0.y, iso_t) << 1 |
-                    (uint32_t)at_or_above<T>(v1.x, iso_t) << 2 | (uint32_t)at_or_above<T>(v1.y, iso_t) << 3;
-            }
-        }
+__device__ __forceinline__ void row_load(const T* __restrict__ row, int i0, int nx, T v[4]) {
+    if constexpr (VEC) {
+        const int ic = min(i0, nx - 4);  // nx % 4 == 0: lanes past the end read the last 4 points
+        typedef T vec4 __attribute__((ext_vector_type(4)));
+        // streamed once: non-temporal, so it does not evict reused lines
+        const vec4 t = __builtin_nontemporal_load(reinterpret_cast<const vec4*>(row + ic));
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (i0 + q < d.nx) b |= (uint32_t)at_or_above<T>(row[i0 + q], iso_t) << q;
+        for (int q = 0; q < 4; q++) v[q] = row[min(i0 + q, nx - 1)];
     }
-    // point x+4: first bit of the next lane; lane 63 reads it itself
-    uint32_t nb = (uint32_t)__shfl_down((int)(b & 1u), 1, kWave);
-    if (lane == kWave - 1) nb = (i0 + 4 < d.nx) ? (uint32_t)at_or_above<T>(row[i0 + 4], iso_t) : 0u;
+}
+
+// bits 0..3: points i0..i0+3 at or above iso; bit 4: point i0+4 (next
+// lane; `edge` is point x0+256 for lane 63)
+template <class T>
+__device__ __forceinline__ uint32_t row_bits(const T v[4], T edge, T iso_t, int lane) {
+    const uint32_t b = (uint32_t)(v[0] >= iso_t) | (uint32_t)(v[1] >= iso_t) << 1 | (uint32_t)(v[2] >= iso_t) << 2 |
+                       (uint32_t)(v[3] >= iso_t) << 3;
+    // DPP wave_shl:1: lane L receives lane L+1's value
+    uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(b & 1u), 0x130, 0xF, 0xF, false);
+    nb = lane == kWave - 1 ? (uint32_t)(edge >= iso_t) : nb;
     return b | (nb << 4);
 }
 
@@ -79,7 +79,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, D
                                                    uint32_t* __restrict__ segCount) {
     constexpr int J = kSweepRows;
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    // wave-uniform from here on (kept in scalar registers)
+    const int64_t wave =
+        (int64_t)blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     if (wave >= (int64_t)S * ntj * ntk) return;
     const int s = (int)(wave % S);
     const int64_t rest = wave / S;
@@ -88,18 +90,40 @@ __global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, D
     const int nj = min(J, d.ny - 1 - j0);
     const int k1 = min(k0 + K, d.nz - 1);
     const int i0 = s * kSeg + lane * 4;
+    const int ie = min(s * kSeg + kSeg, d.nx - 1);  // lane 63's fifth point (uniform)
     const uint64_t lt = (1ull << lane) - 1ull;
-    // cells i0+q with q < 4 that exist (i < nx-1)
+    // cells i0+q (q < 4) that exist (i < nx-1)
     const int ncell = d.nx - 1 - i0;
     const uint32_t valid = ncell >= 4 ? 0xFu : ncell > 0 ? (1u << ncell) - 1u : 0u;
+    const int64_t plane = (int64_t)d.nx * d.ny;
+    // row base pointers are wave-uniform; rows past the last one are clamped
+    // to it (their bits feed no cell)
     uint32_t prev[J + 1], cur[J + 1];
+    {
+        T v[J + 1][4], e[J + 1];
+        const T* pl = field + (int64_t)k0 * plane;
 #pragma unroll
-    for (int jr = 0; jr <= J; jr++)
-        prev[jr] = jr <= nj ? row_bits<T, VEC>(field, d, i0, j0 + jr, k0, iso_t, lane) : 0u;
+        for (int jr = 0; jr <= J; jr++) {
+            const T* row = pl + (int64_t)min(j0 + jr, d.ny - 1) * d.nx;
+            row_load<T, VEC>(row, i0, d.nx, v[jr]);
+            e[jr] = row[ie];
+        }
+#pragma unroll
+        for (int jr = 0; jr <= J; jr++) prev[jr] = row_bits<T>(v[jr], e[jr], iso_t, lane);
+    }
     for (int k = k0; k < k1; k++) {
+        {
+            T v[J + 1][4], e[J + 1];
+            const T* pl = field + (int64_t)(k + 1) * plane;
 #pragma unroll
-        for (int jr = 0; jr <= J; jr++)
-            cur[jr] = jr <= nj ? row_bits<T, VEC>(field, d, i0, j0 + jr, k + 1, iso_t, lane) : 0u;
+            for (int jr = 0; jr <= J; jr++) {
+                const T* row = pl + (int64_t)min(j0 + jr, d.ny - 1) * d.nx;
+                row_load<T, VEC>(row, i0, d.nx, v[jr]);
+                e[jr] = row[ie];
+            }
+#pragma unroll
+            for (int jr = 0; jr <= J; jr++) cur[jr] = row_bits<T>(v[jr], e[jr], iso_t, lane);
+        }
 #pragma unroll
         for (int jr = 0; jr < J; jr++) {
             if (jr < nj) {
