@@ -26,8 +26,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-# scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024.md)
-TRAFFIC = {}
+# scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024_traffic.md)
+TRAFFIC = {"schwarz_1024": 5375285233}
 
 
 def log(msg):
@@ -70,6 +70,21 @@ def cpu_baseline(field, grid, iso, block):
                       f"V={nv} Q={nq}; host CPU: {cpu_model()} ({os.cpu_count()} logical)"}
 
 
+def calibrate_copy():
+    """A plain copy kernel of known size: 1 GiB read + 1 GiB written per
+    launch.  Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, the ratio of
+    the known bytes to the counter reading scales the counters of the DMC
+    kernels.  Uses torch only (not the DMC library) in its own process."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +93,12 @@ def main():
     ap.add_argument("--workload", default="schwarz_1024", choices=["schwarz_1024", "blobs_512", "genus2_128"])
     ap.add_argument("--cpu-block", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); the multi-GPU "

@@ -184,7 +184,8 @@ void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, dou
     const int64_t waves = per_plane * ntk;
     const bool vec = (g.nx % 4 == 0) && ((uintptr_t)field % 16 == 0);
     mark(ctx, "sweep");
-    const dim3 sgrid((unsigned)((waves + kBlock / kWave - 1) / (kBlock / kWave)));
+    const int64_t sblocks = (waves + kBlock / kWave - 1) / (kBlock / kWave);
+    const dim3 sgrid((unsigned)((sblocks + kXcds - 1) / kXcds * kXcds));
     if (vec)
         hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, field, d, iso_t, S, ntj, K, ntk,
                            lists, segCount);

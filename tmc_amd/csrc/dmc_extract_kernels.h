@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, D
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform from here on (kept in scalar registers)
     const int64_t wave =
-        (int64_t)blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+        xcd_block() * (kBlock / kWave) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     if (wave >= (int64_t)S * ntj * ntk) return;
     const int s = (int)(wave % S);
     const int64_t rest = wave / S;

@@ -8,6 +8,17 @@ namespace tmc {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+// The 8 XCDs each have their own L2 and receive workgroups round-robin
+// (block b runs on XCD b % 8).  xcd_block() renumbers blocks so that each
+// XCD works on one contiguous range of logical blocks, keeping the halo
+// shared by neighbouring tiles in one L2.  It is a bijection when the grid
+// is a multiple of 8 blocks (the launcher rounds up), else the identity.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    if (nb % kXcds) return b;
+    return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
 
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan(T x) {
