@@ -110,6 +110,27 @@ dmc_status dmc_destroy(dmc_context* ctx);
 dmc_status dmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type,
                        double iso, uint32_t flags, const double* gradients, dmc_mesh* out);
 
+/* One volume of a batch: grid, field (element type shared by the batch),
+ * optional gradients (DMC_GRADIENTS_SUPPLIED) and iso-value.  With
+ * DMC_FIELD_ON_DEVICE the pointers are device pointers. */
+typedef struct {
+    dmc_grid grid;
+    const void* field;
+    const double* gradients;
+    double iso;
+} dmc_volume;
+
+/* Extract many independent volumes in one pass (the batched mode; no
+ * counterpart in the reference, which extracts one grid per dualMC call,
+ * cpp_dmc/DualMarchingCubes.h:163-165).  All volumes share field_type and
+ * flags.  out[v] receives volume v's mesh exactly as dmc_extract would
+ * return it for that volume alone: same counts, same arrays, ids starting
+ * at 0; device pointers stay valid until the next extraction on ctx.
+ * Diagnostics invalid_refs and valence_zero are batch-wide; all other
+ * fields are per volume. */
+dmc_status dmc_extract_batch(dmc_context* ctx, const dmc_volume* vols, int32_t nvol, int field_type, uint32_t flags,
+                             dmc_mesh* out);
+
 /* Copy the last mesh to host memory; any pointer may be NULL.
  * positions/normals: 3*nv doubles; quads 4*nq; tris 3*nt;
  * halfedges: 5 ints per halfedge {origin, face, next, twin, flag} as in
@@ -118,6 +139,9 @@ dmc_status dmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field
 dmc_status dmc_mesh_download(dmc_context* ctx, double* positions, double* normals, int32_t* quads,
                              int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f,
                              int32_t* bnd);
+/* The same for volume `index` of the last dmc_extract_batch. */
+dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions, double* normals, int32_t* quads,
+                              int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f, int32_t* bnd);
 
 /* FP32 export as the CUDA reference returns it (dmc/DualMarchingCubes.cu:
  * 1076-1090): positions and normals as float triples; when negate_normals

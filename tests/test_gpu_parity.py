@@ -234,3 +234,94 @@ def test_cpp_dropin_programs(oracle, tmp_path):
     assert_same(rd("file_v.f64", np.float64).reshape(-1, 3), exp2["v"], "file v")
     assert_same(rd("file_quads.i32", np.int32).reshape(-1, 4), exp2["quads"], "file quads")
 
+
+
+# ---------------------------------------------------------------------------
+# batched extraction: every volume of a batch must come back exactly as the
+# same volume extracted alone (and the alone path is pinned to the oracle
+# above), whatever else is in the batch
+# ---------------------------------------------------------------------------
+SUMMARY_KEYS = ("nv", "nq", "nt", "nbnd", "nonmanifold_edges", "edge_errors", "failed_projections_l1",
+                "failed_projections_l2", "active_cells", "nv_emitted", "nv_cells", "nq_cells")
+
+
+def _batch_volumes(seed, count):
+    from tmc_amd import workloads
+    vols = []
+    for it, f, g, iso in _fields(seed, count, max_n=12):
+        if f.dtype != np.float32:
+            f = f.astype(np.float32)
+        vols.append((f, iso, g))
+    # mixed generators of the batched workload, small sizes
+    for s in workloads.batch_specs(8, seed=seed)[:8]:
+        shape = tuple(max(2, x // 3) for x in s["shape"])
+        f, g, iso = workloads.make(dict(s, shape=shape))
+        vols.append((f, iso, g))
+    # degenerate members: no surface, minimum size, all equal to iso
+    vols.append((np.zeros((2, 2, 2), np.float32), 0.5, tmc_amd.Grid.make((2, 2, 2))))
+    vols.append((np.full((3, 4, 5), 0.25, np.float32), 0.25, tmc_amd.Grid.make((3, 4, 5))))
+    return vols
+
+
+def _alone(engine, f, iso, g, flags):
+    m = engine.extract(f, iso, g, flags=flags)
+    return m.summary(), engine.download()
+
+
+@pytest.mark.parametrize("flags", [SIMPLIFY_ALL | TRIANGLES, TRIANGLES, SIMPLIFY_3X3Y_ISOLATED | SIMPLIFY_3333])
+def test_batch_matches_alone(engine, flags):
+    vols = _batch_volumes(41, 25)
+    _say(f"[gpu] batch of {len(vols)} volumes, flags {flags}")
+    alone = [_alone(engine, f, iso, g, flags) for f, iso, g in vols]
+    meshes = engine.extract_batch(vols, flags=flags)
+    assert len(meshes) == len(vols)
+    for i, (exp_sum, exp) in enumerate(alone):
+        got_sum = meshes[i].summary()
+        for k in SUMMARY_KEYS:
+            assert got_sum[k] == exp_sum[k], f"volume {i}: {k} {got_sum[k]} vs {exp_sum[k]}"
+        got = engine.download_batch(i)
+        for k in ("v", "n", "quads", "tris", "he", "he_v", "he_f", "bnd"):
+            assert_same(got[k], exp[k], f"volume {i}:{k}")
+
+
+def test_batch_order_and_reuse(engine):
+    """The same volumes in another order, and a batch after a bigger one,
+    give the same meshes (no state leaks between batches or members)."""
+    vols = _batch_volumes(43, 10)
+    flags = SIMPLIFY_ALL | TRIANGLES
+    engine.extract_batch(vols, flags=flags)
+    first = [engine.download_batch(i) for i in range(len(vols))]
+    order = list(range(len(vols)))[::-1]
+    engine.extract_batch([vols[i] for i in order], flags=flags)
+    for j, i in enumerate(order):
+        got = engine.download_batch(j)
+        for k in ("v", "n", "quads", "tris", "he", "he_v", "bnd"):
+            assert_same(got[k], first[i][k], f"reordered volume {i}:{k}")
+    engine.extract_batch(vols[:3], flags=flags)
+    for i in range(3):
+        got = engine.download_batch(i)
+        for k in ("v", "quads", "he"):
+            assert_same(got[k], first[i][k], f"smaller batch volume {i}:{k}")
+
+
+def test_batch_device_fields(engine):
+    """Device-resident fields of a batch (one allocation, volumes at offsets)."""
+    vols = _batch_volumes(47, 6)
+    flags = SIMPLIFY_ALL | TRIANGLES
+    offs, total = [], 0
+    for f, _, _ in vols:
+        offs.append(total)
+        total += (f.nbytes + 255) // 256 * 256
+    buf = engine.alloc(total)
+    host = np.zeros(total, np.uint8)
+    for (f, _, _), o in zip(vols, offs):
+        host[o:o + f.nbytes] = np.frombuffer(f.tobytes(), np.uint8)
+    buf.upload(host)
+    dvols = [(buf.ptr.value + o, iso, g) for (f, iso, g), o in zip(vols, offs)]
+    exp = [_alone(engine, f, iso, g, flags)[1] for f, iso, g in vols]
+    engine.extract_batch(dvols, flags=flags | tmc_amd.FIELD_ON_DEVICE)
+    for i in range(len(vols)):
+        got = engine.download_batch(i)
+        for k in ("v", "n", "quads", "he", "bnd"):
+            assert_same(got[k], exp[i][k], f"device volume {i}:{k}")
+    buf.free()

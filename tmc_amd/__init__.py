@@ -37,9 +37,9 @@ FIELD_F32 = 0
 FIELD_F64 = 1
 
 # every symbol include/tmc_dmc.h declares
-EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_mesh_download", "dmc_mesh_download_f32",
-           "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host",
-           "dmc_synchronize", "dmc_status_str", "dmc_last_error"]
+EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc_mesh_download",
+           "dmc_batch_download", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc", "dmc_device_free",
+           "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str", "dmc_last_error"]
 
 
 class DMCError(RuntimeError):
@@ -56,6 +56,11 @@ class Grid(C.Structure):
         """shape is (nz, ny, nx), the numpy order of an x-fastest field."""
         nz, ny, nx = (int(s) for s in shape)
         return cls(nx, ny, nz, *[float(o) for o in origin], *[float(s) for s in spacing])
+
+
+class Volume(C.Structure):
+    """dmc_volume: one volume of a batch."""
+    _fields_ = [("grid", Grid), ("field", C.c_void_p), ("gradients", C.c_void_p), ("iso", C.c_double)]
 
 
 class Mesh(C.Structure):
@@ -89,6 +94,8 @@ def load_library(path=ENGINE_SO):
     L.dmc_destroy.argtypes = [vp]
     L.dmc_extract.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, vp, C.POINTER(Mesh)]
     L.dmc_mesh_download.argtypes = [vp, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
+    L.dmc_extract_batch.argtypes = [vp, C.POINTER(Volume), C.c_int32, C.c_int, C.c_uint32, C.POINTER(Mesh)]
+    L.dmc_batch_download.argtypes = [vp, C.c_int32, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     L.dmc_mesh_download_f32.argtypes = [vp, f32p, f32p, C.c_int]
     L.dmc_stage_times.argtypes = [vp, C.POINTER(C.c_char_p), f32p, C.POINTER(C.c_int)]
     L.dmc_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
@@ -100,7 +107,8 @@ def load_library(path=ENGINE_SO):
     L.dmc_status_str.restype = C.c_char_p
     L.dmc_last_error.argtypes = [vp]
     L.dmc_last_error.restype = C.c_char_p
-    for name in ("dmc_create", "dmc_destroy", "dmc_extract", "dmc_mesh_download", "dmc_mesh_download_f32",
+    for name in ("dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc_mesh_download",
+                 "dmc_batch_download", "dmc_mesh_download_f32",
                  "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
                  "dmc_copy_to_host", "dmc_synchronize"):
         getattr(L, name).restype = C.c_int
@@ -143,6 +151,7 @@ class Engine:
         if s != 0:
             raise DMCError(f"dmc_create: {self.lib.dmc_status_str(s).decode()}")
         self.mesh = None
+        self.batch = None
 
     def _check(self, s):
         if s != 0:
@@ -202,9 +211,47 @@ class Engine:
         self.mesh = m
         return m
 
-    def download(self, halfedges=True, fp32=False):
-        """Copy the last mesh to host numpy arrays."""
-        m = self.mesh
+    def extract_batch(self, volumes, flags=SIMPLIFY_ALL | TRIANGLES, dtype=np.float32):
+        """Extract many volumes in one pass.  `volumes` is a list of
+        (field, iso, grid) with numpy fields (nz, ny, nx) -- or
+        (device_pointer, iso, grid) with FIELD_ON_DEVICE in `flags`.  All
+        fields share one element type.  Returns one Mesh per volume."""
+        n = len(volumes)
+        vols = (Volume * n)()
+        keep = []
+        ftype = FIELD_F32 if np.dtype(dtype) == np.float32 else FIELD_F64
+        for i, (field, iso, grid) in enumerate(volumes):
+            if flags & FIELD_ON_DEVICE:
+                ptr = field.ptr.value if isinstance(field, DeviceBuffer) else int(field)
+                if grid is None:
+                    raise DMCError("a device field needs its grid")
+            else:
+                arr = np.ascontiguousarray(field, dtype=np.dtype(dtype))
+                keep.append(arr)
+                ptr = arr.ctypes.data
+                if grid is None:
+                    grid = Grid.make(arr.shape)
+            if not isinstance(grid, Grid):
+                grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+            vols[i].grid = grid
+            vols[i].field = ptr
+            vols[i].gradients = None
+            vols[i].iso = float(iso)
+        meshes = (Mesh * n)()
+        self._check(self.lib.dmc_extract_batch(self.ctx, vols, n, ftype, int(flags), meshes))
+        del keep
+        self.mesh = None
+        self.batch = list(meshes)
+        return self.batch
+
+    def download_batch(self, index, halfedges=True):
+        """Copy volume `index` of the last extract_batch to host arrays."""
+        return self.download(halfedges=halfedges, index=index)
+
+    def download(self, halfedges=True, fp32=False, index=None):
+        """Copy the last mesh (or volume `index` of the last batch) to host
+        numpy arrays."""
+        m = self.mesh if index is None else self.batch[index]
         if m is None:
             raise DMCError("no mesh: call extract first")
         out = {}
@@ -217,9 +264,12 @@ class Engine:
         hf = np.empty(m.nq, np.int32)
         bnd = np.empty(m.nbnd, np.int32)
         i32 = C.c_int32
-        self._check(self.lib.dmc_mesh_download(
-            self.ctx, _p(v, C.c_double), _p(n, C.c_double), _p(q, i32), _p(t, i32) if m.nt else None,
-            _p(he, i32) if he is not None else None, _p(hv, i32), _p(hf, i32), _p(bnd, i32) if m.nbnd else None))
+        args = (_p(v, C.c_double), _p(n, C.c_double), _p(q, i32), _p(t, i32) if m.nt else None,
+                _p(he, i32) if he is not None else None, _p(hv, i32), _p(hf, i32), _p(bnd, i32) if m.nbnd else None)
+        if index is None:
+            self._check(self.lib.dmc_mesh_download(self.ctx, *args))
+        else:
+            self._check(self.lib.dmc_batch_download(self.ctx, int(index), *args))
         out.update(v=v, n=n, quads=q, tris=t, he_v=hv, he_f=hf, bnd=bnd)
         if he is not None:
             out["he"] = he

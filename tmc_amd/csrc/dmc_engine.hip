@@ -52,6 +52,12 @@ struct dmc_context {
     // cell sweep
     Buf segCount, segBase, lists, acell, acode, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
     Buf field, grads, fails, scanPartial, scanTotal;
+    // volume table of the last extraction (one volume, or a batch) and the
+    // per-volume first active cell / vertex / quadrilateral, kept through
+    // compactions
+    Buf vtab, vstat, vbound, qbound, vhe;
+    std::vector<uint8_t> vtab_host;
+    int nvol = 0;
     // mesh (double-buffered where compacted)
     Buf pos[2], nrm[2], quads[2], colors;
     int cur = 0;  // which of the double buffers holds the live mesh
@@ -62,6 +68,9 @@ struct dmc_context {
     dmc_mesh mesh{};
     bool have_mesh = false;
     std::vector<int32_t> bnd;  // sorted boundary vertices of the last final halfedge build
+    // per-volume results of the last extraction (ids rebased to each volume)
+    std::vector<dmc_mesh> vmesh;
+    std::vector<std::vector<int32_t>> vbnd;
     // stage timing
     bool timing = false;
     std::vector<StageMark> marks;
@@ -72,7 +81,7 @@ struct dmc_context {
                       &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
-                      &keepQ, &newQ, &clist, &ccnt, &tris, &stats};
+                      &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -165,38 +174,100 @@ uint64_t read_u64(dmc_context* ctx, const void* dptr) {
 // ---------------------------------------------------------------------------
 // cell stage
 // ---------------------------------------------------------------------------
+// One volume of an extraction, with device pointers.
+struct HostVol {
+    dmc_grid g;
+    const void* field;
+    const double* grads;
+    double iso;
+};
+
+float smallest_float_not_below(double iso) {
+    float f = (float)iso;
+    if (std::isnan(iso)) return f;
+    if ((double)f < iso) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+int64_t* vol_ptr(dmc_context* ctx, int which) {  // 0 waveBase, 1 rowBase, 2 aBase
+    const size_t n = (size_t)ctx->nvol;
+    uint8_t* base = static_cast<uint8_t*>(ctx->vtab.p) + n * sizeof(VolDesc);
+    return reinterpret_cast<int64_t*>(base) + (size_t)which * (n + 1);
+}
+
+VolTable vol_table(dmc_context* ctx) {
+    return VolTable{static_cast<const VolDesc*>(ctx->vtab.p), vol_ptr(ctx, 0), vol_ptr(ctx, 1), vol_ptr(ctx, 2),
+                    ctx->nvol};
+}
+
 template <class T>
-void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, double iso, const double* grads) {
-    Dims d{g.nx, g.ny, g.nz, (int64_t)g.nx - 1, (int64_t)(g.ny - 1) * (g.nz - 1)};
-    const int64_t R = d.R;
-    const int S = (int)((d.C + kSeg - 1) / kSeg);
-    const int64_t RS = R * S;
+void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
+    const int n = (int)hv.size();
+    ctx->nvol = n;
+    // sweep tiling: J rows per wave; one plane count K for the whole
+    // extraction, chosen so that about kSweepWaves waves fill the chip
+    std::vector<VolDesc> desc(n);
+    int64_t units = 0;
+    for (int v = 0; v < n; v++) {
+        const dmc_grid& g = hv[v].g;
+        VolDesc& D = desc[v];
+        D.field = hv[v].field;
+        D.grads = hv[v].grads;
+        D.d = Dims{g.nx, g.ny, g.nz, (int64_t)g.nx - 1, (int64_t)(g.ny - 1) * (g.nz - 1)};
+        D.geo = GridGeom{g.x0, g.y0, g.z0, g.dx, g.dy, g.dz};
+        D.iso = hv[v].iso;
+        D.iso_t = sizeof(T) == 4 ? (double)smallest_float_not_below(hv[v].iso) : hv[v].iso;
+        D.S = (int)((D.d.C + kSeg - 1) / kSeg);
+        D.ntj = (g.ny - 1 + kSweepRows - 1) / kSweepRows;
+        units += (int64_t)(g.nz - 1) * D.S * D.ntj;
+    }
+    const int Kg = (int)std::max<int64_t>(8, (units + kSweepWaves - 1) / kSweepWaves);
+    const size_t tab_bytes = (size_t)n * sizeof(VolDesc) + 3 * (size_t)(n + 1) * sizeof(int64_t);
+    ctx->vtab_host.assign(tab_bytes, 0);
+    int64_t* waveBase = reinterpret_cast<int64_t*>(ctx->vtab_host.data() + (size_t)n * sizeof(VolDesc));
+    int64_t* rowBase = waveBase + (n + 1);
+    int64_t W = 0, R = 0, RS = 0;
+    bool vec = true;
+    for (int v = 0; v < n; v++) {
+        VolDesc& D = desc[v];
+        D.K = std::min(Kg, D.d.nz - 1);
+        D.ntk = (D.d.nz - 1 + D.K - 1) / D.K;
+        D.rowBase = R;
+        D.rsBase = RS;
+        waveBase[v] = W;
+        rowBase[v] = R;
+        W += (int64_t)D.S * D.ntj * D.ntk;
+        R += D.d.R;
+        RS += D.d.R * D.S;
+        vec = vec && (D.d.nx % 4 == 0) && ((uintptr_t)D.field % (4 * sizeof(T)) == 0);
+    }
+    waveBase[n] = W;
+    rowBase[n] = R;
+    memcpy(ctx->vtab_host.data(), desc.data(), (size_t)n * sizeof(VolDesc));
+    ensure<uint8_t>(ctx, ctx->vtab, tab_bytes);
+    // pageable source: the copy has been staged when the call returns
+    HIPC(ctx, hipMemcpyAsync(ctx->vtab.p, ctx->vtab_host.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+    const VolTable vt = vol_table(ctx);
+    VolStat* vstat = ensure<VolStat>(ctx, ctx->vstat, n);
+    HIPC(ctx, hipMemsetAsync(vstat, 0, (size_t)n * sizeof(VolStat), ctx->stream));
+
     uint32_t* segCount = ensure<uint32_t>(ctx, ctx->segCount, RS);
     uint32_t* segBase = ensure<uint32_t>(ctx, ctx->segBase, RS);
     uint32_t* lists = ensure<uint32_t>(ctx, ctx->lists, (size_t)RS * kSeg);
-    // sweep tiling: J rows per wave, K planes per wave, enough waves to fill
-    // the chip (about 32 per CU)
-    const int ntj = (g.ny - 1 + kSweepRows - 1) / kSweepRows;
-    const int64_t per_plane = (int64_t)S * ntj;
-    int K = (int)std::max<int64_t>(8, ((int64_t)(g.nz - 1) * per_plane + kSweepWaves - 1) / kSweepWaves);
-    K = std::min(K, g.nz - 1);
-    const int ntk = (g.nz - 1 + K - 1) / K;
-    const int64_t waves = per_plane * ntk;
-    const bool vec = (g.nx % 4 == 0) && ((uintptr_t)field % 16 == 0);
     mark(ctx, "sweep");
-    const int64_t sblocks = (waves + kBlock / kWave - 1) / (kBlock / kWave);
+    const int64_t sblocks = (W + kBlock / kWave - 1) / (kBlock / kWave);
     const dim3 sgrid((unsigned)((sblocks + kXcds - 1) / kXcds * kXcds));
     if (vec)
-        hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, field, d, iso_t, S, ntj, K, ntk,
-                           lists, segCount);
+        hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount);
     else
-        hipLaunchKernelGGL((k_sweep<T, false>), sgrid, dim3(kBlock), 0, ctx->stream, field, d, iso_t, S, ntj, K, ntk,
-                           lists, segCount);
+        hipLaunchKernelGGL((k_sweep<T, false>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount);
     check_launch(ctx, "k_sweep");
     mark(ctx, "rows");
     const uint64_t A = scan(ctx, segCount, segBase, RS);
     if (A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)A);
     ctx->mesh.active_cells = (int64_t)A;
+    hipLaunchKernelGGL(k_vol_abase, dim3(grid_for(n + 1)), dim3(kBlock), 0, ctx->stream, vt, segBase, (int64_t)A,
+                       vol_ptr(ctx, 2));
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
     uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, A);
@@ -205,49 +276,43 @@ void run_cells(dmc_context* ctx, const dmc_grid& g, const T* field, T iso_t, dou
     uint32_t* rowQ = ensure<uint32_t>(ctx, ctx->rowQ, R);
     uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
     uint32_t* rowBaseQ = ensure<uint32_t>(ctx, ctx->rowBaseQ, R);
-    // fails[0..1]: failed projections; fails[2..3]: 64-bit contour count;
     // fails[4]: vertices of valence 0 (all valence passes); fails[5]: quads
     // with an unresolved neighbour cell
     uint32_t* fails = ensure<uint32_t>(ctx, ctx->fails, 6);
-    unsigned long long* ncont = reinterpret_cast<unsigned long long*>(fails + 2);
     HIPC(ctx, hipMemsetAsync(fails, 0, 6 * sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, field, d, iso, S,
-                       lists, segCount, segBase, acell, acode, avloc, aqloc, rowV, rowQ, ncont);
+    hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, vt, R, lists,
+                       segCount, segBase, acell, acode, avloc, aqloc, rowV, rowQ, vstat);
     check_launch(ctx, "k_rows");
     const uint64_t V = scan(ctx, rowV, rowBaseV, R);
     const uint64_t Q = scan(ctx, rowQ, rowBaseQ, R);
     if (V > 0x7FFFFFFFull || Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)V, (unsigned long long)Q);
+    // [0, n]: live bounds; [n+1, 2n+1]: the cell stage's (before simplification)
+    int64_t* vb = ensure<int64_t>(ctx, ctx->vbound, 2 * (size_t)(n + 1));
+    int64_t* qb = ensure<int64_t>(ctx, ctx->qbound, 2 * (size_t)(n + 1));
+    hipLaunchKernelGGL(k_vol_bounds, dim3(grid_for(n + 1)), dim3(kBlock), 0, ctx->stream, vt, rowBaseV, rowBaseQ,
+                       (int64_t)V, (int64_t)Q, vb, qb);
+    HIPC(ctx, hipMemcpyAsync(vb + n + 1, vb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync(qb + n + 1, qb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
     ctx->cur = 0;
     double* pos = ensure<double>(ctx, ctx->pos[0], 3 * V);
     double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * V);
     int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * Q);
     int32_t* colors = ensure<int32_t>(ctx, ctx->colors, Q);
-    GridGeom geo{g.x0, g.y0, g.z0, g.dx, g.dy, g.dz};
     mark(ctx, "vertices");
     if (A > 0) {
-        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, field, grads, d, geo, iso,
-                           acell, acode, avloc, rowBaseV, (int64_t)A, pos, nrm, fails);
+        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc,
+                           rowBaseV, (int64_t)A, pos, nrm, vstat);
         check_launch(ctx, "k_vertices");
         mark(ctx, "quads");
-        SegLists L{lists, segCount, segBase, S};
-        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, d, acell, acode, avloc, aqloc,
-                           rowBaseV, rowBaseQ, (int64_t)A, L, quads, colors, fails + 5);
+        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc, aqloc,
+                           rowBaseV, rowBaseQ, (int64_t)A, lists, segCount, segBase, quads, colors, fails + 5);
         check_launch(ctx, "k_quads");
     }
-    uint32_t f[4] = {0, 0, 0, 0};
-    HIPC(ctx, hipMemcpyAsync(f, fails, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->mesh.failed_projections_l1 = f[0];
-    ctx->mesh.failed_projections_l2 = f[1];
     ctx->mesh.nv = (int64_t)V;
     ctx->mesh.nq = (int64_t)Q;
     ctx->mesh.nv_cells = (int64_t)V;
     ctx->mesh.nq_cells = (int64_t)Q;
-    // vertices the reference emits before removeUnusedVertices: every contour
-    unsigned long long nc;
-    memcpy(&nc, f + 2, sizeof(nc));
-    ctx->mesh.nv_emitted = (int64_t)nc;
 }
 
 // ---------------------------------------------------------------------------
@@ -303,8 +368,13 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     if (nhe > 0) {
         hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
                            cursor, adj);
+        unsigned long long* vst = nullptr;
+        if (final_pass && ctx->nvol > 1) {
+            vst = ensure<unsigned long long>(ctx, ctx->vhe, 2 * (size_t)ctx->nvol);
+            HIPC(ctx, hipMemsetAsync(vst, 0, 2 * (size_t)ctx->nvol * sizeof(unsigned long long), ctx->stream));
+        }
         hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, deg,
-                           adj, twin, flag, st);
+                           adj, twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst);
         check_launch(ctx, "k_he_twins");
     }
     HIPC(ctx, hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
@@ -359,6 +429,14 @@ void color_faces(dmc_context* ctx) {
     check_launch(ctx, "colour");
 }
 
+// Per-volume first vertex / quadrilateral after an order-preserving
+// compaction (newidx = exclusive scan of the keep flags).
+void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, int64_t nold, int64_t nnew) {
+    hipLaunchKernelGGL(k_vol_remap, dim3(grid_for(ctx->nvol + 1)), dim3(kBlock), 0, ctx->stream,
+                       static_cast<int64_t*>(bounds.p), ctx->nvol + 1, newidx, nold, nnew);
+    check_launch(ctx, "k_vol_remap");
+}
+
 // Compact vertices by `removed`; vsecond (if non-null) receives new indices of
 // kept vertices.  Returns the new count; swaps the live buffers.
 int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond, uint32_t** newidx_out) {
@@ -376,6 +454,7 @@ int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsec
                            static_cast<double*>(ctx->pos[src].p), static_cast<double*>(ctx->nrm[src].p), pos2, nrm2,
                            vsecond);
     check_launch(ctx, "k_compact_vertices");
+    remap_bounds(ctx, ctx->vbound, newidx, nv, nn);
     if (newidx_out) *newidx_out = newidx;
     return nn;
 }
@@ -424,6 +503,7 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
         hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
                            vfirst, vsecond, nv, q2, &st->bad);
     check_launch(ctx, "k_compact_quads3x3y");
+    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;
@@ -471,16 +551,10 @@ void simplify3333(dmc_context* ctx) {
         hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
                            newQ, p3333, vtwin, vflag, newv, nv, q2, &st->bad);
     check_launch(ctx, "k_compact_quads3333");
+    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;
-}
-
-float smallest_float_not_below(double iso) {
-    float f = (float)iso;
-    if (std::isnan(iso)) return f;
-    if ((double)f < iso) f = std::nextafter(f, INFINITY);
-    return f;
 }
 
 void finish_timing(dmc_context* ctx) {
@@ -510,58 +584,27 @@ dmc_status guard(dmc_context* ctx, void (*fn)(dmc_context*, void*), void* arg) {
     }
 }
 
-struct ExtractArgs {
-    const dmc_grid* grid;
-    const void* field;
-    int field_type;
-    double iso;
-    uint32_t flags;
-    const double* grads;
-    dmc_mesh* out;
-};
-
-void do_extract(dmc_context* ctx, void* a_) {
-    ExtractArgs& a = *static_cast<ExtractArgs*>(a_);
-    const dmc_grid& g = *a.grid;
+void check_grid(dmc_context* ctx, const dmc_grid& g) {
     if (g.nx < 2 || g.ny < 2 || g.nz < 2) fail(ctx, DMC_ERR_INVALID_ARG, "grid needs at least 2 points per axis");
     if (g.nx > (1 << 24)) fail(ctx, DMC_ERR_INVALID_ARG, "nx above 16777216");
-    const uint64_t npts = (uint64_t)g.nx * g.ny * g.nz;
     const uint64_t ncells = (uint64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
     if (ncells > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "more than 2^32 cells");
-    if (a.field_type != DMC_FIELD_F32 && a.field_type != DMC_FIELD_F64)
-        fail(ctx, DMC_ERR_INVALID_ARG, "unknown field type %d", a.field_type);
-    if (!a.field) fail(ctx, DMC_ERR_INVALID_ARG, "field is NULL");
-    if ((a.flags & DMC_GRADIENTS_SUPPLIED) && !a.grads) fail(ctx, DMC_ERR_INVALID_ARG, "gradients are NULL");
-    const size_t esz = a.field_type == DMC_FIELD_F32 ? 4 : 8;
-    ctx->timing = (a.flags & DMC_STAGE_TIMING) != 0;
-    ctx->have_mesh = false;
-    ctx->mesh = dmc_mesh{};
-    mark(ctx, "upload");
-    const void* dfield = a.field;
-    const double* dgrads = (a.flags & DMC_GRADIENTS_SUPPLIED) ? a.grads : nullptr;
-    if (!(a.flags & DMC_FIELD_ON_DEVICE)) {
-        void* f = ensure<uint8_t>(ctx, ctx->field, npts * esz);
-        HIPC(ctx, hipMemcpyAsync(f, a.field, npts * esz, hipMemcpyHostToDevice, ctx->stream));
-        dfield = f;
-        if (dgrads) {
-            double* gd = ensure<double>(ctx, ctx->grads, 3 * npts);
-            HIPC(ctx, hipMemcpyAsync(gd, a.grads, 3 * npts * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-            dgrads = gd;
-        }
-    }
-    if (a.field_type == DMC_FIELD_F32)
-        run_cells<float>(ctx, g, static_cast<const float*>(dfield), smallest_float_not_below(a.iso), a.iso, dgrads);
-    else
-        run_cells<double>(ctx, g, static_cast<const double*>(dfield), a.iso, a.iso, dgrads);
-    if (a.flags & DMC_SIMPLIFY_3X3Y_COLOR) {
+}
+
+// Everything after the upload, for one volume or a batch.
+void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint32_t flags) {
+    const int n = (int)hv.size();
+    if (field_type == DMC_FIELD_F32) run_cells<float>(ctx, hv);
+    else run_cells<double>(ctx, hv);
+    if (flags & DMC_SIMPLIFY_3X3Y_COLOR) {
         mark(ctx, "p3x3y_color");
         simplify3x3y(ctx, true);
     }
-    if (a.flags & DMC_SIMPLIFY_3X3Y_ISOLATED) {
+    if (flags & DMC_SIMPLIFY_3X3Y_ISOLATED) {
         mark(ctx, "p3x3y_isolated");
         simplify3x3y(ctx, false);
     }
-    if (a.flags & DMC_SIMPLIFY_3333) {
+    if (flags & DMC_SIMPLIFY_3333) {
         mark(ctx, "p3333");
         simplify3333(ctx);
     }
@@ -572,20 +615,41 @@ void do_extract(dmc_context* ctx, void* a_) {
     ctx->mesh.invalid_refs = (int64_t)hs.bad;
     ctx->mesh.nbnd = (int64_t)ctx->bnd.size();
     ctx->mesh.nt = 0;
-    if (a.flags & DMC_TRIANGLES) {
+    const int64_t nq = ctx->mesh.nq, nv = ctx->mesh.nv;
+    int32_t* tris = nullptr;
+    if (flags & DMC_TRIANGLES) {
         mark(ctx, "triangles");
-        const int64_t nq = ctx->mesh.nq;
-        int32_t* tris = ensure<int32_t>(ctx, ctx->tris, 6 * nq);
+        tris = ensure<int32_t>(ctx, ctx->tris, 6 * nq);
         if (nq > 0)
             hipLaunchKernelGGL(k_triangles, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream,
-                               static_cast<int32_t*>(ctx->quads[ctx->cur].p), nq, ctx->mesh.nv,
+                               static_cast<int32_t*>(ctx->quads[ctx->cur].p), nq, nv,
                                static_cast<double*>(ctx->pos[ctx->cur].p), tris);
         check_launch(ctx, "k_triangles");
         ctx->mesh.nt = 2 * nq;
         ctx->mesh.d_tris = tris;
     }
+    int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
+    int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    int32_t* hev = static_cast<int32_t*>(ctx->hev.p);
+    if (n > 1 && (nq > 0 || nv > 0)) {
+        // each volume's ids start at 0
+        hipLaunchKernelGGL(k_rebase, dim3(grid_for(std::max(nq, nv))), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const int64_t*>(ctx->vbound.p), static_cast<const int64_t*>(ctx->qbound.p), n,
+                           quads, twin, tris, nq, hev, nv);
+        check_launch(ctx, "k_rebase");
+    }
     mark(ctx, "end");
+    // per-volume bookkeeping back to the host
+    std::vector<int64_t> vb(2 * (n + 1)), qb(2 * (n + 1)), ab(n + 1), vhe(n > 1 ? 2 * n : 0);
+    std::vector<VolStat> vs(n);
     uint32_t diag[6];
+    HIPC(ctx, hipMemcpyAsync(vb.data(), ctx->vbound.p, vb.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync(qb.data(), ctx->qbound.p, qb.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync(ab.data(), vol_ptr(ctx, 2), ab.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync(vs.data(), ctx->vstat.p, vs.size() * sizeof(VolStat), hipMemcpyDeviceToHost, ctx->stream));
+    if (n > 1)
+        HIPC(ctx, hipMemcpyAsync(vhe.data(), ctx->vhe.p, vhe.size() * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipMemcpyAsync(diag, ctx->fails.p, sizeof(diag), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
     ctx->mesh.valence_zero = diag[4];
@@ -593,12 +657,117 @@ void do_extract(dmc_context* ctx, void* a_) {
     finish_timing(ctx);
     ctx->mesh.d_positions = static_cast<double*>(ctx->pos[ctx->cur].p);
     ctx->mesh.d_normals = static_cast<double*>(ctx->nrm[ctx->cur].p);
-    ctx->mesh.d_quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
-    ctx->mesh.d_twins = static_cast<int32_t*>(ctx->twin.p);
+    ctx->mesh.d_quads = quads;
+    ctx->mesh.d_twins = twin;
     ctx->mesh.d_he_flags = static_cast<uint8_t*>(ctx->flag.p);
-    ctx->mesh.d_he_v = static_cast<int32_t*>(ctx->hev.p);
+    ctx->mesh.d_he_v = hev;
+    uint64_t f1 = 0, f2 = 0, nc = 0;
+    for (const VolStat& x : vs) {
+        f1 += x.fail1;
+        f2 += x.fail2;
+        nc += x.contours;
+    }
+    ctx->mesh.failed_projections_l1 = (uint32_t)f1;
+    ctx->mesh.failed_projections_l2 = (uint32_t)f2;
+    ctx->mesh.nv_emitted = (int64_t)nc;
+    // one view per volume
+    ctx->vmesh.assign(n, dmc_mesh{});
+    ctx->vbnd.assign(n, std::vector<int32_t>());
+    if (n == 1) {
+        ctx->vmesh[0] = ctx->mesh;
+        ctx->vbnd[0] = ctx->bnd;
+    } else {
+        for (size_t x = 0; x + 1 < ctx->bnd.size(); x += 2) {
+            const int64_t a = ctx->bnd[x];
+            const int v = (int)(std::upper_bound(vb.begin(), vb.begin() + n, a) - vb.begin()) - 1;
+            ctx->vbnd[v].push_back((int32_t)(ctx->bnd[x] - vb[v]));
+            ctx->vbnd[v].push_back((int32_t)(ctx->bnd[x + 1] - vb[v]));
+        }
+        for (int v = 0; v < n; v++) {
+            dmc_mesh& m = ctx->vmesh[v];
+            m.nv = vb[v + 1] - vb[v];
+            m.nq = qb[v + 1] - qb[v];
+            m.nt = (flags & DMC_TRIANGLES) ? 2 * m.nq : 0;
+            m.d_positions = ctx->mesh.d_positions + 3 * vb[v];
+            m.d_normals = ctx->mesh.d_normals + 3 * vb[v];
+            m.d_quads = quads + 4 * qb[v];
+            m.d_tris = tris ? tris + 6 * qb[v] : nullptr;
+            m.d_twins = twin + 4 * qb[v];
+            m.d_he_flags = ctx->mesh.d_he_flags + 4 * qb[v];
+            m.d_he_v = hev + vb[v];
+            m.nonmanifold_edges = (int64_t)vhe[2 * v];
+            m.nbnd = (int64_t)ctx->vbnd[v].size();
+            m.edge_errors = (int64_t)vhe[2 * v + 1];
+            m.invalid_refs = ctx->mesh.invalid_refs;   // batch-wide
+            m.valence_zero = ctx->mesh.valence_zero;   // batch-wide
+            m.failed_projections_l1 = vs[v].fail1;
+            m.failed_projections_l2 = vs[v].fail2;
+            m.active_cells = ab[v + 1] - ab[v];
+            m.nv_emitted = (int64_t)vs[v].contours;
+            m.nv_cells = vb[n + 1 + v + 1] - vb[n + 1 + v];
+            m.nq_cells = qb[n + 1 + v + 1] - qb[n + 1 + v];
+        }
+    }
     ctx->have_mesh = true;
-    if (a.out) *a.out = ctx->mesh;
+}
+
+struct ExtractArgs {
+    const dmc_volume* vols;
+    int32_t nvol;
+    int field_type;
+    uint32_t flags;
+    dmc_mesh* out;
+};
+
+void do_extract(dmc_context* ctx, void* a_) {
+    ExtractArgs& a = *static_cast<ExtractArgs*>(a_);
+    if (a.nvol < 1) fail(ctx, DMC_ERR_INVALID_ARG, "no volumes");
+    if (a.field_type != DMC_FIELD_F32 && a.field_type != DMC_FIELD_F64)
+        fail(ctx, DMC_ERR_INVALID_ARG, "unknown field type %d", a.field_type);
+    const bool supplied = (a.flags & DMC_GRADIENTS_SUPPLIED) != 0;
+    const size_t esz = a.field_type == DMC_FIELD_F32 ? 4 : 8;
+    // layout of host fields packed into one device buffer (256-B aligned)
+    std::vector<size_t> foff(a.nvol), goff(a.nvol);
+    size_t fbytes = 0, gbytes = 0;
+    for (int v = 0; v < a.nvol; v++) {
+        const dmc_volume& x = a.vols[v];
+        check_grid(ctx, x.grid);
+        if (!x.field) fail(ctx, DMC_ERR_INVALID_ARG, "field of volume %d is NULL", v);
+        if (supplied && !x.gradients) fail(ctx, DMC_ERR_INVALID_ARG, "gradients of volume %d are NULL", v);
+        const size_t npts = (size_t)x.grid.nx * x.grid.ny * x.grid.nz;
+        foff[v] = fbytes;
+        fbytes += (npts * esz + 255) & ~(size_t)255;
+        goff[v] = gbytes;
+        if (supplied) gbytes += (3 * npts * sizeof(double) + 255) & ~(size_t)255;
+    }
+    ctx->timing = (a.flags & DMC_STAGE_TIMING) != 0;
+    ctx->have_mesh = false;
+    ctx->mesh = dmc_mesh{};
+    mark(ctx, "upload");
+    std::vector<HostVol> hv(a.nvol);
+    uint8_t* fdev = nullptr;
+    uint8_t* gdev = nullptr;
+    if (!(a.flags & DMC_FIELD_ON_DEVICE)) {
+        fdev = ensure<uint8_t>(ctx, ctx->field, fbytes);
+        if (supplied) gdev = ensure<uint8_t>(ctx, ctx->grads, gbytes);
+    }
+    for (int v = 0; v < a.nvol; v++) {
+        const dmc_volume& x = a.vols[v];
+        const size_t npts = (size_t)x.grid.nx * x.grid.ny * x.grid.nz;
+        hv[v] = HostVol{x.grid, x.field, supplied ? x.gradients : nullptr, x.iso};
+        if (fdev) {
+            HIPC(ctx, hipMemcpyAsync(fdev + foff[v], x.field, npts * esz, hipMemcpyHostToDevice, ctx->stream));
+            hv[v].field = fdev + foff[v];
+            if (supplied) {
+                HIPC(ctx, hipMemcpyAsync(gdev + goff[v], x.gradients, 3 * npts * sizeof(double),
+                                         hipMemcpyHostToDevice, ctx->stream));
+                hv[v].grads = reinterpret_cast<const double*>(gdev + goff[v]);
+            }
+        }
+    }
+    run(ctx, hv, a.field_type, a.flags);
+    if (a.out)
+        for (int v = 0; v < a.nvol; v++) a.out[v] = ctx->vmesh[v];
 }
 
 }  // namespace
@@ -663,19 +832,40 @@ dmc_status dmc_destroy(dmc_context* ctx) {
 dmc_status dmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type, double iso,
                        uint32_t flags, const double* gradients, dmc_mesh* out) {
     if (!ctx || !grid) return DMC_ERR_INVALID_ARG;
-    ExtractArgs a{grid, field, field_type, iso, flags, gradients, out};
+    const dmc_volume vol{*grid, field, gradients, iso};
+    ExtractArgs a{&vol, 1, field_type, flags, out};
+    return guard(ctx, do_extract, &a);
+}
+
+dmc_status dmc_extract_batch(dmc_context* ctx, const dmc_volume* vols, int32_t nvol, int field_type, uint32_t flags,
+                             dmc_mesh* out) {
+    if (!ctx || !vols || nvol < 1) return DMC_ERR_INVALID_ARG;
+    ExtractArgs a{vols, nvol, field_type, flags, out};
     return guard(ctx, do_extract, &a);
 }
 
 struct DownloadArgs {
+    int32_t index;   // volume of the last extraction; -1: the only one
     double *pos, *nrm;
     int32_t *quads, *tris, *he, *he_v, *he_f, *bnd;
 };
 
+static int32_t volume_index(dmc_context* ctx, int32_t index) {
+    if (!ctx->have_mesh) fail(ctx, DMC_ERR_STATE, "no mesh: call dmc_extract first");
+    if (index < 0) {
+        if (ctx->vmesh.size() != 1)
+            fail(ctx, DMC_ERR_STATE, "the last extraction was a batch: use dmc_batch_download");
+        return 0;
+    }
+    if (index >= (int32_t)ctx->vmesh.size()) fail(ctx, DMC_ERR_INVALID_ARG, "volume index %d out of range", index);
+    return index;
+}
+
 static void do_download(dmc_context* ctx, void* a_) {
     DownloadArgs& a = *static_cast<DownloadArgs*>(a_);
-    if (!ctx->have_mesh) fail(ctx, DMC_ERR_STATE, "no mesh: call dmc_extract first");
-    const dmc_mesh& m = ctx->mesh;
+    const int32_t vi = volume_index(ctx, a.index);
+    const dmc_mesh& m = ctx->vmesh[vi];
+    const std::vector<int32_t>& bnd = ctx->vbnd[vi];
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (dst && bytes) HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     };
@@ -707,12 +897,19 @@ static void do_download(dmc_context* ctx, void* a_) {
     }
     if (a.he_f)
         for (int64_t f = 0; f < m.nq; f++) a.he_f[f] = (int32_t)(4 * f);
-    if (a.bnd && !ctx->bnd.empty()) memcpy(a.bnd, ctx->bnd.data(), ctx->bnd.size() * sizeof(int32_t));
+    if (a.bnd && !bnd.empty()) memcpy(a.bnd, bnd.data(), bnd.size() * sizeof(int32_t));
 }
 
 dmc_status dmc_mesh_download(dmc_context* ctx, double* positions, double* normals, int32_t* quads, int32_t* tris,
                              int32_t* halfedges, int32_t* he_v, int32_t* he_f, int32_t* bnd) {
-    DownloadArgs a{positions, normals, quads, tris, halfedges, he_v, he_f, bnd};
+    DownloadArgs a{-1, positions, normals, quads, tris, halfedges, he_v, he_f, bnd};
+    return guard(ctx, do_download, &a);
+}
+
+dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions, double* normals, int32_t* quads,
+                              int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f, int32_t* bnd) {
+    if (index < 0) return DMC_ERR_INVALID_ARG;
+    DownloadArgs a{index, positions, normals, quads, tris, halfedges, he_v, he_f, bnd};
     return guard(ctx, do_download, &a);
 }
 
@@ -723,15 +920,15 @@ struct F32Args {
 
 static void do_download_f32(dmc_context* ctx, void* a_) {
     F32Args& a = *static_cast<F32Args*>(a_);
-    if (!ctx->have_mesh) fail(ctx, DMC_ERR_STATE, "no mesh: call dmc_extract first");
-    const int64_t n = 3 * ctx->mesh.nv;
+    const dmc_mesh& m = ctx->vmesh[volume_index(ctx, -1)];
+    const int64_t n = 3 * m.nv;
     std::vector<double> tmp(n);
     if (a.pos) {
-        HIPC(ctx, hipMemcpy(tmp.data(), ctx->mesh.d_positions, n * sizeof(double), hipMemcpyDeviceToHost));
+        HIPC(ctx, hipMemcpy(tmp.data(), m.d_positions, n * sizeof(double), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; i++) a.pos[i] = (float)tmp[i];
     }
     if (a.nrm) {
-        HIPC(ctx, hipMemcpy(tmp.data(), ctx->mesh.d_normals, n * sizeof(double), hipMemcpyDeviceToHost));
+        HIPC(ctx, hipMemcpy(tmp.data(), m.d_normals, n * sizeof(double), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; i++) a.nrm[i] = a.negate ? -(float)tmp[i] : (float)tmp[i];
     }
 }

@@ -27,6 +27,42 @@ struct Dims {
     int64_t R;      // rows (ny-1)*(nz-1)
 };
 
+struct GridGeom {
+    double x0, y0, z0, dx, dy, dz;
+};
+
+// ---------------------------------------------------------------------------
+// Volume table.  One extraction processes one or many independent volumes
+// (a batch) in the same launches: the cell stage looks up the volume of each
+// sweep wave, cell row and active cell; the mesh stage works on the union
+// mesh, whose vertices and quadrilaterals are the volumes' in volume order.
+// ---------------------------------------------------------------------------
+struct VolDesc {
+    const void* field;     // T* (device)
+    const double* grads;   // 3 doubles per point, or null (gradients from the field)
+    Dims d;
+    GridGeom geo;
+    double iso;            // iso-value as the reference uses it (double)
+    double iso_t;          // comparison threshold, exact in the field type
+    int S, ntj, K, ntk;    // sweep tiling: segments per row, row tiles, planes per wave, plane tiles
+    int64_t rowBase;       // first global cell row
+    int64_t rsBase;        // first global (row, segment)
+};
+
+struct VolStat {
+    unsigned int fail1, fail2;      // failed projections, levels 1 and 2
+    unsigned long long contours;    // every contour (vertices before removeUnusedVertices)
+};
+
+struct VolTable {
+    const VolDesc* desc;
+    const int64_t* waveBase;   // n+1: first sweep wave of each volume
+    const int64_t* rowBase;    // n+1: first global cell row
+    const int64_t* aBase;      // n+1: first active cell (set after the sweep's scan)
+    int n;
+};
+
+
 // ---------------------------------------------------------------------------
 // K1: the field sweep.  A wave owns one x-segment of 252 cells for J
 // consecutive cell rows and marches over K planes, keeping only the
@@ -74,15 +110,23 @@ __device__ __forceinline__ uint32_t row_bits(const T v[4], T edge, T iso_t, int 
 }
 
 template <class T, bool VEC>
-__global__ void __launch_bounds__(kBlock) k_sweep(const T* __restrict__ field, Dims d, T iso_t, int S, int ntj,
-                                                   int K, int ntk, uint32_t* __restrict__ lists,
+__global__ void __launch_bounds__(kBlock) k_sweep(VolTable vt, int64_t waves, uint32_t* __restrict__ lists,
                                                    uint32_t* __restrict__ segCount) {
     constexpr int J = kSweepRows;
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform from here on (kept in scalar registers)
-    const int64_t wave =
+    const int64_t gwave =
         xcd_block() * (kBlock / kWave) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    if (wave >= (int64_t)S * ntj * ntk) return;
+    if (gwave >= waves) return;
+    const int v = vol_of(vt.waveBase, vt.n, gwave);
+    const VolDesc& V = vt.desc[v];
+    const T* __restrict__ field = static_cast<const T*>(V.field);
+    const Dims d = V.d;
+    const T iso_t = (T)V.iso_t;
+    const int S = V.S, ntj = V.ntj, K = V.K;
+    const int64_t wave = gwave - vt.waveBase[v];
+    lists += V.rsBase * kSeg;
+    segCount += V.rsBase;
     const int s = (int)(wave % S);
     const int64_t rest = wave / S;
     const int tj = (int)(rest % ntj), tk = (int)(rest / ntj);
@@ -197,24 +241,38 @@ __device__ __forceinline__ unsigned used_contours(unsigned icase, uint32_t map, 
 // contour for the reference's pre-removal vertex count.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_rows(const T* __restrict__ field, Dims d, double iso, int S,
-                                                  const uint32_t* __restrict__ lists,
+__global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, const uint32_t* __restrict__ lists,
                                                   const uint32_t* __restrict__ segCount,
                                                   const uint32_t* __restrict__ segBase,
                                                   uint32_t* __restrict__ acell, uint32_t* __restrict__ acode,
                                                   uint32_t* __restrict__ avloc, uint32_t* __restrict__ aqloc,
                                                   uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowQ,
-                                                  unsigned long long* __restrict__ ncontours) {
+                                                  VolStat* __restrict__ vstat) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wpb = kBlock / kWave;
     unsigned long long total = 0;
-    for (int64_t r = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; r < d.R; r += (int64_t)gridDim.x * wpb) {
+    int cur = -1;  // volume whose contour count `total` holds (wave-uniform)
+    for (int64_t g = (int64_t)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave); g < Rtot;
+         g += (int64_t)gridDim.x * wpb) {
+        const int v = vol_of(vt.rowBase, vt.n, g);
+        if (v != cur) {
+            total = wave_sum(total);
+            if (lane == 0 && total) atomicAdd(&vstat[cur].contours, total);
+            total = 0;
+            cur = v;
+        }
+        const VolDesc& V = vt.desc[v];
+        const T* __restrict__ field = static_cast<const T*>(V.field);
+        const Dims d = V.d;
+        const double iso = V.iso;
+        const int S = V.S;
+        const int64_t r = g - V.rowBase;
         const int j = (int)(r % (d.ny - 1));
         const int k = (int)(r / (d.ny - 1));
         uint32_t cv = 0, cq = 0;
         // the row's active cells are contiguous in the flat order: process
         // them 64 at a time across segment boundaries
-        const int64_t rs0 = r * S;
+        const int64_t rs0 = V.rsBase + r * S;
         const uint32_t first = segBase[rs0];
         const uint32_t last = segBase[rs0 + S - 1] + segCount[rs0 + S - 1];
         for (uint32_t a0 = first; a0 < last; a0 += kWave) {
@@ -267,12 +325,12 @@ __global__ void __launch_bounds__(kBlock) k_rows(const T* __restrict__ field, Di
             cq += (uint32_t)__shfl((int)iq, kWave - 1, kWave);
         }
         if (lane == 0) {
-            rowV[r] = cv;
-            rowQ[r] = cq;
+            rowV[g] = cv;
+            rowQ[g] = cq;
         }
     }
     total = wave_sum(total);
-    if (lane == 0 && total) atomicAdd(ncontours, total);
+    if (lane == 0 && total) atomicAdd(&vstat[cur].contours, total);
 }
 
 // Gradient of UniformGrid::estimateGradient followed by flip_gradient
@@ -303,27 +361,27 @@ __device__ __forceinline__ void point_gradient(const T* __restrict__ field, cons
 // rowBaseV[row] + avloc[a] on.  Failed projections are counted for every
 // contour, as the reference does.
 // ---------------------------------------------------------------------------
-struct GridGeom {
-    double x0, y0, z0, dx, dy, dz;
-};
-
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_vertices(const T* __restrict__ field, const double* __restrict__ grads,
-                                                      Dims d, GridGeom geo, double iso,
-                                                      const uint32_t* __restrict__ acell,
+__global__ void __launch_bounds__(kBlock) k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
                                                       const uint32_t* __restrict__ avloc,
                                                       const uint32_t* __restrict__ rowBaseV, int64_t A,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
-                                                      uint32_t* __restrict__ fails) {
+                                                      VolStat* __restrict__ vstat) {
     static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
     __shared__ float boxes[kBoxFloats * kBoxStride];
     __shared__ double corners[8 * kCornerStride];
     const BoxRef B{boxes + threadIdx.x};
     const CornerRef Fc{corners + threadIdx.x};
-    uint32_t f1 = 0, f2 = 0;
-    const double dx2 = 2 * geo.dx, dy2 = 2 * geo.dy, dz2 = 2 * geo.dz;
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
+        const int vol = vol_of(vt.aBase, vt.n, a);
+        const VolDesc& V = vt.desc[vol];
+        const T* __restrict__ field = static_cast<const T*>(V.field);
+        const double* __restrict__ grads = V.grads;
+        const Dims d = V.d;
+        const GridGeom geo = V.geo;
+        const double iso = V.iso;
+        const double dx2 = 2 * geo.dx, dy2 = 2 * geo.dy, dz2 = 2 * geo.dz;
         int i, j, k;
         uint32_t r;
         cell_ijk(acell[a], d, i, j, k, r);
@@ -336,12 +394,13 @@ __global__ void __launch_bounds__(kBlock) k_vertices(const T* __restrict__ field
         const int cnt = poly_count(w);
         cell_boxes(Fc, iso, w, cnt, B);
         const unsigned used = used_contours(icase, code >> 8, i, j, k, d);
-        uint32_t out = rowBaseV[r] + avloc[a];
+        uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
         for (int t = 0; t < cnt; t++) {
             double p[3];
             const int fl = contour_representative(F, Fc, iso, w, cnt, t, B, p);
-            f1 += fl >= 1;
-            f2 += fl >= 2;
+            // rare: counted per volume where they happen
+            if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
+            if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
             if (!((used >> t) & 1u)) continue;
             // normal: trilinear blend of the corner gradients, normalised
             const double u = p[0], v = p[1], ww = p[2];
@@ -392,12 +451,6 @@ __global__ void __launch_bounds__(kBlock) k_vertices(const T* __restrict__ field
             out++;
         }
     }
-    f1 = wave_sum(f1);
-    f2 = wave_sum(f2);
-    if ((threadIdx.x & (kWave - 1)) == 0 && (f1 | f2)) {
-        atomicAdd(&fails[0], f1);
-        atomicAdd(&fails[1], f2);
-    }
 }
 
 // Position of cell (i, j, k) in the active arrays: binary search in the
@@ -408,10 +461,11 @@ struct SegLists {
     const uint32_t* count;
     const uint32_t* base;
     int S;
+    int64_t rsBase;   // first (row, segment) of the volume
 };
 
 __device__ __forceinline__ int64_t find_active(const SegLists& L, const Dims& d, int i, int j, int k) {
-    const int64_t rs = ((int64_t)k * (d.ny - 1) + j) * L.S + (i / kSeg);
+    const int64_t rs = L.rsBase + ((int64_t)k * (d.ny - 1) + j) * L.S + (i / kSeg);
     const uint32_t* list = L.lists + rs * kSeg;
     const uint32_t n = L.count[rs];
     uint32_t lo = 0, hi = n;
@@ -425,7 +479,7 @@ __device__ __forceinline__ int64_t find_active(const SegLists& L, const Dims& d,
 }
 
 // Final id of the vertex of active cell `a` = (i, j, k) whose contour
-// contains edge e.
+// contains edge e.  rowBaseV points at the volume's first row.
 __device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode, const uint32_t* __restrict__ avloc,
                                              const uint32_t* __restrict__ rowBaseV, int64_t a, int e, int i, int j,
                                              int k, const Dims& d) {
@@ -446,22 +500,30 @@ __device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode,
 // (cpp_dmc/DualMarchingCubes.h:215-223, :663-676).  Colour: 3*parity + dir
 // (cpp_dmc/DualMarchingCubes.cpp:180-183).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_quads(Dims d, const uint32_t* __restrict__ acell,
+__global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* __restrict__ acell,
                                                    const uint32_t* __restrict__ acode,
                                                    const uint32_t* __restrict__ avloc,
                                                    const uint32_t* __restrict__ aqloc,
-                                                   const uint32_t* __restrict__ rowBaseV,
-                                                   const uint32_t* __restrict__ rowBaseQ, int64_t A, SegLists L,
+                                                   const uint32_t* __restrict__ rowBaseV0,
+                                                   const uint32_t* __restrict__ rowBaseQ, int64_t A,
+                                                   const uint32_t* __restrict__ lists,
+                                                   const uint32_t* __restrict__ segCount,
+                                                   const uint32_t* __restrict__ segBase,
                                                    int32_t* __restrict__ quads, int32_t* __restrict__ colors,
                                                    uint32_t* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
+        const int vol = vol_of(vt.aBase, vt.n, a);
+        const VolDesc& V = vt.desc[vol];
+        const Dims d = V.d;
+        const SegLists L{lists, segCount, segBase, V.S, V.rsBase};
+        const uint32_t* __restrict__ rowBaseV = rowBaseV0 + V.rowBase;
         int i, j, k;
         uint32_t r;
         cell_ijk(acell[a], d, i, j, k, r);
         const unsigned icase = acode[a] & 0xFFu;
         const int bit = (icase & 1u) ? 0 : 1;     // field at the edge origin below iso
         const int parity = (i & 1) | (j & 1) << 1 | (k & 1) << 2;
-        int64_t q = (int64_t)rowBaseQ[r] + aqloc[a];
+        int64_t q = (int64_t)rowBaseQ[V.rowBase + r] + aqloc[a];
         const bool ex = edge_cut(icase, 0) && j >= 1 && k >= 1;
         const bool ey = edge_cut(icase, 3) && i >= 1 && k >= 1;
         const bool ez = edge_cut(icase, 8) && i >= 1 && j >= 1;
@@ -508,6 +570,62 @@ __global__ void __launch_bounds__(kBlock) k_quads(Dims d, const uint32_t* __rest
             reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
             colors[q] = 3 * parity + 2;
             q++;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-volume bookkeeping of a batch: first active cell, vertex and
+// quadrilateral of each volume (n+1 entries, the last one the total), kept
+// up to date through order-preserving compactions, and the final rebase of
+// each volume's ids to start at 0.
+// ---------------------------------------------------------------------------
+__global__ void k_vol_abase(VolTable vt, const uint32_t* __restrict__ segBase, int64_t A, int64_t* __restrict__ aBase) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v <= vt.n; v += gridDim.x * blockDim.x)
+        aBase[v] = v < vt.n ? (int64_t)segBase[vt.desc[v].rsBase] : A;
+}
+
+__global__ void k_vol_bounds(VolTable vt, const uint32_t* __restrict__ rowBaseV, const uint32_t* __restrict__ rowBaseQ,
+                             int64_t V, int64_t Q, int64_t* __restrict__ vb, int64_t* __restrict__ qb) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v <= vt.n; v += gridDim.x * blockDim.x) {
+        vb[v] = v < vt.n ? (int64_t)rowBaseV[vt.desc[v].rowBase] : V;
+        qb[v] = v < vt.n ? (int64_t)rowBaseQ[vt.desc[v].rowBase] : Q;
+    }
+}
+
+// b[v] -> number of kept elements before it (newIdx = exclusive scan of the
+// keep flags over nOld elements).
+__global__ void k_vol_remap(int64_t* __restrict__ b, int n1, const uint32_t* __restrict__ newIdx, int64_t nOld,
+                            int64_t nNew) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n1; v += gridDim.x * blockDim.x) {
+        const int64_t x = b[v];
+        b[v] = x < nOld ? (int64_t)newIdx[x] : nNew;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_rebase(const int64_t* __restrict__ vb, const int64_t* __restrict__ qb,
+                                                    int n, int32_t* __restrict__ quads, int32_t* __restrict__ twin,
+                                                    int32_t* __restrict__ tris, int64_t nq,
+                                                    int32_t* __restrict__ hev, int64_t nv) {
+    const int64_t m = nq > nv ? nq : nv;
+    for (int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x; x < m; x += (int64_t)gridDim.x * kBlock) {
+        if (x < nq) {
+            const int vol = vol_of(qb, n, x);
+            const int32_t v0 = (int32_t)vb[vol], h0 = (int32_t)(4 * qb[vol]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                quads[4 * x + k] -= v0;
+                const int32_t t = twin[4 * x + k];
+                if (t >= 0) twin[4 * x + k] = t - h0;
+            }
+            if (tris)
+#pragma unroll
+                for (int k = 0; k < 6; k++) tris[6 * x + k] -= v0;
+        }
+        if (x < nv) {
+            const int vol = vol_of(vb, n, x);
+            const int32_t e = hev[x];
+            if (e >= 0) hev[x] = e - (int32_t)(4 * qb[vol]);
         }
     }
 }

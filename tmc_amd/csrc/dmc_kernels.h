@@ -20,6 +20,18 @@ __device__ __forceinline__ int64_t xcd_block() {
     return (b % kXcds) * (nb / kXcds) + b / kXcds;
 }
 
+// Largest v < n with base[v] <= x (base ascending, base[0] <= x): the
+// volume of item x when base holds each volume's first item.
+__device__ __forceinline__ int vol_of(const int64_t* __restrict__ base, int n, int64_t x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan(T x) {
     const int lane = threadIdx.x & (kWave - 1);

@@ -136,7 +136,9 @@ struct HeStats {
 __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
                                                       const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
-                                                      uint8_t* __restrict__ flag, HeStats* __restrict__ st) {
+                                                      uint8_t* __restrict__ flag, HeStats* __restrict__ st,
+                                                      const int64_t* __restrict__ qb, int nvol,
+                                                      unsigned long long* __restrict__ vst) {
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         int32_t g[5];
@@ -150,6 +152,11 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
             if (c == 3 || c == 4) atomicAdd(&st->nonmanifold, 1ull);
             if (c == 3) atomicAdd(&st->three, 1ull);
             if (c > 4) atomicAdd(&st->errors, 1ull);
+            // per volume of a batch (rare events): [2v] non-manifold, [2v+1] errors
+            if (vst && c >= 3) {
+                const int vol = vol_of(qb, nvol, h >> 2);
+                atomicAdd(&vst[2 * vol + (c > 4)], 1ull);
+            }
         }
     }
 }

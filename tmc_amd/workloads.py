@@ -7,7 +7,13 @@ Each generator returns (field float32 array (nz, ny, nx), Grid, iso):
                           on [0,1]^3, iso 0.5
 * ``schwarz(n, seed)`` -- cos x + cos y + cos z on [-8pi,8pi]^3, node-aligned
                           (x_i = -8pi + i*16pi/(n-1)), + N(0, 0.01^2), iso 0.3
-* ``batch_specs``      -- the mixed batch of small volumes
+* ``batch_specs``      -- the mixed batch of small volumes (4,096 volumes,
+                          16-96 points per axis, log-uniform)
+* ``serve_specs``      -- the serving mix (70% 32-128^3, 25% 256^3, 5% 512^3)
+
+Mixed volumes draw their generator uniformly from the three above, each on
+its own domain sampled at the volume's size, with iso U[-0.2, 0.2]
+(genus-2, Schwarz-P) or U[0.3, 0.7] (blobs).
 """
 import ctypes as C
 import math
@@ -88,3 +94,55 @@ def uniform(seed, k):
 
 def normal(seed, idx):
     return _load().wl_normal(seed, idx)
+
+
+KINDS = ("genus2", "schwarz", "blobs")
+
+
+def _mixed(kind, shape, seed, iso):
+    if kind == "genus2":
+        f, g, _ = genus2(shape)
+    elif kind == "schwarz":
+        f, g, _ = schwarz(shape, seed=seed)
+    else:
+        f, g, _ = blobs(shape, seed=seed)
+    return f, g, iso
+
+
+def _spec(seed, i, shape):
+    """Generator, iso and noise seed of mixed volume i: counter-based
+    uniforms u_k = uniform(seed, 16 i + k)."""
+    u = [uniform(seed, 16 * i + k) for k in range(6)]
+    kind = KINDS[min(int(u[3] * 3), 2)]
+    iso = 0.3 + 0.4 * u[4] if kind == "blobs" else -0.2 + 0.4 * u[4]
+    return {"kind": kind, "shape": shape, "iso": iso, "seed": (seed * 1000003 + i) & 0xFFFFFFFFFFFF}
+
+
+def batch_specs(count=4096, seed=4):
+    """The batched workload: each axis round(exp(U[ln 16, ln 96]))."""
+    lo, hi = math.log(16), math.log(96)
+    out = []
+    for i in range(count):
+        shape = tuple(int(round(math.exp(lo + (hi - lo) * uniform(seed, 16 * i + k)))) for k in (2, 1, 0))
+        out.append(_spec(seed, i, shape))
+    return out
+
+
+def serve_specs(count, seed=5):
+    """The serving mix: 70% U{32..128}^3, 25% 256^3, 5% 512^3."""
+    out = []
+    for i in range(count):
+        u = uniform(seed, 16 * i + 7)
+        if u < 0.70:
+            n = 32 + min(int(uniform(seed, 16 * i + 8) * 97), 96)
+        elif u < 0.95:
+            n = 256
+        else:
+            n = 512
+        out.append(_spec(seed, i, (n, n, n)))
+    return out
+
+
+def make(spec):
+    """(field, grid, iso) of one mixed-volume spec."""
+    return _mixed(spec["kind"], spec["shape"], spec["seed"], spec["iso"])
