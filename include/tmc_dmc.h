@@ -143,6 +143,52 @@ dmc_status dmc_mesh_download(dmc_context* ctx, double* positions, double* normal
 dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions, double* normals, int32_t* quads,
                               int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f, int32_t* bnd);
 
+/* ------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, which extracts one grid per
+ * synchronous dualMC call).  A server owns one context and a worker thread:
+ * waiting requests are admitted oldest first while their estimated working
+ * set plus the results still held fits budget_bytes (and at most max_batch
+ * per pass), run as one dmc_extract_batch, and completed together.  Results
+ * stay in device memory (pooled blocks) until dmc_server_release.  With
+ * max_batch = 1 the same loop serves one request at a time.
+ * ---------------------------------------------------------------------- */
+typedef struct dmc_server dmc_server;
+
+typedef struct {
+    int device;
+    int field_type;          /* DMC_FIELD_F32 / DMC_FIELD_F64, for every request */
+    uint32_t flags;          /* extraction flags for every request (DMC_FIELD_ON_DEVICE: device fields) */
+    int32_t max_batch;       /* volumes per pass, >= 1 */
+    size_t budget_bytes;     /* admission budget; 0 = 96 GiB */
+} dmc_server_opts;
+
+typedef struct {
+    dmc_mesh mesh;           /* counts and device pointers (valid until release) */
+    double t_submit;         /* seconds since dmc_server_create */
+    double t_start;          /* its batch started */
+    double t_done;           /* its batch completed */
+    int32_t batch_size;      /* volumes in that batch */
+    dmc_status status;
+} dmc_result;
+
+dmc_status dmc_server_create(dmc_server** server, const dmc_server_opts* opts);
+/* Finishes the queued requests, then frees everything. */
+dmc_status dmc_server_destroy(dmc_server* server);
+/* Queue one volume; its field must stay valid until the request is done. */
+dmc_status dmc_server_submit(dmc_server* server, const dmc_volume* vol, uint64_t* ticket);
+dmc_status dmc_server_poll(dmc_server* server, uint64_t ticket, int* done);
+/* Block until the request is done; returns its status. */
+dmc_status dmc_server_wait(dmc_server* server, uint64_t ticket, dmc_result* out);
+/* As dmc_mesh_download, for a completed request. */
+dmc_status dmc_server_download(dmc_server* server, uint64_t ticket, double* positions, double* normals,
+                               int32_t* quads, int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f,
+                               int32_t* bnd);
+/* Free the request and its share of the result memory. */
+dmc_status dmc_server_release(dmc_server* server, uint64_t ticket);
+dmc_status dmc_server_stats(dmc_server* server, uint64_t* batches, uint64_t* served, size_t* held_bytes,
+                            size_t* pool_bytes);
+const char* dmc_server_last_error(dmc_server* server);
+
 /* FP32 export as the CUDA reference returns it (dmc/DualMarchingCubes.cu:
  * 1076-1090): positions and normals as float triples; when negate_normals
  * is non-zero the normals are negated like the CUDA path does. */

@@ -325,3 +325,85 @@ def test_batch_device_fields(engine):
         for k in ("v", "n", "quads", "he", "bnd"):
             assert_same(got[k], exp[i][k], f"device volume {i}:{k}")
     buf.free()
+
+
+# ---------------------------------------------------------------------------
+# serving loop: results identical alone and in the mix, whatever the batch
+# the server formed; admission under a budget
+# ---------------------------------------------------------------------------
+def _server_results(srv, vols, threads=1):
+    import threading
+    tickets = [None] * len(vols)
+
+    def sub(idx):
+        for i in idx:
+            f, iso, g = vols[i]
+            tickets[i] = srv.submit(f, iso, g)
+
+    parts = [list(range(t, len(vols), threads)) for t in range(threads)]
+    ts = [threading.Thread(target=sub, args=(p,)) for p in parts]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    out = []
+    for t in tickets:
+        info = srv.wait(t)
+        got = srv.download(t)
+        got["info"] = info
+        srv.release(t)
+        out.append(got)
+    return out
+
+
+@pytest.mark.parametrize("max_batch,threads", [(1, 1), (1 << 20, 1), (1 << 20, 4), (3, 2)])
+def test_server_matches_alone(engine, max_batch, threads):
+    flags = SIMPLIFY_ALL | TRIANGLES
+    vols = _batch_volumes(53, 12)
+    _say(f"[gpu] server max_batch={max_batch} threads={threads}, {len(vols)} volumes")
+    exp = [_alone(engine, f, iso, g, flags) for f, iso, g in vols]
+    srv = tmc_amd.Server(0, max_batch=max_batch, flags=flags)
+    try:
+        got = _server_results(srv, vols, threads)
+        for i, (es, e) in enumerate(exp):
+            info = got[i]["info"]
+            assert 1 <= info["batch_size"] <= max_batch
+            assert info["t_submit"] <= info["t_start"] <= info["t_done"]
+            for k in SUMMARY_KEYS:
+                assert info[k] == es[k], f"request {i}: {k}"
+            for k in ("v", "n", "quads", "tris", "he", "he_v", "he_f", "bnd"):
+                assert_same(got[i][k], e[k], f"request {i}:{k}")
+        st = srv.stats()
+        assert st["served"] == len(vols) and st["held_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def test_server_budget_admission(engine):
+    """A small budget splits the waiting requests into several batches and a
+    request larger than the budget fails with DMC_ERR_BUDGET; the others
+    are unaffected."""
+    flags = SIMPLIFY_ALL | TRIANGLES
+    rng = np.random.default_rng(5)
+    vols = [(rng.standard_normal((10, 10, 10)).astype(np.float32), 0.2, tmc_amd.Grid.make((10, 10, 10)))
+            for _ in range(6)]
+    big = (rng.standard_normal((40, 40, 40)).astype(np.float32), 0.2, tmc_amd.Grid.make((40, 40, 40)))
+    exp = [_alone(engine, f, iso, g, flags)[1] for f, iso, g in vols]
+    # one 10^3 volume: 81 rows x 1 segment x 1032 B + 40 B x 1000 points + 4,000 B of field = 127,592 B
+    srv = tmc_amd.Server(0, budget_bytes=300_000, flags=flags)
+    try:
+        tk = [srv.submit(f, iso, g) for f, iso, g in vols[:3]]
+        tb = srv.submit(*big)
+        tk += [srv.submit(f, iso, g) for f, iso, g in vols[3:]]
+        with pytest.raises(tmc_amd.DMCError, match="budget"):
+            srv.wait(tb)
+        srv.release(tb)
+        for i, t in enumerate(tk):
+            info = srv.wait(t)
+            assert info["batch_size"] <= 2, info["batch_size"]
+            got = srv.download(t)
+            for k in ("v", "quads", "he", "tris"):
+                assert_same(got[k], exp[i][k], f"request {i}:{k}")
+            srv.release(t)
+    finally:
+        srv.close()

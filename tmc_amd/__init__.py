@@ -20,7 +20,7 @@ import numpy as np
 
 from .build import ENGINE_SO
 
-__all__ = ["Engine", "DualMarchingCubes", "UniformGrid", "DMCError", "SIMPLIFY_3X3Y_COLOR",
+__all__ = ["Engine", "Server", "DualMarchingCubes", "UniformGrid", "DMCError", "SIMPLIFY_3X3Y_COLOR",
            "SIMPLIFY_3X3Y_ISOLATED", "SIMPLIFY_3333", "SIMPLIFY_ALL", "TRIANGLES", "GRADIENTS_SUPPLIED",
            "FIELD_ON_DEVICE", "STAGE_TIMING", "load_library"]
 
@@ -38,8 +38,11 @@ FIELD_F64 = 1
 
 # every symbol include/tmc_dmc.h declares
 EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc_mesh_download",
-           "dmc_batch_download", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc", "dmc_device_free",
-           "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str", "dmc_last_error"]
+           "dmc_batch_download", "dmc_server_create", "dmc_server_destroy", "dmc_server_submit", "dmc_server_poll",
+           "dmc_server_wait", "dmc_server_download", "dmc_server_release", "dmc_server_stats",
+           "dmc_server_last_error", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc",
+           "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
+           "dmc_last_error"]
 
 
 class DMCError(RuntimeError):
@@ -78,6 +81,16 @@ class Mesh(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("d_")}
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("field_type", C.c_int), ("flags", C.c_uint32), ("max_batch", C.c_int32),
+                ("budget_bytes", C.c_size_t)]
+
+
+class Result(C.Structure):
+    _fields_ = [("mesh", Mesh), ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("batch_size", C.c_int32), ("status", C.c_int)]
+
+
 _lib = None
 
 
@@ -96,6 +109,18 @@ def load_library(path=ENGINE_SO):
     L.dmc_mesh_download.argtypes = [vp, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     L.dmc_extract_batch.argtypes = [vp, C.POINTER(Volume), C.c_int32, C.c_int, C.c_uint32, C.POINTER(Mesh)]
     L.dmc_batch_download.argtypes = [vp, C.c_int32, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
+    u64 = C.c_uint64
+    L.dmc_server_create.argtypes = [C.POINTER(vp), C.POINTER(ServerOpts)]
+    L.dmc_server_destroy.argtypes = [vp]
+    L.dmc_server_submit.argtypes = [vp, C.POINTER(Volume), C.POINTER(u64)]
+    L.dmc_server_poll.argtypes = [vp, u64, C.POINTER(C.c_int)]
+    L.dmc_server_wait.argtypes = [vp, u64, C.POINTER(Result)]
+    L.dmc_server_download.argtypes = [vp, u64, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
+    L.dmc_server_release.argtypes = [vp, u64]
+    L.dmc_server_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t)]
+    L.dmc_server_last_error.argtypes = [vp]
+    L.dmc_server_last_error.restype = C.c_char_p
     L.dmc_mesh_download_f32.argtypes = [vp, f32p, f32p, C.c_int]
     L.dmc_stage_times.argtypes = [vp, C.POINTER(C.c_char_p), f32p, C.POINTER(C.c_int)]
     L.dmc_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
@@ -108,7 +133,9 @@ def load_library(path=ENGINE_SO):
     L.dmc_last_error.argtypes = [vp]
     L.dmc_last_error.restype = C.c_char_p
     for name in ("dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc_mesh_download",
-                 "dmc_batch_download", "dmc_mesh_download_f32",
+                 "dmc_batch_download", "dmc_server_create", "dmc_server_destroy", "dmc_server_submit",
+                 "dmc_server_poll", "dmc_server_wait", "dmc_server_download", "dmc_server_release",
+                 "dmc_server_stats", "dmc_mesh_download_f32",
                  "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
                  "dmc_copy_to_host", "dmc_synchronize"):
         getattr(L, name).restype = C.c_int
@@ -118,6 +145,26 @@ def load_library(path=ENGINE_SO):
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def _download(m, halfedges, call):
+    """Host arrays for mesh `m`, filled by call(positions, normals, quads,
+    tris, halfedges, he_v, he_f, bnd)."""
+    v = np.empty((m.nv, 3), np.float64)
+    n = np.empty((m.nv, 3), np.float64)
+    q = np.empty((m.nq, 4), np.int32)
+    t = np.empty((m.nt, 3), np.int32)
+    he = np.empty((4 * m.nq, 5), np.int32) if halfedges else None
+    hv = np.empty(m.nv, np.int32)
+    hf = np.empty(m.nq, np.int32)
+    bnd = np.empty(m.nbnd, np.int32)
+    i32 = C.c_int32
+    call(_p(v, C.c_double), _p(n, C.c_double), _p(q, i32), _p(t, i32) if m.nt else None,
+         _p(he, i32) if he is not None else None, _p(hv, i32), _p(hf, i32), _p(bnd, i32) if m.nbnd else None)
+    out = dict(v=v, n=n, quads=q, tris=t, he_v=hv, he_f=hf, bnd=bnd)
+    if he is not None:
+        out["he"] = he
+    return out
 
 
 class DeviceBuffer:
@@ -254,25 +301,11 @@ class Engine:
         m = self.mesh if index is None else self.batch[index]
         if m is None:
             raise DMCError("no mesh: call extract first")
-        out = {}
-        v = np.empty((m.nv, 3), np.float64)
-        n = np.empty((m.nv, 3), np.float64)
-        q = np.empty((m.nq, 4), np.int32)
-        t = np.empty((m.nt, 3), np.int32)
-        he = np.empty((4 * m.nq, 5), np.int32) if halfedges else None
-        hv = np.empty(m.nv, np.int32)
-        hf = np.empty(m.nq, np.int32)
-        bnd = np.empty(m.nbnd, np.int32)
-        i32 = C.c_int32
-        args = (_p(v, C.c_double), _p(n, C.c_double), _p(q, i32), _p(t, i32) if m.nt else None,
-                _p(he, i32) if he is not None else None, _p(hv, i32), _p(hf, i32), _p(bnd, i32) if m.nbnd else None)
         if index is None:
-            self._check(self.lib.dmc_mesh_download(self.ctx, *args))
+            out = _download(m, halfedges, lambda *a: self._check(self.lib.dmc_mesh_download(self.ctx, *a)))
         else:
-            self._check(self.lib.dmc_batch_download(self.ctx, int(index), *args))
-        out.update(v=v, n=n, quads=q, tris=t, he_v=hv, he_f=hf, bnd=bnd)
-        if he is not None:
-            out["he"] = he
+            out = _download(m, halfedges,
+                            lambda *a: self._check(self.lib.dmc_batch_download(self.ctx, int(index), *a)))
         if fp32:
             v32 = np.empty((m.nv, 3), np.float32)
             n32 = np.empty((m.nv, 3), np.float32)
@@ -290,6 +323,94 @@ class Engine:
 
     def synchronize(self):
         self._check(self.lib.dmc_synchronize(self.ctx))
+
+
+class Server:
+    """Serving loop (dmc_server_*): submit volumes, wait for them, download,
+    release.  Requests waiting together are extracted as one batch (up to
+    `max_batch`, within `budget_bytes` of HBM); max_batch=1 serves them one
+    at a time."""
+
+    def __init__(self, device=0, max_batch=1 << 20, budget_bytes=0, flags=SIMPLIFY_ALL | TRIANGLES,
+                 dtype=np.float32):
+        self.lib = load_library()
+        self.srv = C.c_void_p()
+        self.dtype = np.dtype(dtype)
+        self.flags = int(flags)
+        opts = ServerOpts(int(device), FIELD_F32 if self.dtype == np.float32 else FIELD_F64, self.flags,
+                          int(max_batch), int(budget_bytes))
+        s = self.lib.dmc_server_create(C.byref(self.srv), C.byref(opts))
+        if s != 0:
+            raise DMCError(f"dmc_server_create: {self.lib.dmc_status_str(s).decode()}")
+        self._keep = {}
+
+    def _check(self, s):
+        if s != 0:
+            raise DMCError(f"{self.lib.dmc_status_str(s).decode()}: "
+                           f"{self.lib.dmc_server_last_error(self.srv).decode()}")
+
+    def submit(self, field, iso, grid=None):
+        """`field`: numpy array (kept alive until release) or, with
+        FIELD_ON_DEVICE in the server's flags, a device pointer (int)."""
+        vol = Volume()
+        keep = None
+        if self.flags & FIELD_ON_DEVICE:
+            vol.field = field.ptr.value if isinstance(field, DeviceBuffer) else int(field)
+        else:
+            keep = np.ascontiguousarray(field, dtype=self.dtype)
+            vol.field = keep.ctypes.data
+            if grid is None:
+                grid = Grid.make(keep.shape)
+        if not isinstance(grid, Grid):
+            grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+        vol.grid = grid
+        vol.gradients = None
+        vol.iso = float(iso)
+        t = C.c_uint64()
+        self._check(self.lib.dmc_server_submit(self.srv, C.byref(vol), C.byref(t)))
+        self._keep[t.value] = keep
+        return t.value
+
+    def poll(self, ticket):
+        d = C.c_int()
+        self._check(self.lib.dmc_server_poll(self.srv, int(ticket), C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        r = Result()
+        self._check(self.lib.dmc_server_wait(self.srv, int(ticket), C.byref(r)))
+        out = r.mesh.summary()
+        out.update(t_submit=r.t_submit, t_start=r.t_start, t_done=r.t_done, batch_size=r.batch_size,
+                   latency=r.t_done - r.t_submit, mesh=r.mesh)
+        return out
+
+    def download(self, ticket, halfedges=True):
+        r = Result()
+        self._check(self.lib.dmc_server_wait(self.srv, int(ticket), C.byref(r)))
+        return _download(r.mesh, halfedges,
+                         lambda *a: self._check(self.lib.dmc_server_download(self.srv, int(ticket), *a)))
+
+    def release(self, ticket):
+        self._check(self.lib.dmc_server_release(self.srv, int(ticket)))
+        self._keep.pop(int(ticket), None)
+
+    def stats(self):
+        b, n = C.c_uint64(), C.c_uint64()
+        h, p = C.c_size_t(), C.c_size_t()
+        self._check(self.lib.dmc_server_stats(self.srv, C.byref(b), C.byref(n), C.byref(h), C.byref(p)))
+        return {"batches": b.value, "served": n.value, "held_bytes": h.value, "pool_bytes": p.value}
+
+    def close(self):
+        if self.srv and self.srv.value:
+            self.lib.dmc_server_destroy(self.srv)
+            self.srv = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class UniformGrid:

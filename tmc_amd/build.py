@@ -36,10 +36,13 @@ def _run(cmd, cwd=ROOT):
     return out
 
 
+ENGINE_SOURCES = ["dmc_engine.hip", "dmc_server.hip"]
+
+
 def build_engine():
-    src = os.path.join(CSRC, "dmc_engine.hip")
+    srcs = [os.path.join(CSRC, f) for f in ENGINE_SOURCES]
     tmp = ENGINE_SO + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src])
+    _run([HIPCC, *HIP_FLAGS, "-pthread", "-o", tmp, *srcs])
     os.replace(tmp, ENGINE_SO)
     return ENGINE_SO
 

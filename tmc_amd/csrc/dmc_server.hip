@@ -1,0 +1,436 @@
+// dmc_server.hip -- serving loop over the batched extraction (host code).
+//
+// Requests (one volume each) are queued by dmc_server_submit.  One worker
+// thread owns a DMC context and repeatedly takes the oldest waiting
+// requests -- as many as fit the HBM budget and max_batch -- runs them as
+// one dmc_extract_batch, copies the union mesh into a result block (one
+// device-to-device copy per array) and completes the requests.  A request
+// keeps its slice of the block until dmc_server_release; blocks are pooled
+// and reused.  With max_batch = 1 the same loop serves one request at a
+// time, the baseline the batched mode is measured against.
+//
+// Built only on the public C ABI of include/tmc_dmc.h plus HIP stream/copy
+// calls; no counterpart in the reference (which has no serving path).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/tmc_dmc.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// One pooled device allocation holding the meshes of one batch.
+struct Block {
+    void* p = nullptr;
+    size_t cap = 0;
+    int refs = 0;
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    dmc_volume vol{};
+    size_t est = 0;          // admission estimate (bytes)
+    double t_submit = 0, t_start = 0, t_done = 0;
+    int32_t batch_size = 0;
+    dmc_status status = DMC_OK;
+    bool done = false;
+    dmc_mesh mesh{};         // device pointers into the block
+    std::vector<int32_t> bnd;
+    Block* block = nullptr;
+    size_t held = 0;         // bytes of the block charged to this request
+};
+
+// Working set of one volume in the engine: the (row, segment) lists and
+// counts (256 slots per row and 256-cell segment), plus a bound for the
+// active-cell, mesh and halfedge arrays proportional to the points.
+size_t estimate_bytes(const dmc_volume& v, int field_type, bool on_device) {
+    const size_t nx = (size_t)v.grid.nx, ny = (size_t)v.grid.ny, nz = (size_t)v.grid.nz;
+    const size_t rows = (ny - 1) * (nz - 1), segs = (nx - 1 + 255) / 256;
+    const size_t pts = nx * ny * nz;
+    size_t b = rows * segs * (256 * 4 + 8) + 40 * pts;
+    if (!on_device) b += pts * (field_type == DMC_FIELD_F32 ? 4 : 8);
+    return b;
+}
+
+}  // namespace
+
+struct dmc_server {
+    dmc_server_opts opts{};
+    dmc_context* ctx = nullptr;
+    hipStream_t copy_stream = nullptr;  // result copies (worker thread)
+    hipStream_t dl_stream = nullptr;    // downloads (caller threads, one at a time)
+    std::mutex mu, dl_mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Block*> blocks;         // every block (pooled)
+    size_t held_bytes = 0;              // results not yet released
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+    Clock::time_point t0 = Clock::now();
+    std::string err;
+    // totals
+    uint64_t batches = 0, served = 0;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Bytes of a block for a union mesh of V vertices and Q quadrilaterals:
+// positions, normals, quads, triangles, twins, halfedge flags, he_v.
+size_t block_bytes(int64_t V, int64_t Q) {
+    return align256(24 * V) * 2 + align256(16 * Q) + align256(24 * Q) + align256(16 * Q) + align256(4 * Q) +
+           align256(4 * V);
+}
+
+// A free block of at least `bytes` (smallest fit), or a new one.  Called
+// with the lock held.
+Block* get_block(dmc_server* s, size_t bytes) {
+    Block* best = nullptr;
+    for (Block* b : s->blocks)
+        if (b->refs == 0 && b->cap >= bytes && (!best || b->cap < best->cap)) best = b;
+    if (best) return best;
+    Block* b = new Block();
+    if (hipMalloc(&b->p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return nullptr;
+    }
+    b->cap = bytes;
+    s->blocks.push_back(b);
+    return b;
+}
+
+void run_batch(dmc_server* s, std::vector<Request*>& batch) {
+    const int n = (int)batch.size();
+    std::vector<dmc_volume> vols(n);
+    for (int i = 0; i < n; i++) vols[i] = batch[i]->vol;
+    std::vector<dmc_mesh> out(n);
+    const double t_start = s->now();
+    dmc_status st = dmc_extract_batch(s->ctx, vols.data(), n, s->opts.field_type, s->opts.flags, out.data());
+    int64_t V = 0, Q = 0;
+    for (int i = 0; i < n && st == DMC_OK; i++) {
+        V += out[i].nv;
+        Q += out[i].nq;
+    }
+    Block* blk = nullptr;
+    std::vector<std::vector<int32_t>> bnd(n);
+    if (st == DMC_OK) {
+        for (int i = 0; i < n; i++) {
+            bnd[i].resize(out[i].nbnd);
+            if (out[i].nbnd)
+                st = dmc_batch_download(s->ctx, i, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                        bnd[i].data());
+            if (st != DMC_OK) break;
+        }
+    }
+    if (st == DMC_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        blk = get_block(s, std::max<size_t>(block_bytes(V, Q), 256));
+        if (!blk) st = DMC_ERR_OUT_OF_MEMORY;
+        else blk->refs = n;
+    }
+    if (st == DMC_OK && (V > 0 || Q > 0)) {
+        // the union arrays are contiguous in volume order, starting at
+        // volume 0's pointers
+        char* p = static_cast<char*>(blk->p);
+        const dmc_mesh& m0 = out[0];
+        const bool tris = (s->opts.flags & DMC_TRIANGLES) != 0;
+        struct Sec {
+            const void* src;
+            size_t bytes;
+            size_t off;
+        };
+        size_t off = 0;
+        std::vector<Sec> secs;
+        auto add = [&](const void* src, size_t bytes, size_t room) {
+            secs.push_back(Sec{src, bytes, off});
+            off += align256(room);
+        };
+        add(m0.d_positions, 24 * V, 24 * V);
+        add(m0.d_normals, 24 * V, 24 * V);
+        add(m0.d_quads, 16 * Q, 16 * Q);
+        add(tris ? m0.d_tris : nullptr, tris ? 24 * Q : 0, 24 * Q);
+        add(m0.d_twins, 16 * Q, 16 * Q);
+        add(m0.d_he_flags, 4 * Q, 4 * Q);
+        add(m0.d_he_v, 4 * V, 4 * V);
+        for (const Sec& x : secs)
+            if (x.src && x.bytes && st == DMC_OK &&
+                hipMemcpyAsync(p + x.off, x.src, x.bytes, hipMemcpyDeviceToDevice, s->copy_stream) != hipSuccess)
+                st = DMC_ERR_HIP;
+        if (hipStreamSynchronize(s->copy_stream) != hipSuccess) st = DMC_ERR_HIP;
+        // re-point every volume's mesh into the block
+        int64_t vb = 0, qb = 0;
+        for (int i = 0; i < n; i++) {
+            dmc_mesh& m = out[i];
+            m.d_positions = reinterpret_cast<const double*>(p + secs[0].off) + 3 * vb;
+            m.d_normals = reinterpret_cast<const double*>(p + secs[1].off) + 3 * vb;
+            m.d_quads = reinterpret_cast<const int32_t*>(p + secs[2].off) + 4 * qb;
+            m.d_tris = tris ? reinterpret_cast<const int32_t*>(p + secs[3].off) + 6 * qb : nullptr;
+            m.d_twins = reinterpret_cast<const int32_t*>(p + secs[4].off) + 4 * qb;
+            m.d_he_flags = reinterpret_cast<const uint8_t*>(p + secs[5].off) + 4 * qb;
+            m.d_he_v = reinterpret_cast<const int32_t*>(p + secs[6].off) + vb;
+            vb += m.nv;
+            qb += m.nq;
+        }
+    }
+    const double t_done = s->now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batches++;
+    for (int i = 0; i < n; i++) {
+        Request* r = batch[i];
+        r->t_start = t_start;
+        r->t_done = t_done;
+        r->batch_size = n;
+        r->status = st;
+        if (st == DMC_OK) {
+            r->mesh = out[i];
+            r->bnd = std::move(bnd[i]);
+            r->block = blk;
+            r->held = block_bytes(out[i].nv, out[i].nq);
+            s->held_bytes += r->held;
+        } else {
+            r->mesh = dmc_mesh{};
+            if (blk) blk->refs--;
+            s->err = dmc_last_error(s->ctx);
+        }
+        r->done = true;
+        s->served++;
+    }
+    s->cv_done.notify_all();
+}
+
+void worker_main(dmc_server* s) {
+    (void)hipSetDevice(s->opts.device);
+    for (;;) {
+        std::vector<Request*> batch;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+            if (s->stop && s->queue.empty()) return;
+            // admission: oldest first, while the working set plus the held
+            // results fit the budget (a request that never fits fails)
+            size_t used = 0;
+            while (!s->queue.empty() && (int)batch.size() < s->opts.max_batch) {
+                Request* r = s->queue.front();
+                if (r->est > s->opts.budget_bytes) {
+                    s->queue.pop_front();
+                    r->status = DMC_ERR_BUDGET;
+                    r->done = true;
+                    r->t_start = r->t_done = s->now();
+                    s->cv_done.notify_all();
+                    continue;
+                }
+                if (!batch.empty() && used + r->est + s->held_bytes > s->opts.budget_bytes) break;
+                batch.push_back(r);
+                used += r->est;
+                s->queue.pop_front();
+            }
+        }
+        if (!batch.empty()) run_batch(s, batch);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+dmc_status dmc_server_create(dmc_server** out, const dmc_server_opts* opts) {
+    if (!out || !opts) return DMC_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (opts->max_batch < 1 || (opts->field_type != DMC_FIELD_F32 && opts->field_type != DMC_FIELD_F64))
+        return DMC_ERR_INVALID_ARG;
+    dmc_server* s = new dmc_server();
+    s->opts = *opts;
+    if (s->opts.budget_bytes == 0) s->opts.budget_bytes = (size_t)96 << 30;
+    s->opts.flags &= ~(uint32_t)DMC_STAGE_TIMING;
+    dmc_status st = dmc_create(&s->ctx, opts->device);
+    if (st != DMC_OK) {
+        delete s;
+        return st;
+    }
+    if (hipSetDevice(opts->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->dl_stream, hipStreamNonBlocking) != hipSuccess) {
+        if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+        dmc_destroy(s->ctx);
+        delete s;
+        return DMC_ERR_HIP;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return DMC_OK;
+}
+
+dmc_status dmc_server_destroy(dmc_server* s) {
+    if (!s) return DMC_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->opts.device);
+    (void)hipStreamSynchronize(s->copy_stream);
+    for (Block* b : s->blocks) {
+        if (b->p) (void)hipFree(b->p);
+        delete b;
+    }
+    (void)hipStreamSynchronize(s->dl_stream);
+    (void)hipStreamDestroy(s->copy_stream);
+    (void)hipStreamDestroy(s->dl_stream);
+    dmc_destroy(s->ctx);
+    delete s;
+    return DMC_OK;
+}
+
+dmc_status dmc_server_submit(dmc_server* s, const dmc_volume* vol, uint64_t* ticket) {
+    if (!s || !vol || !ticket || !vol->field) return DMC_ERR_INVALID_ARG;
+    const dmc_grid& g = vol->grid;
+    if (g.nx < 2 || g.ny < 2 || g.nz < 2) return DMC_ERR_INVALID_ARG;
+    if ((s->opts.flags & DMC_GRADIENTS_SUPPLIED) && !vol->gradients) return DMC_ERR_INVALID_ARG;
+    auto r = std::make_unique<Request>();
+    r->vol = *vol;
+    r->est = estimate_bytes(*vol, s->opts.field_type, (s->opts.flags & DMC_FIELD_ON_DEVICE) != 0);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->ticket = s->next_ticket++;
+        r->t_submit = s->now();
+        *ticket = r->ticket;
+        s->queue.push_back(r.get());
+        s->reqs.emplace(r->ticket, std::move(r));
+    }
+    s->cv_work.notify_one();
+    return DMC_OK;
+}
+
+dmc_status dmc_server_poll(dmc_server* s, uint64_t ticket, int* done) {
+    if (!s || !done) return DMC_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return DMC_ERR_INVALID_ARG;
+    *done = it->second->done ? 1 : 0;
+    return DMC_OK;
+}
+
+dmc_status dmc_server_wait(dmc_server* s, uint64_t ticket, dmc_result* out) {
+    if (!s) return DMC_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return DMC_ERR_INVALID_ARG;
+    Request* r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (out) {
+        out->mesh = r->mesh;
+        out->t_submit = r->t_submit;
+        out->t_start = r->t_start;
+        out->t_done = r->t_done;
+        out->batch_size = r->batch_size;
+        out->status = r->status;
+    }
+    return r->status;
+}
+
+dmc_status dmc_server_download(dmc_server* s, uint64_t ticket, double* positions, double* normals, int32_t* quads,
+                               int32_t* tris, int32_t* halfedges, int32_t* he_v, int32_t* he_f, int32_t* bnd) {
+    if (!s) return DMC_ERR_INVALID_ARG;
+    dmc_mesh m;
+    std::vector<int32_t> b;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return DMC_ERR_INVALID_ARG;
+        Request* r = it->second.get();
+        if (!r->done) return DMC_ERR_STATE;
+        if (r->status != DMC_OK) return r->status;
+        m = r->mesh;
+        b = r->bnd;
+    }
+    (void)hipSetDevice(s->opts.device);
+    // the request's slice of its block stays valid until it is released
+    std::lock_guard<std::mutex> dl(s->dl_mu);
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        return (dst && bytes) ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->dl_stream) : hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    std::vector<int32_t> q, tw;
+    std::vector<uint8_t> fl;
+    if (halfedges) {
+        q.resize(4 * m.nq);
+        tw.resize(4 * m.nq);
+        fl.resize(4 * m.nq);
+    }
+    const hipError_t es[] = {d2h(positions, m.d_positions, 24 * m.nv), d2h(normals, m.d_normals, 24 * m.nv),
+                             d2h(quads, m.d_quads, 16 * m.nq), d2h(tris, m.d_tris, 12 * m.nt),
+                             d2h(he_v, m.d_he_v, 4 * m.nv),
+                             d2h(halfedges ? q.data() : nullptr, m.d_quads, 16 * m.nq),
+                             d2h(halfedges ? tw.data() : nullptr, m.d_twins, 16 * m.nq),
+                             d2h(halfedges ? fl.data() : nullptr, m.d_he_flags, 4 * m.nq),
+                             hipStreamSynchronize(s->dl_stream)};
+    for (hipError_t x : es)
+        if (x != hipSuccess) e = x;
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = hipGetErrorString(e);
+        return DMC_ERR_HIP;
+    }
+    if (halfedges)
+        for (int64_t h = 0; h < 4 * m.nq; h++) {
+            int32_t* o = halfedges + 5 * h;
+            o[0] = q[h];
+            o[1] = (int32_t)(h / 4);
+            o[2] = (int32_t)((h & ~3) | ((h + 1) & 3));
+            o[3] = tw[h];
+            o[4] = fl[h];
+        }
+    if (he_f)
+        for (int64_t f = 0; f < m.nq; f++) he_f[f] = (int32_t)(4 * f);
+    if (bnd && !b.empty()) memcpy(bnd, b.data(), b.size() * sizeof(int32_t));
+    return DMC_OK;
+}
+
+dmc_status dmc_server_release(dmc_server* s, uint64_t ticket) {
+    if (!s) return DMC_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return DMC_ERR_INVALID_ARG;
+    Request* r = it->second.get();
+    if (!r->done) return DMC_ERR_STATE;
+    if (r->block) r->block->refs--;
+    s->held_bytes -= r->held;
+    s->reqs.erase(it);
+    return DMC_OK;
+}
+
+dmc_status dmc_server_stats(dmc_server* s, uint64_t* batches, uint64_t* served, size_t* held_bytes,
+                            size_t* pool_bytes) {
+    if (!s) return DMC_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    size_t pool = 0;
+    for (Block* b : s->blocks) pool += b->cap;
+    if (batches) *batches = s->batches;
+    if (served) *served = s->served;
+    if (held_bytes) *held_bytes = s->held_bytes;
+    if (pool_bytes) *pool_bytes = pool;
+    return DMC_OK;
+}
+
+const char* dmc_server_last_error(dmc_server* s) { return s ? s->err.c_str() : "null server"; }
+
+}  // extern "C"
