@@ -70,6 +70,159 @@ def cpu_baseline(field, grid, iso, block):
                       f"V={nv} Q={nq}; host CPU: {cpu_model()} ({os.cpu_count()} logical)"}
 
 
+def pack_device(eng, fields):
+    """All fields in one device allocation at 256-B aligned offsets; returns
+    the buffer and each field's device address."""
+    offs, total = [], 0
+    for f in fields:
+        offs.append(total)
+        total += (f.nbytes + 255) // 256 * 256
+    host = np.zeros(total, np.uint8)
+    for f, o in zip(fields, offs):
+        host[o:o + f.nbytes] = np.frombuffer(f.tobytes(), np.uint8)
+    buf = eng.alloc(total).upload(host)
+    return buf, [buf.ptr.value + o for o in offs]
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else float("nan")
+
+
+def bench_batch(count=4096, seed=4, steps=3, plain_sample=8):
+    """Batched workload: `count` small volumes (16-96 per axis, log-uniform,
+    mixed generators), fields resident in HBM.  Batched = one
+    dmc_extract_batch over all of them; plain = one dmc_extract per volume
+    (every `plain_sample`-th volume, extrapolated by volumes/s)."""
+    import tmc_amd
+    from tmc_amd import workloads
+    t0 = time.perf_counter()
+    specs = workloads.batch_specs(count, seed=seed)
+    vols = workloads.make_many(specs)
+    log(f"[batch] {count} volumes generated in {time.perf_counter() - t0:.1f} s")
+    nvox = sum(f.size for f, _, _ in vols)
+    eng = tmc_amd.Engine(0)
+    buf, ptrs = pack_device(eng, [f for f, _, _ in vols])
+    dv = [(p, iso, g) for p, (f, g, iso) in zip(ptrs, vols)]
+    flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES | tmc_amd.FIELD_ON_DEVICE
+    eng.extract_batch(dv, flags=flags)
+    t = time.perf_counter()
+    for _ in range(steps):
+        ms = eng.extract_batch(dv, flags=flags)
+    batched_s = (time.perf_counter() - t) / steps
+    nq = sum(m.nq for m in ms)
+    log(f"[batch] batched: {batched_s * 1e3:.1f} ms per {count} volumes")
+    sample = dv[::plain_sample]
+    for p, iso, g in sample[:8]:
+        eng.extract(p, iso, g, flags=flags)
+    lat = []
+    for p, iso, g in sample:
+        t1 = time.perf_counter()
+        eng.extract(p, iso, g, flags=flags)
+        lat.append(time.perf_counter() - t1)
+    plain_vps = len(sample) / sum(lat)
+    log(f"[batch] plain: {len(sample)} volumes, {np.mean(lat) * 1e3:.2f} ms each")
+    buf.free()
+    eng.close()
+    return {"workload": f"batch_{count}", "volumes": count, "voxels": int(nvox), "quads": int(nq),
+            "spec": "axes round(exp(U[ln16,ln96])), generator U{genus2, schwarz_p, blobs}, iso U[-0.2,0.2] / "
+                    "U[0.3,0.7] (blobs), seed 4, fields in HBM",
+            "batched": {"volumes_per_s": round(count / batched_s, 1), "ms_per_batch": round(batched_s * 1e3, 3),
+                        "gvoxels_per_s": round(nvox / batched_s / 1e9, 3)},
+            "plain": {"volumes_per_s": round(plain_vps, 1), "sampled": len(sample),
+                      "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3)},
+            "speedup": round(count / batched_s / plain_vps, 2)}
+
+
+def serve_pool(seed=5):
+    """Request pool with the serving mix exactly: 14 volumes U{32..128}^3,
+    5 of 256^3, 1 of 512^3 (70/25/5 % under uniform draws)."""
+    from tmc_amd import workloads
+    specs = []
+    rng_small = [s for s in workloads.serve_specs(400, seed=seed) if s["shape"][0] < 256]
+    specs += rng_small[:14]
+    base = workloads.serve_specs(200, seed=seed + 1)
+    specs += [dict(s, shape=(256, 256, 256)) for s in base[:5]]
+    specs += [dict(base[5], shape=(512, 512, 512))]
+    return specs
+
+
+def serve_run(srv, pool, rate, n, seed, label):
+    """Poisson arrivals at `rate` per second (inf: all at once) drawing
+    uniformly from `pool`; returns latencies and throughput."""
+    import queue
+    import threading
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(pool), size=n)
+    gaps = np.zeros(n) if not np.isfinite(rate) else rng.exponential(1.0 / rate, size=n)
+    arrive = np.cumsum(gaps)
+    q = queue.Queue()
+    lat, batch_sizes, done_t = [], [], []
+
+    def collect():
+        for _ in range(n):
+            t = q.get()
+            info = srv.wait(t)
+            lat.append(info["latency"])
+            batch_sizes.append(info["batch_size"])
+            done_t.append(info["t_done"])
+            srv.release(t)
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(n):
+        wait = arrive[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        p, iso, g = pool[pick[i]]
+        q.put(srv.submit(p, iso, g))
+    th.join()
+    span = time.perf_counter() - t0
+    out = {"rate_offered": None if not np.isfinite(rate) else round(rate, 1), "requests": n,
+           "volumes_per_s": round(n / span, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+           "p99_ms": round(pct(lat, 99) * 1e3, 3), "mean_batch": round(float(np.mean(batch_sizes)), 2)}
+    log(f"[serve] {label}: {out}")
+    return out
+
+
+def bench_serve(seed=5, secs=4.0, factor=5.0):
+    """Served mix: Poisson arrivals, 70% U{32..128}^3, 25% 256^3, 5% 512^3,
+    fields in HBM.  One-at-a-time (max_batch 1) is run at 70% of its
+    capacity; the batched server is run at `factor` times that rate; both
+    report p50/p99 latency (submit to mesh in HBM) and volumes/s."""
+    import tmc_amd
+    from tmc_amd import workloads
+    t0 = time.perf_counter()
+    specs = serve_pool(seed)
+    vols = [workloads.make(s) for s in specs]
+    log(f"[serve] pool of {len(vols)} volumes generated in {time.perf_counter() - t0:.1f} s")
+    eng = tmc_amd.Engine(0)
+    buf, ptrs = pack_device(eng, [f for f, _, _ in vols])
+    pool = [(p, iso, g) for p, (f, g, iso) in zip(ptrs, vols)]
+    flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES | tmc_amd.FIELD_ON_DEVICE
+    budget = 96 << 30
+    plain_srv = tmc_amd.Server(0, max_batch=1, flags=flags, budget_bytes=budget)
+    serve_run(plain_srv, pool, float("inf"), 2 * len(pool), seed, "plain warm-up")
+    cap_plain = serve_run(plain_srv, pool, float("inf"), 120, seed + 1, "plain capacity")["volumes_per_s"]
+    rate = 0.7 * cap_plain
+    plain = serve_run(plain_srv, pool, rate, max(100, int(rate * secs)), seed + 2, "plain at 70% capacity")
+    plain_srv.close()
+    bsrv = tmc_amd.Server(0, max_batch=1 << 20, flags=flags, budget_bytes=budget)
+    serve_run(bsrv, pool, float("inf"), 4 * len(pool), seed, "batched warm-up")
+    cap_b = serve_run(bsrv, pool, float("inf"), 1200, seed + 3, "batched capacity")["volumes_per_s"]
+    batched = serve_run(bsrv, pool, factor * rate, max(200, int(factor * rate * secs)), seed + 4,
+                        f"batched at {factor}x the plain rate")
+    bsrv.close()
+    buf.free()
+    eng.close()
+    return {"workload": "serve_mix", "mix": "70% U{32..128}^3, 25% 256^3, 5% 512^3 (pool of 20, uniform draws), "
+                                           "Poisson arrivals, fields in HBM, budget 96 GiB",
+            "plain": dict(plain, capacity_volumes_per_s=cap_plain),
+            "batched": dict(batched, capacity_volumes_per_s=cap_b),
+            "rate_ratio": round(batched["volumes_per_s"] / plain["volumes_per_s"], 2),
+            "p99_ratio": round(batched["p99_ms"] / plain["p99_ms"], 3)}
+
+
 def calibrate_copy():
     """A plain copy kernel of known size: 1 GiB read + 1 GiB written per
     launch.  Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, the ratio of
@@ -95,9 +248,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (4,096 volumes) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson mix) leg")
+    ap.add_argument("--only", choices=["batch", "serve"], help="run only the batched or served leg")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only == "batch":
+        print(json.dumps(bench_batch()), flush=True)
+        return
+    if args.only == "serve":
+        print(json.dumps(bench_serve()), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -188,6 +188,11 @@ class DeviceBuffer:
             self.ptr = C.c_void_p()
 
 
+class _RawDevice:
+    def __init__(self, ptr):
+        self.ptr = C.c_void_p(int(ptr))
+
+
 class Engine:
     """One device context (one HIP stream + reusable device buffers)."""
 
@@ -223,7 +228,9 @@ class Engine:
                 dtype=np.float32):
         """Run the DMC path.  `field` is a numpy array (nz, ny, nx) or a
         DeviceBuffer holding one (then pass `shape` and `dtype`)."""
-        if isinstance(field, DeviceBuffer):
+        if isinstance(field, int):   # raw device pointer
+            field = _RawDevice(field)
+        if isinstance(field, (DeviceBuffer, _RawDevice)):
             fptr = field.ptr
             flags |= FIELD_ON_DEVICE
             if grid is None:

@@ -342,16 +342,18 @@ void three_face_edges(dmc_context* ctx, const HeStats& hs, int64_t nv, ThreeLaun
     }
 }
 
-// Halfedges through a CSR of outgoing (halfedge, destination) pairs.
+// Halfedges through a CSR of outgoing (halfedge, destination) pairs: rows
+// off[v] .. off[v+1] (off has nv+1 entries).
 HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq, nhe = 4 * nq;
     const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
     uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, nv);
-    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, nv);
+    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, nv + 1);
     uint32_t* cursor = ensure<uint32_t>(ctx, ctx->cursor, nv);
     int2* adj = ensure<int2>(ctx, ctx->adj, nhe);
     int32_t* twin = ensure<int32_t>(ctx, ctx->twin, nhe);
     uint8_t* flag = ensure<uint8_t>(ctx, ctx->flag, nhe);
+    int32_t* hev = final_pass ? ensure<int32_t>(ctx, ctx->hev, nv) : nullptr;
     HeStats* st = ensure<HeStats>(ctx, ctx->stats, 1);
     HIPC(ctx, hipMemsetAsync(st, 0, sizeof(HeStats), ctx->stream));
     HeStats hs{};
@@ -364,30 +366,35 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
                            &st->bad);
         check_launch(ctx, "k_he_degree");
     }
-    scan(ctx, deg, off, nv);
+    const uint64_t rows = scan(ctx, deg, off, nv);
+    HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + nv), (int)(uint32_t)rows, 1, ctx->stream));
     if (nhe > 0) {
         hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
                            cursor, adj);
-        unsigned long long* vst = nullptr;
-        if (final_pass && ctx->nvol > 1) {
-            vst = ensure<unsigned long long>(ctx, ctx->vhe, 2 * (size_t)ctx->nvol);
-            HIPC(ctx, hipMemsetAsync(vst, 0, 2 * (size_t)ctx->nvol * sizeof(unsigned long long), ctx->stream));
-        }
-        hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, deg,
-                           adj, twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst);
+        check_launch(ctx, "k_he_scatter");
+    }
+    unsigned long long* vst = nullptr;
+    if (final_pass && ctx->nvol > 1) {
+        vst = ensure<unsigned long long>(ctx, ctx->vhe, 2 * (size_t)ctx->nvol);
+        HIPC(ctx, hipMemsetAsync(vst, 0, 2 * (size_t)ctx->nvol * sizeof(unsigned long long), ctx->stream));
+    }
+    if (nhe > 0) {
+        hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, adj,
+                           twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst);
         check_launch(ctx, "k_he_twins");
+    }
+    if (final_pass && nv > 0) {
+        hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, adj, nv, hev);
+        check_launch(ctx, "k_he_v");
     }
     HIPC(ctx, hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    if (final_pass) {
-        int32_t* hev = ensure<int32_t>(ctx, ctx->hev, nv);
-        if (nv > 0)
-            hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, deg, adj, nv, hev);
+    (void)rows;
+    if (final_pass)
         three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
             hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
-                               deg, adj, ovr, recs, nrec, pass, hev);
+                               adj, flag, ovr, recs, nrec, pass, hev);
         });
-    }
     return hs;
 }
 
@@ -397,8 +404,8 @@ int32_t* valence(dmc_context* ctx) {
     unsigned int* zero = static_cast<unsigned int*>(ctx->fails.p) + 4;
     if (nv > 0) {
         hipLaunchKernelGGL(k_valence, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
-                           static_cast<uint32_t*>(ctx->off.p), static_cast<uint32_t*>(ctx->deg.p),
-                           static_cast<int2*>(ctx->adj.p), static_cast<int32_t*>(ctx->twin.p), nv, val, zero);
+                           static_cast<uint32_t*>(ctx->off.p), static_cast<int2*>(ctx->adj.p),
+                           static_cast<int32_t*>(ctx->twin.p), nv, val, zero);
     }
     check_launch(ctx, "valence");
     return val;

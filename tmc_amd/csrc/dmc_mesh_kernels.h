@@ -66,37 +66,29 @@ __device__ __forceinline__ void group_insert(int32_t g[5], int32_t h) {
 
 // Members of the edge group of halfedge h = (a -> b): out-halfedges of a
 // ending at b and of b ending at a, in ascending order (5 smallest kept);
-// the true count is returned.
+// the true count is returned.  The CSR rows are off[v] .. off[v+1].
 __device__ __forceinline__ int he_group(const int32_t* __restrict__ quads, const uint32_t* __restrict__ off,
-                                        const uint32_t* __restrict__ deg, const int2* __restrict__ adj, int32_t h,
-                                        int64_t nv, int32_t g[5]) {
+                                        const int2* __restrict__ adj, int32_t h, int64_t nv, int32_t g[5]) {
 #pragma unroll
     for (int p = 0; p < 5; p++) g[p] = 0x7FFFFFFF;
     const int32_t a = quads[h], b = he_dest(quads, h);
     if (!vid_ok(a, nv) || !vid_ok(b, nv)) return 0;
     int c = 0;
-    {
-        const int2* p = adj + off[a];
-        const int n = (int)deg[a];
-        for (int x = 0; x < n; x++) {
-            const int2 e = p[x];
-            if (e.y == b) {
-                group_insert(g, e.x);
+    for (uint32_t x = off[a], e = off[a + 1]; x < e; x++) {
+        const int2 r = adj[x];
+        if (r.y == b) {
+            group_insert(g, r.x);
+            c++;
+        }
+    }
+    if (b != a)
+        for (uint32_t x = off[b], e = off[b + 1]; x < e; x++) {
+            const int2 r = adj[x];
+            if (r.y == a) {
+                group_insert(g, r.x);
                 c++;
             }
         }
-    }
-    if (b != a) {
-        const int2* p = adj + off[b];
-        const int n = (int)deg[b];
-        for (int x = 0; x < n; x++) {
-            const int2 e = p[x];
-            if (e.y == a) {
-                group_insert(g, e.x);
-                c++;
-            }
-        }
-    }
     return c;
 }
 
@@ -133,8 +125,11 @@ struct HeStats {
     unsigned int pad;
 };
 
+// Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669),
+// one thread per halfedge.  The group's first member also counts the edge
+// (non-manifold / error).
 __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
-                                                      const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                                                      const uint32_t* __restrict__ off,
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
                                                       uint8_t* __restrict__ flag, HeStats* __restrict__ st,
                                                       const int64_t* __restrict__ qb, int nvol,
@@ -142,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         int32_t g[5];
-        const int c = he_group(quads, off, deg, adj, h, nv, g);
+        const int c = he_group(quads, off, adj, h, nv, g);
         int32_t tw;
         uint8_t fl;
         twin_rule(quads, c, g, h, tw, fl);
@@ -163,14 +158,11 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
 
 // he_v: the last halfedge (in face order) leaving each vertex
 // (cpp_dmc/DualMarchingCubes.cpp:561-564).
-__global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                  const int2* __restrict__ adj, int64_t nv,
-                                                  int32_t* __restrict__ he_v) {
+__global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ off, const int2* __restrict__ adj,
+                                                  int64_t nv, int32_t* __restrict__ he_v) {
     TMC_GRID_STRIDE(v, nv) {
-        const int2* p = adj + off[v];
-        const int n = (int)deg[v];
         int32_t last = -1;
-        for (int x = 0; x < n; x++) last = p[x].x > last ? p[x].x : last;
+        for (uint32_t x = off[v], e = off[v + 1]; x < e; x++) last = adj[x].x > last ? adj[x].x : last;
         he_v[v] = last;
     }
 }
@@ -188,15 +180,16 @@ struct BndRec {
 };
 
 __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
-                                                      const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                      const int2* __restrict__ adj,
+                                                      const uint32_t* __restrict__ off, const int2* __restrict__ adj,
+                                                      const uint8_t* __restrict__ flag,
                                                       unsigned long long* __restrict__ ovr, BndRec* __restrict__ recs,
                                                       unsigned int* __restrict__ nrec, int pass,
                                                       int32_t* __restrict__ he_v) {
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
+        if (flag[h] != 2) continue;   // every member of a 3-face group, and only those
         int32_t g[5];
-        const int c = he_group(quads, off, deg, adj, h, nv, g);
+        const int c = he_group(quads, off, adj, h, nv, g);
         if (c != 3 || g[0] != h) continue;
         const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
         int32_t ba, bb, lone;
@@ -220,13 +213,12 @@ __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__
 // Valence: outgoing halfedges plus boundary halfedges ending at the vertex
 // (cpp_dmc/DualMarchingCubes.cpp:847-871).  A halfedge ends at v exactly
 // when it is the predecessor of one leaving v.
-__global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ off, const uint32_t* __restrict__ deg,
-                                                     const int2* __restrict__ adj, const int32_t* __restrict__ twin,
-                                                     int64_t nv, int32_t* __restrict__ val,
-                                                     unsigned int* __restrict__ zero) {
+__global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ off, const int2* __restrict__ adj,
+                                                     const int32_t* __restrict__ twin, int64_t nv,
+                                                     int32_t* __restrict__ val, unsigned int* __restrict__ zero) {
     TMC_GRID_STRIDE(v, nv) {
         const int2* p = adj + off[v];
-        const int n = (int)deg[v];
+        const int n = (int)(off[v + 1] - off[v]);
         int s = 0;
         for (int x = 0; x < n; x++) s += 1 + (twin[he_prev(p[x].x)] == -1);
         val[v] = s;

@@ -51,32 +51,35 @@ def _shape3(n):
     return (n, n, n) if np.isscalar(n) else tuple(int(x) for x in n)   # (nz, ny, nx)
 
 
-def _threads():
+def _threads(threads=None):
+    """Generator threads (0: all hardware threads)."""
+    if threads is not None:
+        return int(threads)
     return int(os.environ.get("TMC_WORKLOAD_THREADS", "0"))
 
 
-def genus2(n=128, lo=-1.5, hi=1.5):
+def genus2(n=128, lo=-1.5, hi=1.5, threads=None):
     nz, ny, nx = _shape3(n)
     f = np.empty((nz, ny, nx), np.float32)
-    _load().wl_genus2(nx, ny, nz, lo, hi, f.ctypes.data_as(C.POINTER(C.c_float)), _threads())
+    _load().wl_genus2(nx, ny, nz, lo, hi, f.ctypes.data_as(C.POINTER(C.c_float)), _threads(threads))
     h = [(hi - lo) / m for m in (nx, ny, nz)]
     g = Grid(nx, ny, nz, lo + 0.5 * h[0], lo + 0.5 * h[1], lo + 0.5 * h[2], h[0], h[1], h[2])
     return f, g, 0.0
 
 
-def schwarz(n=1024, seed=3, sigma=0.01, lo=-8 * math.pi, hi=8 * math.pi, iso=0.3):
+def schwarz(n=1024, seed=3, sigma=0.01, lo=-8 * math.pi, hi=8 * math.pi, iso=0.3, threads=None):
     nz, ny, nx = _shape3(n)
     f = np.empty((nz, ny, nx), np.float32)
-    _load().wl_schwarz(nx, ny, nz, lo, hi, sigma, seed, f.ctypes.data_as(C.POINTER(C.c_float)), _threads())
+    _load().wl_schwarz(nx, ny, nz, lo, hi, sigma, seed, f.ctypes.data_as(C.POINTER(C.c_float)), _threads(threads))
     g = Grid(nx, ny, nz, lo, lo, lo, (hi - lo) / (nx - 1), (hi - lo) / (ny - 1), (hi - lo) / (nz - 1))
     return f, g, iso
 
 
-def blobs(n=512, seed=2, nblobs=64, sigma=0.05, iso=0.5):
+def blobs(n=512, seed=2, nblobs=64, sigma=0.05, iso=0.5, threads=None):
     nz, ny, nx = _shape3(n)
     f = np.empty((nz, ny, nx), np.float32)
     _load().wl_blobs(nx, ny, nz, nblobs, 0.02, 0.08, 0.5, 1.5, sigma, seed,
-                     f.ctypes.data_as(C.POINTER(C.c_float)), _threads())
+                     f.ctypes.data_as(C.POINTER(C.c_float)), _threads(threads))
     g = Grid(nx, ny, nz, 0.5 / nx, 0.5 / ny, 0.5 / nz, 1.0 / nx, 1.0 / ny, 1.0 / nz)
     return f, g, iso
 
@@ -99,13 +102,13 @@ def normal(seed, idx):
 KINDS = ("genus2", "schwarz", "blobs")
 
 
-def _mixed(kind, shape, seed, iso):
+def _mixed(kind, shape, seed, iso, threads=None):
     if kind == "genus2":
-        f, g, _ = genus2(shape)
+        f, g, _ = genus2(shape, threads=threads)
     elif kind == "schwarz":
-        f, g, _ = schwarz(shape, seed=seed)
+        f, g, _ = schwarz(shape, seed=seed, threads=threads)
     else:
-        f, g, _ = blobs(shape, seed=seed)
+        f, g, _ = blobs(shape, seed=seed, threads=threads)
     return f, g, iso
 
 
@@ -143,6 +146,13 @@ def serve_specs(count, seed=5):
     return out
 
 
-def make(spec):
+def make(spec, threads=None):
     """(field, grid, iso) of one mixed-volume spec."""
-    return _mixed(spec["kind"], spec["shape"], spec["seed"], spec["iso"])
+    return _mixed(spec["kind"], spec["shape"], spec["seed"], spec["iso"], threads)
+
+
+def make_many(specs, workers=16):
+    """Generate many (small) volumes in parallel, one generator thread each."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda s: make(s, threads=1), specs))
