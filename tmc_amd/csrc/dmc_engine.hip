@@ -61,6 +61,7 @@ struct dmc_context {
     // mesh (double-buffered where compacted)
     Buf pos[2], nrm[2], quads[2], colors;
     int cur = 0;  // which of the double buffers holds the live mesh
+    bool csr_ready = false;  // off/adj already hold the live mesh's halfedge rows
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
         keepV, newV, keepQ, newQ, clist, ccnt, tris, stats;
@@ -299,16 +300,29 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * V);
     int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * Q);
     int32_t* colors = ensure<int32_t>(ctx, ctx->colors, Q);
+    // first halfedge CSR straight from the cells: degree of each vertex from
+    // k_vertices, rows filled by k_quads
+    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, V);
+    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, V + 1);
+    int2* adj = ensure<int2>(ctx, ctx->adj, 4 * Q);
     mark(ctx, "vertices");
     if (A > 0) {
         hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc,
-                           rowBaseV, (int64_t)A, pos, nrm, vstat);
+                           rowBaseV, (int64_t)A, pos, nrm, deg, vstat);
         check_launch(ctx, "k_vertices");
-        mark(ctx, "quads");
+    }
+    mark(ctx, "quads");
+    const uint64_t rows = scan(ctx, deg, off, V);
+    if (rows != 4 * Q) fail(ctx, DMC_ERR_STATE, "halfedge rows %llu != 4 Q %llu", (unsigned long long)rows,
+                            (unsigned long long)(4 * Q));
+    HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + V), (int)(uint32_t)rows, 1, ctx->stream));
+    if (A > 0) {
         hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc, aqloc,
-                           rowBaseV, rowBaseQ, (int64_t)A, lists, segCount, segBase, quads, colors, fails + 5);
+                           rowBaseV, rowBaseQ, (int64_t)A, lists, segCount, segBase, quads, colors, off, adj,
+                           fails + 5);
         check_launch(ctx, "k_quads");
     }
+    ctx->csr_ready = true;
     ctx->mesh.nv = (int64_t)V;
     ctx->mesh.nq = (int64_t)Q;
     ctx->mesh.nv_cells = (int64_t)V;
@@ -357,21 +371,34 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     HeStats* st = ensure<HeStats>(ctx, ctx->stats, 1);
     HIPC(ctx, hipMemsetAsync(st, 0, sizeof(HeStats), ctx->stream));
     HeStats hs{};
-    if (nv > 0) {
-        HIPC(ctx, hipMemsetAsync(deg, 0, nv * sizeof(uint32_t), ctx->stream));
-        HIPC(ctx, hipMemsetAsync(cursor, 0, nv * sizeof(uint32_t), ctx->stream));
+    bool fresh = ctx->csr_ready;
+    ctx->csr_ready = false;
+    if (fresh) {
+        // rows from the cell stage are complete unless a quadrilateral had an
+        // unresolved neighbour cell
+        uint32_t unresolved = 0;
+        HIPC(ctx, hipMemcpyAsync(&unresolved, static_cast<uint32_t*>(ctx->fails.p) + 5, sizeof(unresolved),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        fresh = unresolved == 0;
     }
-    if (nhe > 0) {
-        hipLaunchKernelGGL(k_he_degree, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, deg,
-                           &st->bad);
-        check_launch(ctx, "k_he_degree");
-    }
-    const uint64_t rows = scan(ctx, deg, off, nv);
-    HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + nv), (int)(uint32_t)rows, 1, ctx->stream));
-    if (nhe > 0) {
-        hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
-                           cursor, adj);
-        check_launch(ctx, "k_he_scatter");
+    if (!fresh) {
+        if (nv > 0) {
+            HIPC(ctx, hipMemsetAsync(deg, 0, nv * sizeof(uint32_t), ctx->stream));
+            HIPC(ctx, hipMemsetAsync(cursor, 0, nv * sizeof(uint32_t), ctx->stream));
+        }
+        if (nhe > 0) {
+            hipLaunchKernelGGL(k_he_degree, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, deg,
+                               &st->bad);
+            check_launch(ctx, "k_he_degree");
+        }
+        const uint64_t rows = scan(ctx, deg, off, nv);
+        HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + nv), (int)(uint32_t)rows, 1, ctx->stream));
+        if (nhe > 0) {
+            hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
+                               cursor, adj);
+            check_launch(ctx, "k_he_scatter");
+        }
     }
     unsigned long long* vst = nullptr;
     if (final_pass && ctx->nvol > 1) {
@@ -389,7 +416,6 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     }
     HIPC(ctx, hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    (void)rows;
     if (final_pass)
         three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
             hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,

@@ -233,6 +233,25 @@ __device__ __forceinline__ unsigned used_contours(unsigned icase, uint32_t map, 
     return used;
 }
 
+// Edges of contour t (cut edges whose polygon is t), and the interior ones
+// of cell (i,j,k), as 12-bit masks.  A vertex's quadrilaterals are those of
+// the interior edges of its contour, one each: its degree in the mesh and,
+// ordered by edge id, its rows in the first halfedge CSR.
+__device__ __forceinline__ unsigned contour_edges(unsigned icase, uint32_t map, unsigned t) {
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < 12; e++)
+        if (edge_cut(icase, e) && ((map >> (2 * e)) & 3u) == t) m |= 1u << e;
+    return m;
+}
+__device__ __forceinline__ unsigned interior_edges(int i, int j, int k, const Dims& d) {
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < 12; e++)
+        if (edge_interior(e, i, j, k, d.nx, d.ny, d.nz)) m |= 1u << e;
+    return m;
+}
+
 // ---------------------------------------------------------------------------
 // K2: one wave per row, cells in ascending x: polygons of every active cell
 // -> code, kept vertices and owned quadrilaterals (edges e0, e3, e8 of the
@@ -367,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) k_vertices(VolTable vt, const uint32_t
                                                       const uint32_t* __restrict__ avloc,
                                                       const uint32_t* __restrict__ rowBaseV, int64_t A,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
-                                                      VolStat* __restrict__ vstat) {
+                                                      uint32_t* __restrict__ vdeg, VolStat* __restrict__ vstat) {
     static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
     __shared__ float boxes[kBoxFloats * kBoxStride];
     __shared__ double corners[8 * kCornerStride];
@@ -448,6 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_vertices(VolTable vt, const uint32_t
             vnrm[3 * (int64_t)out + 0] = n[0];
             vnrm[3 * (int64_t)out + 1] = n[1];
             vnrm[3 * (int64_t)out + 2] = n[2];
+            vdeg[out] = (uint32_t)__popc(contour_edges(icase, code >> 8, (unsigned)t) & interior_edges(i, j, k, d));
             out++;
         }
     }
@@ -493,6 +513,23 @@ __device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode,
     return (int32_t)(vb + __popc(used & ((1u << t) - 1u)));
 }
 
+// The same vertex and the row of edge e among its contour's interior edges.
+__device__ __forceinline__ int32_t vertex_slot_of(const uint32_t* __restrict__ acode,
+                                                  const uint32_t* __restrict__ avloc,
+                                                  const uint32_t* __restrict__ rowBaseV, int64_t a, int e, int i,
+                                                  int j, int k, const Dims& d, int& slot) {
+    slot = 0;
+    if (a < 0) return -1;
+    const uint32_t code = acode[a];
+    const unsigned icase = code & 0xFFu;
+    const uint32_t map = code >> 8;
+    const unsigned t = (map >> (2 * e)) & 3u;
+    const unsigned used = used_contours(icase, map, i, j, k, d);
+    slot = __popc(contour_edges(icase, map, t) & interior_edges(i, j, k, d) & ((1u << e) - 1u));
+    const uint32_t vb = rowBaseV[(int64_t)k * (d.ny - 1) + j] + avloc[a];
+    return (int32_t)(vb + __popc(used & ((1u << t) - 1u)));
+}
+
 // ---------------------------------------------------------------------------
 // K4: quadrilaterals.  Each complete quadrilateral belongs to the grid edge
 // e0 (x), e3 (y) or e8 (z) of the cell at its lower corner; ascending
@@ -510,6 +547,7 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
                                                    const uint32_t* __restrict__ segCount,
                                                    const uint32_t* __restrict__ segBase,
                                                    int32_t* __restrict__ quads, int32_t* __restrict__ colors,
+                                                   const uint32_t* __restrict__ off, int2* __restrict__ adj,
                                                    uint32_t* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         const int vol = vol_of(vt.aBase, vt.n, a);
@@ -528,48 +566,58 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
         const bool ey = edge_cut(icase, 3) && i >= 1 && k >= 1;
         const bool ez = edge_cut(icase, 8) && i >= 1 && j >= 1;
         if (!(ex || ey || ez)) continue;
-        const int32_t own0 = vertex_of(acode, avloc, rowBaseV, a, 0, i, j, k, d);
-        const int32_t own3 = vertex_of(acode, avloc, rowBaseV, a, 3, i, j, k, d);
-        const int32_t own8 = vertex_of(acode, avloc, rowBaseV, a, 8, i, j, k, d);
+        int s0, s3, s8;
+        const int32_t own0 = vertex_slot_of(acode, avloc, rowBaseV, a, 0, i, j, k, d, s0);
+        const int32_t own3 = vertex_slot_of(acode, avloc, rowBaseV, a, 3, i, j, k, d, s3);
+        const int32_t own8 = vertex_slot_of(acode, avloc, rowBaseV, a, 8, i, j, k, d, s8);
         // cells of the previous row (j-1) and plane (k-1) found once
         const int64_t cjm = (ex || ez) ? find_active(L, d, i, j - 1, k) : -1;
         const int64_t ckm = (ex || ey) ? find_active(L, d, i, j, k - 1) : -1;
         int32_t out[4];
+        int sl[4];
+        // quadrilateral q with its vertices in out[]: store it, and each of
+        // its halfedges (4q+p: out[p] -> out[p+1]) in its origin's CSR row
+        auto emit = [&](int c) {
+            if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) {
+                atomicAdd(unresolved, 1u);
+            } else {
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+                    adj[off[out[p]] + sl[p]] = make_int2((int32_t)(4 * q + p), out[(p + 1) & 3]);
+            }
+            reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
+            colors[q] = 3 * parity + c;
+            q++;
+        };
         // x edge: cells (i,j,k) e0, (i,j-1,k) e2, (i,j,k-1) e4, (i,j-1,k-1) e6
         if (ex) {
             const int64_t e = find_active(L, d, i, j - 1, k - 1);
             out[edge_slot(0, bit)] = own0;
-            out[edge_slot(2, bit)] = vertex_of(acode, avloc, rowBaseV, cjm, 2, i, j - 1, k, d);
-            out[edge_slot(4, bit)] = vertex_of(acode, avloc, rowBaseV, ckm, 4, i, j, k - 1, d);
-            out[edge_slot(6, bit)] = vertex_of(acode, avloc, rowBaseV, e, 6, i, j - 1, k - 1, d);
-            if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) atomicAdd(unresolved, 1u);
-            reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
-            colors[q] = 3 * parity;
-            q++;
+            sl[edge_slot(0, bit)] = s0;
+            out[edge_slot(2, bit)] = vertex_slot_of(acode, avloc, rowBaseV, cjm, 2, i, j - 1, k, d, sl[edge_slot(2, bit)]);
+            out[edge_slot(4, bit)] = vertex_slot_of(acode, avloc, rowBaseV, ckm, 4, i, j, k - 1, d, sl[edge_slot(4, bit)]);
+            out[edge_slot(6, bit)] = vertex_slot_of(acode, avloc, rowBaseV, e, 6, i, j - 1, k - 1, d, sl[edge_slot(6, bit)]);
+            emit(0);
         }
         // y edge: cells (i,j,k) e3, (i-1,j,k) e1, (i,j,k-1) e7, (i-1,j,k-1) e5
         if (ey) {
             const int64_t e = ckm > 0 ? ckm - 1 : -1;
             out[edge_slot(3, bit)] = own3;
-            out[edge_slot(1, bit)] = vertex_of(acode, avloc, rowBaseV, a - 1, 1, i - 1, j, k, d);
-            out[edge_slot(7, bit)] = vertex_of(acode, avloc, rowBaseV, ckm, 7, i, j, k - 1, d);
-            out[edge_slot(5, bit)] = vertex_of(acode, avloc, rowBaseV, e, 5, i - 1, j, k - 1, d);
-            if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) atomicAdd(unresolved, 1u);
-            reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
-            colors[q] = 3 * parity + 1;
-            q++;
+            sl[edge_slot(3, bit)] = s3;
+            out[edge_slot(1, bit)] = vertex_slot_of(acode, avloc, rowBaseV, a - 1, 1, i - 1, j, k, d, sl[edge_slot(1, bit)]);
+            out[edge_slot(7, bit)] = vertex_slot_of(acode, avloc, rowBaseV, ckm, 7, i, j, k - 1, d, sl[edge_slot(7, bit)]);
+            out[edge_slot(5, bit)] = vertex_slot_of(acode, avloc, rowBaseV, e, 5, i - 1, j, k - 1, d, sl[edge_slot(5, bit)]);
+            emit(1);
         }
         // z edge: cells (i,j,k) e8, (i-1,j,k) e9, (i,j-1,k) e11, (i-1,j-1,k) e10
         if (ez) {
             const int64_t e = cjm > 0 ? cjm - 1 : -1;
             out[edge_slot(8, bit)] = own8;
-            out[edge_slot(9, bit)] = vertex_of(acode, avloc, rowBaseV, a - 1, 9, i - 1, j, k, d);
-            out[edge_slot(11, bit)] = vertex_of(acode, avloc, rowBaseV, cjm, 11, i, j - 1, k, d);
-            out[edge_slot(10, bit)] = vertex_of(acode, avloc, rowBaseV, e, 10, i - 1, j - 1, k, d);
-            if (out[0] < 0 || out[1] < 0 || out[2] < 0 || out[3] < 0) atomicAdd(unresolved, 1u);
-            reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
-            colors[q] = 3 * parity + 2;
-            q++;
+            sl[edge_slot(8, bit)] = s8;
+            out[edge_slot(9, bit)] = vertex_slot_of(acode, avloc, rowBaseV, a - 1, 9, i - 1, j, k, d, sl[edge_slot(9, bit)]);
+            out[edge_slot(11, bit)] = vertex_slot_of(acode, avloc, rowBaseV, cjm, 11, i, j - 1, k, d, sl[edge_slot(11, bit)]);
+            out[edge_slot(10, bit)] = vertex_slot_of(acode, avloc, rowBaseV, e, 10, i - 1, j - 1, k, d, sl[edge_slot(10, bit)]);
+            emit(2);
         }
     }
 }
