@@ -61,10 +61,11 @@ struct dmc_context {
     // mesh (double-buffered where compacted)
     Buf pos[2], nrm[2], quads[2], colors;
     int cur = 0;  // which of the double buffers holds the live mesh
-    bool csr_ready = false;  // off/adj already hold the live mesh's halfedge rows
+    int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats;
+        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad;
+    uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // results
     dmc_mesh mesh{};
     bool have_mesh = false;
@@ -82,7 +83,8 @@ struct dmc_context {
                       &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
-                      &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe};
+                      &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
+                      &off2, &adj2, &rbad};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -322,7 +324,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
                            fails + 5);
         check_launch(ctx, "k_quads");
     }
-    ctx->csr_ready = true;
+    ctx->csr_ready = 1;
     ctx->mesh.nv = (int64_t)V;
     ctx->mesh.nq = (int64_t)Q;
     ctx->mesh.nv_cells = (int64_t)V;
@@ -371,9 +373,10 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     HeStats* st = ensure<HeStats>(ctx, ctx->stats, 1);
     HIPC(ctx, hipMemsetAsync(st, 0, sizeof(HeStats), ctx->stream));
     HeStats hs{};
-    bool fresh = ctx->csr_ready;
-    ctx->csr_ready = false;
-    if (fresh) {
+    const int ready = ctx->csr_ready;
+    ctx->csr_ready = 0;
+    bool fresh = ready != 0;
+    if (ready == 1) {
         // rows from the cell stage are complete unless a quadrilateral had an
         // unresolved neighbour cell
         uint32_t unresolved = 0;
@@ -381,6 +384,9 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
                                  hipMemcpyDeviceToHost, ctx->stream));
         HIPC(ctx, hipStreamSynchronize(ctx->stream));
         fresh = unresolved == 0;
+    } else if (ready == 2) {
+        // remapped rows: halfedges left without a valid origin were counted
+        HIPC(ctx, hipMemcpyAsync(&st->bad, ctx->rbad.p, sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
     }
     if (!fresh) {
         if (nv > 0) {
@@ -416,6 +422,7 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     }
     HIPC(ctx, hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->last_bad = hs.bad;
     if (final_pass)
         three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
             hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
@@ -492,6 +499,39 @@ int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsec
     return nn;
 }
 
+// Halfedge rows of the compacted mesh (nn vertices, nq2 quadrilaterals)
+// from the current rows, into the spare CSR buffers, which then become the
+// current ones.  Skipped (the next build counts from scratch) when the
+// current rows left halfedges out.
+void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2) {
+    if (ctx->last_bad != 0) return;
+    const uint32_t* off = static_cast<const uint32_t*>(ctx->off.p);
+    const int2* adj = static_cast<const int2*>(ctx->adj.p);
+    uint32_t* deg2 = ensure<uint32_t>(ctx, ctx->deg, nn);
+    uint32_t* off2 = ensure<uint32_t>(ctx, ctx->off2, nn + 1);
+    uint32_t* cur2 = ensure<uint32_t>(ctx, ctx->cursor, nn);
+    int2* adj2 = ensure<int2>(ctx, ctx->adj2, 4 * nq2);
+    unsigned int* bad = ensure<unsigned int>(ctx, ctx->rbad, 1);
+    HIPC(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
+    if (m.nv > 0) {
+        for (int pass = 0; pass < 2; pass++)
+            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(m.nv)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
+                               deg2, pass, bad);
+        check_launch(ctx, "k_remap_degree");
+    }
+    const uint64_t rows = scan(ctx, deg2, off2, nn);
+    HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off2 + nn), (int)(uint32_t)rows, 1, ctx->stream));
+    if (m.nv > 0) {
+        for (int pass = 0; pass < 2; pass++)
+            hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(m.nv)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
+                               off2, cur2, adj2, pass);
+        check_launch(ctx, "k_remap_rows");
+    }
+    std::swap(ctx->off, ctx->off2);
+    std::swap(ctx->adj, ctx->adj2);
+    ctx->csr_ready = 2;
+}
+
 void simplify3x3y(dmc_context* ctx, bool colored) {
     build_halfedges(ctx, false);
     if (colored) color_faces(ctx);
@@ -537,6 +577,7 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
                            vfirst, vsecond, nv, q2, &st->bad);
     check_launch(ctx, "k_compact_quads3x3y");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
+    remap_rows(ctx, Remap{0, em, newQ, vfirst, vsecond, nullptr, nullptr, nullptr, nv}, nn, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;
@@ -585,6 +626,7 @@ void simplify3333(dmc_context* ctx) {
                            newQ, p3333, vtwin, vflag, newv, nv, q2, &st->bad);
     check_launch(ctx, "k_compact_quads3333");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
+    remap_rows(ctx, Remap{1, rflag, newQ, vflag, nullptr, p3333, vtwin, newv, nv}, nn, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;

@@ -582,6 +582,105 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Halfedge rows of a compacted mesh from the rows of the mesh before it.
+// The compaction keeps quadrilateral f (unless dropped) with each vertex x
+// renamed to remap_vertex(f, x) -- exactly the ids k_compact_quads3x3y /
+// k_compact_quads3333 write -- so halfedge 4f+k (x -> y) becomes
+// 4 newq[f] + k (remap(f, x) -> remap(f, y)) in the row of remap(f, x).
+// A kept vertex is the only kept preimage of its new id and all its
+// surviving halfedges stay with it: its row is written with plain stores
+// (pass 0).  Removed vertices hand their surviving halfedges to other rows
+// with atomics (pass 1, after pass 0).  Row order is irrelevant to every
+// consumer (twins sort their groups; valence counts; he_v takes the max).
+// ---------------------------------------------------------------------------
+struct Remap {
+    int mode;                   // 0: 3X3Y, 1: 3333
+    const uint8_t* qdrop;       // em (3X3Y) / rflag (3333)
+    const uint32_t* newq;
+    const uint8_t* vrem;        // vfirst (3X3Y) / vflag (3333)
+    const int32_t* vsecond;     // 3X3Y: new id of kept x, target of removed x
+    const uint8_t* p3333;       // 3333
+    const int32_t* vtwin;       // 3333
+    const uint32_t* newv;       // 3333
+    int64_t nv;                 // vertices before
+};
+
+__device__ __forceinline__ int32_t remap_vertex(const Remap& m, int64_t f, int32_t x) {
+    if (m.mode == 0) {
+        if (!vid_ok(x, m.nv)) return -1;
+        if (m.vrem[x]) {
+            const int32_t t = m.vsecond[x];
+            return vid_ok(t, m.nv) ? m.vsecond[t] : -1;
+        }
+        return m.vsecond[x];
+    }
+    if (m.p3333[f] && vid_ok(x, m.nv)) x = m.vtwin[x];
+    return (vid_ok(x, m.nv) && !m.vrem[x]) ? (int32_t)m.newv[x] : -1;
+}
+
+// pass 0: degree of every new row from its kept preimage; pass 1: removed
+// vertices add theirs (and count halfedges left without a valid origin)
+__global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t* __restrict__ off,
+                                                          const int2* __restrict__ adj, int64_t nn,
+                                                          uint32_t* __restrict__ deg2, int pass,
+                                                          unsigned int* __restrict__ bad) {
+    TMC_GRID_STRIDE(x64, m.nv) {
+        const int32_t x = (int32_t)x64;
+        const bool kept = !m.vrem[x];
+        if (pass == 0) {
+            if (!kept) continue;
+            uint32_t c = 0;
+            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) c += !m.qdrop[adj[r].x >> 2];
+            deg2[m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x]] = c;
+        } else {
+            if (kept) continue;
+            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) {
+                const int64_t f = adj[r].x >> 2;
+                if (m.qdrop[f]) continue;
+                const int32_t y = remap_vertex(m, f, x);
+                if (vid_ok(y, nn)) atomicAdd(&deg2[y], 1u);
+                else atomicAdd(bad, 1u);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* __restrict__ off,
+                                                        const int2* __restrict__ adj, int64_t nn,
+                                                        const uint32_t* __restrict__ off2,
+                                                        uint32_t* __restrict__ cursor2, int2* __restrict__ adj2,
+                                                        int pass) {
+    TMC_GRID_STRIDE(x64, m.nv) {
+        const int32_t x = (int32_t)x64;
+        const bool kept = !m.vrem[x];
+        if (pass == 0) {
+            if (!kept) continue;
+            const int32_t y = m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x];
+            uint32_t p = off2[y];
+            const uint32_t p0 = p;
+            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) {
+                const int2 a = adj[r];
+                const int64_t f = a.x >> 2;
+                if (m.qdrop[f]) continue;
+                adj2[p++] = make_int2((int32_t)(4 * m.newq[f] + (a.x & 3)), remap_vertex(m, f, a.y));
+            }
+            cursor2[y] = p - p0;
+        } else {
+            if (kept) continue;
+            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) {
+                const int2 a = adj[r];
+                const int64_t f = a.x >> 2;
+                if (m.qdrop[f]) continue;
+                const int32_t y = remap_vertex(m, f, x);
+                if (!vid_ok(y, nn)) continue;
+                const uint32_t slot = atomicAdd(&cursor2[y], 1u);
+                adj2[off2[y] + slot] = make_int2((int32_t)(4 * m.newq[f] + (a.x & 3)), remap_vertex(m, f, a.y));
+            }
+        }
+    }
+}
+
 // collectTriangles (cpp_dmc/DualMarchingCubes.cpp:1387-1460) with minAngle
 // (DualMarchingCubes.h:746-756): split along the diagonal whose two
 // triangles have the larger minimum angle.
