@@ -148,7 +148,9 @@ dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions
  * synchronous dualMC call).  A server owns one context and a worker thread:
  * waiting requests are admitted oldest first while their estimated working
  * set plus the results still held fits budget_bytes (and at most max_batch
- * per pass), run as one dmc_extract_batch, and completed together.  Results
+ * volumes and max_batch_points grid points per pass, the first request of a
+ * pass always admitted), run as one dmc_extract_batch, and completed
+ * together.  Results
  * stay in device memory (pooled blocks) until dmc_server_release.  With
  * max_batch = 1 the same loop serves one request at a time.
  * ---------------------------------------------------------------------- */
@@ -160,6 +162,7 @@ typedef struct {
     uint32_t flags;          /* extraction flags for every request (DMC_FIELD_ON_DEVICE: device fields) */
     int32_t max_batch;       /* volumes per pass, >= 1 */
     size_t budget_bytes;     /* admission budget; 0 = 96 GiB */
+    int64_t max_batch_points;  /* grid points per pass (bounds a pass's duration); 0 = no limit */
 } dmc_server_opts;
 
 typedef struct {

@@ -52,6 +52,20 @@ __global__ void __launch_bounds__(kBlock) k_he_scatter(const int32_t* __restrict
 
 __device__ __forceinline__ int32_t he_dest(const int32_t* __restrict__ quads, int32_t h) { return quads[he_next(h)]; }
 
+// Visit the entries of CSR row [s, e) four at a time: the four loads of a
+// chunk are issued together (indices clamped into the row), then used.
+template <class F>
+__device__ __forceinline__ void for_row(const int2* __restrict__ adj, uint32_t s, uint32_t e, F&& fn) {
+    for (uint32_t x = s; x < e; x += 4) {
+        int2 r[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = adj[min(x + (uint32_t)i, e - 1)];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (x + (uint32_t)i < e) fn(r[i]);
+    }
+}
+
 // Keep the 5 smallest members in ascending order (unrolled insertion).
 __device__ __forceinline__ void group_insert(int32_t g[5], int32_t h) {
 #pragma unroll
@@ -74,21 +88,20 @@ __device__ __forceinline__ int he_group(const int32_t* __restrict__ quads, const
     const int32_t a = quads[h], b = he_dest(quads, h);
     if (!vid_ok(a, nv) || !vid_ok(b, nv)) return 0;
     int c = 0;
-    for (uint32_t x = off[a], e = off[a + 1]; x < e; x++) {
-        const int2 r = adj[x];
+    const uint32_t sa = off[a], ea = off[a + 1], sb = off[b], eb = off[b + 1];
+    for_row(adj, sa, ea, [&](int2 r) {
         if (r.y == b) {
             group_insert(g, r.x);
             c++;
         }
-    }
+    });
     if (b != a)
-        for (uint32_t x = off[b], e = off[b + 1]; x < e; x++) {
-            const int2 r = adj[x];
+        for_row(adj, sb, eb, [&](int2 r) {
             if (r.y == a) {
                 group_insert(g, r.x);
                 c++;
             }
-        }
+        });
     return c;
 }
 
@@ -217,10 +230,8 @@ __global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__
                                                      const int32_t* __restrict__ twin, int64_t nv,
                                                      int32_t* __restrict__ val, unsigned int* __restrict__ zero) {
     TMC_GRID_STRIDE(v, nv) {
-        const int2* p = adj + off[v];
-        const int n = (int)(off[v + 1] - off[v]);
         int s = 0;
-        for (int x = 0; x < n; x++) s += 1 + (twin[he_prev(p[x].x)] == -1);
+        for_row(adj, off[v], off[v + 1], [&](int2 r) { s += 1 + (twin[he_prev(r.x)] == -1); });
         val[v] = s;
         if (s == 0) atomicAdd(zero, 1u);
     }

@@ -227,6 +227,7 @@ void worker_main(dmc_server* s) {
             // admission: oldest first, while the working set plus the held
             // results fit the budget (a request that never fits fails)
             size_t used = 0;
+            int64_t points = 0;
             while (!s->queue.empty() && (int)batch.size() < s->opts.max_batch) {
                 Request* r = s->queue.front();
                 if (r->est > s->opts.budget_bytes) {
@@ -237,9 +238,12 @@ void worker_main(dmc_server* s) {
                     s->cv_done.notify_all();
                     continue;
                 }
+                const int64_t pts = (int64_t)r->vol.grid.nx * r->vol.grid.ny * r->vol.grid.nz;
                 if (!batch.empty() && used + r->est + s->held_bytes > s->opts.budget_bytes) break;
+                if (!batch.empty() && s->opts.max_batch_points > 0 && points + pts > s->opts.max_batch_points) break;
                 batch.push_back(r);
                 used += r->est;
+                points += pts;
                 s->queue.pop_front();
             }
         }
