@@ -50,7 +50,7 @@ struct dmc_context {
     std::string err;
     std::vector<Buf*> bufs;
     // cell sweep
-    Buf segCount, segBase, lists, acell, acode, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
+    Buf segCount, segBase, segMask, lists, acell, acode, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
     Buf field, grads, fails, scanPartial, scanTotal;
     // volume table of the last extraction (one volume, or a batch) and the
     // per-volume first active cell / vertex / quadrilateral, kept through
@@ -79,7 +79,7 @@ struct dmc_context {
     std::vector<std::pair<const char*, float>> stage_ms;
 
     dmc_context() {
-        Buf* all[] = {&segCount, &segBase, &lists, &acell, &acode, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
+        Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
                       &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
@@ -257,13 +257,15 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint32_t* segCount = ensure<uint32_t>(ctx, ctx->segCount, RS);
     uint32_t* segBase = ensure<uint32_t>(ctx, ctx->segBase, RS);
     uint32_t* lists = ensure<uint32_t>(ctx, ctx->lists, (size_t)RS * kSeg);
+    unsigned long long* segMask = ensure<unsigned long long>(ctx, ctx->segMask, 4 * (size_t)RS);
     mark(ctx, "sweep");
     const int64_t sblocks = (W + kBlock / kWave - 1) / (kBlock / kWave);
     const dim3 sgrid((unsigned)((sblocks + kXcds - 1) / kXcds * kXcds));
     if (vec)
-        hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount);
+        hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount, segMask);
     else
-        hipLaunchKernelGGL((k_sweep<T, false>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount);
+        hipLaunchKernelGGL((k_sweep<T, false>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount,
+                           segMask);
     check_launch(ctx, "k_sweep");
     mark(ctx, "rows");
     const uint64_t A = scan(ctx, segCount, segBase, RS);
@@ -320,8 +322,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + V), (int)(uint32_t)rows, 1, ctx->stream));
     if (A > 0) {
         hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc, aqloc,
-                           rowBaseV, rowBaseQ, (int64_t)A, lists, segCount, segBase, quads, colors, off, adj,
-                           fails + 5);
+                           rowBaseV, rowBaseQ, (int64_t)A, segMask, segBase, quads, colors, off, adj, fails + 5);
         check_launch(ctx, "k_quads");
     }
     ctx->csr_ready = 1;

@@ -111,7 +111,8 @@ __device__ __forceinline__ uint32_t row_bits(const T v[4], T edge, T iso_t, int 
 
 template <class T, bool VEC>
 __global__ void __launch_bounds__(kBlock) k_sweep(VolTable vt, int64_t waves, uint32_t* __restrict__ lists,
-                                                   uint32_t* __restrict__ segCount) {
+                                                   uint32_t* __restrict__ segCount,
+                                                   unsigned long long* __restrict__ segMask) {
     constexpr int J = kSweepRows;
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform from here on (kept in scalar registers)
@@ -127,6 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(VolTable vt, int64_t waves, ui
     const int64_t wave = gwave - vt.waveBase[v];
     lists += V.rsBase * kSeg;
     segCount += V.rsBase;
+    segMask += 4 * V.rsBase;
     const int s = (int)(wave % S);
     const int64_t rest = wave / S;
     const int tj = (int)(rest % ntj), tk = (int)(rest / ntj);
@@ -193,6 +195,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(VolTable vt, int64_t waves, ui
                     }
                 }
                 if (lane == 0) segCount[r * S + s] = total;
+                // activity of the segment: bit L of word q <-> cell x0 + 4L + q
+                if (lane < 4) segMask[4 * (r * S + s) + lane] = lane == 0 ? b0 : lane == 1 ? b1 : lane == 2 ? b2 : b3;
             }
         }
 #pragma unroll
@@ -477,25 +481,27 @@ __global__ void __launch_bounds__(kBlock) k_vertices(VolTable vt, const uint32_t
 // list of its (row, segment).  The cell is active whenever one of its edges
 // is cut; -1 is returned (and counted by the caller) if it is not found.
 struct SegLists {
-    const uint32_t* lists;
-    const uint32_t* count;
-    const uint32_t* base;
+    const unsigned long long* mask;   // 4 words per (row, segment): bit L of word q <-> cell 4L + q
+    const uint32_t* base;             // first active index of each (row, segment)
     int S;
-    int64_t rsBase;   // first (row, segment) of the volume
+    int64_t rsBase;                   // first (row, segment) of the volume
 };
 
+// Position of cell (i, j, k) in the active arrays from its segment's
+// activity words: the active cells before it are those of lower lanes (any
+// word) and of its own lane in lower words.  -1 if the cell is not active.
 __device__ __forceinline__ int64_t find_active(const SegLists& L, const Dims& d, int i, int j, int k) {
     const int64_t rs = L.rsBase + ((int64_t)k * (d.ny - 1) + j) * L.S + (i / kSeg);
-    const uint32_t* list = L.lists + rs * kSeg;
-    const uint32_t n = L.count[rs];
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if ((list[mid] & 0xFFFFFFu) < (uint32_t)i) lo = mid + 1;
-        else hi = mid;
-    }
-    if (lo >= n || (list[lo] & 0xFFFFFFu) != (uint32_t)i) return -1;
-    return (int64_t)L.base[rs] + lo;
+    const int il = i % kSeg, ln = il >> 2, q = il & 3;
+    const ulonglong2* m2 = reinterpret_cast<const ulonglong2*>(L.mask + 4 * rs);
+    const ulonglong2 w01 = m2[0], w23 = m2[1];
+    const unsigned long long w[4] = {w01.x, w01.y, w23.x, w23.y};
+    if (!((w[q] >> ln) & 1ull)) return -1;
+    const unsigned long long below = (1ull << ln) - 1ull;
+    int rank = 0;
+#pragma unroll
+    for (int t = 0; t < 4; t++) rank += __popcll(w[t] & below) + (t < q ? (int)((w[t] >> ln) & 1ull) : 0);
+    return (int64_t)L.base[rs] + rank;
 }
 
 // Final id of the vertex of active cell `a` = (i, j, k) whose contour
@@ -543,8 +549,7 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
                                                    const uint32_t* __restrict__ aqloc,
                                                    const uint32_t* __restrict__ rowBaseV0,
                                                    const uint32_t* __restrict__ rowBaseQ, int64_t A,
-                                                   const uint32_t* __restrict__ lists,
-                                                   const uint32_t* __restrict__ segCount,
+                                                   const unsigned long long* __restrict__ segMask,
                                                    const uint32_t* __restrict__ segBase,
                                                    int32_t* __restrict__ quads, int32_t* __restrict__ colors,
                                                    const uint32_t* __restrict__ off, int2* __restrict__ adj,
@@ -553,7 +558,7 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
         const int vol = vol_of(vt.aBase, vt.n, a);
         const VolDesc& V = vt.desc[vol];
         const Dims d = V.d;
-        const SegLists L{lists, segCount, segBase, V.S, V.rsBase};
+        const SegLists L{segMask, segBase, V.S, V.rsBase};
         const uint32_t* __restrict__ rowBaseV = rowBaseV0 + V.rowBase;
         int i, j, k;
         uint32_t r;

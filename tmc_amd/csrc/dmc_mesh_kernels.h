@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ of
                                                   int64_t nv, int32_t* __restrict__ he_v) {
     TMC_GRID_STRIDE(v, nv) {
         int32_t last = -1;
-        for (uint32_t x = off[v], e = off[v + 1]; x < e; x++) last = adj[x].x > last ? adj[x].x : last;
+        for_row(adj, off[v], off[v + 1], [&](int2 r) { last = r.x > last ? r.x : last; });
         he_v[v] = last;
     }
 }
@@ -631,7 +631,9 @@ __device__ __forceinline__ int32_t remap_vertex(const Remap& m, int64_t f, int32
 }
 
 // pass 0: degree of every new row from its kept preimage; pass 1: removed
-// vertices add theirs (and count halfedges left without a valid origin)
+// vertices add theirs (and count halfedges left without a valid origin).
+// Rows are walked four entries at a time with every lookup of a chunk
+// issued before any is used.
 __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t* __restrict__ off,
                                                           const int2* __restrict__ adj, int64_t nn,
                                                           uint32_t* __restrict__ deg2, int pass,
@@ -639,21 +641,29 @@ __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t
     TMC_GRID_STRIDE(x64, m.nv) {
         const int32_t x = (int32_t)x64;
         const bool kept = !m.vrem[x];
-        if (pass == 0) {
-            if (!kept) continue;
-            uint32_t c = 0;
-            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) c += !m.qdrop[adj[r].x >> 2];
-            deg2[m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x]] = c;
-        } else {
-            if (kept) continue;
-            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) {
-                const int64_t f = adj[r].x >> 2;
-                if (m.qdrop[f]) continue;
-                const int32_t y = remap_vertex(m, f, x);
-                if (vid_ok(y, nn)) atomicAdd(&deg2[y], 1u);
-                else atomicAdd(bad, 1u);
+        if ((pass == 0) != kept) continue;
+        const uint32_t s = off[x], e = off[x + 1];
+        uint32_t c = 0;
+        for (uint32_t r = s; r < e; r += 4) {
+            int32_t h[4];
+            uint8_t dr[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) h[i] = adj[min(r + (uint32_t)i, e - 1)].x;
+#pragma unroll
+            for (int i = 0; i < 4; i++) dr[i] = m.qdrop[h[i] >> 2];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (r + (uint32_t)i >= e || dr[i]) continue;
+                if (pass == 0) {
+                    c++;
+                } else {
+                    const int32_t y = remap_vertex(m, h[i] >> 2, x);
+                    if (vid_ok(y, nn)) atomicAdd(&deg2[y], 1u);
+                    else atomicAdd(bad, 1u);
+                }
             }
         }
+        if (pass == 0) deg2[m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x]] = c;
     }
 }
 
@@ -665,30 +675,40 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
     TMC_GRID_STRIDE(x64, m.nv) {
         const int32_t x = (int32_t)x64;
         const bool kept = !m.vrem[x];
-        if (pass == 0) {
-            if (!kept) continue;
-            const int32_t y = m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x];
-            uint32_t p = off2[y];
-            const uint32_t p0 = p;
-            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) {
-                const int2 a = adj[r];
-                const int64_t f = a.x >> 2;
-                if (m.qdrop[f]) continue;
-                adj2[p++] = make_int2((int32_t)(4 * m.newq[f] + (a.x & 3)), remap_vertex(m, f, a.y));
+        if ((pass == 0) != kept) continue;
+        const uint32_t s = off[x], e = off[x + 1];
+        const int32_t y0 = kept ? (m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x]) : -1;
+        uint32_t p = kept ? off2[y0] : 0u;
+        const uint32_t p0 = p;
+        for (uint32_t r = s; r < e; r += 4) {
+            int2 a[4];
+            uint8_t dr[4];
+            uint32_t nq[4];
+            int32_t dst[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = adj[min(r + (uint32_t)i, e - 1)];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                dr[i] = m.qdrop[a[i].x >> 2];
+                nq[i] = m.newq[a[i].x >> 2];
             }
-            cursor2[y] = p - p0;
-        } else {
-            if (kept) continue;
-            for (uint32_t r = off[x], e = off[x + 1]; r < e; r++) {
-                const int2 a = adj[r];
-                const int64_t f = a.x >> 2;
-                if (m.qdrop[f]) continue;
-                const int32_t y = remap_vertex(m, f, x);
-                if (!vid_ok(y, nn)) continue;
-                const uint32_t slot = atomicAdd(&cursor2[y], 1u);
-                adj2[off2[y] + slot] = make_int2((int32_t)(4 * m.newq[f] + (a.x & 3)), remap_vertex(m, f, a.y));
+#pragma unroll
+            for (int i = 0; i < 4; i++) dst[i] = remap_vertex(m, a[i].x >> 2, a[i].y);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (r + (uint32_t)i >= e || dr[i]) continue;
+                const int2 out = make_int2((int32_t)(4 * nq[i] + (a[i].x & 3)), dst[i]);
+                if (pass == 0) {
+                    adj2[p++] = out;
+                } else {
+                    const int32_t y = remap_vertex(m, a[i].x >> 2, x);
+                    if (!vid_ok(y, nn)) continue;
+                    const uint32_t slot = atomicAdd(&cursor2[y], 1u);
+                    adj2[off2[y] + slot] = out;
+                }
             }
         }
+        if (pass == 0) cursor2[y0] = p - p0;
     }
 }
 
