@@ -233,19 +233,22 @@ struct BoxRef {
 };
 
 // cpp_dmc/DualMarchingCubes.h:590-597
-__device__ __forceinline__ void trilinear_gradient(const double f[8], double u, double v, double w, double g[3]) {
-    g[0] = (1 - w) * ((1 - v) * (f[1] - f[0]) + v * (f[3] - f[2])) + w * ((1 - v) * (f[5] - f[4]) + v * (f[7] - f[6]));
-    g[1] = (1 - w) * ((1 - u) * (f[2] - f[0]) + u * (f[3] - f[1])) + w * ((1 - u) * (f[6] - f[4]) + u * (f[7] - f[5]));
-    g[2] = (1 - v) * ((1 - u) * (f[4] - f[0]) + u * (f[5] - f[1])) + v * ((1 - u) * (f[6] - f[2]) + u * (f[7] - f[3]));
+__device__ __forceinline__ void trilinear_gradient(const CornerRef& f, double u, double v, double w, double g[3]) {
+    g[0] = (1 - w) * ((1 - v) * (f(1) - f(0)) + v * (f(3) - f(2))) + w * ((1 - v) * (f(5) - f(4)) + v * (f(7) - f(6)));
+    g[1] = (1 - w) * ((1 - u) * (f(2) - f(0)) + u * (f(3) - f(1))) + w * ((1 - u) * (f(6) - f(4)) + u * (f(7) - f(5)));
+    g[2] = (1 - v) * ((1 - u) * (f(4) - f(0)) + u * (f(5) - f(1))) + v * ((1 - u) * (f(6) - f(2)) + u * (f(7) - f(3)));
 }
 
 // Projection sampling (cpp_dmc/DualMarchingCubes.cpp:274-326).  The frame
-// (a, b, c) maps the projected (u, v, w) axes to cell axes; g holds the
-// eight corner values in frame order (g[0..3] = bilinear face at w = 0,
-// g[4..7] at w = 1).  Boxes of other size-3 contours are only consulted for
-// multi-contour cells (DualMarchingCubes.h:500-519).
-__device__ inline bool projection(const double g[8], const BoxRef& B, int axa, int axb, int axc, double iso, int t,
-                                  int cnt, uint64_t w, double p[3], int ns) {
+// (a, b, c) maps the projected (u, v, w) axes to cell axes; corner q of the
+// frame (q = bit0 along a, bit1 along b, bit2 along c) is cell corner
+// (cpk >> 3q) & 7 (corners 0..3 = bilinear face at w = 0, 4..7 at w = 1).
+// The blends along u depend on the row i only and are formed once per row,
+// exactly as the reference's expression groups them.  Boxes of other size-3
+// contours are only consulted for multi-contour cells
+// (DualMarchingCubes.h:500-519).
+__device__ inline bool projection(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
+                                  double iso, int t, int cnt, uint64_t w, double p[3], int ns) {
     const double umin = B.lo(axa, t), umax = B.hi(axa, t);
     const double vmin = B.lo(axb, t), vmax = B.hi(axb, t);
     const double du = (umax - umin) / (ns - 1);
@@ -264,16 +267,18 @@ __device__ inline bool projection(const double g[8], const BoxRef& B, int axa, i
 #pragma unroll
     for (int q = 0; q < 4; q++) nbmask |= (q < cnt && q != t && poly_size(w, q) == 3) ? (1u << q) : 0u;
     if (cnt == 1 || cnt_size == 3) nbmask = 0;
-    const double f0 = g[0], f1 = g[1], f2 = g[2], f3 = g[3];
-    const double f4 = g[4], f5 = g[5], f6 = g[6], f7 = g[7];
 #pragma unroll 1
     for (int i = 1; i < (ns - 1); i++) {
         const float u = umin + i * du;
+        const double u01 = (1 - u) * Fc((cpk >> 0) & 7) + u * Fc((cpk >> 3) & 7);
+        const double u23 = (1 - u) * Fc((cpk >> 6) & 7) + u * Fc((cpk >> 9) & 7);
+        const double u45 = (1 - u) * Fc((cpk >> 12) & 7) + u * Fc((cpk >> 15) & 7);
+        const double u67 = (1 - u) * Fc((cpk >> 18) & 7) + u * Fc((cpk >> 21) & 7);
 #pragma unroll 1
         for (int j = 1; j < (ns - 1); j++) {
             const float v = vmin + j * dv;
-            const float g1 = (1 - v) * ((1 - u) * f0 + u * f1) + v * ((1 - u) * f2 + u * f3);
-            const float g2 = (1 - v) * ((1 - u) * f4 + u * f5) + v * ((1 - u) * f6 + u * f7);
+            const float g1 = (1 - v) * u01 + v * u23;
+            const float g2 = (1 - v) * u45 + v * u67;
             if (g1 == g2) continue;
             const float ww = (iso - g1) / (g2 - g1);
             if (wmin <= ww && ww <= wmax) {
@@ -312,8 +317,8 @@ __device__ inline bool projection(const double g[8], const BoxRef& B, int axa, i
 // Representative of contour t in local cell coordinates
 // (cpp_dmc/DualMarchingCubes.cpp:141-156 and :194-272).  Returns the
 // number of failed projection levels (0, 1 or 2).
-__device__ inline int contour_representative(const double F[8], const CornerRef& Fc, double iso, uint64_t w,
-                                             int cnt, int t, const BoxRef& B, double p[3]) {
+__device__ inline int contour_representative(const CornerRef& Fc, double iso, uint64_t w, int cnt, int t,
+                                             const BoxRef& B, double p[3]) {
     // mean of the edge intersections, summed in contour order
     const int start = poly_start(w, t), size = poly_size(w, t);
     p[0] = 0;
@@ -334,7 +339,7 @@ __device__ inline int contour_representative(const double F[8], const CornerRef&
     int prj;
     if (size >= 6) {
         double g[3];
-        trilinear_gradient(F, p[0], p[1], p[2], g);
+        trilinear_gradient(Fc, p[0], p[1], p[2], g);
         const double dx = fabs(g[0]), dy = fabs(g[1]), dz = fabs(g[2]);
         if (dz >= dx && dz >= dy) prj = 1;
         else if (dy >= dx && dy >= dz) prj = 2;
@@ -353,20 +358,18 @@ __device__ inline int contour_representative(const double F[8], const CornerRef&
     const int axa = prj == 3 ? 1 : 0;
     const int axb = prj == 1 ? 1 : 2;
     const int axc = prj == 1 ? 2 : prj == 2 ? 1 : 0;
-    double g[8];
+    uint32_t cpk = 0;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int corner = ((q & 1) << axa) | (((q >> 1) & 1) << axb) | (((q >> 2) & 1) << axc);
-        g[q] = Fc(corner);
-    }
+    for (int q = 0; q < 8; q++)
+        cpk |= (uint32_t)(((q & 1) << axa) | (((q >> 1) & 1) << axb) | (((q >> 2) & 1) << axc)) << (3 * q);
     double pt[3];
     pt[0] = axa == 0 ? p[0] : p[1];
     pt[1] = axb == 1 ? p[1] : p[2];
     pt[2] = axc == 2 ? p[2] : axc == 1 ? p[1] : p[0];
     int fails = 0;
-    if (!projection(g, B, axa, axb, axc, iso, t, cnt, w, pt, 9)) {
+    if (!projection(Fc, cpk, B, axa, axb, axc, iso, t, cnt, w, pt, 9)) {
         fails = 1;
-        if (!projection(g, B, axa, axb, axc, iso, t, cnt, w, pt, 28)) fails = 2;
+        if (!projection(Fc, cpk, B, axa, axb, axc, iso, t, cnt, w, pt, 28)) fails = 2;
     }
     double q0 = p[0], q1 = p[1], q2 = p[2];
     if (axa == 0) q0 = pt[0]; else q1 = pt[0];

@@ -314,6 +314,9 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
         hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc,
                            rowBaseV, (int64_t)A, pos, nrm, deg, vstat);
         check_launch(ctx, "k_vertices");
+        hipLaunchKernelGGL(k_normals<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc,
+                           rowBaseV, (int64_t)A, nrm);
+        check_launch(ctx, "k_normals");
     }
     mark(ctx, "quads");
     const uint64_t rows = scan(ctx, deg, off, V);

@@ -82,6 +82,9 @@ constexpr int kSeg = 256;           // cells per segment (one wave) and list slo
 #define TMC_SWEEP_ROWS 16
 #endif
 constexpr int kSweepRows = TMC_SWEEP_ROWS;  // J
+#ifndef TMC_VERTICES_ATTR
+#define TMC_VERTICES_ATTR
+#endif
 
 template <class T, bool VEC>
 __device__ __forceinline__ void row_load(const T* __restrict__ row, int i0, int nx, T v[4]) {
@@ -385,7 +388,7 @@ __device__ __forceinline__ void point_gradient(const T* __restrict__ field, cons
 // contour, as the reference does.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
+__global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
                                                       const uint32_t* __restrict__ avloc,
                                                       const uint32_t* __restrict__ rowBaseV, int64_t A,
@@ -410,76 +413,141 @@ __global__ void __launch_bounds__(kBlock) k_vertices(VolTable vt, const uint32_t
         cell_ijk(acell[a], d, i, j, k, r);
         const uint32_t code = acode[a];
         const unsigned icase = code & 0xFFu;
-        double F[8];
-        load_corners(field, d, i, j, k, F);
-        Fc.store(F);
-        const uint64_t w = cell_polygons(icase, iso, F);
+        uint64_t w;
+        {
+            double F[8];
+            load_corners(field, d, i, j, k, F);
+            Fc.store(F);
+            w = cell_polygons(icase, iso, F);
+        }
         const int cnt = poly_count(w);
         cell_boxes(Fc, iso, w, cnt, B);
         const unsigned used = used_contours(icase, code >> 8, i, j, k, d);
         uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
         for (int t = 0; t < cnt; t++) {
             double p[3];
-            const int fl = contour_representative(F, Fc, iso, w, cnt, t, B, p);
+            const int fl = contour_representative(Fc, iso, w, cnt, t, B, p);
             // rare: counted per volume where they happen
             if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
             if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
             if (!((used >> t) & 1u)) continue;
-            // normal: trilinear blend of the corner gradients, normalised
-            const double u = p[0], v = p[1], ww = p[2];
-            double n[3] = {0, 0, 0}, lo[3], hi[3], a0[3], a1[3];
-#pragma unroll 1
-            for (int c = 0; c < 8; c++) {
-                const int ci = i + (c & 1), cj = j + ((c >> 1) & 1), ck = k + ((c >> 2) & 1);
-                double G[3];
-                if (grads) {
-                    const int64_t q = ((int64_t)ck * d.ny + cj) * d.nx + ci;
-                    G[0] = grads[3 * q];
-                    G[1] = grads[3 * q + 1];
-                    G[2] = grads[3 * q + 2];
-                } else {
-                    point_gradient(field, d, ci, cj, ck, dx2, dy2, dz2, G);
-                }
-                // (1-u) n_even + u n_odd, then rows by v, then planes by w
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    if ((c & 1) == 0) a0[q] = G[q] * (1 - u);
-                    else a1[q] = a0[q] + G[q] * u;
-                }
-                if (c & 1) {
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        if ((c & 2) == 0) lo[q] = a1[q] * (1 - v);
-                        else hi[q] = lo[q] + a1[q] * v;
-                    }
-                    if (c & 2) {
-#pragma unroll
-                        for (int q = 0; q < 3; q++) {
-                            if ((c & 4) == 0) n[q] = hi[q] * (1 - ww);
-                            else n[q] = n[q] + hi[q] * ww;
-                        }
-                    }
-                }
-            }
-            const double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            n[0] /= s;
-            n[1] /= s;
-            n[2] /= s;
-            vpos[3 * (int64_t)out + 0] = geo.x0 + (i + u) * geo.dx;
-            vpos[3 * (int64_t)out + 1] = geo.y0 + (j + v) * geo.dy;
-            vpos[3 * (int64_t)out + 2] = geo.z0 + (k + ww) * geo.dz;
-            vnrm[3 * (int64_t)out + 0] = n[0];
-            vnrm[3 * (int64_t)out + 1] = n[1];
-            vnrm[3 * (int64_t)out + 2] = n[2];
+            // the normal is formed from these local coordinates by k_normals
+            vpos[3 * (int64_t)out + 0] = geo.x0 + (i + p[0]) * geo.dx;
+            vpos[3 * (int64_t)out + 1] = geo.y0 + (j + p[1]) * geo.dy;
+            vpos[3 * (int64_t)out + 2] = geo.z0 + (k + p[2]) * geo.dz;
+            vnrm[3 * (int64_t)out + 0] = p[0];
+            vnrm[3 * (int64_t)out + 1] = p[1];
+            vnrm[3 * (int64_t)out + 2] = p[2];
             vdeg[out] = (uint32_t)__popc(contour_edges(icase, code >> 8, (unsigned)t) & interior_edges(i, j, k, d));
             out++;
         }
     }
 }
 
-// Position of cell (i, j, k) in the active arrays: binary search in the
-// list of its (row, segment).  The cell is active whenever one of its edges
-// is cut; -1 is returned (and counted by the caller) if it is not found.
+// ---------------------------------------------------------------------------
+// K3b: normals (cpp_dmc/UniformGrid.h:286-293): trilinear blend of the eight
+// corner gradients at the vertex's local coordinates, normalised.  One
+// thread per active cell: the 32 field values the cell's corner gradients
+// need (its 8 corners and their +-1 neighbours along each axis, with
+// estimateGradient's index rule) are loaded together, the 24 gradient
+// components formed once, and every used contour of the cell blended from
+// them.  k_vertices left each contour's local coordinates where its normal
+// goes.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_normals(VolTable vt, const uint32_t* __restrict__ acell,
+                                                     const uint32_t* __restrict__ acode,
+                                                     const uint32_t* __restrict__ avloc,
+                                                     const uint32_t* __restrict__ rowBaseV, int64_t A,
+                                                     double* __restrict__ vnrm) {
+    for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
+        const int vol = vol_of(vt.aBase, vt.n, a);
+        const VolDesc& V = vt.desc[vol];
+        const T* __restrict__ field = static_cast<const T*>(V.field);
+        const double* __restrict__ grads = V.grads;
+        const Dims d = V.d;
+        const double dx2 = 2 * V.geo.dx, dy2 = 2 * V.geo.dy, dz2 = 2 * V.geo.dz;
+        int i, j, k;
+        uint32_t r;
+        cell_ijk(acell[a], d, i, j, k, r);
+        const uint32_t code = acode[a];
+        const unsigned used = used_contours(code & 0xFFu, code >> 8, i, j, k, d);
+        // G[c][axis] of corner c = (c&1, c>>1&1, c>>2&1), flipped (x -1)
+        double G[8][3];
+        if (grads) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int64_t q = ((int64_t)(k + ((c >> 2) & 1)) * d.ny + (j + ((c >> 1) & 1))) * d.nx + i + (c & 1);
+                G[c][0] = grads[3 * q];
+                G[c][1] = grads[3 * q + 1];
+                G[c][2] = grads[3 * q + 2];
+            }
+        } else {
+            // index rule of estimateGradient: -1 -> 0, n -> 0 (the top
+            // boundary wraps), cpp_dmc/UniformGrid.cpp:158-161
+            auto idx = [](int x, int m) { return x < 0 ? 0 : (x >= m ? m - x : x); };
+            const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
+            const int xs[4] = {idx(i - 1, d.nx), i, i + 1, idx(i + 2, d.nx)};
+            const int ys[4] = {idx(j - 1, d.ny), j, j + 1, idx(j + 2, d.ny)};
+            const int zs[4] = {idx(k - 1, d.nz), k, k + 1, idx(k + 2, d.nz)};
+            // X[b][c]: points (xs[c], j + b&1, k + b>>1); Y[b][c]: (i + b&1, ys[c], k + b>>1);
+            // Z[b][c]: (i + b&1, j + b>>1, zs[c]); c = 0 and 3 are the outer neighbours
+            double X[4][4], Y[4][2], Z[4][2];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int64_t row = (int64_t)(k + (b >> 1)) * sz + (int64_t)(j + (b & 1)) * sy;
+#pragma unroll
+                for (int c = 0; c < 4; c++) X[b][c] = (double)field[row + xs[c]];
+                const int64_t colY = (int64_t)(k + (b >> 1)) * sz + i + (b & 1);
+                Y[b][0] = (double)field[colY + (int64_t)ys[0] * sy];
+                Y[b][1] = (double)field[colY + (int64_t)ys[3] * sy];
+                const int64_t colZ = (int64_t)(j + (b >> 1)) * sy + i + (b & 1);
+                Z[b][0] = (double)field[colZ + (int64_t)zs[0] * sz];
+                Z[b][1] = (double)field[colZ + (int64_t)zs[3] * sz];
+            }
+            const double s = -1.;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+                // x: (xs[cx+2] - xs[cx]) in row (cy, cz)
+                const double* xr = X[cy | cz << 1];
+                const double gx = (xr[cx + 2] - xr[cx]) / dx2;
+                // y: at (cx, cz): ys[cy+2] - ys[cy]; rows j, j+1 are corners
+                const double yhi = cy ? Y[cx | cz << 1][1] : X[1 | cz << 1][1 + cx];
+                const double ylo = cy ? X[0 | cz << 1][1 + cx] : Y[cx | cz << 1][0];
+                const double gy = (yhi - ylo) / dy2;
+                // z: at (cx, cy): zs[cz+2] - zs[cz]; planes k, k+1 are corners
+                const double zhi = cz ? Z[cx | cy << 1][1] : X[cy | 2][1 + cx];
+                const double zlo = cz ? X[cy][1 + cx] : Z[cx | cy << 1][0];
+                const double gz = (zhi - zlo) / dz2;
+                G[c][0] = gx * s;
+                G[c][1] = gy * s;
+                G[c][2] = gz * s;
+            }
+        }
+        uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
+        for (unsigned um = used; um; um &= um - 1, out++) {
+            double* nv = vnrm + 3 * (int64_t)out;
+            const double u = nv[0], v = nv[1], ww = nv[2];
+            double n[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                // (1-u) n_even + u n_odd, then rows by v, then planes by w
+                const double lo0 = (G[0][q] * (1 - u) + G[1][q] * u) * (1 - v);
+                const double hi0 = lo0 + (G[2][q] * (1 - u) + G[3][q] * u) * v;
+                const double lo1 = (G[4][q] * (1 - u) + G[5][q] * u) * (1 - v);
+                const double hi1 = lo1 + (G[6][q] * (1 - u) + G[7][q] * u) * v;
+                n[q] = hi0 * (1 - ww) + hi1 * ww;
+            }
+            const double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            nv[0] = n[0] / s;
+            nv[1] = n[1] / s;
+            nv[2] = n[2] / s;
+        }
+    }
+}
+
+// Activity of the cells of one volume, for neighbour lookups.
 struct SegLists {
     const unsigned long long* mask;   // 4 words per (row, segment): bit L of word q <-> cell 4L + q
     const uint32_t* base;             // first active index of each (row, segment)
