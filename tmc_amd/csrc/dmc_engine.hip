@@ -50,7 +50,7 @@ struct dmc_context {
     std::string err;
     std::vector<Buf*> bufs;
     // cell sweep
-    Buf segCount, segBase, segMask, lists, acell, acode, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
+    Buf segCount, segBase, segMask, lists, acell, acode, aaux, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
     Buf field, grads, fails, scanPartial, scanTotal;
     // volume table of the last extraction (one volume, or a batch) and the
     // per-volume first active cell / vertex / quadrilateral, kept through
@@ -79,7 +79,7 @@ struct dmc_context {
     std::vector<std::pair<const char*, float>> stage_ms;
 
     dmc_context() {
-        Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
+        Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &aaux, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
                       &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
@@ -275,6 +275,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
                        vol_ptr(ctx, 2));
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
+    uint32_t* aaux = ensure<uint32_t>(ctx, ctx->aaux, A);
     uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, A);
     uint32_t* aqloc = ensure<uint32_t>(ctx, ctx->aqloc, A);
     uint32_t* rowV = ensure<uint32_t>(ctx, ctx->rowV, R);
@@ -286,7 +287,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint32_t* fails = ensure<uint32_t>(ctx, ctx->fails, 6);
     HIPC(ctx, hipMemsetAsync(fails, 0, 6 * sizeof(uint32_t), ctx->stream));
     hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, vt, R, lists,
-                       segCount, segBase, acell, acode, avloc, aqloc, rowV, rowQ, vstat);
+                       segCount, segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, vstat);
     check_launch(ctx, "k_rows");
     const uint64_t V = scan(ctx, rowV, rowBaseV, R);
     const uint64_t Q = scan(ctx, rowQ, rowBaseQ, R);
@@ -311,10 +312,10 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     int2* adj = ensure<int2>(ctx, ctx->adj, 4 * Q);
     mark(ctx, "vertices");
     if (A > 0) {
-        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc,
-                           rowBaseV, (int64_t)A, pos, nrm, deg, vstat);
+        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux,
+                           avloc, rowBaseV, (int64_t)A, pos, nrm, deg, vstat);
         check_launch(ctx, "k_vertices");
-        hipLaunchKernelGGL(k_normals<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc,
+        hipLaunchKernelGGL(k_normals<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, aaux, avloc,
                            rowBaseV, (int64_t)A, nrm);
         check_launch(ctx, "k_normals");
     }
@@ -324,7 +325,8 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
                             (unsigned long long)(4 * Q));
     HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + V), (int)(uint32_t)rows, 1, ctx->stream));
     if (A > 0) {
-        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, avloc, aqloc,
+        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux, avloc,
+                           aqloc,
                            rowBaseV, rowBaseQ, (int64_t)A, segMask, segBase, quads, colors, off, adj, fails + 5);
         check_launch(ctx, "k_quads");
     }

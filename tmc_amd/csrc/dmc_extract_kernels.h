@@ -229,34 +229,54 @@ __device__ __forceinline__ void cell_ijk(uint32_t c, const Dims& d, int& i, int&
     k = (int)(r / (uint32_t)(d.ny - 1));
 }
 
-// Used contours: those with at least one interior edge (the others only
-// feed incomplete quadrilaterals and are dropped by removeUnusedVertices,
-// cpp_dmc/DualMarchingCubes.h:685-729).
-__device__ __forceinline__ unsigned used_contours(unsigned icase, uint32_t map, int i, int j, int k, const Dims& d) {
-    unsigned used = 0;
-#pragma unroll
-    for (int e = 0; e < 12; e++)
-        if (edge_cut(icase, e) && edge_interior(e, i, j, k, d.nx, d.ny, d.nz)) used |= 1u << ((map >> (2 * e)) & 3u);
-    return used;
-}
+// Edge masks in "spread" form: edge e at bit 2e, the layout of the contour
+// map (2 bits per edge).  Cut edges of each case from a table; interior
+// edges of a cell away from the grid boundary are all twelve.
+struct CutSpread {
+    uint32_t v[256];
+    constexpr CutSpread() : v() {
+        for (int c = 0; c < 256; c++) {
+            uint32_t m = 0;
+            for (int e = 0; e < 12; e++)
+                if (((c >> kEdgeV0[e]) ^ (c >> kEdgeV1[e])) & 1) m |= 1u << (2 * e);
+            v[c] = m;
+        }
+    }
+};
+__device__ __constant__ CutSpread c_cut_spread{};
+constexpr uint32_t kAllEdgesSpread = 0x555555u;
 
-// Edges of contour t (cut edges whose polygon is t), and the interior ones
-// of cell (i,j,k), as 12-bit masks.  A vertex's quadrilaterals are those of
-// the interior edges of its contour, one each: its degree in the mesh and,
-// ordered by edge id, its rows in the first halfedge CSR.
-__device__ __forceinline__ unsigned contour_edges(unsigned icase, uint32_t map, unsigned t) {
-    unsigned m = 0;
+__device__ __forceinline__ uint32_t interior_spread(int i, int j, int k, const Dims& d) {
+    if (i >= 1 && i <= d.nx - 3 && j >= 1 && j <= d.ny - 3 && k >= 1 && k <= d.nz - 3) return kAllEdgesSpread;
+    uint32_t m = 0;
 #pragma unroll
     for (int e = 0; e < 12; e++)
-        if (edge_cut(icase, e) && ((map >> (2 * e)) & 3u) == t) m |= 1u << e;
+        if (edge_interior(e, i, j, k, d.nx, d.ny, d.nz)) m |= 1u << (2 * e);
     return m;
 }
-__device__ __forceinline__ unsigned interior_edges(int i, int j, int k, const Dims& d) {
-    unsigned m = 0;
+// bit 2e set iff edge e belongs to contour t in the map
+__device__ __forceinline__ uint32_t contour_spread(uint32_t map, unsigned t) {
+    const uint32_t x = map ^ (t * kAllEdgesSpread);
+    return ~(x | (x >> 1)) & kAllEdgesSpread;
+}
+// Per active cell: interior cut edges (spread, bits 0..23) and used
+// contours (bits 24..27), formed once in k_rows.  Used contours are those
+// with at least one interior edge; the others only feed incomplete
+// quadrilaterals and are dropped by removeUnusedVertices
+// (cpp_dmc/DualMarchingCubes.h:685-729).  A vertex's quadrilaterals are
+// those of its contour's interior edges, one each: its degree and, ordered
+// by edge id, its rows in the first halfedge CSR.
+__device__ __forceinline__ uint32_t cell_aux(unsigned icase, uint32_t map, int i, int j, int k, const Dims& d) {
+    const uint32_t live = c_cut_spread.v[icase] & interior_spread(i, j, k, d);
+    uint32_t used = 0;
 #pragma unroll
-    for (int e = 0; e < 12; e++)
-        if (edge_interior(e, i, j, k, d.nx, d.ny, d.nz)) m |= 1u << e;
-    return m;
+    for (unsigned t = 0; t < 4; t++)
+        if (contour_spread(map, t) & live) used |= 1u << t;
+    return (live & kAllEdgesSpread) | used << 24;
+}
+// interior cut edges of contour t below edge e / in all: a vertex's rows
+__device__ __forceinline__ int contour_rank(uint32_t map, uint32_t aux, unsigned icase, unsigned t, int e) {
+    return __popc(contour_spread(map, t) & aux & c_cut_spread.v[icase] & kAllEdgesSpread & ((1u << (2 * e)) - 1u));
 }
 
 // ---------------------------------------------------------------------------
@@ -271,6 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                                                   const uint32_t* __restrict__ segCount,
                                                   const uint32_t* __restrict__ segBase,
                                                   uint32_t* __restrict__ acell, uint32_t* __restrict__ acode,
+                                                  uint32_t* __restrict__ aaux,
                                                   uint32_t* __restrict__ avloc, uint32_t* __restrict__ aqloc,
                                                   uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowQ,
                                                   VolStat* __restrict__ vstat) {
@@ -304,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
         for (uint32_t a0 = first; a0 < last; a0 += kWave) {
             const uint32_t a = a0 + lane;
             const bool act = a < last;
-            uint32_t nv = 0, nq = 0, code = 0;
+            uint32_t nv = 0, nq = 0, code = 0, aux = 0;
             int i = 0;
             if (act) {
                 // segment of a: the last s with segBase[rs0 + s] <= a
@@ -334,7 +355,8 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                     pos += sz;
                 }
                 total += (unsigned long long)poly_count(w);
-                nv = (uint32_t)__popc(used_contours(icase, map, i, j, k, d));
+                aux = cell_aux(icase, map, i, j, k, d);
+                nv = (uint32_t)__popc(aux >> 24);
                 nq = (uint32_t)(edge_cut(icase, 0) && j >= 1 && k >= 1) +
                      (uint32_t)(edge_cut(icase, 3) && i >= 1 && k >= 1) +
                      (uint32_t)(edge_cut(icase, 8) && i >= 1 && j >= 1);
@@ -344,6 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
             if (act) {
                 acell[a] = (uint32_t)(r * d.C) + (uint32_t)i;
                 acode[a] = code;
+                aaux[a] = aux;
                 avloc[a] = cv + iv - nv;
                 aqloc[a] = cq + iq - nq;
             }
@@ -390,6 +413,7 @@ __device__ __forceinline__ void point_gradient(const T* __restrict__ field, cons
 template <class T>
 __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
+                                                      const uint32_t* __restrict__ aaux,
                                                       const uint32_t* __restrict__ avloc,
                                                       const uint32_t* __restrict__ rowBaseV, int64_t A,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
@@ -422,7 +446,8 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
         }
         const int cnt = poly_count(w);
         cell_boxes(Fc, iso, w, cnt, B);
-        const unsigned used = used_contours(icase, code >> 8, i, j, k, d);
+        const uint32_t aux = aaux[a];
+        const unsigned used = aux >> 24;
         uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
         for (int t = 0; t < cnt; t++) {
             double p[3];
@@ -438,7 +463,7 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
             vnrm[3 * (int64_t)out + 0] = p[0];
             vnrm[3 * (int64_t)out + 1] = p[1];
             vnrm[3 * (int64_t)out + 2] = p[2];
-            vdeg[out] = (uint32_t)__popc(contour_edges(icase, code >> 8, (unsigned)t) & interior_edges(i, j, k, d));
+            vdeg[out] = (uint32_t)contour_rank(code >> 8, aux, icase, (unsigned)t, 12);
             out++;
         }
     }
@@ -456,7 +481,7 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_normals(VolTable vt, const uint32_t* __restrict__ acell,
-                                                     const uint32_t* __restrict__ acode,
+                                                     const uint32_t* __restrict__ aaux,
                                                      const uint32_t* __restrict__ avloc,
                                                      const uint32_t* __restrict__ rowBaseV, int64_t A,
                                                      double* __restrict__ vnrm) {
@@ -470,8 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_normals(VolTable vt, const uint32_t*
         int i, j, k;
         uint32_t r;
         cell_ijk(acell[a], d, i, j, k, r);
-        const uint32_t code = acode[a];
-        const unsigned used = used_contours(code & 0xFFu, code >> 8, i, j, k, d);
+        const unsigned used = aaux[a] >> 24;
         // G[c][axis] of corner c = (c&1, c>>1&1, c>>2&1), flipped (x -1)
         double G[8][3];
         if (grads) {
@@ -572,36 +596,24 @@ __device__ __forceinline__ int64_t find_active(const SegLists& L, const Dims& d,
     return (int64_t)L.base[rs] + rank;
 }
 
-// Final id of the vertex of active cell `a` = (i, j, k) whose contour
-// contains edge e.  rowBaseV points at the volume's first row.
-__device__ __forceinline__ int32_t vertex_of(const uint32_t* __restrict__ acode, const uint32_t* __restrict__ avloc,
-                                             const uint32_t* __restrict__ rowBaseV, int64_t a, int e, int i, int j,
-                                             int k, const Dims& d) {
-    if (a < 0) return -1;
-    const uint32_t code = acode[a];
-    const unsigned icase = code & 0xFFu;
-    const uint32_t map = code >> 8;
-    const unsigned t = (map >> (2 * e)) & 3u;
-    const unsigned used = used_contours(icase, map, i, j, k, d);
-    const uint32_t vb = rowBaseV[(int64_t)k * (d.ny - 1) + j] + avloc[a];
-    return (int32_t)(vb + __popc(used & ((1u << t) - 1u)));
-}
-
-// The same vertex and the row of edge e among its contour's interior edges.
+// Final id of the vertex of active cell `a` whose contour contains edge e,
+// and the row of edge e among that contour's interior edges (its slot in the
+// first halfedge CSR).  rowBase points at the volume's first row; r is the
+// cell's row in the volume.
 __device__ __forceinline__ int32_t vertex_slot_of(const uint32_t* __restrict__ acode,
+                                                  const uint32_t* __restrict__ aaux,
                                                   const uint32_t* __restrict__ avloc,
-                                                  const uint32_t* __restrict__ rowBaseV, int64_t a, int e, int i,
-                                                  int j, int k, const Dims& d, int& slot) {
+                                                  const uint32_t* __restrict__ rowBaseV, int64_t a, int e,
+                                                  int64_t r, int& slot) {
     slot = 0;
     if (a < 0) return -1;
-    const uint32_t code = acode[a];
+    const uint32_t code = acode[a], aux = aaux[a];
     const unsigned icase = code & 0xFFu;
     const uint32_t map = code >> 8;
     const unsigned t = (map >> (2 * e)) & 3u;
-    const unsigned used = used_contours(icase, map, i, j, k, d);
-    slot = __popc(contour_edges(icase, map, t) & interior_edges(i, j, k, d) & ((1u << e) - 1u));
-    const uint32_t vb = rowBaseV[(int64_t)k * (d.ny - 1) + j] + avloc[a];
-    return (int32_t)(vb + __popc(used & ((1u << t) - 1u)));
+    slot = contour_rank(map, aux, icase, t, e);
+    const uint32_t vb = rowBaseV[r] + avloc[a];
+    return (int32_t)(vb + __popc((aux >> 24) & ((1u << t) - 1u)));
 }
 
 // ---------------------------------------------------------------------------
@@ -613,6 +625,7 @@ __device__ __forceinline__ int32_t vertex_slot_of(const uint32_t* __restrict__ a
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* __restrict__ acell,
                                                    const uint32_t* __restrict__ acode,
+                                                   const uint32_t* __restrict__ aaux,
                                                    const uint32_t* __restrict__ avloc,
                                                    const uint32_t* __restrict__ aqloc,
                                                    const uint32_t* __restrict__ rowBaseV0,
@@ -640,9 +653,11 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
         const bool ez = edge_cut(icase, 8) && i >= 1 && j >= 1;
         if (!(ex || ey || ez)) continue;
         int s0, s3, s8;
-        const int32_t own0 = vertex_slot_of(acode, avloc, rowBaseV, a, 0, i, j, k, d, s0);
-        const int32_t own3 = vertex_slot_of(acode, avloc, rowBaseV, a, 3, i, j, k, d, s3);
-        const int32_t own8 = vertex_slot_of(acode, avloc, rowBaseV, a, 8, i, j, k, d, s8);
+        const int32_t own0 = vertex_slot_of(acode, aaux, avloc, rowBaseV, a, 0, r, s0);
+        const int32_t own3 = vertex_slot_of(acode, aaux, avloc, rowBaseV, a, 3, r, s3);
+        const int32_t own8 = vertex_slot_of(acode, aaux, avloc, rowBaseV, a, 8, r, s8);
+        // rows of the cells (., j-1, k) and (., j, k-1)
+        const int64_t rj = (int64_t)r - 1, rk = (int64_t)r - (d.ny - 1);
         // cells of the previous row (j-1) and plane (k-1) found once
         const int64_t cjm = (ex || ez) ? find_active(L, d, i, j - 1, k) : -1;
         const int64_t ckm = (ex || ey) ? find_active(L, d, i, j, k - 1) : -1;
@@ -667,9 +682,9 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
             const int64_t e = find_active(L, d, i, j - 1, k - 1);
             out[edge_slot(0, bit)] = own0;
             sl[edge_slot(0, bit)] = s0;
-            out[edge_slot(2, bit)] = vertex_slot_of(acode, avloc, rowBaseV, cjm, 2, i, j - 1, k, d, sl[edge_slot(2, bit)]);
-            out[edge_slot(4, bit)] = vertex_slot_of(acode, avloc, rowBaseV, ckm, 4, i, j, k - 1, d, sl[edge_slot(4, bit)]);
-            out[edge_slot(6, bit)] = vertex_slot_of(acode, avloc, rowBaseV, e, 6, i, j - 1, k - 1, d, sl[edge_slot(6, bit)]);
+            out[edge_slot(2, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, cjm, 2, rj, sl[edge_slot(2, bit)]);
+            out[edge_slot(4, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, ckm, 4, rk, sl[edge_slot(4, bit)]);
+            out[edge_slot(6, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, e, 6, rk - 1, sl[edge_slot(6, bit)]);
             emit(0);
         }
         // y edge: cells (i,j,k) e3, (i-1,j,k) e1, (i,j,k-1) e7, (i-1,j,k-1) e5
@@ -677,9 +692,9 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
             const int64_t e = ckm > 0 ? ckm - 1 : -1;
             out[edge_slot(3, bit)] = own3;
             sl[edge_slot(3, bit)] = s3;
-            out[edge_slot(1, bit)] = vertex_slot_of(acode, avloc, rowBaseV, a - 1, 1, i - 1, j, k, d, sl[edge_slot(1, bit)]);
-            out[edge_slot(7, bit)] = vertex_slot_of(acode, avloc, rowBaseV, ckm, 7, i, j, k - 1, d, sl[edge_slot(7, bit)]);
-            out[edge_slot(5, bit)] = vertex_slot_of(acode, avloc, rowBaseV, e, 5, i - 1, j, k - 1, d, sl[edge_slot(5, bit)]);
+            out[edge_slot(1, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, a - 1, 1, r, sl[edge_slot(1, bit)]);
+            out[edge_slot(7, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, ckm, 7, rk, sl[edge_slot(7, bit)]);
+            out[edge_slot(5, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, e, 5, rk, sl[edge_slot(5, bit)]);
             emit(1);
         }
         // z edge: cells (i,j,k) e8, (i-1,j,k) e9, (i,j-1,k) e11, (i-1,j-1,k) e10
@@ -687,9 +702,9 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
             const int64_t e = cjm > 0 ? cjm - 1 : -1;
             out[edge_slot(8, bit)] = own8;
             sl[edge_slot(8, bit)] = s8;
-            out[edge_slot(9, bit)] = vertex_slot_of(acode, avloc, rowBaseV, a - 1, 9, i - 1, j, k, d, sl[edge_slot(9, bit)]);
-            out[edge_slot(11, bit)] = vertex_slot_of(acode, avloc, rowBaseV, cjm, 11, i, j - 1, k, d, sl[edge_slot(11, bit)]);
-            out[edge_slot(10, bit)] = vertex_slot_of(acode, avloc, rowBaseV, e, 10, i - 1, j - 1, k, d, sl[edge_slot(10, bit)]);
+            out[edge_slot(9, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, a - 1, 9, r, sl[edge_slot(9, bit)]);
+            out[edge_slot(11, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, cjm, 11, rj, sl[edge_slot(11, bit)]);
+            out[edge_slot(10, bit)] = vertex_slot_of(acode, aaux, avloc, rowBaseV, e, 10, rj, sl[edge_slot(10, bit)]);
             emit(2);
         }
     }
