@@ -418,8 +418,11 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
         HIPC(ctx, hipMemsetAsync(vst, 0, 2 * (size_t)ctx->nvol * sizeof(unsigned long long), ctx->stream));
     }
     if (nhe > 0) {
+        // deg holds every vertex's out-degree here; the twins add boundary
+        // arrivals, turning it into the valence
         hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, adj,
-                           twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst);
+                           twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst,
+                           final_pass ? nullptr : static_cast<uint32_t*>(ctx->deg.p));
         check_launch(ctx, "k_he_twins");
     }
     if (final_pass && nv > 0) {
@@ -437,17 +440,16 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     return hs;
 }
 
+// Valence of the live mesh after a non-final halfedge build: the degree
+// array, completed by the twins pass (out-degree + boundary arrivals).
 int32_t* valence(dmc_context* ctx) {
     const int64_t nv = ctx->mesh.nv;
-    int32_t* val = ensure<int32_t>(ctx, ctx->val, nv);
+    uint32_t* val = static_cast<uint32_t*>(ctx->deg.p);
     unsigned int* zero = static_cast<unsigned int*>(ctx->fails.p) + 4;
-    if (nv > 0) {
-        hipLaunchKernelGGL(k_valence, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
-                           static_cast<uint32_t*>(ctx->off.p), static_cast<int2*>(ctx->adj.p),
-                           static_cast<int32_t*>(ctx->twin.p), nv, val, zero);
-    }
+    if (nv > 0)
+        hipLaunchKernelGGL(k_count_zero, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, val, nv, zero);
     check_launch(ctx, "valence");
-    return val;
+    return reinterpret_cast<int32_t*>(val);
 }
 
 void color_faces(dmc_context* ctx) {

@@ -146,7 +146,8 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
                                                       uint8_t* __restrict__ flag, HeStats* __restrict__ st,
                                                       const int64_t* __restrict__ qb, int nvol,
-                                                      unsigned long long* __restrict__ vst) {
+                                                      unsigned long long* __restrict__ vst,
+                                                      uint32_t* __restrict__ val) {
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         int32_t g[5];
@@ -156,6 +157,11 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
         twin_rule(quads, c, g, h, tw, fl);
         twin[h] = tw;
         flag[h] = fl;
+        // valence: a boundary halfedge adds one to the vertex it ends at
+        if (tw == -1 && val) {
+            const int32_t b = he_dest(quads, h);
+            if (vid_ok(b, nv)) atomicAdd(&val[b], 1u);
+        }
         if (c > 0 && g[0] == h) {
             if (c == 3 || c == 4) atomicAdd(&st->nonmanifold, 1ull);
             if (c == 3) atomicAdd(&st->three, 1ull);
@@ -223,18 +229,16 @@ __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__
     }
 }
 
-// Valence: outgoing halfedges plus boundary halfedges ending at the vertex
-// (cpp_dmc/DualMarchingCubes.cpp:847-871).  A halfedge ends at v exactly
-// when it is the predecessor of one leaving v.
-__global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ off, const int2* __restrict__ adj,
-                                                     const int32_t* __restrict__ twin, int64_t nv,
-                                                     int32_t* __restrict__ val, unsigned int* __restrict__ zero) {
-    TMC_GRID_STRIDE(v, nv) {
-        int s = 0;
-        for_row(adj, off[v], off[v + 1], [&](int2 r) { s += 1 + (twin[he_prev(r.x)] == -1); });
-        val[v] = s;
-        if (s == 0) atomicAdd(zero, 1u);
-    }
+// Valence (cpp_dmc/DualMarchingCubes.cpp:847-871): outgoing halfedges plus
+// boundary halfedges ending at the vertex.  The out-degree is the vertex's
+// CSR row length; k_he_twins adds the boundary arrivals.  A vertex has
+// valence 0 exactly when it has no row entries.
+__global__ void __launch_bounds__(kBlock) k_count_zero(const uint32_t* __restrict__ deg, int64_t nv,
+                                                        unsigned int* __restrict__ zero) {
+    unsigned int z = 0;
+    TMC_GRID_STRIDE(v, nv) z += deg[v] == 0;
+    z = wave_sum(z);
+    if ((threadIdx.x & (kWave - 1)) == 0 && z) atomicAdd(zero, z);
 }
 
 __device__ __forceinline__ void face_neighbors(const int32_t* __restrict__ twin, int64_t f, int32_t nb[4]) {
