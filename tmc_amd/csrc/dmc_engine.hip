@@ -64,7 +64,7 @@ struct dmc_context {
     int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad;
+        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // results
     dmc_mesh mesh{};
@@ -84,7 +84,7 @@ struct dmc_context {
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
                       &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
-                      &off2, &adj2, &rbad};
+                      &off2, &adj2, &rbad, &qmap, &vmap};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -579,13 +579,17 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
     const int dst = 1 - ctx->cur;
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
+    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
+    int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
     HeStats* st = static_cast<HeStats*>(ctx->stats.p);
     if (nq > 0)
         hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
-                           vfirst, vsecond, nv, q2, &st->bad);
+                           vfirst, vsecond, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3x3y");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
-    remap_rows(ctx, Remap{0, em, newQ, vfirst, vsecond, nullptr, nullptr, nullptr, nv}, nn, qn);
+    if (nv > 0)
+        hipLaunchKernelGGL(k_vmap3x3y, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, vfirst, vsecond, nv, vmap);
+    remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, nv}, nn, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;
@@ -628,13 +632,17 @@ void simplify3333(dmc_context* ctx) {
     const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
     const int dst = 1 - ctx->cur;
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
+    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
+    int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
     HeStats* st = static_cast<HeStats*>(ctx->stats.p);
     if (nq > 0)
         hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
-                           newQ, p3333, vtwin, vflag, newv, nv, q2, &st->bad);
+                           newQ, p3333, vtwin, vflag, newv, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3333");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
-    remap_rows(ctx, Remap{1, rflag, newQ, vflag, nullptr, p3333, vtwin, newv, nv}, nn, qn);
+    if (nv > 0)
+        hipLaunchKernelGGL(k_vmap3333, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, vflag, newv, nv, vmap);
+    remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, nv}, nn, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;

@@ -482,8 +482,10 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __r
                                                                const uint8_t* __restrict__ vfirst,
                                                                const int32_t* __restrict__ vsecond, int64_t nv,
                                                                int32_t* __restrict__ quads2,
+                                                               int32_t* __restrict__ qmap,
                                                                unsigned int* __restrict__ bad) {
     TMC_GRID_STRIDE(f, nq) {
+        qmap[f] = em[f] ? -1 : (int32_t)newq[f];
         if (em[f]) continue;
         int32_t r[4];
 #pragma unroll
@@ -580,8 +582,10 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
                                                                const uint8_t* __restrict__ vflag,
                                                                const uint32_t* __restrict__ newv, int64_t nv,
                                                                int32_t* __restrict__ quads2,
+                                                               int32_t* __restrict__ qmap,
                                                                unsigned int* __restrict__ bad) {
     TMC_GRID_STRIDE(f, nq) {
+        qmap[f] = rflag[f] ? -1 : (int32_t)(newq[f] | (p3333[f] ? 0x80000000u : 0u));
         if (rflag[f]) continue;
         int32_t r[4];
 #pragma unroll
@@ -609,29 +613,45 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
 // with atomics (pass 1, after pass 0).  Row order is irrelevant to every
 // consumer (twins sort their groups; valence counts; he_v takes the max).
 // ---------------------------------------------------------------------------
+// The compaction as two maps: qmap[f] = new index of quadrilateral f, -1
+// if dropped, with bit 31 set for a 3333 pattern face (its vertices go
+// through vtwin); vmap[x] = the new id vertex x takes in the quadrilaterals
+// that are not pattern faces (-1 if none).  These are exactly the ids
+// k_compact_quads3x3y / k_compact_quads3333 write.
 struct Remap {
-    int mode;                   // 0: 3X3Y, 1: 3333
-    const uint8_t* qdrop;       // em (3X3Y) / rflag (3333)
-    const uint32_t* newq;
-    const uint8_t* vrem;        // vfirst (3X3Y) / vflag (3333)
-    const int32_t* vsecond;     // 3X3Y: new id of kept x, target of removed x
-    const uint8_t* p3333;       // 3333
-    const int32_t* vtwin;       // 3333
-    const uint32_t* newv;       // 3333
+    const int32_t* qmap;
+    const int32_t* vmap;
+    const uint8_t* vrem;        // removed vertices (vfirst / vflag)
+    const int32_t* vtwin;       // 3333: replacement of a pattern face's vertex
     int64_t nv;                 // vertices before
 };
 
-__device__ __forceinline__ int32_t remap_vertex(const Remap& m, int64_t f, int32_t x) {
-    if (m.mode == 0) {
+__device__ __forceinline__ int32_t remap_vertex(const Remap& m, int32_t qm, int32_t x) {
+    if (!vid_ok(x, m.nv)) return -1;
+    if (qm < -1) {   // pattern face (bit 31 on a non-negative index)
+        x = m.vtwin[x];
         if (!vid_ok(x, m.nv)) return -1;
-        if (m.vrem[x]) {
-            const int32_t t = m.vsecond[x];
-            return vid_ok(t, m.nv) ? m.vsecond[t] : -1;
-        }
-        return m.vsecond[x];
     }
-    if (m.p3333[f] && vid_ok(x, m.nv)) x = m.vtwin[x];
-    return (vid_ok(x, m.nv) && !m.vrem[x]) ? (int32_t)m.newv[x] : -1;
+    return m.vmap[x];
+}
+
+// 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
+// compaction), vsecond[vsecond[x]] for a removed one (its target's).
+__global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__ vfirst,
+                                                      const int32_t* __restrict__ vsecond, int64_t nv,
+                                                      int32_t* __restrict__ vmap) {
+    TMC_GRID_STRIDE(x, nv) {
+        int32_t y = vsecond[x];
+        if (vfirst[x]) y = vid_ok(y, nv) ? vsecond[y] : -1;
+        vmap[x] = y;
+    }
+}
+
+// 3333: vmap[x] = new id of a kept x, -1 for a removed one.
+__global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__ vflag,
+                                                      const uint32_t* __restrict__ newv, int64_t nv,
+                                                      int32_t* __restrict__ vmap) {
+    TMC_GRID_STRIDE(x, nv) vmap[x] = vflag[x] ? -1 : (int32_t)newv[x];
 }
 
 // pass 0: degree of every new row from its kept preimage; pass 1: removed
@@ -649,25 +669,22 @@ __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t
         const uint32_t s = off[x], e = off[x + 1];
         uint32_t c = 0;
         for (uint32_t r = s; r < e; r += 4) {
-            int32_t h[4];
-            uint8_t dr[4];
+            int32_t qm[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) h[i] = adj[min(r + (uint32_t)i, e - 1)].x;
-#pragma unroll
-            for (int i = 0; i < 4; i++) dr[i] = m.qdrop[h[i] >> 2];
+            for (int i = 0; i < 4; i++) qm[i] = m.qmap[adj[min(r + (uint32_t)i, e - 1)].x >> 2];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                if (r + (uint32_t)i >= e || dr[i]) continue;
+                if (r + (uint32_t)i >= e || qm[i] == -1) continue;
                 if (pass == 0) {
                     c++;
                 } else {
-                    const int32_t y = remap_vertex(m, h[i] >> 2, x);
+                    const int32_t y = remap_vertex(m, qm[i], x);
                     if (vid_ok(y, nn)) atomicAdd(&deg2[y], 1u);
                     else atomicAdd(bad, 1u);
                 }
             }
         }
-        if (pass == 0) deg2[m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x]] = c;
+        if (pass == 0) deg2[m.vmap[x]] = c;
     }
 }
 
@@ -681,31 +698,26 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
         const bool kept = !m.vrem[x];
         if ((pass == 0) != kept) continue;
         const uint32_t s = off[x], e = off[x + 1];
-        const int32_t y0 = kept ? (m.mode == 0 ? m.vsecond[x] : (int32_t)m.newv[x]) : -1;
+        const int32_t y0 = kept ? m.vmap[x] : -1;
         uint32_t p = kept ? off2[y0] : 0u;
         const uint32_t p0 = p;
         for (uint32_t r = s; r < e; r += 4) {
             int2 a[4];
-            uint8_t dr[4];
-            uint32_t nq[4];
-            int32_t dst[4];
+            int32_t qm[4], dst[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) a[i] = adj[min(r + (uint32_t)i, e - 1)];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                dr[i] = m.qdrop[a[i].x >> 2];
-                nq[i] = m.newq[a[i].x >> 2];
-            }
+            for (int i = 0; i < 4; i++) qm[i] = m.qmap[a[i].x >> 2];
 #pragma unroll
-            for (int i = 0; i < 4; i++) dst[i] = remap_vertex(m, a[i].x >> 2, a[i].y);
+            for (int i = 0; i < 4; i++) dst[i] = remap_vertex(m, qm[i], a[i].y);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                if (r + (uint32_t)i >= e || dr[i]) continue;
-                const int2 out = make_int2((int32_t)(4 * nq[i] + (a[i].x & 3)), dst[i]);
+                if (r + (uint32_t)i >= e || qm[i] == -1) continue;
+                const int2 out = make_int2(4 * (qm[i] & 0x7FFFFFFF) + (a[i].x & 3), dst[i]);
                 if (pass == 0) {
                     adj2[p++] = out;
                 } else {
-                    const int32_t y = remap_vertex(m, a[i].x >> 2, x);
+                    const int32_t y = remap_vertex(m, qm[i], x);
                     if (!vid_ok(y, nn)) continue;
                     const uint32_t slot = atomicAdd(&cursor2[y], 1u);
                     adj2[off2[y] + slot] = out;
