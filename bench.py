@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
 # scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024_traffic.md)
-TRAFFIC = {"schwarz_1024": 5375285233}
+TRAFFIC = {"schwarz_1024": 5514175672}
 
 
 def log(msg):
@@ -323,11 +323,12 @@ def main():
     summ = m.summary()
     V, Q, A = summ["nv_cells"], summ["nq_cells"], summ["active_cells"]
     # field sweep: one read of the field (4 B per point) plus what it writes:
-    # 4 B per active cell into the segment lists and 4 B per segment count
+    # 4 B per active cell into the segment lists, and per (row, segment) a
+    # 4-B count and 32 B of activity words
     sweep_ms = stages.get("sweep", float("nan"))
     R = (grid.ny - 1) * (grid.nz - 1)
     S = (grid.nx - 1 + 255) // 256
-    sweep_bytes = 4 * nvox + 4 * A + 4 * R * S
+    sweep_bytes = 4 * nvox + 4 * A + 36 * R * S
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(args.workload),
@@ -359,6 +360,12 @@ def main():
     }
     buf.free()
     eng.close()
+    if not args.no_batch:
+        log("[bench] batched leg: 4,096 mixed volumes")
+        out["batched"] = bench_batch()
+    if not args.no_serve:
+        log("[bench] served leg: Poisson serving mix")
+        out["served"] = bench_serve()
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_block}^3 block")
         out["cpu_baseline"] = cpu_baseline(field, grid, iso, min(args.cpu_block, min(field.shape)))
