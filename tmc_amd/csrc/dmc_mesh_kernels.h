@@ -139,8 +139,20 @@ struct HeStats {
 };
 
 // Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669),
-// one thread per halfedge.  The group's first member also counts the edge
+// one thread per halfedge.  The four halfedges of a quadrilateral sit in
+// four adjacent lanes (4q+k in lane 4m+k), and halfedge k needs the rows of
+// its origin v[k] and of v[k+1] -- the row lane k+1 loads for itself.  So
+// every lane loads only its own row (up to kRowRegs entries in registers)
+// and takes its neighbour's with a DPP quad permutation; longer rows are
+// finished from memory.  The group's first member also counts the edge
 // (non-manifold / error).
+constexpr int kRowRegs = 8;
+
+__device__ __forceinline__ int next_in_quad(int x) {
+    // quad_perm(1, 2, 3, 0): lane k receives lane (k+1) & 3 of its quad
+    return __builtin_amdgcn_update_dpp(0, x, 0x39, 0xF, 0xF, false);
+}
+
 __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
                                                       const uint32_t* __restrict__ off,
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
@@ -148,20 +160,74 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
                                                       const int64_t* __restrict__ qb, int nvol,
                                                       unsigned long long* __restrict__ vst,
                                                       uint32_t* __restrict__ val) {
+    static_assert(kBlock % 4 == 0, "quads must not straddle blocks");
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
+        const int32_t a = quads[h];
+        const bool aok = vid_ok(a, nv);
+        uint32_t s = 0, n = 0;
+        if (aok) {
+            s = off[a];
+            n = off[a + 1] - s;
+        }
+        int2 r[kRowRegs];
+        if (n > 0) {
+#pragma unroll
+            for (int i = 0; i < kRowRegs; i++) r[i] = adj[s + min((uint32_t)i, n - 1)];
+        } else {
+#pragma unroll
+            for (int i = 0; i < kRowRegs; i++) r[i] = make_int2(-1, -1);
+        }
+        // the next lane of the quad: its origin is this halfedge's destination
+        const int32_t b = next_in_quad(a);
+        const uint32_t nb = (uint32_t)next_in_quad((int)n);
+        int2 q[kRowRegs];
+#pragma unroll
+        for (int i = 0; i < kRowRegs; i++) q[i] = make_int2(next_in_quad(r[i].x), next_in_quad(r[i].y));
         int32_t g[5];
-        const int c = he_group(quads, off, adj, h, nv, g);
+#pragma unroll
+        for (int p = 0; p < 5; p++) g[p] = 0x7FFFFFFF;
+        int c = 0;
+        if (aok && vid_ok(b, nv)) {
+#pragma unroll
+            for (int i = 0; i < kRowRegs; i++)
+                if ((uint32_t)i < n && r[i].y == b) {
+                    group_insert(g, r[i].x);
+                    c++;
+                }
+            for (uint32_t x = kRowRegs; x < n; x++) {
+                const int2 e = adj[s + x];
+                if (e.y == b) {
+                    group_insert(g, e.x);
+                    c++;
+                }
+            }
+            if (b != a) {
+#pragma unroll
+                for (int i = 0; i < kRowRegs; i++)
+                    if ((uint32_t)i < nb && q[i].y == a) {
+                        group_insert(g, q[i].x);
+                        c++;
+                    }
+                if (nb > kRowRegs) {
+                    const uint32_t sb = off[b];
+                    for (uint32_t x = kRowRegs; x < nb; x++) {
+                        const int2 e = adj[sb + x];
+                        if (e.y == a) {
+                            group_insert(g, e.x);
+                            c++;
+                        }
+                    }
+                }
+            }
+        }
         int32_t tw;
         uint8_t fl;
         twin_rule(quads, c, g, h, tw, fl);
         twin[h] = tw;
         flag[h] = fl;
         // valence: a boundary halfedge adds one to the vertex it ends at
-        if (tw == -1 && val) {
-            const int32_t b = he_dest(quads, h);
-            if (vid_ok(b, nv)) atomicAdd(&val[b], 1u);
-        }
+        if (tw == -1 && val && vid_ok(b, nv)) atomicAdd(&val[b], 1u);
         if (c > 0 && g[0] == h) {
             if (c == 3 || c == 4) atomicAdd(&st->nonmanifold, 1ull);
             if (c == 3) atomicAdd(&st->three, 1ull);
