@@ -121,21 +121,40 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* __restrict
                                                        int64_t n, const uint64_t* __restrict__ partial) {
     __shared__ uint32_t sm[kBlock / kWave + 1];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    const bool full = base + kScanIPT <= n;   // all 16 items in range: 16-B vector accesses
     uint32_t v[kScanIPT];
+    if (full) {
+#pragma unroll
+        for (int q = 0; q < kScanIPT / 4; q++) {
+            const uint4 t = reinterpret_cast<const uint4*>(in + base)[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++) v[q] = base + q < n ? in[base + q] : 0u;
+    }
     uint32_t s = 0;
 #pragma unroll
-    for (int q = 0; q < kScanIPT; q++) {
-        const int64_t idx = base + q;
-        v[q] = idx < n ? in[idx] : 0u;
-        s += v[q];
-    }
+    for (int q = 0; q < kScanIPT; q++) s += v[q];
     uint32_t tot;
     uint32_t run = block_exclusive_scan<uint32_t, kBlock>(s, sm, tot) + (uint32_t)partial[blockIdx.x];
+    uint32_t o[kScanIPT];
 #pragma unroll
     for (int q = 0; q < kScanIPT; q++) {
-        const int64_t idx = base + q;
-        if (idx < n) out[idx] = run;
+        o[q] = run;
         run += v[q];
+    }
+    if (full) {
+#pragma unroll
+        for (int q = 0; q < kScanIPT / 4; q++)
+            reinterpret_cast<uint4*>(out + base)[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++)
+            if (base + q < n) out[base + q] = o[q];
     }
 }
 
