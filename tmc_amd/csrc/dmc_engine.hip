@@ -59,8 +59,11 @@ struct dmc_context {
     std::vector<uint8_t> vtab_host;
     int nvol = 0;
     // mesh (double-buffered where compacted)
-    Buf pos[2], nrm[2], quads[2], colors;
-    int cur = 0;  // which of the double buffers holds the live mesh
+    Buf pos[2], nrm[2], quads[2], colors, vslot[2];
+    int cur = 0;   // which quads buffer holds the live mesh
+    int pcur = 0;  // which positions/normals buffer holds the vertex data
+    int scur = 0;  // which slot map is live ...
+    bool slots = false;  // ... once a compaction has run (else slots are the identity)
     int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
@@ -80,7 +83,7 @@ struct dmc_context {
 
     dmc_context() {
         Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &aaux, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
-                      &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1],
+                      &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
                       &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
@@ -301,6 +304,9 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     HIPC(ctx, hipMemcpyAsync(vb + n + 1, vb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
     HIPC(ctx, hipMemcpyAsync(qb + n + 1, qb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
     ctx->cur = 0;
+    ctx->pcur = 0;
+    ctx->scur = 0;
+    ctx->slots = false;
     double* pos = ensure<double>(ctx, ctx->pos[0], 3 * V);
     double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * V);
     int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * Q);
@@ -486,7 +492,9 @@ void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, int64_t
 }
 
 // Compact vertices by `removed`; vsecond (if non-null) receives new indices of
-// kept vertices.  Returns the new count; swaps the live buffers.
+// kept vertices.  Returns the new count.  Positions and normals stay where
+// they are; the slot map follows the vertices (gather_vertices lays them out
+// at the end).
 int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond, uint32_t** newidx_out) {
     const int64_t nv = ctx->mesh.nv;
     uint32_t* keep = ensure<uint32_t>(ctx, ctx->keepV, nv);
@@ -494,17 +502,35 @@ int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsec
     if (nv > 0)
         hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, nv, keep);
     const int64_t nn = (int64_t)scan(ctx, keep, newidx, nv);
-    const int src = ctx->cur, dst = 1 - ctx->cur;
-    double* pos2 = ensure<double>(ctx, ctx->pos[dst], 3 * nn);
-    double* nrm2 = ensure<double>(ctx, ctx->nrm[dst], 3 * nn);
+    const uint32_t* slot = ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr;
+    uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], nn);
     if (nv > 0)
-        hipLaunchKernelGGL(k_compact_vertices, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv,
-                           static_cast<double*>(ctx->pos[src].p), static_cast<double*>(ctx->nrm[src].p), pos2, nrm2,
-                           vsecond);
-    check_launch(ctx, "k_compact_vertices");
+        hipLaunchKernelGGL(k_compact_slots, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv, slot,
+                           slot2, vsecond);
+    check_launch(ctx, "k_compact_slots");
+    ctx->scur = 1 - ctx->scur;
+    ctx->slots = true;
     remap_bounds(ctx, ctx->vbound, newidx, nv, nn);
     if (newidx_out) *newidx_out = newidx;
     return nn;
+}
+
+// Lay the live vertices' positions and normals out densely (after the last
+// compaction).
+void gather_vertices(dmc_context* ctx) {
+    if (!ctx->slots) return;
+    const int64_t nv = ctx->mesh.nv;
+    const int src = ctx->pcur, dst = 1 - ctx->pcur;
+    double* pos2 = ensure<double>(ctx, ctx->pos[dst], 3 * nv);
+    double* nrm2 = ensure<double>(ctx, ctx->nrm[dst], 3 * nv);
+    if (nv > 0)
+        hipLaunchKernelGGL(k_gather_vertices, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p), nv,
+                           static_cast<const double*>(ctx->pos[src].p), static_cast<const double*>(ctx->nrm[src].p),
+                           pos2, nrm2);
+    check_launch(ctx, "k_gather_vertices");
+    ctx->pcur = dst;
+    ctx->slots = false;
 }
 
 // Halfedge rows of the compacted mesh (nn vertices, nq2 quadrilaterals)
@@ -566,8 +592,9 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
         hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
                            colors, twin, flag, lastT, lastR, em, vfirst);
         hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, val,
-                           lastT, lastR, static_cast<double*>(ctx->pos[ctx->cur].p),
-                           static_cast<double*>(ctx->nrm[ctx->cur].p), vsecond);
+                           lastT, lastR, static_cast<double*>(ctx->pos[ctx->pcur].p),
+                           static_cast<double*>(ctx->nrm[ctx->pcur].p),
+                           ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
         check_launch(ctx, "merge3x3y");
     }
     const int64_t nn = compact_vertices(ctx, vfirst, vsecond, nullptr);
@@ -707,6 +734,7 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
     ctx->mesh.nbnd = (int64_t)ctx->bnd.size();
     ctx->mesh.nt = 0;
     const int64_t nq = ctx->mesh.nq, nv = ctx->mesh.nv;
+    gather_vertices(ctx);
     int32_t* tris = nullptr;
     if (flags & DMC_TRIANGLES) {
         mark(ctx, "triangles");
@@ -714,7 +742,7 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
         if (nq > 0)
             hipLaunchKernelGGL(k_triangles, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream,
                                static_cast<int32_t*>(ctx->quads[ctx->cur].p), nq, nv,
-                               static_cast<double*>(ctx->pos[ctx->cur].p), tris);
+                               static_cast<double*>(ctx->pos[ctx->pcur].p), tris);
         check_launch(ctx, "k_triangles");
         ctx->mesh.nt = 2 * nq;
         ctx->mesh.d_tris = tris;
@@ -746,8 +774,8 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
     ctx->mesh.valence_zero = diag[4];
     ctx->mesh.invalid_refs += diag[5];
     finish_timing(ctx);
-    ctx->mesh.d_positions = static_cast<double*>(ctx->pos[ctx->cur].p);
-    ctx->mesh.d_normals = static_cast<double*>(ctx->nrm[ctx->cur].p);
+    ctx->mesh.d_positions = static_cast<double*>(ctx->pos[ctx->pcur].p);
+    ctx->mesh.d_normals = static_cast<double*>(ctx->nrm[ctx->pcur].p);
     ctx->mesh.d_quads = quads;
     ctx->mesh.d_twins = twin;
     ctx->mesh.d_he_flags = static_cast<uint8_t*>(ctx->flag.p);

@@ -422,35 +422,51 @@ __device__ __forceinline__ bool face_nonmanifold(const uint8_t* __restrict__ fla
     return flag[4 * f] == 1 || flag[4 * f + 1] == 1 || flag[4 * f + 2] == 1 || flag[4 * f + 3] == 1;
 }
 
+// any of face g's four halfedges flagged NON_MANIFOLD (flag bytes 4g..4g+3)
+__device__ __forceinline__ bool flags_nonmanifold(uint32_t w) {
+    return (w & 0xFFu) == 1u || ((w >> 8) & 0xFFu) == 1u || ((w >> 16) & 0xFFu) == 1u || (w >> 24) == 1u;
+}
+
 // Decision of mergeVertices3X3Y for face f (colored: :916-1012, isolated:
 // :1014-1094).  Depends only on data fixed before the merge, so all faces
 // decide in parallel.  Returns 0 (keep), 1 (move v0, drop v2), 2 (move v1,
-// drop v3).
+// drop v3).  Every lookup of the neighbours is issued before any is tested.
 __device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restrict__ quads,
                                               const int32_t* __restrict__ val, const uint8_t* __restrict__ p,
                                               const int32_t* __restrict__ colors, const int32_t* __restrict__ twin,
                                               const uint8_t* __restrict__ flag, int64_t nv) {
     if (!p[f]) return 0;
-    int32_t nb[4];
-    face_neighbors(twin, f, nb);
+    const int4 tw = reinterpret_cast<const int4*>(twin)[f];
+    const int32_t nb[4] = {tw.x >= 0 ? tw.x >> 2 : -1, tw.y >= 0 ? tw.y >> 2 : -1, tw.z >= 0 ? tw.z >> 2 : -1,
+                           tw.w >= 0 ? tw.w >> 2 : -1};
+    const int4 q = reinterpret_cast<const int4*>(quads)[f];
+    uint32_t fw[4];
+    uint8_t pn[4];
+    int32_t cn[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int32_t g = nb[k] >= 0 ? nb[k] : (int32_t)f;   // any valid face for a missing neighbour
+        fw[k] = reinterpret_cast<const uint32_t*>(flag)[g];
+        pn[k] = p[g];
+        cn[k] = colors ? colors[g] : 0;
+    }
+    const int a = val[q.x], b = val[q.y], c = val[q.z], d = val[q.w];
     if (colors) {
-        const int32_t c = colors[f];
+        const int32_t cf = colors[f];
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (nb[k] < 0 || face_nonmanifold(flag, nb[k])) return 0;
+            if (nb[k] < 0 || flags_nonmanifold(fw[k])) return 0;
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (p[nb[k]] && colors[nb[k]] <= c) return 0;
+            if (pn[k] && cn[k] <= cf) return 0;
     } else {
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (nb[k] >= 0 && face_nonmanifold(flag, nb[k])) return 0;
+            if (nb[k] >= 0 && flags_nonmanifold(fw[k])) return 0;
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (nb[k] >= 0 && p[nb[k]]) return 0;
+            if (nb[k] >= 0 && pn[k]) return 0;
     }
-    const int4 q = reinterpret_cast<const int4*>(quads)[f];
-    const int a = val[q.x], b = val[q.y], c = val[q.z], d = val[q.w];
     if (a == 3 && b >= 5 && c == 3 && d >= 5) return 1;
     if (a >= 5 && b == 3 && c >= 5 && d == 3) return 2;
     return 0;
@@ -486,7 +502,8 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
                                                          const int32_t* __restrict__ val,
                                                          const int32_t* __restrict__ lastT,
                                                          const int32_t* __restrict__ lastR, double* __restrict__ pos,
-                                                         double* __restrict__ nrm, int32_t* __restrict__ vsecond) {
+                                                         double* __restrict__ nrm, const uint32_t* __restrict__ vslot,
+                                                         int32_t* __restrict__ vsecond) {
     TMC_GRID_STRIDE(f, nq) {
         if (!em[f]) continue;
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
@@ -496,11 +513,15 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
         const int32_t s0 = first ? q.y : q.x;
         const int32_t s1 = first ? q.w : q.z;
         if (lastT[t] == (int32_t)f) {
+            // storage slots of the vertices (positions and normals are not
+            // moved by the compactions until the end)
+            const int64_t a0 = vslot ? vslot[s0] : s0, a1 = vslot ? vslot[s1] : s1;
+            const int64_t at = vslot ? vslot[t] : t;
             double np[3], nn[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                np[c] = pos[3 * (int64_t)s0 + c] + 0.5 * (pos[3 * (int64_t)s1 + c] - pos[3 * (int64_t)s0 + c]);
-                nn[c] = nrm[3 * (int64_t)s0 + c] + 0.5 * (nrm[3 * (int64_t)s1 + c] - nrm[3 * (int64_t)s0 + c]);
+                np[c] = pos[3 * a0 + c] + 0.5 * (pos[3 * a1 + c] - pos[3 * a0 + c]);
+                nn[c] = nrm[3 * a0 + c] + 0.5 * (nrm[3 * a1 + c] - nrm[3 * a0 + c]);
             }
             const double s = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
             nn[0] /= s;
@@ -508,8 +529,8 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
             nn[2] /= s;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                pos[3 * (int64_t)t + c] = np[c];
-                nrm[3 * (int64_t)t + c] = nn[c];
+                pos[3 * at + c] = np[c];
+                nrm[3 * at + c] = nn[c];
             }
         }
         if (lastR[r] == (int32_t)f) vsecond[r] = t;
@@ -523,20 +544,33 @@ __global__ void __launch_bounds__(kBlock) k_keep_from_flag(const uint8_t* __rest
 }
 
 // Kept vertices: record their new index in vsecond and move their data.
-__global__ void __launch_bounds__(kBlock) k_compact_vertices(const uint8_t* __restrict__ removed,
-                                                              const uint32_t* __restrict__ newidx, int64_t nv,
-                                                              const double* __restrict__ pos,
-                                                              const double* __restrict__ nrm,
-                                                              double* __restrict__ pos2, double* __restrict__ nrm2,
-                                                              int32_t* __restrict__ vsecond) {
+// Vertex compaction without moving positions and normals: vertex v's data
+// lives at slot vslot[v] (the identity when vslot is null); the compaction
+// only carries the slots along (4 B per vertex instead of 48 B), and one
+// gather at the end lays the survivors out densely.
+__global__ void __launch_bounds__(kBlock) k_compact_slots(const uint8_t* __restrict__ removed,
+                                                           const uint32_t* __restrict__ newidx, int64_t nv,
+                                                           const uint32_t* __restrict__ vslot,
+                                                           uint32_t* __restrict__ vslot2,
+                                                           int32_t* __restrict__ vsecond) {
     TMC_GRID_STRIDE(v, nv) {
         if (removed[v]) continue;
-        const int64_t o = newidx[v];
+        const uint32_t o = newidx[v];
+        vslot2[o] = vslot ? vslot[v] : (uint32_t)v;
         if (vsecond) vsecond[v] = (int32_t)o;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather_vertices(const uint32_t* __restrict__ vslot, int64_t nv,
+                                                             const double* __restrict__ pos,
+                                                             const double* __restrict__ nrm,
+                                                             double* __restrict__ pos2, double* __restrict__ nrm2) {
+    TMC_GRID_STRIDE(v, nv) {
+        const int64_t s = vslot[v];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            pos2[3 * o + c] = pos[3 * v + c];
-            nrm2[3 * o + c] = nrm[3 * v + c];
+            pos2[3 * v + c] = pos[3 * s + c];
+            nrm2[3 * v + c] = nrm[3 * s + c];
         }
     }
 }
