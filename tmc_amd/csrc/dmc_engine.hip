@@ -609,13 +609,13 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
     HeStats* st = static_cast<HeStats*>(ctx->stats.p);
-    if (nq > 0)
-        hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
-                           vfirst, vsecond, nv, q2, qmap, &st->bad);
-    check_launch(ctx, "k_compact_quads3x3y");
-    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     if (nv > 0)
         hipLaunchKernelGGL(k_vmap3x3y, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, vfirst, vsecond, nv, vmap);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
+                           vmap, nv, q2, qmap, &st->bad);
+    check_launch(ctx, "k_compact_quads3x3y");
+    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, nv}, nn, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
@@ -662,13 +662,13 @@ void simplify3333(dmc_context* ctx) {
     int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
     HeStats* st = static_cast<HeStats*>(ctx->stats.p);
-    if (nq > 0)
-        hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
-                           newQ, p3333, vtwin, vflag, newv, nv, q2, qmap, &st->bad);
-    check_launch(ctx, "k_compact_quads3333");
-    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     if (nv > 0)
         hipLaunchKernelGGL(k_vmap3333, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, vflag, newv, nv, vmap);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
+                           newQ, p3333, vtwin, vmap, nv, q2, qmap, &st->bad);
+    check_launch(ctx, "k_compact_quads3333");
+    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, nv}, nn, qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;

@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restric
                                                          uint8_t* __restrict__ vfirst) {
     TMC_GRID_STRIDE(f, nq) {
         const int m = merge_decision(f, quads, val, p, colors, twin, flag, nv);
-        em[f] = m != 0;
+        em[f] = (uint8_t)m;   // 0 kept; 1 / 2 which pair of vertices merges
         if (!m) continue;
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
         const int32_t t = m == 1 ? q.x : q.y;
@@ -505,18 +505,20 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
                                                          double* __restrict__ nrm, const uint32_t* __restrict__ vslot,
                                                          int32_t* __restrict__ vsecond) {
     TMC_GRID_STRIDE(f, nq) {
-        if (!em[f]) continue;
+        const int m = em[f];
+        if (!m) continue;
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
-        const bool first = val[q.x] == 3;     // pattern (3, >=5, 3, >=5)
+        const bool first = m == 1;     // pattern (3, >=5, 3, >=5)
         const int32_t t = first ? q.x : q.y;
         const int32_t r = first ? q.z : q.w;
         const int32_t s0 = first ? q.y : q.x;
         const int32_t s1 = first ? q.w : q.z;
-        if (lastT[t] == (int32_t)f) {
-            // storage slots of the vertices (positions and normals are not
-            // moved by the compactions until the end)
-            const int64_t a0 = vslot ? vslot[s0] : s0, a1 = vslot ? vslot[s1] : s1;
-            const int64_t at = vslot ? vslot[t] : t;
+        // storage slots of the vertices (positions and normals are not moved
+        // by the compactions until the end); loaded with the claim
+        const int32_t lt = lastT[t];
+        const int64_t a0 = vslot ? vslot[s0] : s0, a1 = vslot ? vslot[s1] : s1;
+        const int64_t at = vslot ? vslot[t] : t;
+        if (lt == (int32_t)f) {
             double np[3], nn[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -576,33 +578,26 @@ __global__ void __launch_bounds__(kBlock) k_gather_vertices(const uint32_t* __re
 }
 
 // removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137)
+// The vertex ids come from vmap (k_vmap3x3y): vm[vm[x]]-style lookups of a
+// removed vertex x, exactly as the reference composes them.
 __global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, int64_t nq,
                                                                const uint8_t* __restrict__ em,
                                                                const uint32_t* __restrict__ newq,
-                                                               const uint8_t* __restrict__ vfirst,
-                                                               const int32_t* __restrict__ vsecond, int64_t nv,
+                                                               const int32_t* __restrict__ vmap, int64_t nv,
                                                                int32_t* __restrict__ quads2,
                                                                int32_t* __restrict__ qmap,
                                                                unsigned int* __restrict__ bad) {
     TMC_GRID_STRIDE(f, nq) {
         qmap[f] = em[f] ? -1 : (int32_t)newq[f];
         if (em[f]) continue;
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        const int32_t x[4] = {q.x, q.y, q.z, q.w};
         int32_t r[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int32_t x = quads[4 * f + k];
-            int32_t y = -1;
-            if (vid_ok(x, nv)) {
-                if (vfirst[x]) {
-                    const int32_t t = vsecond[x];
-                    y = vid_ok(t, nv) ? vsecond[t] : -1;
-                } else {
-                    y = vsecond[x];
-                }
-            }
-            if (y < 0) atomicAdd(bad, 1u);
-            r[k] = y;
-        }
+        for (int k = 0; k < 4; k++) r[k] = vid_ok(x[k], nv) ? vmap[x[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (r[k] < 0) atomicAdd(bad, 1u);
         reinterpret_cast<int4*>(quads2)[newq[f]] = make_int4(r[0], r[1], r[2], r[3]);
     }
 }
@@ -679,24 +674,27 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
                                                                const uint32_t* __restrict__ newq,
                                                                const uint8_t* __restrict__ p3333,
                                                                const int32_t* __restrict__ vtwin,
-                                                               const uint8_t* __restrict__ vflag,
-                                                               const uint32_t* __restrict__ newv, int64_t nv,
+                                                               const int32_t* __restrict__ vmap, int64_t nv,
                                                                int32_t* __restrict__ quads2,
                                                                int32_t* __restrict__ qmap,
                                                                unsigned int* __restrict__ bad) {
     TMC_GRID_STRIDE(f, nq) {
         qmap[f] = rflag[f] ? -1 : (int32_t)(newq[f] | (p3333[f] ? 0x80000000u : 0u));
         if (rflag[f]) continue;
-        int32_t r[4];
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        int32_t x[4] = {q.x, q.y, q.z, q.w};
+        if (p3333[f]) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int32_t x = quads[4 * f + k];
-            if (p3333[f] && vid_ok(x, nv)) x = vtwin[x];
-            int32_t y = -1;
-            if (vid_ok(x, nv) && !vflag[x]) y = (int32_t)newv[x];
-            if (y < 0) atomicAdd(bad, 1u);
-            r[k] = y;
+            for (int k = 0; k < 4; k++)
+                if (vid_ok(x[k], nv)) x[k] = vtwin[x[k]];
         }
+        int32_t r[4];
+        // vmap[x] = new id of a kept x, -1 for a removed one (k_vmap3333)
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = vid_ok(x[k], nv) ? vmap[x[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (r[k] < 0) atomicAdd(bad, 1u);
         reinterpret_cast<int4*>(quads2)[newq[f]] = make_int4(r[0], r[1], r[2], r[3]);
     }
 }
