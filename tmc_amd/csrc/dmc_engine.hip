@@ -67,7 +67,7 @@ struct dmc_context {
     int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap;
+        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // results
     dmc_mesh mesh{};
@@ -87,7 +87,7 @@ struct dmc_context {
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
                       &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
-                      &off2, &adj2, &rbad, &qmap, &vmap};
+                      &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -504,9 +504,10 @@ int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsec
     const int64_t nn = (int64_t)scan(ctx, keep, newidx, nv);
     const uint32_t* slot = ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr;
     uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], nn);
+    int32_t* rlist = ensure<int32_t>(ctx, ctx->rlist, nv - nn);
     if (nv > 0)
         hipLaunchKernelGGL(k_compact_slots, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv, slot,
-                           slot2, vsecond);
+                           slot2, vsecond, rlist);
     check_launch(ctx, "k_compact_slots");
     ctx->scur = 1 - ctx->scur;
     ctx->slots = true;
@@ -549,16 +550,16 @@ void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2) {
     HIPC(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
     if (m.nv > 0) {
         for (int pass = 0; pass < 2; pass++)
-            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(m.nv)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
-                               deg2, pass, bad);
+            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
+                               m, off, adj, nn, deg2, pass, bad);
         check_launch(ctx, "k_remap_degree");
     }
     const uint64_t rows = scan(ctx, deg2, off2, nn);
     HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off2 + nn), (int)(uint32_t)rows, 1, ctx->stream));
     if (m.nv > 0) {
         for (int pass = 0; pass < 2; pass++)
-            hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(m.nv)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
-                               off2, cur2, adj2, pass);
+            hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
+                               m, off, adj, nn, off2, cur2, adj2, pass);
         check_launch(ctx, "k_remap_rows");
     }
     std::swap(ctx->off, ctx->off2);
@@ -587,23 +588,27 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
+    uint32_t* keepQ = ensure<uint32_t>(ctx, ctx->keepQ, nq);
+    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
     if (nq > 0) {
         hipLaunchKernelGGL(k_mark3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p);
         hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
                            colors, twin, flag, lastT, lastR, em, vfirst);
-        hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, val,
-                           lastT, lastR, static_cast<double*>(ctx->pos[ctx->pcur].p),
-                           static_cast<double*>(ctx->nrm[ctx->pcur].p),
-                           ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
+        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, nq, keepQ);
         check_launch(ctx, "merge3x3y");
     }
-    const int64_t nn = compact_vertices(ctx, vfirst, vsecond, nullptr);
-    // quads
-    uint32_t* keepQ = ensure<uint32_t>(ctx, ctx->keepQ, nq);
-    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
-    if (nq > 0)
-        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, nq, keepQ);
+    // the quadrilateral compaction's scan also lists the merging faces
     const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
+    if (nq > qn) {
+        int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, nq - qn);
+        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, newQ, nq, mlist);
+        hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(nq - qn)), dim3(kBlock), 0, ctx->stream, quads, mlist, nq - qn,
+                           em, val, lastT, lastR, static_cast<double*>(ctx->pos[ctx->pcur].p),
+                           static_cast<double*>(ctx->nrm[ctx->pcur].p),
+                           ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
+        check_launch(ctx, "k_merge_apply");
+    }
+    const int64_t nn = compact_vertices(ctx, vfirst, vsecond, nullptr);
     const int dst = 1 - ctx->cur;
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
     int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
@@ -616,7 +621,8 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
                            vmap, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3x3y");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
-    remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, nv}, nn, qn);
+    remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
+               qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;
@@ -669,7 +675,8 @@ void simplify3333(dmc_context* ctx) {
                            newQ, p3333, vtwin, vmap, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3333");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
-    remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, nv}, nn, qn);
+    remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
+               qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;

@@ -497,16 +497,17 @@ __global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restric
 // Pass B: the winning face writes the midpoint position and the normalised
 // mean normal (reads only valence >= 5 vertices, which are never written)
 // and the drop map.
-__global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restrict__ quads, int64_t nq,
+__global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restrict__ quads,
+                                                         const int32_t* __restrict__ mlist, int64_t n,
                                                          const uint8_t* __restrict__ em,
                                                          const int32_t* __restrict__ val,
                                                          const int32_t* __restrict__ lastT,
                                                          const int32_t* __restrict__ lastR, double* __restrict__ pos,
                                                          double* __restrict__ nrm, const uint32_t* __restrict__ vslot,
                                                          int32_t* __restrict__ vsecond) {
-    TMC_GRID_STRIDE(f, nq) {
+    TMC_GRID_STRIDE(i, n) {
+        const int64_t f = mlist[i];
         const int m = em[f];
-        if (!m) continue;
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
         const bool first = m == 1;     // pattern (3, >=5, 3, >=5)
         const int32_t t = first ? q.x : q.y;
@@ -539,6 +540,16 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
     }
 }
 
+// The flagged items, ascending, at i - newidx[i] (newidx = exclusive scan of
+// the keep flags): a dense list without atomics.
+__global__ void __launch_bounds__(kBlock) k_list_flagged(const uint8_t* __restrict__ flag,
+                                                          const uint32_t* __restrict__ newidx, int64_t n,
+                                                          int32_t* __restrict__ list) {
+    TMC_GRID_STRIDE(i, n) {
+        if (flag[i]) list[i - newidx[i]] = (int32_t)i;
+    }
+}
+
 // keep flag (1 - removed) as uint32 for the scan
 __global__ void __launch_bounds__(kBlock) k_keep_from_flag(const uint8_t* __restrict__ removed, int64_t n,
                                                             uint32_t* __restrict__ keep) {
@@ -550,13 +561,19 @@ __global__ void __launch_bounds__(kBlock) k_keep_from_flag(const uint8_t* __rest
 // lives at slot vslot[v] (the identity when vslot is null); the compaction
 // only carries the slots along (4 B per vertex instead of 48 B), and one
 // gather at the end lays the survivors out densely.
+// Removed vertices are listed densely at v - newidx[v] (the number of
+// removed vertices before v).
 __global__ void __launch_bounds__(kBlock) k_compact_slots(const uint8_t* __restrict__ removed,
                                                            const uint32_t* __restrict__ newidx, int64_t nv,
                                                            const uint32_t* __restrict__ vslot,
                                                            uint32_t* __restrict__ vslot2,
-                                                           int32_t* __restrict__ vsecond) {
+                                                           int32_t* __restrict__ vsecond,
+                                                           int32_t* __restrict__ rlist) {
     TMC_GRID_STRIDE(v, nv) {
-        if (removed[v]) continue;
+        if (removed[v]) {
+            rlist[v - newidx[v]] = (int32_t)v;
+            continue;
+        }
         const uint32_t o = newidx[v];
         vslot2[o] = vslot ? vslot[v] : (uint32_t)v;
         if (vsecond) vsecond[v] = (int32_t)o;
@@ -721,7 +738,9 @@ struct Remap {
     const int32_t* vmap;
     const uint8_t* vrem;        // removed vertices (vfirst / vflag)
     const int32_t* vtwin;       // 3333: replacement of a pattern face's vertex
+    const int32_t* rlist;       // the removed vertices, ascending
     int64_t nv;                 // vertices before
+    int64_t nrem;               // removed vertices
 };
 
 __device__ __forceinline__ int32_t remap_vertex(const Remap& m, int32_t qm, int32_t x) {
@@ -760,10 +779,11 @@ __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t
                                                           const int2* __restrict__ adj, int64_t nn,
                                                           uint32_t* __restrict__ deg2, int pass,
                                                           unsigned int* __restrict__ bad) {
-    TMC_GRID_STRIDE(x64, m.nv) {
-        const int32_t x = (int32_t)x64;
-        const bool kept = !m.vrem[x];
-        if ((pass == 0) != kept) continue;
+    // pass 0 visits every old vertex, pass 1 the list of removed ones
+    TMC_GRID_STRIDE(x64, pass == 0 ? m.nv : m.nrem) {
+        const int32_t x = pass == 0 ? (int32_t)x64 : m.rlist[x64];
+        const bool kept = pass == 0 && !m.vrem[x];
+        if (pass == 0 && !kept) continue;
         const uint32_t s = off[x], e = off[x + 1];
         uint32_t c = 0;
         for (uint32_t r = s; r < e; r += 4) {
@@ -791,10 +811,10 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
                                                         const uint32_t* __restrict__ off2,
                                                         uint32_t* __restrict__ cursor2, int2* __restrict__ adj2,
                                                         int pass) {
-    TMC_GRID_STRIDE(x64, m.nv) {
-        const int32_t x = (int32_t)x64;
-        const bool kept = !m.vrem[x];
-        if ((pass == 0) != kept) continue;
+    TMC_GRID_STRIDE(x64, pass == 0 ? m.nv : m.nrem) {
+        const int32_t x = pass == 0 ? (int32_t)x64 : m.rlist[x64];
+        const bool kept = pass == 0 && !m.vrem[x];
+        if (pass == 0 && !kept) continue;
         const uint32_t s = off[x], e = off[x + 1];
         const int32_t y0 = kept ? m.vmap[x] : -1;
         uint32_t p = kept ? off2[y0] : 0u;
