@@ -247,41 +247,96 @@ __device__ __forceinline__ void trilinear_gradient(const CornerRef& f, double u,
 // exactly as the reference's expression groups them.  Boxes of other size-3
 // contours are only consulted for multi-contour cells
 // (DualMarchingCubes.h:500-519).
-__device__ inline bool projection(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
-                                  double iso, int t, int cnt, uint64_t w, double p[3], int ns) {
-    const double umin = B.lo(axa, t), umax = B.hi(axa, t);
-    const double vmin = B.lo(axb, t), vmax = B.hi(axb, t);
-    const double du = (umax - umin) / (ns - 1);
-    const double dv = (vmax - vmin) / (ns - 1);
+struct ProjFrame {
+    double umin, vmin, du, dv, wmin, wmax;
+};
+
+__device__ __forceinline__ ProjFrame proj_frame(const BoxRef& B, int axa, int axb, int axc, int t, int ns) {
+    ProjFrame f;
+    const double umax = B.hi(axa, t), vmax = B.hi(axb, t);
+    f.umin = B.lo(axa, t);
+    f.vmin = B.lo(axb, t);
+    f.du = (umax - f.umin) / (ns - 1);
+    f.dv = (vmax - f.vmin) / (ns - 1);
+    f.wmin = B.lo(axc, t);
+    f.wmax = B.hi(axc, t);
+    const double eps = 1e-5;
+    if (fabs(f.wmax - f.wmin) < eps) {
+        f.wmin -= eps;
+        f.wmax += eps;
+    }
+    return f;
+}
+
+// Blends of the four frame edges along u at row i.
+__device__ __forceinline__ void proj_row(const CornerRef& Fc, uint32_t cpk, float u, double& u01, double& u23,
+                                         double& u45, double& u67) {
+    u01 = (1 - u) * Fc((cpk >> 0) & 7) + u * Fc((cpk >> 3) & 7);
+    u23 = (1 - u) * Fc((cpk >> 6) & 7) + u * Fc((cpk >> 9) & 7);
+    u45 = (1 - u) * Fc((cpk >> 12) & 7) + u * Fc((cpk >> 15) & 7);
+    u67 = (1 - u) * Fc((cpk >> 18) & 7) + u * Fc((cpk >> 21) & 7);
+}
+
+// 9 x 9 sampling of a cell without a neighbouring size-3 contour box to
+// avoid: the interior 7 x 7 samples are evaluated without branches.  A
+// sample with g1 == g2 gives w = +-inf or NaN, which no [wmin, wmax] test
+// accepts, exactly as the reference's skip of that sample.  The best sample
+// is kept by a strict comparison in the reference's (i, j) order.
+__device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
+                                   double iso, int t, double p[3]) {
+    constexpr int ns = 9;
+    const ProjFrame f = proj_frame(B, axa, axb, axc, t, ns);
     double minDistance = 100;
     float bu = 0.f, bv = 0.f, bw = -1.f;
     bool found = false;
-    double wmin = B.lo(axc, t), wmax = B.hi(axc, t);
-    const double eps = 1e-5;
-    if (fabs(wmax - wmin) < eps) {
-        wmin -= eps;
-        wmax += eps;
-    }
-    const int cnt_size = poly_size(w, t);
-    unsigned nbmask = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) nbmask |= (q < cnt && q != t && poly_size(w, q) == 3) ? (1u << q) : 0u;
-    if (cnt == 1 || cnt_size == 3) nbmask = 0;
 #pragma unroll 1
     for (int i = 1; i < (ns - 1); i++) {
-        const float u = umin + i * du;
-        const double u01 = (1 - u) * Fc((cpk >> 0) & 7) + u * Fc((cpk >> 3) & 7);
-        const double u23 = (1 - u) * Fc((cpk >> 6) & 7) + u * Fc((cpk >> 9) & 7);
-        const double u45 = (1 - u) * Fc((cpk >> 12) & 7) + u * Fc((cpk >> 15) & 7);
-        const double u67 = (1 - u) * Fc((cpk >> 18) & 7) + u * Fc((cpk >> 21) & 7);
+        const float u = f.umin + i * f.du;
+        double u01, u23, u45, u67;
+        proj_row(Fc, cpk, u, u01, u23, u45, u67);
+#pragma unroll
+        for (int j = 1; j < (ns - 1); j++) {
+            const float v = f.vmin + j * f.dv;
+            const float g1 = (1 - v) * u01 + v * u23;
+            const float g2 = (1 - v) * u45 + v * u67;
+            const float ww = (iso - g1) / (g2 - g1);
+            const float d = (p[0] - u) * (p[0] - u) + (p[1] - v) * (p[1] - v) + (p[2] - ww) * (p[2] - ww);
+            const bool take = f.wmin <= ww && ww <= f.wmax && minDistance > d;
+            minDistance = take ? (double)d : minDistance;
+            bu = take ? u : bu;
+            bv = take ? v : bv;
+            bw = take ? ww : bw;
+            found = found || take;
+        }
+    }
+    if (found) {
+        p[0] = bu;
+        p[1] = bv;
+        p[2] = bw;
+    }
+    return found;
+}
+
+// General sampling with ns x ns samples and the neighbour-box test.
+__device__ inline bool projection(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
+                                  double iso, int t, unsigned nbmask, double p[3], int ns) {
+    const ProjFrame f = proj_frame(B, axa, axb, axc, t, ns);
+    double minDistance = 100;
+    float bu = 0.f, bv = 0.f, bw = -1.f;
+    bool found = false;
+#pragma unroll 1
+    for (int i = 1; i < (ns - 1); i++) {
+        const float u = f.umin + i * f.du;
+        double u01, u23, u45, u67;
+        proj_row(Fc, cpk, u, u01, u23, u45, u67);
 #pragma unroll 1
         for (int j = 1; j < (ns - 1); j++) {
-            const float v = vmin + j * dv;
+            const float v = f.vmin + j * f.dv;
             const float g1 = (1 - v) * u01 + v * u23;
             const float g2 = (1 - v) * u45 + v * u67;
             if (g1 == g2) continue;
             const float ww = (iso - g1) / (g2 - g1);
-            if (wmin <= ww && ww <= wmax) {
+            if (f.wmin <= ww && ww <= f.wmax) {
                 bool inside = false;
                 if (nbmask) {
                     const double du_ = u, dv_ = v, dw_ = ww;
@@ -366,10 +421,18 @@ __device__ inline int contour_representative(const CornerRef& Fc, double iso, ui
     pt[0] = axa == 0 ? p[0] : p[1];
     pt[1] = axb == 1 ? p[1] : p[2];
     pt[2] = axc == 2 ? p[2] : axc == 1 ? p[1] : p[0];
+    // size-3 contours of the cell other than t whose boxes samples must avoid
+    unsigned nbmask = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) nbmask |= (q < cnt && q != t && poly_size(w, q) == 3) ? (1u << q) : 0u;
+    if (cnt == 1 || size == 3) nbmask = 0;
     int fails = 0;
-    if (!projection(Fc, cpk, B, axa, axb, axc, iso, t, cnt, w, pt, 9)) {
+    bool ok;
+    if (__any(nbmask != 0)) ok = projection(Fc, cpk, B, axa, axb, axc, iso, t, nbmask, pt, 9);
+    else ok = projection9(Fc, cpk, B, axa, axb, axc, iso, t, pt);
+    if (!ok) {
         fails = 1;
-        if (!projection(Fc, cpk, B, axa, axb, axc, iso, t, cnt, w, pt, 28)) fails = 2;
+        if (!projection(Fc, cpk, B, axa, axb, axc, iso, t, nbmask, pt, 28)) fails = 2;
     }
     double q0 = p[0], q1 = p[1], q2 = p[2];
     if (axa == 0) q0 = pt[0]; else q1 = pt[0];
