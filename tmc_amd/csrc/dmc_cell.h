@@ -369,11 +369,17 @@ __device__ inline bool projection(const CornerRef& Fc, uint32_t cpk, const BoxRe
     return found;
 }
 
-// Representative of contour t in local cell coordinates
-// (cpp_dmc/DualMarchingCubes.cpp:141-156 and :194-272).  Returns the
-// number of failed projection levels (0, 1 or 2).
-__device__ inline int contour_representative(const CornerRef& Fc, double iso, uint64_t w, int cnt, int t,
-                                             const BoxRef& B, double p[3]) {
+// Projection frame of contour t (cpp_dmc/DualMarchingCubes.cpp:141-156 and
+// :194-272): p is the mean of the contour's edge intersections in local cell
+// coordinates and pt the same point in the frame (a, b, c) of the chosen
+// projection axis.
+struct ContourFrame {
+    int axa, axb, axc;
+    uint32_t cpk;
+};
+
+__device__ inline ContourFrame contour_frame(const CornerRef& Fc, double iso, uint64_t w, int t, const BoxRef& B,
+                                             double p[3], double pt[3]) {
     // mean of the edge intersections, summed in contour order
     const int start = poly_start(w, t), size = poly_size(w, t);
     p[0] = 0;
@@ -410,37 +416,75 @@ __device__ inline int contour_representative(const CornerRef& Fc, double iso, ui
         else prj = 1;
     }
     // frame (a, b, c): W = (x, y, z), V = (x, z, y), U = (y, z, x)
-    const int axa = prj == 3 ? 1 : 0;
-    const int axb = prj == 1 ? 1 : 2;
-    const int axc = prj == 1 ? 2 : prj == 2 ? 1 : 0;
-    uint32_t cpk = 0;
+    ContourFrame f;
+    f.axa = prj == 3 ? 1 : 0;
+    f.axb = prj == 1 ? 1 : 2;
+    f.axc = prj == 1 ? 2 : prj == 2 ? 1 : 0;
+    f.cpk = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++)
-        cpk |= (uint32_t)(((q & 1) << axa) | (((q >> 1) & 1) << axb) | (((q >> 2) & 1) << axc)) << (3 * q);
+        f.cpk |= (uint32_t)(((q & 1) << f.axa) | (((q >> 1) & 1) << f.axb) | (((q >> 2) & 1) << f.axc)) << (3 * q);
+    pt[0] = f.axa == 0 ? p[0] : p[1];
+    pt[1] = f.axb == 1 ? p[1] : p[2];
+    pt[2] = f.axc == 2 ? p[2] : f.axc == 1 ? p[1] : p[0];
+    return f;
+}
+
+// Back from the frame to cell coordinates.
+__device__ __forceinline__ void contour_unframe(const ContourFrame& f, const double pt[3], double p[3]) {
+    double q0 = p[0], q1 = p[1], q2 = p[2];
+    if (f.axa == 0) q0 = pt[0]; else q1 = pt[0];
+    if (f.axb == 1) q1 = pt[1]; else q2 = pt[1];
+    if (f.axc == 2) q2 = pt[2]; else if (f.axc == 1) q1 = pt[2]; else q0 = pt[2];
+    p[0] = q0;
+    p[1] = q1;
+    p[2] = q2;
+}
+
+// A cell whose contours' samples must avoid the box of another contour: it
+// has several contours, at least one of size 3 and at least one of another
+// size (DualMarchingCubes.h:500-519 tests the boxes of the other size-3
+// contours when the contour itself is not of size 3).
+__device__ __forceinline__ bool cell_needs_box_test(uint64_t w, int cnt) {
+    if (cnt < 2) return false;
+    bool three = false, other = false;
+    for (int q = 0; q < cnt; q++) {
+        if (poly_size(w, q) == 3) three = true;
+        else other = true;
+    }
+    return three && other;
+}
+
+// Representative of contour t of a cell without the box test, from the 9 x 9
+// sampling; false when that sampling finds no point (the caller then takes
+// the general path, which also samples at 28 x 28).
+__device__ inline bool contour_representative9(const CornerRef& Fc, double iso, uint64_t w, int t, const BoxRef& B,
+                                               double p[3]) {
     double pt[3];
-    pt[0] = axa == 0 ? p[0] : p[1];
-    pt[1] = axb == 1 ? p[1] : p[2];
-    pt[2] = axc == 2 ? p[2] : axc == 1 ? p[1] : p[0];
+    const ContourFrame f = contour_frame(Fc, iso, w, t, B, p, pt);
+    if (!projection9(Fc, f.cpk, B, f.axa, f.axb, f.axc, iso, t, pt)) return false;
+    contour_unframe(f, pt, p);
+    return true;
+}
+
+// Representative of contour t in local cell coordinates
+// (cpp_dmc/DualMarchingCubes.cpp:141-156 and :194-272), general form.
+// Returns the number of failed projection levels (0, 1 or 2).
+__device__ inline int contour_representative(const CornerRef& Fc, double iso, uint64_t w, int cnt, int t,
+                                             const BoxRef& B, double p[3]) {
+    double pt[3];
+    const ContourFrame f = contour_frame(Fc, iso, w, t, B, p, pt);
     // size-3 contours of the cell other than t whose boxes samples must avoid
     unsigned nbmask = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) nbmask |= (q < cnt && q != t && poly_size(w, q) == 3) ? (1u << q) : 0u;
-    if (cnt == 1 || size == 3) nbmask = 0;
+    if (cnt == 1 || poly_size(w, t) == 3) nbmask = 0;
     int fails = 0;
-    bool ok;
-    if (__any(nbmask != 0)) ok = projection(Fc, cpk, B, axa, axb, axc, iso, t, nbmask, pt, 9);
-    else ok = projection9(Fc, cpk, B, axa, axb, axc, iso, t, pt);
-    if (!ok) {
+    if (!projection(Fc, f.cpk, B, f.axa, f.axb, f.axc, iso, t, nbmask, pt, 9)) {
         fails = 1;
-        if (!projection(Fc, cpk, B, axa, axb, axc, iso, t, nbmask, pt, 28)) fails = 2;
+        if (!projection(Fc, f.cpk, B, f.axa, f.axb, f.axc, iso, t, nbmask, pt, 28)) fails = 2;
     }
-    double q0 = p[0], q1 = p[1], q2 = p[2];
-    if (axa == 0) q0 = pt[0]; else q1 = pt[0];
-    if (axb == 1) q1 = pt[1]; else q2 = pt[1];
-    if (axc == 2) q2 = pt[2]; else if (axc == 1) q1 = pt[2]; else q0 = pt[2];
-    p[0] = q0;
-    p[1] = q1;
-    p[2] = q2;
+    contour_unframe(f, pt, p);
     return fails;
 }
 

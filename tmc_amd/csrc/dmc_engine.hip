@@ -67,7 +67,7 @@ struct dmc_context {
     int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist;
+        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // results
     dmc_mesh mesh{};
@@ -87,7 +87,7 @@ struct dmc_context {
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
                       &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
-                      &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist};
+                      &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -316,13 +316,19 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, V);
     uint32_t* off = ensure<uint32_t>(ctx, ctx->off, V + 1);
     int2* adj = ensure<int2>(ctx, ctx->adj, 4 * Q);
+    // cells deferred to the general sampling: slow[0] = count, then cell ids
+    uint32_t* slow = ensure<uint32_t>(ctx, ctx->slow, A + 1);
     mark(ctx, "vertices");
     if (A > 0) {
+        HIPC(ctx, hipMemsetAsync(slow, 0, sizeof(uint32_t), ctx->stream));
         hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux,
-                           avloc, rowBaseV, (int64_t)A, pos, nrm, deg, vstat);
+                           avloc, rowBaseV, (int64_t)A, nrm, deg, slow);
         check_launch(ctx, "k_vertices");
+        hipLaunchKernelGGL(k_vertices_slow<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0,
+                           ctx->stream, vt, acell, acode, aaux, avloc, rowBaseV, slow, nrm, deg, vstat);
+        check_launch(ctx, "k_vertices_slow");
         hipLaunchKernelGGL(k_normals<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, aaux, avloc,
-                           rowBaseV, (int64_t)A, nrm);
+                           rowBaseV, (int64_t)A, pos, nrm);
         check_launch(ctx, "k_normals");
     }
     mark(ctx, "quads");

@@ -404,20 +404,43 @@ __device__ __forceinline__ void point_gradient(const T* __restrict__ field, cons
 }
 
 // ---------------------------------------------------------------------------
-// K3: vertex representatives, normals and positions
-// (cpp_dmc/DualMarchingCubes.cpp:141-171, UniformGrid.h:286-305).
-// One thread per active cell; vertices of used contours are written from
-// rowBaseV[row] + avloc[a] on.  Failed projections are counted for every
-// contour, as the reference does.
+// K3: vertex representatives (cpp_dmc/DualMarchingCubes.cpp:141-171).
+// One thread per active cell; the local coordinates of the vertices of used
+// contours are written from rowBaseV[row] + avloc[a] on, with each vertex's
+// count of interior cut edges.  Cells that need the general sampling (a
+// contour whose samples must avoid another contour's box, or a 9 x 9
+// sampling that finds no point) are appended to `slow` (slow[0] = count,
+// entries from slow[1]) and finished by k_vertices_slow, which also counts
+// the failed projections: only those cells can have any.
 // ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ uint64_t cell_corners(const VolDesc& V, uint32_t cellword, uint32_t code,
+                                                 const CornerRef& Fc, uint32_t& r) {
+    const T* __restrict__ field = static_cast<const T*>(V.field);
+    int i, j, k;
+    cell_ijk(cellword, V.d, i, j, k, r);
+    double F[8];
+    load_corners(field, V.d, i, j, k, F);
+    Fc.store(F);
+    return cell_polygons(code & 0xFFu, V.iso, F);
+}
+
+__device__ __forceinline__ void store_vertex(double* __restrict__ vnrm, uint32_t* __restrict__ vdeg, uint32_t out,
+                                             const double p[3], uint32_t code, uint32_t aux, int t) {
+    vnrm[3 * (int64_t)out + 0] = p[0];
+    vnrm[3 * (int64_t)out + 1] = p[1];
+    vnrm[3 * (int64_t)out + 2] = p[2];
+    vdeg[out] = (uint32_t)contour_rank(code >> 8, aux, code & 0xFFu, (unsigned)t, 12);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
                                                       const uint32_t* __restrict__ aaux,
                                                       const uint32_t* __restrict__ avloc,
                                                       const uint32_t* __restrict__ rowBaseV, int64_t A,
-                                                      double* __restrict__ vpos, double* __restrict__ vnrm,
-                                                      uint32_t* __restrict__ vdeg, VolStat* __restrict__ vstat) {
+                                                      double* __restrict__ vnrm, uint32_t* __restrict__ vdeg,
+                                                      uint32_t* __restrict__ slow) {
     static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
     __shared__ float boxes[kBoxFloats * kBoxStride];
     __shared__ double corners[8 * kCornerStride];
@@ -426,24 +449,66 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         const int vol = vol_of(vt.aBase, vt.n, a);
         const VolDesc& V = vt.desc[vol];
-        const T* __restrict__ field = static_cast<const T*>(V.field);
-        const double* __restrict__ grads = V.grads;
-        const Dims d = V.d;
-        const GridGeom geo = V.geo;
         const double iso = V.iso;
-        const double dx2 = 2 * geo.dx, dy2 = 2 * geo.dy, dz2 = 2 * geo.dz;
-        int i, j, k;
-        uint32_t r;
-        cell_ijk(acell[a], d, i, j, k, r);
         const uint32_t code = acode[a];
-        const unsigned icase = code & 0xFFu;
-        uint64_t w;
-        {
-            double F[8];
-            load_corners(field, d, i, j, k, F);
-            Fc.store(F);
-            w = cell_polygons(icase, iso, F);
+        uint32_t r;
+        const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, r);
+        const int cnt = poly_count(w);
+        bool defer = cell_needs_box_test(w, cnt);
+        if (!defer) {
+            cell_boxes(Fc, iso, w, cnt, B);
+            const uint32_t aux = aaux[a];
+            const unsigned used = aux >> 24;
+            uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
+            for (int t = 0; t < cnt; t++) {
+                double p[3];
+                if (!contour_representative9(Fc, iso, w, t, B, p)) {
+                    defer = true;
+                    break;
+                }
+                if (!((used >> t) & 1u)) continue;
+                store_vertex(vnrm, vdeg, out, p, code, aux, t);
+                out++;
+            }
         }
+        // one counter update per wave for its deferred cells
+        const uint64_t m = __ballot(defer);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(slow, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, leader, kWave);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (defer) slow[1 + base + below] = (uint32_t)a;
+        }
+    }
+}
+
+// General path of K3 for the cells k_vertices deferred: every contour is
+// sampled with the box test at 9 x 9 and, where that finds no point, at
+// 28 x 28; failed projections are counted per volume.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uint32_t* __restrict__ acell,
+                                                          const uint32_t* __restrict__ acode,
+                                                          const uint32_t* __restrict__ aaux,
+                                                          const uint32_t* __restrict__ avloc,
+                                                          const uint32_t* __restrict__ rowBaseV,
+                                                          const uint32_t* __restrict__ slow,
+                                                          double* __restrict__ vnrm, uint32_t* __restrict__ vdeg,
+                                                          VolStat* __restrict__ vstat) {
+    __shared__ float boxes[kBoxFloats * kBoxStride];
+    __shared__ double corners[8 * kCornerStride];
+    const BoxRef B{boxes + threadIdx.x};
+    const CornerRef Fc{corners + threadIdx.x};
+    const uint32_t n = slow[0];
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
+        const int64_t a = slow[1 + s];
+        const int vol = vol_of(vt.aBase, vt.n, a);
+        const VolDesc& V = vt.desc[vol];
+        const double iso = V.iso;
+        const uint32_t code = acode[a];
+        uint32_t r;
+        const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, r);
         const int cnt = poly_count(w);
         cell_boxes(Fc, iso, w, cnt, B);
         const uint32_t aux = aaux[a];
@@ -452,31 +517,25 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
         for (int t = 0; t < cnt; t++) {
             double p[3];
             const int fl = contour_representative(Fc, iso, w, cnt, t, B, p);
-            // rare: counted per volume where they happen
             if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
             if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
             if (!((used >> t) & 1u)) continue;
-            // the normal is formed from these local coordinates by k_normals
-            vpos[3 * (int64_t)out + 0] = geo.x0 + (i + p[0]) * geo.dx;
-            vpos[3 * (int64_t)out + 1] = geo.y0 + (j + p[1]) * geo.dy;
-            vpos[3 * (int64_t)out + 2] = geo.z0 + (k + p[2]) * geo.dz;
-            vnrm[3 * (int64_t)out + 0] = p[0];
-            vnrm[3 * (int64_t)out + 1] = p[1];
-            vnrm[3 * (int64_t)out + 2] = p[2];
-            vdeg[out] = (uint32_t)contour_rank(code >> 8, aux, icase, (unsigned)t, 12);
+            store_vertex(vnrm, vdeg, out, p, code, aux, t);
             out++;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// K3b: normals (cpp_dmc/UniformGrid.h:286-293): trilinear blend of the eight
-// corner gradients at the vertex's local coordinates, normalised.  One
+// K3b: positions and normals (cpp_dmc/DualMarchingCubes.cpp:160-171,
+// UniformGrid.h:286-293): the position from the vertex's local coordinates,
+// the normal as the trilinear blend of the eight corner gradients there,
+// normalised.  One
 // thread per active cell: the 32 field values the cell's corner gradients
 // need (its 8 corners and their +-1 neighbours along each axis, with
 // estimateGradient's index rule) are loaded together, the 24 gradient
 // components formed once, and every used contour of the cell blended from
-// them.  k_vertices left each contour's local coordinates where its normal
+// them.  k_vertices left each vertex's local coordinates where its normal
 // goes.
 // ---------------------------------------------------------------------------
 template <class T>
@@ -484,7 +543,7 @@ __global__ void __launch_bounds__(kBlock) k_normals(VolTable vt, const uint32_t*
                                                      const uint32_t* __restrict__ aaux,
                                                      const uint32_t* __restrict__ avloc,
                                                      const uint32_t* __restrict__ rowBaseV, int64_t A,
-                                                     double* __restrict__ vnrm) {
+                                                     double* __restrict__ vpos, double* __restrict__ vnrm) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         const int vol = vol_of(vt.aBase, vt.n, a);
         const VolDesc& V = vt.desc[vol];
@@ -553,6 +612,11 @@ __global__ void __launch_bounds__(kBlock) k_normals(VolTable vt, const uint32_t*
         for (unsigned um = used; um; um &= um - 1, out++) {
             double* nv = vnrm + 3 * (int64_t)out;
             const double u = nv[0], v = nv[1], ww = nv[2];
+            const GridGeom& geo = V.geo;
+            double* vp = vpos + 3 * (int64_t)out;
+            vp[0] = geo.x0 + (i + u) * geo.dx;
+            vp[1] = geo.y0 + (j + v) * geo.dy;
+            vp[2] = geo.z0 + (k + ww) * geo.dz;
             double n[3];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
