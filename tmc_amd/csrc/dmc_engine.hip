@@ -322,14 +322,11 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     if (A > 0) {
         HIPC(ctx, hipMemsetAsync(slow, 0, sizeof(uint32_t), ctx->stream));
         hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux,
-                           avloc, rowBaseV, (int64_t)A, nrm, deg, slow);
+                           avloc, rowBaseV, (int64_t)A, pos, nrm, deg, slow);
         check_launch(ctx, "k_vertices");
         hipLaunchKernelGGL(k_vertices_slow<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0,
-                           ctx->stream, vt, acell, acode, aaux, avloc, rowBaseV, slow, nrm, deg, vstat);
+                           ctx->stream, vt, acell, acode, aaux, avloc, rowBaseV, slow, pos, nrm, deg, vstat);
         check_launch(ctx, "k_vertices_slow");
-        hipLaunchKernelGGL(k_normals<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, aaux, avloc,
-                           rowBaseV, (int64_t)A, pos, nrm);
-        check_launch(ctx, "k_normals");
     }
     mark(ctx, "quads");
     const uint64_t rows = scan(ctx, deg, off, V);

@@ -382,32 +382,11 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
     if (lane == 0 && total) atomicAdd(&vstat[cur].contours, total);
 }
 
-// Gradient of UniformGrid::estimateGradient followed by flip_gradient
-// (cpp_dmc/UniformGrid.cpp:156-197) at grid point (i,j,k).  Out-of-range
-// neighbours: index -1 clamps to 0, index n maps to 0 (the reference's
-// index lambda returns max - i), the divisor stays 2*dx.
-template <class T>
-__device__ __forceinline__ void point_gradient(const T* __restrict__ field, const Dims& d, int i, int j, int k,
-                                               double dx2, double dy2, double dz2, double g[3]) {
-    auto idx = [](int a, int m) { return a < 0 ? 0 : (a >= m ? m - a : a); };
-    const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
-    const int64_t row = (int64_t)k * sz + (int64_t)j * sy;
-    const double gx = ((double)field[row + idx(i + 1, d.nx)] - (double)field[row + idx(i - 1, d.nx)]) / dx2;
-    const int64_t col = (int64_t)k * sz + i;
-    const double gy = ((double)field[col + idx(j + 1, d.ny) * sy] - (double)field[col + idx(j - 1, d.ny) * sy]) / dy2;
-    const int64_t pil = (int64_t)j * sy + i;
-    const double gz = ((double)field[pil + idx(k + 1, d.nz) * sz] - (double)field[pil + idx(k - 1, d.nz) * sz]) / dz2;
-    const double s = -1.;
-    g[0] = gx * s;
-    g[1] = gy * s;
-    g[2] = gz * s;
-}
-
 // ---------------------------------------------------------------------------
-// K3: vertex representatives (cpp_dmc/DualMarchingCubes.cpp:141-171).
-// One thread per active cell; the local coordinates of the vertices of used
-// contours are written from rowBaseV[row] + avloc[a] on, with each vertex's
-// count of interior cut edges.  Cells that need the general sampling (a
+// K3: vertices (cpp_dmc/DualMarchingCubes.cpp:141-171, UniformGrid.h:286-305).
+// One thread per active cell; position, normal and count of interior cut
+// edges of the vertices of used contours are written from
+// rowBaseV[row] + avloc[a] on.  Cells that need the general sampling (a
 // contour whose samples must avoid another contour's box, or a 9 x 9
 // sampling that finds no point) are appended to `slow` (slow[0] = count,
 // entries from slow[1]) and finished by k_vertices_slow, which also counts
@@ -415,9 +394,8 @@ __device__ __forceinline__ void point_gradient(const T* __restrict__ field, cons
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ uint64_t cell_corners(const VolDesc& V, uint32_t cellword, uint32_t code,
-                                                 const CornerRef& Fc, uint32_t& r) {
+                                                 const CornerRef& Fc, int& i, int& j, int& k, uint32_t& r) {
     const T* __restrict__ field = static_cast<const T*>(V.field);
-    int i, j, k;
     cell_ijk(cellword, V.d, i, j, k, r);
     double F[8];
     load_corners(field, V.d, i, j, k, F);
@@ -425,11 +403,87 @@ __device__ __forceinline__ uint64_t cell_corners(const VolDesc& V, uint32_t cell
     return cell_polygons(code & 0xFFu, V.iso, F);
 }
 
-__device__ __forceinline__ void store_vertex(double* __restrict__ vnrm, uint32_t* __restrict__ vdeg, uint32_t out,
-                                             const double p[3], uint32_t code, uint32_t aux, int t) {
-    vnrm[3 * (int64_t)out + 0] = p[0];
-    vnrm[3 * (int64_t)out + 1] = p[1];
-    vnrm[3 * (int64_t)out + 2] = p[2];
+// Position, normal and interior-edge count of the vertex at local
+// coordinates p of cell (i, j, k) (cpp_dmc/DualMarchingCubes.cpp:160-171,
+// UniformGrid.h:286-305).  The normal is the trilinear blend of the eight
+// corner gradients, normalised.  Corner gradients are those of
+// estimateGradient + flip_gradient (cpp_dmc/UniformGrid.cpp:156-197): central
+// differences whose inner values are the cell's corners (in LDS) and whose
+// outer values (one step beyond the cell along each axis) are loaded here,
+// with the reference's index rule (-1 -> 0, n -> 0: the top boundary wraps).
+template <class T>
+__device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int k, const CornerRef& Fc,
+                                             const double p[3], uint32_t code, uint32_t aux, int t, uint32_t out,
+                                             double* __restrict__ vpos, double* __restrict__ vnrm,
+                                             uint32_t* __restrict__ vdeg) {
+    const Dims& d = V.d;
+    double G[8][3];
+    if (V.grads) {
+        const double* __restrict__ grads = V.grads;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int64_t q = ((int64_t)(k + ((c >> 2) & 1)) * d.ny + (j + ((c >> 1) & 1))) * d.nx + i + (c & 1);
+            G[c][0] = grads[3 * q];
+            G[c][1] = grads[3 * q + 1];
+            G[c][2] = grads[3 * q + 2];
+        }
+    } else {
+        const T* __restrict__ field = static_cast<const T*>(V.field);
+        auto idx = [](int x, int m) { return x < 0 ? 0 : (x >= m ? m - x : x); };
+        const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
+        const int xlo = idx(i - 1, d.nx), xhi = idx(i + 2, d.nx);
+        const int ylo = idx(j - 1, d.ny), yhi = idx(j + 2, d.ny);
+        const int zlo = idx(k - 1, d.nz), zhi = idx(k + 2, d.nz);
+        // xo[b]: row (j + b&1, k + b>>1) at xlo, xhi; yo[b]: column
+        // (i + b&1, k + b>>1) at ylo, yhi; zo[b]: column (i + b&1, j + b>>1)
+        // at zlo, zhi
+        double xo[4][2], yo[4][2], zo[4][2];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int64_t row = (int64_t)(k + (b >> 1)) * sz + (int64_t)(j + (b & 1)) * sy;
+            xo[b][0] = (double)field[row + xlo];
+            xo[b][1] = (double)field[row + xhi];
+            const int64_t colY = (int64_t)(k + (b >> 1)) * sz + i + (b & 1);
+            yo[b][0] = (double)field[colY + (int64_t)ylo * sy];
+            yo[b][1] = (double)field[colY + (int64_t)yhi * sy];
+            const int64_t colZ = (int64_t)(j + (b >> 1)) * sy + i + (b & 1);
+            zo[b][0] = (double)field[colZ + (int64_t)zlo * sz];
+            zo[b][1] = (double)field[colZ + (int64_t)zhi * sz];
+        }
+        const double dx2 = 2 * V.geo.dx, dy2 = 2 * V.geo.dy, dz2 = 2 * V.geo.dz;
+        const double s = -1.;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+            const double gx = (cx ? xo[cy | cz << 1][1] - Fc(c & ~1) : Fc(c | 1) - xo[cy | cz << 1][0]) / dx2;
+            const double gy = (cy ? yo[cx | cz << 1][1] - Fc(c & ~2) : Fc(c | 2) - yo[cx | cz << 1][0]) / dy2;
+            const double gz = (cz ? zo[cx | cy << 1][1] - Fc(c & ~4) : Fc(c | 4) - zo[cx | cy << 1][0]) / dz2;
+            G[c][0] = gx * s;
+            G[c][1] = gy * s;
+            G[c][2] = gz * s;
+        }
+    }
+    const double u = p[0], v = p[1], ww = p[2];
+    double n[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        // (1-u) n_even + u n_odd, then rows by v, then planes by w
+        const double lo0 = (G[0][q] * (1 - u) + G[1][q] * u) * (1 - v);
+        const double hi0 = lo0 + (G[2][q] * (1 - u) + G[3][q] * u) * v;
+        const double lo1 = (G[4][q] * (1 - u) + G[5][q] * u) * (1 - v);
+        const double hi1 = lo1 + (G[6][q] * (1 - u) + G[7][q] * u) * v;
+        n[q] = hi0 * (1 - ww) + hi1 * ww;
+    }
+    const double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const GridGeom& geo = V.geo;
+    double* vp = vpos + 3 * (int64_t)out;
+    vp[0] = geo.x0 + (i + u) * geo.dx;
+    vp[1] = geo.y0 + (j + v) * geo.dy;
+    vp[2] = geo.z0 + (k + ww) * geo.dz;
+    double* vn = vnrm + 3 * (int64_t)out;
+    vn[0] = n[0] / s;
+    vn[1] = n[1] / s;
+    vn[2] = n[2] / s;
     vdeg[out] = (uint32_t)contour_rank(code >> 8, aux, code & 0xFFu, (unsigned)t, 12);
 }
 
@@ -439,8 +493,8 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
                                                       const uint32_t* __restrict__ aaux,
                                                       const uint32_t* __restrict__ avloc,
                                                       const uint32_t* __restrict__ rowBaseV, int64_t A,
-                                                      double* __restrict__ vnrm, uint32_t* __restrict__ vdeg,
-                                                      uint32_t* __restrict__ slow) {
+                                                      double* __restrict__ vpos, double* __restrict__ vnrm,
+                                                      uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow) {
     static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
     __shared__ float boxes[kBoxFloats * kBoxStride];
     __shared__ double corners[8 * kCornerStride];
@@ -451,8 +505,9 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
         const VolDesc& V = vt.desc[vol];
         const double iso = V.iso;
         const uint32_t code = acode[a];
+        int i, j, k;
         uint32_t r;
-        const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, r);
+        const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, i, j, k, r);
         const int cnt = poly_count(w);
         bool defer = cell_needs_box_test(w, cnt);
         if (!defer) {
@@ -467,7 +522,7 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
                     break;
                 }
                 if (!((used >> t) & 1u)) continue;
-                store_vertex(vnrm, vdeg, out, p, code, aux, t);
+                store_vertex<T>(V, i, j, k, Fc, p, code, aux, t, out, vpos, vnrm, vdeg);
                 out++;
             }
         }
@@ -494,8 +549,8 @@ __global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uin
                                                           const uint32_t* __restrict__ avloc,
                                                           const uint32_t* __restrict__ rowBaseV,
                                                           const uint32_t* __restrict__ slow,
-                                                          double* __restrict__ vnrm, uint32_t* __restrict__ vdeg,
-                                                          VolStat* __restrict__ vstat) {
+                                                          double* __restrict__ vpos, double* __restrict__ vnrm,
+                                                          uint32_t* __restrict__ vdeg, VolStat* __restrict__ vstat) {
     __shared__ float boxes[kBoxFloats * kBoxStride];
     __shared__ double corners[8 * kCornerStride];
     const BoxRef B{boxes + threadIdx.x};
@@ -507,8 +562,9 @@ __global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uin
         const VolDesc& V = vt.desc[vol];
         const double iso = V.iso;
         const uint32_t code = acode[a];
+        int i, j, k;
         uint32_t r;
-        const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, r);
+        const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, i, j, k, r);
         const int cnt = poly_count(w);
         cell_boxes(Fc, iso, w, cnt, B);
         const uint32_t aux = aaux[a];
@@ -520,117 +576,8 @@ __global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uin
             if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
             if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
             if (!((used >> t) & 1u)) continue;
-            store_vertex(vnrm, vdeg, out, p, code, aux, t);
+            store_vertex<T>(V, i, j, k, Fc, p, code, aux, t, out, vpos, vnrm, vdeg);
             out++;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// K3b: positions and normals (cpp_dmc/DualMarchingCubes.cpp:160-171,
-// UniformGrid.h:286-293): the position from the vertex's local coordinates,
-// the normal as the trilinear blend of the eight corner gradients there,
-// normalised.  One
-// thread per active cell: the 32 field values the cell's corner gradients
-// need (its 8 corners and their +-1 neighbours along each axis, with
-// estimateGradient's index rule) are loaded together, the 24 gradient
-// components formed once, and every used contour of the cell blended from
-// them.  k_vertices left each vertex's local coordinates where its normal
-// goes.
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_normals(VolTable vt, const uint32_t* __restrict__ acell,
-                                                     const uint32_t* __restrict__ aaux,
-                                                     const uint32_t* __restrict__ avloc,
-                                                     const uint32_t* __restrict__ rowBaseV, int64_t A,
-                                                     double* __restrict__ vpos, double* __restrict__ vnrm) {
-    for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
-        const int vol = vol_of(vt.aBase, vt.n, a);
-        const VolDesc& V = vt.desc[vol];
-        const T* __restrict__ field = static_cast<const T*>(V.field);
-        const double* __restrict__ grads = V.grads;
-        const Dims d = V.d;
-        const double dx2 = 2 * V.geo.dx, dy2 = 2 * V.geo.dy, dz2 = 2 * V.geo.dz;
-        int i, j, k;
-        uint32_t r;
-        cell_ijk(acell[a], d, i, j, k, r);
-        const unsigned used = aaux[a] >> 24;
-        // G[c][axis] of corner c = (c&1, c>>1&1, c>>2&1), flipped (x -1)
-        double G[8][3];
-        if (grads) {
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const int64_t q = ((int64_t)(k + ((c >> 2) & 1)) * d.ny + (j + ((c >> 1) & 1))) * d.nx + i + (c & 1);
-                G[c][0] = grads[3 * q];
-                G[c][1] = grads[3 * q + 1];
-                G[c][2] = grads[3 * q + 2];
-            }
-        } else {
-            // index rule of estimateGradient: -1 -> 0, n -> 0 (the top
-            // boundary wraps), cpp_dmc/UniformGrid.cpp:158-161
-            auto idx = [](int x, int m) { return x < 0 ? 0 : (x >= m ? m - x : x); };
-            const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
-            const int xs[4] = {idx(i - 1, d.nx), i, i + 1, idx(i + 2, d.nx)};
-            const int ys[4] = {idx(j - 1, d.ny), j, j + 1, idx(j + 2, d.ny)};
-            const int zs[4] = {idx(k - 1, d.nz), k, k + 1, idx(k + 2, d.nz)};
-            // X[b][c]: points (xs[c], j + b&1, k + b>>1); Y[b][c]: (i + b&1, ys[c], k + b>>1);
-            // Z[b][c]: (i + b&1, j + b>>1, zs[c]); c = 0 and 3 are the outer neighbours
-            double X[4][4], Y[4][2], Z[4][2];
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int64_t row = (int64_t)(k + (b >> 1)) * sz + (int64_t)(j + (b & 1)) * sy;
-#pragma unroll
-                for (int c = 0; c < 4; c++) X[b][c] = (double)field[row + xs[c]];
-                const int64_t colY = (int64_t)(k + (b >> 1)) * sz + i + (b & 1);
-                Y[b][0] = (double)field[colY + (int64_t)ys[0] * sy];
-                Y[b][1] = (double)field[colY + (int64_t)ys[3] * sy];
-                const int64_t colZ = (int64_t)(j + (b >> 1)) * sy + i + (b & 1);
-                Z[b][0] = (double)field[colZ + (int64_t)zs[0] * sz];
-                Z[b][1] = (double)field[colZ + (int64_t)zs[3] * sz];
-            }
-            const double s = -1.;
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-                // x: (xs[cx+2] - xs[cx]) in row (cy, cz)
-                const double* xr = X[cy | cz << 1];
-                const double gx = (xr[cx + 2] - xr[cx]) / dx2;
-                // y: at (cx, cz): ys[cy+2] - ys[cy]; rows j, j+1 are corners
-                const double yhi = cy ? Y[cx | cz << 1][1] : X[1 | cz << 1][1 + cx];
-                const double ylo = cy ? X[0 | cz << 1][1 + cx] : Y[cx | cz << 1][0];
-                const double gy = (yhi - ylo) / dy2;
-                // z: at (cx, cy): zs[cz+2] - zs[cz]; planes k, k+1 are corners
-                const double zhi = cz ? Z[cx | cy << 1][1] : X[cy | 2][1 + cx];
-                const double zlo = cz ? X[cy][1 + cx] : Z[cx | cy << 1][0];
-                const double gz = (zhi - zlo) / dz2;
-                G[c][0] = gx * s;
-                G[c][1] = gy * s;
-                G[c][2] = gz * s;
-            }
-        }
-        uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
-        for (unsigned um = used; um; um &= um - 1, out++) {
-            double* nv = vnrm + 3 * (int64_t)out;
-            const double u = nv[0], v = nv[1], ww = nv[2];
-            const GridGeom& geo = V.geo;
-            double* vp = vpos + 3 * (int64_t)out;
-            vp[0] = geo.x0 + (i + u) * geo.dx;
-            vp[1] = geo.y0 + (j + v) * geo.dy;
-            vp[2] = geo.z0 + (k + ww) * geo.dz;
-            double n[3];
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                // (1-u) n_even + u n_odd, then rows by v, then planes by w
-                const double lo0 = (G[0][q] * (1 - u) + G[1][q] * u) * (1 - v);
-                const double hi0 = lo0 + (G[2][q] * (1 - u) + G[3][q] * u) * v;
-                const double lo1 = (G[4][q] * (1 - u) + G[5][q] * u) * (1 - v);
-                const double hi1 = lo1 + (G[6][q] * (1 - u) + G[7][q] * u) * v;
-                n[q] = hi0 * (1 - ww) + hi1 * ww;
-            }
-            const double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            nv[0] = n[0] / s;
-            nv[1] = n[1] / s;
-            nv[2] = n[2] / s;
         }
     }
 }
