@@ -35,7 +35,7 @@ struct Buf {
 // long and keep many loads in flight.
 int kGridCap = 16384;
 // field sweep: waves to aim for (sets the planes per wave)
-int kSweepWaves = 8192;
+int kSweepWaves = 32768;
 
 struct StageMark {
     const char* name;

@@ -16,6 +16,8 @@
 //             of the cell relative to its row); rowBaseV / rowBaseQ are the
 //             scans of the row totals.
 #pragma once
+#include <utility>
+
 #include "dmc_cell.h"
 #include "dmc_kernels.h"
 
@@ -64,11 +66,12 @@ struct VolTable {
 
 
 // ---------------------------------------------------------------------------
-// K1: the field sweep.  A wave owns one x-segment of 252 cells for J
+// K1: the field sweep.  A wave owns one x-segment of 256 cells for J
 // consecutive cell rows and marches over K planes, keeping only the
 // comparison bits (field >= iso) of the previous point plane in registers,
-// so every field value is read once (plus one halo row per J rows and one
-// halo plane per K planes).  Lane L reads the 4 points x0+4L..x0+4L+3 with
+// so every field value is read once from HBM (the halo row shared with the
+// next row tile and the halo plane shared with the next plane tile are read
+// through L2 by both waves).  Lane L reads the 4 points x0+4L..x0+4L+3 with
 // one vector load and owns the cells starting at them; the fifth point
 // comes from the next lane, and for lane 63 from a wave-uniform scalar load
 // of point x0+256.  All loads of a plane are issued before any comparison
@@ -86,13 +89,16 @@ constexpr int kSweepRows = TMC_SWEEP_ROWS;  // J
 #define TMC_VERTICES_ATTR
 #endif
 
-template <class T, bool VEC>
+// Loads of the 4 points i0..i0+3 of a row.  Rows read by one wave only are
+// streamed (non-temporal, so they do not evict reused lines); rows that a
+// neighbouring wave also reads (KEEP) are loaded through L2.
+template <class T, bool VEC, bool KEEP>
 __device__ __forceinline__ void row_load(const T* __restrict__ row, int i0, int nx, T v[4]) {
     if constexpr (VEC) {
         const int ic = min(i0, nx - 4);  // nx % 4 == 0: lanes past the end read the last 4 points
         typedef T vec4 __attribute__((ext_vector_type(4)));
-        // streamed once: non-temporal, so it does not evict reused lines
-        const vec4 t = __builtin_nontemporal_load(reinterpret_cast<const vec4*>(row + ic));
+        const vec4* src = reinterpret_cast<const vec4*>(row + ic);
+        const vec4 t = KEEP ? *src : __builtin_nontemporal_load(src);
         v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     } else {
 #pragma unroll
@@ -110,6 +116,147 @@ __device__ __forceinline__ uint32_t row_bits(const T v[4], T edge, T iso_t, int 
     uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(b & 1u), 0x130, 0xF, 0xF, false);
     nb = lane == kWave - 1 ? (uint32_t)(edge >= iso_t) : nb;
     return b | (nb << 4);
+}
+
+// Comparison bits of rows j0..j0+J of one point plane.  All loads are
+// issued before any comparison (clamped row addresses, no branches): rows
+// past the last one are clamped to it and their bits feed no cell.  The
+// first and last rows are shared with the neighbouring row tiles; a whole
+// plane is shared when KEEP.
+template <class T, bool VEC, bool KEEP>
+__device__ __forceinline__ void plane_bits(const T* __restrict__ pl, const Dims& d, int j0, int i0, int ie,
+                                           T iso_t, int lane, uint32_t bits[kSweepRows + 1]) {
+    constexpr int J = kSweepRows;
+    T v[J + 1][4], e[J + 1];
+#pragma unroll
+    for (int jr = 0; jr <= J; jr++) {
+        const T* row = pl + (int64_t)min(j0 + jr, d.ny - 1) * d.nx;
+        if (KEEP || jr == 0 || jr == J) row_load<T, VEC, true>(row, i0, d.nx, v[jr]);
+        else row_load<T, VEC, false>(row, i0, d.nx, v[jr]);
+        e[jr] = row[ie];
+    }
+#pragma unroll
+    for (int jr = 0; jr <= J; jr++) bits[jr] = row_bits<T>(v[jr], e[jr], iso_t, lane);
+}
+
+// v_writelane_b32 through its LLVM intrinsic, so that the compiler sees the
+// instruction and inserts the wait states it needs after the VALU write of x.
+extern "C" __device__ int tmc_writelane_i32(int x, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// Lane L of v takes the wave-uniform value x.
+template <int L>
+__device__ __forceinline__ void write_lane(uint32_t& v, uint32_t x) {
+    static_assert(L >= 0 && L < kWave, "lane index");
+    v = (uint32_t)tmc_writelane_i32((int)x, L, (int)v);
+}
+
+// State of the cells of one plane of a wave's tile.
+struct PlaneCells {
+    const uint32_t* lo;   // bits of point plane k, rows j0..j0+J
+    const uint32_t* hi;   // bits of point plane k+1
+    uint32_t* lists;      // list of the tile's first (row, segment)
+    int S, i0, nj;
+    uint32_t valid;
+    // lane l of stage[h]: dword (l & 7) of the activity words of row
+    // 8h + (l >> 3); lane jr of counts: active cells of row jr
+    uint32_t stage[2];
+    uint32_t counts;
+};
+
+// Cell row jr of a plane: the activity ballots, staged in lanes, and the
+// list entries of the active cells in ascending x.
+template <int JR>
+__device__ __forceinline__ void cell_row(PlaneCells& c) {
+    const uint32_t A = c.lo[JR], B = c.lo[JR + 1], Cm = c.hi[JR], D = c.hi[JR + 1];
+    // bit q: cell i0+q has all 8 corners in / any corner in
+    const uint32_t all_in = A & (A >> 1) & B & (B >> 1) & Cm & (Cm >> 1) & D & (D >> 1);
+    const uint32_t any_in = A | (A >> 1) | B | (B >> 1) | Cm | (Cm >> 1) | D | (D >> 1);
+    uint32_t act = any_in & ~all_in & (JR < c.nj ? c.valid : 0u);
+    const uint64_t b0 = __ballot(act & 1u), b1 = __ballot(act & 2u), b2 = __ballot(act & 4u), b3 = __ballot(act & 8u);
+    const uint32_t total = (uint32_t)(__popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3));
+    uint32_t& st = c.stage[JR >> 3];
+    constexpr int l0 = 8 * (JR & 7);
+    write_lane<l0 + 0>(st, (uint32_t)b0);
+    write_lane<l0 + 1>(st, (uint32_t)(b0 >> 32));
+    write_lane<l0 + 2>(st, (uint32_t)b1);
+    write_lane<l0 + 3>(st, (uint32_t)(b1 >> 32));
+    write_lane<l0 + 4>(st, (uint32_t)b2);
+    write_lane<l0 + 5>(st, (uint32_t)(b2 >> 32));
+    write_lane<l0 + 6>(st, (uint32_t)b3);
+    write_lane<l0 + 7>(st, (uint32_t)(b3 >> 32));
+    write_lane<JR>(c.counts, total);
+    if (total) {
+        // cells of lower lanes (any q) come first, then this lane's lower q
+        uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(b0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b0, 0u));
+        pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(b1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b1, pos));
+        pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(b2 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b2, pos));
+        pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(b3 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b3, pos));
+        uint32_t* out = c.lists + (int64_t)JR * c.S * kSeg;
+        // 5-bit row words side by side: bits q, q+1 of each byte are the two
+        // corners of cell q in that row
+        const uint32_t X = A | B << 8 | Cm << 16 | D << 24;
+        while (act) {
+            const int q = __builtin_ctz(act);
+            act &= act - 1u;
+            const uint32_t m = (X >> q) & 0x03030303u;
+            const uint32_t ic = (m | m >> 6 | m >> 12 | m >> 18) & 0xFFu;
+            out[pos++] = (uint32_t)(c.i0 + q) | (ic << 24);
+        }
+    }
+}
+
+template <size_t... JR>
+__device__ __forceinline__ void cell_rows(PlaneCells& c, std::index_sequence<JR...>) {
+    (cell_row<(int)JR>(c), ...);
+}
+
+// The cells of plane k of the wave's tile from the bits of point planes k
+// (lo) and k+1 (hi); the activity words and active counts of the rows below
+// nj are stored once per plane from their lanes.
+__device__ __forceinline__ void plane_cells(const uint32_t lo[kSweepRows + 1], const uint32_t hi[kSweepRows + 1],
+                                            int64_t k, const Dims& d, int j0, int nj, int S, int s, int i0,
+                                            uint32_t valid, int lane, uint32_t* __restrict__ lists,
+                                            uint32_t* __restrict__ segCount,
+                                            unsigned long long* __restrict__ segMask) {
+    static_assert(kSweepRows <= 16, "activity words of the rows are staged in two VGPRs");
+    const int64_t rs0 = (k * (d.ny - 1) + j0) * S + s;  // (row, segment) of the tile's first row
+    PlaneCells c{lo, hi, lists + rs0 * kSeg, S, i0, nj, valid, {0u, 0u}, 0u};
+    cell_rows(c, std::make_index_sequence<kSweepRows>{});
+    // activity of the segment: bit L of word q <-> cell x0 + 4L + q
+    uint32_t* mask32 = reinterpret_cast<uint32_t*>(segMask);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int row = 8 * h + (lane >> 3);
+        if (row < nj && row < kSweepRows) mask32[8 * (rs0 + (int64_t)row * S) + (lane & 7)] = c.stage[h];
+    }
+    if (lane < nj) segCount[rs0 + (int64_t)lane * S] = c.counts;
+}
+
+// March of one wave over its planes k0..k1, upwards (DOWN false) or
+// downwards.  Plane tiles alternate direction, so two neighbouring tiles
+// read their shared boundary plane both first or both last, at about the
+// same time: the second read is served by L2.
+template <class T, bool VEC, bool DOWN>
+__device__ __forceinline__ void sweep_march(const T* __restrict__ field, const Dims& d, int k0, int k1, int j0,
+                                            int nj, int S, int s, int i0, int ie, uint32_t valid,
+                                            T iso_t, int lane, uint32_t* __restrict__ lists,
+                                            uint32_t* __restrict__ segCount,
+                                            unsigned long long* __restrict__ segMask) {
+    constexpr int J = kSweepRows;
+    const int64_t plane = (int64_t)d.nx * d.ny;
+    uint32_t prev[J + 1], cur[J + 1];
+    int kp = DOWN ? k1 : k0;
+    plane_bits<T, VEC, true>(field + (int64_t)kp * plane, d, j0, i0, ie, iso_t, lane, prev);
+    for (int n = k1 - k0; n > 0; n--) {
+        const int kn = DOWN ? kp - 1 : kp + 1;
+        if (n == 1) plane_bits<T, VEC, true>(field + (int64_t)kn * plane, d, j0, i0, ie, iso_t, lane, cur);
+        else plane_bits<T, VEC, false>(field + (int64_t)kn * plane, d, j0, i0, ie, iso_t, lane, cur);
+        if (DOWN) plane_cells(cur, prev, kn, d, j0, nj, S, s, i0, valid, lane, lists, segCount, segMask);
+        else plane_cells(prev, cur, kp, d, j0, nj, S, s, i0, valid, lane, lists, segCount, segMask);
+#pragma unroll
+        for (int jr = 0; jr <= J; jr++) prev[jr] = cur[jr];
+        kp = kn;
+    }
 }
 
 template <class T, bool VEC>
@@ -140,71 +287,15 @@ __global__ void __launch_bounds__(kBlock) k_sweep(VolTable vt, int64_t waves, ui
     const int k1 = min(k0 + K, d.nz - 1);
     const int i0 = s * kSeg + lane * 4;
     const int ie = min(s * kSeg + kSeg, d.nx - 1);  // lane 63's fifth point (uniform)
-    const uint64_t lt = (1ull << lane) - 1ull;
     // cells i0+q (q < 4) that exist (i < nx-1)
     const int ncell = d.nx - 1 - i0;
     const uint32_t valid = ncell >= 4 ? 0xFu : ncell > 0 ? (1u << ncell) - 1u : 0u;
-    const int64_t plane = (int64_t)d.nx * d.ny;
-    // row base pointers are wave-uniform; rows past the last one are clamped
-    // to it (their bits feed no cell)
-    uint32_t prev[J + 1], cur[J + 1];
-    {
-        T v[J + 1][4], e[J + 1];
-        const T* pl = field + (int64_t)k0 * plane;
-#pragma unroll
-        for (int jr = 0; jr <= J; jr++) {
-            const T* row = pl + (int64_t)min(j0 + jr, d.ny - 1) * d.nx;
-            row_load<T, VEC>(row, i0, d.nx, v[jr]);
-            e[jr] = row[ie];
-        }
-#pragma unroll
-        for (int jr = 0; jr <= J; jr++) prev[jr] = row_bits<T>(v[jr], e[jr], iso_t, lane);
-    }
-    for (int k = k0; k < k1; k++) {
-        {
-            T v[J + 1][4], e[J + 1];
-            const T* pl = field + (int64_t)(k + 1) * plane;
-#pragma unroll
-            for (int jr = 0; jr <= J; jr++) {
-                const T* row = pl + (int64_t)min(j0 + jr, d.ny - 1) * d.nx;
-                row_load<T, VEC>(row, i0, d.nx, v[jr]);
-                e[jr] = row[ie];
-            }
-#pragma unroll
-            for (int jr = 0; jr <= J; jr++) cur[jr] = row_bits<T>(v[jr], e[jr], iso_t, lane);
-        }
-#pragma unroll
-        for (int jr = 0; jr < J; jr++) {
-            if (jr < nj) {
-                const uint32_t A = prev[jr], B = prev[jr + 1], Cm = cur[jr], D = cur[jr + 1];
-                // bit q: cell i0+q has all 8 corners in / any corner in
-                const uint32_t all_in = A & (A >> 1) & B & (B >> 1) & Cm & (Cm >> 1) & D & (D >> 1);
-                const uint32_t any_in = A | (A >> 1) | B | (B >> 1) | Cm | (Cm >> 1) | D | (D >> 1);
-                uint32_t act = any_in & ~all_in & valid;
-                const uint64_t b0 = __ballot(act & 1u), b1 = __ballot(act & 2u), b2 = __ballot(act & 4u),
-                               b3 = __ballot(act & 8u);
-                const int64_t r = (int64_t)k * (d.ny - 1) + (j0 + jr);
-                const uint32_t total = (uint32_t)(__popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3));
-                if (total) {
-                    uint32_t pos = (uint32_t)(__popcll(b0 & lt) + __popcll(b1 & lt) + __popcll(b2 & lt) +
-                                              __popcll(b3 & lt));
-                    uint32_t* out = lists + (r * S + s) * kSeg;
-                    while (act) {
-                        const int q = __builtin_ctz(act);
-                        act &= act - 1u;
-                        const uint32_t ic = ((A >> q) & 3u) | ((B >> q) & 3u) << 2 | ((Cm >> q) & 3u) << 4 |
-                                            ((D >> q) & 3u) << 6;
-                        out[pos++] = (uint32_t)(i0 + q) | (ic << 24);
-                    }
-                }
-                if (lane == 0) segCount[r * S + s] = total;
-                // activity of the segment: bit L of word q <-> cell x0 + 4L + q
-                if (lane < 4) segMask[4 * (r * S + s) + lane] = lane == 0 ? b0 : lane == 1 ? b1 : lane == 2 ? b2 : b3;
-            }
-        }
-#pragma unroll
-        for (int jr = 0; jr <= J; jr++) prev[jr] = cur[jr];
-    }
+    if (tk & 1)
+        sweep_march<T, VEC, true>(field, d, k0, k1, j0, nj, S, s, i0, ie, valid, iso_t, lane, lists, segCount,
+                                  segMask);
+    else
+        sweep_march<T, VEC, false>(field, d, k0, k1, j0, nj, S, s, i0, ie, valid, iso_t, lane, lists, segCount,
+                                   segMask);
 }
 
 template <class T>
