@@ -494,19 +494,51 @@ __device__ __forceinline__ uint64_t cell_corners(const VolDesc& V, uint32_t cell
     return cell_polygons(code & 0xFFu, V.iso, F);
 }
 
+// The 24 field values one step beyond cell (i, j, k) along each axis that
+// the corner gradients need besides the cell's corners, with the index rule
+// of estimateGradient (cpp_dmc/UniformGrid.cpp:156-197: -1 -> 0, n -> 0, the
+// top boundary wraps).  x[b]: row (j + b&1, k + b>>1) at i-1 and i+2;
+// y[b]: column (i + b&1, k + b>>1) at j-1 and j+2; z[b]: column
+// (i + b&1, j + b>>1) at k-1 and k+2.
+template <class T>
+struct OuterValues {
+    T x[4][2], y[4][2], z[4][2];
+};
+
+template <class T>
+__device__ __forceinline__ void load_outer(const VolDesc& V, int i, int j, int k, OuterValues<T>& o) {
+    const Dims& d = V.d;
+    const T* __restrict__ field = static_cast<const T*>(V.field);
+    auto idx = [](int x, int m) { return x < 0 ? 0 : (x >= m ? m - x : x); };
+    const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
+    const int xlo = idx(i - 1, d.nx), xhi = idx(i + 2, d.nx);
+    const int ylo = idx(j - 1, d.ny), yhi = idx(j + 2, d.ny);
+    const int zlo = idx(k - 1, d.nz), zhi = idx(k + 2, d.nz);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int64_t row = (int64_t)(k + (b >> 1)) * sz + (int64_t)(j + (b & 1)) * sy;
+        o.x[b][0] = field[row + xlo];
+        o.x[b][1] = field[row + xhi];
+        const int64_t colY = (int64_t)(k + (b >> 1)) * sz + i + (b & 1);
+        o.y[b][0] = field[colY + (int64_t)ylo * sy];
+        o.y[b][1] = field[colY + (int64_t)yhi * sy];
+        const int64_t colZ = (int64_t)(j + (b >> 1)) * sy + i + (b & 1);
+        o.z[b][0] = field[colZ + (int64_t)zlo * sz];
+        o.z[b][1] = field[colZ + (int64_t)zhi * sz];
+    }
+}
+
 // Position, normal and interior-edge count of the vertex at local
 // coordinates p of cell (i, j, k) (cpp_dmc/DualMarchingCubes.cpp:160-171,
 // UniformGrid.h:286-305).  The normal is the trilinear blend of the eight
-// corner gradients, normalised.  Corner gradients are those of
-// estimateGradient + flip_gradient (cpp_dmc/UniformGrid.cpp:156-197): central
-// differences whose inner values are the cell's corners (in LDS) and whose
-// outer values (one step beyond the cell along each axis) are loaded here,
-// with the reference's index rule (-1 -> 0, n -> 0: the top boundary wraps).
+// corner gradients, normalised.  Corner gradients are the supplied ones, or
+// those of estimateGradient + flip_gradient: central differences of the
+// cell's corners (in LDS) and the outer values o.
 template <class T>
 __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int k, const CornerRef& Fc,
-                                             const double p[3], uint32_t code, uint32_t aux, int t, uint32_t out,
-                                             double* __restrict__ vpos, double* __restrict__ vnrm,
-                                             uint32_t* __restrict__ vdeg) {
+                                             const OuterValues<T>& o, const double p[3], uint32_t code, uint32_t aux,
+                                             int t, uint32_t out, double* __restrict__ vpos,
+                                             double* __restrict__ vnrm, uint32_t* __restrict__ vdeg) {
     const Dims& d = V.d;
     double G[8][3];
     if (V.grads) {
@@ -519,36 +551,14 @@ __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int
             G[c][2] = grads[3 * q + 2];
         }
     } else {
-        const T* __restrict__ field = static_cast<const T*>(V.field);
-        auto idx = [](int x, int m) { return x < 0 ? 0 : (x >= m ? m - x : x); };
-        const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
-        const int xlo = idx(i - 1, d.nx), xhi = idx(i + 2, d.nx);
-        const int ylo = idx(j - 1, d.ny), yhi = idx(j + 2, d.ny);
-        const int zlo = idx(k - 1, d.nz), zhi = idx(k + 2, d.nz);
-        // xo[b]: row (j + b&1, k + b>>1) at xlo, xhi; yo[b]: column
-        // (i + b&1, k + b>>1) at ylo, yhi; zo[b]: column (i + b&1, j + b>>1)
-        // at zlo, zhi
-        double xo[4][2], yo[4][2], zo[4][2];
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int64_t row = (int64_t)(k + (b >> 1)) * sz + (int64_t)(j + (b & 1)) * sy;
-            xo[b][0] = (double)field[row + xlo];
-            xo[b][1] = (double)field[row + xhi];
-            const int64_t colY = (int64_t)(k + (b >> 1)) * sz + i + (b & 1);
-            yo[b][0] = (double)field[colY + (int64_t)ylo * sy];
-            yo[b][1] = (double)field[colY + (int64_t)yhi * sy];
-            const int64_t colZ = (int64_t)(j + (b >> 1)) * sy + i + (b & 1);
-            zo[b][0] = (double)field[colZ + (int64_t)zlo * sz];
-            zo[b][1] = (double)field[colZ + (int64_t)zhi * sz];
-        }
         const double dx2 = 2 * V.geo.dx, dy2 = 2 * V.geo.dy, dz2 = 2 * V.geo.dz;
         const double s = -1.;
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-            const double gx = (cx ? xo[cy | cz << 1][1] - Fc(c & ~1) : Fc(c | 1) - xo[cy | cz << 1][0]) / dx2;
-            const double gy = (cy ? yo[cx | cz << 1][1] - Fc(c & ~2) : Fc(c | 2) - yo[cx | cz << 1][0]) / dy2;
-            const double gz = (cz ? zo[cx | cy << 1][1] - Fc(c & ~4) : Fc(c | 4) - zo[cx | cy << 1][0]) / dz2;
+            const double gx = (cx ? (double)o.x[cy | cz << 1][1] - Fc(c & ~1) : Fc(c | 1) - (double)o.x[cy | cz << 1][0]) / dx2;
+            const double gy = (cy ? (double)o.y[cx | cz << 1][1] - Fc(c & ~2) : Fc(c | 2) - (double)o.y[cx | cz << 1][0]) / dy2;
+            const double gz = (cz ? (double)o.z[cx | cy << 1][1] - Fc(c & ~4) : Fc(c | 4) - (double)o.z[cx | cy << 1][0]) / dz2;
             G[c][0] = gx * s;
             G[c][1] = gy * s;
             G[c][2] = gz * s;
@@ -602,6 +612,9 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
         const int cnt = poly_count(w);
         bool defer = cell_needs_box_test(w, cnt);
         if (!defer) {
+            // issued before the sampling, which hides their latency
+            OuterValues<T> o;
+            if (!V.grads) load_outer<T>(V, i, j, k, o);
             cell_boxes(Fc, iso, w, cnt, B);
             const uint32_t aux = aaux[a];
             const unsigned used = aux >> 24;
@@ -613,7 +626,7 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
                     break;
                 }
                 if (!((used >> t) & 1u)) continue;
-                store_vertex<T>(V, i, j, k, Fc, p, code, aux, t, out, vpos, vnrm, vdeg);
+                store_vertex<T>(V, i, j, k, Fc, o, p, code, aux, t, out, vpos, vnrm, vdeg);
                 out++;
             }
         }
@@ -657,6 +670,8 @@ __global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uin
         uint32_t r;
         const uint64_t w = cell_corners<T>(V, acell[a], code, Fc, i, j, k, r);
         const int cnt = poly_count(w);
+        OuterValues<T> o;
+        if (!V.grads) load_outer<T>(V, i, j, k, o);
         cell_boxes(Fc, iso, w, cnt, B);
         const uint32_t aux = aaux[a];
         const unsigned used = aux >> 24;
@@ -667,7 +682,7 @@ __global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uin
             if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
             if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
             if (!((used >> t) & 1u)) continue;
-            store_vertex<T>(V, i, j, k, Fc, p, code, aux, t, out, vpos, vnrm, vdeg);
+            store_vertex<T>(V, i, j, k, Fc, o, p, code, aux, t, out, vpos, vnrm, vdeg);
             out++;
         }
     }
