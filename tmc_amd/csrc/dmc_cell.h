@@ -278,35 +278,80 @@ __device__ __forceinline__ void proj_row(const CornerRef& Fc, uint32_t cpk, floa
 }
 
 // 9 x 9 sampling of a cell without a neighbouring size-3 contour box to
-// avoid: the interior 7 x 7 samples are evaluated without branches.  A
-// sample with g1 == g2 gives w = +-inf or NaN, which no [wmin, wmax] test
-// accepts, exactly as the reference's skip of that sample.  The best sample
-// is kept by a strict comparison in the reference's (i, j) order.
+// avoid.  The reference keeps the first sample (in (i, j) order) of least
+// distance d among those whose w lies in [wmin, wmax]; any evaluation order
+// that breaks ties of d by that order finds the same sample.  Rows are
+// evaluated nearest to p first, outwards on both sides, and a side stops at
+// the first row that cannot hold a winner: with a_i = (p0 - u_i)^2 and
+// b_j = (p1 - v_j)^2 as the distance expression rounds them, every d of row
+// i is at least float(a_i + min_j b_j) (rounding is monotone and the w term
+// is not negative), and a_i grows away from p on each side.  A sample with
+// g1 == g2 gives w = +-inf or NaN, which no [wmin, wmax] test accepts,
+// exactly as the reference's skip of that sample.
 __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
                                    double iso, int t, double p[3]) {
     constexpr int ns = 9;
+    constexpr int kIn = ns - 2;  // interior samples per axis: i, j = 1 .. 7
     const ProjFrame f = proj_frame(B, axa, axb, axc, t, ns);
-    double minDistance = 100;
+    float vj[kIn];
+    double bj[kIn];
+    double bmin = __builtin_huge_val();
+#pragma unroll
+    for (int j = 1; j <= kIn; j++) {
+        const float v = f.vmin + j * f.dv;
+        vj[j - 1] = v;
+        bj[j - 1] = (p[1] - v) * (p[1] - v);
+        bmin = bj[j - 1] < bmin ? bj[j - 1] : bmin;
+    }
+    auto row_a = [&](int i) {
+        const float u = f.umin + i * f.du;
+        return (p[0] - u) * (p[0] - u);
+    };
+    auto row_bound = [&](int i) { return (double)(float)(row_a(i) + bmin); };
+    // first row: about the nearest to p0 (any row is correct)
+    const float x = fminf(fmaxf((float)((p[0] - f.umin) / f.du) + 0.5f, 1.f), (float)kIn);
+    int cur = (int)x, lo = cur - 1, hi = cur + 1;
+    double best = 100;
+    int bestO = -1;
     float bu = 0.f, bv = 0.f, bw = -1.f;
     bool found = false;
-#pragma unroll 1
-    for (int i = 1; i < (ns - 1); i++) {
-        const float u = f.umin + i * f.du;
-        double u01, u23, u45, u67;
-        proj_row(Fc, cpk, u, u01, u23, u45, u67);
+    while (__any(cur >= 0)) {
+        if (cur >= 0) {
+            const int i = cur;
+            const float u = f.umin + i * f.du;
+            const double ai = (p[0] - u) * (p[0] - u);
+            double u01, u23, u45, u67;
+            proj_row(Fc, cpk, u, u01, u23, u45, u67);
 #pragma unroll
-        for (int j = 1; j < (ns - 1); j++) {
-            const float v = f.vmin + j * f.dv;
-            const float g1 = (1 - v) * u01 + v * u23;
-            const float g2 = (1 - v) * u45 + v * u67;
-            const float ww = (iso - g1) / (g2 - g1);
-            const float d = (p[0] - u) * (p[0] - u) + (p[1] - v) * (p[1] - v) + (p[2] - ww) * (p[2] - ww);
-            const bool take = f.wmin <= ww && ww <= f.wmax && minDistance > d;
-            minDistance = take ? (double)d : minDistance;
-            bu = take ? u : bu;
-            bv = take ? v : bv;
-            bw = take ? ww : bw;
-            found = found || take;
+            for (int j = 1; j <= kIn; j++) {
+                const float v = vj[j - 1];
+                const float g1 = (1 - v) * u01 + v * u23;
+                const float g2 = (1 - v) * u45 + v * u67;
+                const float ww = (iso - g1) / (g2 - g1);
+                const double s = ai + bj[j - 1];
+                const float d = s + (p[2] - ww) * (p[2] - ww);
+                const int o = ns * i + j;
+                const bool take = f.wmin <= ww && ww <= f.wmax && ((double)d < best || ((double)d == best && o < bestO));
+                best = take ? (double)d : best;
+                bestO = take ? o : bestO;
+                bu = take ? u : bu;
+                bv = take ? v : bv;
+                bw = take ? ww : bw;
+                found = found || take;
+            }
+            // next row: the side whose next row has the smaller bound, among
+            // the sides that can still hold a winner
+            double bl = 0, bh = 0;
+            if (lo >= 1) {
+                bl = row_bound(lo);
+                if (bl > best) lo = 0;
+            }
+            if (hi <= kIn) {
+                bh = row_bound(hi);
+                if (bh > best) hi = kIn + 1;
+            }
+            const bool okl = lo >= 1, okh = hi <= kIn;
+            cur = (okl && (!okh || bl <= bh)) ? lo-- : okh ? hi++ : -1;
         }
     }
     if (found) {
