@@ -253,6 +253,30 @@ def calibrate_copy():
     print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}), flush=True)
 
 
+def bench_diagnostics(eng, reps=3):
+    """Time the mesh diagnostics on the last mesh (wall clock per call,
+    results copied to the host included): unique edges, valence
+    distributions of the quad and triangle meshes, element quality."""
+    calls = {"edges": lambda: eng.edges(),
+             "valence_quads": lambda: eng.valence_histogram(),
+             "valence_tris": lambda: eng.valence_histogram(triangles=True),
+             "quality_quads": lambda: eng.element_quality(),
+             "quality_tris": lambda: eng.element_quality(triangles=True)}
+    out = {}
+    for name, fn in calls.items():
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        out[name + "_ms"] = round((time.perf_counter() - t) / reps * 1e3, 3)
+        if name == "edges":
+            out["edges"] = int(len(r))
+        if name.startswith("valence"):
+            out[name + "_hist"] = [int(x) for x in r[0]]
+    log(f"[bench] diagnostics {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +382,7 @@ def main():
                        stage_ms=stages),
         "roofline": roofline,
     }
+    out["diagnostics"] = bench_diagnostics(eng)
     buf.free()
     eng.close()
     if not args.no_batch:

@@ -197,6 +197,42 @@ const char* dmc_server_last_error(dmc_server* server);
  * is non-zero the normals are negated like the CUDA path does. */
 dmc_status dmc_mesh_download_f32(dmc_context* ctx, float* positions, float* normals, int negate_normals);
 
+/* ---- diagnostics of the last extraction's mesh (a single volume:
+ * DMC_ERR_STATE after a batch); each call computes on the device and copies
+ * the result out ---- */
+
+/* Unique undirected edges, one per vertex pair
+ * (dmc/HalfedgeMesh.cu:430-450, HalfedgeMesh::edges): *count receives their
+ * number; edges (2 int32 per edge, may be NULL) receives each as the
+ * (origin, destination) of the pair's smallest halfedge, in ascending order
+ * of that halfedge.  The reference's order and orientation are those of its
+ * hash table; the set of pairs is the same. */
+dmc_status dmc_mesh_edges(dmc_context* ctx, int32_t* edges, int64_t* count);
+
+/* Vertex valence distribution (dmc/VertexValence.cu:79-170,
+ * VertexValence::vertexValence for quadrilaterals, or for the triangles when
+ * triangles != 0, extraction with DMC_TRIANGLES): hist[k] = number of
+ * vertices with k distinct incident edges (a loop counting twice), for
+ * k < 100 (hist has room for 100); *len = 1 + the largest k > 0 with
+ * hist[k] > 0, or 1 (the reference's trimmed size); *beyond (may be NULL) =
+ * vertices of valence 100 or more, for which the reference has no bin. */
+dmc_status dmc_valence_histogram(dmc_context* ctx, int triangles, uint32_t* hist, int32_t* len, int64_t* beyond);
+
+/* Element quality (dmc/EstimateElementQuality.cu, dmc/ElementQuality.h:65-91)
+ * in FP32 from the positions rounded to float.  which = 0: one value per
+ * quadrilateral, the least corner ratio 2|e1 x e2| / (|e1|^2 + |e2|^2);
+ * which = 1: one per triangle (extraction with DMC_TRIANGLES),
+ * 2 sqrt(3) |e1 x e2| / (l0 + l1 + l2) over the squared edge lengths, 0 when
+ * an edge has length 0. */
+dmc_status dmc_element_quality(dmc_context* ctx, int which, float* quality);
+
+/* The mesh as Wavefront OBJ text in cpp_dmc's main program layout
+ * (cpp_dmc/main.cpp:35-87): a comment line, "v x y z" and "vn x y z" for
+ * every vertex in the C++ stream's default format (printf "%g"), then
+ * "f a//a b//b c//c" with 1-based indices -- triangles when triangles != 0
+ * (extraction with DMC_TRIANGLES), else quadrilaterals. */
+dmc_status dmc_write_obj(dmc_context* ctx, const char* path, int triangles);
+
 /* Per-stage device times (ms) of the last extraction run with
  * DMC_STAGE_TIMING.  names[i] point to static strings.  *count is in/out:
  * capacity in, number of stages out. */

@@ -222,7 +222,49 @@ private:
 // analytic central-difference gradients, stored unflipped).
 class Volumes {
 public:
-    enum class Surface { Sphere, Torus, TwoHoledTorus, MonkeySaddle, GenusTwo, iWP, Neovius, SteinerRoman, Kummer };
+    enum class Surface {
+        Sphere,
+        Torus,
+        TwoHoledTorus,
+        MonkeySaddle,
+        GenusTwo,
+        iWP,
+        Neovius,
+        SteinerRoman,
+        Kummer,
+        Tetrahedron
+    };
+
+    // cpp_dmc/Volumes.h:45-106: 3 int dims, 3 float spacings, then nx*ny*nz
+    // values of type T (x fastest).  The box runs from 0 to
+    // double(float spacing * (n-1)) on each axis, the spacings are the file's
+    // floats, and the gradients are estimated and flipped.  Errors throw
+    // instead of exiting.
+    template <typename T>
+    void readFromFile(const std::string& i_file, UniformGrid& ugrid) {
+        std::ifstream f(i_file, std::ios::binary);
+        if (!f.is_open()) throw Error("Volumes::readFromFile: cannot open " + i_file);
+        int n[3];
+        float d[3];
+        f.read(reinterpret_cast<char*>(n), sizeof(n));
+        f.read(reinterpret_cast<char*>(d), sizeof(d));
+        if (!f || n[0] < 2 || n[1] < 2 || n[2] < 2) throw Error("Volumes::readFromFile: bad header in " + i_file);
+        const double mx[3] = {static_cast<double>(d[0] * (n[0] - 1)), static_cast<double>(d[1] * (n[1] - 1)),
+                              static_cast<double>(d[2] * (n[2] - 1))};
+        UniformGrid::BBox bb;
+        for (int c = 0; c < 8; c++)
+            for (int a = 0; a < 3; a++) bb[c][a] = ((c >> a) & 1) ? mx[a] : 0.0;
+        ugrid.init(n[0], n[1], n[2], bb, 0);
+        ugrid.set_dx(d[0]);
+        ugrid.set_dy(d[1]);
+        ugrid.set_dz(d[2]);
+        std::vector<T> buf((size_t)n[0] * n[1] * n[2]);
+        f.read(reinterpret_cast<char*>(buf.data()), buf.size() * sizeof(T));
+        if (!f) throw Error("Volumes::readFromFile: short file " + i_file);
+        for (size_t q = 0; q < buf.size(); q++) ugrid.scalar((int)q, static_cast<double>(buf[q]));
+        ugrid.estimateGradient();
+        ugrid.flip_gradient();
+    }
 
     template <Surface T>
     void scalar(UniformGrid& ugrid, const int nx, const int ny, const int nz) {
@@ -299,9 +341,74 @@ public:
             v = v * v;
             return v - lambda * p * q * r * t;
         }
+        case Surface::Tetrahedron: return tetrahedron(x_, y_, z_);
         }
         return 0;
     }
+
+private:
+    // cpp_dmc/Vector.h:165-184 and Volumes.h:248-335: signed distance to the
+    // nearest face of the tetrahedron (-0.8,-0.8,-0.8), (0.8,-0.8,-0.8),
+    // (0.8,0.8,-0.8), (0,0,0.8), the same operations in the same order.
+    struct V3 {
+        double x, y, z;
+    };
+    static V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+    static V3 add(const V3& a, const V3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+    static V3 scale(double s, const V3& a) { return {a.x * s, a.y * s, a.z * s}; }
+    static double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+    static V3 cross(const V3& a, const V3& b) {
+        return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    }
+    static double dist(const V3& a, const V3& b) {
+        const double d = (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z);
+        return std::sqrt(d);
+    }
+    static V3 unit(V3 a) {
+        const double s = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+        a.x /= s;
+        a.y /= s;
+        a.z /= s;
+        return a;
+    }
+    static double point_edge(const V3& v0, const V3& v1, const V3& p) {
+        const V3 n = unit(sub(v1, v0));
+        const double l = dot(sub(p, v0), n);
+        if (l <= 0 || l >= dist(v0, v1)) {
+            const double l0 = dist(p, v0), l1 = dist(p, v1);
+            return l0 < l1 ? l0 : l1;
+        }
+        return dist(p, add(v0, scale(l, n)));
+    }
+    static double point_triangle(const V3& v0, const V3& v1, const V3& v2, const V3& p) {
+        const V3 n = unit(cross(sub(v1, v0), sub(v2, v0)));
+        const double l = -dot(sub(p, v0), n);
+        const V3 q = add(p, scale(l, n));
+        const double d0 = dot(n, cross(sub(v0, q), sub(v1, q)));
+        const double d1 = dot(n, cross(sub(v1, q), sub(v2, q)));
+        const double d2 = dot(n, cross(sub(v2, q), sub(v0, q)));
+        if (d0 > 0 && d1 > 0 && d2 > 0) return dot(n, sub(p, v0));
+        const double e0 = point_edge(v0, v1, p), e1 = point_edge(v1, v2, p), e2 = point_edge(v2, v0, p);
+        return std::min(std::min(e0, e1), e2);
+    }
+    static double tetrahedron(const double x_, const double y_, const double z_) {
+        const V3 v0{-0.8, -0.8, -0.8}, v1{0.8, -0.8, -0.8}, v2{0.8, 0.8, -0.8}, v3{0.0, 0.0, 0.8};
+        const V3 p{x_, y_, z_};
+        const double d0 = point_triangle(v0, v2, v1, p);
+        const double d1 = point_triangle(v0, v1, v3, p);
+        const double d2 = point_triangle(v1, v2, v3, p);
+        const double d3 = point_triangle(v0, v3, v2, p);
+        double d = std::min(std::fabs(d0), std::fabs(d1));
+        d = std::min(d, std::fabs(d2));
+        d = std::min(d, std::fabs(d3));
+        if (d == std::fabs(d0)) return d0;
+        if (d == std::fabs(d1)) return d1;
+        if (d == std::fabs(d2)) return d2;
+        if (d == std::fabs(d3)) return d3;
+        return d;
+    }
+
+public:
 };
 
 // cpp_dmc/DualMarchingCubes.h:122-182
@@ -415,17 +522,43 @@ public:
                 }
     }
 
-    // dmc/DualMarchingCubes.h:74-77.  config keys: p3X3YColor, p3X3YOld,
-    // p3333, halfedge-datastructure (dmc/kernel.cu:25-32); missing keys are 0.
+    // dmc/DualMarchingCubes.h:74-77.  config keys (dmc/kernel.cu:25-32;
+    // missing keys are 0): p3X3YColor, p3X3YOld, p3333,
+    // halfedge-datastructure; valence fills the valenceDist* members with
+    // the valence distributions of the mesh after each enabled stage and of
+    // the triangles (dmc/DualMarchingCubes.cu:966-1043), element-quality
+    // fills elementQualityQuads / elementQualityTris (:1050-1060), and
+    // non-manifold sets nrNonManifoldEdges.  The distribution after a stage
+    // is that of the mesh extracted with the stages up to it.
     void dualMC(const float i0, std::vector<Vertex>& v, std::vector<Normal>& n, std::vector<Triangle>& t,
                 std::vector<Quadrilateral>& q, std::vector<Halfedge>& o_he, std::vector<HalfedgeFace>& o_hef,
                 std::vector<HalfedgeVertex>& o_hev, std::map<std::string, int>& config) {
         if (field_.empty()) throw std::runtime_error("dualMC: call init first");
         auto key = [&](const char* k) { auto it = config.find(k); return it != config.end() && it->second; };
+        clearInfo();
         uint32_t flags = DMC_TRIANGLES;
-        if (key("p3X3YColor")) flags |= DMC_SIMPLIFY_3X3Y_COLOR;
-        if (key("p3X3YOld")) flags |= DMC_SIMPLIFY_3X3Y_ISOLATED;
-        if (key("p3333")) flags |= DMC_SIMPLIFY_3333;
+        const bool valence = key("valence");
+        auto dist = [&](uint32_t f, bool tris, std::vector<unsigned int>& out) {
+            dmc_mesh mm{};
+            tmc_detail::check(engine_.get(),
+                              dmc_extract(engine_.get(), &grid_, field_.data(), DMC_FIELD_F32, (double)i0, f, nullptr,
+                                          &mm),
+                              "dmc_extract");
+            histogram(tris, out);
+        };
+        if (valence) dist(flags, false, valenceDistDMC);
+        if (key("p3X3YColor")) {
+            flags |= DMC_SIMPLIFY_3X3Y_COLOR;
+            if (valence) dist(flags, false, valenceDistP3X3YColor);
+        }
+        if (key("p3X3YOld")) {
+            flags |= DMC_SIMPLIFY_3X3Y_ISOLATED;
+            if (valence) dist(flags, false, valenceDistP3X3YOld);
+        }
+        if (key("p3333")) {
+            flags |= DMC_SIMPLIFY_3333;
+            if (valence) dist(flags, false, valenceDistP3333);
+        }
         dmc_mesh m{};
         tmc_detail::check(engine_.get(),
                           dmc_extract(engine_.get(), &grid_, field_.data(), DMC_FIELD_F32, (double)i0, flags, nullptr,
@@ -451,9 +584,64 @@ public:
         o_he.resize(he ? 4 * m.nq : 0);
         for (size_t h = 0; h < o_he.size(); h++)
             for (int c = 0; c < 4; c++) o_he[h][c] = hes[5 * h + c];
+        if (valence) histogram(true, valenceDistTris);
+        if (key("element-quality")) {
+            elementQualityQuads.resize(m.nq);
+            elementQualityTris.resize(m.nt);
+            if (m.nq)
+                tmc_detail::check(engine_.get(), dmc_element_quality(engine_.get(), 0, elementQualityQuads.data()),
+                                  "dmc_element_quality");
+            if (m.nt)
+                tmc_detail::check(engine_.get(), dmc_element_quality(engine_.get(), 1, elementQualityTris.data()),
+                                  "dmc_element_quality");
+        }
+        if (key("non-manifold")) nrNonManifoldEdges = (int)m.nonmanifold_edges;
+        nrFailedProjections1 = (int)m.failed_projections_l1;
+        nrFailedProjections2 = (int)m.failed_projections_l2;
     }
+    // dmc/HalfedgeMesh.cu:430-450: unique edges of the last mesh (rendering)
+    int edges(std::vector<std::array<int, 2>>& e) {
+        int64_t ne = 0;
+        tmc_detail::check(engine_.get(), dmc_mesh_edges(engine_.get(), nullptr, &ne), "dmc_mesh_edges");
+        e.resize((size_t)ne);
+        if (ne)
+            tmc_detail::check(engine_.get(), dmc_mesh_edges(engine_.get(), e[0].data(), &ne), "dmc_mesh_edges");
+        return (int)ne;
+    }
+    // dmc/DualMarchingCubes.h:94-127
+    void clearInfo() {
+        valenceDistDMC.clear();
+        valenceDistP3X3YColor.clear();
+        valenceDistP3X3YOld.clear();
+        valenceDistP3333.clear();
+        valenceDistTris.clear();
+        elementQualityTris.clear();
+        elementQualityQuads.clear();
+        nrNonManifoldEdges = nrFailedProjections1 = nrFailedProjections2 = 0;
+    }
+    std::vector<unsigned int> valenceDistDMC;
+    std::vector<unsigned int> valenceDistP3X3YColor;
+    std::vector<unsigned int> valenceDistP3X3YOld;
+    std::vector<unsigned int> valenceDistP3333;
+    std::vector<unsigned int> valenceDistTris;
+    std::vector<float> elementQualityTris;
+    std::vector<float> elementQualityQuads;
+    int nrNonManifoldEdges{0};
+    int nrFailedProjections1{0};
+    int nrFailedProjections2{0};
+
     const std::vector<float>& scalars() const { return field_; }
     const dmc_grid& grid() const { return grid_; }
+
+    // valence distribution of the engine's last mesh, trimmed as the
+    // reference trims it (dmc/VertexValence.cu:114-120)
+    void histogram(bool tris, std::vector<unsigned int>& out) {
+        uint32_t h[100];
+        int32_t len = 0;
+        tmc_detail::check(engine_.get(), dmc_valence_histogram(engine_.get(), tris ? 1 : 0, h, &len, nullptr),
+                          "dmc_valence_histogram");
+        out.assign(h, h + len);
+    }
 
     static float surface(SurfaceCase sc, const float x, const float y, const float z) {
         auto sq = [](const float a) { return a * a; };

@@ -79,5 +79,33 @@ int main(int argc, char** argv) {
         if (he[h][3] >= 0 && he[(size_t)he[h][3]][3] != (int)h) twins_ok = 0;
     std::printf("{\"case\": \"cuda_genus2_64\", \"nv\": %zu, \"nq\": %zu, \"nhe\": %zu, \"twins_ok\": %d}\n",
                 cv.size(), cq.size(), he.size(), twins_ok);
+    // the same with the CUDA path's diagnostics switched on (dmc/kernel.cu:26-32)
+    config["valence"] = 1;
+    config["element-quality"] = 1;
+    config["non-manifold"] = 1;
+    cu.dualMC(0.0f, cv, cn, ct, cq, he, hef, hev, config);
+    auto total = [](const std::vector<unsigned int>& d) {
+        size_t s = 0;
+        for (unsigned int x : d) s += x;
+        return s;
+    };
+    std::printf("{\"case\": \"cuda_diag_64\", \"nv\": %zu, \"nq\": %zu, \"dist_dmc\": %zu, \"dist_color\": %zu, "
+                "\"dist_old\": %zu, \"dist_3333\": %zu, \"dist_tris\": %zu, \"nquality_q\": %zu, "
+                "\"nquality_t\": %zu, \"nonmanifold\": %d}\n",
+                cv.size(), cq.size(), total(cu.valenceDistDMC), total(cu.valenceDistP3X3YColor),
+                total(cu.valenceDistP3X3YOld), total(cu.valenceDistP3333), total(cu.valenceDistTris),
+                cu.elementQualityQuads.size(), cu.elementQualityTris.size(), cu.nrNonManifoldEdges);
+    if (!out.empty()) {
+        dump(out + "/cu_v.f32", cv.empty() ? nullptr : cv[0].data(), 3 * cv.size());
+        dump(out + "/cu_quads.i32", cq.empty() ? nullptr : cq[0].data(), 4 * cq.size());
+        dump(out + "/cu_tris.i32", ct.empty() ? nullptr : ct[0].data(), 3 * ct.size());
+        dump(out + "/cu_dist_3333.u32", cu.valenceDistP3333.data(), cu.valenceDistP3333.size());
+        dump(out + "/cu_dist_tris.u32", cu.valenceDistTris.data(), cu.valenceDistTris.size());
+        dump(out + "/cu_quality_q.f32", cu.elementQualityQuads.data(), cu.elementQualityQuads.size());
+        dump(out + "/cu_quality_t.f32", cu.elementQualityTris.data(), cu.elementQualityTris.size());
+        std::vector<std::array<int, 2>> e;
+        cu.edges(e);
+        dump(out + "/cu_edges.i32", e.empty() ? nullptr : e[0].data(), 2 * e.size());
+    }
     return 0;
 }

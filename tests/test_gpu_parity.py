@@ -233,7 +233,75 @@ def test_cpp_dropin_programs(oracle, tmp_path):
     exp2 = oracle.dualmc(g0, s32, oracle.gradient(g0, s32), 0.0, True)
     assert_same(rd("file_v.f64", np.float64).reshape(-1, 3), exp2["v"], "file v")
     assert_same(rd("file_quads.i32", np.int32).reshape(-1, 4), exp2["quads"], "file quads")
+    # the CUDA path's diagnostics switches: every stage's distribution counts
+    # every vertex of its mesh; the final ones, the quality and the edges
+    # match the restatements on the returned mesh
+    from oracle import diagnostics as D
+    cd = res["cuda_diag_64"]
+    assert cd["dist_3333"] == cd["nv"] and cd["dist_tris"] == cd["nv"]
+    assert cd["dist_dmc"] >= cd["dist_color"] >= cd["dist_old"] >= cd["dist_3333"] > 0
+    assert cd["nquality_q"] == cd["nq"] and cd["nquality_t"] == 2 * cd["nq"]
+    cv = rd("cu_v.f32", np.float32).reshape(-1, 3).astype(np.float64)
+    cq = rd("cu_quads.i32", np.int32).reshape(-1, 4)
+    ct = rd("cu_tris.i32", np.int32).reshape(-1, 3)
+    assert np.array_equal(rd("cu_dist_3333.u32", np.uint32), D.valence_histogram(len(cv), cq)[0])
+    assert np.array_equal(rd("cu_dist_tris.u32", np.uint32), D.valence_histogram(len(cv), ct)[0])
+    assert np.array_equal(rd("cu_quality_q.f32", np.float32).view(np.uint32), D.quality_quads(cv, cq).view(np.uint32))
+    assert np.array_equal(rd("cu_quality_t.f32", np.float32).view(np.uint32), D.quality_tris(cv, ct).view(np.uint32))
+    assert np.array_equal(rd("cu_edges.i32", np.int32).reshape(-1, 2), D.mesh_edges(cq))
 
+
+
+# ---------------------------------------------------------------------------
+# diagnostics of the finished mesh (the CUDA reference's edges, valence
+# distributions and element quality) and the OBJ writer, against the
+# restatements of oracle/diagnostics.py on the downloaded mesh
+# ---------------------------------------------------------------------------
+def _diag_meshes():
+    from tmc_amd import workloads
+    f, g, iso = workloads.genus2(40)
+    yield "genus2_40", f, g, iso
+    f, g, iso = workloads.blobs(36, seed=3)
+    yield "blobs_36", f, g, iso
+    for it, f, g, iso in _fields(17, 10):
+        yield f"random_{it}", f, g, iso
+
+
+def _same_f32(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_mesh_diagnostics_match_oracle(engine, tmp_path):
+    from oracle import diagnostics as D
+    nnm = 0
+    for name, f, g, iso in _diag_meshes():
+        _say(f"[gpu] diagnostics {name} shape {f.shape}")
+        m = engine.extract(f, iso, g, flags=SIMPLIFY_ALL | TRIANGLES)
+        got = engine.download(halfedges=False)
+        nv, quads, tris = len(got["v"]), got["quads"], got["tris"]
+        nnm += got["nonmanifold_edges"]
+        assert np.array_equal(engine.edges(), D.mesh_edges(quads)), name
+        for tri, faces in ((False, quads), (True, tris)):
+            h, beyond = engine.valence_histogram(triangles=tri)
+            eh, eb = D.valence_histogram(nv, faces)
+            assert np.array_equal(h, eh) and beyond == eb, (name, tri, h, eh)
+        assert _same_f32(engine.element_quality(), D.quality_quads(got["v"], quads)), name
+        assert _same_f32(engine.element_quality(triangles=True), D.quality_tris(got["v"], tris)), name
+        for tri, faces in ((False, quads), (True, tris)):
+            path = tmp_path / f"{name}_{int(tri)}.obj"
+            engine.write_obj(str(path), triangles=tri)
+            assert path.read_text() == D.obj_text(got["v"], got["n"], faces, tri), (name, tri)
+    assert nnm > 0, "sample must reach non-manifold edges"
+
+
+def test_mesh_diagnostics_need_a_single_extraction(engine):
+    """After a batch the mesh ids are per volume: diagnostics refuse."""
+    from tmc_amd import workloads
+    vols = [workloads.genus2(16), workloads.blobs(16, seed=5)]
+    engine.extract_batch([(f, iso, g) for f, g, iso in vols], flags=SIMPLIFY_ALL | TRIANGLES)
+    for call in (engine.edges, engine.valence_histogram, engine.element_quality):
+        with pytest.raises(tmc_amd.DMCError):
+            call()
 
 
 # ---------------------------------------------------------------------------

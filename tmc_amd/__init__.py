@@ -42,7 +42,7 @@ EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc
            "dmc_server_wait", "dmc_server_download", "dmc_server_release", "dmc_server_stats",
            "dmc_server_last_error", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc",
            "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
-           "dmc_last_error"]
+           "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj"]
 
 
 class DMCError(RuntimeError):
@@ -128,6 +128,11 @@ def load_library(path=ENGINE_SO):
     L.dmc_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
     L.dmc_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
     L.dmc_synchronize.argtypes = [vp]
+    L.dmc_mesh_edges.argtypes = [vp, i32p, C.POINTER(C.c_int64)]
+    L.dmc_valence_histogram.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int64)]
+    L.dmc_element_quality.argtypes = [vp, C.c_int, f32p]
+    L.dmc_write_obj.argtypes = [vp, C.c_char_p, C.c_int]
     L.dmc_status_str.argtypes = [C.c_int]
     L.dmc_status_str.restype = C.c_char_p
     L.dmc_last_error.argtypes = [vp]
@@ -137,7 +142,8 @@ def load_library(path=ENGINE_SO):
                  "dmc_server_poll", "dmc_server_wait", "dmc_server_download", "dmc_server_release",
                  "dmc_server_stats", "dmc_mesh_download_f32",
                  "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
-                 "dmc_copy_to_host", "dmc_synchronize"):
+                 "dmc_copy_to_host", "dmc_synchronize", "dmc_mesh_edges", "dmc_valence_histogram",
+                 "dmc_element_quality", "dmc_write_obj"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -330,6 +336,39 @@ class Engine:
 
     def synchronize(self):
         self._check(self.lib.dmc_synchronize(self.ctx))
+
+    # -- diagnostics of the last mesh ------------------------------------
+    def edges(self):
+        """Unique undirected edges (n, 2) int32, one per vertex pair."""
+        n = C.c_int64(0)
+        self._check(self.lib.dmc_mesh_edges(self.ctx, None, C.byref(n)))
+        e = np.empty((n.value, 2), np.int32)
+        if n.value:
+            self._check(self.lib.dmc_mesh_edges(self.ctx, e.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n)))
+        return e
+
+    def valence_histogram(self, triangles=False):
+        """(hist trimmed like the reference's, vertices of valence >= 100)
+        of the quadrilateral mesh, or of the triangle mesh."""
+        h = (C.c_uint32 * 100)()
+        n = C.c_int32(0)
+        beyond = C.c_int64(0)
+        self._check(self.lib.dmc_valence_histogram(self.ctx, 1 if triangles else 0, h, C.byref(n),
+                                                   C.byref(beyond)))
+        return np.array(h[:n.value], np.uint32), beyond.value
+
+    def element_quality(self, triangles=False):
+        """Per-element quality (float32): quadrilaterals, or triangles."""
+        meshes = [self.mesh] if self.mesh is not None else list(self.batch or [])
+        n = sum(m.nt if triangles else m.nq for m in meshes)
+        q = np.empty(n, np.float32)
+        self._check(self.lib.dmc_element_quality(self.ctx, 1 if triangles else 0,
+                                                 q.ctypes.data_as(C.POINTER(C.c_float))))
+        return q
+
+    def write_obj(self, path, triangles=False):
+        """The mesh as OBJ text in cpp_dmc's main program layout."""
+        self._check(self.lib.dmc_write_obj(self.ctx, os.fsencode(path), 1 if triangles else 0))
 
 
 class Server:

@@ -68,6 +68,8 @@ struct dmc_context {
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
         keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
+    // diagnostics of the last mesh
+    Buf drep, dnew, dval, dhist, dedges, dqual;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // results
     dmc_mesh mesh{};
@@ -87,7 +89,8 @@ struct dmc_context {
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
                       &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
-                      &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow};
+                      &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
+                      &dhist, &dedges, &dqual};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -1065,6 +1068,192 @@ static void do_download_f32(dmc_context* ctx, void* a_) {
 dmc_status dmc_mesh_download_f32(dmc_context* ctx, float* positions, float* normals, int negate_normals) {
     F32Args a{positions, normals, negate_normals};
     return guard(ctx, do_download_f32, &a);
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics of the last mesh
+// ---------------------------------------------------------------------------
+// Diagnostics work on the mesh of a single-volume extraction (a batch's
+// arrays hold per-volume ids).
+static void diag_mesh_ok(dmc_context* ctx) {
+    if (!ctx->have_mesh) fail(ctx, DMC_ERR_STATE, "no mesh: call dmc_extract first");
+    if (ctx->vmesh.size() != 1) fail(ctx, DMC_ERR_STATE, "diagnostics need a single-volume extraction");
+}
+
+struct EdgeArgs {
+    int32_t* edges;
+    int64_t* count;
+};
+
+static void do_edges(dmc_context* ctx, void* a_) {
+    EdgeArgs& a = *static_cast<EdgeArgs*>(a_);
+    diag_mesh_ok(ctx);
+    const dmc_mesh& m = ctx->mesh;
+    const int64_t nv = m.nv, nhe = 4 * m.nq;
+    uint32_t* rep = ensure<uint32_t>(ctx, ctx->drep, nhe);
+    uint32_t* nid = ensure<uint32_t>(ctx, ctx->dnew, nhe);
+    // the halfedge rows and twins are those of the final build
+    if (nhe > 0)
+        hipLaunchKernelGGL(k_edge_reps, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const int32_t*>(m.d_quads), nhe, nv,
+                           static_cast<const uint32_t*>(ctx->off.p), static_cast<const int2*>(ctx->adj.p),
+                           static_cast<const int32_t*>(m.d_twins), static_cast<const uint8_t*>(m.d_he_flags), rep);
+    check_launch(ctx, "k_edge_reps");
+    const int64_t ne = (int64_t)scan(ctx, rep, nid, nhe);
+    *a.count = ne;
+    if (a.edges && ne > 0) {
+        int2* e = ensure<int2>(ctx, ctx->dedges, ne);
+        hipLaunchKernelGGL(k_edge_emit, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const int32_t*>(m.d_quads), nhe, rep, nid, e);
+        check_launch(ctx, "k_edge_emit");
+        HIPC(ctx, hipMemcpyAsync(a.edges, e, ne * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    }
+}
+
+dmc_status dmc_mesh_edges(dmc_context* ctx, int32_t* edges, int64_t* count) {
+    if (!count) return DMC_ERR_INVALID_ARG;
+    EdgeArgs a{edges, count};
+    return guard(ctx, do_edges, &a);
+}
+
+struct ValenceArgs {
+    int triangles;
+    uint32_t* hist;
+    int32_t* len;
+    int64_t* beyond;
+};
+
+static void do_valence_hist(dmc_context* ctx, void* a_) {
+    ValenceArgs& a = *static_cast<ValenceArgs*>(a_);
+    diag_mesh_ok(ctx);
+    const dmc_mesh& m = ctx->mesh;
+    if (a.triangles && !m.d_tris) fail(ctx, DMC_ERR_STATE, "no triangles: extract with DMC_TRIANGLES");
+    const int64_t nv = m.nv;
+    uint32_t* vval = ensure<uint32_t>(ctx, ctx->dval, nv);
+    uint32_t* hist = ensure<uint32_t>(ctx, ctx->dhist, kValenceBins + 1);
+    HIPC(ctx, hipMemsetAsync(hist, 0, (kValenceBins + 1) * sizeof(uint32_t), ctx->stream));
+    if (nv > 0) {
+        hipLaunchKernelGGL(k_vertex_valence, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const int32_t*>(m.d_quads), a.triangles ? m.d_tris : nullptr, nv,
+                           static_cast<const uint32_t*>(ctx->off.p), static_cast<const int2*>(ctx->adj.p), vval);
+        hipLaunchKernelGGL(k_valence_hist, dim3(grid_for(nv, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, vval, nv,
+                           hist);
+    }
+    check_launch(ctx, "valence histogram");
+    uint32_t h[kValenceBins + 1];
+    HIPC(ctx, hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    int32_t len = 1;
+    for (int k = 1; k < kValenceBins; k++)
+        if (h[k] > 0) len = k + 1;
+    memcpy(a.hist, h, kValenceBins * sizeof(uint32_t));
+    *a.len = len;
+    if (a.beyond) *a.beyond = h[kValenceBins];
+}
+
+dmc_status dmc_valence_histogram(dmc_context* ctx, int triangles, uint32_t* hist, int32_t* len, int64_t* beyond) {
+    if (!hist || !len) return DMC_ERR_INVALID_ARG;
+    ValenceArgs a{triangles, hist, len, beyond};
+    return guard(ctx, do_valence_hist, &a);
+}
+
+struct QualityArgs {
+    int which;
+    float* quality;
+};
+
+static void do_quality(dmc_context* ctx, void* a_) {
+    QualityArgs& a = *static_cast<QualityArgs*>(a_);
+    diag_mesh_ok(ctx);
+    const dmc_mesh& m = ctx->mesh;
+    if (a.which == 1 && !m.d_tris) fail(ctx, DMC_ERR_STATE, "no triangles: extract with DMC_TRIANGLES");
+    const int64_t n = a.which == 0 ? m.nq : m.nt;
+    if (n == 0) return;
+    float* q = ensure<float>(ctx, ctx->dqual, n);
+    if (a.which == 0)
+        hipLaunchKernelGGL(k_quality_quads, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const int32_t*>(m.d_quads), n, static_cast<const double*>(m.d_positions), q);
+    else
+        hipLaunchKernelGGL(k_quality_tris, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+                           static_cast<const int32_t*>(m.d_tris), n, static_cast<const double*>(m.d_positions), q);
+    check_launch(ctx, "element quality");
+    HIPC(ctx, hipMemcpyAsync(a.quality, q, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+}
+
+dmc_status dmc_element_quality(dmc_context* ctx, int which, float* quality) {
+    if (!quality || (which != 0 && which != 1)) return DMC_ERR_INVALID_ARG;
+    QualityArgs a{which, quality};
+    return guard(ctx, do_quality, &a);
+}
+
+struct ObjArgs {
+    const char* path;
+    int triangles;
+};
+
+// cpp_dmc/main.cpp:35-87 writes doubles with the stream's default format,
+// which is printf's "%g" (precision 6), and 1-based "i//i" face corners.
+static void do_write_obj(dmc_context* ctx, void* a_) {
+    ObjArgs& a = *static_cast<ObjArgs*>(a_);
+    diag_mesh_ok(ctx);
+    const dmc_mesh& m = ctx->mesh;
+    if (a.triangles && !m.d_tris) fail(ctx, DMC_ERR_STATE, "no triangles: extract with DMC_TRIANGLES");
+    const int corners = a.triangles ? 3 : 4;
+    const int64_t nf = a.triangles ? m.nt : m.nq;
+    std::vector<double> pos(3 * m.nv), nrm(3 * m.nv);
+    std::vector<int32_t> faces(corners * nf);
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    };
+    d2h(pos.data(), m.d_positions, pos.size() * sizeof(double));
+    d2h(nrm.data(), m.d_normals, nrm.size() * sizeof(double));
+    d2h(faces.data(), a.triangles ? m.d_tris : m.d_quads, faces.size() * sizeof(int32_t));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    FILE* f = fopen(a.path, "wb");
+    if (!f) fail(ctx, DMC_ERR_INVALID_ARG, "cannot open %s for writing", a.path);
+    std::string out;
+    out.reserve(1 << 20);
+    char buf[160];
+    auto flush = [&](bool force) {
+        if (force || out.size() > (1u << 20)) {
+            if (!out.empty() && fwrite(out.data(), 1, out.size(), f) != out.size()) {
+                fclose(f);
+                fail(ctx, DMC_ERR_INVALID_ARG, "short write to %s", a.path);
+            }
+            out.clear();
+        }
+    };
+    out += a.triangles ? "#Dual Marching Cubes, triangle mesh\n" : "#Dual Marching Cubes, quad only mesh\n";
+    for (int64_t i = 0; i < m.nv; i++) {
+        snprintf(buf, sizeof(buf), "v %g %g %g\n", pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        out += buf;
+        flush(false);
+    }
+    for (int64_t i = 0; i < m.nv; i++) {
+        snprintf(buf, sizeof(buf), "vn %g %g %g\n", nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        out += buf;
+        flush(false);
+    }
+    for (int64_t i = 0; i < nf; i++) {
+        out += "f";
+        for (int c = 0; c < corners; c++) {
+            const long long x = (long long)faces[corners * i + c] + 1;
+            snprintf(buf, sizeof(buf), " %lld//%lld", x, x);
+            out += buf;
+        }
+        out += "\n";
+        flush(false);
+    }
+    flush(true);
+    if (fclose(f) != 0) fail(ctx, DMC_ERR_INVALID_ARG, "cannot close %s", a.path);
+}
+
+dmc_status dmc_write_obj(dmc_context* ctx, const char* path, int triangles) {
+    if (!path) return DMC_ERR_INVALID_ARG;
+    ObjArgs a{path, triangles};
+    return guard(ctx, do_write_obj, &a);
 }
 
 dmc_status dmc_stage_times(dmc_context* ctx, const char** names, float* ms, int* count) {

@@ -846,6 +846,197 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Diagnostics of a finished mesh
+// ---------------------------------------------------------------------------
+// Unique undirected edges (dmc/HalfedgeMesh.cu:430-450): one per group of
+// halfedges sharing a vertex pair -- the reference's hash key (min, max)
+// (dmc/EdgeHashTable.h:101-127) -- represented by the group's smallest
+// halfedge.  A halfedge with a twin and no flag is in a group of two; the
+// others (boundary, three or four faces, more) take the group from the rows.
+__global__ void __launch_bounds__(kBlock) k_edge_reps(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+                                                       const uint32_t* __restrict__ off,
+                                                       const int2* __restrict__ adj,
+                                                       const int32_t* __restrict__ twin,
+                                                       const uint8_t* __restrict__ flag, uint32_t* __restrict__ rep) {
+    TMC_GRID_STRIDE(h64, nhe) {
+        const int32_t h = (int32_t)h64;
+        const int32_t a = quads[h], b = he_dest(quads, h);
+        bool r = false;
+        if (vid_ok(a, nv) && vid_ok(b, nv)) {
+            const int32_t tw = twin[h];
+            if (flag[h] == 0 && tw >= 0) {
+                r = h < tw;
+            } else {
+                int32_t g[5];
+                const int c = he_group(quads, off, adj, h, nv, g);
+                r = c > 0 && g[0] == h;
+            }
+        }
+        rep[h] = r;
+    }
+}
+
+// Edge list in ascending order of the representatives (newidx = exclusive
+// scan of rep): (origin, destination) of each.
+__global__ void __launch_bounds__(kBlock) k_edge_emit(const int32_t* __restrict__ quads, int64_t nhe,
+                                                       const uint32_t* __restrict__ rep,
+                                                       const uint32_t* __restrict__ newidx,
+                                                       int2* __restrict__ edges) {
+    TMC_GRID_STRIDE(h, nhe) {
+        if (!rep[h]) continue;
+        edges[newidx[h]] = make_int2(quads[h], he_dest(quads, (int32_t)h));
+    }
+}
+
+// Valence as dmc/VertexValence.cu:1-125 counts it: every distinct edge key
+// (min, max) adds one to both end vertices, so a vertex's valence is its
+// number of distinct neighbours, a loop counting twice.  One thread per
+// vertex walks the corners in its halfedge row: with tris == nullptr the
+// neighbours are the quad's vertices before and after the corner, else the
+// other vertices of the quad's two triangles (tris[6q..6q+5]) that contain
+// it.  Rows with more than kValRow corners take a slow loop.
+constexpr int kValRow = 8;
+__device__ __forceinline__ void corner_neighbours(const int32_t* __restrict__ quads,
+                                                  const int32_t* __restrict__ tris, int32_t h, int32_t v,
+                                                  int32_t nb[4]) {
+    const int32_t q = h >> 2, k = h & 3;
+    if (!tris) {
+        nb[0] = quads[4 * q + ((k + 1) & 3)];
+        nb[1] = quads[4 * q + ((k + 3) & 3)];
+        nb[2] = nb[3] = -1;
+        return;
+    }
+    const int32_t* t = tris + 6 * (int64_t)q;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int32_t a = t[3 * s], b = t[3 * s + 1], c = t[3 * s + 2];
+        const bool in = a == v || b == v || c == v;
+        // the two other corners of a triangle containing v (v itself where
+        // the triangle repeats it)
+        nb[2 * s] = !in ? -1 : (a == v ? b : a);
+        nb[2 * s + 1] = !in ? -1 : (a == v || b == v ? c : b);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_vertex_valence(const int32_t* __restrict__ quads,
+                                                            const int32_t* __restrict__ tris, int64_t nv,
+                                                            const uint32_t* __restrict__ off,
+                                                            const int2* __restrict__ adj,
+                                                            uint32_t* __restrict__ vval) {
+    TMC_GRID_STRIDE(v64, nv) {
+        const int32_t v = (int32_t)v64;
+        const uint32_t s = off[v], e = off[v + 1], n = e - s;
+        uint32_t val = 0;
+        if (n <= (uint32_t)kValRow) {
+            int32_t nb[4 * kValRow];
+#pragma unroll
+            for (int i = 0; i < kValRow; i++) {
+                int32_t c[4] = {-1, -1, -1, -1};
+                if ((uint32_t)i < n) corner_neighbours(quads, tris, adj[s + i].x, v, c);
+#pragma unroll
+                for (int x = 0; x < 4; x++) nb[4 * i + x] = c[x];
+            }
+            // count each neighbour at its first occurrence
+#pragma unroll
+            for (int i = 0; i < 4 * kValRow; i++) {
+                const int32_t u = nb[i];
+                bool first = u >= 0;
+#pragma unroll
+                for (int j = 0; j < i; j++) first = first && nb[j] != u;
+                if (first) val += u == v ? 2u : 1u;
+            }
+        } else {
+            for (uint32_t i = s; i < e; i++) {
+                int32_t c[4];
+                corner_neighbours(quads, tris, adj[i].x, v, c);
+                for (int x = 0; x < 4; x++) {
+                    const int32_t u = c[x];
+                    if (u < 0) continue;
+                    bool first = true;
+                    // earlier corners of the row, then earlier slots of this one
+                    for (uint32_t j = s; j < i && first; j++) {
+                        int32_t d[4];
+                        corner_neighbours(quads, tris, adj[j].x, v, d);
+                        first = d[0] != u && d[1] != u && d[2] != u && d[3] != u;
+                    }
+                    for (int y = 0; y < x && first; y++) first = c[y] != u;
+                    if (first) val += u == v ? 2u : 1u;
+                }
+            }
+        }
+        vval[v] = val;
+    }
+}
+
+// Histogram of the valence counts: bins 0 .. kValenceBins-1 and, in the
+// last entry, the vertices beyond them.
+constexpr int kValenceBins = 100;
+__global__ void __launch_bounds__(kBlock) k_valence_hist(const uint32_t* __restrict__ vval, int64_t nv,
+                                                          uint32_t* __restrict__ hist) {
+    __shared__ uint32_t sm[kValenceBins + 1];
+    for (int b = threadIdx.x; b <= kValenceBins; b += kBlock) sm[b] = 0;
+    __syncthreads();
+    TMC_GRID_STRIDE(v, nv) atomicAdd(&sm[min(vval[v], (uint32_t)kValenceBins)], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b <= kValenceBins; b += kBlock)
+        if (sm[b]) atomicAdd(&hist[b], sm[b]);
+}
+
+// Element quality (dmc/ElementQuality.h:65-91, dmc/EstimateElementQuality.cu)
+// in FP32 on the positions rounded to float, with the reference's operand
+// order (its cross product's x component has the opposite sign, which the
+// norm does not see).
+struct QVec {
+    float x, y, z;
+};
+__device__ __forceinline__ QVec q_vertex(const double* __restrict__ pos, int32_t v) {
+    const double* p = pos + 3 * (int64_t)v;
+    return QVec{(float)p[0], (float)p[1], (float)p[2]};
+}
+__device__ __forceinline__ QVec q_diff(const QVec& a, const QVec& b) { return QVec{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ QVec q_cross(const QVec& a, const QVec& b) {
+    return QVec{a.z * b.y - a.y * b.z, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float q_norm2(const QVec& v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+__device__ __forceinline__ float q_norm(const QVec& v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+__global__ void __launch_bounds__(kBlock) k_quality_quads(const int32_t* __restrict__ quads, int64_t nq,
+                                                           const double* __restrict__ pos,
+                                                           float* __restrict__ quality) {
+    TMC_GRID_STRIDE(f, nq) {
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        const QVec v[4] = {q_vertex(pos, q.x), q_vertex(pos, q.y), q_vertex(pos, q.z), q_vertex(pos, q.w)};
+        float m = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            // corner c: edges to the next and to the previous vertex
+            const QVec e1 = q_diff(v[(c + 1) & 3], v[c]), e2 = q_diff(v[(c + 3) & 3], v[c]);
+            const float a = q_norm(q_cross(e1, e2));
+            const float r = 2 * a / (q_norm2(e1) + q_norm2(e2));
+            m = c == 0 ? r : (m < r ? m : r);
+        }
+        quality[f] = m;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_quality_tris(const int32_t* __restrict__ tris, int64_t nt,
+                                                          const double* __restrict__ pos,
+                                                          float* __restrict__ quality) {
+    const float factor = 2 * sqrtf(3.f);
+    TMC_GRID_STRIDE(f, nt) {
+        const int32_t* t = tris + 3 * f;
+        const QVec v0 = q_vertex(pos, t[0]), v1 = q_vertex(pos, t[1]), v2 = q_vertex(pos, t[2]);
+        const float a = q_norm(q_cross(q_diff(v1, v0), q_diff(v2, v0)));
+        const float l0 = q_norm2(q_diff(v1, v0));
+        const float l1 = q_norm2(q_diff(v2, v1));
+        const float l2 = q_norm2(q_diff(v0, v2));
+        float q = 0.f;
+        if (l0 != 0 && l1 != 0 && l2 != 0) q = factor * a / (l0 + l1 + l2);
+        quality[f] = q;
+    }
+}
+
 // collectTriangles (cpp_dmc/DualMarchingCubes.cpp:1387-1460) with minAngle
 // (DualMarchingCubes.h:746-756): split along the diagonal whose two
 // triangles have the larger minimum angle.
