@@ -146,7 +146,7 @@ struct HeStats {
 // and takes its neighbour's with a DPP quad permutation; longer rows are
 // finished from memory.  The group's first member also counts the edge
 // (non-manifold / error).
-constexpr int kRowRegs = 8;
+constexpr int kRowRegs = 6;
 
 __device__ __forceinline__ int next_in_quad(int x) {
     // quad_perm(1, 2, 3, 0): lane k receives lane (k+1) & 3 of its quad
