@@ -544,7 +544,7 @@ void gather_vertices(dmc_context* ctx) {
 // from the current rows, into the spare CSR buffers, which then become the
 // current ones.  Skipped (the next build counts from scratch) when the
 // current rows left halfedges out.
-void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2) {
+void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2, const int32_t* dlist, int64_t ndrop) {
     if (ctx->last_bad != 0) return;
     const uint32_t* off = static_cast<const uint32_t*>(ctx->off.p);
     const int2* adj = static_cast<const int2*>(ctx->adj.p);
@@ -555,9 +555,14 @@ void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2) {
     unsigned int* bad = ensure<unsigned int>(ctx, ctx->rbad, 1);
     HIPC(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
     if (m.nv > 0) {
-        for (int pass = 0; pass < 2; pass++)
-            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
-                               m, off, adj, nn, deg2, pass, bad);
+        // kept vertices: old row length less the corners in dropped quads
+        hipLaunchKernelGGL(k_remap_degree_kept, dim3(grid_for(m.nv)), dim3(kBlock), 0, ctx->stream, m, off, deg2);
+        if (ndrop > 0)
+            hipLaunchKernelGGL(k_remap_drop, dim3(grid_for(ndrop)), dim3(kBlock), 0, ctx->stream, m,
+                               static_cast<const int32_t*>(ctx->quads[ctx->cur].p), dlist, ndrop, deg2);
+        if (m.nrem > 0)
+            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(m.nrem)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
+                               deg2, 1, bad);
         check_launch(ctx, "k_remap_degree");
     }
     const uint64_t rows = scan(ctx, deg2, off2, nn);
@@ -628,7 +633,7 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     check_launch(ctx, "k_compact_quads3x3y");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
-               qn);
+               qn, static_cast<const int32_t*>(ctx->mlist.p), nq - qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;
@@ -681,8 +686,12 @@ void simplify3333(dmc_context* ctx) {
                            newQ, p3333, vtwin, vmap, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3333");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
+    // the dropped quadrilaterals, for the row degrees
+    int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, std::max<int64_t>(nq - qn, 1));
+    if (nq > qn)
+        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, rflag, newQ, nq, dlist);
     remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
-               qn);
+               qn, dlist, nq - qn);
     ctx->cur = dst;
     ctx->mesh.nv = nn;
     ctx->mesh.nq = qn;

@@ -771,6 +771,30 @@ __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__
     TMC_GRID_STRIDE(x, nv) vmap[x] = vflag[x] ? -1 : (int32_t)newv[x];
 }
 
+// Degree of the new row of every kept vertex: its old row length (a
+// streaming pass), less one for each of its corners in a dropped quadrilateral
+// (k_remap_drop, over the list of dropped ones).  The removed vertices then
+// add their surviving entries (k_remap_degree, pass 1).
+__global__ void __launch_bounds__(kBlock) k_remap_degree_kept(Remap m, const uint32_t* __restrict__ off,
+                                                               uint32_t* __restrict__ deg2) {
+    TMC_GRID_STRIDE(x, m.nv) {
+        if (m.vrem[x]) continue;
+        deg2[m.vmap[x]] = off[x + 1] - off[x];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_remap_drop(Remap m, const int32_t* __restrict__ quads,
+                                                        const int32_t* __restrict__ dlist, int64_t nd,
+                                                        uint32_t* __restrict__ deg2) {
+    TMC_GRID_STRIDE(i, nd) {
+        const int4 q = reinterpret_cast<const int4*>(quads)[dlist[i]];
+        const int32_t v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (vid_ok(v[c], m.nv) && !m.vrem[v[c]]) atomicSub(&deg2[m.vmap[v[c]]], 1u);
+    }
+}
+
 // pass 0: degree of every new row from its kept preimage; pass 1: removed
 // vertices add theirs (and count halfedges left without a valid origin).
 // Rows are walked four entries at a time with every lookup of a chunk
