@@ -411,22 +411,41 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
         // the row's active cells are contiguous in the flat order: process
         // them 64 at a time across segment boundaries
         const int64_t rs0 = V.rsBase + r * S;
-        const uint32_t first = segBase[rs0];
+        // the row's segment bases, one per lane (rows of at most 64 segments;
+        // wider rows search memory)
+        const bool regs = S <= kWave;
+        const uint32_t mybase = (regs && lane < S) ? segBase[rs0 + lane] : 0u;
+        const uint32_t first = regs ? (uint32_t)__builtin_amdgcn_readfirstlane((int)mybase) : segBase[rs0];
         const uint32_t last = segBase[rs0 + S - 1] + segCount[rs0 + S - 1];
         for (uint32_t a0 = first; a0 < last; a0 += kWave) {
             const uint32_t a = a0 + lane;
             const bool act = a < last;
+            // segment of a: the last s with segBase[rs0 + s] <= a
+            int seg = 0;
+            uint32_t sbase = first;
+            if (regs) {
+                for (int s2 = 1; s2 < S; s2++) {
+                    const uint32_t b = (uint32_t)__shfl((int)mybase, s2, kWave);
+                    if (b <= a) {
+                        seg = s2;
+                        sbase = b;
+                    }
+                }
+            }
             uint32_t nv = 0, nq = 0, code = 0, aux = 0;
             int i = 0;
             if (act) {
-                // segment of a: the last s with segBase[rs0 + s] <= a
-                int lo = 0, hi = S - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (segBase[rs0 + mid] <= a) lo = mid;
-                    else hi = mid - 1;
+                if (!regs) {
+                    int lo = 0, hi = S - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (segBase[rs0 + mid] <= a) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    seg = lo;
+                    sbase = segBase[rs0 + lo];
                 }
-                const uint32_t x = lists[(rs0 + lo) * kSeg + (a - segBase[rs0 + lo])];
+                const uint32_t x = lists[(rs0 + seg) * kSeg + (a - sbase)];
                 i = (int)(x & 0xFFFFFFu);
                 const unsigned icase = x >> 24;
                 uint64_t w;
