@@ -473,7 +473,7 @@ void color_faces(dmc_context* ctx) {
     uint32_t* totals = hist + (size_t)kColorBlocks * 24;
     int32_t* list = ensure<int32_t>(ctx, ctx->clist, nq);
     hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist);
-    hipLaunchKernelGGL(k_color_offsets, dim3(1), dim3(64), 0, ctx->stream, hist, nb, totals);
+    hipLaunchKernelGGL(k_color_offsets, dim3(1), dim3(kColorBlocks), 0, ctx->stream, hist, nb, totals);
     hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist, list);
     uint32_t h[48];
     HIPC(ctx, hipMemcpyAsync(h, totals, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

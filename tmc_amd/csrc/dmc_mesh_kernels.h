@@ -340,35 +340,24 @@ __global__ void __launch_bounds__(kBlock) k_color_hist(const int32_t* __restrict
     if (threadIdx.x < 24) hist[blockIdx.x * 24 + threadIdx.x] = sm[threadIdx.x];
 }
 
-// One block: totals per colour, class starts, and per-(block, colour)
-// offsets in place of the histogram.
-__global__ void __launch_bounds__(64) k_color_offsets(uint32_t* __restrict__ hist, int nblocks,
-                                                       uint32_t* __restrict__ totals) {
-    __shared__ uint32_t tot[24], start[24];
-    const int c = threadIdx.x;
-    if (c < 24) {
-        uint32_t t = 0;
-        for (int b = 0; b < nblocks; b++) t += hist[b * 24 + c];
-        tot[c] = t;
-    }
-    __syncthreads();
-    if (c == 0) {
-        uint32_t o = 0;
-        for (int x = 0; x < 24; x++) {
-            start[x] = o;
-            o += tot[x];
+// One block of kColorBlocks threads: totals per colour, class starts, and
+// per-(block, colour) offsets in place of the histogram -- colour by colour,
+// a block scan over the histogram blocks (thread b holds block b).
+__global__ void __launch_bounds__(kColorBlocks) k_color_offsets(uint32_t* __restrict__ hist, int nblocks,
+                                                                 uint32_t* __restrict__ totals) {
+    __shared__ uint32_t sm[kColorBlocks / kWave + 1];
+    const int b = threadIdx.x;
+    uint32_t start = 0;
+    for (int c = 0; c < 24; c++) {
+        const uint32_t h = b < nblocks ? hist[b * 24 + c] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<uint32_t, kColorBlocks>(h, sm, tot);
+        if (b < nblocks) hist[b * 24 + c] = start + ex;
+        if (b == 0) {
+            totals[c] = tot;
+            totals[24 + c] = start;
         }
-    }
-    __syncthreads();
-    if (c < 24) {
-        uint32_t o = start[c];
-        for (int b = 0; b < nblocks; b++) {
-            const uint32_t h = hist[b * 24 + c];
-            hist[b * 24 + c] = o;
-            o += h;
-        }
-        totals[c] = tot[c];
-        totals[24 + c] = start[c];
+        start += tot;
     }
 }
 
