@@ -67,7 +67,7 @@ struct dmc_context {
     int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
     // halfedges / simplification scratch
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        keepV, newV, keepQ, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
+        newV, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
@@ -87,8 +87,8 @@ struct dmc_context {
         Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &aaux, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
                       &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
-                      &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &keepV, &newV,
-                      &keepQ, &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
+                      &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
+                      &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
                       &dhist, &dedges, &dqual};
         for (Buf* b : all) bufs.push_back(b);
@@ -158,19 +158,31 @@ void check_launch(dmc_context* ctx, const char* what) {
 }
 
 // Exclusive scan of n uint32 counts into out; returns the 64-bit total.
-uint64_t scan(dmc_context* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
+template <class In>
+uint64_t scan_any(dmc_context* ctx, In in, uint32_t* out, int64_t n) {
     if (n <= 0) return 0;
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
     uint64_t* tot = ensure<uint64_t>(ctx, ctx->scanTotal, 1);
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, n, part);
+    hipLaunchKernelGGL(k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, n, part);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, tiles, tot);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
+    hipLaunchKernelGGL(k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
     check_launch(ctx, "scan");
     uint64_t total = 0;
     HIPC(ctx, hipMemcpyAsync(&total, tot, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
     return total;
+}
+
+// Exclusive scan of uint32 counts; returns the total.
+uint64_t scan(dmc_context* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
+    return scan_any(ctx, ScanCounts{in}, out, n);
+}
+
+// New indices of the kept items (exclusive scan of removed[i] == 0);
+// returns the number kept.
+uint64_t scan_keep(dmc_context* ctx, const uint8_t* removed, uint32_t* out, int64_t n) {
+    return scan_any(ctx, ScanKeep{removed}, out, n);
 }
 
 uint64_t read_u64(dmc_context* ctx, const void* dptr) {
@@ -503,11 +515,8 @@ void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, int64_t
 // at the end).
 int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond, uint32_t** newidx_out) {
     const int64_t nv = ctx->mesh.nv;
-    uint32_t* keep = ensure<uint32_t>(ctx, ctx->keepV, nv);
     uint32_t* newidx = ensure<uint32_t>(ctx, ctx->newV, nv);
-    if (nv > 0)
-        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, nv, keep);
-    const int64_t nn = (int64_t)scan(ctx, keep, newidx, nv);
+    const int64_t nn = (int64_t)scan_keep(ctx, removed, newidx, nv);
     const uint32_t* slot = ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr;
     uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], nn);
     int32_t* rlist = ensure<int32_t>(ctx, ctx->rlist, nv - nn);
@@ -599,17 +608,15 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
-    uint32_t* keepQ = ensure<uint32_t>(ctx, ctx->keepQ, nq);
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
     if (nq > 0) {
         hipLaunchKernelGGL(k_mark3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p);
         hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
                            colors, twin, flag, lastT, lastR, em, vfirst);
-        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, nq, keepQ);
         check_launch(ctx, "merge3x3y");
     }
     // the quadrilateral compaction's scan also lists the merging faces
-    const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
+    const int64_t qn = (int64_t)scan_keep(ctx, em, newQ, nq);
     if (nq > qn) {
         int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, nq - qn);
         hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, newQ, nq, mlist);
@@ -669,11 +676,8 @@ void simplify3333(dmc_context* ctx) {
     }
     uint32_t* newv = nullptr;
     const int64_t nn = compact_vertices(ctx, vflag, nullptr, &newv);
-    uint32_t* keepQ = ensure<uint32_t>(ctx, ctx->keepQ, nq);
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
-    if (nq > 0)
-        hipLaunchKernelGGL(k_keep_from_flag, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, rflag, nq, keepQ);
-    const int64_t qn = (int64_t)scan(ctx, keepQ, newQ, nq);
+    const int64_t qn = (int64_t)scan_keep(ctx, rflag, newQ, nq);
     const int dst = 1 - ctx->cur;
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
     int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);

@@ -73,23 +73,58 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T* smem, T& total) {
 }
 
 // ---------------------------------------------------------------------------
-// Device-wide exclusive scan of uint32 counts (three passes over tiles of
-// kScanTile items).  Totals are accumulated in 64 bits so that overflow of
+// Device-wide exclusive scan of uint32 counts, or of keep flags read from
+// byte flags (three passes over tiles of kScanTile items).  Totals are accumulated in 64 bits so that overflow of
 // the 32-bit offsets can be detected by the host.
 // ---------------------------------------------------------------------------
 constexpr int kScanIPT = 16;
 constexpr int kScanTile = kBlock * kScanIPT;
 
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
-                                                         uint64_t* __restrict__ partial) {
+// Scan inputs: uint32 counts, or keep flags (1 where a byte flag is 0).
+// Thread t of a tile holds the 16 consecutive items from base = tile * kScanTile + 16 t.
+struct ScanCounts {
+    const uint32_t* p;
+    __device__ __forceinline__ uint32_t at(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ void vec16(int64_t base, uint32_t v[kScanIPT]) const {
+#pragma unroll
+        for (int q = 0; q < kScanIPT / 4; q++) {
+            const uint4 t = reinterpret_cast<const uint4*>(p + base)[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    }
+};
+struct ScanKeep {
+    const uint8_t* p;   // removed flags
+    __device__ __forceinline__ uint32_t at(int64_t i) const { return p[i] ? 0u : 1u; }
+    __device__ __forceinline__ void vec16(int64_t base, uint32_t v[kScanIPT]) const {
+        const uint4 t = *reinterpret_cast<const uint4*>(p + base);
+        const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++) v[q] = ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) ? 0u : 1u;
+    }
+};
+
+template <class In>
+__device__ __forceinline__ void scan_items(const In& in, int64_t n, int64_t base, uint32_t v[kScanIPT]) {
+    if (base + kScanIPT <= n) {
+        in.vec16(base, v);   // all 16 items in range: vector loads
+    } else {
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++) v[q] = base + q < n ? in.at(base + q) : 0u;
+    }
+}
+
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, int64_t n, uint64_t* __restrict__ partial) {
     __shared__ uint64_t sm[kBlock / kWave + 1];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t v[kScanIPT];
+    scan_items(in, n, (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT, v);
     uint64_t s = 0;
 #pragma unroll
-    for (int q = 0; q < kScanIPT; q++) {
-        const int64_t idx = base + (int64_t)q * kBlock + threadIdx.x;
-        if (idx < n) s += in[idx];
-    }
+    for (int q = 0; q < kScanIPT; q++) s += v[q];
     s = wave_sum(s);
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     if (lane == 0) sm[wid] = s;
@@ -117,25 +152,14 @@ __global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ p
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                       int64_t n, const uint64_t* __restrict__ partial) {
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restrict__ out, int64_t n,
+                                                       const uint64_t* __restrict__ partial) {
     __shared__ uint32_t sm[kBlock / kWave + 1];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
-    const bool full = base + kScanIPT <= n;   // all 16 items in range: 16-B vector accesses
+    const bool full = base + kScanIPT <= n;
     uint32_t v[kScanIPT];
-    if (full) {
-#pragma unroll
-        for (int q = 0; q < kScanIPT / 4; q++) {
-            const uint4 t = reinterpret_cast<const uint4*>(in + base)[q];
-            v[4 * q] = t.x;
-            v[4 * q + 1] = t.y;
-            v[4 * q + 2] = t.z;
-            v[4 * q + 3] = t.w;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < kScanIPT; q++) v[q] = base + q < n ? in[base + q] : 0u;
-    }
+    scan_items(in, n, base, v);
     uint32_t s = 0;
 #pragma unroll
     for (int q = 0; q < kScanIPT; q++) s += v[q];

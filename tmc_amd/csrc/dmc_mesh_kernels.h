@@ -539,13 +539,6 @@ __global__ void __launch_bounds__(kBlock) k_list_flagged(const uint8_t* __restri
     }
 }
 
-// keep flag (1 - removed) as uint32 for the scan
-__global__ void __launch_bounds__(kBlock) k_keep_from_flag(const uint8_t* __restrict__ removed, int64_t n,
-                                                            uint32_t* __restrict__ keep) {
-    TMC_GRID_STRIDE(x, n) keep[x] = removed[x] ? 0u : 1u;
-}
-
-// Kept vertices: record their new index in vsecond and move their data.
 // Vertex compaction without moving positions and normals: vertex v's data
 // lives at slot vslot[v] (the identity when vslot is null); the compaction
 // only carries the slots along (4 B per vertex instead of 48 B), and one
