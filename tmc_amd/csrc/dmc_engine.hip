@@ -71,6 +71,9 @@ struct dmc_context {
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
+    // pinned host staging of device-to-host reads (counts, totals, tables)
+    void* pin = nullptr;
+    size_t pin_cap = 0;
     // results
     dmc_mesh mesh{};
     bool have_mesh = false;
@@ -157,6 +160,62 @@ void check_launch(dmc_context* ctx, const char* what) {
     if (e != hipSuccess) fail(ctx, DMC_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
 }
 
+// Device-to-host reads, gathered into pinned host memory so that each is a
+// direct transfer (a pageable destination would be staged through the
+// runtime's bounce buffer), then one stream synchronisation for all.
+class HostReads {
+   public:
+    explicit HostReads(dmc_context* ctx) : ctx_(ctx) {}
+    void add(void* dst, const void* src, size_t bytes) {
+        if (bytes == 0) return;
+        outs_.push_back({dst, src, used_, bytes});
+        used_ += (bytes + 255) & ~size_t(255);
+    }
+    void wait() {
+        if (outs_.empty()) return;
+        if (ctx_->pin_cap < used_) {
+            HIPC(ctx_, hipStreamSynchronize(ctx_->stream));
+            if (ctx_->pin) {
+                (void)hipHostFree(ctx_->pin);
+                ctx_->pin = nullptr;
+                ctx_->pin_cap = 0;
+            }
+            const size_t want = std::max<size_t>(used_, 64 << 10);
+            if (hipHostMalloc(&ctx_->pin, want) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx_->pin = nullptr;
+                fail(ctx_, DMC_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed", want);
+            }
+            ctx_->pin_cap = want;
+        }
+        char* pin = static_cast<char*>(ctx_->pin);
+        for (const Out& o : outs_)
+            HIPC(ctx_, hipMemcpyAsync(pin + o.at, o.src, o.bytes, hipMemcpyDeviceToHost, ctx_->stream));
+        HIPC(ctx_, hipStreamSynchronize(ctx_->stream));
+        for (const Out& o : outs_) std::memcpy(o.dst, pin + o.at, o.bytes);
+        outs_.clear();
+        used_ = 0;
+    }
+
+   private:
+    struct Out {
+        void* dst;
+        const void* src;
+        size_t at, bytes;
+    };
+    dmc_context* ctx_;
+    std::vector<Out> outs_;
+    size_t used_ = 0;
+};
+
+uint64_t read_u64(dmc_context* ctx, const void* dptr) {
+    uint64_t v = 0;
+    HostReads r(ctx);
+    r.add(&v, dptr, sizeof(v));
+    r.wait();
+    return v;
+}
+
 // Exclusive scan of n uint32 counts into out; returns the 64-bit total.
 template <class In>
 uint64_t scan_any(dmc_context* ctx, In in, uint32_t* out, int64_t n) {
@@ -168,10 +227,7 @@ uint64_t scan_any(dmc_context* ctx, In in, uint32_t* out, int64_t n) {
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, tiles, tot);
     hipLaunchKernelGGL(k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
     check_launch(ctx, "scan");
-    uint64_t total = 0;
-    HIPC(ctx, hipMemcpyAsync(&total, tot, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    return total;
+    return read_u64(ctx, tot);
 }
 
 // Exclusive scan of uint32 counts; returns the total.
@@ -185,12 +241,6 @@ uint64_t scan_keep(dmc_context* ctx, const uint8_t* removed, uint32_t* out, int6
     return scan_any(ctx, ScanKeep{removed}, out, n);
 }
 
-uint64_t read_u64(dmc_context* ctx, const void* dptr) {
-    uint64_t v = 0;
-    HIPC(ctx, hipMemcpyAsync(&v, dptr, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // cell stage
@@ -379,8 +429,9 @@ void three_face_edges(dmc_context* ctx, const HeStats& hs, int64_t nv, ThreeLaun
     launch(ovr, recs, nrec, 1);
     check_launch(ctx, "three-face edges");
     std::vector<BndRec> h(hs.three);
-    HIPC(ctx, hipMemcpyAsync(h.data(), recs, hs.three * sizeof(BndRec), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    HostReads r(ctx);
+    r.add(h.data(), recs, hs.three * sizeof(BndRec));
+    r.wait();
     std::sort(h.begin(), h.end(), [](const BndRec& a, const BndRec& b) { return a.key < b.key; });
     for (const BndRec& r : h) {
         ctx->bnd.push_back(r.a);
@@ -410,9 +461,9 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
         // rows from the cell stage are complete unless a quadrilateral had an
         // unresolved neighbour cell
         uint32_t unresolved = 0;
-        HIPC(ctx, hipMemcpyAsync(&unresolved, static_cast<uint32_t*>(ctx->fails.p) + 5, sizeof(unresolved),
-                                 hipMemcpyDeviceToHost, ctx->stream));
-        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        HostReads r(ctx);
+        r.add(&unresolved, static_cast<uint32_t*>(ctx->fails.p) + 5, sizeof(unresolved));
+        r.wait();
         fresh = unresolved == 0;
     } else if (ready == 2) {
         // remapped rows: halfedges left without a valid origin were counted
@@ -453,8 +504,11 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
         hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, adj, nv, hev);
         check_launch(ctx, "k_he_v");
     }
-    HIPC(ctx, hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    {
+        HostReads r(ctx);
+        r.add(&hs, st, sizeof(hs));
+        r.wait();
+    }
     ctx->last_bad = hs.bad;
     if (final_pass)
         three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
@@ -488,8 +542,9 @@ void color_faces(dmc_context* ctx) {
     hipLaunchKernelGGL(k_color_offsets, dim3(1), dim3(kColorBlocks), 0, ctx->stream, hist, nb, totals);
     hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist, list);
     uint32_t h[48];
-    HIPC(ctx, hipMemcpyAsync(h, totals, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    HostReads r(ctx);
+    r.add(h, totals, sizeof(h));
+    r.wait();
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     for (int c = 5; c < 24; c++)
         if (h[c])
@@ -788,15 +843,14 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
     std::vector<int64_t> vb(2 * (n + 1)), qb(2 * (n + 1)), ab(n + 1), vhe(n > 1 ? 2 * n : 0);
     std::vector<VolStat> vs(n);
     uint32_t diag[6];
-    HIPC(ctx, hipMemcpyAsync(vb.data(), ctx->vbound.p, vb.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipMemcpyAsync(qb.data(), ctx->qbound.p, qb.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipMemcpyAsync(ab.data(), vol_ptr(ctx, 2), ab.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipMemcpyAsync(vs.data(), ctx->vstat.p, vs.size() * sizeof(VolStat), hipMemcpyDeviceToHost, ctx->stream));
-    if (n > 1)
-        HIPC(ctx, hipMemcpyAsync(vhe.data(), ctx->vhe.p, vhe.size() * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipMemcpyAsync(diag, ctx->fails.p, sizeof(diag), hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    HostReads r(ctx);
+    r.add(vb.data(), ctx->vbound.p, vb.size() * sizeof(int64_t));
+    r.add(qb.data(), ctx->qbound.p, qb.size() * sizeof(int64_t));
+    r.add(ab.data(), vol_ptr(ctx, 2), ab.size() * sizeof(int64_t));
+    r.add(vs.data(), ctx->vstat.p, vs.size() * sizeof(VolStat));
+    if (n > 1) r.add(vhe.data(), ctx->vhe.p, vhe.size() * sizeof(unsigned long long));
+    r.add(diag, ctx->fails.p, sizeof(diag));
+    r.wait();
     ctx->mesh.valence_zero = diag[4];
     ctx->mesh.invalid_refs += diag[5];
     finish_timing(ctx);
@@ -969,6 +1023,7 @@ dmc_status dmc_destroy(dmc_context* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (Buf* b : ctx->bufs)
         if (b->p) (void)hipFree(b->p);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DMC_OK;
