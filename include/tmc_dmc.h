@@ -145,14 +145,17 @@ dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions
 
 /* ------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference, which extracts one grid per
- * synchronous dualMC call).  A server owns one context and a worker thread:
- * waiting requests are admitted oldest first while their estimated working
- * set plus the results still held fits budget_bytes (and at most max_batch
- * volumes and max_batch_points grid points per pass, the first request of a
- * pass always admitted), run as one dmc_extract_batch, and completed
- * together.  Results
- * stay in device memory (pooled blocks) until dmc_server_release.  With
- * max_batch = 1 the same loop serves one request at a time.
+ * synchronous dualMC call).  A server owns `workers` worker threads, each
+ * with its own context (and so its own stream): a free worker admits the
+ * waiting requests oldest first while their estimated working set plus the
+ * passes in flight and the results still held fits budget_bytes (and at
+ * most max_batch volumes and max_batch_points grid points per pass, the
+ * first request of a pass always admitted), runs them as one
+ * dmc_extract_batch and completes them together.  Passes of different
+ * workers overlap on the GPU, so one fills the other's host round trips.
+ * Results stay in device memory (pooled blocks) until dmc_server_release.
+ * With max_batch = 1 and one worker the same loop serves one request at a
+ * time.
  * ---------------------------------------------------------------------- */
 typedef struct dmc_server dmc_server;
 
@@ -163,6 +166,7 @@ typedef struct {
     int32_t max_batch;       /* volumes per pass, >= 1 */
     size_t budget_bytes;     /* admission budget; 0 = 96 GiB */
     int64_t max_batch_points;  /* grid points per pass (bounds a pass's duration); 0 = no limit */
+    int32_t workers;         /* passes run concurrently, each with its own context and stream; 0 = 1 */
 } dmc_server_opts;
 
 typedef struct {

@@ -424,13 +424,16 @@ def _server_results(srv, vols, threads=1):
     return out
 
 
-@pytest.mark.parametrize("max_batch,threads", [(1, 1), (1 << 20, 1), (1 << 20, 4), (3, 2)])
-def test_server_matches_alone(engine, max_batch, threads):
+@pytest.mark.parametrize("max_batch,threads,workers",
+                         [(1, 1, 1), (1 << 20, 1, 1), (1 << 20, 4, 1), (3, 2, 1), (3, 4, 3), (1, 2, 2)])
+def test_server_matches_alone(engine, max_batch, threads, workers):
+    """Every request's mesh equals its own dualMC, whatever the batching,
+    the submitting threads and the concurrent passes."""
     flags = SIMPLIFY_ALL | TRIANGLES
     vols = _batch_volumes(53, 12)
-    _say(f"[gpu] server max_batch={max_batch} threads={threads}, {len(vols)} volumes")
+    _say(f"[gpu] server max_batch={max_batch} threads={threads} workers={workers}, {len(vols)} volumes")
     exp = [_alone(engine, f, iso, g, flags) for f, iso, g in vols]
-    srv = tmc_amd.Server(0, max_batch=max_batch, flags=flags)
+    srv = tmc_amd.Server(0, max_batch=max_batch, flags=flags, workers=workers)
     try:
         got = _server_results(srv, vols, threads)
         for i, (es, e) in enumerate(exp):
@@ -447,10 +450,12 @@ def test_server_matches_alone(engine, max_batch, threads):
         srv.close()
 
 
-def test_server_budget_admission(engine):
+@pytest.mark.parametrize("workers", [1, 2])
+def test_server_budget_admission(engine, workers):
     """A small budget splits the waiting requests into several batches and a
     request larger than the budget fails with DMC_ERR_BUDGET; the others
-    are unaffected."""
+    are unaffected.  With two workers the passes in flight share the
+    budget."""
     flags = SIMPLIFY_ALL | TRIANGLES
     rng = np.random.default_rng(5)
     vols = [(rng.standard_normal((10, 10, 10)).astype(np.float32), 0.2, tmc_amd.Grid.make((10, 10, 10)))
@@ -458,7 +463,7 @@ def test_server_budget_admission(engine):
     big = (rng.standard_normal((40, 40, 40)).astype(np.float32), 0.2, tmc_amd.Grid.make((40, 40, 40)))
     exp = [_alone(engine, f, iso, g, flags)[1] for f, iso, g in vols]
     # one 10^3 volume: 81 rows x 1 segment x 1032 B + 40 B x 1000 points + 4,000 B of field = 127,592 B
-    srv = tmc_amd.Server(0, budget_bytes=300_000, flags=flags)
+    srv = tmc_amd.Server(0, budget_bytes=300_000, flags=flags, workers=workers)
     try:
         tk = [srv.submit(f, iso, g) for f, iso, g in vols[:3]]
         tb = srv.submit(*big)

@@ -83,7 +83,7 @@ class Mesh(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("field_type", C.c_int), ("flags", C.c_uint32), ("max_batch", C.c_int32),
-                ("budget_bytes", C.c_size_t), ("max_batch_points", C.c_int64)]
+                ("budget_bytes", C.c_size_t), ("max_batch_points", C.c_int64), ("workers", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -374,17 +374,18 @@ class Engine:
 class Server:
     """Serving loop (dmc_server_*): submit volumes, wait for them, download,
     release.  Requests waiting together are extracted as one batch (up to
-    `max_batch`, within `budget_bytes` of HBM); max_batch=1 serves them one
-    at a time."""
+    `max_batch`, within `budget_bytes` of HBM); `workers` passes run
+    concurrently, each on its own context and stream; max_batch=1 with one
+    worker serves them one at a time."""
 
     def __init__(self, device=0, max_batch=1 << 20, budget_bytes=0, flags=SIMPLIFY_ALL | TRIANGLES,
-                 dtype=np.float32, max_batch_points=0):
+                 dtype=np.float32, max_batch_points=0, workers=1):
         self.lib = load_library()
         self.srv = C.c_void_p()
         self.dtype = np.dtype(dtype)
         self.flags = int(flags)
         opts = ServerOpts(int(device), FIELD_F32 if self.dtype == np.float32 else FIELD_F64, self.flags,
-                          int(max_batch), int(budget_bytes), int(max_batch_points))
+                          int(max_batch), int(budget_bytes), int(max_batch_points), int(workers))
         s = self.lib.dmc_server_create(C.byref(self.srv), C.byref(opts))
         if s != 0:
             raise DMCError(f"dmc_server_create: {self.lib.dmc_status_str(s).decode()}")

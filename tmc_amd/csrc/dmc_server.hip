@@ -1,13 +1,16 @@
 // dmc_server.hip -- serving loop over the batched extraction (host code).
 //
-// Requests (one volume each) are queued by dmc_server_submit.  One worker
-// thread owns a DMC context and repeatedly takes the oldest waiting
-// requests -- as many as fit the HBM budget and max_batch -- runs them as
-// one dmc_extract_batch, copies the union mesh into a result block (one
-// device-to-device copy per array) and completes the requests.  A request
-// keeps its slice of the block until dmc_server_release; blocks are pooled
-// and reused.  With max_batch = 1 the same loop serves one request at a
-// time, the baseline the batched mode is measured against.
+// Requests (one volume each) are queued by dmc_server_submit.  Each worker
+// thread owns a DMC context (and so a stream of its own) and repeatedly
+// takes the oldest waiting requests -- as many as fit the HBM budget and
+// max_batch -- runs them as one dmc_extract_batch, copies the union mesh
+// into a result block (one device-to-device copy per array) and completes
+// the requests.  The passes of different workers overlap on the GPU: while
+// one waits on a host round trip (a count read back between stages), the
+// other's kernels run.  A request keeps its slice of the block until
+// dmc_server_release; blocks are pooled and reused.  With max_batch = 1 and
+// one worker the same loop serves one request at a time, the baseline the
+// batched mode is measured against.
 //
 // Built only on the public C ABI of include/tmc_dmc.h plus HIP stream/copy
 // calls; no counterpart in the reference (which has no serving path).
@@ -30,6 +33,10 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Upper bound of dmc_server_opts.workers (each worker holds a context's
+// buffers, grown to its largest pass).
+constexpr int kMaxWorkers = 8;
 
 // One pooled device allocation holding the meshes of one batch.
 struct Block {
@@ -66,10 +73,16 @@ size_t estimate_bytes(const dmc_volume& v, int field_type, bool on_device) {
 
 }  // namespace
 
+// One worker: a context and a stream for its result copies.
+struct Worker {
+    dmc_context* ctx = nullptr;
+    hipStream_t copy_stream = nullptr;
+    std::thread th;
+};
+
 struct dmc_server {
     dmc_server_opts opts{};
-    dmc_context* ctx = nullptr;
-    hipStream_t copy_stream = nullptr;  // result copies (worker thread)
+    std::vector<std::unique_ptr<Worker>> workers;
     hipStream_t dl_stream = nullptr;    // downloads (caller threads, one at a time)
     std::mutex mu, dl_mu;
     std::condition_variable cv_work, cv_done;
@@ -77,9 +90,9 @@ struct dmc_server {
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     std::vector<Block*> blocks;         // every block (pooled)
     size_t held_bytes = 0;              // results not yet released
+    size_t inflight_bytes = 0;          // working sets of the passes running
     uint64_t next_ticket = 1;
     bool stop = false;
-    std::thread worker;
     Clock::time_point t0 = Clock::now();
     std::string err;
     // totals
@@ -117,13 +130,13 @@ Block* get_block(dmc_server* s, size_t bytes) {
     return b;
 }
 
-void run_batch(dmc_server* s, std::vector<Request*>& batch) {
+void run_batch(dmc_server* s, Worker* w, std::vector<Request*>& batch) {
     const int n = (int)batch.size();
     std::vector<dmc_volume> vols(n);
     for (int i = 0; i < n; i++) vols[i] = batch[i]->vol;
     std::vector<dmc_mesh> out(n);
     const double t_start = s->now();
-    dmc_status st = dmc_extract_batch(s->ctx, vols.data(), n, s->opts.field_type, s->opts.flags, out.data());
+    dmc_status st = dmc_extract_batch(w->ctx, vols.data(), n, s->opts.field_type, s->opts.flags, out.data());
     int64_t V = 0, Q = 0;
     for (int i = 0; i < n && st == DMC_OK; i++) {
         V += out[i].nv;
@@ -135,7 +148,7 @@ void run_batch(dmc_server* s, std::vector<Request*>& batch) {
         for (int i = 0; i < n; i++) {
             bnd[i].resize(out[i].nbnd);
             if (out[i].nbnd)
-                st = dmc_batch_download(s->ctx, i, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                st = dmc_batch_download(w->ctx, i, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                         bnd[i].data());
             if (st != DMC_OK) break;
         }
@@ -172,9 +185,9 @@ void run_batch(dmc_server* s, std::vector<Request*>& batch) {
         add(m0.d_he_v, 4 * V, 4 * V);
         for (const Sec& x : secs)
             if (x.src && x.bytes && st == DMC_OK &&
-                hipMemcpyAsync(p + x.off, x.src, x.bytes, hipMemcpyDeviceToDevice, s->copy_stream) != hipSuccess)
+                hipMemcpyAsync(p + x.off, x.src, x.bytes, hipMemcpyDeviceToDevice, w->copy_stream) != hipSuccess)
                 st = DMC_ERR_HIP;
-        if (hipStreamSynchronize(s->copy_stream) != hipSuccess) st = DMC_ERR_HIP;
+        if (hipStreamSynchronize(w->copy_stream) != hipSuccess) st = DMC_ERR_HIP;
         // re-point every volume's mesh into the block
         int64_t vb = 0, qb = 0;
         for (int i = 0; i < n; i++) {
@@ -208,7 +221,7 @@ void run_batch(dmc_server* s, std::vector<Request*>& batch) {
         } else {
             r->mesh = dmc_mesh{};
             if (blk) blk->refs--;
-            s->err = dmc_last_error(s->ctx);
+            s->err = dmc_last_error(w->ctx);
         }
         r->done = true;
         s->served++;
@@ -216,21 +229,31 @@ void run_batch(dmc_server* s, std::vector<Request*>& batch) {
     s->cv_done.notify_all();
 }
 
-void worker_main(dmc_server* s) {
+void worker_main(dmc_server* s, Worker* w) {
     (void)hipSetDevice(s->opts.device);
+    const size_t budget = s->opts.budget_bytes;
+    // a pass may start when no other is running, or when the oldest request
+    // fits beside the passes in flight and the held results (one that can
+    // never fit is failed at once)
+    auto ready = [&] {
+        if (s->stop) return true;
+        if (s->queue.empty()) return false;
+        const size_t est = s->queue.front()->est;
+        return s->inflight_bytes == 0 || est > budget || est + s->inflight_bytes + s->held_bytes <= budget;
+    };
     for (;;) {
         std::vector<Request*> batch;
+        size_t used = 0;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+            s->cv_work.wait(lk, ready);
             if (s->stop && s->queue.empty()) return;
-            // admission: oldest first, while the working set plus the held
-            // results fit the budget (a request that never fits fails)
-            size_t used = 0;
+            // admission: oldest first, while the working set plus the passes
+            // in flight and the held results fit the budget
             int64_t points = 0;
             while (!s->queue.empty() && (int)batch.size() < s->opts.max_batch) {
                 Request* r = s->queue.front();
-                if (r->est > s->opts.budget_bytes) {
+                if (r->est > budget) {
                     s->queue.pop_front();
                     r->status = DMC_ERR_BUDGET;
                     r->done = true;
@@ -239,15 +262,23 @@ void worker_main(dmc_server* s) {
                     continue;
                 }
                 const int64_t pts = (int64_t)r->vol.grid.nx * r->vol.grid.ny * r->vol.grid.nz;
-                if (!batch.empty() && used + r->est + s->held_bytes > s->opts.budget_bytes) break;
+                if (!batch.empty() && used + r->est + s->inflight_bytes + s->held_bytes > budget) break;
                 if (!batch.empty() && s->opts.max_batch_points > 0 && points + pts > s->opts.max_batch_points) break;
                 batch.push_back(r);
                 used += r->est;
                 points += pts;
                 s->queue.pop_front();
             }
+            s->inflight_bytes += used;
+            // requests left over are for another idle worker
+            if (!s->queue.empty()) s->cv_work.notify_one();
         }
-        if (!batch.empty()) run_batch(s, batch);
+        if (!batch.empty()) run_batch(s, w, batch);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->inflight_bytes -= used;
+        }
+        s->cv_work.notify_all();
     }
 }
 
@@ -258,26 +289,36 @@ extern "C" {
 dmc_status dmc_server_create(dmc_server** out, const dmc_server_opts* opts) {
     if (!out || !opts) return DMC_ERR_INVALID_ARG;
     *out = nullptr;
-    if (opts->max_batch < 1 || (opts->field_type != DMC_FIELD_F32 && opts->field_type != DMC_FIELD_F64))
+    if (opts->max_batch < 1 || opts->workers < 0 || opts->workers > kMaxWorkers ||
+        (opts->field_type != DMC_FIELD_F32 && opts->field_type != DMC_FIELD_F64))
         return DMC_ERR_INVALID_ARG;
     dmc_server* s = new dmc_server();
     s->opts = *opts;
     if (s->opts.budget_bytes == 0) s->opts.budget_bytes = (size_t)96 << 30;
+    if (s->opts.workers == 0) s->opts.workers = 1;
     s->opts.flags &= ~(uint32_t)DMC_STAGE_TIMING;
-    dmc_status st = dmc_create(&s->ctx, opts->device);
+    dmc_status st = DMC_OK;
+    if (hipSetDevice(opts->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->dl_stream, hipStreamNonBlocking) != hipSuccess)
+        st = DMC_ERR_HIP;
+    for (int i = 0; i < s->opts.workers && st == DMC_OK; i++) {
+        auto w = std::make_unique<Worker>();
+        st = dmc_create(&w->ctx, opts->device);
+        if (st == DMC_OK && hipStreamCreateWithFlags(&w->copy_stream, hipStreamNonBlocking) != hipSuccess)
+            st = DMC_ERR_HIP;
+        s->workers.push_back(std::move(w));
+    }
     if (st != DMC_OK) {
+        (void)hipGetLastError();
+        for (auto& w : s->workers) {
+            if (w->copy_stream) (void)hipStreamDestroy(w->copy_stream);
+            if (w->ctx) dmc_destroy(w->ctx);
+        }
+        if (s->dl_stream) (void)hipStreamDestroy(s->dl_stream);
         delete s;
         return st;
     }
-    if (hipSetDevice(opts->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->dl_stream, hipStreamNonBlocking) != hipSuccess) {
-        if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
-        dmc_destroy(s->ctx);
-        delete s;
-        return DMC_ERR_HIP;
-    }
-    s->worker = std::thread(worker_main, s);
+    for (auto& w : s->workers) w->th = std::thread(worker_main, s, w.get());
     *out = s;
     return DMC_OK;
 }
@@ -289,17 +330,20 @@ dmc_status dmc_server_destroy(dmc_server* s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (auto& w : s->workers)
+        if (w->th.joinable()) w->th.join();
     (void)hipSetDevice(s->opts.device);
-    (void)hipStreamSynchronize(s->copy_stream);
+    for (auto& w : s->workers) (void)hipStreamSynchronize(w->copy_stream);
     for (Block* b : s->blocks) {
         if (b->p) (void)hipFree(b->p);
         delete b;
     }
     (void)hipStreamSynchronize(s->dl_stream);
-    (void)hipStreamDestroy(s->copy_stream);
+    for (auto& w : s->workers) {
+        (void)hipStreamDestroy(w->copy_stream);
+        dmc_destroy(w->ctx);
+    }
     (void)hipStreamDestroy(s->dl_stream);
-    dmc_destroy(s->ctx);
     delete s;
     return DMC_OK;
 }
