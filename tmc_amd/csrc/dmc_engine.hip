@@ -74,6 +74,13 @@ struct dmc_context {
     // pinned host staging of device-to-host reads (counts, totals, tables)
     void* pin = nullptr;
     size_t pin_cap = 0;
+    // deferred reads: device values copied into pinned slots without
+    // waiting, valid once a later stream synchronisation has returned
+    void* dpin = nullptr;
+    int64_t syncs = 0;         // stream synchronisations of HostReads so far
+    int64_t bad_epoch = -1;    // halfedge stats of a non-final build pending in slot 0
+    int64_t rows_epoch = -1;   // first CSR's row total pending in slot 1 ...
+    uint64_t rows_expect = 0;  // ... and the value it must have (4 Q)
     // results
     dmc_mesh mesh{};
     bool have_mesh = false;
@@ -192,6 +199,7 @@ class HostReads {
         for (const Out& o : outs_)
             HIPC(ctx_, hipMemcpyAsync(pin + o.at, o.src, o.bytes, hipMemcpyDeviceToHost, ctx_->stream));
         HIPC(ctx_, hipStreamSynchronize(ctx_->stream));
+        ctx_->syncs++;
         for (const Out& o : outs_) std::memcpy(o.dst, pin + o.at, o.bytes);
         outs_.clear();
         used_ = 0;
@@ -216,18 +224,66 @@ uint64_t read_u64(dmc_context* ctx, const void* dptr) {
     return v;
 }
 
+// Deferred reads: `bytes` (at most kDeferSlot) of device memory copied into
+// pinned slot `slot` without waiting.  Returns the synchronisation count at
+// the copy; the slot is valid once a later synchronisation has returned.
+constexpr size_t kDeferSlot = 256;
+int64_t defer_read(dmc_context* ctx, int slot, const void* src, size_t bytes) {
+    if (!ctx->dpin) {
+        if (hipHostMalloc(&ctx->dpin, 4 * kDeferSlot) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->dpin = nullptr;
+            fail(ctx, DMC_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed", 4 * kDeferSlot);
+        }
+    }
+    HIPC(ctx, hipMemcpyAsync(static_cast<char*>(ctx->dpin) + slot * kDeferSlot, src, bytes, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    return ctx->syncs;
+}
+
+// A deferred slot's contents, synchronising first if nothing has since
+// the copy.
+const void* deferred(dmc_context* ctx, int slot, int64_t epoch) {
+    if (ctx->syncs <= epoch) {
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->syncs++;
+    }
+    return static_cast<const char*>(ctx->dpin) + slot * kDeferSlot;
+}
+
+// Device slots for scan totals: 0 for the scans whose total is read at
+// once, the others for totals kept on the device or read later.
+enum TotalSlot { kTotRead = 0, kTotV, kTotQ, kTotRows, kTotRemap, kTotSlots };
+uint64_t* totals(dmc_context* ctx) { return ensure<uint64_t>(ctx, ctx->scanTotal, kTotSlots); }
+
+// Exclusive scan of n uint32 counts into out; the 64-bit total goes to
+// *dtotal (device memory) and the host does not wait for it.
+template <class In>
+void scan_to(dmc_context* ctx, In in, uint32_t* out, int64_t n, uint64_t* dtotal) {
+    if (n <= 0) {
+        HIPC(ctx, hipMemsetAsync(dtotal, 0, sizeof(uint64_t), ctx->stream));
+        return;
+    }
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
+    hipLaunchKernelGGL(k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, n, part);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, tiles, dtotal);
+    hipLaunchKernelGGL(k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
+    check_launch(ctx, "scan");
+}
+
 // Exclusive scan of n uint32 counts into out; returns the 64-bit total.
 template <class In>
 uint64_t scan_any(dmc_context* ctx, In in, uint32_t* out, int64_t n) {
     if (n <= 0) return 0;
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
-    uint64_t* tot = ensure<uint64_t>(ctx, ctx->scanTotal, 1);
-    hipLaunchKernelGGL(k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, n, part);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, tiles, tot);
-    hipLaunchKernelGGL(k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
-    check_launch(ctx, "scan");
+    uint64_t* tot = totals(ctx) + kTotRead;
+    scan_to(ctx, in, out, n, tot);
     return read_u64(ctx, tot);
+}
+
+// off[n] = total of a scan held on the device (low 32 bits).
+void set_row_end(dmc_context* ctx, uint32_t* off, int64_t n, const uint64_t* dtotal) {
+    HIPC(ctx, hipMemcpyAsync(off + n, dtotal, sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
 }
 
 // Exclusive scan of uint32 counts; returns the total.
@@ -357,8 +413,16 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, vt, R, lists,
                        segCount, segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, vstat);
     check_launch(ctx, "k_rows");
-    const uint64_t V = scan(ctx, rowV, rowBaseV, R);
-    const uint64_t Q = scan(ctx, rowQ, rowBaseQ, R);
+    uint64_t* tot = totals(ctx);
+    scan_to(ctx, ScanCounts{rowV}, rowBaseV, R, tot + kTotV);
+    scan_to(ctx, ScanCounts{rowQ}, rowBaseQ, R, tot + kTotQ);
+    uint64_t V = 0, Q = 0;
+    {
+        HostReads rd(ctx);
+        rd.add(&V, tot + kTotV, sizeof(V));
+        rd.add(&Q, tot + kTotQ, sizeof(Q));
+        rd.wait();
+    }
     if (V > 0x7FFFFFFFull || Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)V, (unsigned long long)Q);
     // [0, n]: live bounds; [n+1, 2n+1]: the cell stage's (before simplification)
@@ -394,10 +458,11 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
         check_launch(ctx, "k_vertices_slow");
     }
     mark(ctx, "quads");
-    const uint64_t rows = scan(ctx, deg, off, V);
-    if (rows != 4 * Q) fail(ctx, DMC_ERR_STATE, "halfedge rows %llu != 4 Q %llu", (unsigned long long)rows,
-                            (unsigned long long)(4 * Q));
-    HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + V), (int)(uint32_t)rows, 1, ctx->stream));
+    // the row total must be 4 Q; checked at the next synchronisation
+    scan_to(ctx, ScanCounts{deg}, off, (int64_t)V, tot + kTotRows);
+    set_row_end(ctx, off, (int64_t)V, tot + kTotRows);
+    ctx->rows_epoch = defer_read(ctx, 1, tot + kTotRows, sizeof(uint64_t));
+    ctx->rows_expect = 4 * Q;
     if (A > 0) {
         hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux, avloc,
                            aqloc,
@@ -439,8 +504,33 @@ void three_face_edges(dmc_context* ctx, const HeStats& hs, int64_t nv, ThreeLaun
     }
 }
 
+// The first CSR's row total against 4 Q, once it has arrived.
+void check_rows(dmc_context* ctx) {
+    if (ctx->rows_epoch < 0) return;
+    uint64_t rows = 0;
+    std::memcpy(&rows, deferred(ctx, 1, ctx->rows_epoch), sizeof(rows));
+    ctx->rows_epoch = -1;
+    if (rows != ctx->rows_expect)
+        fail(ctx, DMC_ERR_STATE, "halfedge rows %llu != 4 Q %llu", (unsigned long long)rows,
+             (unsigned long long)ctx->rows_expect);
+}
+
+// Invalid vertex references seen by the last halfedge build (a non-final
+// build's stats arrive deferred).
+uint32_t last_bad(dmc_context* ctx) {
+    if (ctx->bad_epoch >= 0) {
+        HeStats hs{};
+        std::memcpy(&hs, deferred(ctx, 0, ctx->bad_epoch), sizeof(hs));
+        ctx->bad_epoch = -1;
+        ctx->last_bad = hs.bad;
+    }
+    return ctx->last_bad;
+}
+
 // Halfedges through a CSR of outgoing (halfedge, destination) pairs: rows
-// off[v] .. off[v+1] (off has nv+1 entries).
+// off[v] .. off[v+1] (off has nv+1 entries).  A non-final build leaves its
+// stats on the device (read when the next stage needs them) and returns
+// zeros.
 HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq, nhe = 4 * nq;
     const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
@@ -464,6 +554,7 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
         HostReads r(ctx);
         r.add(&unresolved, static_cast<uint32_t*>(ctx->fails.p) + 5, sizeof(unresolved));
         r.wait();
+        check_rows(ctx);
         fresh = unresolved == 0;
     } else if (ready == 2) {
         // remapped rows: halfedges left without a valid origin were counted
@@ -504,11 +595,16 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
         hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, adj, nv, hev);
         check_launch(ctx, "k_he_v");
     }
+    if (!final_pass) {
+        ctx->bad_epoch = defer_read(ctx, 0, st, sizeof(HeStats));
+        return hs;
+    }
     {
         HostReads r(ctx);
         r.add(&hs, st, sizeof(hs));
         r.wait();
     }
+    ctx->bad_epoch = -1;
     ctx->last_bad = hs.bad;
     if (final_pass)
         three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
@@ -609,7 +705,7 @@ void gather_vertices(dmc_context* ctx) {
 // current ones.  Skipped (the next build counts from scratch) when the
 // current rows left halfedges out.
 void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2, const int32_t* dlist, int64_t ndrop) {
-    if (ctx->last_bad != 0) return;
+    if (last_bad(ctx) != 0) return;
     const uint32_t* off = static_cast<const uint32_t*>(ctx->off.p);
     const int2* adj = static_cast<const int2*>(ctx->adj.p);
     uint32_t* deg2 = ensure<uint32_t>(ctx, ctx->deg, nn);
@@ -629,8 +725,9 @@ void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2, const
                                deg2, 1, bad);
         check_launch(ctx, "k_remap_degree");
     }
-    const uint64_t rows = scan(ctx, deg2, off2, nn);
-    HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off2 + nn), (int)(uint32_t)rows, 1, ctx->stream));
+    uint64_t* tot = totals(ctx) + kTotRemap;
+    scan_to(ctx, ScanCounts{deg2}, off2, nn, tot);
+    set_row_end(ctx, off2, nn, tot);
     if (m.nv > 0) {
         for (int pass = 0; pass < 2; pass++)
             hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
@@ -851,6 +948,7 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
     if (n > 1) r.add(vhe.data(), ctx->vhe.p, vhe.size() * sizeof(unsigned long long));
     r.add(diag, ctx->fails.p, sizeof(diag));
     r.wait();
+    check_rows(ctx);
     ctx->mesh.valence_zero = diag[4];
     ctx->mesh.invalid_refs += diag[5];
     finish_timing(ctx);
@@ -942,6 +1040,7 @@ void do_extract(dmc_context* ctx, void* a_) {
     ctx->timing = (a.flags & DMC_STAGE_TIMING) != 0;
     ctx->have_mesh = false;
     ctx->mesh = dmc_mesh{};
+    ctx->bad_epoch = ctx->rows_epoch = -1;
     mark(ctx, "upload");
     std::vector<HostVol> hv(a.nvol);
     uint8_t* fdev = nullptr;
@@ -1024,6 +1123,7 @@ dmc_status dmc_destroy(dmc_context* ctx) {
     for (Buf* b : ctx->bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->dpin) (void)hipHostFree(ctx->dpin);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DMC_OK;
