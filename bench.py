@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
 # scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024_traffic.md)
-TRAFFIC = {"schwarz_1024": 5736541262}
+TRAFFIC = {"schwarz_1024": 5737299838}
 
 
 def log(msg):
@@ -185,7 +185,7 @@ def serve_run(srv, pool, rate, n, seed, label):
     return out
 
 
-def bench_serve(seed=5, secs=2.5, batch_points=64 << 20, workers=2):
+def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2):
     """Served mix: Poisson arrivals, 70% U{32..128}^3, 25% 256^3, 5% 512^3,
     fields in HBM.  One-at-a-time (max_batch 1, one worker) runs at 70% of
     its capacity and sets the latency bar (its p99).  The batched server
@@ -215,11 +215,11 @@ def bench_serve(seed=5, secs=2.5, batch_points=64 << 20, workers=2):
     cap_b = serve_run(bsrv, pool, float("inf"), 1200, seed + 3, "batched capacity")["volumes_per_s"]
     best = None
     tried = []
-    for mult in (2, 2.5, 3, 3.5, 4, 5, 6, 8, 10):
+    for mult in (1.25, 1.5, 1.75, 2, 2.5, 3, 3.5, 4, 5, 6, 8, 10):
         r = mult * rate
         if r > 1.05 * cap_b:
             break
-        res = serve_run(bsrv, pool, r, max(200, int(r * secs)), seed + 4 + mult, f"batched at {mult}x")
+        res = serve_run(bsrv, pool, r, max(200, int(r * secs)), seed + 4 + int(4 * mult), f"batched at {mult}x")
         tried.append(dict(res, multiple=mult))
         if res["p99_ms"] <= plain["p99_ms"] and res["volumes_per_s"] >= 0.95 * r:
             best = dict(res, multiple=mult)
@@ -290,6 +290,7 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (4,096 volumes) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson mix) leg")
+    ap.add_argument("--no-diagnostics", action="store_true", help="skip the mesh diagnostics timings")
     ap.add_argument("--only", choices=["batch", "serve"], help="run only the batched or served leg")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -383,7 +384,7 @@ def main():
                        stage_ms=stages),
         "roofline": roofline,
     }
-    out["diagnostics"] = bench_diagnostics(eng)
+    out["diagnostics"] = None if args.no_diagnostics else bench_diagnostics(eng)
     buf.free()
     eng.close()
     if not args.no_batch:
