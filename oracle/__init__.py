@@ -1,8 +1,11 @@
 """Parity checkers for the DMC engine -- TEST INFRASTRUCTURE ONLY.
 
 ``Oracle`` wraps liboracle.so, the C restatement of the reference cpp_dmc
-path (dmc_oracle.c).  ``Reference`` wraps oracle/_ref/libdmcref.so, the
-reference cpp_dmc compiled from its own sources (ref_harness.cpp).  Only
+path (dmc_oracle.c) and of the reference's topologically correct MC
+(tmc_oracle.c).  ``Reference`` wraps oracle/_ref/libdmcref.so, the
+reference cpp_dmc compiled from its own sources (ref_harness.cpp);
+``TmcReference`` wraps oracle/_ref/libtmcref.so, mc/MarchingCubes.cpp
+compiled from its own sources (tmc_ref_harness.cpp).  Only
 tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of bench.py use
 this package; the engine (tmc_amd) never does.
 """
@@ -14,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libdmcref.so")
+TMC_REF_SO = os.path.join(HERE, "_ref", "libtmcref.so")
 
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
@@ -110,6 +114,30 @@ class Oracle:
                                         C.POINTER(_Mesh), C.POINTER(_HE)]
         L.dmco_cell_polygons.argtypes = [f64p, C.c_double, i32p, i32p]
         L.dmco_cell_vertices.argtypes = [f64p, C.c_double, f64p, u32p, u32p]
+        L.tmco_tmc.argtypes = [C.POINTER(Grid), f64p, C.c_double, i64p, C.POINTER(f64p), C.POINTER(f64p), i64p,
+                               C.POINTER(i32p), i64p]
+        L.tmco_tmc.restype = C.c_int
+
+    def tmc(self, grid, scalars, iso):
+        """Topologically correct MC (mc/MarchingCubes.cpp t_mc): dict with
+        v, n (float64, 3 per vertex), tris (int32, 3 per triangle),
+        ambiguous (cells sliced with the decider), face_failures."""
+        s = np.ascontiguousarray(scalars, dtype=np.float64).reshape(-1)
+        nv, nt = C.c_int64(), C.c_int64()
+        v, n, t = f64p(), f64p(), i32p()
+        st = (C.c_int64 * 2)()
+        if self.lib.tmco_tmc(C.byref(grid), _p(s, f64p), float(iso), C.byref(nv), C.byref(v), C.byref(n),
+                             C.byref(nt), C.byref(t), st) != 0:
+            raise MemoryError("tmco_tmc")
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        out = {"v": _arr(v, 3 * nv.value, np.float64).reshape(-1, 3).copy(),
+               "n": _arr(n, 3 * nv.value, np.float64).reshape(-1, 3).copy(),
+               "tris": _arr(t, 3 * nt.value, np.int32).reshape(-1, 3).copy(),
+               "ambiguous": int(st[0]), "face_failures": int(st[1])}
+        for ptr in (v, n, t):
+            libc.free(C.cast(ptr, C.c_void_p))
+        return out
 
     # -- grid ---------------------------------------------------------------
     def gradient(self, grid, scalars, flip=True):
@@ -463,3 +491,26 @@ class Reference:
         a, b = C.c_uint32(0), C.c_uint32(0)
         cnt = self.lib.dmcref_cell_vertices(_p(f, f64p), float(iso), _p(loc, f64p), C.byref(a), C.byref(b))
         return loc[:3 * cnt].reshape(-1, 3), a.value, b.value
+
+
+class TmcReference:
+    """ctypes view of oracle/_ref/libtmcref.so (the reference's t_mc
+    compiled from mc/MarchingCubes.cpp)."""
+
+    def __init__(self, path=TMC_REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = C.CDLL(path)
+        L.tmcref_run.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [f64p, C.c_double, i64p, i64p]
+        L.tmcref_fetch.argtypes = [f64p, f64p, i32p]
+
+    def tmc(self, grid, scalars, iso):
+        s = np.ascontiguousarray(scalars, dtype=np.float64).reshape(-1)
+        nv, nt = C.c_int64(), C.c_int64()
+        self.lib.tmcref_run(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz,
+                            _p(s, f64p), float(iso), C.byref(nv), C.byref(nt))
+        v = np.zeros((nv.value, 3), np.float64)
+        n = np.zeros((nv.value, 3), np.float64)
+        t = np.zeros((nt.value, 3), np.int32)
+        self.lib.tmcref_fetch(_p(v, f64p), _p(n, f64p), _p(t, i32p))
+        return {"v": v, "n": n, "tris": t}
