@@ -237,6 +237,41 @@ dmc_status dmc_element_quality(dmc_context* ctx, int which, float* quality);
  * (extraction with DMC_TRIANGLES), else quadrilaterals. */
 dmc_status dmc_write_obj(dmc_context* ctx, const char* path, int triangles);
 
+/* ------------------------------------------------------------------------
+ * Topologically correct Marching Cubes (the reference's other extraction
+ * path).  tmc_extract replaces tmc::MarchingCubes::t_mc
+ * (mc/MarchingCubes.cpp:150-318, reached from operator() at :11-147 after
+ * UniformGrid::gradient) and cuda_tmc's shared-vertex extraction
+ * (cuda_tmc/MarchingCubes.h:71, mc_sharedvertex; t_slice at
+ * cuda_tmc/MarchingCubes.cu:1366).  Vertices on grid edges are shared and
+ * numbered in the order t_mc first meets them; ambiguous cells get the
+ * asymptotic-decider slicing of p_slice (mc/MarchingCubes.cpp:322-1303) with
+ * its inner vertices.  Positions and normals are FP64 (normals: the
+ * reference's central-difference gradients, one-sided at the border,
+ * interpolated and normalised); triangles are three vertex ids each, in the
+ * reference's order.  The optional orientation pass (connectivity(),
+ * operator()'s c_flag) is not part of it.  flags: DMC_FIELD_ON_DEVICE and
+ * DMC_STAGE_TIMING only.  The mesh stays on the device (tmc_mesh pointers)
+ * until the next extraction on the context.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int64_t nv, nt;
+    const double* d_positions;   /* 3 per vertex */
+    const double* d_normals;     /* 3 per vertex */
+    const int32_t* d_tris;       /* 3 vertex ids per triangle */
+    int64_t active_cells;        /* cells with a cut edge */
+    int64_t ambiguous_cells;     /* cells sliced with the asymptotic decider */
+    int64_t face_failures;       /* of those, cells left without triangles: a face
+                                    whose decider equals the iso-value and whose
+                                    intersections admit no triangulation */
+} tmc_mesh;
+
+dmc_status tmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type, double iso,
+                       uint32_t flags, tmc_mesh* out);
+/* Copy the last topologically correct mesh to host arrays (any may be NULL):
+ * positions/normals 3*nv doubles, triangles 3*nt ints. */
+dmc_status tmc_download(dmc_context* ctx, double* positions, double* normals, int32_t* triangles);
+
 /* Per-stage device times (ms) of the last extraction run with
  * DMC_STAGE_TIMING.  names[i] point to static strings.  *count is in/out:
  * capacity in, number of stages out. */

@@ -14,7 +14,7 @@ from conftest import ROOT
 def _declared(header):
     src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(dmc_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b((?:dmc|tmc)_[a-z0-9_]+)\s*\(", src)))
 
 
 @pytest.fixture(scope="module")

@@ -1,0 +1,116 @@
+"""Parity of the HIP topologically correct MC (tmc_extract, through the C
+ABI) with the oracle's restatement of the reference's t_mc
+(oracle/tmc_oracle.c, itself pinned bit for bit against
+mc/MarchingCubes.cpp): vertex positions and normals (FP64), triangles and
+the reference's vertex numbering must be bit-exact."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import Grid as OGrid
+
+pytestmark = pytest.mark.gpu
+
+import tmc_amd  # noqa: E402
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check(engine, oracle, f, iso, origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0), what=""):
+    og = OGrid.make(f.shape, origin, spacing)
+    exp = oracle.tmc(og, f.astype(np.float64), iso)
+    g = tmc_amd.Grid.make(f.shape, origin, spacing)
+    got = engine.tmc(f, iso, g)
+    assert got["nv"] == len(exp["v"]) and got["nt"] == len(exp["tris"]), (what, got["nv"], len(exp["v"]),
+                                                                           got["nt"], len(exp["tris"]))
+    assert got["ambiguous_cells"] == exp["ambiguous"] and got["face_failures"] == exp["face_failures"], what
+    assert np.array_equal(got["tris"], exp["tris"]), what
+    assert np.array_equal(got["v"].view(np.uint64), exp["v"].view(np.uint64)), what
+    assert np.array_equal(got["n"].view(np.uint64), exp["n"].view(np.uint64)), what
+    return got
+
+
+def _smooth(kind, n, lo, hi):
+    z, y, x = np.meshgrid(*[np.linspace(lo, hi, n)] * 3, indexing="ij")
+    if kind == "genus2":
+        return 2 * y * (y * y - 3 * x * x) * (1 - z * z) + (x * x + y * y) ** 2 - (9 * z * z - 1) * (1 - z * z)
+    return np.cos(x) + np.cos(y) + np.cos(z)
+
+
+CASES = [
+    ("uniform16", lambda r: r.uniform(0, 1, (16, 16, 16)), 0.5),
+    ("normal_12x14x18", lambda r: r.standard_normal((12, 14, 18)), 0.1),
+    ("ints_1.5", lambda r: r.integers(0, 4, (14, 13, 12)).astype(np.float64), 1.5),
+    ("ints_on_iso", lambda r: r.integers(0, 5, (14, 14, 14)).astype(np.float64), 2.0),
+    ("tiny_2x2x2", lambda r: r.uniform(0, 1, (2, 2, 2)), 0.5),
+    ("slab_3x40x2", lambda r: r.uniform(0, 1, (3, 40, 2)), 0.5),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_tmc_matches_oracle_f64(engine, oracle, case):
+    name, make, iso = case
+    f = make(np.random.default_rng(17))
+    _say(f"[gpu] tmc {name} {f.shape} iso={iso}")
+    _check(engine, oracle, f, iso, what=name)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tmc_matches_oracle_random_f32(engine, oracle, seed):
+    """Float fields (the engine reads them as float, the oracle gets the
+    same values as doubles), odd shapes, origins and spacings."""
+    rng = np.random.default_rng(100 + seed)
+    sh = tuple(int(x) for x in rng.integers(2, 23, size=3))
+    f = rng.standard_normal(sh).astype(np.float32)
+    origin = tuple(float(x) for x in rng.uniform(-3, 3, 3))
+    spacing = tuple(float(x) for x in rng.uniform(0.1, 2, 3))
+    _say(f"[gpu] tmc random f32 seed={seed} shape={sh}")
+    _check(engine, oracle, f, 0.05, origin, spacing, what=f"seed {seed}")
+
+
+@pytest.mark.parametrize("kind,n,iso", [("genus2", 96, 0.0), ("schwarz", 130, 0.3)])
+def test_tmc_matches_oracle_surfaces(engine, oracle, kind, n, iso):
+    lo, hi = (-2.0, 2.0) if kind == "genus2" else (-4 * np.pi, 4 * np.pi)
+    f = _smooth(kind, n, lo, hi).astype(np.float32)
+    _say(f"[gpu] tmc {kind} {n}^3")
+    got = _check(engine, oracle, f, iso, (lo,) * 3, ((hi - lo) / (n - 1),) * 3, what=kind)
+    if kind == "genus2":
+        # closed surface inside the grid: V - E + F = -2, every edge in two triangles
+        t = got["tris"].astype(np.int64)
+        he = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+        key = np.minimum(he[:, 0], he[:, 1]) * (1 << 32) + np.maximum(he[:, 0], he[:, 1])
+        _, count = np.unique(key, return_counts=True)
+        assert np.all(count == 2)
+        assert got["nv"] - len(count) + got["nt"] == -2
+
+
+def test_tmc_device_field_and_repeat(engine, oracle):
+    """The field already in HBM (DMC_FIELD_ON_DEVICE), twice in a row with a
+    smaller grid second: the same results as from host memory."""
+    rng = np.random.default_rng(9)
+    for sh in ((20, 21, 22), (9, 8, 7)):
+        f = rng.uniform(-1, 1, sh).astype(np.float32)
+        exp = oracle.tmc(OGrid.make(sh), f.astype(np.float64), 0.0)
+        buf = engine.alloc(f.nbytes).upload(f)
+        got = engine.tmc(buf, 0.0, shape=sh, dtype=np.float32)
+        buf.free()
+        assert np.array_equal(got["tris"], exp["tris"])
+        assert np.array_equal(got["v"].view(np.uint64), exp["v"].view(np.uint64))
+
+
+def test_tmc_then_dmc_on_one_context(engine, oracle):
+    """A DMC extraction after a TMC one on the same context is unaffected."""
+    rng = np.random.default_rng(3)
+    f = rng.uniform(0, 1, (12, 12, 12)).astype(np.float32)
+    engine.tmc(f, 0.5)
+    m = engine.extract(f, 0.5, flags=tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES)
+    got = engine.download()
+    engine.extract(f, 0.5, flags=tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES)
+    again = engine.download()
+    assert m.nv == len(got["v"])
+    for k in ("v", "quads", "tris"):
+        assert np.array_equal(got[k], again[k])

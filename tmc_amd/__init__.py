@@ -42,7 +42,8 @@ EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc
            "dmc_server_wait", "dmc_server_download", "dmc_server_release", "dmc_server_stats",
            "dmc_server_last_error", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc",
            "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
-           "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj"]
+           "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj",
+           "tmc_extract", "tmc_download"]
 
 
 class DMCError(RuntimeError):
@@ -64,6 +65,14 @@ class Grid(C.Structure):
 class Volume(C.Structure):
     """dmc_volume: one volume of a batch."""
     _fields_ = [("grid", Grid), ("field", C.c_void_p), ("gradients", C.c_void_p), ("iso", C.c_double)]
+
+
+class TmcMesh(C.Structure):
+    """tmc_mesh: the topologically correct MC result (counts and device
+    pointers)."""
+    _fields_ = [("nv", C.c_int64), ("nt", C.c_int64), ("d_positions", C.c_void_p), ("d_normals", C.c_void_p),
+                ("d_tris", C.c_void_p), ("active_cells", C.c_int64), ("ambiguous_cells", C.c_int64),
+                ("face_failures", C.c_int64)]
 
 
 class Mesh(C.Structure):
@@ -133,6 +142,8 @@ def load_library(path=ENGINE_SO):
                                         C.POINTER(C.c_int64)]
     L.dmc_element_quality.argtypes = [vp, C.c_int, f32p]
     L.dmc_write_obj.argtypes = [vp, C.c_char_p, C.c_int]
+    L.tmc_extract.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, C.POINTER(TmcMesh)]
+    L.tmc_download.argtypes = [vp, f64p, f64p, i32p]
     L.dmc_status_str.argtypes = [C.c_int]
     L.dmc_status_str.restype = C.c_char_p
     L.dmc_last_error.argtypes = [vp]
@@ -270,6 +281,45 @@ class Engine:
         del keep
         self.mesh = m
         return m
+
+    def tmc(self, field, iso, grid=None, shape=None, dtype=np.float32, download=True, timing=False):
+        """Topologically correct Marching Cubes (tmc_extract).  `field` is a
+        numpy array (nz, ny, nx) or a DeviceBuffer / device pointer holding
+        one (then pass `shape` and `dtype`).  Returns a dict with the counts
+        and, with download=True, v / n (float64, 3 per vertex) and tris
+        (int32, 3 per triangle)."""
+        flags = STAGE_TIMING if timing else 0
+        if isinstance(field, int):
+            field = _RawDevice(field)
+        if isinstance(field, (DeviceBuffer, _RawDevice)):
+            fptr = field.ptr
+            flags |= FIELD_ON_DEVICE
+            if grid is None:
+                grid = Grid.make(shape)
+            ftype = FIELD_F32 if np.dtype(dtype) == np.float32 else FIELD_F64
+            keep = None
+        else:
+            arr = np.ascontiguousarray(field)
+            if arr.dtype not in (np.float32, np.float64):
+                arr = arr.astype(np.float32)
+            ftype = FIELD_F32 if arr.dtype == np.float32 else FIELD_F64
+            if grid is None:
+                grid = Grid.make(arr.shape)
+            fptr = C.c_void_p(arr.ctypes.data)
+            keep = [arr]
+        if not isinstance(grid, Grid):
+            grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+        m = TmcMesh()
+        self._check(self.lib.tmc_extract(self.ctx, C.byref(grid), fptr, ftype, float(iso), int(flags), C.byref(m)))
+        del keep
+        out = {f: getattr(m, f) for f, _ in m._fields_ if not f.startswith("d_")}
+        if download:
+            v = np.zeros((m.nv, 3), np.float64)
+            n = np.zeros((m.nv, 3), np.float64)
+            t = np.zeros((m.nt, 3), np.int32)
+            self._check(self.lib.tmc_download(self.ctx, _p(v, C.c_double), _p(n, C.c_double), _p(t, C.c_int32)))
+            out.update(v=v, n=n, tris=t)
+        return out
 
     def extract_batch(self, volumes, flags=SIMPLIFY_ALL | TRIANGLES, dtype=np.float32):
         """Extract many volumes in one pass.  `volumes` is a list of

@@ -20,6 +20,7 @@
 #include "../../include/tmc_dmc.h"
 #include "dmc_extract_kernels.h"
 #include "dmc_mesh_kernels.h"
+#include "tmc_kernels.h"
 
 using namespace tmc;
 
@@ -70,6 +71,10 @@ struct dmc_context {
         newV, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
+    // topologically correct MC: vertex and triangle row offsets, stats, mesh
+    Buf tloc, rowT, rowBaseT, tstats, tpos, tnrm, ttris;
+    tmc_mesh tmesh{};
+    bool have_tmesh = false;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // pinned host staging of device-to-host reads (counts, totals, tables)
     void* pin = nullptr;
@@ -100,7 +105,7 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
-                      &dhist, &dedges, &dqual};
+                      &dhist, &dedges, &dqual, &tloc, &rowT, &rowBaseT, &tstats, &tpos, &tnrm, &ttris};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -327,8 +332,20 @@ VolTable vol_table(dmc_context* ctx) {
                     ctx->nvol};
 }
 
+// What the field sweep leaves for the cell stages.
+struct Swept {
+    uint64_t A;     // active cells
+    int64_t R;      // cell rows (all volumes)
+    VolTable vt;
+    VolStat* vstat;
+    uint32_t *lists, *segCount, *segBase;
+    unsigned long long* segMask;
+};
+
+// Volume table, classification sweep and the scan of the active cells --
+// shared by Dual Marching Cubes and the topologically correct MC.
 template <class T>
-void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
+Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const int n = (int)hv.size();
     ctx->nvol = n;
     // sweep tiling: J rows per wave; one plane count K for the whole
@@ -397,6 +414,19 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     ctx->mesh.active_cells = (int64_t)A;
     hipLaunchKernelGGL(k_vol_abase, dim3(grid_for(n + 1)), dim3(kBlock), 0, ctx->stream, vt, segBase, (int64_t)A,
                        vol_ptr(ctx, 2));
+    return Swept{A, R, vt, vstat, lists, segCount, segBase, segMask};
+}
+
+template <class T>
+void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
+    const Swept sw = sweep_cells<T>(ctx, hv);
+    const int n = (int)hv.size();
+    const uint64_t A = sw.A;
+    const int64_t R = sw.R;
+    const VolTable vt = sw.vt;
+    VolStat* vstat = sw.vstat;
+    uint32_t *lists = sw.lists, *segCount = sw.segCount, *segBase = sw.segBase;
+    unsigned long long* segMask = sw.segMask;
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
     uint32_t* aaux = ensure<uint32_t>(ctx, ctx->aaux, A);
@@ -887,6 +917,72 @@ void check_grid(dmc_context* ctx, const dmc_grid& g) {
     if (ncells > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "more than 2^32 cells");
 }
 
+// Topologically correct MC of one volume (mc/MarchingCubes.cpp t_mc): the
+// sweep's active cells, their vertex and triangle counts per row, the row
+// scans, then vertices and triangles.
+template <class T>
+void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
+    const Swept sw = sweep_cells<T>(ctx, hv);
+    const uint64_t A = sw.A;
+    const int64_t R = sw.R;
+    uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
+    uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
+    uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, A);
+    uint32_t* atloc = ensure<uint32_t>(ctx, ctx->tloc, A);
+    uint32_t* rowV = ensure<uint32_t>(ctx, ctx->rowV, R);
+    uint32_t* rowT = ensure<uint32_t>(ctx, ctx->rowT, R);
+    uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
+    uint32_t* rowBaseT = ensure<uint32_t>(ctx, ctx->rowBaseT, R);
+    unsigned long long* st = ensure<unsigned long long>(ctx, ctx->tstats, 3);
+    HIPC(ctx, hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL(k_tmc_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
+                       sw.lists, sw.segCount, sw.segBase, acell, acode, avloc, atloc, rowV, rowT, st);
+    check_launch(ctx, "k_tmc_rows");
+    uint64_t* tot = totals(ctx);
+    scan_to(ctx, ScanCounts{rowV}, rowBaseV, R, tot + kTotV);
+    scan_to(ctx, ScanCounts{rowT}, rowBaseT, R, tot + kTotQ);
+    uint64_t V = 0, Tn = 0;
+    {
+        HostReads rd(ctx);
+        rd.add(&V, tot + kTotV, sizeof(V));
+        rd.add(&Tn, tot + kTotQ, sizeof(Tn));
+        rd.wait();
+    }
+    if (V > 0x7FFFFFFFull || Tn > 0x7FFFFFFFull)
+        fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, T=%llu)", (unsigned long long)V,
+             (unsigned long long)Tn);
+    double* pos = ensure<double>(ctx, ctx->tpos, 3 * V);
+    double* nrm = ensure<double>(ctx, ctx->tnrm, 3 * V);
+    int32_t* tris = ensure<int32_t>(ctx, ctx->ttris, 3 * Tn);
+    mark(ctx, "emit");
+    if (A > 0) {
+        hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode,
+                           avloc, atloc, rowBaseV, rowBaseT, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
+                           reinterpret_cast<uint32_t*>(st + 2));
+        check_launch(ctx, "k_tmc_emit");
+    }
+    mark(ctx, "end");
+    unsigned long long hs[3];
+    {
+        HostReads rd(ctx);
+        rd.add(hs, st, sizeof(hs));
+        rd.wait();
+    }
+    finish_timing(ctx);
+    if (hs[2] != 0) fail(ctx, DMC_ERR_STATE, "%llu cut edges without an active owner cell", hs[2]);
+    tmc_mesh& m = ctx->tmesh;
+    m = tmc_mesh{};
+    m.nv = (int64_t)V;
+    m.nt = (int64_t)Tn;
+    m.d_positions = pos;
+    m.d_normals = nrm;
+    m.d_tris = tris;
+    m.active_cells = (int64_t)A;
+    m.ambiguous_cells = (int64_t)hs[0];
+    m.face_failures = (int64_t)hs[1];
+    ctx->have_tmesh = true;
+}
+
 // Everything after the upload, for one volume or a batch.
 void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint32_t flags) {
     const int n = (int)hv.size();
@@ -1014,6 +1110,7 @@ struct ExtractArgs {
     int field_type;
     uint32_t flags;
     dmc_mesh* out;
+    tmc_mesh* tout = nullptr;   // set: the topologically correct MC of vols[0]
 };
 
 void do_extract(dmc_context* ctx, void* a_) {
@@ -1039,6 +1136,7 @@ void do_extract(dmc_context* ctx, void* a_) {
     }
     ctx->timing = (a.flags & DMC_STAGE_TIMING) != 0;
     ctx->have_mesh = false;
+    ctx->have_tmesh = false;
     ctx->mesh = dmc_mesh{};
     ctx->bad_epoch = ctx->rows_epoch = -1;
     mark(ctx, "upload");
@@ -1062,6 +1160,12 @@ void do_extract(dmc_context* ctx, void* a_) {
                 hv[v].grads = reinterpret_cast<const double*>(gdev + goff[v]);
             }
         }
+    }
+    if (a.tout) {
+        if (a.field_type == DMC_FIELD_F32) run_tmc<float>(ctx, hv);
+        else run_tmc<double>(ctx, hv);
+        *a.tout = ctx->tmesh;
+        return;
     }
     run(ctx, hv, a.field_type, a.flags);
     if (a.out)
@@ -1142,6 +1246,37 @@ dmc_status dmc_extract_batch(dmc_context* ctx, const dmc_volume* vols, int32_t n
     if (!ctx || !vols || nvol < 1) return DMC_ERR_INVALID_ARG;
     ExtractArgs a{vols, nvol, field_type, flags, out};
     return guard(ctx, do_extract, &a);
+}
+
+dmc_status tmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type, double iso,
+                       uint32_t flags, tmc_mesh* out) {
+    if (!ctx || !grid) return DMC_ERR_INVALID_ARG;
+    if (flags & DMC_GRADIENTS_SUPPLIED) return DMC_ERR_INVALID_ARG;
+    const dmc_volume vol{*grid, field, nullptr, iso};
+    ExtractArgs a{&vol, 1, field_type, flags & (DMC_FIELD_ON_DEVICE | DMC_STAGE_TIMING), nullptr, out};
+    return guard(ctx, do_extract, &a);
+}
+
+struct TmcDownloadArgs {
+    double *pos, *nrm;
+    int32_t* tris;
+};
+
+static void do_tmc_download(dmc_context* ctx, void* a_) {
+    TmcDownloadArgs& a = *static_cast<TmcDownloadArgs*>(a_);
+    if (!ctx->have_tmesh) fail(ctx, DMC_ERR_STATE, "no mesh: call tmc_extract first");
+    const tmc_mesh& m = ctx->tmesh;
+    HostReads r(ctx);
+    if (a.pos) r.add(a.pos, m.d_positions, 24 * (size_t)m.nv);
+    if (a.nrm) r.add(a.nrm, m.d_normals, 24 * (size_t)m.nv);
+    if (a.tris) r.add(a.tris, m.d_tris, 12 * (size_t)m.nt);
+    r.wait();
+}
+
+dmc_status tmc_download(dmc_context* ctx, double* positions, double* normals, int32_t* triangles) {
+    if (!ctx) return DMC_ERR_INVALID_ARG;
+    TmcDownloadArgs a{positions, normals, triangles};
+    return guard(ctx, do_tmc_download, &a);
 }
 
 struct DownloadArgs {
