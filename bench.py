@@ -239,6 +239,49 @@ def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2):
     return out
 
 
+def bench_tmc(eng, buf, field, grid, iso, steps, warmup, cpu_block=None):
+    """Topologically correct MC (tmc_extract) of the same field, resident in
+    HBM: ms per step and Gvoxels/s, per-stage device times, the mesh counts.
+    With cpu_block, the CPU baseline is the reference's t_mc compiled from
+    mc/MarchingCubes.cpp (oracle/_ref) -- or the C restatement if that is
+    absent -- on the cpu_block^3 corner block of the field, one core."""
+    for _ in range(warmup):
+        eng.tmc(buf, iso, grid, shape=field.shape, download=False)
+    eng.synchronize()
+    stage = {}
+    t = time.perf_counter()
+    for _ in range(steps):
+        r = eng.tmc(buf, iso, grid, shape=field.shape, download=False, timing=True)
+        for k, v in eng.stage_times().items():
+            stage[k] = stage.get(k, 0.0) + v
+    eng.synchronize()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    out = {"algorithm": "topologically correct MC (mc/MarchingCubes.cpp t_mc)", "steps": steps,
+           "ms_per_step": round(ms, 4), "gvoxels_per_s": round(field.size / (ms * 1e-3) / 1e9, 3),
+           "mtris_per_s": round(r["nt"] / (ms * 1e-3) / 1e6, 2),
+           "stage_ms": {k: round(v / steps, 4) for k, v in stage.items()},
+           "V": r["nv"], "T": r["nt"], "active_cells": r["active_cells"], "ambiguous_cells": r["ambiguous_cells"],
+           "face_failures": r["face_failures"]}
+    log(f"[bench] tmc: {ms:.3f} ms/step")
+    if cpu_block:
+        import oracle
+        b = min(cpu_block, *field.shape)
+        sub = np.ascontiguousarray(field[:b, :b, :b], dtype=np.float64)
+        g = oracle.Grid(b, b, b, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+        if os.path.exists(oracle.TMC_REF_SO):
+            kind, run = "reference", oracle.TmcReference().tmc
+        else:
+            kind, run = "port", oracle.Oracle().tmc
+        t = time.perf_counter()
+        res = run(g, sub, iso)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(sub.size / dt / 1e9, 6), "unit": "Gvoxels/s", "cores": 1, "kind": kind,
+                               "sample": f"{b}^3 corner block of the same field; t_mc with gradient(), "
+                                         f"{dt:.2f} s, V={len(res['v'])} T={len(res['tris'])}; host CPU: "
+                                         f"{cpu_model()}"}
+    return out
+
+
 def calibrate_copy():
     """A plain copy kernel of known size: 1 GiB read + 1 GiB written per
     launch.  Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, the ratio of
@@ -292,6 +335,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson mix) leg")
     ap.add_argument("--no-diagnostics", action="store_true", help="skip the mesh diagnostics timings")
     ap.add_argument("--only", choices=["batch", "serve"], help="run only the batched or served leg")
+    ap.add_argument("--tmc-only", action="store_true",
+                    help="run only the topologically correct MC leg on the workload (for profiling)")
+    ap.add_argument("--no-tmc", action="store_true", help="skip the topologically correct MC leg")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -330,6 +376,11 @@ def main():
 
     eng = tmc_amd.Engine(0)
     buf = eng.alloc(field.nbytes).upload(field)
+    if args.tmc_only:
+        print(json.dumps(dict(desc, tmc=bench_tmc(eng, buf, field, grid, iso, args.steps, args.warmup))), flush=True)
+        buf.free()
+        eng.close()
+        return
     flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES
     for w in range(args.warmup):
         eng.extract(buf, iso, grid, flags=flags, shape=field.shape)
@@ -385,6 +436,10 @@ def main():
         "roofline": roofline,
     }
     out["diagnostics"] = None if args.no_diagnostics else bench_diagnostics(eng)
+    if not args.no_tmc:
+        log("[bench] topologically correct MC leg")
+        out["tmc"] = bench_tmc(eng, buf, field, grid, iso, args.steps, 1,
+                               None if args.no_cpu_baseline else min(320, args.cpu_block))
     buf.free()
     eng.close()
     if not args.no_batch:

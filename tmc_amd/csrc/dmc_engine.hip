@@ -72,7 +72,7 @@ struct dmc_context {
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     // topologically correct MC: vertex and triangle row offsets, stats, mesh
-    Buf tloc, rowT, rowBaseT, tstats, tpos, tnrm, ttris;
+    Buf tloc, rowT, rowBaseT, tstats, tpos, tnrm, ttris, tamb;
     tmc_mesh tmesh{};
     bool have_tmesh = false;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
@@ -105,7 +105,7 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
-                      &dhist, &dedges, &dqual, &tloc, &rowT, &rowBaseT, &tstats, &tpos, &tnrm, &ttris};
+                      &dhist, &dedges, &dqual, &tloc, &rowT, &rowBaseT, &tstats, &tpos, &tnrm, &ttris, &tamb};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -934,9 +934,11 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
     uint32_t* rowBaseT = ensure<uint32_t>(ctx, ctx->rowBaseT, R);
     unsigned long long* st = ensure<unsigned long long>(ctx, ctx->tstats, 3);
+    uint32_t* amb = ensure<uint32_t>(ctx, ctx->tamb, A + 1);
     HIPC(ctx, hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    HIPC(ctx, hipMemsetAsync(amb, 0, sizeof(uint32_t), ctx->stream));
     hipLaunchKernelGGL(k_tmc_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
-                       sw.lists, sw.segCount, sw.segBase, acell, acode, avloc, atloc, rowV, rowT, st);
+                       sw.lists, sw.segCount, sw.segBase, acell, acode, avloc, atloc, rowV, rowT, amb, st);
     check_launch(ctx, "k_tmc_rows");
     uint64_t* tot = totals(ctx);
     scan_to(ctx, ScanCounts{rowV}, rowBaseV, R, tot + kTotV);
@@ -960,6 +962,10 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
                            avloc, atloc, rowBaseV, rowBaseT, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
                            reinterpret_cast<uint32_t*>(st + 2));
         check_launch(ctx, "k_tmc_emit");
+        hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
+                           amb, acell, acode, avloc, atloc, rowBaseV, rowBaseT, sw.segMask, sw.segBase, pos, nrm,
+                           tris, reinterpret_cast<uint32_t*>(st + 2));
+        check_launch(ctx, "k_tmc_emit_amb");
     }
     mark(ctx, "end");
     unsigned long long hs[3];

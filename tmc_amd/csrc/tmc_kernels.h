@@ -458,7 +458,7 @@ __device__ __forceinline__ void tmc_cell_counts(const VolDesc& V, int i, int j, 
 
 // K_T1: one wave per row, cells in ascending x (the flat active order): per
 // active cell the case, and its vertex and triangle prefixes in the row; the
-// row totals.  acode bit 8 marks a cell whose singular face could not be
+// row totals; the list of ambiguous cells (amb[0] = count, from amb[1]).  acode bit 8 marks a cell whose singular face could not be
 // triangulated; stats[0] counts the ambiguous cells, stats[1] those.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, const uint32_t* __restrict__ lists,
@@ -467,6 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, 
                                                       uint32_t* __restrict__ acell, uint32_t* __restrict__ acode,
                                                       uint32_t* __restrict__ avloc, uint32_t* __restrict__ atloc,
                                                       uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowT,
+                                                      uint32_t* __restrict__ amb,
                                                       unsigned long long* __restrict__ stats) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wpb = kBlock / kWave;
@@ -488,6 +489,7 @@ __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, 
             const uint32_t a = a0 + lane;
             const bool act = a < last;
             uint32_t nv = 0, nt = 0;
+            bool isamb = false;
             if (act) {
                 int lo = 0, hi = S - 1;
                 while (lo < hi) {
@@ -500,10 +502,22 @@ __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, 
                 const unsigned icase = x >> 24;
                 bool fail;
                 tmc_cell_counts<T>(V, i, j, k, icase, nv, nt, fail);
-                namb += tmc_ambiguous(icase);
+                isamb = tmc_ambiguous(icase);
+                namb += isamb;
                 nfail += fail;
                 acell[a] = (uint32_t)(r * d.C) + (uint32_t)i;
                 acode[a] = icase | (fail ? 0x100u : 0u);
+            }
+            // ambiguous cells go to the list k_tmc_emit_amb works through
+            const uint64_t m = __ballot(isamb);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(amb, (uint32_t)__popcll(m));
+                base = (uint32_t)__shfl((int)base, leader, kWave);
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (isamb) amb[1 + base + below] = a;
             }
             const uint32_t iv = wave_inclusive_scan(nv), it = wave_inclusive_scan(nt);
             if (act) {
@@ -583,8 +597,106 @@ struct TmcWrite {
     }
 };
 
-// K_T2: one thread per active cell: its owned edge vertices, inner
-// vertices and triangles.
+// Vertices and triangles of active cell a.  AMB: the cell is ambiguous
+// (sliced with the decider, inner vertices); else the case table.  Owned
+// edge vertices take the gradients of their two corners only.
+template <class T, bool AMB>
+__device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, const uint32_t* __restrict__ acell,
+                                              const uint32_t* __restrict__ acode,
+                                              const uint32_t* __restrict__ avloc,
+                                              const uint32_t* __restrict__ atloc,
+                                              const uint32_t* __restrict__ rowBaseV0,
+                                              const uint32_t* __restrict__ rowBaseT0,
+                                              const unsigned long long* __restrict__ segMask,
+                                              const uint32_t* __restrict__ segBase, double* __restrict__ vpos,
+                                              double* __restrict__ vnrm, int32_t* __restrict__ tris,
+                                              uint32_t* __restrict__ unresolved) {
+    const int vol = vol_of(vt.aBase, vt.n, a);
+    const VolDesc& V = vt.desc[vol];
+    const Dims d = V.d;
+    const GridGeom geo = V.geo;
+    const T* __restrict__ field = static_cast<const T*>(V.field);
+    const SegLists L{segMask, segBase, V.S, V.rsBase};
+    const uint32_t* __restrict__ rowBaseV = rowBaseV0 + V.rowBase;
+    int i, j, k;
+    uint32_t r;
+    cell_ijk(acell[a], d, i, j, k, r);
+    const uint32_t code = acode[a];
+    const unsigned icase = code & 0xFFu;
+    const unsigned cut = tmc_cut_mask(icase);
+    const unsigned own = cut & tmc_owned_mask(i, j, k, d);
+    const double i0 = V.iso;
+    const int32_t vbase = (int32_t)(rowBaseV[r] + avloc[a]);
+    double F[8];
+    tmc_corners(field, d, i, j, k, F);
+    auto corner_pos = [&](int c, double p[3]) {
+        p[0] = geo.x0 + (i + (c & 1)) * geo.dx;
+        p[1] = geo.y0 + (j + ((c >> 1) & 1)) * geo.dy;
+        p[2] = geo.z0 + (k + (c >> 2)) * geo.dz;
+    };
+    int32_t ev[12];
+    int nown = 0;
+    for (int e = 0; e < 12; e++) {
+        ev[e] = -1;
+        if (!((cut >> e) & 1u)) continue;
+        if ((own >> e) & 1u) {
+            ev[e] = vbase + nown++;
+            const int v0 = c_tmc_ev[e] & 0xF, v1 = c_tmc_ev[e] >> 4;
+            const double l = (i0 - F[v0]) / (F[v1] - F[v0]);
+            double P0[3], P1[3], N0[3], N1[3];
+            corner_pos(v0, P0);
+            corner_pos(v1, P1);
+            tmc_point_gradient(field, d, geo, i + (v0 & 1), j + ((v0 >> 1) & 1), k + (v0 >> 2), N0);
+            tmc_point_gradient(field, d, geo, i + (v1 & 1), j + ((v1 >> 1) & 1), k + (v1 >> 2), N1);
+            double* p = vpos + 3 * (int64_t)ev[e];
+            double* n = vnrm + 3 * (int64_t)ev[e];
+            double q[3];
+#pragma unroll
+            for (int x = 0; x < 3; x++) p[x] = (1 - l) * P0[x] + l * P1[x];
+#pragma unroll
+            for (int x = 0; x < 3; x++) q[x] = (1 - l) * N0[x] + l * N1[x];
+            const double s = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+            n[0] = q[0] / s;
+            n[1] = q[1] / s;
+            n[2] = q[2] / s;
+            continue;
+        }
+        int oi, oj, ok, oe;
+        tmc_edge_owner(i, j, k, e, d, oi, oj, ok, oe);
+        const int64_t ao = find_active(L, d, oi, oj, ok);
+        if (ao < 0) {
+            atomicAdd(unresolved, 1u);
+            continue;
+        }
+        const unsigned ocut = tmc_cut_mask(acode[ao] & 0xFFu) & tmc_owned_mask(oi, oj, ok, d);
+        const int64_t orow = (int64_t)ok * (d.ny - 1) + oj;
+        ev[e] = (int32_t)(rowBaseV[orow] + avloc[ao] + (uint32_t)__popc(ocut & ((1u << oe) - 1u)));
+    }
+    const int64_t tbase = (int64_t)rowBaseT0[V.rowBase + r] + atloc[a];
+    if constexpr (AMB) {
+        if (code & 0x100u) return;   // singular face: no triangles
+        double P[8][3], N[8][3];
+        for (int c = 0; c < 8; c++) {
+            corner_pos(c, P[c]);
+            tmc_point_gradient(field, d, geo, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2), N[c]);
+        }
+        TmcWrite w{P, N, ev, vbase + nown, vpos, vnrm, tris, tbase};
+        tmc_slice(F, icase, i0, w);
+    } else {
+        const uint64_t wd = c_tmc_tri[icase];
+        const int nt = (int)(wd >> 60);
+        for (int t = 0; t < nt; t++) {
+            const int e0 = (int)((wd >> (12 * t)) & 0xF), e1 = (int)((wd >> (12 * t + 4)) & 0xF),
+                      e2 = (int)((wd >> (12 * t + 8)) & 0xF);
+            int32_t* o = tris + 3 * (tbase + t);
+            o[0] = ev[e2];   // reversed: positive corners outside
+            o[1] = ev[e1];
+            o[2] = ev[e0];
+        }
+    }
+}
+
+// K_T2: one thread per active cell that is not ambiguous (the case table).
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_tmc_emit(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
@@ -597,86 +709,31 @@ __global__ void __launch_bounds__(kBlock) k_tmc_emit(VolTable vt, const uint32_t
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
                                                       int32_t* __restrict__ tris, uint32_t* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
-        const int vol = vol_of(vt.aBase, vt.n, a);
-        const VolDesc& V = vt.desc[vol];
-        const Dims d = V.d;
-        const GridGeom geo = V.geo;
-        const T* __restrict__ field = static_cast<const T*>(V.field);
-        const SegLists L{segMask, segBase, V.S, V.rsBase};
-        const uint32_t* __restrict__ rowBaseV = rowBaseV0 + V.rowBase;
-        int i, j, k;
-        uint32_t r;
-        cell_ijk(acell[a], d, i, j, k, r);
-        const uint32_t code = acode[a];
-        const unsigned icase = code & 0xFFu;
-        const unsigned cut = tmc_cut_mask(icase);
-        const unsigned own = cut & tmc_owned_mask(i, j, k, d);
-        const double i0 = V.iso;
-        const int32_t vbase = (int32_t)(rowBaseV[r] + avloc[a]);
-        double F[8], P[8][3], N[8][3];
-        tmc_corners(field, d, i, j, k, F);
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int ci = i + (c & 1), cj = j + ((c >> 1) & 1), ck = k + (c >> 2);
-            P[c][0] = geo.x0 + ci * geo.dx;
-            P[c][1] = geo.y0 + cj * geo.dy;
-            P[c][2] = geo.z0 + ck * geo.dz;
-        }
-        const bool amb = tmc_ambiguous(icase);
-        if (own || amb)
-            for (int c = 0; c < 8; c++)
-                tmc_point_gradient(field, d, geo, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2), N[c]);
-        int32_t ev[12];
-        int nown = 0;
-        for (int e = 0; e < 12; e++) {
-            ev[e] = -1;
-            if (!((cut >> e) & 1u)) continue;
-            if ((own >> e) & 1u) {
-                ev[e] = vbase + nown++;
-                const int v0 = c_tmc_ev[e] & 0xF, v1 = c_tmc_ev[e] >> 4;
-                const double l = (i0 - F[v0]) / (F[v1] - F[v0]);
-                double* p = vpos + 3 * (int64_t)ev[e];
-                double* n = vnrm + 3 * (int64_t)ev[e];
-                double q[3];
-#pragma unroll
-                for (int x = 0; x < 3; x++) p[x] = (1 - l) * P[v0][x] + l * P[v1][x];
-#pragma unroll
-                for (int x = 0; x < 3; x++) q[x] = (1 - l) * N[v0][x] + l * N[v1][x];
-                const double s = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-                n[0] = q[0] / s;
-                n[1] = q[1] / s;
-                n[2] = q[2] / s;
-                continue;
-            }
-            int oi, oj, ok, oe;
-            tmc_edge_owner(i, j, k, e, d, oi, oj, ok, oe);
-            const int64_t ao = find_active(L, d, oi, oj, ok);
-            if (ao < 0) {
-                atomicAdd(unresolved, 1u);
-                continue;
-            }
-            const unsigned ocut = tmc_cut_mask(acode[ao] & 0xFFu) & tmc_owned_mask(oi, oj, ok, d);
-            const int64_t orow = (int64_t)ok * (d.ny - 1) + oj;
-            ev[e] = (int32_t)(rowBaseV[orow] + avloc[ao] + (uint32_t)__popc(ocut & ((1u << oe) - 1u)));
-        }
-        const int64_t tbase = (int64_t)rowBaseT0[V.rowBase + r] + atloc[a];
-        if (amb) {
-            if (code & 0x100u) continue;   // singular face: no triangles
-            TmcWrite w{P, N, ev, vbase + nown, vpos, vnrm, tris, tbase};
-            tmc_slice(F, icase, i0, w);
-        } else {
-            const uint64_t wd = c_tmc_tri[icase];
-            const int nt = (int)(wd >> 60);
-            for (int t = 0; t < nt; t++) {
-                const int e0 = (int)((wd >> (12 * t)) & 0xF), e1 = (int)((wd >> (12 * t + 4)) & 0xF),
-                          e2 = (int)((wd >> (12 * t + 8)) & 0xF);
-                int32_t* o = tris + 3 * (tbase + t);
-                o[0] = ev[e2];   // reversed: positive corners outside
-                o[1] = ev[e1];
-                o[2] = ev[e0];
-            }
-        }
+        if (tmc_ambiguous(acode[a] & 0xFFu)) continue;
+        tmc_emit_cell<T, false>(vt, a, acell, acode, avloc, atloc, rowBaseV0, rowBaseT0, segMask, segBase, vpos, vnrm,
+                                tris, unresolved);
     }
+}
+
+// K_T3: the ambiguous cells k_tmc_rows listed (amb[0] = count, cells from
+// amb[1]): sliced with the asymptotic decider.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_tmc_emit_amb(VolTable vt, const uint32_t* __restrict__ amb,
+                                                          const uint32_t* __restrict__ acell,
+                                                          const uint32_t* __restrict__ acode,
+                                                          const uint32_t* __restrict__ avloc,
+                                                          const uint32_t* __restrict__ atloc,
+                                                          const uint32_t* __restrict__ rowBaseV0,
+                                                          const uint32_t* __restrict__ rowBaseT0,
+                                                          const unsigned long long* __restrict__ segMask,
+                                                          const uint32_t* __restrict__ segBase,
+                                                          double* __restrict__ vpos, double* __restrict__ vnrm,
+                                                          int32_t* __restrict__ tris,
+                                                          uint32_t* __restrict__ unresolved) {
+    const uint32_t n = amb[0];
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
+        tmc_emit_cell<T, true>(vt, (int64_t)amb[1 + s], acell, acode, avloc, atloc, rowBaseV0, rowBaseT0, segMask,
+                               segBase, vpos, vnrm, tris, unresolved);
 }
 
 }  // namespace tmc
