@@ -72,7 +72,7 @@ struct dmc_context {
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     // topologically correct MC: vertex and triangle row offsets, stats, mesh
-    Buf tloc, rowT, rowBaseT, tstats, tpos, tnrm, ttris, tamb;
+    Buf tloc, rowT, rowBaseT, tstats, tpos, tnrm, ttris, tamb, tvb;
     tmc_mesh tmesh{};
     bool have_tmesh = false;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
@@ -105,7 +105,7 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
-                      &dhist, &dedges, &dqual, &tloc, &rowT, &rowBaseT, &tstats, &tpos, &tnrm, &ttris, &tamb};
+                      &dhist, &dedges, &dqual, &tloc, &rowT, &rowBaseT, &tstats, &tpos, &tnrm, &ttris, &tamb, &tvb};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -956,15 +956,19 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     double* pos = ensure<double>(ctx, ctx->tpos, 3 * V);
     double* nrm = ensure<double>(ctx, ctx->tnrm, 3 * V);
     int32_t* tris = ensure<int32_t>(ctx, ctx->ttris, 3 * Tn);
+    uint2* avb = ensure<uint2>(ctx, ctx->tvb, A);
     mark(ctx, "emit");
     if (A > 0) {
-        hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode,
-                           avloc, atloc, rowBaseV, rowBaseT, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
+        hipLaunchKernelGGL(k_tmc_vbase, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avloc,
+                           rowBaseV, (int64_t)A, avb);
+        check_launch(ctx, "k_tmc_vbase");
+        hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avb,
+                           atloc, rowBaseT, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
                            reinterpret_cast<uint32_t*>(st + 2));
         check_launch(ctx, "k_tmc_emit");
         hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
-                           amb, acell, acode, avloc, atloc, rowBaseV, rowBaseT, sw.segMask, sw.segBase, pos, nrm,
-                           tris, reinterpret_cast<uint32_t*>(st + 2));
+                           amb, acell, acode, avb, atloc, rowBaseT, sw.segMask, sw.segBase, pos, nrm, tris,
+                           reinterpret_cast<uint32_t*>(st + 2));
         check_launch(ctx, "k_tmc_emit_amb");
     }
     mark(ctx, "end");

@@ -540,22 +540,45 @@ __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, 
     }
 }
 
-// Gradient at grid point (x, y, z): central differences inside, one-sided
-// at the border (mc/MarchingCubes.cpp:1450-1512).
+// Gradient at corner c of cell (i, j, k): central differences inside the
+// grid, one-sided at its border (mc/MarchingCubes.cpp:1450-1512).  The
+// neighbours inside the cell come from its corner values F (the same
+// doubles); only the three outside it are loaded.
 template <class T>
-__device__ __forceinline__ void tmc_point_gradient(const T* __restrict__ field, const Dims& d, const GridGeom& geo,
-                                                   int x, int y, int z, double n[3]) {
+__device__ __forceinline__ void tmc_corner_gradient(const T* __restrict__ field, const Dims& d, const GridGeom& geo,
+                                                    const double F[8], int i, int j, int k, int c, double n[3]) {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+    const int x = i + cx, y = j + cy, z = k + cz;
     const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
     const T* p = field + (int64_t)z * sz + (int64_t)y * sy + x;
-    if (x == 0) n[0] = ((double)p[1] - (double)p[0]) / geo.dx;
-    else if (x == d.nx - 1) n[0] = ((double)p[0] - (double)p[-1]) / geo.dx;
-    else n[0] = ((double)p[1] - (double)p[-1]) / (2 * geo.dx);
-    if (y == 0) n[1] = ((double)p[sy] - (double)p[0]) / geo.dy;
-    else if (y == d.ny - 1) n[1] = ((double)p[0] - (double)p[-sy]) / geo.dy;
-    else n[1] = ((double)p[sy] - (double)p[-sy]) / (2 * geo.dy);
-    if (z == 0) n[2] = ((double)p[sz] - (double)p[0]) / geo.dz;
-    else if (z == d.nz - 1) n[2] = ((double)p[0] - (double)p[-sz]) / geo.dz;
-    else n[2] = ((double)p[sz] - (double)p[-sz]) / (2 * geo.dz);
+    // x == 0 only at cx == 0, x == nx - 1 only at cx == 1 (likewise y, z)
+    if (x == 0) n[0] = (F[c | 1] - F[c]) / geo.dx;
+    else if (x == d.nx - 1) n[0] = (F[c] - F[c & ~1]) / geo.dx;
+    else n[0] = (cx ? (double)p[1] - F[c & ~1] : F[c | 1] - (double)p[-1]) / (2 * geo.dx);
+    if (y == 0) n[1] = (F[c | 2] - F[c]) / geo.dy;
+    else if (y == d.ny - 1) n[1] = (F[c] - F[c & ~2]) / geo.dy;
+    else n[1] = (cy ? (double)p[sy] - F[c & ~2] : F[c | 2] - (double)p[-sy]) / (2 * geo.dy);
+    if (z == 0) n[2] = (F[c | 4] - F[c]) / geo.dz;
+    else if (z == d.nz - 1) n[2] = (F[c] - F[c & ~4]) / geo.dz;
+    else n[2] = (cz ? (double)p[sz] - F[c & ~4] : F[c | 4] - (double)p[-sz]) / (2 * geo.dz);
+}
+
+// After the row scans: per active cell its owned cut edges (x) and first
+// vertex id (y), one 8-byte word for the owner lookups of k_tmc_emit.
+__global__ void __launch_bounds__(kBlock) k_tmc_vbase(VolTable vt, const uint32_t* __restrict__ acell,
+                                                       const uint32_t* __restrict__ acode,
+                                                       const uint32_t* __restrict__ avloc,
+                                                       const uint32_t* __restrict__ rowBaseV0, int64_t A,
+                                                       uint2* __restrict__ avb) {
+    for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
+        const int vol = vol_of(vt.aBase, vt.n, a);
+        const VolDesc& V = vt.desc[vol];
+        int i, j, k;
+        uint32_t r;
+        cell_ijk(acell[a], V.d, i, j, k, r);
+        const unsigned own = tmc_cut_mask(acode[a] & 0xFFu) & tmc_owned_mask(i, j, k, V.d);
+        avb[a] = make_uint2(own, rowBaseV0[V.rowBase + r] + avloc[a]);
+    }
 }
 
 // Writing emitter: triangles with global vertex ids (edge refs through ev,
@@ -603,9 +626,8 @@ struct TmcWrite {
 template <class T, bool AMB>
 __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, const uint32_t* __restrict__ acell,
                                               const uint32_t* __restrict__ acode,
-                                              const uint32_t* __restrict__ avloc,
+                                              const uint2* __restrict__ avb,
                                               const uint32_t* __restrict__ atloc,
-                                              const uint32_t* __restrict__ rowBaseV0,
                                               const uint32_t* __restrict__ rowBaseT0,
                                               const unsigned long long* __restrict__ segMask,
                                               const uint32_t* __restrict__ segBase, double* __restrict__ vpos,
@@ -617,16 +639,16 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
     const GridGeom geo = V.geo;
     const T* __restrict__ field = static_cast<const T*>(V.field);
     const SegLists L{segMask, segBase, V.S, V.rsBase};
-    const uint32_t* __restrict__ rowBaseV = rowBaseV0 + V.rowBase;
     int i, j, k;
     uint32_t r;
     cell_ijk(acell[a], d, i, j, k, r);
     const uint32_t code = acode[a];
     const unsigned icase = code & 0xFFu;
     const unsigned cut = tmc_cut_mask(icase);
-    const unsigned own = cut & tmc_owned_mask(i, j, k, d);
+    const uint2 self = avb[a];
+    const unsigned own = self.x;
     const double i0 = V.iso;
-    const int32_t vbase = (int32_t)(rowBaseV[r] + avloc[a]);
+    const int32_t vbase = (int32_t)self.y;
     double F[8];
     tmc_corners(field, d, i, j, k, F);
     auto corner_pos = [&](int c, double p[3]) {
@@ -646,8 +668,8 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
             double P0[3], P1[3], N0[3], N1[3];
             corner_pos(v0, P0);
             corner_pos(v1, P1);
-            tmc_point_gradient(field, d, geo, i + (v0 & 1), j + ((v0 >> 1) & 1), k + (v0 >> 2), N0);
-            tmc_point_gradient(field, d, geo, i + (v1 & 1), j + ((v1 >> 1) & 1), k + (v1 >> 2), N1);
+            tmc_corner_gradient(field, d, geo, F, i, j, k, v0, N0);
+            tmc_corner_gradient(field, d, geo, F, i, j, k, v1, N1);
             double* p = vpos + 3 * (int64_t)ev[e];
             double* n = vnrm + 3 * (int64_t)ev[e];
             double q[3];
@@ -668,9 +690,8 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
             atomicAdd(unresolved, 1u);
             continue;
         }
-        const unsigned ocut = tmc_cut_mask(acode[ao] & 0xFFu) & tmc_owned_mask(oi, oj, ok, d);
-        const int64_t orow = (int64_t)ok * (d.ny - 1) + oj;
-        ev[e] = (int32_t)(rowBaseV[orow] + avloc[ao] + (uint32_t)__popc(ocut & ((1u << oe) - 1u)));
+        const uint2 ob = avb[ao];
+        ev[e] = (int32_t)(ob.y + (uint32_t)__popc(ob.x & ((1u << oe) - 1u)));
     }
     const int64_t tbase = (int64_t)rowBaseT0[V.rowBase + r] + atloc[a];
     if constexpr (AMB) {
@@ -678,7 +699,7 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
         double P[8][3], N[8][3];
         for (int c = 0; c < 8; c++) {
             corner_pos(c, P[c]);
-            tmc_point_gradient(field, d, geo, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2), N[c]);
+            tmc_corner_gradient(field, d, geo, F, i, j, k, c, N[c]);
         }
         TmcWrite w{P, N, ev, vbase + nown, vpos, vnrm, tris, tbase};
         tmc_slice(F, icase, i0, w);
@@ -700,9 +721,8 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_tmc_emit(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
-                                                      const uint32_t* __restrict__ avloc,
+                                                      const uint2* __restrict__ avb,
                                                       const uint32_t* __restrict__ atloc,
-                                                      const uint32_t* __restrict__ rowBaseV0,
                                                       const uint32_t* __restrict__ rowBaseT0,
                                                       const unsigned long long* __restrict__ segMask,
                                                       const uint32_t* __restrict__ segBase, int64_t A,
@@ -710,8 +730,8 @@ __global__ void __launch_bounds__(kBlock) k_tmc_emit(VolTable vt, const uint32_t
                                                       int32_t* __restrict__ tris, uint32_t* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         if (tmc_ambiguous(acode[a] & 0xFFu)) continue;
-        tmc_emit_cell<T, false>(vt, a, acell, acode, avloc, atloc, rowBaseV0, rowBaseT0, segMask, segBase, vpos, vnrm,
-                                tris, unresolved);
+        tmc_emit_cell<T, false>(vt, a, acell, acode, avb, atloc, rowBaseT0, segMask, segBase, vpos, vnrm, tris,
+                                unresolved);
     }
 }
 
@@ -721,9 +741,8 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_tmc_emit_amb(VolTable vt, const uint32_t* __restrict__ amb,
                                                           const uint32_t* __restrict__ acell,
                                                           const uint32_t* __restrict__ acode,
-                                                          const uint32_t* __restrict__ avloc,
+                                                          const uint2* __restrict__ avb,
                                                           const uint32_t* __restrict__ atloc,
-                                                          const uint32_t* __restrict__ rowBaseV0,
                                                           const uint32_t* __restrict__ rowBaseT0,
                                                           const unsigned long long* __restrict__ segMask,
                                                           const uint32_t* __restrict__ segBase,
@@ -732,8 +751,8 @@ __global__ void __launch_bounds__(kBlock) k_tmc_emit_amb(VolTable vt, const uint
                                                           uint32_t* __restrict__ unresolved) {
     const uint32_t n = amb[0];
     for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
-        tmc_emit_cell<T, true>(vt, (int64_t)amb[1 + s], acell, acode, avloc, atloc, rowBaseV0, rowBaseT0, segMask,
-                               segBase, vpos, vnrm, tris, unresolved);
+        tmc_emit_cell<T, true>(vt, (int64_t)amb[1 + s], acell, acode, avb, atloc, rowBaseT0, segMask, segBase, vpos,
+                               vnrm, tris, unresolved);
 }
 
 }  // namespace tmc
