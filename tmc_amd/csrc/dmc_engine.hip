@@ -72,7 +72,7 @@ struct dmc_context {
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     // topologically correct MC: vertex and triangle row offsets, stats, mesh
-    Buf tloc, rowT, rowBaseT, tstats, tpos, tnrm, ttris, tamb, tvb;
+    Buf tloc, rowT, tstats, tpos, tnrm, ttris, tamb, tvb;
     tmc_mesh tmesh{};
     bool have_tmesh = false;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
@@ -105,7 +105,7 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
-                      &dhist, &dedges, &dqual, &tloc, &rowT, &rowBaseT, &tstats, &tpos, &tnrm, &ttris, &tamb, &tvb};
+                      &dhist, &dedges, &dqual, &tloc, &rowT, &tstats, &tpos, &tnrm, &ttris, &tamb, &tvb};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -918,8 +918,9 @@ void check_grid(dmc_context* ctx, const dmc_grid& g) {
 }
 
 // Topologically correct MC of one volume (mc/MarchingCubes.cpp t_mc): the
-// sweep's active cells, their vertex and triangle counts per row, the row
-// scans, then vertices and triangles.
+// sweep's active cells, their vertex and triangle counts, scans over the
+// active cells (whose order is the reference's cell order), then vertices
+// and triangles.
 template <class T>
 void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const Swept sw = sweep_cells<T>(ctx, hv);
@@ -927,22 +928,25 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const int64_t R = sw.R;
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
-    uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, A);
-    uint32_t* atloc = ensure<uint32_t>(ctx, ctx->tloc, A);
-    uint32_t* rowV = ensure<uint32_t>(ctx, ctx->rowV, R);
-    uint32_t* rowT = ensure<uint32_t>(ctx, ctx->rowT, R);
-    uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
-    uint32_t* rowBaseT = ensure<uint32_t>(ctx, ctx->rowBaseT, R);
+    uint32_t* cnt_v = ensure<uint32_t>(ctx, ctx->avloc, A);
+    uint32_t* cnt_t = ensure<uint32_t>(ctx, ctx->tloc, A);
+    uint32_t* vofs = ensure<uint32_t>(ctx, ctx->rowV, A);
+    uint32_t* tofs = ensure<uint32_t>(ctx, ctx->rowT, A);
     unsigned long long* st = ensure<unsigned long long>(ctx, ctx->tstats, 3);
     uint32_t* amb = ensure<uint32_t>(ctx, ctx->tamb, A + 1);
     HIPC(ctx, hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), ctx->stream));
     HIPC(ctx, hipMemsetAsync(amb, 0, sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_tmc_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
-                       sw.lists, sw.segCount, sw.segBase, acell, acode, avloc, atloc, rowV, rowT, amb, st);
-    check_launch(ctx, "k_tmc_rows");
+    hipLaunchKernelGGL(k_tmc_cells, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
+                       sw.lists, sw.segCount, sw.segBase, acell, acode, cnt_v, cnt_t, amb, st);
+    check_launch(ctx, "k_tmc_cells");
+    if (A > 0) {
+        hipLaunchKernelGGL(k_tmc_amb_count<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
+                           amb, acell, acode, cnt_v, cnt_t, st);
+        check_launch(ctx, "k_tmc_amb_count");
+    }
     uint64_t* tot = totals(ctx);
-    scan_to(ctx, ScanCounts{rowV}, rowBaseV, R, tot + kTotV);
-    scan_to(ctx, ScanCounts{rowT}, rowBaseT, R, tot + kTotQ);
+    scan_to(ctx, ScanCounts{cnt_v}, vofs, (int64_t)A, tot + kTotV);
+    scan_to(ctx, ScanCounts{cnt_t}, tofs, (int64_t)A, tot + kTotQ);
     uint64_t V = 0, Tn = 0;
     {
         HostReads rd(ctx);
@@ -959,15 +963,15 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint2* avb = ensure<uint2>(ctx, ctx->tvb, A);
     mark(ctx, "emit");
     if (A > 0) {
-        hipLaunchKernelGGL(k_tmc_vbase, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avloc,
-                           rowBaseV, (int64_t)A, avb);
+        hipLaunchKernelGGL(k_tmc_vbase, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, vofs,
+                           (int64_t)A, avb);
         check_launch(ctx, "k_tmc_vbase");
         hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avb,
-                           atloc, rowBaseT, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
+                           tofs, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
                            reinterpret_cast<uint32_t*>(st + 2));
         check_launch(ctx, "k_tmc_emit");
         hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
-                           amb, acell, acode, avb, atloc, rowBaseT, sw.segMask, sw.segBase, pos, nrm, tris,
+                           amb, acell, acode, avb, tofs, sw.segMask, sw.segBase, pos, nrm, tris,
                            reinterpret_cast<uint32_t*>(st + 2));
         check_launch(ctx, "k_tmc_emit_amb");
     }

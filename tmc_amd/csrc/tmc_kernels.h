@@ -3,14 +3,21 @@
 // counterpart is cuda_tmc/MarchingCubes.cu t_slice :1366).
 //
 // The active cells come from the DMC sweep (same classification, f >= iso,
-// same per-(row, segment) lists).  Two passes follow:
-//   k_tmc_rows  one wave per row: each active cell's new vertices (the cut
-//               edges it owns, then its inner vertices) and triangles, with
-//               their prefix inside the row -- the reference's numbering,
-//               which is first sight in (k, j, i) cell order;
-//   k_tmc_emit  one thread per active cell: positions and normals of the
-//               cell's new vertices, and its triangles with the vertex ids
-//               of cut edges owned by earlier cells.
+// same per-(row, segment) lists, whose flat order is the reference's
+// (k, j, i) cell order).  Then:
+//   k_tmc_cells      one wave per row: each active cell's case, new vertices
+//                    (the cut edges it owns) and table triangles; the
+//                    ambiguous cells listed;
+//   k_tmc_amb_count  the listed cells' inner vertices and triangles from the
+//                    decider slicing (kept out of the row walk, where one
+//                    ambiguous lane would hold up its wave);
+//   scans            over the active cells: each cell's first vertex and
+//                    triangle -- the reference's numbering, first sight in
+//                    cell order;
+//   k_tmc_emit       one thread per non-ambiguous active cell, and
+//   k_tmc_emit_amb   one per listed cell: positions and normals of the cell's
+//                    new vertices, and its triangles with the vertex ids of
+//                    cut edges owned by earlier cells.
 // A grid edge is owned by the first cell containing it in (k, j, i) order;
 // that cell is active because the edge is cut, so its active index is found
 // from the sweep's activity words.
@@ -432,46 +439,22 @@ __device__ __forceinline__ void tmc_corners(const T* __restrict__ field, const D
     for (int c = 0; c < 8; c++) F[c] = (double)p[(c & 1) + ((c >> 1) & 1) * sy + (c >> 2) * sz];
 }
 
-// New vertices (owned cut edges, then inner vertices) and triangles of an
-// active cell.
-template <class T>
-__device__ __forceinline__ void tmc_cell_counts(const VolDesc& V, int i, int j, int k, unsigned icase, uint32_t& nv,
-                                                uint32_t& nt, bool& fail) {
-    const unsigned cut = tmc_cut_mask(icase);
-    nv = (uint32_t)__popc(cut & tmc_owned_mask(i, j, k, V.d));
-    fail = false;
-    if (tmc_ambiguous(icase)) {
-        double F[8];
-        tmc_corners(static_cast<const T*>(V.field), V.d, i, j, k, F);
-        TmcCount c;
-        if (tmc_slice(F, icase, V.iso, c)) {
-            nv += (uint32_t)c.ninner;
-            nt = (uint32_t)c.nt;
-        } else {
-            nt = 0;
-            fail = true;
-        }
-    } else {
-        nt = (uint32_t)(c_tmc_tri[icase] >> 60);
-    }
-}
-
-// K_T1: one wave per row, cells in ascending x (the flat active order): per
-// active cell the case, and its vertex and triangle prefixes in the row; the
-// row totals; the list of ambiguous cells (amb[0] = count, from amb[1]).  acode bit 8 marks a cell whose singular face could not be
-// triangulated; stats[0] counts the ambiguous cells, stats[1] those.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, const uint32_t* __restrict__ lists,
-                                                      const uint32_t* __restrict__ segCount,
-                                                      const uint32_t* __restrict__ segBase,
-                                                      uint32_t* __restrict__ acell, uint32_t* __restrict__ acode,
-                                                      uint32_t* __restrict__ avloc, uint32_t* __restrict__ atloc,
-                                                      uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowT,
-                                                      uint32_t* __restrict__ amb,
-                                                      unsigned long long* __restrict__ stats) {
+// K_T1: one wave per row, cells in ascending x (the flat active order):
+// per active cell its cell word, case (acode), new vertices (cnt_v: the cut
+// edges it owns) and triangles (cnt_t: the case table's); the ambiguous
+// cells are listed (amb[0] = count, from amb[1]) for k_tmc_amb_count, which
+// adds their inner vertices and sets their triangles.  stats[0] counts
+// them.
+__global__ void __launch_bounds__(kBlock) k_tmc_cells(VolTable vt, int64_t Rtot, const uint32_t* __restrict__ lists,
+                                                       const uint32_t* __restrict__ segCount,
+                                                       const uint32_t* __restrict__ segBase,
+                                                       uint32_t* __restrict__ acell, uint32_t* __restrict__ acode,
+                                                       uint32_t* __restrict__ cnt_v, uint32_t* __restrict__ cnt_t,
+                                                       uint32_t* __restrict__ amb,
+                                                       unsigned long long* __restrict__ stats) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wpb = kBlock / kWave;
-    unsigned long long namb = 0, nfail = 0;
+    unsigned long long namb = 0;
     for (int64_t g = (int64_t)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave); g < Rtot;
          g += (int64_t)gridDim.x * wpb) {
         const int v = vol_of(vt.rowBase, vt.n, g);
@@ -482,33 +465,50 @@ __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, 
         const int j = (int)(r % (d.ny - 1));
         const int k = (int)(r / (d.ny - 1));
         const int64_t rs0 = V.rsBase + r * S;
-        const uint32_t first = segBase[rs0];
+        // the row's segment bases, one per lane (rows of at most 64
+        // segments; wider rows search memory)
+        const bool regs = S <= kWave;
+        const uint32_t mybase = (regs && lane < S) ? segBase[rs0 + lane] : 0u;
+        const uint32_t first = regs ? (uint32_t)__builtin_amdgcn_readfirstlane((int)mybase) : segBase[rs0];
         const uint32_t last = segBase[rs0 + S - 1] + segCount[rs0 + S - 1];
-        uint32_t cv = 0, ct = 0;
         for (uint32_t a0 = first; a0 < last; a0 += kWave) {
             const uint32_t a = a0 + lane;
             const bool act = a < last;
-            uint32_t nv = 0, nt = 0;
+            int seg = 0;
+            uint32_t sbase = first;
+            if (regs) {
+                for (int s2 = 1; s2 < S; s2++) {
+                    const uint32_t b = (uint32_t)__shfl((int)mybase, s2, kWave);
+                    if (b <= a) {
+                        seg = s2;
+                        sbase = b;
+                    }
+                }
+            }
             bool isamb = false;
             if (act) {
-                int lo = 0, hi = S - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (segBase[rs0 + mid] <= a) lo = mid;
-                    else hi = mid - 1;
+                if (!regs) {
+                    int lo = 0, hi = S - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (segBase[rs0 + mid] <= a) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    seg = lo;
+                    sbase = segBase[rs0 + lo];
                 }
-                const uint32_t x = lists[(rs0 + lo) * kSeg + (a - segBase[rs0 + lo])];
+                const uint32_t x = lists[(rs0 + seg) * kSeg + (a - sbase)];
                 const int i = (int)(x & 0xFFFFFFu);
                 const unsigned icase = x >> 24;
-                bool fail;
-                tmc_cell_counts<T>(V, i, j, k, icase, nv, nt, fail);
                 isamb = tmc_ambiguous(icase);
                 namb += isamb;
-                nfail += fail;
                 acell[a] = (uint32_t)(r * d.C) + (uint32_t)i;
-                acode[a] = icase | (fail ? 0x100u : 0u);
+                acode[a] = icase;
+                cnt_v[a] = (uint32_t)__popc(tmc_cut_mask(icase) & tmc_owned_mask(i, j, k, d));
+                cnt_t[a] = isamb ? 0u : (uint32_t)(c_tmc_tri[icase] >> 60);
             }
-            // ambiguous cells go to the list k_tmc_emit_amb works through
+            // ambiguous cells go to the list k_tmc_amb_count and k_tmc_emit_amb
+            // work through
             const uint64_t m = __ballot(isamb);
             if (m) {
                 const int leader = __ffsll((unsigned long long)m) - 1;
@@ -519,24 +519,41 @@ __global__ void __launch_bounds__(kBlock) k_tmc_rows(VolTable vt, int64_t Rtot, 
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 if (isamb) amb[1 + base + below] = a;
             }
-            const uint32_t iv = wave_inclusive_scan(nv), it = wave_inclusive_scan(nt);
-            if (act) {
-                avloc[a] = cv + iv - nv;
-                atloc[a] = ct + it - nt;
-            }
-            cv += (uint32_t)__shfl((int)iv, kWave - 1, kWave);
-            ct += (uint32_t)__shfl((int)it, kWave - 1, kWave);
-        }
-        if (lane == 0) {
-            rowV[g] = cv;
-            rowT[g] = ct;
         }
     }
     namb = wave_sum(namb);
-    nfail = wave_sum(nfail);
-    if (lane == 0 && (namb || nfail)) {
-        atomicAdd(&stats[0], namb);
-        atomicAdd(&stats[1], nfail);
+    if (lane == 0 && namb) atomicAdd(&stats[0], namb);
+}
+
+// K_T2: the listed ambiguous cells' inner vertices (added to cnt_v) and
+// triangles (cnt_t) from the decider slicing; acode bit 8 marks a cell
+// whose singular face could not be triangulated (no triangles), counted in
+// stats[1].
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_tmc_amb_count(VolTable vt, const uint32_t* __restrict__ amb,
+                                                           const uint32_t* __restrict__ acell,
+                                                           uint32_t* __restrict__ acode, uint32_t* __restrict__ cnt_v,
+                                                           uint32_t* __restrict__ cnt_t,
+                                                           unsigned long long* __restrict__ stats) {
+    const uint32_t n = amb[0];
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
+        const uint32_t a = amb[1 + s];
+        const int vol = vol_of(vt.aBase, vt.n, a);
+        const VolDesc& V = vt.desc[vol];
+        int i, j, k;
+        uint32_t r;
+        cell_ijk(acell[a], V.d, i, j, k, r);
+        const unsigned icase = acode[a] & 0xFFu;
+        double F[8];
+        tmc_corners(static_cast<const T*>(V.field), V.d, i, j, k, F);
+        TmcCount c;
+        if (tmc_slice(F, icase, V.iso, c)) {
+            cnt_v[a] += (uint32_t)c.ninner;
+            cnt_t[a] = (uint32_t)c.nt;
+        } else {
+            acode[a] = icase | 0x100u;
+            atomicAdd(&stats[1], 1ull);
+        }
     }
 }
 
@@ -563,12 +580,12 @@ __device__ __forceinline__ void tmc_corner_gradient(const T* __restrict__ field,
     else n[2] = (cz ? (double)p[sz] - F[c & ~4] : F[c | 4] - (double)p[-sz]) / (2 * geo.dz);
 }
 
-// After the row scans: per active cell its owned cut edges (x) and first
-// vertex id (y), one 8-byte word for the owner lookups of k_tmc_emit.
+// After the scans: per active cell its owned cut edges (x) and first vertex
+// id (y, the exclusive scan of cnt_v), one 8-byte word for the owner
+// lookups of k_tmc_emit.
 __global__ void __launch_bounds__(kBlock) k_tmc_vbase(VolTable vt, const uint32_t* __restrict__ acell,
                                                        const uint32_t* __restrict__ acode,
-                                                       const uint32_t* __restrict__ avloc,
-                                                       const uint32_t* __restrict__ rowBaseV0, int64_t A,
+                                                       const uint32_t* __restrict__ vofs, int64_t A,
                                                        uint2* __restrict__ avb) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         const int vol = vol_of(vt.aBase, vt.n, a);
@@ -577,7 +594,7 @@ __global__ void __launch_bounds__(kBlock) k_tmc_vbase(VolTable vt, const uint32_
         uint32_t r;
         cell_ijk(acell[a], V.d, i, j, k, r);
         const unsigned own = tmc_cut_mask(acode[a] & 0xFFu) & tmc_owned_mask(i, j, k, V.d);
-        avb[a] = make_uint2(own, rowBaseV0[V.rowBase + r] + avloc[a]);
+        avb[a] = make_uint2(own, vofs[a]);
     }
 }
 
@@ -627,8 +644,7 @@ template <class T, bool AMB>
 __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, const uint32_t* __restrict__ acell,
                                               const uint32_t* __restrict__ acode,
                                               const uint2* __restrict__ avb,
-                                              const uint32_t* __restrict__ atloc,
-                                              const uint32_t* __restrict__ rowBaseT0,
+                                              const uint32_t* __restrict__ tofs,
                                               const unsigned long long* __restrict__ segMask,
                                               const uint32_t* __restrict__ segBase, double* __restrict__ vpos,
                                               double* __restrict__ vnrm, int32_t* __restrict__ tris,
@@ -693,7 +709,7 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
         const uint2 ob = avb[ao];
         ev[e] = (int32_t)(ob.y + (uint32_t)__popc(ob.x & ((1u << oe) - 1u)));
     }
-    const int64_t tbase = (int64_t)rowBaseT0[V.rowBase + r] + atloc[a];
+    const int64_t tbase = tofs[a];
     if constexpr (AMB) {
         if (code & 0x100u) return;   // singular face: no triangles
         double P[8][3], N[8][3];
@@ -717,33 +733,30 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
     }
 }
 
-// K_T2: one thread per active cell that is not ambiguous (the case table).
+// K_T3: one thread per active cell that is not ambiguous (the case table).
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_tmc_emit(VolTable vt, const uint32_t* __restrict__ acell,
                                                       const uint32_t* __restrict__ acode,
                                                       const uint2* __restrict__ avb,
-                                                      const uint32_t* __restrict__ atloc,
-                                                      const uint32_t* __restrict__ rowBaseT0,
+                                                      const uint32_t* __restrict__ tofs,
                                                       const unsigned long long* __restrict__ segMask,
                                                       const uint32_t* __restrict__ segBase, int64_t A,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
                                                       int32_t* __restrict__ tris, uint32_t* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         if (tmc_ambiguous(acode[a] & 0xFFu)) continue;
-        tmc_emit_cell<T, false>(vt, a, acell, acode, avb, atloc, rowBaseT0, segMask, segBase, vpos, vnrm, tris,
-                                unresolved);
+        tmc_emit_cell<T, false>(vt, a, acell, acode, avb, tofs, segMask, segBase, vpos, vnrm, tris, unresolved);
     }
 }
 
-// K_T3: the ambiguous cells k_tmc_rows listed (amb[0] = count, cells from
+// K_T4: the ambiguous cells k_tmc_cells listed (amb[0] = count, cells from
 // amb[1]): sliced with the asymptotic decider.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_tmc_emit_amb(VolTable vt, const uint32_t* __restrict__ amb,
                                                           const uint32_t* __restrict__ acell,
                                                           const uint32_t* __restrict__ acode,
                                                           const uint2* __restrict__ avb,
-                                                          const uint32_t* __restrict__ atloc,
-                                                          const uint32_t* __restrict__ rowBaseT0,
+                                                          const uint32_t* __restrict__ tofs,
                                                           const unsigned long long* __restrict__ segMask,
                                                           const uint32_t* __restrict__ segBase,
                                                           double* __restrict__ vpos, double* __restrict__ vnrm,
@@ -751,8 +764,8 @@ __global__ void __launch_bounds__(kBlock) k_tmc_emit_amb(VolTable vt, const uint
                                                           uint32_t* __restrict__ unresolved) {
     const uint32_t n = amb[0];
     for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
-        tmc_emit_cell<T, true>(vt, (int64_t)amb[1 + s], acell, acode, avb, atloc, rowBaseT0, segMask, segBase, vpos,
-                               vnrm, tris, unresolved);
+        tmc_emit_cell<T, true>(vt, (int64_t)amb[1 + s], acell, acode, avb, tofs, segMask, segBase, vpos, vnrm, tris,
+                               unresolved);
 }
 
 }  // namespace tmc
