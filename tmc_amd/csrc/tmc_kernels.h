@@ -674,6 +674,13 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
     };
     int32_t ev[12];
     int nown = 0;
+    // active index of each owner cell met, by its offset (i - oi) | (j - oj)
+    // << 1 | (k - ok) << 2 (-1: not looked up yet); the cell before in the
+    // row is the previous active cell
+    int64_t oidx[8];
+#pragma unroll
+    for (int x = 0; x < 8; x++) oidx[x] = -1;
+    oidx[1] = a - 1;
     for (int e = 0; e < 12; e++) {
         ev[e] = -1;
         if (!((cut >> e) & 1u)) continue;
@@ -701,7 +708,12 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
         }
         int oi, oj, ok, oe;
         tmc_edge_owner(i, j, k, e, d, oi, oj, ok, oe);
-        const int64_t ao = find_active(L, d, oi, oj, ok);
+        const int off = (i - oi) | (j - oj) << 1 | (k - ok) << 2;
+        int64_t ao = oidx[off];
+        if (ao < 0) {
+            ao = find_active(L, d, oi, oj, ok);
+            oidx[off] = ao;
+        }
         if (ao < 0) {
             atomicAdd(unresolved, 1u);
             continue;
