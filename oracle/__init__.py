@@ -126,6 +126,7 @@ class Oracle:
         nv, nt = C.c_int64(), C.c_int64()
         v, n, t = f64p(), f64p(), i32p()
         st = (C.c_int64 * 2)()
+        grid = _g(grid)
         if self.lib.tmco_tmc(C.byref(grid), _p(s, f64p), float(iso), C.byref(nv), C.byref(v), C.byref(n),
                              C.byref(nt), C.byref(t), st) != 0:
             raise MemoryError("tmco_tmc")
