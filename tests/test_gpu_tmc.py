@@ -8,6 +8,7 @@ import sys
 import numpy as np
 import pytest
 
+from conftest import golden_files, grid_from, load_golden
 from oracle import Grid as OGrid
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +40,18 @@ def _smooth(kind, n, lo, hi):
     if kind == "genus2":
         return 2 * y * (y * y - 3 * x * x) * (1 - z * z) + (x * x + y * y) ** 2 - (9 * z * z - 1) * (1 - z * z)
     return np.cos(x) + np.cos(y) + np.cos(z)
+
+
+@pytest.mark.parametrize("name", golden_files("tmc_"))
+def test_tmc_matches_reference_fixture(engine, name):
+    """Fixtures of the reference's t_mc compiled from mc/MarchingCubes.cpp."""
+    z = load_golden(name)
+    g = grid_from(z["grid"])
+    _say(f"[gpu] tmc fixture {name}")
+    got = engine.tmc(z["field"], float(z["iso"]), g)
+    assert np.array_equal(got["tris"], z["tris"]), name
+    assert np.array_equal(got["v"].view(np.uint64), z["v"].view(np.uint64)), name
+    assert np.array_equal(got["n"].view(np.uint64), z["n"].view(np.uint64)), name
 
 
 CASES = [

@@ -70,3 +70,17 @@ def test_tmc_oracle_closed_surface_invariants():
     assert np.all(count == 2)
     nv, ne, nf = len(got["v"]), len(count), len(t)
     assert nv - ne + nf == -2
+
+
+def test_tmc_oracle_matches_fixtures():
+    """The committed fixtures of the compiled reference
+    (tests/golden/make_tmc_golden.py), checked without the reference."""
+    from conftest import golden_files, grid_from, load_golden
+    names = golden_files("tmc_")
+    assert len(names) >= 4
+    for name in names:
+        z = load_golden(name)
+        got = Oracle().tmc(grid_from(z["grid"]), z["field"], float(z["iso"]))
+        assert np.array_equal(got["tris"], z["tris"]), name
+        assert np.array_equal(got["v"].view(np.uint64), z["v"].view(np.uint64)), name
+        assert np.array_equal(got["n"].view(np.uint64), z["n"].view(np.uint64)), name
