@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
 # scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024_traffic.md)
-TRAFFIC = {"schwarz_1024": 5737299838}
+TRAFFIC = {"schwarz_1024": 5737975335}
 
 
 def log(msg):
@@ -439,7 +439,7 @@ def main():
     if not args.no_tmc:
         log("[bench] topologically correct MC leg")
         out["tmc"] = bench_tmc(eng, buf, field, grid, iso, args.steps, 1,
-                               None if args.no_cpu_baseline else min(320, args.cpu_block))
+                               None if args.no_cpu_baseline else args.cpu_block)
     buf.free()
     eng.close()
     if not args.no_batch:
