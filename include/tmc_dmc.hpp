@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -706,3 +707,183 @@ private:
 };
 
 }  // namespace p_mc
+
+// ---------------------------------------------------------------------------
+// tmc::MarchingCubes -- the reference's topologically correct Marching Cubes
+// (mc/MarchingCubes.h:113-420, mc/MarchingCubes.cpp).  operator() reads the
+// volume file, extracts (tmc_extract: t_mc + p_slice, bit-identical to the
+// reference), optionally orients the triangles (connectivity(), c_flag) and
+// writes the OBJ / OFF files in the reference's formats.  The mesh is also
+// available through points(), normals() and triangles().
+// ---------------------------------------------------------------------------
+namespace tmc {
+
+using Error = std::runtime_error;
+
+namespace detail {
+
+// connectivity() of mc/MarchingCubes.cpp:1769-2226, the part that changes
+// the mesh: each triangle's neighbour across its edges (v1,v2), (v2,v0),
+// (v0,v1) is the first other triangle holding both vertices in the vertex's
+// triangle list (triangle order); then, triangle by triangle, a depth-first
+// walk over unvisited neighbours flips (v[1] <-> v[2], with its neighbours
+// across edges 1 and 2) every neighbour that does not run the shared edge
+// the other way -- before the neighbour itself is visited, so a triangle can
+// be flipped more than once.  Returns the number of flips.
+inline long orient_triangles(int64_t nv, std::vector<std::array<int, 3>>& T) {
+    const int64_t nt = (int64_t)T.size();
+    std::vector<std::vector<int>> v_tris((size_t)nv);
+    for (int64_t t = 0; t < nt; t++)
+        for (int x = 0; x < 3; x++) v_tris[(size_t)T[t][x]].push_back((int)t);
+    auto has_edge = [](int a, int b, const std::array<int, 3>& q) {
+        return (a == q[1] && b == q[2]) || (a == q[2] && b == q[1]) || (a == q[0] && b == q[2]) ||
+               (a == q[2] && b == q[0]) || (a == q[1] && b == q[0]) || (a == q[0] && b == q[1]);
+    };
+    std::vector<std::array<int, 3>> nb((size_t)nt);
+    for (int64_t t = 0; t < nt; t++) {
+        const int v0 = T[t][0], v1 = T[t][1], v2 = T[t][2];
+        nb[t] = {-1, -1, -1};
+        const int e[3][2] = {{v1, v2}, {v2, v0}, {v0, v1}};
+        for (int x = 0; x < 3; x++)
+            for (int tt : v_tris[(size_t)e[x][0]])
+                if (tt != t && has_edge(e[x][0], e[x][1], T[tt])) {
+                    nb[t][x] = tt;
+                    break;
+                }
+    }
+    std::vector<char> oriented((size_t)nt, 0);
+    std::vector<int> stack;
+    long flips = 0;
+    // the neighbour must run the shared edge a -> b as b -> a; else flip it
+    auto match = [&](int a, int b, int id) {
+        std::array<int, 3>& q = T[id];
+        const int t0 = q[0], t1 = q[1], t2 = q[2];
+        if ((a == t2 && b == t1) || (a == t0 && b == t2) || (a == t1 && b == t0)) return;
+        flips++;
+        q[1] = t2;
+        q[2] = t1;
+        std::swap(nb[id][1], nb[id][2]);
+    };
+    auto visit = [&](int c) {
+        const std::array<int, 3> q = T[c];
+        const std::array<int, 3> n = nb[c];
+        const int e[3][2] = {{q[1], q[2]}, {q[2], q[0]}, {q[0], q[1]}};
+        for (int x = 0; x < 3; x++)
+            if (n[x] != -1 && !oriented[n[x]]) {
+                match(e[x][0], e[x][1], n[x]);
+                stack.push_back(n[x]);
+            }
+    };
+    for (int64_t t = 0; t < nt; t++) {
+        if (oriented[t]) continue;
+        oriented[t] = 1;
+        visit((int)t);
+        while (!stack.empty()) {
+            const int c = stack.back();
+            stack.pop_back();
+            if (oriented[c]) continue;
+            oriented[c] = 1;
+            visit(c);
+        }
+    }
+    return flips;
+}
+
+}  // namespace detail
+
+class MarchingCubes {
+public:
+    using Point = std::array<double, 3>;
+    using Normal = std::array<double, 3>;
+    using Triangle = std::array<int, 3>;
+
+    explicit MarchingCubes(int device = 0) : engine_(device) {}
+
+    // mc/MarchingCubes.cpp:11-147.  The file holds 3 ushort dims, 3 float
+    // spacings and the ushort values, x fastest; the grid's origin is 0 and
+    // its spacing (d * (n - 1.)) / (n - 1), as the reference's bounding box
+    // gives it.
+    void operator()(const double i0, const std::string& i_file, const bool c_flag, const bool obj_flag,
+                    const std::string& o_objF, const bool off_flag, const std::string& o_offF) {
+        std::ifstream f(i_file, std::ios::binary);
+        if (!f) throw Error("can't open input file " + i_file);
+        uint16_t dims[3];
+        float sp[3];
+        f.read(reinterpret_cast<char*>(dims), sizeof(dims));
+        f.read(reinterpret_cast<char*>(sp), sizeof(sp));
+        const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+        std::vector<uint16_t> raw(n);
+        f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)(n * sizeof(uint16_t)));
+        if (!f) throw Error("short volume file " + i_file);
+        std::vector<float> field(raw.begin(), raw.end());
+        dmc_grid g{};
+        g.nx = dims[0];
+        g.ny = dims[1];
+        g.nz = dims[2];
+        const int nn[3] = {g.nx, g.ny, g.nz};
+        double d[3];
+        for (int x = 0; x < 3; x++) d[x] = ((double)sp[x] * (nn[x] - 1.)) / (nn[x] - 1);
+        g.dx = d[0];
+        g.dy = d[1];
+        g.dz = d[2];
+        t_mc(g, field.data(), i0);
+        if (c_flag) connectivity();
+        if (obj_flag) write_obj(o_objF);
+        if (off_flag) write_off(o_offF);
+    }
+
+    // t_mc (mc/MarchingCubes.cpp:150-318) on a float field of g's shape
+    void t_mc(const dmc_grid& g, const float* field, double i0) {
+        tmc_mesh m{};
+        tmc_detail::check(engine_.get(), tmc_extract(engine_.get(), &g, field, DMC_FIELD_F32, i0, 0, &m),
+                          "tmc_extract");
+        points_.resize((size_t)m.nv);
+        normals_.resize((size_t)m.nv);
+        triangles_.resize((size_t)m.nt);
+        tmc_detail::check(engine_.get(),
+                          tmc_download(engine_.get(), points_.empty() ? nullptr : points_[0].data(),
+                                       normals_.empty() ? nullptr : normals_[0].data(),
+                                       triangles_.empty() ? nullptr : triangles_[0].data()),
+                          "tmc_download");
+    }
+
+    // mc/MarchingCubes.cpp:1769-2226: consistent triangle orientation
+    void connectivity() { detail::orient_triangles((int64_t)points_.size(), triangles_); }
+
+    const std::vector<Point>& points() const { return points_; }
+    const std::vector<Normal>& normals() const { return normals_; }
+    const std::vector<Triangle>& triangles() const { return triangles_; }
+
+    // mc/MarchingCubes.cpp:95-118 (the face line repeats the second normal
+    // index for the third vertex, as there)
+    void write_obj(const std::string& path) const {
+        std::ofstream o(path);
+        if (!o) throw Error("can't open output file " + path);
+        o << "# Topologically correct and manifold isosurface\n";
+        for (const Point& p : points_)
+            o << "v " << std::setprecision(7) << std::fixed << p[0] << " " << p[1] << " " << p[2] << std::endl;
+        for (const Normal& q : normals_)
+            o << "vn " << std::setprecision(7) << std::fixed << q[0] << " " << q[1] << " " << q[2] << std::endl;
+        for (const Triangle& t : triangles_)
+            o << "f " << (t[0] + 1) << "//" << (t[0] + 1) << " " << (t[1] + 1) << "//" << (t[1] + 1) << " "
+              << (t[2] + 1) << "//" << (t[1] + 1) << std::endl;
+    }
+
+    // mc/MarchingCubes.cpp:122-143
+    void write_off(const std::string& path) const {
+        std::ofstream o(path);
+        if (!o) throw Error("can't open output file " + path);
+        o << "OFF\n";
+        o << points_.size() << " " << triangles_.size() << " " << 0 << std::endl;
+        for (const Point& p : points_) o << p[0] << " " << p[1] << " " << p[2] << std::endl;
+        for (const Triangle& t : triangles_) o << "3 " << t[0] << " " << t[1] << " " << t[2] << std::endl;
+    }
+
+private:
+    tmc_detail::Engine engine_;
+    std::vector<Point> points_;
+    std::vector<Normal> normals_;
+    std::vector<Triangle> triangles_;
+};
+
+}  // namespace tmc

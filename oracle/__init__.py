@@ -502,14 +502,17 @@ class TmcReference:
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         L = self.lib = C.CDLL(path)
-        L.tmcref_run.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [f64p, C.c_double, i64p, i64p]
+        L.tmcref_run.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [f64p, C.c_double, C.c_int, i64p,
+                                                                                    i64p]
         L.tmcref_fetch.argtypes = [f64p, f64p, i32p]
 
-    def tmc(self, grid, scalars, iso):
+    def tmc(self, grid, scalars, iso, orient=False):
+        """t_mc; with orient=True followed by connectivity() (operator()'s
+        c_flag)."""
         s = np.ascontiguousarray(scalars, dtype=np.float64).reshape(-1)
         nv, nt = C.c_int64(), C.c_int64()
         self.lib.tmcref_run(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz,
-                            _p(s, f64p), float(iso), C.byref(nv), C.byref(nt))
+                            _p(s, f64p), float(iso), int(orient), C.byref(nv), C.byref(nt))
         v = np.zeros((nv.value, 3), np.float64)
         n = np.zeros((nv.value, 3), np.float64)
         t = np.zeros((nt.value, 3), np.int32)

@@ -28,10 +28,11 @@ tmc::MarchingCubes* g_mc = nullptr;
 extern "C" {
 
 // t_mc on an nx*ny*nz field of doubles (x fastest), point (i, j, k) at
-// (x0 + i dx, ...); gradients from UniformGrid::gradient.  Returns the
-// vertex and triangle counts; tmcref_fetch copies the lists.
+// (x0 + i dx, ...); gradients from UniformGrid::gradient; with c_flag then
+// connectivity() (the triangle orientation pass).  Returns the vertex and
+// triangle counts; tmcref_fetch copies the lists.
 int tmcref_run(int nx, int ny, int nz, double x0, double y0, double z0, double dx, double dy, double dz,
-               const double* field, double iso, int64_t* nv, int64_t* nt) {
+               const double* field, double iso, int c_flag, int64_t* nv, int64_t* nt) {
     delete g_mc;
     g_mc = new tmc::MarchingCubes();
     auto& g = g_mc->m_ugrid;
@@ -49,6 +50,11 @@ int tmcref_run(int nx, int ny, int nz, double x0, double y0, double z0, double d
         Mute m;
         g.gradient();
         g_mc->t_mc(iso);
+        if (c_flag) {
+            std::streambuf* err = std::cerr.rdbuf(m.sink.rdbuf());
+            g_mc->connectivity();   // operator()'s orientation pass (c_flag)
+            std::cerr.rdbuf(err);
+        }
     }
     *nv = (int64_t)g_mc->m_points.size();
     *nt = (int64_t)g_mc->m_triangles.size();

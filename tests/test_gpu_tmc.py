@@ -127,3 +127,80 @@ def test_tmc_then_dmc_on_one_context(engine, oracle):
     assert m.nv == len(got["v"])
     for k in ("v", "quads", "tris"):
         assert np.array_equal(got[k], again[k])
+
+
+def _ushort_volume(path, f, spacing):
+    with open(path, "wb") as fh:
+        np.array([f.shape[2], f.shape[1], f.shape[0]], np.uint16).tofile(fh)
+        np.array(spacing, np.float32).tofile(fh)
+        np.ascontiguousarray(f, np.uint16).tofile(fh)
+
+
+def _f7(x):
+    """std::fixed with precision 7 as the C library writes it (a NaN with its
+    sign bit set is "-nan")."""
+    x = float(x)
+    if x != x:
+        return "-nan" if np.signbit(x) else "nan"
+    return "%.7f" % x
+
+
+def _obj_text(v, n, t):
+    out = ["# Topologically correct and manifold isosurface\n"]
+    out += ["v %s %s %s\n" % (_f7(p[0]), _f7(p[1]), _f7(p[2])) for p in v]
+    out += ["vn %s %s %s\n" % (_f7(q[0]), _f7(q[1]), _f7(q[2])) for q in n]
+    out += ["f %d//%d %d//%d %d//%d\n" % (a + 1, a + 1, b + 1, b + 1, c + 1, b + 1) for a, b, c in t]
+    return "".join(out)
+
+
+def _off_text(v, t):
+    from oracle.diagnostics import fmt_g
+    out = ["OFF\n", "%d %d 0\n" % (len(v), len(t))]
+    out += ["%s %s %s\n" % (fmt_g(p[0]), fmt_g(p[1]), fmt_g(p[2])) for p in v]
+    out += ["3 %d %d %d\n" % (a, b, c) for a, b, c in t]
+    return "".join(out)
+
+
+def test_tmc_cpp_dropin_driver(tmp_path):
+    """mc/tmc.cpp's driver against include/tmc_dmc.hpp: tmc::MarchingCubes
+    reads a ushort volume file, extracts, orients (c_flag) and writes OBJ and
+    OFF text identical to what the reference's t_mc + connectivity() mesh
+    gives in the reference's formats (checked against the compiled reference
+    where it is built, else against the oracle without orientation)."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    from oracle import TMC_REF_SO, TmcReference
+    exe = str(tmp_path / "tmc_dropin")
+    libdir = os.path.join(ROOT, "tmc_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "tmc_dropin_main.cpp"), "-L" + libdir, "-ltmc_dmc",
+                        "-Wl,-rpath," + libdir, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    z, y, x = np.meshgrid(np.linspace(-2, 2, 22), np.linspace(-2, 2, 25), np.linspace(-2, 2, 28), indexing="ij")
+    g2 = 2 * y * (y * y - 3 * x * x) * (1 - z * z) + (x * x + y * y) ** 2 - (9 * z * z - 1) * (1 - z * z)
+    smooth = np.clip((g2 + 4) * 3000, 0, 65535).astype(np.uint16)
+    rng = np.random.default_rng(43)
+    sh = tuple(int(v) for v in rng.integers(4, 12, 3))
+    ints = rng.integers(0, 3, sh).astype(np.uint16)
+    have_ref = os.path.exists(TMC_REF_SO)
+    for name, f, iso, sp, cflag in (("smooth", smooth, 12000.5, (0.1, 0.2, 0.3), 1), ("ints", ints, 1.0, (1, 1, 1), 1),
+                                    ("ints_plain", ints, 1.0, (0.5, 0.5, 0.5), 0)):
+        vol = str(tmp_path / (name + ".bin"))
+        _ushort_volume(vol, f, sp)
+        obj, off = str(tmp_path / (name + ".obj")), str(tmp_path / (name + ".off"))
+        _say(f"[gpu] tmc drop-in {name} c_flag={cflag}")
+        r = subprocess.run([exe, vol, repr(iso), str(cflag), obj, off], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = [float(s_) * (n - 1.0) / (n - 1) for s_, n in zip(np.float32(sp).astype(np.float64),
+                                                               (f.shape[2], f.shape[1], f.shape[0]))]
+        g = OGrid.make(f.shape, (0.0, 0.0, 0.0), d)
+        if have_ref:
+            exp = TmcReference().tmc(g, f.astype(np.float64), iso, orient=bool(cflag))
+        else:
+            if cflag:
+                continue
+            from oracle import Oracle
+            exp = Oracle().tmc(g, f.astype(np.float64), iso)
+        assert open(obj).read() == _obj_text(exp["v"], exp["n"], exp["tris"]), name
+        assert open(off).read() == _off_text(exp["v"], exp["tris"]), name
