@@ -71,8 +71,9 @@ struct dmc_context {
         newV, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
-    // topologically correct MC: vertex and triangle row offsets, stats, mesh
-    Buf tloc, rowT, tstats, tpos, tnrm, ttris, tamb, tvb;
+    // topologically correct MC: per active cell vertex / triangle counts and
+    // offsets, the ambiguous-cell list, owner words; stats; the mesh
+    Buf tcntv, tcntt, tvofs, ttofs, tamb, tvb, tstats, tpos, tnrm, ttris;
     tmc_mesh tmesh{};
     bool have_tmesh = false;
     uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
@@ -105,7 +106,8 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
-                      &dhist, &dedges, &dqual, &tloc, &rowT, &tstats, &tpos, &tnrm, &ttris, &tamb, &tvb};
+                      &dhist, &dedges, &dqual, &tcntv, &tcntt, &tvofs, &ttofs, &tamb, &tvb, &tstats, &tpos, &tnrm,
+                      &ttris};
         for (Buf* b : all) bufs.push_back(b);
     }
 };
@@ -928,10 +930,10 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const int64_t R = sw.R;
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
-    uint32_t* cnt_v = ensure<uint32_t>(ctx, ctx->avloc, A);
-    uint32_t* cnt_t = ensure<uint32_t>(ctx, ctx->tloc, A);
-    uint32_t* vofs = ensure<uint32_t>(ctx, ctx->rowV, A);
-    uint32_t* tofs = ensure<uint32_t>(ctx, ctx->rowT, A);
+    uint32_t* cnt_v = ensure<uint32_t>(ctx, ctx->tcntv, A);
+    uint32_t* cnt_t = ensure<uint32_t>(ctx, ctx->tcntt, A);
+    uint32_t* vofs = ensure<uint32_t>(ctx, ctx->tvofs, A);
+    uint32_t* tofs = ensure<uint32_t>(ctx, ctx->ttofs, A);
     unsigned long long* st = ensure<unsigned long long>(ctx, ctx->tstats, 3);
     uint32_t* amb = ensure<uint32_t>(ctx, ctx->tamb, A + 1);
     HIPC(ctx, hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), ctx->stream));
@@ -967,12 +969,10 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
                            (int64_t)A, avb);
         check_launch(ctx, "k_tmc_vbase");
         hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avb,
-                           tofs, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris,
-                           reinterpret_cast<uint32_t*>(st + 2));
+                           tofs, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris, st + 2);
         check_launch(ctx, "k_tmc_emit");
         hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
-                           amb, acell, acode, avb, tofs, sw.segMask, sw.segBase, pos, nrm, tris,
-                           reinterpret_cast<uint32_t*>(st + 2));
+                           amb, acell, acode, avb, tofs, sw.segMask, sw.segBase, pos, nrm, tris, st + 2);
         check_launch(ctx, "k_tmc_emit_amb");
     }
     mark(ctx, "end");

@@ -648,7 +648,7 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
                                               const unsigned long long* __restrict__ segMask,
                                               const uint32_t* __restrict__ segBase, double* __restrict__ vpos,
                                               double* __restrict__ vnrm, int32_t* __restrict__ tris,
-                                              uint32_t* __restrict__ unresolved) {
+                                              unsigned long long* __restrict__ unresolved) {
     const int vol = vol_of(vt.aBase, vt.n, a);
     const VolDesc& V = vt.desc[vol];
     const Dims d = V.d;
@@ -715,7 +715,7 @@ __device__ __forceinline__ void tmc_emit_cell(const VolTable& vt, int64_t a, con
             oidx[off] = ao;
         }
         if (ao < 0) {
-            atomicAdd(unresolved, 1u);
+            atomicAdd(unresolved, 1ull);
             continue;
         }
         const uint2 ob = avb[ao];
@@ -754,7 +754,8 @@ __global__ void __launch_bounds__(kBlock) k_tmc_emit(VolTable vt, const uint32_t
                                                       const unsigned long long* __restrict__ segMask,
                                                       const uint32_t* __restrict__ segBase, int64_t A,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
-                                                      int32_t* __restrict__ tris, uint32_t* __restrict__ unresolved) {
+                                                      int32_t* __restrict__ tris,
+                                                      unsigned long long* __restrict__ unresolved) {
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         if (tmc_ambiguous(acode[a] & 0xFFu)) continue;
         tmc_emit_cell<T, false>(vt, a, acell, acode, avb, tofs, segMask, segBase, vpos, vnrm, tris, unresolved);
@@ -773,7 +774,7 @@ __global__ void __launch_bounds__(kBlock) k_tmc_emit_amb(VolTable vt, const uint
                                                           const uint32_t* __restrict__ segBase,
                                                           double* __restrict__ vpos, double* __restrict__ vnrm,
                                                           int32_t* __restrict__ tris,
-                                                          uint32_t* __restrict__ unresolved) {
+                                                          unsigned long long* __restrict__ unresolved) {
     const uint32_t n = amb[0];
     for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
         tmc_emit_cell<T, true>(vt, (int64_t)amb[1 + s], acell, acode, avb, tofs, segMask, segBase, vpos, vnrm, tris,
