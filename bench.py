@@ -256,8 +256,12 @@ def bench_tmc(eng, buf, field, grid, iso, steps, warmup, cpu_block=None):
             stage[k] = stage.get(k, 0.0) + v
     eng.synchronize()
     ms = (time.perf_counter() - t) / steps * 1e3
+    # algorithmic bytes: the field read once (4 B per point) and the mesh
+    # written once (48 B per vertex: FP64 position and normal; 12 B per triangle)
+    alg = 4 * field.size + 48 * r["nv"] + 12 * r["nt"]
     out = {"algorithm": "topologically correct MC (mc/MarchingCubes.cpp t_mc)", "steps": steps,
            "ms_per_step": round(ms, 4), "gvoxels_per_s": round(field.size / (ms * 1e-3) / 1e9, 3),
+           "algorithmic_bytes": alg, "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "mtris_per_s": round(r["nt"] / (ms * 1e-3) / 1e6, 2),
            "stage_ms": {k: round(v / steps, 4) for k, v in stage.items()},
            "V": r["nv"], "T": r["nt"], "active_cells": r["active_cells"], "ambiguous_cells": r["ambiguous_cells"],
