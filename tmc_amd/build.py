@@ -56,14 +56,18 @@ def build_workloads():
 
 
 def build_oracle(with_reference=None):
-    """liboracle.so always; oracle/_ref when the reference sources exist."""
+    """liboracle.so always; oracle/_ref's libraries where the reference
+    sources exist (cpp_dmc for libdmcref.so, mc for libtmcref.so)."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "liboracle.so"])
     ref = os.environ.get("TMC_REFERENCE_DIR", "/root/reference/cpp_dmc")
-    if with_reference is None:
-        with_reference = os.path.isdir(ref)
-    if with_reference:
-        _run(["make", "-s", "-C", odir, "ref", f"REF={ref}"])
+    ref_mc = os.environ.get("TMC_REFERENCE_MC_DIR", "/root/reference/mc")
+    if with_reference is False:
+        return
+    if with_reference or os.path.isdir(ref):
+        _run(["make", "-s", "-C", odir, "_ref/libdmcref.so", f"REF={ref}"])
+    if os.path.isdir(ref_mc):
+        _run(["make", "-s", "-C", odir, "_ref/libtmcref.so", f"REF_MC={ref_mc}"])
 
 
 def build_all(oracle=True):
