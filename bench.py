@@ -105,6 +105,13 @@ def bench_batch(count=4096, seed=4, steps=3, plain_sample=8):
     dv = [(p, iso, g) for p, (f, g, iso) in zip(ptrs, vols)]
     flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES | tmc_amd.FIELD_ON_DEVICE
     eng.extract_batch(dv, flags=flags)
+    # launch shape picked by the autotuner on this batch (untimed)
+    t = time.perf_counter()
+    tune = eng.autotune(dv, flags=flags, reps=2)
+    tune["tune_s"] = round(time.perf_counter() - t, 2)
+    tune["best"]["ms"] = round(tune["best"]["ms"], 3)
+    tune["trials"] = [dict(x, ms=round(x["ms"], 3)) for x in tune["trials"]]
+    log(f"[batch] autotune: {tune['best']} in {tune['tune_s']} s")
     t = time.perf_counter()
     for _ in range(steps):
         ms = eng.extract_batch(dv, flags=flags)
@@ -130,7 +137,8 @@ def bench_batch(count=4096, seed=4, steps=3, plain_sample=8):
                         "gvoxels_per_s": round(nvox / batched_s / 1e9, 3)},
             "plain": {"volumes_per_s": round(plain_vps, 1), "sampled": len(sample),
                       "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3)},
-            "speedup": round(count / batched_s / plain_vps, 2)}
+            "speedup": round(count / batched_s / plain_vps, 2),
+            "launch": dict(tune, note="both legs run with the tuned shape; results do not depend on it")}
 
 
 def serve_pool(seed=5):

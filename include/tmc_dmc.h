@@ -272,6 +272,47 @@ dmc_status tmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field
  * positions/normals 3*nv doubles, triangles 3*nt ints. */
 dmc_status tmc_download(dmc_context* ctx, double* positions, double* normals, int32_t* triangles);
 
+/* ------------------------------------------------------------------------
+ * Launch shape and its autotuner (no counterpart in the reference, whose
+ * kernels launch with fixed block sizes over one thread per item,
+ * dmc/DualMarchingCubes.cu).  Results never depend on the launch shape,
+ * only the time does.
+ *   grid_cap     blocks (256 threads) of every grid-stride kernel, which
+ *                loops over its items: 64 .. 2^24, default 16384 (env
+ *                TMC_GRID_CAP);
+ *   sweep_waves  waves the field sweep aims for, one 256-cell x 16-row tile
+ *                of K planes each; sets K (never below 8): 256 .. 2^24,
+ *                default 32768 (env TMC_SWEEP_WAVES).
+ * There is no hash-table load factor to choose: vertex and quadrilateral
+ * ids come from prefix sums over the cells, not from hash tables.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int32_t grid_cap;
+    int32_t sweep_waves;
+} dmc_launch;
+
+dmc_status dmc_get_launch(dmc_context* ctx, dmc_launch* out);
+/* DMC_ERR_INVALID_ARG for a value outside its range. */
+dmc_status dmc_set_launch(dmc_context* ctx, const dmc_launch* launch);
+
+typedef struct {
+    dmc_launch launch;
+    double ms;          /* median wall time of one dmc_extract_batch */
+} dmc_tune_trial;
+
+/* Time dmc_extract_batch(vols, nvol, field_type, flags) on ctx under
+ * candidate launch shapes and keep the fastest on ctx: sweep_waves over
+ * {8192, 16384, 32768, 65536} at the current grid_cap, then grid_cap over
+ * {4096, 8192, 16384, 32768, 65536} at the best sweep_waves.  Each
+ * candidate runs once untimed, then `reps` (1..64) times; its figure is the
+ * median.  trials (optional) receives every candidate, *ntrials is its
+ * capacity in and the number written out; *best (optional) the chosen
+ * shape.  On an extraction error the previous shape is restored.  The
+ * context's last mesh is that of a tuning run (identical under every
+ * shape). */
+dmc_status dmc_autotune(dmc_context* ctx, const dmc_volume* vols, int32_t nvol, int field_type, uint32_t flags,
+                        int32_t reps, dmc_tune_trial* trials, int32_t* ntrials, dmc_tune_trial* best);
+
 /* Per-stage device times (ms) of the last extraction run with
  * DMC_STAGE_TIMING.  names[i] point to static strings.  *count is in/out:
  * capacity in, number of stages out. */

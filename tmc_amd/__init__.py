@@ -43,7 +43,7 @@ EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc
            "dmc_server_last_error", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc",
            "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
            "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj",
-           "tmc_extract", "tmc_download"]
+           "tmc_extract", "tmc_download", "dmc_get_launch", "dmc_set_launch", "dmc_autotune"]
 
 
 class DMCError(RuntimeError):
@@ -88,6 +88,14 @@ class Mesh(C.Structure):
 
     def summary(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("d_")}
+
+
+class Launch(C.Structure):
+    _fields_ = [("grid_cap", C.c_int32), ("sweep_waves", C.c_int32)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("launch", Launch), ("ms", C.c_double)]
 
 
 class ServerOpts(C.Structure):
@@ -144,6 +152,10 @@ def load_library(path=ENGINE_SO):
     L.dmc_write_obj.argtypes = [vp, C.c_char_p, C.c_int]
     L.tmc_extract.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, C.POINTER(TmcMesh)]
     L.tmc_download.argtypes = [vp, f64p, f64p, i32p]
+    L.dmc_get_launch.argtypes = [vp, C.POINTER(Launch)]
+    L.dmc_set_launch.argtypes = [vp, C.POINTER(Launch)]
+    L.dmc_autotune.argtypes = [vp, C.POINTER(Volume), C.c_int32, C.c_int, C.c_uint32, C.c_int32,
+                               C.POINTER(TuneTrial), C.POINTER(C.c_int32), C.POINTER(TuneTrial)]
     L.dmc_status_str.argtypes = [C.c_int]
     L.dmc_status_str.restype = C.c_char_p
     L.dmc_last_error.argtypes = [vp]
@@ -321,11 +333,41 @@ class Engine:
             out.update(v=v, n=n, tris=t)
         return out
 
-    def extract_batch(self, volumes, flags=SIMPLIFY_ALL | TRIANGLES, dtype=np.float32):
-        """Extract many volumes in one pass.  `volumes` is a list of
-        (field, iso, grid) with numpy fields (nz, ny, nx) -- or
-        (device_pointer, iso, grid) with FIELD_ON_DEVICE in `flags`.  All
-        fields share one element type.  Returns one Mesh per volume."""
+    @property
+    def launch(self):
+        """Launch shape of this context: {"grid_cap", "sweep_waves"}."""
+        l = Launch()
+        self._check(self.lib.dmc_get_launch(self.ctx, C.byref(l)))
+        return {"grid_cap": l.grid_cap, "sweep_waves": l.sweep_waves}
+
+    @launch.setter
+    def launch(self, value):
+        cur = self.launch
+        cur.update(value)
+        l = Launch(int(cur["grid_cap"]), int(cur["sweep_waves"]))
+        self._check(self.lib.dmc_set_launch(self.ctx, C.byref(l)))
+
+    def autotune(self, volumes, flags=SIMPLIFY_ALL | TRIANGLES, dtype=np.float32, reps=3):
+        """Time extract_batch(volumes) under candidate launch shapes and keep
+        the fastest on this context.  Returns {"best": {...}, "trials": [...]}
+        with each shape and its median ms."""
+        vols, ftype, keep = self._volumes(volumes, flags, dtype)
+        cap = 16
+        trials = (TuneTrial * cap)()
+        nt = C.c_int32(cap)
+        best = TuneTrial()
+        self._check(self.lib.dmc_autotune(self.ctx, vols, len(volumes), ftype, int(flags), int(reps), trials,
+                                          C.byref(nt), C.byref(best)))
+        del keep
+        self.mesh = None
+        self.batch = None
+
+        def rec(t):
+            return {"grid_cap": t.launch.grid_cap, "sweep_waves": t.launch.sweep_waves, "ms": t.ms}
+
+        return {"best": rec(best), "trials": [rec(trials[i]) for i in range(nt.value)]}
+
+    def _volumes(self, volumes, flags, dtype):
         n = len(volumes)
         vols = (Volume * n)()
         keep = []
@@ -347,6 +389,15 @@ class Engine:
             vols[i].field = ptr
             vols[i].gradients = None
             vols[i].iso = float(iso)
+        return vols, ftype, keep
+
+    def extract_batch(self, volumes, flags=SIMPLIFY_ALL | TRIANGLES, dtype=np.float32):
+        """Extract many volumes in one pass.  `volumes` is a list of
+        (field, iso, grid) with numpy fields (nz, ny, nx) -- or
+        (device_pointer, iso, grid) with FIELD_ON_DEVICE in `flags`.  All
+        fields share one element type.  Returns one Mesh per volume."""
+        n = len(volumes)
+        vols, ftype, keep = self._volumes(volumes, flags, dtype)
         meshes = (Mesh * n)()
         self._check(self.lib.dmc_extract_batch(self.ctx, vols, n, ftype, int(flags), meshes))
         del keep

@@ -36,7 +36,7 @@ def _run(cmd, cwd=ROOT):
     return out
 
 
-ENGINE_SOURCES = ["dmc_engine.hip", "dmc_server.hip"]
+ENGINE_SOURCES = ["dmc_engine.hip", "dmc_server.hip", "dmc_tune.hip"]
 
 
 def build_engine():

@@ -31,12 +31,16 @@ struct Buf {
     size_t cap = 0;
 };
 
-// Grid-stride kernels run on at most this many blocks (persistent style):
-// enough to fill every CU several times over, few enough that waves live
-// long and keep many loads in flight.
-int kGridCap = 16384;
-// field sweep: waves to aim for (sets the planes per wave)
-int kSweepWaves = 32768;
+// Default launch shape of a new context (env TMC_GRID_CAP /
+// TMC_SWEEP_WAVES override it; dmc_set_launch and dmc_autotune change it per
+// context).  Grid-stride kernels run on at most kGridCap blocks (persistent
+// style): enough to fill every CU several times over, few enough that waves
+// live long and keep many loads in flight.  The field sweep aims for
+// kSweepWaves waves, which sets its planes per wave.
+constexpr int kGridCap = 16384;
+constexpr int kSweepWaves = 32768;
+constexpr int64_t kGridCapMin = 64, kGridCapMax = 1 << 24;
+constexpr int64_t kSweepWavesMin = 256, kSweepWavesMax = 1 << 24;
 
 struct StageMark {
     const char* name;
@@ -48,6 +52,9 @@ struct StageMark {
 struct dmc_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // launch shape (results do not depend on it)
+    int grid_cap = kGridCap;
+    int sweep_waves = kSweepWaves;
     std::string err;
     std::vector<Buf*> bufs;
     // cell sweep
@@ -156,7 +163,8 @@ T* ensure(dmc_context* ctx, Buf& b, size_t n) {
     return static_cast<T*>(b.p);
 }
 
-int grid_for(int64_t n, int per_block = kBlock, int64_t cap = kGridCap) {
+int grid_for(const dmc_context* ctx, int64_t n, int per_block = kBlock, int64_t cap = 0) {
+    if (cap <= 0) cap = ctx->grid_cap;
     int64_t g = (n + per_block - 1) / per_block;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
@@ -351,7 +359,7 @@ Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const int n = (int)hv.size();
     ctx->nvol = n;
     // sweep tiling: J rows per wave; one plane count K for the whole
-    // extraction, chosen so that about kSweepWaves waves fill the chip
+    // extraction, chosen so that about ctx->sweep_waves waves fill the chip
     std::vector<VolDesc> desc(n);
     int64_t units = 0;
     for (int v = 0; v < n; v++) {
@@ -367,7 +375,7 @@ Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
         D.ntj = (g.ny - 1 + kSweepRows - 1) / kSweepRows;
         units += (int64_t)(g.nz - 1) * D.S * D.ntj;
     }
-    const int Kg = (int)std::max<int64_t>(8, (units + kSweepWaves - 1) / kSweepWaves);
+    const int Kg = (int)std::max<int64_t>(8, (units + ctx->sweep_waves - 1) / ctx->sweep_waves);
     const size_t tab_bytes = (size_t)n * sizeof(VolDesc) + 3 * (size_t)(n + 1) * sizeof(int64_t);
     ctx->vtab_host.assign(tab_bytes, 0);
     int64_t* waveBase = reinterpret_cast<int64_t*>(ctx->vtab_host.data() + (size_t)n * sizeof(VolDesc));
@@ -414,7 +422,7 @@ Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const uint64_t A = scan(ctx, segCount, segBase, RS);
     if (A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)A);
     ctx->mesh.active_cells = (int64_t)A;
-    hipLaunchKernelGGL(k_vol_abase, dim3(grid_for(n + 1)), dim3(kBlock), 0, ctx->stream, vt, segBase, (int64_t)A,
+    hipLaunchKernelGGL(k_vol_abase, dim3(grid_for(ctx, n + 1)), dim3(kBlock), 0, ctx->stream, vt, segBase, (int64_t)A,
                        vol_ptr(ctx, 2));
     return Swept{A, R, vt, vstat, lists, segCount, segBase, segMask};
 }
@@ -442,7 +450,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     // with an unresolved neighbour cell
     uint32_t* fails = ensure<uint32_t>(ctx, ctx->fails, 6);
     HIPC(ctx, hipMemsetAsync(fails, 0, 6 * sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, vt, R, lists,
+    hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, vt, R, lists,
                        segCount, segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, vstat);
     check_launch(ctx, "k_rows");
     uint64_t* tot = totals(ctx);
@@ -460,7 +468,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     // [0, n]: live bounds; [n+1, 2n+1]: the cell stage's (before simplification)
     int64_t* vb = ensure<int64_t>(ctx, ctx->vbound, 2 * (size_t)(n + 1));
     int64_t* qb = ensure<int64_t>(ctx, ctx->qbound, 2 * (size_t)(n + 1));
-    hipLaunchKernelGGL(k_vol_bounds, dim3(grid_for(n + 1)), dim3(kBlock), 0, ctx->stream, vt, rowBaseV, rowBaseQ,
+    hipLaunchKernelGGL(k_vol_bounds, dim3(grid_for(ctx, n + 1)), dim3(kBlock), 0, ctx->stream, vt, rowBaseV, rowBaseQ,
                        (int64_t)V, (int64_t)Q, vb, qb);
     HIPC(ctx, hipMemcpyAsync(vb + n + 1, vb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
     HIPC(ctx, hipMemcpyAsync(qb + n + 1, qb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
@@ -482,10 +490,10 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     mark(ctx, "vertices");
     if (A > 0) {
         HIPC(ctx, hipMemsetAsync(slow, 0, sizeof(uint32_t), ctx->stream));
-        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux,
+        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux,
                            avloc, rowBaseV, (int64_t)A, pos, nrm, deg, slow);
         check_launch(ctx, "k_vertices");
-        hipLaunchKernelGGL(k_vertices_slow<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0,
+        hipLaunchKernelGGL(k_vertices_slow<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), 0,
                            ctx->stream, vt, acell, acode, aaux, avloc, rowBaseV, slow, pos, nrm, deg, vstat);
         check_launch(ctx, "k_vertices_slow");
     }
@@ -496,7 +504,7 @@ void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     ctx->rows_epoch = defer_read(ctx, 1, tot + kTotRows, sizeof(uint64_t));
     ctx->rows_expect = 4 * Q;
     if (A > 0) {
-        hipLaunchKernelGGL(k_quads, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux, avloc,
+        hipLaunchKernelGGL(k_quads, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux, avloc,
                            aqloc,
                            rowBaseV, rowBaseQ, (int64_t)A, segMask, segBase, quads, colors, off, adj, fails + 5);
         check_launch(ctx, "k_quads");
@@ -598,14 +606,14 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
             HIPC(ctx, hipMemsetAsync(cursor, 0, nv * sizeof(uint32_t), ctx->stream));
         }
         if (nhe > 0) {
-            hipLaunchKernelGGL(k_he_degree, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, deg,
+            hipLaunchKernelGGL(k_he_degree, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, deg,
                                &st->bad);
             check_launch(ctx, "k_he_degree");
         }
         const uint64_t rows = scan(ctx, deg, off, nv);
         HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + nv), (int)(uint32_t)rows, 1, ctx->stream));
         if (nhe > 0) {
-            hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
+            hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
                                cursor, adj);
             check_launch(ctx, "k_he_scatter");
         }
@@ -618,13 +626,13 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     if (nhe > 0) {
         // deg holds every vertex's out-degree here; the twins add boundary
         // arrivals, turning it into the valence
-        hipLaunchKernelGGL(k_he_twins, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, adj,
+        hipLaunchKernelGGL(k_he_twins, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, adj,
                            twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst,
                            final_pass ? nullptr : static_cast<uint32_t*>(ctx->deg.p));
         check_launch(ctx, "k_he_twins");
     }
     if (final_pass && nv > 0) {
-        hipLaunchKernelGGL(k_he_v, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, off, adj, nv, hev);
+        hipLaunchKernelGGL(k_he_v, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, off, adj, nv, hev);
         check_launch(ctx, "k_he_v");
     }
     if (!final_pass) {
@@ -640,7 +648,7 @@ HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
     ctx->last_bad = hs.bad;
     if (final_pass)
         three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
-            hipLaunchKernelGGL(k_he_three, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
+            hipLaunchKernelGGL(k_he_three, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
                                adj, flag, ovr, recs, nrec, pass, hev);
         });
     return hs;
@@ -653,7 +661,7 @@ int32_t* valence(dmc_context* ctx) {
     uint32_t* val = static_cast<uint32_t*>(ctx->deg.p);
     unsigned int* zero = static_cast<unsigned int*>(ctx->fails.p) + 4;
     if (nv > 0)
-        hipLaunchKernelGGL(k_count_zero, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, val, nv, zero);
+        hipLaunchKernelGGL(k_count_zero, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, val, nv, zero);
     check_launch(ctx, "valence");
     return reinterpret_cast<int32_t*>(val);
 }
@@ -676,10 +684,10 @@ void color_faces(dmc_context* ctx) {
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     for (int c = 5; c < 24; c++)
         if (h[c])
-            hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[c])), dim3(kBlock), 0, ctx->stream, list + h[24 + c],
+            hipLaunchKernelGGL(k_color_class, dim3(grid_for(ctx, h[c])), dim3(kBlock), 0, ctx->stream, list + h[24 + c],
                                (int64_t)h[c], twin, colors, 5);
     if (h[4])
-        hipLaunchKernelGGL(k_color_class, dim3(grid_for(h[4])), dim3(kBlock), 0, ctx->stream, list + h[24 + 4],
+        hipLaunchKernelGGL(k_color_class, dim3(grid_for(ctx, h[4])), dim3(kBlock), 0, ctx->stream, list + h[24 + 4],
                            (int64_t)h[4], twin, colors, 4);
     check_launch(ctx, "colour");
 }
@@ -687,7 +695,7 @@ void color_faces(dmc_context* ctx) {
 // Per-volume first vertex / quadrilateral after an order-preserving
 // compaction (newidx = exclusive scan of the keep flags).
 void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, int64_t nold, int64_t nnew) {
-    hipLaunchKernelGGL(k_vol_remap, dim3(grid_for(ctx->nvol + 1)), dim3(kBlock), 0, ctx->stream,
+    hipLaunchKernelGGL(k_vol_remap, dim3(grid_for(ctx, ctx->nvol + 1)), dim3(kBlock), 0, ctx->stream,
                        static_cast<int64_t*>(bounds.p), ctx->nvol + 1, newidx, nold, nnew);
     check_launch(ctx, "k_vol_remap");
 }
@@ -704,7 +712,7 @@ int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsec
     uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], nn);
     int32_t* rlist = ensure<int32_t>(ctx, ctx->rlist, nv - nn);
     if (nv > 0)
-        hipLaunchKernelGGL(k_compact_slots, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv, slot,
+        hipLaunchKernelGGL(k_compact_slots, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv, slot,
                            slot2, vsecond, rlist);
     check_launch(ctx, "k_compact_slots");
     ctx->scur = 1 - ctx->scur;
@@ -723,7 +731,7 @@ void gather_vertices(dmc_context* ctx) {
     double* pos2 = ensure<double>(ctx, ctx->pos[dst], 3 * nv);
     double* nrm2 = ensure<double>(ctx, ctx->nrm[dst], 3 * nv);
     if (nv > 0)
-        hipLaunchKernelGGL(k_gather_vertices, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_gather_vertices, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream,
                            static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p), nv,
                            static_cast<const double*>(ctx->pos[src].p), static_cast<const double*>(ctx->nrm[src].p),
                            pos2, nrm2);
@@ -748,12 +756,12 @@ void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2, const
     HIPC(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
     if (m.nv > 0) {
         // kept vertices: old row length less the corners in dropped quads
-        hipLaunchKernelGGL(k_remap_degree_kept, dim3(grid_for(m.nv)), dim3(kBlock), 0, ctx->stream, m, off, deg2);
+        hipLaunchKernelGGL(k_remap_degree_kept, dim3(grid_for(ctx, m.nv)), dim3(kBlock), 0, ctx->stream, m, off, deg2);
         if (ndrop > 0)
-            hipLaunchKernelGGL(k_remap_drop, dim3(grid_for(ndrop)), dim3(kBlock), 0, ctx->stream, m,
+            hipLaunchKernelGGL(k_remap_drop, dim3(grid_for(ctx, ndrop)), dim3(kBlock), 0, ctx->stream, m,
                                static_cast<const int32_t*>(ctx->quads[ctx->cur].p), dlist, ndrop, deg2);
         if (m.nrem > 0)
-            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(m.nrem)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
+            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(ctx, m.nrem)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
                                deg2, 1, bad);
         check_launch(ctx, "k_remap_degree");
     }
@@ -762,7 +770,7 @@ void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2, const
     set_row_end(ctx, off2, nn, tot);
     if (m.nv > 0) {
         for (int pass = 0; pass < 2; pass++)
-            hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
+            hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(ctx, pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
                                m, off, adj, nn, off2, cur2, adj2, pass);
         check_launch(ctx, "k_remap_rows");
     }
@@ -794,8 +802,8 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
     if (nq > 0) {
-        hipLaunchKernelGGL(k_mark3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p);
-        hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
+        hipLaunchKernelGGL(k_mark3x3y, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p);
+        hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
                            colors, twin, flag, lastT, lastR, em, vfirst);
         check_launch(ctx, "merge3x3y");
     }
@@ -803,8 +811,8 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     const int64_t qn = (int64_t)scan_keep(ctx, em, newQ, nq);
     if (nq > qn) {
         int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, nq - qn);
-        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, em, newQ, nq, mlist);
-        hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(nq - qn)), dim3(kBlock), 0, ctx->stream, quads, mlist, nq - qn,
+        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, em, newQ, nq, mlist);
+        hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(ctx, nq - qn)), dim3(kBlock), 0, ctx->stream, quads, mlist, nq - qn,
                            em, val, lastT, lastR, static_cast<double*>(ctx->pos[ctx->pcur].p),
                            static_cast<double*>(ctx->nrm[ctx->pcur].p),
                            ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
@@ -817,9 +825,9 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
     HeStats* st = static_cast<HeStats*>(ctx->stats.p);
     if (nv > 0)
-        hipLaunchKernelGGL(k_vmap3x3y, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, vfirst, vsecond, nv, vmap);
+        hipLaunchKernelGGL(k_vmap3x3y, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, vfirst, vsecond, nv, vmap);
     if (nq > 0)
-        hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
+        hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
                            vmap, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3x3y");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
@@ -852,9 +860,9 @@ void simplify3333(dmc_context* ctx) {
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     if (nq > 0) {
-        hipLaunchKernelGGL(k_mark3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag, val,
+        hipLaunchKernelGGL(k_mark3333, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag, val,
                            lastF, vflag, p3333, rflag);
-        hipLaunchKernelGGL(k_twin3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag,
+        hipLaunchKernelGGL(k_twin3333, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag,
                            val, p3333, lastF, vtwin);
         check_launch(ctx, "mark3333");
     }
@@ -868,16 +876,16 @@ void simplify3333(dmc_context* ctx) {
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
     HeStats* st = static_cast<HeStats*>(ctx->stats.p);
     if (nv > 0)
-        hipLaunchKernelGGL(k_vmap3333, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, vflag, newv, nv, vmap);
+        hipLaunchKernelGGL(k_vmap3333, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, vflag, newv, nv, vmap);
     if (nq > 0)
-        hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
+        hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
                            newQ, p3333, vtwin, vmap, nv, q2, qmap, &st->bad);
     check_launch(ctx, "k_compact_quads3333");
     remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
     // the dropped quadrilaterals, for the row degrees
     int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, std::max<int64_t>(nq - qn, 1));
     if (nq > qn)
-        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, rflag, newQ, nq, dlist);
+        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, rflag, newQ, nq, dlist);
     remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
                qn, dlist, nq - qn);
     ctx->cur = dst;
@@ -938,11 +946,11 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint32_t* amb = ensure<uint32_t>(ctx, ctx->tamb, A + 1);
     HIPC(ctx, hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), ctx->stream));
     HIPC(ctx, hipMemsetAsync(amb, 0, sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_tmc_cells, dim3(grid_for(R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
+    hipLaunchKernelGGL(k_tmc_cells, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
                        sw.lists, sw.segCount, sw.segBase, acell, acode, cnt_v, cnt_t, amb, st);
     check_launch(ctx, "k_tmc_cells");
     if (A > 0) {
-        hipLaunchKernelGGL(k_tmc_amb_count<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
+        hipLaunchKernelGGL(k_tmc_amb_count<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
                            amb, acell, acode, cnt_v, cnt_t, st);
         check_launch(ctx, "k_tmc_amb_count");
     }
@@ -965,13 +973,13 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint2* avb = ensure<uint2>(ctx, ctx->tvb, A);
     mark(ctx, "emit");
     if (A > 0) {
-        hipLaunchKernelGGL(k_tmc_vbase, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, vofs,
+        hipLaunchKernelGGL(k_tmc_vbase, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, vofs,
                            (int64_t)A, avb);
         check_launch(ctx, "k_tmc_vbase");
-        hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avb,
+        hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avb,
                            tofs, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris, st + 2);
         check_launch(ctx, "k_tmc_emit");
-        hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
+        hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
                            amb, acell, acode, avb, tofs, sw.segMask, sw.segBase, pos, nrm, tris, st + 2);
         check_launch(ctx, "k_tmc_emit_amb");
     }
@@ -1028,7 +1036,7 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
         mark(ctx, "triangles");
         tris = ensure<int32_t>(ctx, ctx->tris, 6 * nq);
         if (nq > 0)
-            hipLaunchKernelGGL(k_triangles, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream,
+            hipLaunchKernelGGL(k_triangles, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream,
                                static_cast<int32_t*>(ctx->quads[ctx->cur].p), nq, nv,
                                static_cast<double*>(ctx->pos[ctx->pcur].p), tris);
         check_launch(ctx, "k_triangles");
@@ -1040,7 +1048,7 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
     int32_t* hev = static_cast<int32_t*>(ctx->hev.p);
     if (n > 1 && (nq > 0 || nv > 0)) {
         // each volume's ids start at 0
-        hipLaunchKernelGGL(k_rebase, dim3(grid_for(std::max(nq, nv))), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_rebase, dim3(grid_for(ctx, std::max(nq, nv))), dim3(kBlock), 0, ctx->stream,
                            static_cast<const int64_t*>(ctx->vbound.p), static_cast<const int64_t*>(ctx->qbound.p), n,
                            quads, twin, tris, nq, hev, nv);
         check_launch(ctx, "k_rebase");
@@ -1216,16 +1224,16 @@ dmc_status dmc_create(dmc_context** out, int device) {
     }
     if (device < 0 || device >= n) return DMC_ERR_INVALID_ARG;
     if (hipSetDevice(device) != hipSuccess) return DMC_ERR_HIP;
+    dmc_context* ctx = new dmc_context();
+    ctx->device = device;
     if (const char* cap = getenv("TMC_GRID_CAP")) {
         const long v = strtol(cap, nullptr, 10);
-        if (v >= 64 && v <= (1l << 24)) kGridCap = (int)v;
+        if (v >= kGridCapMin && v <= kGridCapMax) ctx->grid_cap = (int)v;
     }
     if (const char* w = getenv("TMC_SWEEP_WAVES")) {
         const long v = strtol(w, nullptr, 10);
-        if (v >= 256 && v <= (1l << 24)) kSweepWaves = (int)v;
+        if (v >= kSweepWavesMin && v <= kSweepWavesMax) ctx->sweep_waves = (int)v;
     }
-    dmc_context* ctx = new dmc_context();
-    ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return DMC_ERR_HIP;
@@ -1411,7 +1419,7 @@ static void do_edges(dmc_context* ctx, void* a_) {
     uint32_t* nid = ensure<uint32_t>(ctx, ctx->dnew, nhe);
     // the halfedge rows and twins are those of the final build
     if (nhe > 0)
-        hipLaunchKernelGGL(k_edge_reps, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_edge_reps, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream,
                            static_cast<const int32_t*>(m.d_quads), nhe, nv,
                            static_cast<const uint32_t*>(ctx->off.p), static_cast<const int2*>(ctx->adj.p),
                            static_cast<const int32_t*>(m.d_twins), static_cast<const uint8_t*>(m.d_he_flags), rep);
@@ -1420,7 +1428,7 @@ static void do_edges(dmc_context* ctx, void* a_) {
     *a.count = ne;
     if (a.edges && ne > 0) {
         int2* e = ensure<int2>(ctx, ctx->dedges, ne);
-        hipLaunchKernelGGL(k_edge_emit, dim3(grid_for(nhe)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_edge_emit, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream,
                            static_cast<const int32_t*>(m.d_quads), nhe, rep, nid, e);
         check_launch(ctx, "k_edge_emit");
         HIPC(ctx, hipMemcpyAsync(a.edges, e, ne * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
@@ -1451,10 +1459,10 @@ static void do_valence_hist(dmc_context* ctx, void* a_) {
     uint32_t* hist = ensure<uint32_t>(ctx, ctx->dhist, kValenceBins + 1);
     HIPC(ctx, hipMemsetAsync(hist, 0, (kValenceBins + 1) * sizeof(uint32_t), ctx->stream));
     if (nv > 0) {
-        hipLaunchKernelGGL(k_vertex_valence, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_vertex_valence, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream,
                            static_cast<const int32_t*>(m.d_quads), a.triangles ? m.d_tris : nullptr, nv,
                            static_cast<const uint32_t*>(ctx->off.p), static_cast<const int2*>(ctx->adj.p), vval);
-        hipLaunchKernelGGL(k_valence_hist, dim3(grid_for(nv, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, vval, nv,
+        hipLaunchKernelGGL(k_valence_hist, dim3(grid_for(ctx, nv, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, vval, nv,
                            hist);
     }
     check_launch(ctx, "valence histogram");
@@ -1489,10 +1497,10 @@ static void do_quality(dmc_context* ctx, void* a_) {
     if (n == 0) return;
     float* q = ensure<float>(ctx, ctx->dqual, n);
     if (a.which == 0)
-        hipLaunchKernelGGL(k_quality_quads, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_quality_quads, dim3(grid_for(ctx, n)), dim3(kBlock), 0, ctx->stream,
                            static_cast<const int32_t*>(m.d_quads), n, static_cast<const double*>(m.d_positions), q);
     else
-        hipLaunchKernelGGL(k_quality_tris, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_quality_tris, dim3(grid_for(ctx, n)), dim3(kBlock), 0, ctx->stream,
                            static_cast<const int32_t*>(m.d_tris), n, static_cast<const double*>(m.d_positions), q);
     check_launch(ctx, "element quality");
     HIPC(ctx, hipMemcpyAsync(a.quality, q, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -1571,6 +1579,25 @@ dmc_status dmc_write_obj(dmc_context* ctx, const char* path, int triangles) {
     if (!path) return DMC_ERR_INVALID_ARG;
     ObjArgs a{path, triangles};
     return guard(ctx, do_write_obj, &a);
+}
+
+dmc_status dmc_get_launch(dmc_context* ctx, dmc_launch* out) {
+    if (!ctx || !out) return DMC_ERR_INVALID_ARG;
+    out->grid_cap = ctx->grid_cap;
+    out->sweep_waves = ctx->sweep_waves;
+    return DMC_OK;
+}
+
+dmc_status dmc_set_launch(dmc_context* ctx, const dmc_launch* l) {
+    if (!ctx || !l) return DMC_ERR_INVALID_ARG;
+    if (l->grid_cap < kGridCapMin || l->grid_cap > kGridCapMax || l->sweep_waves < kSweepWavesMin ||
+        l->sweep_waves > kSweepWavesMax) {
+        ctx->err = "launch shape out of range";
+        return DMC_ERR_INVALID_ARG;
+    }
+    ctx->grid_cap = l->grid_cap;
+    ctx->sweep_waves = l->sweep_waves;
+    return DMC_OK;
 }
 
 dmc_status dmc_stage_times(dmc_context* ctx, const char** names, float* ms, int* count) {
