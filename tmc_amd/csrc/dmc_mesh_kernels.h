@@ -188,7 +188,31 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
 #pragma unroll
         for (int p = 0; p < 5; p++) g[p] = 0x7FFFFFFF;
         int c = 0;
-        if (aok && vid_ok(b, nv)) {
+        // The common edge: both rows fit the registers, one a -> b entry and
+        // one b -> a entry.  Its group is those two, in ascending order --
+        // what the sorted inserts below would build, at a fraction of the
+        // VALU work.
+        uint32_t m1 = 0, m2 = 0;
+        int32_t x1 = 0, x2 = 0;
+#pragma unroll
+        for (int i = 0; i < kRowRegs; i++)
+            if ((uint32_t)i < n && r[i].y == b) {
+                m1 |= 1u << i;
+                x1 = r[i].x;
+            }
+#pragma unroll
+        for (int i = 0; i < kRowRegs; i++)
+            if ((uint32_t)i < nb && q[i].y == a) {
+                m2 |= 1u << i;
+                x2 = q[i].x;
+            }
+        const bool pair = aok && vid_ok(b, nv) && b != a && n <= (uint32_t)kRowRegs && nb <= (uint32_t)kRowRegs &&
+                          __popc(m1) == 1 && __popc(m2) == 1;
+        if (pair) {
+            g[0] = min(x1, x2);
+            g[1] = max(x1, x2);
+            c = 2;
+        } else if (aok && vid_ok(b, nv)) {
 #pragma unroll
             for (int i = 0; i < kRowRegs; i++)
                 if ((uint32_t)i < n && r[i].y == b) {
