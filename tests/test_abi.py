@@ -58,6 +58,10 @@ def test_status_strings_and_null_context(lib):
     assert lib.dmc_create(None, 0) == 1
     n = C.c_int(0)
     assert lib.dmc_stage_times(None, None, None, C.byref(n)) == 1
+    assert lib.dmc_get_launch(None, None) == 1
+    assert lib.dmc_set_launch(None, None) == 1
+    assert lib.dmc_autotune(None, None, 0, 0, 0, 1, None, None, None) == 1
+    assert lib.tmc_extract(None, None, None, 0, 0.0, 0, None) == 1
 
 
 def test_no_device_is_reported(lib):
