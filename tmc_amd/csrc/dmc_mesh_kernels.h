@@ -1070,16 +1070,18 @@ __global__ void __launch_bounds__(kBlock) k_quality_tris(const int32_t* __restri
 // collectTriangles (cpp_dmc/DualMarchingCubes.cpp:1387-1460) with minAngle
 // (DualMarchingCubes.h:746-756): split along the diagonal whose two
 // triangles have the larger minimum angle.
-__device__ __forceinline__ double tri_min_angle(const double* __restrict__ pos, int32_t i0, int32_t i1, int32_t i2) {
-    auto dist = [&](int32_t p, int32_t q) {
-        const double* a = pos + 3 * (int64_t)p;
-        const double* b = pos + 3 * (int64_t)q;
-        const double d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
-        return sqrt(d);
-    };
-    const float a = dist(i0, i1);
-    const float b = dist(i1, i2);
-    const float c = dist(i2, i0);
+// Side lengths as minAngle forms them: the double distance rounded to
+// float.  (p - q)^2 == (q - p)^2 exactly, so each of a quad's six vertex
+// pairs is computed once and shared by the triangles that use it.
+__device__ __forceinline__ float tri_side(const double* __restrict__ pos, int32_t p, int32_t q) {
+    const double* a = pos + 3 * (int64_t)p;
+    const double* b = pos + 3 * (int64_t)q;
+    const double d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+    return (float)sqrt(d);
+}
+
+// minimum angle of the triangle with sides a = |i0 i1|, b = |i1 i2|, c = |i2 i0|
+__device__ __forceinline__ double tri_min_angle(float a, float b, float c) {
     const float A = acosf((b * b + c * c - a * a) / (2 * b * c));
     const float B = acosf((a * a + c * c - b * b) / (2 * a * c));
     const float C = acosf((b * b + a * a - c * c) / (2 * b * a));
@@ -1096,11 +1098,14 @@ __global__ void __launch_bounds__(kBlock) k_triangles(const int32_t* __restrict_
             t[0] = q.x; t[1] = q.y; t[2] = q.z; t[3] = q.x; t[4] = q.z; t[5] = q.w;
             continue;
         }
-        double a1 = tri_min_angle(pos, q.x, q.y, q.z);
-        double a2 = tri_min_angle(pos, q.x, q.z, q.w);
+        const float xy = tri_side(pos, q.x, q.y), yz = tri_side(pos, q.y, q.z), zw = tri_side(pos, q.z, q.w),
+                    wx = tri_side(pos, q.w, q.x), xz = tri_side(pos, q.x, q.z), yw = tri_side(pos, q.y, q.w);
+        // triangles (x,y,z), (x,z,w), then (y,w,x), (y,z,w)
+        double a1 = tri_min_angle(xy, yz, xz);
+        double a2 = tri_min_angle(xz, zw, wx);
         const double b1 = (a2 < a1) ? a2 : a1, b2 = (a1 < a2) ? a2 : a1;
-        a1 = tri_min_angle(pos, q.y, q.w, q.x);
-        a2 = tri_min_angle(pos, q.y, q.z, q.w);
+        a1 = tri_min_angle(yw, wx, xy);
+        a2 = tri_min_angle(yz, zw, yw);
         const double c1 = (a2 < a1) ? a2 : a1, c2 = (a1 < a2) ? a2 : a1;
         if (b1 < c1 || (b1 == c1 && b2 <= c2)) {
             t[0] = q.y; t[1] = q.w; t[2] = q.x; t[3] = q.y; t[4] = q.z; t[5] = q.w;
