@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
 # scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024_traffic.md)
-TRAFFIC = {"schwarz_1024": 5740490679}
+TRAFFIC = {"schwarz_1024": 5738018600}
 
 
 def log(msg):
