@@ -6,11 +6,21 @@
 A step is one full dualMC of the workload's field (already resident in
 HBM): classification, compaction, vertices, quadrilaterals, 3X3Y (colour and
 isolated), 3333, halfedges and triangles; the mesh stays in HBM.  `value` is
-Gvoxels/s = nx*ny*nz / (time per step).  The roofline object is for the
-field sweep (k_sweep), timed with HIP events on the engine's stream over
-the timed steps.  The CPU baseline is the reference cpp_dmc compiled from its
-sources (oracle/_ref) -- or the C restatement if that is absent -- timed on a
-sub-block of the same field on one host core.
+Gvoxels/s = nx*ny*nz / (time per step).
+
+The roofline object is the cell stage as the survey's measurement section
+defines it: algorithmic bytes B = 4 N + 24 V + 17 Q (the field read once,
+FP32 position and normal per vertex, an int4 quadrilateral and its colour
+byte) over the summed device time of the cell-stage kernels (k_sweep,
+k_rows, k_vertices, k_vertices_slow, k_quads and the scans between them),
+timed with HIP events on the engine's stream over the timed steps.
+`kernels` lists the main kernels with their own byte models and fractions.
+The CPU baseline is the reference cpp_dmc compiled from its sources
+(oracle/_ref) -- or the C restatement if that is absent -- on one host
+core: dualMC alone and estimateGradient + flip_gradient + dualMC, on a
+512^3 corner block of the same field, and the whole 128^3 genus-2 field
+(median of 5).  `--cpu-full` runs the survey's complete CPU plan instead
+(medians of 5 on the 128^3 and 512^3 fields, one full 1024^3 run).
 """
 import argparse
 import json
@@ -25,9 +35,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_sweep launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-# scaled by the copy-kernel calibration (profiles/k_sweep_schwarz_1024_traffic.md)
-TRAFFIC = {"schwarz_1024": 5738018600}
+# HBM bytes per extraction of the cell-stage kernels from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration
+# (profiles/dmc_schwarz_1024_kernel_traffic.md)
+TRAFFIC = {}
+CELL_KERNELS = ("k_sweep", "k_scan_cells", "k_vol_abase", "k_rows", "k_vol_bounds", "k_vertices", "k_vertices_slow",
+                "k_quads")
 
 
 def log(msg):
@@ -44,30 +57,77 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(field, grid, iso, block):
-    """Reference cpp_dmc (estimateGradient + flip_gradient + dualMC with
-    sFlag) on the block^3 corner of the field, one core."""
+def _cpu_run(sub, g, iso):
+    """One CPU run on a float field: (V, Q, gradient s, dualMC s, kind)."""
     import oracle
-    sub = np.ascontiguousarray(field[:block, :block, :block])
-    g = oracle.Grid(block, block, block, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
     if os.path.exists(oracle.REF_SO):
-        ref = oracle.Reference()
-        kind = "reference"
-        t = time.perf_counter()
-        nv, nq = ref.run_float(g, sub, iso, True)
-        dt = time.perf_counter() - t
-    else:
-        orc = oracle.Oracle()
-        kind = "port"
-        t = time.perf_counter()
-        s = sub.astype(np.float64).ravel()
-        m = orc.dualmc(g, s, orc.gradient(g, s), iso, True)
-        dt = time.perf_counter() - t
-        nv, nq = len(m["v"]), len(m["quads"])
-    return {"value": round(block ** 3 / dt / 1e9, 6), "unit": "Gvoxels/s", "cores": 1, "kind": kind,
+        nv, nq, tg, td = oracle.Reference().run_float(g, sub, iso, True, legs=True)
+        return nv, nq, tg, td, "reference"
+    orc = oracle.Oracle()
+    s = sub.astype(np.float64).ravel()
+    t = time.perf_counter()
+    grad = orc.gradient(g, s)
+    tg = time.perf_counter() - t
+    t = time.perf_counter()
+    m = orc.dualmc(g, s, grad, iso, True)
+    td = time.perf_counter() - t
+    return len(m["v"]), len(m["quads"]), tg, td, "port"
+
+
+def _ogrid(field, grid, shape=None):
+    import oracle
+    nz, ny, nx = shape or field.shape
+    return oracle.Grid(nx, ny, nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+
+
+def cpu_case(field, grid, iso, reps, label):
+    """`reps` CPU runs of the whole field (median), dualMC alone and with
+    the gradient."""
+    runs = [_cpu_run(field, _ogrid(field, grid), iso) for _ in range(reps)]
+    tg = float(np.median([r[2] for r in runs]))
+    td = float(np.median([r[3] for r in runs]))
+    n = field.size
+    log(f"[cpu] {label}: gradient {tg:.3f} s, dualMC {td:.3f} s (median of {reps})")
+    return {"case": label, "points": int(n), "runs": reps, "kind": runs[0][4], "V": runs[0][0], "Q": runs[0][1],
+            "gradient_s": round(tg, 4), "dualmc_s": round(td, 4),
+            "dualmc_gvoxels_per_s": round(n / td / 1e9, 6),
+            "gradient_plus_dualmc_gvoxels_per_s": round(n / (tg + td) / 1e9, 6)}
+
+
+def cpu_baseline(field, grid, iso, block):
+    """Reference cpp_dmc on one core: the block^3 corner of the field
+    (estimateGradient + flip_gradient and dualMC with sFlag timed apart),
+    and the whole 128^3 genus-2 field, median of 5."""
+    from tmc_amd import workloads
+    sub = np.ascontiguousarray(field[:block, :block, :block])
+    nv, nq, tg, td, kind = _cpu_run(sub, _ogrid(sub, grid), iso)
+    n = block ** 3
+    f1, g1, i1 = workloads.genus2(128)
+    c1 = cpu_case(f1, g1, i1, 5, "genus2_128")
+    return {"value": round(n / (tg + td) / 1e9, 6), "unit": "Gvoxels/s", "cores": 1, "kind": kind,
             "sample": f"{block}^3 corner block of the same field (k,j,i < {block}); UniformGrid from the float "
-                      f"field, estimateGradient + flip_gradient + dualMC(sFlag=true); {dt:.2f} s, "
-                      f"V={nv} Q={nq}; host CPU: {cpu_model()} ({os.cpu_count()} logical)"}
+                      f"field, estimateGradient + flip_gradient {tg:.2f} s + dualMC(sFlag=true) {td:.2f} s, "
+                      f"V={nv} Q={nq}; host CPU: {cpu_model()} ({os.cpu_count()} logical)",
+            "dualmc_only_gvoxels_per_s": round(n / td / 1e9, 6),
+            "gradient_plus_dualmc_gvoxels_per_s": round(n / (tg + td) / 1e9, 6),
+            "genus2_128": c1,
+            "full_plan": "python bench.py --cpu-full (medians of 5 at 128^3 and 512^3, one 1024^3 run): "
+                         "profiles/cpu_baseline_full.md"}
+
+
+def cpu_full():
+    """The survey's CPU plan on this host: 128^3 genus-2 and 512^3 blobs
+    (median of 5 each) and the 1024^3 Schwarz-P field (one run), dualMC
+    alone and with the gradient, one core."""
+    from tmc_amd import workloads
+    mem = next((l.split()[1] for l in open("/proc/meminfo") if l.startswith("MemTotal")), "?")
+    out = {"host": f"{cpu_model()} ({os.cpu_count()} logical), MemTotal {mem} kB", "cores": 1, "cases": []}
+    for name, reps in (("genus2_128", 5), ("blobs_512", 5), ("schwarz_1024", 1)):
+        f, g, iso = workloads.CONFIGS[name]()
+        out["cases"].append(cpu_case(f, g, iso, reps, name))
+        del f
+        print(json.dumps(out["cases"][-1]), flush=True)
+    return out
 
 
 def pack_device(eng, fields):
@@ -112,10 +172,15 @@ def bench_batch(count=4096, seed=4, steps=3, plain_sample=8):
     tune["best"]["ms"] = round(tune["best"]["ms"], 3)
     tune["trials"] = [dict(x, ms=round(x["ms"], 3)) for x in tune["trials"]]
     log(f"[batch] autotune: {tune['best']} in {tune['tune_s']} s")
+    # two untimed passes under the tuned shape (the second is captured into a
+    # HIP graph), then the timed ones replay it
+    for _ in range(2):
+        eng.extract_batch(dv, flags=flags)
     t = time.perf_counter()
     for _ in range(steps):
         ms = eng.extract_batch(dv, flags=flags)
     batched_s = (time.perf_counter() - t) / steps
+    batch_pass = ms[0].pass_mode
     nq = sum(m.nq for m in ms)
     log(f"[batch] batched: {batched_s * 1e3:.1f} ms per {count} volumes")
     sample = dv[::plain_sample]
@@ -134,6 +199,7 @@ def bench_batch(count=4096, seed=4, steps=3, plain_sample=8):
             "spec": "axes round(exp(U[ln16,ln96])), generator U{genus2, schwarz_p, blobs}, iso U[-0.2,0.2] / "
                     "U[0.3,0.7] (blobs), seed 4, fields in HBM",
             "batched": {"volumes_per_s": round(count / batched_s, 1), "ms_per_batch": round(batched_s * 1e3, 3),
+                        "pass": "HIP graph replay" if batch_pass == 2 else "launched kernel by kernel",
                         "gvoxels_per_s": round(nvox / batched_s / 1e9, 3)},
             "plain": {"volumes_per_s": round(plain_vps, 1), "sampled": len(sample),
                       "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3)},
@@ -154,9 +220,10 @@ def serve_pool(seed=5):
     return specs
 
 
-def serve_run(srv, pool, rate, n, seed, label):
+def serve_run(srv, pool, rate, n, seed, label, classes=None):
     """Poisson arrivals at `rate` per second (inf: all at once) drawing
-    uniformly from `pool`; returns latencies and throughput."""
+    uniformly from `pool`; returns latencies and throughput (and p50/p99 per
+    class when `classes` gives each pool entry's class)."""
     import queue
     import threading
     rng = np.random.default_rng(seed)
@@ -164,15 +231,16 @@ def serve_run(srv, pool, rate, n, seed, label):
     gaps = np.zeros(n) if not np.isfinite(rate) else rng.exponential(1.0 / rate, size=n)
     arrive = np.cumsum(gaps)
     q = queue.Queue()
-    lat, batch_sizes, done_t = [], [], []
+    lat, batch_sizes, done_t, kind = [], [], [], []
 
     def collect():
         for _ in range(n):
-            t = q.get()
+            i, t = q.get()
             info = srv.wait(t)
             lat.append(info["latency"])
             batch_sizes.append(info["batch_size"])
             done_t.append(info["t_done"])
+            kind.append(classes[pick[i]] if classes else None)
             srv.release(t)
 
     th = threading.Thread(target=collect)
@@ -183,51 +251,97 @@ def serve_run(srv, pool, rate, n, seed, label):
         if wait > 0:
             time.sleep(wait)
         p, iso, g = pool[pick[i]]
-        q.put(srv.submit(p, iso, g))
+        q.put((i, srv.submit(p, iso, g)))
     th.join()
     span = time.perf_counter() - t0
     out = {"rate_offered": None if not np.isfinite(rate) else round(rate, 1), "requests": n,
            "volumes_per_s": round(n / span, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
            "p99_ms": round(pct(lat, 99) * 1e3, 3), "mean_batch": round(float(np.mean(batch_sizes)), 2)}
+    if classes:
+        per = {}
+        for c in sorted(set(classes)):
+            xs = [l for l, k in zip(lat, kind) if k == c]
+            if xs:
+                per[c] = {"n": len(xs), "p50_ms": round(pct(xs, 50) * 1e3, 3), "p99_ms": round(pct(xs, 99) * 1e3, 3)}
+        out["per_class"] = per
     log(f"[serve] {label}: {out}")
     return out
 
 
-def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2):
-    """Served mix: Poisson arrivals, 70% U{32..128}^3, 25% 256^3, 5% 512^3,
-    fields in HBM.  One-at-a-time (max_batch 1, one worker) runs at 70% of
-    its capacity and sets the latency bar (its p99).  The batched server
-    (passes of at most `batch_points` grid points, `workers` passes running
-    concurrently) is then run at increasing multiples of that rate; reported
-    is the highest rate whose p99 stays at or below the bar, with its
-    p50/p99 and volumes/s."""
+def _alone_ms(eng, pool, flags, reps=3):
+    """Median device-resident one-at-a-time extraction time of each pool
+    entry (ms)."""
+    out = []
+    for p, iso, g in pool:
+        eng.extract(p, iso, g, flags=flags)
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            eng.extract(p, iso, g, flags=flags)
+            ts.append(time.perf_counter() - t)
+        out.append(float(np.median(ts)) * 1e3)
+    return out
+
+
+def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2, pool_kind="mix"):
+    """Served workload: Poisson arrivals drawing uniformly from a pool of
+    device-resident fields.  pool_kind "mix": the serving mix (70%
+    U{32..128}^3, 25% 256^3, 5% 512^3); "small": 256 volumes of the batched
+    workload (16-96 per axis).  One-at-a-time (max_batch 1, one worker) runs
+    at 70% of its capacity and sets the latency bar (its p99).  The batched
+    server (passes of at most `batch_points` grid points, `workers` passes
+    running concurrently) is then run at increasing multiples of that rate;
+    reported is the highest rate whose p99 stays at or below the bar, with
+    its p50/p99 and volumes/s, per size class too.  `bound` is what the
+    pool's one-at-a-time extraction times allow: a batched server cannot
+    beat the GPU time of its large volumes, which batching does not
+    shorten."""
     import tmc_amd
     from tmc_amd import workloads
     t0 = time.perf_counter()
-    specs = serve_pool(seed)
-    vols = [workloads.make(s) for s in specs]
-    log(f"[serve] pool of {len(vols)} volumes generated in {time.perf_counter() - t0:.1f} s")
+    if pool_kind == "mix":
+        specs = serve_pool(seed)
+        budget = 96 << 30
+    else:
+        specs = workloads.batch_specs(256, seed=seed + 100)
+        budget = 96 << 30
+    vols = workloads.make_many(specs)
+    classes = [("small" if max(s["shape"]) < 256 else f"{max(s['shape'])}^3") for s in specs]
+    log(f"[serve] {pool_kind} pool of {len(vols)} volumes generated in {time.perf_counter() - t0:.1f} s")
     eng = tmc_amd.Engine(0)
     buf, ptrs = pack_device(eng, [f for f, _, _ in vols])
     pool = [(p, iso, g) for p, (f, g, iso) in zip(ptrs, vols)]
     flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES | tmc_amd.FIELD_ON_DEVICE
-    budget = 96 << 30
+    alone = _alone_ms(eng, pool, flags)
+    per_class_alone = {c: round(float(np.median([t for t, k in zip(alone, classes) if k == c])), 3)
+                       for c in sorted(set(classes))}
+    big = sum(t for t, k in zip(alone, classes) if k != "small") / len(pool)
+    mean_alone = float(np.mean(alone))
+    bound = {"mean_alone_ms": round(mean_alone, 3), "alone_ms_per_class": per_class_alone,
+             "large_gpu_ms_per_request": round(big, 3),
+             "capacity_ratio_max": round(mean_alone / big, 2) if big > 0 else None,
+             "rate_ratio_at_equal_p99_max": round(mean_alone / big / 0.7, 2) if big > 0 else None,
+             "note": "a batched server spends at least the large volumes' one-at-a-time GPU time per request; "
+                     "the plain server runs at 70% of its capacity"}
     plain_srv = tmc_amd.Server(0, max_batch=1, flags=flags, budget_bytes=budget)
-    serve_run(plain_srv, pool, float("inf"), 2 * len(pool), seed, "plain warm-up")
-    cap_plain = serve_run(plain_srv, pool, float("inf"), 120, seed + 1, "plain capacity")["volumes_per_s"]
+    serve_run(plain_srv, pool, float("inf"), 2 * len(pool) if pool_kind == "mix" else 64, seed, "plain warm-up")
+    cap_plain = serve_run(plain_srv, pool, float("inf"), 120 if pool_kind == "mix" else 400, seed + 1,
+                          "plain capacity")["volumes_per_s"]
     rate = 0.7 * cap_plain
-    plain = serve_run(plain_srv, pool, rate, max(100, int(rate * secs)), seed + 2, "plain at 70% capacity")
+    plain = serve_run(plain_srv, pool, rate, max(100, int(rate * secs)), seed + 2, "plain at 70% capacity", classes)
     plain_srv.close()
     bsrv = tmc_amd.Server(0, flags=flags, budget_bytes=budget, max_batch_points=batch_points, workers=workers)
-    serve_run(bsrv, pool, float("inf"), 4 * len(pool), seed, "batched warm-up")
-    cap_b = serve_run(bsrv, pool, float("inf"), 1200, seed + 3, "batched capacity")["volumes_per_s"]
+    serve_run(bsrv, pool, float("inf"), 4 * len(pool) if pool_kind == "mix" else 512, seed, "batched warm-up")
+    cap_b = serve_run(bsrv, pool, float("inf"), 1200 if pool_kind == "mix" else 8192, seed + 3,
+                      "batched capacity")["volumes_per_s"]
     best = None
     tried = []
-    for mult in (1.25, 1.5, 1.75, 2, 2.5, 3, 3.5, 4, 5, 6, 8, 10):
+    for mult in (1.25, 1.5, 1.75, 2, 2.5, 3, 3.5, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32):
         r = mult * rate
         if r > 1.05 * cap_b:
             break
-        res = serve_run(bsrv, pool, r, max(200, int(r * secs)), seed + 4 + int(4 * mult), f"batched at {mult}x")
+        res = serve_run(bsrv, pool, r, max(200, int(r * secs)), seed + 4 + int(4 * mult), f"batched at {mult}x",
+                        classes)
         tried.append(dict(res, multiple=mult))
         if res["p99_ms"] <= plain["p99_ms"] and res["volumes_per_s"] >= 0.95 * r:
             best = dict(res, multiple=mult)
@@ -236,12 +350,13 @@ def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2):
     bsrv.close()
     buf.free()
     eng.close()
-    out = {"workload": "serve_mix",
-           "mix": "70% U{32..128}^3, 25% 256^3, 5% 512^3 (pool of 20, uniform draws), Poisson arrivals, "
-                  "fields in HBM, budget 96 GiB",
+    mixdesc = ("70% U{32..128}^3, 25% 256^3, 5% 512^3 (pool of 20, uniform draws)" if pool_kind == "mix" else
+               "256 volumes of the batched workload (axes round(exp(U[ln16,ln96])), mixed generators), uniform draws")
+    out = {"workload": "serve_mix" if pool_kind == "mix" else "serve_small",
+           "mix": mixdesc + ", Poisson arrivals, fields in HBM, budget 96 GiB",
            "plain": dict(plain, capacity_volumes_per_s=cap_plain),
            "batched_capacity_volumes_per_s": cap_b, "batch_points": batch_points, "workers": workers,
-           "batched_at_equal_p99": best, "batched_runs": tried}
+           "batched_at_equal_p99": best, "batched_runs": tried, "bound": bound}
     if best:
         out["rate_ratio_at_equal_p99"] = round(best["volumes_per_s"] / plain["volumes_per_s"], 2)
     return out
@@ -309,6 +424,28 @@ def calibrate_copy():
     print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}), flush=True)
 
 
+def bench_dropin(eng, field, grid, iso, reps=2):
+    """The reference call surface: dualMC(i0, ugrid, v, n, tris, quads,
+    sFlag) hands over a host grid and returns host arrays.  Timed: the
+    field's upload, the extraction and the download of positions, normals,
+    quads and triangles into host memory (the C++ drop-in then widens these
+    into std::vector<Vector>)."""
+    import tmc_amd
+    flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES
+    eng.extract(field, iso, grid, flags=flags)
+    eng.download(halfedges=False)
+    t = time.perf_counter()
+    for _ in range(reps):
+        eng.extract(field, iso, grid, flags=flags)
+        d = eng.download(halfedges=False)
+    ms = (time.perf_counter() - t) / reps * 1e3
+    mb = (d["v"].nbytes + d["n"].nbytes + d["quads"].nbytes + d["tris"].nbytes) / 1e6
+    log(f"[bench] drop-in call with host buffers: {ms:.1f} ms")
+    return {"ms_per_call": round(ms, 2), "gvoxels_per_s": round(field.size / (ms * 1e-3) / 1e9, 3),
+            "field_upload_mb": round(field.nbytes / 1e6, 1), "mesh_download_mb": round(mb, 1),
+            "what": "host field in, host v/n/quads/tris out (PCIe both ways), as dualMC's callers see it"}
+
+
 def bench_diagnostics(eng, reps=3):
     """Time the mesh diagnostics on the last mesh (wall clock per call,
     results copied to the host included): unique edges, valence
@@ -346,19 +483,25 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (4,096 volumes) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson mix) leg")
     ap.add_argument("--no-diagnostics", action="store_true", help="skip the mesh diagnostics timings")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in call timing")
     ap.add_argument("--only", choices=["batch", "serve"], help="run only the batched or served leg")
     ap.add_argument("--tmc-only", action="store_true",
                     help="run only the topologically correct MC leg on the workload (for profiling)")
     ap.add_argument("--no-tmc", action="store_true", help="skip the topologically correct MC leg")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only run the full CPU plan (128^3 and 512^3 median of 5, one 1024^3 run; host only)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.cpu_full:
+        print(json.dumps(cpu_full()), flush=True)
         return
     if args.only == "batch":
         print(json.dumps(bench_batch()), flush=True)
         return
     if args.only == "serve":
-        print(json.dumps(bench_serve()), flush=True)
+        print(json.dumps({"served": bench_serve(), "served_small": bench_serve(pool_kind="small")}), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -398,6 +541,17 @@ def main():
         eng.extract(buf, iso, grid, flags=flags, shape=field.shape)
         log(f"[bench] warmup {w + 1}/{args.warmup}")
     eng.synchronize()
+    # timed region: the steps as a caller runs them (after the first passes
+    # of this shape, each step replays the captured pass)
+    t = time.perf_counter()
+    for s in range(args.steps):
+        m = eng.extract(buf, iso, grid, flags=flags, shape=field.shape)
+    eng.synchronize()
+    dt = time.perf_counter() - t
+    ms = dt / args.steps * 1e3
+    pass_mode = m.pass_mode
+    # the same steps launched kernel by kernel with HIP events around every
+    # kernel on the engine's stream: stage and kernel device times
     stage_sum = {}
     t = time.perf_counter()
     for s in range(args.steps):
@@ -405,27 +559,44 @@ def main():
         for k, v in eng.stage_times().items():
             stage_sum[k] = stage_sum.get(k, 0.0) + v
     eng.synchronize()
-    dt = time.perf_counter() - t
-    ms = dt / args.steps * 1e3
-    stages = {k: round(v / args.steps, 4) for k, v in stage_sum.items()}
+    ms_timed = (time.perf_counter() - t) / args.steps * 1e3
+    stages = {k: round(v / args.steps, 4) for k, v in stage_sum.items() if not k.startswith("k_")}
+    kms = {k: v / args.steps for k, v in stage_sum.items() if k.startswith("k_")}
     log(f"[bench] {args.steps} steps, {ms:.3f} ms/step; stages {stages}")
     summ = m.summary()
     V, Q, A = summ["nv_cells"], summ["nq_cells"], summ["active_cells"]
-    # field sweep: one read of the field (4 B per point) plus what it writes:
-    # 4 B per active cell into the segment lists, and per (row, segment) a
-    # 4-B count and 32 B of activity words
-    sweep_ms = stages.get("sweep", float("nan"))
-    R = (grid.ny - 1) * (grid.nz - 1)
-    S = (grid.nx - 1 + 255) // 256
-    sweep_bytes = 4 * nvox + 4 * A + 36 * R * S
-    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    Vf, Qf = summ["nv"], summ["nq"]
+    # cell stage: 4 N + 24 V + 17 Q over the summed device time of its kernels
+    cell_t = sum(kms.get(k, 0.0) for k in CELL_KERNELS)
+    cell_bytes = 4 * nvox + 24 * V + 17 * Q
+    achieved = cell_bytes / (cell_t * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(args.workload),
-                "kernel": "k_sweep", "kernel_ms": sweep_ms, "algorithmic_bytes": sweep_bytes}
-    # cell stage as a whole (sweep, rows, vertices, quads): field read once,
-    # FP64 position + normal per vertex (48 B), 4 ids + 1 colour per quad
-    ext_ms = sum(stages.get(k, 0.0) for k in ("sweep", "rows", "vertices", "quads"))
-    ext_bytes = 4 * nvox + 48 * V + 17 * Q
+                "kernel": "cell stage: " + " + ".join(CELL_KERNELS),
+                "kernel_ms": round(cell_t, 4), "algorithmic_bytes": int(cell_bytes),
+                "bytes_model": "4 N + 24 V + 17 Q: field read once, FP32 position + normal per vertex, int4 quad + "
+                               "colour byte (V, Q before simplification)"}
+    # per-kernel figures: device ms per extraction (HIP events), bytes by
+    # each kernel's own model
+    R = (grid.ny - 1) * (grid.nz - 1)
+    builds = 4
+    models = {
+        "k_sweep": (4 * nvox, "4 N (field read once)"),
+        "k_rows": (4 * A + 20 * A + 8 * R, "4 A list entries read + 20 A per-cell words + 8 R row totals written"),
+        "k_vertices": (24 * V, "24 V (FP32 position + normal; the engine writes FP64, 48 V)"),
+        "k_quads": (17 * Q + 32 * Q, "17 Q (int4 quad + colour) + 32 Q (first halfedge rows)"),
+        "k_he_twins": (builds * (16 * Qf + 68 * Qf + 4 * Vf),
+                       "per build 16 Q read + 64 Q + 4 Q + 4 V written (final V, Q; 4 builds)"),
+        "k_triangles": (16 * Qf + 24 * Qf + 24 * 4 * Qf, "16 Q quads + 96 Q positions read, 24 Q written"),
+    }
+    kernels = []
+    for name, t in sorted(kms.items(), key=lambda kv: -kv[1]):
+        rec = {"name": name, "ms": round(t, 4), "pct_of_step": round(100 * t / ms, 2)}
+        if name in models:
+            b, model = models[name]
+            rec.update(bytes=int(b), bytes_model=model, achieved_gbs=round(b / (t * 1e-3) / 1e9, 1),
+                       frac=round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        kernels.append(rec)
     out = {
         "metric": "DMC Gvoxels/s at 1024^3 f32 (%HBM roofline)",
         "value": round(nvox / (ms * 1e-3) / 1e9, 3),
@@ -434,20 +605,25 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "pass": {0: "launched kernel by kernel", 1: "captured into a HIP graph", 2: "HIP graph replay"}[pass_mode],
+        "ms_per_step_kernel_by_kernel_with_events": round(ms_timed, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": dict(desc, flags="3X3Y colour + 3X3Y isolated + 3333 + halfedges + triangles",
-                       mesh_in="HBM", vertex_math="f64", V_cells=V, Q_cells=Q, V=summ["nv"], Q=summ["nq"],
+                       mesh_in="HBM", vertex_math="f64", V_cells=V, Q_cells=Q, V=Vf, Q=Qf,
                        active_cells=A, mquads_per_s=round(Q / (ms * 1e-3) / 1e6, 2),
-                       extraction_ms=round(ext_ms, 4),
-                       extraction_roofline_frac=round(ext_bytes / (ext_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       stage_ms=stages),
+                       host_syncs_per_step=m.host_syncs, stage_ms=stages,
+                       hash_load_factor="none: vertex and quad ids come from prefix sums over the cells, "
+                                        "there are no hash tables to load"),
         "roofline": roofline,
+        "kernels": kernels,
     }
     out["diagnostics"] = None if args.no_diagnostics else bench_diagnostics(eng)
+    if not args.no_dropin:
+        out["dropin"] = bench_dropin(eng, field, grid, iso)
     if not args.no_tmc:
         log("[bench] topologically correct MC leg")
         out["tmc"] = bench_tmc(eng, buf, field, grid, iso, args.steps, 1,
@@ -460,6 +636,8 @@ def main():
     if not args.no_serve:
         log("[bench] served leg: Poisson serving mix")
         out["served"] = bench_serve()
+        log("[bench] served leg: Poisson stream of small volumes")
+        out["served_small"] = bench_serve(pool_kind="small")
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on a {args.cpu_block}^3 block")
         out["cpu_baseline"] = cpu_baseline(field, grid, iso, min(args.cpu_block, min(field.shape)))

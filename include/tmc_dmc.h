@@ -91,6 +91,9 @@ typedef struct {
     int64_t nv_emitted;         /* vertices produced by the cell sweep */
     int64_t nv_cells;           /* vertices after removeUnusedVertices */
     int64_t nq_cells;           /* quadrilaterals before simplification */
+    int64_t host_syncs;         /* times the extraction waited for the device (batch-wide) */
+    int64_t pass_mode;          /* how its pass ran: 0 launch by launch, 1 captured into a HIP graph and
+                                   launched, 2 a replay of the graph of an earlier pass of the same shape */
 } dmc_mesh;
 
 /* Create a context on HIP device `device`. */
@@ -131,6 +134,11 @@ typedef struct {
 dmc_status dmc_extract_batch(dmc_context* ctx, const dmc_volume* vols, int32_t nvol, int field_type, uint32_t flags,
                              dmc_mesh* out);
 
+/* Device bytes a context holds in its buffer pools (grown to its largest
+ * extraction and kept for reuse until dmc_destroy).  No counterpart in the
+ * reference; the serving loop counts it in its memory budget. */
+dmc_status dmc_context_bytes(dmc_context* ctx, size_t* bytes);
+
 /* Copy the last mesh to host memory; any pointer may be NULL.
  * positions/normals: 3*nv doubles; quads 4*nq; tris 3*nt;
  * halfedges: 5 ints per halfedge {origin, face, next, twin, flag} as in
@@ -147,15 +155,22 @@ dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions
  * Serving loop (no counterpart in the reference, which extracts one grid per
  * synchronous dualMC call).  A server owns `workers` worker threads, each
  * with its own context (and so its own stream): a free worker admits the
- * waiting requests oldest first while their estimated working set plus the
- * passes in flight and the results still held fits budget_bytes (and at
- * most max_batch volumes and max_batch_points grid points per pass, the
- * first request of a pass always admitted), runs them as one
- * dmc_extract_batch and completes them together.  Passes of different
- * workers overlap on the GPU, so one fills the other's host round trips.
- * Results stay in device memory (pooled blocks) until dmc_server_release.
- * With max_batch = 1 and one worker the same loop serves one request at a
- * time.
+ * waiting requests oldest first while their estimated working set fits
+ * budget_bytes beside what the server holds -- result blocks (in use or
+ * idle; idle ones are freed when that makes a request fit), the buffer
+ * pools the worker contexts keep (dmc_context_bytes) and the estimates of
+ * the passes running -- and at most max_batch volumes and max_batch_points
+ * grid points per pass (0: 2^28, which keeps any pass inside the engine's
+ * index limits); the first request of a pass is always admitted.  It runs
+ * them as one dmc_extract_batch and completes them together; a pass that
+ * fails on a size limit of the union or on one bad member is run again one
+ * request at a time, so only a request that fails alone fails.  The working
+ * set of a request is an estimate (lists, counts and 40 bytes per grid
+ * point), so the budget bounds device memory up to that estimate.  Passes
+ * of different workers overlap on the GPU, so one fills the other's host
+ * round trips.  Results stay in device memory (pooled blocks) until
+ * dmc_server_release.  With max_batch = 1 and one worker the same loop
+ * serves one request at a time.
  * ---------------------------------------------------------------------- */
 typedef struct dmc_server dmc_server;
 
@@ -181,7 +196,10 @@ typedef struct {
 dmc_status dmc_server_create(dmc_server** server, const dmc_server_opts* opts);
 /* Finishes the queued requests, then frees everything. */
 dmc_status dmc_server_destroy(dmc_server* server);
-/* Queue one volume; its field must stay valid until the request is done. */
+/* Queue one volume; its field must stay valid until the request is done.
+ * A grid the engine cannot take (an axis below 2 or above 2^24 points,
+ * more than 2^32 cells) is refused here with DMC_ERR_INVALID_ARG or
+ * DMC_ERR_CAPACITY. */
 dmc_status dmc_server_submit(dmc_server* server, const dmc_volume* vol, uint64_t* ticket);
 dmc_status dmc_server_poll(dmc_server* server, uint64_t ticket, int* done);
 /* Block until the request is done; returns its status. */
@@ -192,6 +210,8 @@ dmc_status dmc_server_download(dmc_server* server, uint64_t ticket, double* posi
                                int32_t* bnd);
 /* Free the request and its share of the result memory. */
 dmc_status dmc_server_release(dmc_server* server, uint64_t ticket);
+/* held_bytes: result blocks holding unreleased results; pool_bytes: all
+ * result blocks (held and idle) plus the worker contexts' buffer pools. */
 dmc_status dmc_server_stats(dmc_server* server, uint64_t* batches, uint64_t* served, size_t* held_bytes,
                             size_t* pool_bytes);
 const char* dmc_server_last_error(dmc_server* server);

@@ -327,7 +327,8 @@ class Reference:
         L.dmcref_dualmc.argtypes = [C.POINTER(Grid), f64p, f64p, C.c_double, C.c_int, i64p,
                                     C.POINTER(f64p), C.POINTER(f64p), i64p, C.POINTER(i32p), i64p,
                                     C.POINTER(i32p)]
-        L.dmcref_run_float.argtypes = [C.POINTER(Grid), C.POINTER(C.c_float), C.c_double, C.c_int, i64p]
+        L.dmcref_run_float.argtypes = [C.POINTER(Grid), C.POINTER(C.c_float), C.c_double, C.c_int, i64p,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.dmcref_run_float.restype = C.c_int64
         L.dmcref_cell_polygons.argtypes = [f64p, C.c_double, i32p, i32p]
         L.dmcref_cell_vertices.argtypes = [f64p, C.c_double, f64p, u32p, u32p]
@@ -472,12 +473,16 @@ class Reference:
         out["tris"] = self._take(t, 3 * nt.value, np.int32).reshape(-1, 3)
         return out
 
-    def run_float(self, grid, field32, iso, sflag=True):
+    def run_float(self, grid, field32, iso, sflag=True, legs=False):
+        """UniformGrid from the float field, estimateGradient +
+        flip_gradient, dualMC: (V, Q), or with legs=True (V, Q, gradient
+        seconds, dualMC seconds) timed inside the library."""
         f = np.ascontiguousarray(field32, dtype=np.float32)
         nv = C.c_int64()
+        tg, td = C.c_double(), C.c_double()
         nq = self.lib.dmcref_run_float(C.byref(_g(grid)), f.ctypes.data_as(C.POINTER(C.c_float)),
-                                       float(iso), int(sflag), C.byref(nv))
-        return nv.value, nq
+                                       float(iso), int(sflag), C.byref(nv), C.byref(tg), C.byref(td))
+        return (nv.value, nq, tg.value, td.value) if legs else (nv.value, nq)
 
     def cell_polygons(self, f, iso):
         f = np.ascontiguousarray(f, dtype=np.float64)

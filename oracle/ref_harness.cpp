@@ -4,6 +4,7 @@
 // serve as the timed CPU baseline.  Built with ref_prelude.h (which opens up
 // `private`) so the intermediate stages can be called one by one.
 // Nothing here changes the reference; this file only drives it.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <iostream>
@@ -271,20 +272,29 @@ int dmcref_dualmc(const Grid* g, const double* s, const double* grad, double iso
 }
 
 // Timing entry for the CPU baseline: UniformGrid set-up from a float field,
-// estimateGradient + flip_gradient, then dualMC.  Returns the quad count.
-int64_t dmcref_run_float(const Grid* g, const float* f, double iso, int sflag, int64_t* nv_out) {
+// estimateGradient + flip_gradient, then dualMC.  Returns the quad count;
+// t_grad / t_dualmc (seconds, may be NULL) receive the wall time of the
+// gradient (estimateGradient + flip_gradient) and of dualMC alone.
+int64_t dmcref_run_float(const Grid* g, const float* f, double iso, int sflag, int64_t* nv_out, double* t_grad,
+                         double* t_dualmc) {
+    using clk = std::chrono::steady_clock;
     Mute mute;
     dmc::UniformGrid ug;
     const size_t n = (size_t)g->nx * g->ny * g->nz;
     std::vector<double> s(n);
     for (size_t q = 0; q < n; q++) s[q] = f[q];
     make_grid(ug, g, s.data(), nullptr);
+    const auto t0 = clk::now();
     ug.estimateGradient();
     ug.flip_gradient();
+    const auto t1 = clk::now();
     dmc::DualMarchingCubes d;
     std::vector<dmc::Vector> V, N;
     std::vector<int> T, Q;
     d.dualMC(iso, ug, V, N, T, Q, sflag != 0);
+    const auto t2 = clk::now();
+    if (t_grad) *t_grad = std::chrono::duration<double>(t1 - t0).count();
+    if (t_dualmc) *t_dualmc = std::chrono::duration<double>(t2 - t1).count();
     if (nv_out) *nv_out = (int64_t)V.size();
     return (int64_t)Q.size() / 4;
 }

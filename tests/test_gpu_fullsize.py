@@ -1,7 +1,11 @@
-"""Full-size workloads on the GPU, checked through properties that hold at
-any size, and for the 512^3 blobs against the counts and SHA-256 digests
-the compiled references wrote for that field (tests/golden/
-fullsize_digests.json; the oracle needs minutes at these sizes):
+"""Full-size workloads on the GPU, checked against known answers and through
+properties that hold at any size.  Known answers (tests/golden/
+fullsize_digests.json, tests/golden/make_fullsize_digests.py): counts and
+SHA-256 digests of positions, normals, quads (and for 1024^3 the halfedge
+rows and he_v) or triangles, written for the 512^3 blobs by the compiled
+references and for the 1024^3 Schwarz-P field by the oracle's C
+restatement (the compiled references' int keys overflow at that size).
+Properties:
 
 * 1024^3 Schwarz-P + noise and 512^3 Gaussian blobs + noise, Dual Marching
   Cubes with every simplification: halfedge structure (next cycles, twin
@@ -42,8 +46,7 @@ def eng():
     e.close()
 
 
-# counts and digests the compiled references wrote for blobs_512
-# (tests/golden/make_fullsize_digests.py)
+# counts and digests per workload (tests/golden/make_fullsize_digests.py)
 with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fullsize_digests.json")) as _fh:
     KNOWN = json.load(_fh)
 
@@ -73,11 +76,15 @@ def test_dmc_full_size_properties(eng, which):
     d = eng.download(halfedges=True)
     quads, he, v, n = d["quads"].reshape(-1, 4), d["he"].reshape(-1, 5), d["v"].reshape(-1, 3), d["n"].reshape(-1, 3)
     nv = s["nv"]
-    if which == KNOWN["workload"]:
-        k = KNOWN["dmc"]
-        assert (nv, s["nq"]) == (k["nv"], k["nq"])
-        assert _digest((v, "<f8"), (n, "<f8"), (quads, "<i4")) == k["sha256_v_n_quads"], \
-            "positions, normals or quads differ from the compiled cpp_dmc"
+    k = KNOWN[which]["dmc"]
+    assert (nv, s["nq"]) == (k["nv"], k["nq"])
+    assert _digest((v, "<f8"), (n, "<f8"), (quads, "<i4")) == k["sha256_v_n_quads"], \
+        f"positions, normals or quads differ from the {KNOWN[which]['source']}"
+    if "sha256_he_hev" in k:
+        assert (s["nonmanifold_edges"], s["nbnd"]) == (k["nonmanifold"], k["nbnd"])
+        assert _digest((he, "<i4"), (d["he_v"], "<i4")) == k["sha256_he_hev"], \
+            f"halfedges differ from the {KNOWN[which]['source']}"
+    _say(f"[gpu] {which}: bit-exact against the {KNOWN[which]['source']}")
     assert quads.min() >= 0 and quads.max() < nv
     assert np.all(np.bincount(quads.reshape(-1), minlength=nv) > 0), "unreferenced vertex"
     halfedge_invariants(nv, quads, he)
@@ -115,19 +122,22 @@ def test_dmc_full_size_properties(eng, which):
         eng.launch = {"grid_cap": 16384, "sweep_waves": 32768}
 
 
-def test_tmc_full_size_properties(eng):
-    _say("[gpu] full size TMC blobs_512: generating")
-    f, g, iso = workloads.CONFIGS["blobs_512"]()
+@pytest.mark.parametrize("which", ["blobs_512", "schwarz_1024"])
+def test_tmc_full_size_properties(eng, which):
+    if "tmc" not in KNOWN.get(which, {}):
+        pytest.skip(f"no known TMC answer for {which}")
+    _say(f"[gpu] full size TMC {which}: generating")
+    f, g, iso = workloads.CONFIGS[which]()
     r = eng.tmc(f, iso, g)
     nv, nt = r["nv"], r["nt"]
-    _say(f"[gpu] TMC blobs_512: V {nv} T {nt} ambiguous {r['ambiguous_cells']} failures {r['face_failures']}")
+    _say(f"[gpu] TMC {which}: V {nv} T {nt} ambiguous {r['ambiguous_cells']} failures {r['face_failures']}")
     assert nv > 0 and nt > 0
     t = r["tris"].reshape(-1, 3).astype(np.int64)
     v = r["v"].reshape(-1, 3)
-    k = KNOWN["tmc"]
+    k = KNOWN[which]["tmc"]
     assert (nv, nt) == (k["nv"], k["nt"])
     assert _digest((v, "<f8"), (r["n"], "<f8"), (r["tris"], "<i4")) == k["sha256_v_n_tris"], \
-        "positions, normals or triangles differ from the compiled mc/ t_mc"
+        f"positions, normals or triangles differ from the {KNOWN[which]['source']}"
     assert t.min() >= 0 and t.max() < nv
     assert np.all(np.bincount(t.reshape(-1), minlength=nv) > 0), "unreferenced vertex"
     # every vertex on a grid edge (two coordinates on grid lines), except
@@ -164,4 +174,4 @@ def test_tmc_full_size_properties(eng):
             face |= (np.abs(p[:, ax] - plane) < tol) & (np.abs(q[:, ax] - plane) < tol)
     if r["face_failures"] == 0:
         assert np.all(face), f"{int((~face).sum())} open edges inside the volume"
-    _say(f"[gpu] TMC blobs_512: {len(starts)} edges, {len(one)} on the boundary, {len(many)} at inner vertices")
+    _say(f"[gpu] TMC {which}: {len(starts)} edges, {len(one)} on the boundary, {len(many)} at inner vertices")

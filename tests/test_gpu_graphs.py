@@ -1,0 +1,115 @@
+"""Captured passes: the second pass of a shape (volume dimensions, element
+type, flags, launch shape, capacities, buffers) is captured into a HIP
+graph and launched; later passes of that shape replay it with their own
+fields and iso-values, which the kernels read from the device volume
+table.  Every replay must equal the pass launched kernel by kernel, bit
+for bit, with other work and synchronisations between the replays.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from meshcheck import assert_same
+
+pytestmark = pytest.mark.gpu
+
+import tmc_amd  # noqa: E402
+from tmc_amd import SIMPLIFY_ALL, TRIANGLES, workloads  # noqa: E402
+
+KEYS = ("v", "n", "quads", "tris", "he", "he_v", "bnd")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _eager_engine():
+    os.environ["TMC_GRAPHS"] = "0"
+    try:
+        return tmc_amd.Engine(0)
+    finally:
+        del os.environ["TMC_GRAPHS"]
+
+
+def test_replays_match_eager():
+    flags = SIMPLIFY_ALL | TRIANGLES
+    # two fields of one shape (they share a graph) and one of another
+    a = workloads.genus2(40)
+    b = workloads.schwarz(40, seed=9)
+    c = workloads.blobs((24, 28, 32), seed=3)
+    ref = _eager_engine()
+    exp = {}
+    for name, (f, g, iso) in (("a", a), ("b", b), ("c", c)):
+        m = ref.extract(f, iso, g, flags=flags)
+        assert m.pass_mode == 0
+        exp[name] = (m.summary(), ref.download())
+    ref.close()
+    eng = tmc_amd.Engine(0)
+    try:
+        # the context's capacities follow the densest volume it has seen:
+        # one pass of each first
+        for f, g, iso in (a, b, c):
+            eng.extract(f, iso, g, flags=flags)
+        modes = []
+        order = ["a", "a", "c", "b", "a", "c", "c", "b", "a", "b"]
+        for i, name in enumerate(order):
+            f, g, iso = {"a": a, "b": b, "c": c}[name]
+            m = eng.extract(f, iso, g, flags=flags)
+            modes.append((name, m.pass_mode))
+            s = m.summary()
+            es, e = exp[name]
+            for k in ("nv", "nq", "nt", "nbnd", "nonmanifold_edges", "failed_projections_l1", "active_cells",
+                      "nv_cells", "nq_cells"):
+                assert s[k] == es[k], f"pass {i} ({name}, mode {m.pass_mode}): {k}"
+            d = eng.download()
+            for k in KEYS:
+                assert_same(d[k], e[k], f"pass {i} ({name}, mode {m.pass_mode}): {k}")
+            eng.synchronize()
+        _say(f"[gpu] pass modes {modes}")
+        # a: eager, captured, then replayed; b (same shape as a) replays a's graph
+        assert modes[0][1] == 0 and modes[1][1] == 1
+        assert modes[3][1] == 2 and modes[4][1] == 2
+        assert sum(1 for _, m in modes if m == 2) >= 5
+    finally:
+        eng.close()
+
+
+def test_batch_replays_match_eager():
+    flags = SIMPLIFY_ALL | TRIANGLES | tmc_amd.FIELD_ON_DEVICE
+    specs = workloads.batch_specs(48, seed=21)
+    vols = workloads.make_many(specs)
+    eng = tmc_amd.Engine(0)
+    try:
+        offs, total = [], 0
+        for f, _, _ in vols:
+            offs.append(total)
+            total += (f.nbytes + 255) // 256 * 256
+        host = np.zeros(total, np.uint8)
+        for (f, _, _), o in zip(vols, offs):
+            host[o:o + f.nbytes] = np.frombuffer(f.tobytes(), np.uint8)
+        buf = eng.alloc(total).upload(host)
+        dv = [(buf.ptr.value + o, iso, g) for o, (f, g, iso) in zip(offs, vols)]
+        ms = eng.extract_batch(dv, flags=flags)
+        first = [eng.download_batch(i) for i in range(len(dv))]
+        modes = [ms[0].pass_mode]
+        for rep in range(4):
+            ms = eng.extract_batch(dv, flags=flags)
+            modes.append(ms[0].pass_mode)
+            # other work between the replays
+            f, g, iso = workloads.genus2(20)
+            eng.extract(f, iso, g, flags=flags & ~tmc_amd.FIELD_ON_DEVICE)
+            eng.synchronize()
+            ms = eng.extract_batch(dv, flags=flags)
+            modes.append(ms[0].pass_mode)
+            for i in range(len(dv)):
+                got = eng.download_batch(i)
+                for k in KEYS:
+                    assert_same(got[k], first[i][k], f"replay {rep} volume {i}: {k}")
+        _say(f"[gpu] batch pass modes {modes}")
+        assert 2 in modes
+        buf.free()
+    finally:
+        eng.close()

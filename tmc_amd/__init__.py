@@ -43,7 +43,8 @@ EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc
            "dmc_server_last_error", "dmc_mesh_download_f32", "dmc_stage_times", "dmc_device_alloc",
            "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
            "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj",
-           "tmc_extract", "tmc_download", "dmc_get_launch", "dmc_set_launch", "dmc_autotune"]
+           "tmc_extract", "tmc_download", "dmc_get_launch", "dmc_set_launch", "dmc_autotune",
+           "dmc_context_bytes"]
 
 
 class DMCError(RuntimeError):
@@ -84,10 +85,15 @@ class Mesh(C.Structure):
                 ("invalid_refs", C.c_int64), ("valence_zero", C.c_int64),
                 ("failed_projections_l1", C.c_uint32), ("failed_projections_l2", C.c_uint32),
                 ("active_cells", C.c_int64), ("nv_emitted", C.c_int64), ("nv_cells", C.c_int64),
-                ("nq_cells", C.c_int64)]
+                ("nq_cells", C.c_int64), ("host_syncs", C.c_int64), ("pass_mode", C.c_int64)]
+
+    # how the pass ran, not what it produced
+    RUN_FIELDS = ("host_syncs", "pass_mode")
 
     def summary(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("d_")}
+        """Counts and diagnostics of the mesh (device pointers and the
+        run fields left out)."""
+        return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("d_") and f not in self.RUN_FIELDS}
 
 
 class Launch(C.Structure):
@@ -125,6 +131,7 @@ def load_library(path=ENGINE_SO):
     L.dmc_extract.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, vp, C.POINTER(Mesh)]
     L.dmc_mesh_download.argtypes = [vp, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     L.dmc_extract_batch.argtypes = [vp, C.POINTER(Volume), C.c_int32, C.c_int, C.c_uint32, C.POINTER(Mesh)]
+    L.dmc_context_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.dmc_batch_download.argtypes = [vp, C.c_int32, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     u64 = C.c_uint64
     L.dmc_server_create.argtypes = [C.POINTER(vp), C.POINTER(ServerOpts)]
@@ -166,7 +173,7 @@ def load_library(path=ENGINE_SO):
                  "dmc_server_stats", "dmc_mesh_download_f32",
                  "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
                  "dmc_copy_to_host", "dmc_synchronize", "dmc_mesh_edges", "dmc_valence_histogram",
-                 "dmc_element_quality", "dmc_write_obj"):
+                 "dmc_element_quality", "dmc_write_obj", "dmc_context_bytes"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -174,6 +181,14 @@ def load_library(path=ENGINE_SO):
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def _check_size(arr, grid, what="field", per_point=1):
+    """A host array must hold exactly the grid's points (times per_point):
+    the engine copies nx*ny*nz elements from its address."""
+    need = int(grid.nx) * int(grid.ny) * int(grid.nz) * per_point
+    if arr.size != need:
+        raise DMCError(f"{what} has {arr.size} elements, the grid {grid.nx}x{grid.ny}x{grid.nz} needs {need}")
 
 
 def _download(m, halfedges, call):
@@ -276,11 +291,13 @@ class Engine:
             ftype = FIELD_F32 if arr.dtype == np.float32 else FIELD_F64
             if grid is None:
                 grid = Grid.make(arr.shape)
+            _check_size(arr, grid)
             fptr = C.c_void_p(arr.ctypes.data)
             keep = [arr]
             gptr = None
             if gradients is not None:
                 g = np.ascontiguousarray(gradients, dtype=np.float64)
+                _check_size(g, grid, "gradients", 3)
                 keep.append(g)
                 gptr = C.c_void_p(g.ctypes.data)
         if gradients is not None:
@@ -317,6 +334,7 @@ class Engine:
             ftype = FIELD_F32 if arr.dtype == np.float32 else FIELD_F64
             if grid is None:
                 grid = Grid.make(arr.shape)
+            _check_size(arr, grid)
             fptr = C.c_void_p(arr.ctypes.data)
             keep = [arr]
         if not isinstance(grid, Grid):
@@ -383,6 +401,7 @@ class Engine:
                 ptr = arr.ctypes.data
                 if grid is None:
                     grid = Grid.make(arr.shape)
+                _check_size(arr, grid, f"field of volume {i}")
             if not isinstance(grid, Grid):
                 grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
             vols[i].grid = grid
@@ -404,6 +423,12 @@ class Engine:
         self.mesh = None
         self.batch = list(meshes)
         return self.batch
+
+    def context_bytes(self):
+        """Device bytes held by this context's buffer pools."""
+        n = C.c_size_t()
+        self._check(self.lib.dmc_context_bytes(self.ctx, C.byref(n)))
+        return n.value
 
     def download_batch(self, index, halfedges=True):
         """Copy volume `index` of the last extract_batch to host arrays."""
@@ -429,11 +454,12 @@ class Engine:
         return out
 
     def stage_times(self):
-        n = C.c_int(64)
-        names = (C.c_char_p * 64)()
-        ms = (C.c_float * 64)()
+        cap = 256
+        n = C.c_int(cap)
+        names = (C.c_char_p * cap)()
+        ms = (C.c_float * cap)()
         self._check(self.lib.dmc_stage_times(self.ctx, names, ms, C.byref(n)))
-        return {names[i].decode(): float(ms[i]) for i in range(min(n.value, 64))}
+        return {names[i].decode(): float(ms[i]) for i in range(min(n.value, cap))}
 
     def synchronize(self):
         self._check(self.lib.dmc_synchronize(self.ctx))
@@ -509,6 +535,7 @@ class Server:
             vol.field = keep.ctypes.data
             if grid is None:
                 grid = Grid.make(keep.shape)
+            _check_size(keep, grid)
         if not isinstance(grid, Grid):
             grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
         vol.grid = grid

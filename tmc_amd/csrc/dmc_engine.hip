@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/tmc_dmc.h"
@@ -41,6 +42,11 @@ constexpr int kGridCap = 16384;
 constexpr int kSweepWaves = 32768;
 constexpr int64_t kGridCapMin = 64, kGridCapMax = 1 << 24;
 constexpr int64_t kSweepWavesMin = 256, kSweepWavesMax = 1 << 24;
+// captured passes kept per context
+constexpr size_t kMaxGraphs = 16;
+// k_vertices blocks: 256 CUs x 2 resident blocks (40 KB of LDS, 256 VGPRs
+// each) x 4 rounds
+constexpr int64_t kVerticesBlocks = 2048;
 
 struct StageMark {
     const char* name;
@@ -59,7 +65,7 @@ struct dmc_context {
     std::vector<Buf*> bufs;
     // cell sweep
     Buf segCount, segBase, segMask, lists, acell, acode, aaux, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
-    Buf field, grads, fails, scanPartial, scanTotal;
+    Buf field, grads, scanPartial, scanTotal, sizes;
     // volume table of the last extraction (one volume, or a batch) and the
     // per-volume first active cell / vertex / quadrilateral, kept through
     // compactions
@@ -83,17 +89,33 @@ struct dmc_context {
     Buf tcntv, tcntt, tvofs, ttofs, tamb, tvb, tstats, tpos, tnrm, ttris;
     tmc_mesh tmesh{};
     bool have_tmesh = false;
-    uint32_t last_bad = 0;   // invalid vertex references seen by the last halfedge build
     // pinned host staging of device-to-host reads (counts, totals, tables)
     void* pin = nullptr;
     size_t pin_cap = 0;
-    // deferred reads: device values copied into pinned slots without
-    // waiting, valid once a later stream synchronisation has returned
-    void* dpin = nullptr;
     int64_t syncs = 0;         // stream synchronisations of HostReads so far
-    int64_t bad_epoch = -1;    // halfedge stats of a non-final build pending in slot 0
-    int64_t rows_epoch = -1;   // first CSR's row total pending in slot 1 ...
-    uint64_t rows_expect = 0;  // ... and the value it must have (4 Q)
+    int64_t syncs_at_start = 0;   // ... when the current extraction began
+    // capacities of the pass's arrays (active cells, vertices,
+    // quadrilaterals, 3-face records), planned from the largest counts per
+    // cell seen (ra, rv, rq) and raised to exact sizes after an overflow
+    int64_t acap = 0, vcap = 0, qcap = 0, reccap = 0;
+    int64_t acap_min = 0, vcap_min = 0, qcap_min = 0;
+    double ra = 0, rv = 0, rq = 0;
+    Sizes hsz{};               // the last pass's sizes block
+    // captured passes (replayed when a pass of the same shape comes again)
+    struct PassGraph {
+        std::string key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t use = 0;
+        int cur = 0, pcur = 0, scur = 0, csr_ready = 0;
+        bool slots = false, off_swapped = false, adj_swapped = false;
+    };
+    std::vector<PassGraph> graphs;
+    std::string last_key;      // shape of the last pass run launch by launch
+    uint64_t buf_epoch = 0;    // bumped by every buffer (re)allocation
+    uint64_t graph_clock = 0;
+    bool use_graphs = true;
+    bool capturing = false;
+    int pass_mode = 0;         // the last pass: 0 launched one by one, 1 captured, 2 replayed
     // results
     dmc_mesh mesh{};
     bool have_mesh = false;
@@ -104,11 +126,20 @@ struct dmc_context {
     // stage timing
     bool timing = false;
     std::vector<StageMark> marks;
+    // kernel timers: events recorded right before and after one kernel (or
+    // one scan), summed per name
+    struct KMark {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<KMark> kmarks;
+    std::vector<hipEvent_t> evpool;   // events reused across extractions
+    size_t evused = 0;
     std::vector<std::pair<const char*, float>> stage_ms;
 
     dmc_context() {
         Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &aaux, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
-                      &field, &grads, &fails, &scanPartial, &scanTotal, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
+                      &field, &grads, &scanPartial, &scanTotal, &sizes, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
@@ -145,6 +176,8 @@ template <class T>
 T* ensure(dmc_context* ctx, Buf& b, size_t n) {
     const size_t bytes = std::max<size_t>(n * sizeof(T), 256);
     if (b.cap < bytes) {
+        if (ctx->capturing) fail(ctx, DMC_ERR_STATE, "a buffer would grow inside a captured pass");
+        ctx->buf_epoch++;
         if (b.p) {
             HIPC(ctx, hipStreamSynchronize(ctx->stream));
             HIPC(ctx, hipFree(b.p));
@@ -169,12 +202,43 @@ int grid_for(const dmc_context* ctx, int64_t n, int per_block = kBlock, int64_t 
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
+hipEvent_t take_event(dmc_context* ctx) {
+    if (ctx->evused == ctx->evpool.size()) {
+        hipEvent_t e = nullptr;
+        HIPC(ctx, hipEventCreate(&e));
+        ctx->evpool.push_back(e);
+    }
+    return ctx->evpool[ctx->evused++];
+}
+
+// Stage boundary: a stage's time runs to the next mark.
 void mark(dmc_context* ctx, const char* name) {
     if (!ctx->timing) return;
-    StageMark m{name, nullptr};
-    HIPC(ctx, hipEventCreate(&m.ev));
+    StageMark m{name, take_event(ctx)};
     HIPC(ctx, hipEventRecord(m.ev, ctx->stream));
     ctx->marks.push_back(m);
+}
+
+// Kernel timer around the launches between kbegin and kend (same stream).
+int kbegin(dmc_context* ctx, const char* name) {
+    if (!ctx->timing) return -1;
+    dmc_context::KMark k{name, take_event(ctx), take_event(ctx)};
+    HIPC(ctx, hipEventRecord(k.a, ctx->stream));
+    ctx->kmarks.push_back(k);
+    return (int)ctx->kmarks.size() - 1;
+}
+
+void kend(dmc_context* ctx, int k) {
+    if (k >= 0) HIPC(ctx, hipEventRecord(ctx->kmarks[k].b, ctx->stream));
+}
+
+// Every kernel launch of the engine: on the context's stream, timed under
+// `name` when the extraction records stage times.
+template <class K, class... Args>
+void launch(dmc_context* ctx, const char* name, K kernel, dim3 grid, dim3 block, Args&&... args) {
+    const int kt = kbegin(ctx, name);
+    kernel<<<grid, block, 0, ctx->stream>>>(std::forward<Args>(args)...);
+    kend(ctx, kt);
 }
 
 void check_launch(dmc_context* ctx, const char* what) {
@@ -239,79 +303,50 @@ uint64_t read_u64(dmc_context* ctx, const void* dptr) {
     return v;
 }
 
-// Deferred reads: `bytes` (at most kDeferSlot) of device memory copied into
-// pinned slot `slot` without waiting.  Returns the synchronisation count at
-// the copy; the slot is valid once a later synchronisation has returned.
-constexpr size_t kDeferSlot = 256;
-int64_t defer_read(dmc_context* ctx, int slot, const void* src, size_t bytes) {
-    if (!ctx->dpin) {
-        if (hipHostMalloc(&ctx->dpin, 4 * kDeferSlot) != hipSuccess) {
-            (void)hipGetLastError();
-            ctx->dpin = nullptr;
-            fail(ctx, DMC_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed", 4 * kDeferSlot);
-        }
-    }
-    HIPC(ctx, hipMemcpyAsync(static_cast<char*>(ctx->dpin) + slot * kDeferSlot, src, bytes, hipMemcpyDeviceToHost,
-                             ctx->stream));
-    return ctx->syncs;
+// Device sizes block of the running pass (allocated with the pass).
+Sizes* dsizes(dmc_context* ctx) { return static_cast<Sizes*>(ctx->sizes.p); }
+
+DevN dn(const uint64_t* p, int64_t cap) { return DevN{p, cap}; }
+DevN hn(int64_t n) { return DevN{nullptr, n}; }
+
+// Fill n (device count, capped) 32-bit words or bytes with a value.
+void fill_u32(dmc_context* ctx, void* p, DevN n, uint32_t value, const uint32_t* gate = nullptr) {
+    if (n.cap <= 0) return;
+    launch(ctx, "k_fill", k_fill_u32, dim3(grid_for(ctx, n.cap)), dim3(kBlock), static_cast<uint32_t*>(p), n, value,
+           gate);
+}
+void fill_u8(dmc_context* ctx, void* p, DevN n, uint8_t value) {
+    if (n.cap <= 0) return;
+    launch(ctx, "k_fill", k_fill_u8, dim3(grid_for(ctx, n.cap)), dim3(kBlock), static_cast<uint8_t*>(p), n, value);
 }
 
-// A deferred slot's contents, synchronising first if nothing has since
-// the copy.
-const void* deferred(dmc_context* ctx, int slot, int64_t epoch) {
-    if (ctx->syncs <= epoch) {
-        HIPC(ctx, hipStreamSynchronize(ctx->stream));
-        ctx->syncs++;
-    }
-    return static_cast<const char*>(ctx->dpin) + slot * kDeferSlot;
-}
-
-// Device slots for scan totals: 0 for the scans whose total is read at
-// once, the others for totals kept on the device or read later.
-enum TotalSlot { kTotRead = 0, kTotV, kTotQ, kTotRows, kTotRemap, kTotSlots };
-uint64_t* totals(dmc_context* ctx) { return ensure<uint64_t>(ctx, ctx->scanTotal, kTotSlots); }
-
-// Exclusive scan of n uint32 counts into out; the 64-bit total goes to
-// *dtotal (device memory) and the host does not wait for it.
+// Exclusive scan of n.get() uint32 items into out (launched for n.cap); the
+// 64-bit total goes to *dtotal (device memory).  Nothing runs when *gate is
+// 0.
 template <class In>
-void scan_to(dmc_context* ctx, In in, uint32_t* out, int64_t n, uint64_t* dtotal) {
-    if (n <= 0) {
-        HIPC(ctx, hipMemsetAsync(dtotal, 0, sizeof(uint64_t), ctx->stream));
-        return;
-    }
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
-    hipLaunchKernelGGL(k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, n, part);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, tiles, dtotal);
-    hipLaunchKernelGGL(k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, in, out, n, part);
+void scan_to(dmc_context* ctx, In in, uint32_t* out, DevN n, uint64_t* dtotal, const char* name = "k_scan",
+             const uint32_t* gate = nullptr) {
+    const int64_t tiles = (std::max<int64_t>(n.cap, 0) + kScanTile - 1) / kScanTile;
+    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, std::max<int64_t>(tiles, 1));
+    if (tiles > 0) launch(ctx, name, k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), in, n, gate, part);
+    launch(ctx, name, k_scan_partials, dim3(1), dim3(1024), part, tiles, gate, dtotal);
+    if (tiles > 0) launch(ctx, name, k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, part);
     check_launch(ctx, "scan");
 }
 
-// Exclusive scan of n uint32 counts into out; returns the 64-bit total.
+// Host-counted scan with the total read back (diagnostics and the
+// topologically correct MC, which size their launches on the host).
 template <class In>
-uint64_t scan_any(dmc_context* ctx, In in, uint32_t* out, int64_t n) {
+uint64_t scan_read(dmc_context* ctx, In in, uint32_t* out, int64_t n) {
     if (n <= 0) return 0;
-    uint64_t* tot = totals(ctx) + kTotRead;
-    scan_to(ctx, in, out, n, tot);
+    uint64_t* tot = ensure<uint64_t>(ctx, ctx->scanTotal, 1);
+    scan_to(ctx, in, out, hn(n), tot);
     return read_u64(ctx, tot);
 }
 
-// off[n] = total of a scan held on the device (low 32 bits).
-void set_row_end(dmc_context* ctx, uint32_t* off, int64_t n, const uint64_t* dtotal) {
-    HIPC(ctx, hipMemcpyAsync(off + n, dtotal, sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
-}
-
-// Exclusive scan of uint32 counts; returns the total.
 uint64_t scan(dmc_context* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
-    return scan_any(ctx, ScanCounts{in}, out, n);
+    return scan_read(ctx, ScanCounts{in}, out, n);
 }
-
-// New indices of the kept items (exclusive scan of removed[i] == 0);
-// returns the number kept.
-uint64_t scan_keep(dmc_context* ctx, const uint8_t* removed, uint32_t* out, int64_t n) {
-    return scan_any(ctx, ScanKeep{removed}, out, n);
-}
-
 
 // ---------------------------------------------------------------------------
 // cell stage
@@ -344,24 +379,30 @@ VolTable vol_table(dmc_context* ctx) {
 
 // What the field sweep leaves for the cell stages.
 struct Swept {
-    uint64_t A;     // active cells
     int64_t R;      // cell rows (all volumes)
+    int64_t cells;  // cells (all volumes)
     VolTable vt;
     VolStat* vstat;
     uint32_t *lists, *segCount, *segBase;
     unsigned long long* segMask;
 };
 
-// Volume table, classification sweep and the scan of the active cells --
-// shared by Dual Marching Cubes and the topologically correct MC.
+// Host side of a pass: the volume table (built in ctx->vtab_host, uploaded
+// by upload_vtab) and the sweep's tiling.
+struct SweepPlan {
+    int64_t W, R, RS, cells;
+    bool vec;
+    size_t tab_bytes;
+};
+
 template <class T>
-Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
+SweepPlan plan_sweep(dmc_context* ctx, const std::vector<HostVol>& hv) {
     const int n = (int)hv.size();
     ctx->nvol = n;
     // sweep tiling: J rows per wave; one plane count K for the whole
     // extraction, chosen so that about ctx->sweep_waves waves fill the chip
     std::vector<VolDesc> desc(n);
-    int64_t units = 0;
+    int64_t units = 0, cells = 0;
     for (int v = 0; v < n; v++) {
         const dmc_grid& g = hv[v].g;
         VolDesc& D = desc[v];
@@ -374,6 +415,7 @@ Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
         D.S = (int)((D.d.C + kSeg - 1) / kSeg);
         D.ntj = (g.ny - 1 + kSweepRows - 1) / kSweepRows;
         units += (int64_t)(g.nz - 1) * D.S * D.ntj;
+        cells += D.d.C * D.d.R;
     }
     const int Kg = (int)std::max<int64_t>(8, (units + ctx->sweep_waves - 1) / ctx->sweep_waves);
     const size_t tab_bytes = (size_t)n * sizeof(VolDesc) + 3 * (size_t)(n + 1) * sizeof(int64_t);
@@ -399,11 +441,29 @@ Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     rowBase[n] = R;
     memcpy(ctx->vtab_host.data(), desc.data(), (size_t)n * sizeof(VolDesc));
     ensure<uint8_t>(ctx, ctx->vtab, tab_bytes);
-    // pageable source: the copy has been staged when the call returns
-    HIPC(ctx, hipMemcpyAsync(ctx->vtab.p, ctx->vtab_host.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+    return SweepPlan{W, R, RS, cells, vec, tab_bytes};
+}
+
+// The volume table to the device (before the pass's launches; a replayed
+// pass reads it from there).  Pageable source: the copy has been staged
+// when the call returns.
+void upload_vtab(dmc_context* ctx, const SweepPlan& P) {
+    HIPC(ctx, hipMemcpyAsync(ctx->vtab.p, ctx->vtab_host.data(), P.tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+
+// Sizes block, classification sweep and the scan of the active cells
+// (total to S->A) -- shared by Dual Marching Cubes and the topologically
+// correct MC.
+template <class T>
+Swept sweep_cells(dmc_context* ctx, const SweepPlan& P) {
+    const int n = ctx->nvol;
+    const int64_t W = P.W, R = P.R, RS = P.RS;
     const VolTable vt = vol_table(ctx);
     VolStat* vstat = ensure<VolStat>(ctx, ctx->vstat, n);
-    HIPC(ctx, hipMemsetAsync(vstat, 0, (size_t)n * sizeof(VolStat), ctx->stream));
+    Sizes* S = reinterpret_cast<Sizes*>(ensure<uint8_t>(ctx, ctx->sizes, sizeof(Sizes)));
+    static_assert(sizeof(Sizes) % 4 == 0 && sizeof(VolStat) % 4 == 0, "filled as words");
+    fill_u32(ctx, S, hn(sizeof(Sizes) / 4), 0u);
+    fill_u32(ctx, vstat, hn((int64_t)n * sizeof(VolStat) / 4), 0u);
 
     uint32_t* segCount = ensure<uint32_t>(ctx, ctx->segCount, RS);
     uint32_t* segBase = ensure<uint32_t>(ctx, ctx->segBase, RS);
@@ -412,368 +472,279 @@ Swept sweep_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
     mark(ctx, "sweep");
     const int64_t sblocks = (W + kBlock / kWave - 1) / (kBlock / kWave);
     const dim3 sgrid((unsigned)((sblocks + kXcds - 1) / kXcds * kXcds));
-    if (vec)
-        hipLaunchKernelGGL((k_sweep<T, true>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount, segMask);
+    if (P.vec)
+        launch(ctx, "k_sweep", (k_sweep<T, true>), sgrid, dim3(kBlock), vt, W, lists, segCount, segMask);
     else
-        hipLaunchKernelGGL((k_sweep<T, false>), sgrid, dim3(kBlock), 0, ctx->stream, vt, W, lists, segCount,
-                           segMask);
+        launch(ctx, "k_sweep", (k_sweep<T, false>), sgrid, dim3(kBlock), vt, W, lists, segCount, segMask);
     check_launch(ctx, "k_sweep");
     mark(ctx, "rows");
-    const uint64_t A = scan(ctx, segCount, segBase, RS);
-    if (A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)A);
-    ctx->mesh.active_cells = (int64_t)A;
-    hipLaunchKernelGGL(k_vol_abase, dim3(grid_for(ctx, n + 1)), dim3(kBlock), 0, ctx->stream, vt, segBase, (int64_t)A,
-                       vol_ptr(ctx, 2));
-    return Swept{A, R, vt, vstat, lists, segCount, segBase, segMask};
+    scan_to(ctx, ScanCounts{segCount}, segBase, hn(RS), &S->A, "k_scan_cells");
+    launch(ctx, "k_vol_abase", k_vol_abase, dim3(grid_for(ctx, n + 1)), dim3(kBlock), vt, segBase, &S->A,
+           vol_ptr(ctx, 2));
+    return Swept{R, P.cells, vt, vstat, lists, segCount, segBase, segMask};
+}
+
+// Capacities of the cell-stage arrays for a pass over `cells` cells: the
+// largest count per cell this context has seen (with a margin), at least a
+// default share; a pass that exceeds one is run again with the exact size.
+// Capacities are rounded up to one of 8 steps per power of two, so that
+// passes of one shape keep the same capacities (and buffers) while the
+// context's ratios creep up.
+int64_t cap_step(int64_t x) {
+    if (x <= 4096) return 4096;
+    int b = 63 - __builtin_clzll((unsigned long long)x);
+    const int64_t unit = (int64_t)1 << (b - 3);
+    return (x + unit - 1) / unit * unit;
+}
+
+void plan_caps(dmc_context* ctx, int64_t cells) {
+    auto cap = [&](double ratio) {
+        const double r = std::max(ratio * 1.02, 0.02);
+        return cap_step((int64_t)std::ceil((double)cells * r) + 1024);
+    };
+    ctx->acap = std::max(cap_step(ctx->acap_min), cap(ctx->ra));
+    ctx->vcap = std::max(cap_step(ctx->vcap_min), cap(ctx->rv));
+    ctx->qcap = std::max(cap_step(ctx->qcap_min), cap(ctx->rq));
+    ctx->acap = std::min<int64_t>(std::min<int64_t>(ctx->acap, cells), 0xFFFFFFFF);
+    ctx->vcap = std::min<int64_t>(ctx->vcap, 0x7FFFFFFF);
+    ctx->qcap = std::min<int64_t>(ctx->qcap, 0x1FFFFFFF);
+    ctx->reccap = std::max<int64_t>(ctx->reccap, 1024);
 }
 
 template <class T>
-void run_cells(dmc_context* ctx, const std::vector<HostVol>& hv) {
-    const Swept sw = sweep_cells<T>(ctx, hv);
-    const int n = (int)hv.size();
-    const uint64_t A = sw.A;
+void run_cells(dmc_context* ctx, const SweepPlan& P) {
+    // canonical buffer roles at the start of every pass, so that a pass of
+    // the same shape launches on the same pointers (and can be replayed)
+    if (ctx->off.p > ctx->off2.p) std::swap(ctx->off, ctx->off2);
+    if (ctx->adj.p > ctx->adj2.p) std::swap(ctx->adj, ctx->adj2);
+    const Swept sw = sweep_cells<T>(ctx, P);
+    const int n = ctx->nvol;
+    const int64_t acap = ctx->acap, vcap = ctx->vcap, qcap = ctx->qcap;
     const int64_t R = sw.R;
     const VolTable vt = sw.vt;
-    VolStat* vstat = sw.vstat;
-    uint32_t *lists = sw.lists, *segCount = sw.segCount, *segBase = sw.segBase;
-    unsigned long long* segMask = sw.segMask;
-    uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
-    uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
-    uint32_t* aaux = ensure<uint32_t>(ctx, ctx->aaux, A);
-    uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, A);
-    uint32_t* aqloc = ensure<uint32_t>(ctx, ctx->aqloc, A);
+    Sizes* S = dsizes(ctx);
+    uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, acap);
+    uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, acap);
+    uint32_t* aaux = ensure<uint32_t>(ctx, ctx->aaux, acap);
+    uint32_t* avloc = ensure<uint32_t>(ctx, ctx->avloc, acap);
+    uint32_t* aqloc = ensure<uint32_t>(ctx, ctx->aqloc, acap);
     uint32_t* rowV = ensure<uint32_t>(ctx, ctx->rowV, R);
     uint32_t* rowQ = ensure<uint32_t>(ctx, ctx->rowQ, R);
     uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
     uint32_t* rowBaseQ = ensure<uint32_t>(ctx, ctx->rowBaseQ, R);
-    // fails[4]: vertices of valence 0 (all valence passes); fails[5]: quads
-    // with an unresolved neighbour cell
-    uint32_t* fails = ensure<uint32_t>(ctx, ctx->fails, 6);
-    HIPC(ctx, hipMemsetAsync(fails, 0, 6 * sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_rows<T>, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, vt, R, lists,
-                       segCount, segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, vstat);
+    launch(ctx, "k_rows", k_rows<T>, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), vt, R, sw.lists,
+           sw.segCount, sw.segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, sw.vstat, (uint32_t)acap);
     check_launch(ctx, "k_rows");
-    uint64_t* tot = totals(ctx);
-    scan_to(ctx, ScanCounts{rowV}, rowBaseV, R, tot + kTotV);
-    scan_to(ctx, ScanCounts{rowQ}, rowBaseQ, R, tot + kTotQ);
-    uint64_t V = 0, Q = 0;
-    {
-        HostReads rd(ctx);
-        rd.add(&V, tot + kTotV, sizeof(V));
-        rd.add(&Q, tot + kTotQ, sizeof(Q));
-        rd.wait();
-    }
-    if (V > 0x7FFFFFFFull || Q > 0x1FFFFFFFull)
-        fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)V, (unsigned long long)Q);
+    scan_to(ctx, ScanCounts{rowV}, rowBaseV, hn(R), &S->V, "k_scan_cells");
+    scan_to(ctx, ScanCounts{rowQ}, rowBaseQ, hn(R), &S->Q, "k_scan_cells");
+    launch(ctx, "k_cell_guard", k_cell_guard, dim3(1), dim3(64), S, (uint64_t)acap, (uint64_t)vcap, (uint64_t)qcap);
     // [0, n]: live bounds; [n+1, 2n+1]: the cell stage's (before simplification)
     int64_t* vb = ensure<int64_t>(ctx, ctx->vbound, 2 * (size_t)(n + 1));
     int64_t* qb = ensure<int64_t>(ctx, ctx->qbound, 2 * (size_t)(n + 1));
-    hipLaunchKernelGGL(k_vol_bounds, dim3(grid_for(ctx, n + 1)), dim3(kBlock), 0, ctx->stream, vt, rowBaseV, rowBaseQ,
-                       (int64_t)V, (int64_t)Q, vb, qb);
-    HIPC(ctx, hipMemcpyAsync(vb + n + 1, vb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
-    HIPC(ctx, hipMemcpyAsync(qb + n + 1, qb, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
+    launch(ctx, "k_vol_bounds", k_vol_bounds, dim3(grid_for(ctx, n + 1)), dim3(kBlock), vt, rowBaseV, rowBaseQ, S,
+           vb, qb);
     ctx->cur = 0;
     ctx->pcur = 0;
     ctx->scur = 0;
     ctx->slots = false;
-    double* pos = ensure<double>(ctx, ctx->pos[0], 3 * V);
-    double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * V);
-    int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * Q);
-    int32_t* colors = ensure<int32_t>(ctx, ctx->colors, Q);
+    double* pos = ensure<double>(ctx, ctx->pos[0], 3 * vcap);
+    double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * vcap);
+    int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * qcap);
+    int32_t* colors = ensure<int32_t>(ctx, ctx->colors, qcap);
     // first halfedge CSR straight from the cells: degree of each vertex from
     // k_vertices, rows filled by k_quads
-    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, V);
-    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, V + 1);
-    int2* adj = ensure<int2>(ctx, ctx->adj, 4 * Q);
+    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, vcap);
+    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, vcap + 1);
+    int2* adj = ensure<int2>(ctx, ctx->adj, 4 * qcap);
     // cells deferred to the general sampling: slow[0] = count, then cell ids
-    uint32_t* slow = ensure<uint32_t>(ctx, ctx->slow, A + 1);
+    uint32_t* slow = ensure<uint32_t>(ctx, ctx->slow, acap + 1);
+    const DevN A = dn(&S->acnt, acap);
     mark(ctx, "vertices");
-    if (A > 0) {
-        HIPC(ctx, hipMemsetAsync(slow, 0, sizeof(uint32_t), ctx->stream));
-        hipLaunchKernelGGL(k_vertices<T>, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux,
-                           avloc, rowBaseV, (int64_t)A, pos, nrm, deg, slow);
-        check_launch(ctx, "k_vertices");
-        hipLaunchKernelGGL(k_vertices_slow<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), 0,
-                           ctx->stream, vt, acell, acode, aaux, avloc, rowBaseV, slow, pos, nrm, deg, vstat);
-        check_launch(ctx, "k_vertices_slow");
-    }
+    fill_u32(ctx, slow, hn(1), 0u);
+    // persistent: a few rounds of the blocks the chip holds at once
+    launch(ctx, "k_vertices", k_vertices<T>, dim3(grid_for(ctx, acap, kBlock, kVerticesBlocks)), dim3(kBlock), vt,
+           acell, acode, aaux, avloc, rowBaseV, A, pos, nrm, deg, slow, &S->qhead);
+    check_launch(ctx, "k_vertices");
+    launch(ctx, "k_vertices_slow", k_vertices_slow<T>, dim3(grid_for(ctx, acap, kBlock, 1024)), dim3(kBlock), vt,
+           acell, acode, aaux, avloc, rowBaseV, slow, pos, nrm, deg, sw.vstat);
+    check_launch(ctx, "k_vertices_slow");
     mark(ctx, "quads");
-    // the row total must be 4 Q; checked at the next synchronisation
-    scan_to(ctx, ScanCounts{deg}, off, (int64_t)V, tot + kTotRows);
-    set_row_end(ctx, off, (int64_t)V, tot + kTotRows);
-    ctx->rows_epoch = defer_read(ctx, 1, tot + kTotRows, sizeof(uint64_t));
-    ctx->rows_expect = 4 * Q;
-    if (A > 0) {
-        hipLaunchKernelGGL(k_quads, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, vt, acell, acode, aaux, avloc,
-                           aqloc,
-                           rowBaseV, rowBaseQ, (int64_t)A, segMask, segBase, quads, colors, off, adj, fails + 5);
-        check_launch(ctx, "k_quads");
-    }
+    // the row total must be 4 Q (checked with the pass's sizes at the end)
+    scan_to(ctx, ScanCounts{deg}, off, dn(&S->nv, vcap), &S->rows, "k_scan_cells");
+    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, dn(&S->nv, vcap), &S->rows,
+           (const uint32_t*)nullptr);
+    launch(ctx, "k_quads", k_quads, dim3(grid_for(ctx, acap)), dim3(kBlock), vt, acell, acode, aaux, avloc, aqloc,
+           rowBaseV, rowBaseQ, A, sw.segMask, sw.segBase, quads, colors, off, adj, (uint64_t)(4 * qcap),
+           &S->unresolved);
+    check_launch(ctx, "k_quads");
     ctx->csr_ready = 1;
-    ctx->mesh.nv = (int64_t)V;
-    ctx->mesh.nq = (int64_t)Q;
-    ctx->mesh.nv_cells = (int64_t)V;
-    ctx->mesh.nq_cells = (int64_t)Q;
 }
 
 // ---------------------------------------------------------------------------
 // halfedges
 // ---------------------------------------------------------------------------
-// Final pass extras shared by both halfedge paths: 3-face edges override
-// he_v and give the boundary vertices (sorted by key on the host).
-template <class ThreeLaunch>
-void three_face_edges(dmc_context* ctx, const HeStats& hs, int64_t nv, ThreeLaunch launch) {
-    ctx->bnd.clear();
-    if (hs.three == 0) return;
-    unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, nv);
-    BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, hs.three);
-    unsigned int* nrec = ensure<unsigned int>(ctx, ctx->nbnd, 1);
-    HIPC(ctx, hipMemsetAsync(ovr, 0, nv * sizeof(unsigned long long), ctx->stream));
-    HIPC(ctx, hipMemsetAsync(nrec, 0, sizeof(unsigned int), ctx->stream));
-    launch(ovr, recs, nrec, 0);
-    launch(ovr, recs, nrec, 1);
-    check_launch(ctx, "three-face edges");
-    std::vector<BndRec> h(hs.three);
-    HostReads r(ctx);
-    r.add(h.data(), recs, hs.three * sizeof(BndRec));
-    r.wait();
-    std::sort(h.begin(), h.end(), [](const BndRec& a, const BndRec& b) { return a.key < b.key; });
-    for (const BndRec& r : h) {
-        ctx->bnd.push_back(r.a);
-        ctx->bnd.push_back(r.b);
-    }
-}
-
-// The first CSR's row total against 4 Q, once it has arrived.
-void check_rows(dmc_context* ctx) {
-    if (ctx->rows_epoch < 0) return;
-    uint64_t rows = 0;
-    std::memcpy(&rows, deferred(ctx, 1, ctx->rows_epoch), sizeof(rows));
-    ctx->rows_epoch = -1;
-    if (rows != ctx->rows_expect)
-        fail(ctx, DMC_ERR_STATE, "halfedge rows %llu != 4 Q %llu", (unsigned long long)rows,
-             (unsigned long long)ctx->rows_expect);
-}
-
-// Invalid vertex references seen by the last halfedge build (a non-final
-// build's stats arrive deferred).
-uint32_t last_bad(dmc_context* ctx) {
-    if (ctx->bad_epoch >= 0) {
-        HeStats hs{};
-        std::memcpy(&hs, deferred(ctx, 0, ctx->bad_epoch), sizeof(hs));
-        ctx->bad_epoch = -1;
-        ctx->last_bad = hs.bad;
-    }
-    return ctx->last_bad;
-}
-
 // Halfedges through a CSR of outgoing (halfedge, destination) pairs: rows
-// off[v] .. off[v+1] (off has nv+1 entries).  A non-final build leaves its
-// stats on the device (read when the next stage needs them) and returns
-// zeros.
-HeStats build_halfedges(dmc_context* ctx, bool final_pass) {
-    const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq, nhe = 4 * nq;
+// off[v] .. off[v+1] (off has nv+1 entries).  The rows are those the cell
+// stage or the last remap left (csr_ready 1 / 2) unless the device finds
+// them incomplete (a quadrilateral with an unresolved neighbour cell, or a
+// remap skipped after invalid vertex references); then, and when nothing is
+// ready (0), the build counts them from the quadrilaterals (generic rows).
+void build_halfedges(dmc_context* ctx, bool final_pass) {
+    const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
+    Sizes* S = dsizes(ctx);
     const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
-    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, nv);
-    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, nv + 1);
-    uint32_t* cursor = ensure<uint32_t>(ctx, ctx->cursor, nv);
-    int2* adj = ensure<int2>(ctx, ctx->adj, nhe);
-    int32_t* twin = ensure<int32_t>(ctx, ctx->twin, nhe);
-    uint8_t* flag = ensure<uint8_t>(ctx, ctx->flag, nhe);
-    int32_t* hev = final_pass ? ensure<int32_t>(ctx, ctx->hev, nv) : nullptr;
-    HeStats* st = ensure<HeStats>(ctx, ctx->stats, 1);
-    HIPC(ctx, hipMemsetAsync(st, 0, sizeof(HeStats), ctx->stream));
-    HeStats hs{};
-    const int ready = ctx->csr_ready;
+    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, vcap);
+    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, vcap + 1);
+    uint32_t* cursor = ensure<uint32_t>(ctx, ctx->cursor, vcap);
+    int2* adj = ensure<int2>(ctx, ctx->adj, hcap);
+    int32_t* twin = ensure<int32_t>(ctx, ctx->twin, hcap);
+    uint8_t* flag = ensure<uint8_t>(ctx, ctx->flag, hcap);
+    int32_t* hev = final_pass ? ensure<int32_t>(ctx, ctx->hev, vcap) : nullptr;
+    launch(ctx, "k_rows_gate", k_rows_gate, dim3(1), dim3(64), S, ctx->csr_ready);
     ctx->csr_ready = 0;
-    bool fresh = ready != 0;
-    if (ready == 1) {
-        // rows from the cell stage are complete unless a quadrilateral had an
-        // unresolved neighbour cell
-        uint32_t unresolved = 0;
-        HostReads r(ctx);
-        r.add(&unresolved, static_cast<uint32_t*>(ctx->fails.p) + 5, sizeof(unresolved));
-        r.wait();
-        check_rows(ctx);
-        fresh = unresolved == 0;
-    } else if (ready == 2) {
-        // remapped rows: halfedges left without a valid origin were counted
-        HIPC(ctx, hipMemcpyAsync(&st->bad, ctx->rbad.p, sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    if (!fresh) {
-        if (nv > 0) {
-            HIPC(ctx, hipMemsetAsync(deg, 0, nv * sizeof(uint32_t), ctx->stream));
-            HIPC(ctx, hipMemsetAsync(cursor, 0, nv * sizeof(uint32_t), ctx->stream));
-        }
-        if (nhe > 0) {
-            hipLaunchKernelGGL(k_he_degree, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, deg,
-                               &st->bad);
-            check_launch(ctx, "k_he_degree");
-        }
-        const uint64_t rows = scan(ctx, deg, off, nv);
-        HIPC(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(off + nv), (int)(uint32_t)rows, 1, ctx->stream));
-        if (nhe > 0) {
-            hipLaunchKernelGGL(k_he_scatter, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, off,
-                               cursor, adj);
-            check_launch(ctx, "k_he_scatter");
-        }
-    }
+    // generic rows (only when S->generic)
+    const uint32_t* gen = &S->generic;
+    const DevN nv = dn(&S->nv, vcap);
+    fill_u32(ctx, deg, nv, 0u, gen);
+    fill_u32(ctx, cursor, nv, 0u, gen);
+    launch(ctx, "k_he_degree", k_he_degree, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, deg,
+           &S->he.bad, gen);
+    scan_to(ctx, ScanCounts{deg}, off, nv, &S->tot, "k_scan", gen);
+    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, nv, &S->tot, gen);
+    launch(ctx, "k_he_scatter", k_he_scatter, dim3(grid_for(ctx, qcap)), dim3(kBlock), quads, (const Sizes*)S, off,
+           cursor, adj, gen);
     unsigned long long* vst = nullptr;
     if (final_pass && ctx->nvol > 1) {
         vst = ensure<unsigned long long>(ctx, ctx->vhe, 2 * (size_t)ctx->nvol);
-        HIPC(ctx, hipMemsetAsync(vst, 0, 2 * (size_t)ctx->nvol * sizeof(unsigned long long), ctx->stream));
+        fill_u32(ctx, vst, hn(4 * (int64_t)ctx->nvol), 0u);
     }
-    if (nhe > 0) {
-        // deg holds every vertex's out-degree here; the twins add boundary
-        // arrivals, turning it into the valence
-        hipLaunchKernelGGL(k_he_twins, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off, adj,
-                           twin, flag, st, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst,
-                           final_pass ? nullptr : static_cast<uint32_t*>(ctx->deg.p));
-        check_launch(ctx, "k_he_twins");
-    }
-    if (final_pass && nv > 0) {
-        hipLaunchKernelGGL(k_he_v, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, off, adj, nv, hev);
-        check_launch(ctx, "k_he_v");
-    }
-    if (!final_pass) {
-        ctx->bad_epoch = defer_read(ctx, 0, st, sizeof(HeStats));
-        return hs;
-    }
-    {
-        HostReads r(ctx);
-        r.add(&hs, st, sizeof(hs));
-        r.wait();
-    }
-    ctx->bad_epoch = -1;
-    ctx->last_bad = hs.bad;
-    if (final_pass)
-        three_face_edges(ctx, hs, nv, [&](unsigned long long* ovr, BndRec* recs, unsigned int* nrec, int pass) {
-            hipLaunchKernelGGL(k_he_three, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream, quads, nhe, nv, off,
-                               adj, flag, ovr, recs, nrec, pass, hev);
-        });
-    return hs;
+    // deg holds every vertex's out-degree here; the twins add boundary
+    // arrivals, turning it into the valence
+    launch(ctx, "k_he_twins", k_he_twins, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, off, adj,
+           twin, flag, &S->he, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst,
+           final_pass ? nullptr : deg);
+    check_launch(ctx, "k_he_twins");
+    if (!final_pass) return;
+    launch(ctx, "k_he_v", k_he_v, dim3(grid_for(ctx, vcap)), dim3(kBlock), off, adj, (const Sizes*)S, hev);
+    // 3-face edges (when the build found any): he_v overrides and the
+    // boundary-vertex records, sorted on the host at the end
+    unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, vcap);
+    BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, ctx->reccap);
+    fill_u32(ctx, ovr, dn(&S->nv, vcap).cap > 0 ? DevN{nullptr, 2 * vcap} : hn(0), 0u, &S->he.pad);
+    for (int pass = 0; pass < 2; pass++)
+        launch(ctx, "k_he_three", k_he_three, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, off,
+               adj, flag, ovr, recs, &S->nrec, (uint32_t)ctx->reccap, &S->ovf, pass, hev);
+    check_launch(ctx, "k_he_three");
 }
 
 // Valence of the live mesh after a non-final halfedge build: the degree
 // array, completed by the twins pass (out-degree + boundary arrivals).
 int32_t* valence(dmc_context* ctx) {
-    const int64_t nv = ctx->mesh.nv;
     uint32_t* val = static_cast<uint32_t*>(ctx->deg.p);
-    unsigned int* zero = static_cast<unsigned int*>(ctx->fails.p) + 4;
-    if (nv > 0)
-        hipLaunchKernelGGL(k_count_zero, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, val, nv, zero);
+    Sizes* S = dsizes(ctx);
+    launch(ctx, "k_count_zero", k_count_zero, dim3(grid_for(ctx, ctx->vcap)), dim3(kBlock), val, (const Sizes*)S,
+           &S->zero);
     check_launch(ctx, "valence");
     return reinterpret_cast<int32_t*>(val);
 }
 
 void color_faces(dmc_context* ctx) {
-    const int64_t nq = ctx->mesh.nq;
-    if (nq == 0) return;
+    const int64_t qcap = ctx->qcap;
+    Sizes* S = dsizes(ctx);
     int32_t* colors = static_cast<int32_t*>(ctx->colors.p);
-    const int nb = (int)std::min<int64_t>(kColorBlocks, (nq + kBlock - 1) / kBlock);
-    uint32_t* hist = ensure<uint32_t>(ctx, ctx->ccnt, (size_t)kColorBlocks * 24 + 48);
-    uint32_t* totals = hist + (size_t)kColorBlocks * 24;
-    int32_t* list = ensure<int32_t>(ctx, ctx->clist, nq);
-    hipLaunchKernelGGL(k_color_hist, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist);
-    hipLaunchKernelGGL(k_color_offsets, dim3(1), dim3(kColorBlocks), 0, ctx->stream, hist, nb, totals);
-    hipLaunchKernelGGL(k_color_scatter, dim3(nb), dim3(kBlock), 0, ctx->stream, colors, nq, hist, list);
-    uint32_t h[48];
-    HostReads r(ctx);
-    r.add(h, totals, sizeof(h));
-    r.wait();
+    const int nb = (int)std::min<int64_t>(kColorBlocks, (qcap + kBlock - 1) / kBlock);
+    uint32_t* hist = ensure<uint32_t>(ctx, ctx->ccnt, (size_t)kColorBlocks * 24);
+    int32_t* list = ensure<int32_t>(ctx, ctx->clist, qcap);
+    launch(ctx, "k_color_hist", k_color_hist, dim3(nb), dim3(kBlock), colors, (const Sizes*)S, hist);
+    launch(ctx, "k_color_offsets", k_color_offsets, dim3(1), dim3(kColorBlocks), hist, nb, S->colors);
+    launch(ctx, "k_color_scatter", k_color_scatter, dim3(nb), dim3(kBlock), colors, (const Sizes*)S, hist, list);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    // a class holds at most a quarter of the faces (four parities of one
+    // edge direction)
+    const int g = grid_for(ctx, qcap / 4 + 1);
     for (int c = 5; c < 24; c++)
-        if (h[c])
-            hipLaunchKernelGGL(k_color_class, dim3(grid_for(ctx, h[c])), dim3(kBlock), 0, ctx->stream, list + h[24 + c],
-                               (int64_t)h[c], twin, colors, 5);
-    if (h[4])
-        hipLaunchKernelGGL(k_color_class, dim3(grid_for(ctx, h[4])), dim3(kBlock), 0, ctx->stream, list + h[24 + 4],
-                           (int64_t)h[4], twin, colors, 4);
+        launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), list, (const uint32_t*)S->colors, c, twin,
+               colors, 5);
+    launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), list, (const uint32_t*)S->colors, 4, twin,
+           colors, 4);
     check_launch(ctx, "colour");
 }
 
 // Per-volume first vertex / quadrilateral after an order-preserving
-// compaction (newidx = exclusive scan of the keep flags).
-void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, int64_t nold, int64_t nnew) {
-    hipLaunchKernelGGL(k_vol_remap, dim3(grid_for(ctx, ctx->nvol + 1)), dim3(kBlock), 0, ctx->stream,
-                       static_cast<int64_t*>(bounds.p), ctx->nvol + 1, newidx, nold, nnew);
+// compaction (newidx = exclusive scan of the keep flags over *nold items,
+// *nnew kept).
+void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, const uint64_t* nold, const uint64_t* nnew) {
+    launch(ctx, "k_vol_remap", k_vol_remap, dim3(grid_for(ctx, ctx->nvol + 1)), dim3(kBlock),
+           static_cast<int64_t*>(bounds.p), ctx->nvol + 1, newidx, nold, nnew);
     check_launch(ctx, "k_vol_remap");
 }
 
-// Compact vertices by `removed`; vsecond (if non-null) receives new indices of
-// kept vertices.  Returns the new count.  Positions and normals stay where
-// they are; the slot map follows the vertices (gather_vertices lays them out
-// at the end).
-int64_t compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond, uint32_t** newidx_out) {
-    const int64_t nv = ctx->mesh.nv;
-    uint32_t* newidx = ensure<uint32_t>(ctx, ctx->newV, nv);
-    const int64_t nn = (int64_t)scan_keep(ctx, removed, newidx, nv);
+// Compact vertices by `removed` (S->nv of them, S->nn kept); vsecond (if
+// non-null) receives new indices of kept vertices.  Positions and normals
+// stay where they are; the slot map follows the vertices (gather_vertices
+// lays them out at the end).
+uint32_t* compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond) {
+    const int64_t vcap = ctx->vcap;
+    Sizes* S = dsizes(ctx);
+    uint32_t* newidx = ensure<uint32_t>(ctx, ctx->newV, vcap);
+    scan_to(ctx, ScanKeep{removed}, newidx, dn(&S->nv, vcap), &S->nn);
     const uint32_t* slot = ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr;
-    uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], nn);
-    int32_t* rlist = ensure<int32_t>(ctx, ctx->rlist, nv - nn);
-    if (nv > 0)
-        hipLaunchKernelGGL(k_compact_slots, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, removed, newidx, nv, slot,
-                           slot2, vsecond, rlist);
+    uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], vcap);
+    int32_t* rlist = ensure<int32_t>(ctx, ctx->rlist, vcap);
+    launch(ctx, "k_compact_slots", k_compact_slots, dim3(grid_for(ctx, vcap)), dim3(kBlock), removed, newidx,
+           (const Sizes*)S, slot, slot2, vsecond, rlist);
     check_launch(ctx, "k_compact_slots");
     ctx->scur = 1 - ctx->scur;
     ctx->slots = true;
-    remap_bounds(ctx, ctx->vbound, newidx, nv, nn);
-    if (newidx_out) *newidx_out = newidx;
-    return nn;
+    remap_bounds(ctx, ctx->vbound, newidx, &S->nv, &S->nn);
+    return newidx;
 }
 
 // Lay the live vertices' positions and normals out densely (after the last
 // compaction).
 void gather_vertices(dmc_context* ctx) {
     if (!ctx->slots) return;
-    const int64_t nv = ctx->mesh.nv;
+    const int64_t vcap = ctx->vcap;
     const int src = ctx->pcur, dst = 1 - ctx->pcur;
-    double* pos2 = ensure<double>(ctx, ctx->pos[dst], 3 * nv);
-    double* nrm2 = ensure<double>(ctx, ctx->nrm[dst], 3 * nv);
-    if (nv > 0)
-        hipLaunchKernelGGL(k_gather_vertices, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream,
-                           static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p), nv,
-                           static_cast<const double*>(ctx->pos[src].p), static_cast<const double*>(ctx->nrm[src].p),
-                           pos2, nrm2);
+    double* pos2 = ensure<double>(ctx, ctx->pos[dst], 3 * vcap);
+    double* nrm2 = ensure<double>(ctx, ctx->nrm[dst], 3 * vcap);
+    launch(ctx, "k_gather_vertices", k_gather_vertices, dim3(grid_for(ctx, vcap)), dim3(kBlock),
+           static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p), (const Sizes*)dsizes(ctx),
+           static_cast<const double*>(ctx->pos[src].p), static_cast<const double*>(ctx->nrm[src].p), pos2, nrm2);
     check_launch(ctx, "k_gather_vertices");
     ctx->pcur = dst;
     ctx->slots = false;
 }
 
-// Halfedge rows of the compacted mesh (nn vertices, nq2 quadrilaterals)
-// from the current rows, into the spare CSR buffers, which then become the
-// current ones.  Skipped (the next build counts from scratch) when the
-// current rows left halfedges out.
-void remap_rows(dmc_context* ctx, const Remap& m, int64_t nn, int64_t nq2, const int32_t* dlist, int64_t ndrop) {
-    if (last_bad(ctx) != 0) return;
+// Halfedge rows of the compacted mesh (S->nn vertices, S->qn
+// quadrilaterals) from the current rows, into the spare CSR buffers, which
+// then become the current ones.  On the device this runs only when the
+// last build saw no invalid vertex reference (S->remap_ok); otherwise the
+// next build counts its rows from scratch.
+void remap_rows(dmc_context* ctx, const Remap& m, const int32_t* dlist) {
+    const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
+    Sizes* S = dsizes(ctx);
     const uint32_t* off = static_cast<const uint32_t*>(ctx->off.p);
     const int2* adj = static_cast<const int2*>(ctx->adj.p);
-    uint32_t* deg2 = ensure<uint32_t>(ctx, ctx->deg, nn);
-    uint32_t* off2 = ensure<uint32_t>(ctx, ctx->off2, nn + 1);
-    uint32_t* cur2 = ensure<uint32_t>(ctx, ctx->cursor, nn);
-    int2* adj2 = ensure<int2>(ctx, ctx->adj2, 4 * nq2);
-    unsigned int* bad = ensure<unsigned int>(ctx, ctx->rbad, 1);
-    HIPC(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
-    if (m.nv > 0) {
-        // kept vertices: old row length less the corners in dropped quads
-        hipLaunchKernelGGL(k_remap_degree_kept, dim3(grid_for(ctx, m.nv)), dim3(kBlock), 0, ctx->stream, m, off, deg2);
-        if (ndrop > 0)
-            hipLaunchKernelGGL(k_remap_drop, dim3(grid_for(ctx, ndrop)), dim3(kBlock), 0, ctx->stream, m,
-                               static_cast<const int32_t*>(ctx->quads[ctx->cur].p), dlist, ndrop, deg2);
-        if (m.nrem > 0)
-            hipLaunchKernelGGL(k_remap_degree, dim3(grid_for(ctx, m.nrem)), dim3(kBlock), 0, ctx->stream, m, off, adj, nn,
-                               deg2, 1, bad);
-        check_launch(ctx, "k_remap_degree");
-    }
-    uint64_t* tot = totals(ctx) + kTotRemap;
-    scan_to(ctx, ScanCounts{deg2}, off2, nn, tot);
-    set_row_end(ctx, off2, nn, tot);
-    if (m.nv > 0) {
-        for (int pass = 0; pass < 2; pass++)
-            hipLaunchKernelGGL(k_remap_rows, dim3(grid_for(ctx, pass == 0 ? m.nv : m.nrem)), dim3(kBlock), 0, ctx->stream,
-                               m, off, adj, nn, off2, cur2, adj2, pass);
-        check_launch(ctx, "k_remap_rows");
-    }
+    uint32_t* deg2 = ensure<uint32_t>(ctx, ctx->deg, vcap);
+    uint32_t* off2 = ensure<uint32_t>(ctx, ctx->off2, vcap + 1);
+    uint32_t* cur2 = ensure<uint32_t>(ctx, ctx->cursor, vcap);
+    int2* adj2 = ensure<int2>(ctx, ctx->adj2, 4 * qcap);
+    launch(ctx, "k_remap_gate", k_remap_gate, dim3(1), dim3(64), S);
+    const uint32_t* ok = &S->remap_ok;
+    const int g = grid_for(ctx, vcap);
+    // kept vertices: old row length less the corners in dropped quads
+    launch(ctx, "k_remap_degree_kept", k_remap_degree_kept, dim3(g), dim3(kBlock), m, off, deg2, ok);
+    launch(ctx, "k_remap_drop", k_remap_drop, dim3(grid_for(ctx, qcap)), dim3(kBlock), m,
+           static_cast<const int32_t*>(ctx->quads[ctx->cur].p), dlist, deg2, ok);
+    launch(ctx, "k_remap_degree", k_remap_degree, dim3(g), dim3(kBlock), m, off, adj, deg2, 1, &S->rbad, ok);
+    check_launch(ctx, "k_remap_degree");
+    scan_to(ctx, ScanCounts{deg2}, off2, dn(&S->nn, vcap), &S->tot, "k_scan", ok);
+    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off2, dn(&S->nn, vcap), &S->tot, ok);
+    for (int pass = 0; pass < 2; pass++)
+        launch(ctx, "k_remap_rows", k_remap_rows, dim3(g), dim3(kBlock), m, off, adj, (const uint32_t*)off2, cur2, adj2,
+               pass, ok);
+    check_launch(ctx, "k_remap_rows");
     std::swap(ctx->off, ctx->off2);
     std::swap(ctx->adj, ctx->adj2);
     ctx->csr_ready = 2;
@@ -783,127 +754,133 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     build_halfedges(ctx, false);
     if (colored) color_faces(ctx);
     int32_t* val = valence(ctx);
-    const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq;
+    const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
+    Sizes* S = dsizes(ctx);
+    const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
     const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
-    uint8_t* p = ensure<uint8_t>(ctx, ctx->p, nq);
-    uint8_t* em = ensure<uint8_t>(ctx, ctx->em, nq);
-    uint8_t* vfirst = ensure<uint8_t>(ctx, ctx->vfirst, nv);
-    int32_t* vsecond = ensure<int32_t>(ctx, ctx->vsecond, nv);
-    int32_t* lastT = ensure<int32_t>(ctx, ctx->lastT, nv);
-    int32_t* lastR = ensure<int32_t>(ctx, ctx->lastR, nv);
-    if (nv > 0) {
-        HIPC(ctx, hipMemsetAsync(vfirst, 0, nv, ctx->stream));
-        HIPC(ctx, hipMemsetAsync(vsecond, 0xFF, nv * sizeof(int32_t), ctx->stream));
-        HIPC(ctx, hipMemsetAsync(lastT, 0xFF, nv * sizeof(int32_t), ctx->stream));
-        HIPC(ctx, hipMemsetAsync(lastR, 0xFF, nv * sizeof(int32_t), ctx->stream));
-    }
+    uint8_t* p = ensure<uint8_t>(ctx, ctx->p, qcap);
+    uint8_t* em = ensure<uint8_t>(ctx, ctx->em, qcap);
+    uint8_t* vfirst = ensure<uint8_t>(ctx, ctx->vfirst, vcap);
+    int32_t* vsecond = ensure<int32_t>(ctx, ctx->vsecond, vcap);
+    int32_t* lastT = ensure<int32_t>(ctx, ctx->lastT, vcap);
+    int32_t* lastR = ensure<int32_t>(ctx, ctx->lastR, vcap);
+    fill_u8(ctx, vfirst, nv, 0);
+    fill_u32(ctx, vsecond, nv, 0xFFFFFFFFu);
+    fill_u32(ctx, lastT, nv, 0xFFFFFFFFu);
+    fill_u32(ctx, lastR, nv, 0xFFFFFFFFu);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
-    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
-    if (nq > 0) {
-        hipLaunchKernelGGL(k_mark3x3y, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p);
-        hipLaunchKernelGGL(k_merge_claim, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, val, p,
-                           colors, twin, flag, lastT, lastR, em, vfirst);
-        check_launch(ctx, "merge3x3y");
-    }
+    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
+    const int gq = grid_for(ctx, qcap);
+    launch(ctx, "k_mark3x3y", k_mark3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, (const int32_t*)val, p);
+    launch(ctx, "k_merge_claim", k_merge_claim, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, (const int32_t*)val,
+           (const uint8_t*)p, colors, twin, flag, lastT, lastR, em, vfirst);
+    check_launch(ctx, "merge3x3y");
     // the quadrilateral compaction's scan also lists the merging faces
-    const int64_t qn = (int64_t)scan_keep(ctx, em, newQ, nq);
-    if (nq > qn) {
-        int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, nq - qn);
-        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, em, newQ, nq, mlist);
-        hipLaunchKernelGGL(k_merge_apply, dim3(grid_for(ctx, nq - qn)), dim3(kBlock), 0, ctx->stream, quads, mlist, nq - qn,
-                           em, val, lastT, lastR, static_cast<double*>(ctx->pos[ctx->pcur].p),
-                           static_cast<double*>(ctx->nrm[ctx->pcur].p),
-                           ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
-        check_launch(ctx, "k_merge_apply");
-    }
-    const int64_t nn = compact_vertices(ctx, vfirst, vsecond, nullptr);
+    scan_to(ctx, ScanKeep{em}, newQ, nq, &S->qn);
+    int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
+    launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)em, (const uint32_t*)newQ,
+           nq, mlist);
+    launch(ctx, "k_merge_apply", k_merge_apply, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
+           (const Sizes*)S, (const uint8_t*)em, (const int32_t*)val, (const int32_t*)lastT, (const int32_t*)lastR,
+           static_cast<double*>(ctx->pos[ctx->pcur].p), static_cast<double*>(ctx->nrm[ctx->pcur].p),
+           ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
+    check_launch(ctx, "k_merge_apply");
+    compact_vertices(ctx, vfirst, vsecond);
     const int dst = 1 - ctx->cur;
-    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
-    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
-    int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
-    HeStats* st = static_cast<HeStats*>(ctx->stats.p);
-    if (nv > 0)
-        hipLaunchKernelGGL(k_vmap3x3y, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, vfirst, vsecond, nv, vmap);
-    if (nq > 0)
-        hipLaunchKernelGGL(k_compact_quads3x3y, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, em, newQ,
-                           vmap, nv, q2, qmap, &st->bad);
+    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qcap);
+    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, qcap);
+    int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
+    launch(ctx, "k_vmap3x3y", k_vmap3x3y, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vfirst,
+           (const int32_t*)vsecond, (const Sizes*)S, vmap);
+    launch(ctx, "k_compact_quads3x3y", k_compact_quads3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,
+           (const uint8_t*)em, (const uint32_t*)newQ, (const int32_t*)vmap, q2, qmap, &S->cbad);
     check_launch(ctx, "k_compact_quads3x3y");
-    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
-    remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
-               qn, static_cast<const int32_t*>(ctx->mlist.p), nq - qn);
+    remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
+    remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, static_cast<const int32_t*>(ctx->rlist.p), S}, mlist);
+    launch(ctx, "k_commit", k_commit, dim3(1), dim3(64), S);
     ctx->cur = dst;
-    ctx->mesh.nv = nn;
-    ctx->mesh.nq = qn;
 }
 
 void simplify3333(dmc_context* ctx) {
     build_halfedges(ctx, false);
     int32_t* val = valence(ctx);
-    const int64_t nv = ctx->mesh.nv, nq = ctx->mesh.nq;
+    const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
+    Sizes* S = dsizes(ctx);
+    const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
     const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
-    uint8_t* vflag = ensure<uint8_t>(ctx, ctx->vfirst, nv);
-    int32_t* vtwin = ensure<int32_t>(ctx, ctx->vsecond, nv);
-    int32_t* lastF = ensure<int32_t>(ctx, ctx->lastT, nv);
-    uint8_t* p3333 = ensure<uint8_t>(ctx, ctx->p, nq);
-    uint8_t* rflag = ensure<uint8_t>(ctx, ctx->em, nq);
-    if (nv > 0) {
-        HIPC(ctx, hipMemsetAsync(vflag, 0, nv, ctx->stream));
-        HIPC(ctx, hipMemsetAsync(vtwin, 0xFF, nv * sizeof(int32_t), ctx->stream));
-        HIPC(ctx, hipMemsetAsync(lastF, 0xFF, nv * sizeof(int32_t), ctx->stream));
-    }
-    if (nq > 0) {
-        HIPC(ctx, hipMemsetAsync(p3333, 0, nq, ctx->stream));
-        HIPC(ctx, hipMemsetAsync(rflag, 0, nq, ctx->stream));
-    }
+    uint8_t* vflag = ensure<uint8_t>(ctx, ctx->vfirst, vcap);
+    int32_t* vtwin = ensure<int32_t>(ctx, ctx->vsecond, vcap);
+    int32_t* lastF = ensure<int32_t>(ctx, ctx->lastT, vcap);
+    uint8_t* p3333 = ensure<uint8_t>(ctx, ctx->p, qcap);
+    uint8_t* rflag = ensure<uint8_t>(ctx, ctx->em, qcap);
+    fill_u8(ctx, vflag, nv, 0);
+    fill_u32(ctx, vtwin, nv, 0xFFFFFFFFu);
+    fill_u32(ctx, lastF, nv, 0xFFFFFFFFu);
+    fill_u8(ctx, p3333, nq, 0);
+    fill_u8(ctx, rflag, nq, 0);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
-    if (nq > 0) {
-        hipLaunchKernelGGL(k_mark3333, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag, val,
-                           lastF, vflag, p3333, rflag);
-        hipLaunchKernelGGL(k_twin3333, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, nv, twin, flag,
-                           val, p3333, lastF, vtwin);
-        check_launch(ctx, "mark3333");
-    }
-    uint32_t* newv = nullptr;
-    const int64_t nn = compact_vertices(ctx, vflag, nullptr, &newv);
-    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, nq);
-    const int64_t qn = (int64_t)scan_keep(ctx, rflag, newQ, nq);
+    const int gq = grid_for(ctx, qcap);
+    launch(ctx, "k_mark3333", k_mark3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag,
+           (const int32_t*)val, lastF, vflag, p3333, rflag);
+    launch(ctx, "k_twin3333", k_twin3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag,
+           (const int32_t*)val, (const uint8_t*)p3333, (const int32_t*)lastF, vtwin);
+    check_launch(ctx, "mark3333");
+    uint32_t* newv = compact_vertices(ctx, vflag, nullptr);
+    uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
+    scan_to(ctx, ScanKeep{rflag}, newQ, nq, &S->qn);
     const int dst = 1 - ctx->cur;
-    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qn);
-    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, nq);
-    int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, nv);
-    HeStats* st = static_cast<HeStats*>(ctx->stats.p);
-    if (nv > 0)
-        hipLaunchKernelGGL(k_vmap3333, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream, vflag, newv, nv, vmap);
-    if (nq > 0)
-        hipLaunchKernelGGL(k_compact_quads3333, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, quads, nq, rflag,
-                           newQ, p3333, vtwin, vmap, nv, q2, qmap, &st->bad);
+    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qcap);
+    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, qcap);
+    int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
+    launch(ctx, "k_vmap3333", k_vmap3333, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vflag,
+           (const uint32_t*)newv, (const Sizes*)S, vmap);
+    launch(ctx, "k_compact_quads3333", k_compact_quads3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,
+           (const uint8_t*)rflag, (const uint32_t*)newQ, (const uint8_t*)p3333, (const int32_t*)vtwin,
+           (const int32_t*)vmap, q2, qmap, &S->cbad);
     check_launch(ctx, "k_compact_quads3333");
-    remap_bounds(ctx, ctx->qbound, newQ, nq, qn);
+    remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
     // the dropped quadrilaterals, for the row degrees
-    int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, std::max<int64_t>(nq - qn, 1));
-    if (nq > qn)
-        hipLaunchKernelGGL(k_list_flagged, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream, rflag, newQ, nq, dlist);
-    remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, static_cast<const int32_t*>(ctx->rlist.p), nv, nv - nn}, nn,
-               qn, dlist, nq - qn);
+    int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
+    launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)rflag,
+           (const uint32_t*)newQ, nq, dlist);
+    remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, static_cast<const int32_t*>(ctx->rlist.p), S}, dlist);
+    launch(ctx, "k_commit", k_commit, dim3(1), dim3(64), S);
     ctx->cur = dst;
-    ctx->mesh.nv = nn;
-    ctx->mesh.nq = qn;
 }
 
+void clear_timing(dmc_context* ctx) {
+    ctx->marks.clear();
+    ctx->kmarks.clear();
+    ctx->evused = 0;
+}
+
+// Stage times (consecutive marks), then kernel times summed per name in
+// order of first launch.
 void finish_timing(dmc_context* ctx) {
     ctx->stage_ms.clear();
-    if (!ctx->timing || ctx->marks.empty()) return;
+    if (!ctx->timing || (ctx->marks.empty() && ctx->kmarks.empty())) {
+        clear_timing(ctx);
+        return;
+    }
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i + 1 < ctx->marks.size(); i++) {
         float ms = 0;
         HIPC(ctx, hipEventElapsedTime(&ms, ctx->marks[i].ev, ctx->marks[i + 1].ev));
         ctx->stage_ms.emplace_back(ctx->marks[i].name, ms);
     }
-    for (auto& m : ctx->marks) (void)hipEventDestroy(m.ev);
-    ctx->marks.clear();
+    const size_t first_kernel = ctx->stage_ms.size();
+    for (const auto& k : ctx->kmarks) {
+        float ms = 0;
+        HIPC(ctx, hipEventElapsedTime(&ms, k.a, k.b));
+        size_t i = first_kernel;
+        while (i < ctx->stage_ms.size() && strcmp(ctx->stage_ms[i].first, k.name) != 0) i++;
+        if (i == ctx->stage_ms.size()) ctx->stage_ms.emplace_back(k.name, 0.0f);
+        ctx->stage_ms[i].second += ms;
+    }
+    clear_timing(ctx);
 }
 
 dmc_status guard(dmc_context* ctx, void (*fn)(dmc_context*, void*), void* arg) {
@@ -914,15 +891,15 @@ dmc_status guard(dmc_context* ctx, void (*fn)(dmc_context*, void*), void* arg) {
         fn(ctx, arg);
         return DMC_OK;
     } catch (const Fail& f) {
-        for (auto& m : ctx->marks) (void)hipEventDestroy(m.ev);
-        ctx->marks.clear();
+        clear_timing(ctx);
         return f.s;
     }
 }
 
 void check_grid(dmc_context* ctx, const dmc_grid& g) {
     if (g.nx < 2 || g.ny < 2 || g.nz < 2) fail(ctx, DMC_ERR_INVALID_ARG, "grid needs at least 2 points per axis");
-    if (g.nx > (1 << 24)) fail(ctx, DMC_ERR_INVALID_ARG, "nx above 16777216");
+    if (g.nx > (1 << 24) || g.ny > (1 << 24) || g.nz > (1 << 24))
+        fail(ctx, DMC_ERR_INVALID_ARG, "an axis above 16777216 points");
     const uint64_t ncells = (uint64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
     if (ncells > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "more than 2^32 cells");
 }
@@ -933,8 +910,11 @@ void check_grid(dmc_context* ctx, const dmc_grid& g) {
 // and triangles.
 template <class T>
 void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
-    const Swept sw = sweep_cells<T>(ctx, hv);
-    const uint64_t A = sw.A;
+    const SweepPlan P = plan_sweep<T>(ctx, hv);
+    upload_vtab(ctx, P);
+    const Swept sw = sweep_cells<T>(ctx, P);
+    const uint64_t A = read_u64(ctx, &dsizes(ctx)->A);
+    if (A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)A);
     const int64_t R = sw.R;
     uint32_t* acell = ensure<uint32_t>(ctx, ctx->acell, A);
     uint32_t* acode = ensure<uint32_t>(ctx, ctx->acode, A);
@@ -944,26 +924,26 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint32_t* tofs = ensure<uint32_t>(ctx, ctx->ttofs, A);
     unsigned long long* st = ensure<unsigned long long>(ctx, ctx->tstats, 3);
     uint32_t* amb = ensure<uint32_t>(ctx, ctx->tamb, A + 1);
-    HIPC(ctx, hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), ctx->stream));
-    HIPC(ctx, hipMemsetAsync(amb, 0, sizeof(uint32_t), ctx->stream));
-    hipLaunchKernelGGL(k_tmc_cells, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), 0, ctx->stream, sw.vt, R,
-                       sw.lists, sw.segCount, sw.segBase, acell, acode, cnt_v, cnt_t, amb, st);
+    fill_u32(ctx, st, hn(6), 0u);
+    fill_u32(ctx, amb, hn(1), 0u);
+    launch(ctx, "k_tmc_cells", k_tmc_cells, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), sw.vt, R, sw.lists,
+           sw.segCount, sw.segBase, acell, acode, cnt_v, cnt_t, amb, st);
     check_launch(ctx, "k_tmc_cells");
     if (A > 0) {
-        hipLaunchKernelGGL(k_tmc_amb_count<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
-                           amb, acell, acode, cnt_v, cnt_t, st);
+        launch(ctx, "k_tmc_amb_count", k_tmc_amb_count<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), sw.vt,
+               amb, acell, acode, cnt_v, cnt_t, st);
         check_launch(ctx, "k_tmc_amb_count");
     }
-    uint64_t* tot = totals(ctx);
-    scan_to(ctx, ScanCounts{cnt_v}, vofs, (int64_t)A, tot + kTotV);
-    scan_to(ctx, ScanCounts{cnt_t}, tofs, (int64_t)A, tot + kTotQ);
-    uint64_t V = 0, Tn = 0;
+    uint64_t* tot = ensure<uint64_t>(ctx, ctx->scanTotal, 2);
+    scan_to(ctx, ScanCounts{cnt_v}, vofs, hn((int64_t)A), tot);
+    scan_to(ctx, ScanCounts{cnt_t}, tofs, hn((int64_t)A), tot + 1);
+    uint64_t VT[2] = {0, 0};
     {
         HostReads rd(ctx);
-        rd.add(&V, tot + kTotV, sizeof(V));
-        rd.add(&Tn, tot + kTotQ, sizeof(Tn));
+        rd.add(VT, tot, sizeof(VT));
         rd.wait();
     }
+    const uint64_t V = VT[0], Tn = VT[1];
     if (V > 0x7FFFFFFFull || Tn > 0x7FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, T=%llu)", (unsigned long long)V,
              (unsigned long long)Tn);
@@ -973,14 +953,14 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     uint2* avb = ensure<uint2>(ctx, ctx->tvb, A);
     mark(ctx, "emit");
     if (A > 0) {
-        hipLaunchKernelGGL(k_tmc_vbase, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, vofs,
-                           (int64_t)A, avb);
+        launch(ctx, "k_tmc_vbase", k_tmc_vbase, dim3(grid_for(ctx, A)), dim3(kBlock), sw.vt, acell, acode, vofs,
+               (int64_t)A, avb);
         check_launch(ctx, "k_tmc_vbase");
-        hipLaunchKernelGGL(k_tmc_emit<T>, dim3(grid_for(ctx, A)), dim3(kBlock), 0, ctx->stream, sw.vt, acell, acode, avb,
-                           tofs, sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris, st + 2);
+        launch(ctx, "k_tmc_emit", k_tmc_emit<T>, dim3(grid_for(ctx, A)), dim3(kBlock), sw.vt, acell, acode, avb, tofs,
+               sw.segMask, sw.segBase, (int64_t)A, pos, nrm, tris, st + 2);
         check_launch(ctx, "k_tmc_emit");
-        hipLaunchKernelGGL(k_tmc_emit_amb<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, sw.vt,
-                           amb, acell, acode, avb, tofs, sw.segMask, sw.segBase, pos, nrm, tris, st + 2);
+        launch(ctx, "k_tmc_emit_amb", k_tmc_emit_amb<T>, dim3(grid_for(ctx, A, kBlock, 1024)), dim3(kBlock), sw.vt,
+               amb, acell, acode, avb, tofs, sw.segMask, sw.segBase, pos, nrm, tris, st + 2);
         check_launch(ctx, "k_tmc_emit_amb");
     }
     mark(ctx, "end");
@@ -1005,11 +985,14 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
     ctx->have_tmesh = true;
 }
 
-// Everything after the upload, for one volume or a batch.
-void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint32_t flags) {
-    const int n = (int)hv.size();
-    if (field_type == DMC_FIELD_F32) run_cells<float>(ctx, hv);
-    else run_cells<double>(ctx, hv);
+// A pass (everything after the upload, for one volume or a batch) is queued
+// without reading anything back (enqueue_pass); then the sizes block and
+// the per-volume bookkeeping come back in one transfer (finish_pass).
+// The pass's launches, in order; nothing is read back.
+void enqueue_pass(dmc_context* ctx, const SweepPlan& P, int field_type, uint32_t flags) {
+    const int n = ctx->nvol;
+    if (field_type == DMC_FIELD_F32) run_cells<float>(ctx, P);
+    else run_cells<double>(ctx, P);
     if (flags & DMC_SIMPLIFY_3X3Y_COLOR) {
         mark(ctx, "p3x3y_color");
         simplify3x3y(ctx, true);
@@ -1023,73 +1006,128 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
         simplify3333(ctx);
     }
     mark(ctx, "halfedges");
-    const HeStats hs = build_halfedges(ctx, true);
-    ctx->mesh.nonmanifold_edges = (int64_t)hs.nonmanifold;
-    ctx->mesh.edge_errors = (int64_t)hs.errors;
-    ctx->mesh.invalid_refs = (int64_t)hs.bad;
-    ctx->mesh.nbnd = (int64_t)ctx->bnd.size();
-    ctx->mesh.nt = 0;
-    const int64_t nq = ctx->mesh.nq, nv = ctx->mesh.nv;
+    build_halfedges(ctx, true);
     gather_vertices(ctx);
+    Sizes* S = dsizes(ctx);
     int32_t* tris = nullptr;
     if (flags & DMC_TRIANGLES) {
         mark(ctx, "triangles");
-        tris = ensure<int32_t>(ctx, ctx->tris, 6 * nq);
-        if (nq > 0)
-            hipLaunchKernelGGL(k_triangles, dim3(grid_for(ctx, nq)), dim3(kBlock), 0, ctx->stream,
-                               static_cast<int32_t*>(ctx->quads[ctx->cur].p), nq, nv,
-                               static_cast<double*>(ctx->pos[ctx->pcur].p), tris);
+        tris = ensure<int32_t>(ctx, ctx->tris, 6 * ctx->qcap);
+        launch(ctx, "k_triangles", k_triangles, dim3(grid_for(ctx, ctx->qcap)), dim3(kBlock),
+               static_cast<int32_t*>(ctx->quads[ctx->cur].p), (const Sizes*)S,
+               static_cast<double*>(ctx->pos[ctx->pcur].p), tris);
         check_launch(ctx, "k_triangles");
-        ctx->mesh.nt = 2 * nq;
-        ctx->mesh.d_tris = tris;
     }
     int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
     int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     int32_t* hev = static_cast<int32_t*>(ctx->hev.p);
-    if (n > 1 && (nq > 0 || nv > 0)) {
+    if (n > 1) {
         // each volume's ids start at 0
-        hipLaunchKernelGGL(k_rebase, dim3(grid_for(ctx, std::max(nq, nv))), dim3(kBlock), 0, ctx->stream,
-                           static_cast<const int64_t*>(ctx->vbound.p), static_cast<const int64_t*>(ctx->qbound.p), n,
-                           quads, twin, tris, nq, hev, nv);
+        launch(ctx, "k_rebase", k_rebase, dim3(grid_for(ctx, std::max(ctx->qcap, ctx->vcap))), dim3(kBlock),
+               static_cast<const int64_t*>(ctx->vbound.p), static_cast<const int64_t*>(ctx->qbound.p), n, quads, twin,
+               tris, (const Sizes*)S, hev);
         check_launch(ctx, "k_rebase");
     }
     mark(ctx, "end");
-    // per-volume bookkeeping back to the host
+}
+
+// The pass's sizes and per-volume bookkeeping back to the host (one
+// transfer), and the meshes.  Returns false when a capacity was exceeded.
+bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flags) {
+    const int n = ctx->nvol;
+    Sizes* S = dsizes(ctx);
+    int32_t* tris = (flags & DMC_TRIANGLES) ? static_cast<int32_t*>(ctx->tris.p) : nullptr;
+    int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
+    int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    int32_t* hev = static_cast<int32_t*>(ctx->hev.p);
+    // sizes and per-volume bookkeeping back to the host
     std::vector<int64_t> vb(2 * (n + 1)), qb(2 * (n + 1)), ab(n + 1), vhe(n > 1 ? 2 * n : 0);
     std::vector<VolStat> vs(n);
-    uint32_t diag[6];
-    HostReads r(ctx);
-    r.add(vb.data(), ctx->vbound.p, vb.size() * sizeof(int64_t));
-    r.add(qb.data(), ctx->qbound.p, qb.size() * sizeof(int64_t));
-    r.add(ab.data(), vol_ptr(ctx, 2), ab.size() * sizeof(int64_t));
-    r.add(vs.data(), ctx->vstat.p, vs.size() * sizeof(VolStat));
-    if (n > 1) r.add(vhe.data(), ctx->vhe.p, vhe.size() * sizeof(unsigned long long));
-    r.add(diag, ctx->fails.p, sizeof(diag));
-    r.wait();
-    check_rows(ctx);
-    ctx->mesh.valence_zero = diag[4];
-    ctx->mesh.invalid_refs += diag[5];
+    Sizes& hs = ctx->hsz;
+    {
+        HostReads r(ctx);
+        r.add(&hs, S, sizeof(Sizes));
+        r.add(vb.data(), ctx->vbound.p, vb.size() * sizeof(int64_t));
+        r.add(qb.data(), ctx->qbound.p, qb.size() * sizeof(int64_t));
+        r.add(ab.data(), vol_ptr(ctx, 2), ab.size() * sizeof(int64_t));
+        r.add(vs.data(), ctx->vstat.p, vs.size() * sizeof(VolStat));
+        if (n > 1) r.add(vhe.data(), ctx->vhe.p, vhe.size() * sizeof(unsigned long long));
+        r.wait();
+    }
     finish_timing(ctx);
-    ctx->mesh.d_positions = static_cast<double*>(ctx->pos[ctx->pcur].p);
-    ctx->mesh.d_normals = static_cast<double*>(ctx->nrm[ctx->pcur].p);
-    ctx->mesh.d_quads = quads;
-    ctx->mesh.d_twins = twin;
-    ctx->mesh.d_he_flags = static_cast<uint8_t*>(ctx->flag.p);
-    ctx->mesh.d_he_v = hev;
+    if (hs.A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)hs.A);
+    if (hs.V > 0x7FFFFFFFull || hs.Q > 0x1FFFFFFFull)
+        fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)hs.V,
+             (unsigned long long)hs.Q);
+    // what this context has seen, for the next pass's capacities
+    int64_t cells = 0;
+    for (const HostVol& h : hv) cells += (int64_t)(h.g.nx - 1) * (h.g.ny - 1) * (h.g.nz - 1);
+    if (cells > 0) {
+        ctx->ra = std::max(ctx->ra, (double)hs.A / (double)cells);
+        ctx->rv = std::max(ctx->rv, (double)hs.V / (double)cells);
+        ctx->rq = std::max(ctx->rq, (double)hs.Q / (double)cells);
+    }
+    if (hs.ovf) {
+        // exceeded a capacity: the same pass again with room for the exact
+        // sizes (the counts above are exact whatever the capacities)
+        ctx->acap_min = std::max<int64_t>(ctx->acap_min, (int64_t)hs.A);
+        ctx->vcap_min = std::max<int64_t>(ctx->vcap_min, (int64_t)hs.V);
+        ctx->qcap_min = std::max<int64_t>(ctx->qcap_min, (int64_t)hs.Q);
+        if (hs.ovf & 8u) ctx->reccap = std::max<int64_t>(ctx->reccap, (int64_t)hs.nrec);
+        return false;
+    }
+    if (hs.rows != 4 * hs.Q)
+        fail(ctx, DMC_ERR_STATE, "halfedge rows %llu != 4 Q %llu", (unsigned long long)hs.rows,
+             (unsigned long long)(4 * hs.Q));
+    // getBndVertices: the 3-face records, in key order
+    ctx->bnd.clear();
+    if (hs.nrec > 0) {
+        std::vector<BndRec> h(hs.nrec);
+        HostReads r(ctx);
+        r.add(h.data(), ctx->bndrec.p, h.size() * sizeof(BndRec));
+        r.wait();
+        std::sort(h.begin(), h.end(), [](const BndRec& a, const BndRec& b) { return a.key < b.key; });
+        for (const BndRec& x : h) {
+            ctx->bnd.push_back(x.a);
+            ctx->bnd.push_back(x.b);
+        }
+    }
+    const int64_t nv = (int64_t)hs.nv, nq = (int64_t)hs.nq;
+    dmc_mesh& M = ctx->mesh;
+    M.nv = nv;
+    M.nq = nq;
+    M.nt = (flags & DMC_TRIANGLES) ? 2 * nq : 0;
+    M.d_tris = tris;
+    M.nonmanifold_edges = (int64_t)hs.he.nonmanifold;
+    M.edge_errors = (int64_t)hs.he.errors;
+    M.invalid_refs = (int64_t)hs.he.bad + (int64_t)hs.unresolved;
+    M.valence_zero = hs.zero;
+    M.nbnd = (int64_t)ctx->bnd.size();
+    M.active_cells = (int64_t)hs.A;
+    M.nv_cells = (int64_t)hs.V;
+    M.nq_cells = (int64_t)hs.Q;
+    M.d_positions = static_cast<double*>(ctx->pos[ctx->pcur].p);
+    M.d_normals = static_cast<double*>(ctx->nrm[ctx->pcur].p);
+    M.d_quads = quads;
+    M.d_twins = twin;
+    M.d_he_flags = static_cast<uint8_t*>(ctx->flag.p);
+    M.d_he_v = hev;
+    M.host_syncs = ctx->syncs - ctx->syncs_at_start;
+    M.pass_mode = ctx->pass_mode;
     uint64_t f1 = 0, f2 = 0, nc = 0;
     for (const VolStat& x : vs) {
         f1 += x.fail1;
         f2 += x.fail2;
         nc += x.contours;
     }
-    ctx->mesh.failed_projections_l1 = (uint32_t)f1;
-    ctx->mesh.failed_projections_l2 = (uint32_t)f2;
-    ctx->mesh.nv_emitted = (int64_t)nc;
+    M.failed_projections_l1 = (uint32_t)f1;
+    M.failed_projections_l2 = (uint32_t)f2;
+    M.nv_emitted = (int64_t)nc;
     // one view per volume
     ctx->vmesh.assign(n, dmc_mesh{});
     ctx->vbnd.assign(n, std::vector<int32_t>());
     if (n == 1) {
-        ctx->vmesh[0] = ctx->mesh;
+        ctx->vmesh[0] = M;
         ctx->vbnd[0] = ctx->bnd;
     } else {
         for (size_t x = 0; x + 1 < ctx->bnd.size(); x += 2) {
@@ -1103,27 +1141,152 @@ void run(dmc_context* ctx, const std::vector<HostVol>& hv, int field_type, uint3
             m.nv = vb[v + 1] - vb[v];
             m.nq = qb[v + 1] - qb[v];
             m.nt = (flags & DMC_TRIANGLES) ? 2 * m.nq : 0;
-            m.d_positions = ctx->mesh.d_positions + 3 * vb[v];
-            m.d_normals = ctx->mesh.d_normals + 3 * vb[v];
+            m.d_positions = M.d_positions + 3 * vb[v];
+            m.d_normals = M.d_normals + 3 * vb[v];
             m.d_quads = quads + 4 * qb[v];
             m.d_tris = tris ? tris + 6 * qb[v] : nullptr;
             m.d_twins = twin + 4 * qb[v];
-            m.d_he_flags = ctx->mesh.d_he_flags + 4 * qb[v];
+            m.d_he_flags = M.d_he_flags + 4 * qb[v];
             m.d_he_v = hev + vb[v];
             m.nonmanifold_edges = (int64_t)vhe[2 * v];
             m.nbnd = (int64_t)ctx->vbnd[v].size();
             m.edge_errors = (int64_t)vhe[2 * v + 1];
-            m.invalid_refs = ctx->mesh.invalid_refs;   // batch-wide
-            m.valence_zero = ctx->mesh.valence_zero;   // batch-wide
+            m.invalid_refs = M.invalid_refs;   // batch-wide
+            m.valence_zero = M.valence_zero;   // batch-wide
             m.failed_projections_l1 = vs[v].fail1;
             m.failed_projections_l2 = vs[v].fail2;
             m.active_cells = ab[v + 1] - ab[v];
             m.nv_emitted = (int64_t)vs[v].contours;
             m.nv_cells = vb[n + 1 + v + 1] - vb[n + 1 + v];
             m.nq_cells = qb[n + 1 + v + 1] - qb[n + 1 + v];
+            m.host_syncs = M.host_syncs;   // batch-wide
+            m.pass_mode = M.pass_mode;
         }
     }
     ctx->have_mesh = true;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Captured passes.  A pass's launches depend only on its shape: the volumes'
+// dimensions, the element type and alignment, the flags, the launch shape,
+// the capacities and the buffers (every buffer allocation bumps buf_epoch);
+// the fields, iso-values and geometry are read from the volume table in
+// device memory.  The second time a shape comes, its pass is captured into a
+// HIP graph; later passes of that shape replay it.  Host-side state the
+// launches leave (which double buffers hold the mesh) is recorded with the
+// graph and restored after a replay.
+// ---------------------------------------------------------------------------
+std::string pass_key(dmc_context* ctx, const std::vector<HostVol>& hv, const SweepPlan& P, int field_type,
+                     uint32_t flags) {
+    std::string k;
+    auto put = [&](int64_t x) { k.append(reinterpret_cast<const char*>(&x), sizeof(x)); };
+    put(field_type);
+    put(flags & (DMC_SIMPLIFY_ALL | DMC_TRIANGLES));
+    put(P.vec);
+    put(ctx->grid_cap);
+    put(ctx->sweep_waves);
+    put(ctx->acap);
+    put(ctx->vcap);
+    put(ctx->qcap);
+    put(ctx->reccap);
+    put((int64_t)ctx->buf_epoch);
+    put((int64_t)hv.size());
+    for (const HostVol& h : hv) put(((int64_t)h.g.nx << 42) | ((int64_t)h.g.ny << 21) | h.g.nz);
+    return k;
+}
+
+void save_post(dmc_context* ctx, dmc_context::PassGraph& g) {
+    g.cur = ctx->cur;
+    g.pcur = ctx->pcur;
+    g.scur = ctx->scur;
+    g.slots = ctx->slots;
+    g.csr_ready = ctx->csr_ready;
+    g.off_swapped = ctx->off.p > ctx->off2.p;
+    g.adj_swapped = ctx->adj.p > ctx->adj2.p;
+}
+
+void load_post(dmc_context* ctx, const dmc_context::PassGraph& g) {
+    ctx->cur = g.cur;
+    ctx->pcur = g.pcur;
+    ctx->scur = g.scur;
+    ctx->slots = g.slots;
+    ctx->csr_ready = g.csr_ready;
+    if ((ctx->off.p > ctx->off2.p) != g.off_swapped) std::swap(ctx->off, ctx->off2);
+    if ((ctx->adj.p > ctx->adj2.p) != g.adj_swapped) std::swap(ctx->adj, ctx->adj2);
+}
+
+void drop_graphs(dmc_context* ctx) {
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
+}
+
+// One pass: eager, captured and launched, or replayed.
+void run_pass(dmc_context* ctx, const SweepPlan& P, int field_type, uint32_t flags) {
+    ctx->pass_mode = 0;
+    const bool graphs = ctx->use_graphs && !ctx->timing;
+    std::string key;
+    if (graphs) {
+        // hv is only needed for the key: rebuild it from the volume table
+        std::vector<HostVol> hv(ctx->nvol);
+        const VolDesc* d = reinterpret_cast<const VolDesc*>(ctx->vtab_host.data());
+        for (int v = 0; v < ctx->nvol; v++) hv[v].g = dmc_grid{d[v].d.nx, d[v].d.ny, d[v].d.nz, 0, 0, 0, 0, 0, 0};
+        key = pass_key(ctx, hv, P, field_type, flags);
+        for (auto& g : ctx->graphs) {
+            if (g.key != key) continue;
+            upload_vtab(ctx, P);
+            if (ctx->off.p > ctx->off2.p) std::swap(ctx->off, ctx->off2);
+            if (ctx->adj.p > ctx->adj2.p) std::swap(ctx->adj, ctx->adj2);
+            HIPC(ctx, hipGraphLaunch(g.exec, ctx->stream));
+            load_post(ctx, g);
+            g.use = ++ctx->graph_clock;
+            ctx->pass_mode = 2;
+            return;
+        }
+    }
+    upload_vtab(ctx, P);
+    if (!graphs || key != ctx->last_key) {
+        ctx->last_key = key;
+        enqueue_pass(ctx, P, field_type, flags);
+        return;
+    }
+    // the second pass of this shape: capture it, then launch the graph
+    const uint64_t epoch = ctx->buf_epoch;
+    HIPC(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capturing = true;
+    hipGraph_t graph = nullptr;
+    try {
+        enqueue_pass(ctx, P, field_type, flags);
+    } catch (...) {
+        ctx->capturing = false;
+        (void)hipStreamEndCapture(ctx->stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    ctx->capturing = false;
+    HIPC(ctx, hipStreamEndCapture(ctx->stream, &graph));
+    dmc_context::PassGraph g{};
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) fail(ctx, DMC_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    if (ctx->buf_epoch != epoch) {
+        (void)hipGraphExecDestroy(g.exec);
+        fail(ctx, DMC_ERR_STATE, "buffers changed while a pass was captured");
+    }
+    g.key = key;
+    save_post(ctx, g);
+    // the capture recorded the launches without running them: launch now,
+    // from the pass's starting state
+    HIPC(ctx, hipGraphLaunch(g.exec, ctx->stream));
+    g.use = ++ctx->graph_clock;
+    if (ctx->graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                    [](const auto& x, const auto& y) { return x.use < y.use; });
+        (void)hipGraphExecDestroy(lru->exec);
+        ctx->graphs.erase(lru);
+    }
+    ctx->graphs.push_back(g);
+    ctx->pass_mode = 1;
 }
 
 struct ExtractArgs {
@@ -1160,7 +1323,7 @@ void do_extract(dmc_context* ctx, void* a_) {
     ctx->have_mesh = false;
     ctx->have_tmesh = false;
     ctx->mesh = dmc_mesh{};
-    ctx->bad_epoch = ctx->rows_epoch = -1;
+    ctx->syncs_at_start = ctx->syncs;
     mark(ctx, "upload");
     std::vector<HostVol> hv(a.nvol);
     uint8_t* fdev = nullptr;
@@ -1189,7 +1352,15 @@ void do_extract(dmc_context* ctx, void* a_) {
         *a.tout = ctx->tmesh;
         return;
     }
-    run(ctx, hv, a.field_type, a.flags);
+    // a pass that outgrew a capacity learned the exact sizes: once more
+    bool done = false;
+    for (int attempt = 0; attempt < 3 && !done; attempt++) {
+        const SweepPlan P = a.field_type == DMC_FIELD_F32 ? plan_sweep<float>(ctx, hv) : plan_sweep<double>(ctx, hv);
+        plan_caps(ctx, P.cells);
+        run_pass(ctx, P, a.field_type, a.flags);
+        done = finish_pass(ctx, hv, a.flags);
+    }
+    if (!done) fail(ctx, DMC_ERR_STATE, "capacities still exceeded after resizing");
     if (a.out)
         for (int v = 0; v < a.nvol; v++) a.out[v] = ctx->vmesh[v];
 }
@@ -1230,6 +1401,7 @@ dmc_status dmc_create(dmc_context** out, int device) {
         const long v = strtol(cap, nullptr, 10);
         if (v >= kGridCapMin && v <= kGridCapMax) ctx->grid_cap = (int)v;
     }
+    if (const char* gr = getenv("TMC_GRAPHS")) ctx->use_graphs = strtol(gr, nullptr, 10) != 0;
     if (const char* w = getenv("TMC_SWEEP_WAVES")) {
         const long v = strtol(w, nullptr, 10);
         if (v >= kSweepWavesMin && v <= kSweepWavesMax) ctx->sweep_waves = (int)v;
@@ -1238,6 +1410,13 @@ dmc_status dmc_create(dmc_context** out, int device) {
         delete ctx;
         return DMC_ERR_HIP;
     }
+    // pinned staging of count reads
+    if (hipHostMalloc(&ctx->pin, 64 << 10) != hipSuccess) {
+        (void)hipGetLastError();
+        dmc_destroy(ctx);
+        return DMC_ERR_HIP;
+    }
+    ctx->pin_cap = 64 << 10;
     *out = ctx;
     return DMC_OK;
 }
@@ -1249,7 +1428,8 @@ dmc_status dmc_destroy(dmc_context* ctx) {
     for (Buf* b : ctx->bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
-    if (ctx->dpin) (void)hipHostFree(ctx->dpin);
+    for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DMC_OK;
@@ -1419,7 +1599,7 @@ static void do_edges(dmc_context* ctx, void* a_) {
     uint32_t* nid = ensure<uint32_t>(ctx, ctx->dnew, nhe);
     // the halfedge rows and twins are those of the final build
     if (nhe > 0)
-        hipLaunchKernelGGL(k_edge_reps, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream,
+        launch(ctx, "k_edge_reps", k_edge_reps, dim3(grid_for(ctx, nhe)), dim3(kBlock),
                            static_cast<const int32_t*>(m.d_quads), nhe, nv,
                            static_cast<const uint32_t*>(ctx->off.p), static_cast<const int2*>(ctx->adj.p),
                            static_cast<const int32_t*>(m.d_twins), static_cast<const uint8_t*>(m.d_he_flags), rep);
@@ -1428,7 +1608,7 @@ static void do_edges(dmc_context* ctx, void* a_) {
     *a.count = ne;
     if (a.edges && ne > 0) {
         int2* e = ensure<int2>(ctx, ctx->dedges, ne);
-        hipLaunchKernelGGL(k_edge_emit, dim3(grid_for(ctx, nhe)), dim3(kBlock), 0, ctx->stream,
+        launch(ctx, "k_edge_emit", k_edge_emit, dim3(grid_for(ctx, nhe)), dim3(kBlock),
                            static_cast<const int32_t*>(m.d_quads), nhe, rep, nid, e);
         check_launch(ctx, "k_edge_emit");
         HIPC(ctx, hipMemcpyAsync(a.edges, e, ne * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
@@ -1459,10 +1639,10 @@ static void do_valence_hist(dmc_context* ctx, void* a_) {
     uint32_t* hist = ensure<uint32_t>(ctx, ctx->dhist, kValenceBins + 1);
     HIPC(ctx, hipMemsetAsync(hist, 0, (kValenceBins + 1) * sizeof(uint32_t), ctx->stream));
     if (nv > 0) {
-        hipLaunchKernelGGL(k_vertex_valence, dim3(grid_for(ctx, nv)), dim3(kBlock), 0, ctx->stream,
+        launch(ctx, "k_vertex_valence", k_vertex_valence, dim3(grid_for(ctx, nv)), dim3(kBlock),
                            static_cast<const int32_t*>(m.d_quads), a.triangles ? m.d_tris : nullptr, nv,
                            static_cast<const uint32_t*>(ctx->off.p), static_cast<const int2*>(ctx->adj.p), vval);
-        hipLaunchKernelGGL(k_valence_hist, dim3(grid_for(ctx, nv, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, vval, nv,
+        launch(ctx, "k_valence_hist", k_valence_hist, dim3(grid_for(ctx, nv, kBlock, 1024)), dim3(kBlock), vval, nv,
                            hist);
     }
     check_launch(ctx, "valence histogram");
@@ -1497,10 +1677,10 @@ static void do_quality(dmc_context* ctx, void* a_) {
     if (n == 0) return;
     float* q = ensure<float>(ctx, ctx->dqual, n);
     if (a.which == 0)
-        hipLaunchKernelGGL(k_quality_quads, dim3(grid_for(ctx, n)), dim3(kBlock), 0, ctx->stream,
+        launch(ctx, "k_quality_quads", k_quality_quads, dim3(grid_for(ctx, n)), dim3(kBlock),
                            static_cast<const int32_t*>(m.d_quads), n, static_cast<const double*>(m.d_positions), q);
     else
-        hipLaunchKernelGGL(k_quality_tris, dim3(grid_for(ctx, n)), dim3(kBlock), 0, ctx->stream,
+        launch(ctx, "k_quality_tris", k_quality_tris, dim3(grid_for(ctx, n)), dim3(kBlock),
                            static_cast<const int32_t*>(m.d_tris), n, static_cast<const double*>(m.d_positions), q);
     check_launch(ctx, "element quality");
     HIPC(ctx, hipMemcpyAsync(a.quality, q, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -1579,6 +1759,14 @@ dmc_status dmc_write_obj(dmc_context* ctx, const char* path, int triangles) {
     if (!path) return DMC_ERR_INVALID_ARG;
     ObjArgs a{path, triangles};
     return guard(ctx, do_write_obj, &a);
+}
+
+dmc_status dmc_context_bytes(dmc_context* ctx, size_t* bytes) {
+    if (!ctx || !bytes) return DMC_ERR_INVALID_ARG;
+    size_t n = 0;
+    for (const Buf* b : ctx->bufs) n += b->cap;
+    *bytes = n;
+    return DMC_OK;
 }
 
 dmc_status dmc_get_launch(dmc_context* ctx, dmc_launch* out) {

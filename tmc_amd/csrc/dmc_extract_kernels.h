@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                                                   uint32_t* __restrict__ aaux,
                                                   uint32_t* __restrict__ avloc, uint32_t* __restrict__ aqloc,
                                                   uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowQ,
-                                                  VolStat* __restrict__ vstat) {
+                                                  VolStat* __restrict__ vstat, uint32_t acap) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wpb = kBlock / kWave;
     unsigned long long total = 0;
@@ -473,7 +473,9 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                 code = icase | (map << 8);
             }
             const uint32_t iv = wave_inclusive_scan(nv), iq = wave_inclusive_scan(nq);
-            if (act) {
+            // the counts are exact whatever the capacity; entries past it are
+            // left out (the pass is then run again with room for them)
+            if (act && a < acap) {
                 acell[a] = (uint32_t)(r * d.C) + (uint32_t)i;
                 acode[a] = code;
                 aaux[a] = aux;
@@ -559,39 +561,39 @@ __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int
                                              int t, uint32_t out, double* __restrict__ vpos,
                                              double* __restrict__ vnrm, uint32_t* __restrict__ vdeg) {
     const Dims& d = V.d;
-    double G[8][3];
-    if (V.grads) {
-        const double* __restrict__ grads = V.grads;
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int64_t q = ((int64_t)(k + ((c >> 2) & 1)) * d.ny + (j + ((c >> 1) & 1))) * d.nx + i + (c & 1);
-            G[c][0] = grads[3 * q];
-            G[c][1] = grads[3 * q + 1];
-            G[c][2] = grads[3 * q + 2];
-        }
-    } else {
-        const double dx2 = 2 * V.geo.dx, dy2 = 2 * V.geo.dy, dz2 = 2 * V.geo.dz;
-        const double s = -1.;
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-            const double gx = (cx ? (double)o.x[cy | cz << 1][1] - Fc(c & ~1) : Fc(c | 1) - (double)o.x[cy | cz << 1][0]) / dx2;
-            const double gy = (cy ? (double)o.y[cx | cz << 1][1] - Fc(c & ~2) : Fc(c | 2) - (double)o.y[cx | cz << 1][0]) / dy2;
-            const double gz = (cz ? (double)o.z[cx | cy << 1][1] - Fc(c & ~4) : Fc(c | 4) - (double)o.z[cx | cy << 1][0]) / dz2;
-            G[c][0] = gx * s;
-            G[c][1] = gy * s;
-            G[c][2] = gz * s;
-        }
-    }
     const double u = p[0], v = p[1], ww = p[2];
+    const double dx2[3] = {2 * V.geo.dx, 2 * V.geo.dy, 2 * V.geo.dz};
     double n[3];
+    // one component at a time: its eight corner values, then the blend
 #pragma unroll
     for (int q = 0; q < 3; q++) {
+        double G[8];
+        if (V.grads) {
+            const double* __restrict__ grads = V.grads;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int64_t x = ((int64_t)(k + ((c >> 2) & 1)) * d.ny + (j + ((c >> 1) & 1))) * d.nx + i + (c & 1);
+                G[c] = grads[3 * x + q];
+            }
+        } else {
+            const double s = -1.;
+            const int bit = 1 << q;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+                // the outer value on this axis: the other two corner bits
+                // select the row (x), column (y) or pillar (z)
+                const int b = q == 0 ? (cy | cz << 1) : q == 1 ? (cx | cz << 1) : (cx | cy << 1);
+                const T* ov = q == 0 ? o.x[b] : q == 1 ? o.y[b] : o.z[b];
+                const double g = ((c & bit) ? (double)ov[1] - Fc(c & ~bit) : Fc(c | bit) - (double)ov[0]) / dx2[q];
+                G[c] = g * s;
+            }
+        }
         // (1-u) n_even + u n_odd, then rows by v, then planes by w
-        const double lo0 = (G[0][q] * (1 - u) + G[1][q] * u) * (1 - v);
-        const double hi0 = lo0 + (G[2][q] * (1 - u) + G[3][q] * u) * v;
-        const double lo1 = (G[4][q] * (1 - u) + G[5][q] * u) * (1 - v);
-        const double hi1 = lo1 + (G[6][q] * (1 - u) + G[7][q] * u) * v;
+        const double lo0 = (G[0] * (1 - u) + G[1] * u) * (1 - v);
+        const double hi0 = lo0 + (G[2] * (1 - u) + G[3] * u) * v;
+        const double lo1 = (G[4] * (1 - u) + G[5] * u) * (1 - v);
+        const double hi1 = lo1 + (G[6] * (1 - u) + G[7] * u) * v;
         n[q] = hi0 * (1 - ww) + hi1 * ww;
     }
     const double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -612,15 +614,29 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
                                                       const uint32_t* __restrict__ acode,
                                                       const uint32_t* __restrict__ aaux,
                                                       const uint32_t* __restrict__ avloc,
-                                                      const uint32_t* __restrict__ rowBaseV, int64_t A,
+                                                      const uint32_t* __restrict__ rowBaseV, DevN dA,
                                                       double* __restrict__ vpos, double* __restrict__ vnrm,
-                                                      uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow) {
+                                                      uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow,
+                                                      unsigned long long* qhead) {
     static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
+    const int64_t A = dA.get();
     __shared__ float boxes[kBoxFloats * kBoxStride];
     __shared__ double corners[8 * kCornerStride];
     const BoxRef B{boxes + threadIdx.x};
     const CornerRef Fc{corners + threadIdx.x};
-    for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
+    // Persistent blocks pull chunks of kBlock active cells from a device
+    // work queue (*qhead, ascending): cells cost unevenly (one to four
+    // contours, searches of different lengths, the volumes of a batch of
+    // different sizes), so no block is held to a fixed share.
+    __shared__ int64_t chunk;
+    for (;;) {
+        if (threadIdx.x == 0) chunk = (int64_t)atomicAdd(qhead, (unsigned long long)kBlock);
+        __syncthreads();
+        const int64_t a = chunk + threadIdx.x;
+        const bool drained = chunk >= A;
+        __syncthreads();
+        if (drained) break;
+        if (a >= A) continue;
         const int vol = vol_of(vt.aBase, vt.n, a);
         const VolDesc& V = vt.desc[vol];
         const double iso = V.iso;
@@ -765,12 +781,13 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
                                                    const uint32_t* __restrict__ avloc,
                                                    const uint32_t* __restrict__ aqloc,
                                                    const uint32_t* __restrict__ rowBaseV0,
-                                                   const uint32_t* __restrict__ rowBaseQ, int64_t A,
+                                                   const uint32_t* __restrict__ rowBaseQ, DevN dA,
                                                    const unsigned long long* __restrict__ segMask,
                                                    const uint32_t* __restrict__ segBase,
                                                    int32_t* __restrict__ quads, int32_t* __restrict__ colors,
                                                    const uint32_t* __restrict__ off, int2* __restrict__ adj,
-                                                   uint32_t* __restrict__ unresolved) {
+                                                   uint64_t adjcap, uint32_t* unresolved) {
+    const int64_t A = dA.get();
     for (int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x; a < A; a += (int64_t)gridDim.x * kBlock) {
         const int vol = vol_of(vt.aBase, vt.n, a);
         const VolDesc& V = vt.desc[vol];
@@ -806,8 +823,11 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
                 atomicAdd(unresolved, 1u);
             } else {
 #pragma unroll
-                for (int p = 0; p < 4; p++)
-                    adj[off[out[p]] + sl[p]] = make_int2((int32_t)(4 * q + p), out[(p + 1) & 3]);
+                for (int p = 0; p < 4; p++) {
+                    // rows total 4 Q (checked by the host); never past the array
+                    const uint64_t at = (uint64_t)off[out[p]] + (uint64_t)sl[p];
+                    if (at < adjcap) adj[at] = make_int2((int32_t)(4 * q + p), out[(p + 1) & 3]);
+                }
             }
             reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
             colors[q] = 3 * parity + c;
@@ -852,33 +872,55 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
 // up to date through order-preserving compactions, and the final rebase of
 // each volume's ids to start at 0.
 // ---------------------------------------------------------------------------
-__global__ void k_vol_abase(VolTable vt, const uint32_t* __restrict__ segBase, int64_t A, int64_t* __restrict__ aBase) {
+__global__ void k_vol_abase(VolTable vt, const uint32_t* __restrict__ segBase, const uint64_t* A,
+                            int64_t* __restrict__ aBase) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v <= vt.n; v += gridDim.x * blockDim.x)
-        aBase[v] = v < vt.n ? (int64_t)segBase[vt.desc[v].rsBase] : A;
+        aBase[v] = v < vt.n ? (int64_t)segBase[vt.desc[v].rsBase] : (int64_t)*A;
 }
 
+// After the row scans: which capacities the cell stage exceeded.  A pass
+// that exceeded one processes an empty mesh from here on (acnt = nv = nq =
+// 0) and is run again by the host with room for the exact sizes.
+__global__ void k_cell_guard(Sizes* S, uint64_t acap, uint64_t vcap, uint64_t qcap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t o = (S->A > acap ? 1u : 0u) | (S->V > vcap ? 2u : 0u) | (S->Q > qcap ? 4u : 0u);
+        S->ovf |= o;
+        S->acnt = o ? 0 : S->A;
+        S->nv = o ? 0 : S->V;
+        S->nq = o ? 0 : S->Q;
+    }
+}
+
+// Each volume's first vertex and quadrilateral: [0, n] the live bounds,
+// [n+1, 2n+1] the cell stage's (kept for the per-volume counts).
 __global__ void k_vol_bounds(VolTable vt, const uint32_t* __restrict__ rowBaseV, const uint32_t* __restrict__ rowBaseQ,
-                             int64_t V, int64_t Q, int64_t* __restrict__ vb, int64_t* __restrict__ qb) {
+                             const Sizes* S, int64_t* __restrict__ vb, int64_t* __restrict__ qb) {
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v <= vt.n; v += gridDim.x * blockDim.x) {
-        vb[v] = v < vt.n ? (int64_t)rowBaseV[vt.desc[v].rowBase] : V;
-        qb[v] = v < vt.n ? (int64_t)rowBaseQ[vt.desc[v].rowBase] : Q;
+        const int64_t x = v < vt.n ? (int64_t)rowBaseV[vt.desc[v].rowBase] : (int64_t)S->V;
+        const int64_t y = v < vt.n ? (int64_t)rowBaseQ[vt.desc[v].rowBase] : (int64_t)S->Q;
+        vb[v] = x;
+        qb[v] = y;
+        vb[vt.n + 1 + v] = x;
+        qb[vt.n + 1 + v] = y;
     }
 }
 
 // b[v] -> number of kept elements before it (newIdx = exclusive scan of the
-// keep flags over nOld elements).
-__global__ void k_vol_remap(int64_t* __restrict__ b, int n1, const uint32_t* __restrict__ newIdx, int64_t nOld,
-                            int64_t nNew) {
+// keep flags over *nOld elements, *nNew kept).
+__global__ void k_vol_remap(int64_t* __restrict__ b, int n1, const uint32_t* __restrict__ newIdx,
+                            const uint64_t* nOld, const uint64_t* nNew) {
+    const int64_t no = (int64_t)*nOld, nn = (int64_t)*nNew;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n1; v += gridDim.x * blockDim.x) {
         const int64_t x = b[v];
-        b[v] = x < nOld ? (int64_t)newIdx[x] : nNew;
+        b[v] = x < no ? (int64_t)newIdx[x] : nn;
     }
 }
 
 __global__ void __launch_bounds__(kBlock) k_rebase(const int64_t* __restrict__ vb, const int64_t* __restrict__ qb,
                                                     int n, int32_t* __restrict__ quads, int32_t* __restrict__ twin,
-                                                    int32_t* __restrict__ tris, int64_t nq,
-                                                    int32_t* __restrict__ hev, int64_t nv) {
+                                                    int32_t* __restrict__ tris, const Sizes* S,
+                                                    int32_t* __restrict__ hev) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     const int64_t m = nq > nv ? nq : nv;
     for (int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x; x < m; x += (int64_t)gridDim.x * kBlock) {
         if (x < nq) {

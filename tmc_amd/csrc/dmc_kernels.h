@@ -32,6 +32,57 @@ __device__ __forceinline__ int vol_of(const int64_t* __restrict__ base, int n, i
     return lo;
 }
 
+// An item count that lives in device memory (a scan total, or the size a
+// compaction left), clamped to the capacity of the arrays it indexes, so
+// that a launch sized on the host by that capacity never reads or writes
+// past them.  p == nullptr: the host-known count `cap`.
+struct DevN {
+    const uint64_t* p;
+    int64_t cap;
+    __device__ __forceinline__ int64_t get() const {
+        if (!p) return cap;
+        const uint64_t x = *p;
+        return x < (uint64_t)cap ? (int64_t)x : cap;
+    }
+};
+
+// Statistics of one halfedge build.
+struct HeStats {
+    unsigned long long nonmanifold;
+    unsigned long long errors;
+    unsigned long long three;    // number of 3-face edges
+    unsigned int bad;            // invalid vertex references
+    unsigned int pad;
+};
+
+// The sizes of one extraction pass, kept on the device: every launch reads
+// its item count from here, so a pass needs no count read back between its
+// stages.  The host sizes launches and buffers by capacities and reads this
+// block once at the end.
+struct Sizes {
+    uint64_t A, V, Q;            // active cells, vertices, quadrilaterals of the cell stage (exact)
+    uint64_t acnt;               // active cells the cell kernels process: A, or 0 once a capacity is exceeded
+    uint64_t nv, nq;             // the live mesh
+    uint64_t nn, qn;             // vertices / quadrilaterals a compaction keeps
+    uint64_t rows;               // first CSR's row total (4 Q)
+    uint64_t tot;                // a scan total no one else reads
+    unsigned long long qhead;    // work queue of k_vertices: the next chunk of active cells
+    HeStats he;                  // the last halfedge build
+    uint32_t ovf;                // capacities exceeded: 1 active cells, 2 vertices, 4 quads, 8 3-face records
+    uint32_t unresolved;         // quadrilaterals with an unresolved neighbour cell
+    uint32_t zero;               // vertices of valence 0 (all valence passes)
+    uint32_t rbad;               // halfedges a remap left without a valid origin
+    uint32_t generic;            // this halfedge build counts its rows from the quadrilaterals
+    uint32_t remap_ok;           // this remap runs (the last build saw no invalid vertex reference)
+    uint32_t cbad;               // invalid references a quadrilateral compaction met
+    uint32_t nrec;               // 3-face edge records
+    uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
+};
+
+// A launch whose work is conditional on device data: it runs only when
+// *gate != 0 (gate == nullptr: always).
+__device__ __forceinline__ bool gated_off(const uint32_t* gate) { return gate && *gate == 0u; }
+
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan(T x) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -117,9 +168,14 @@ __device__ __forceinline__ void scan_items(const In& in, int64_t n, int64_t base
     }
 }
 
+// Device-wide exclusive scan over n = dn.get() items, launched on
+// ceil(dn.cap / kScanTile) tiles; skipped entirely when gated off.
 template <class In>
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, int64_t n, uint64_t* __restrict__ partial) {
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, DevN dn, const uint32_t* __restrict__ gate,
+                                                       uint64_t* __restrict__ partial) {
     __shared__ uint64_t sm[kBlock / kWave + 1];
+    if (gated_off(gate)) return;
+    const int64_t n = dn.get();
     uint32_t v[kScanIPT];
     scan_items(in, n, (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT, v);
     uint64_t s = 0;
@@ -138,8 +194,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, int64_t n, uint64
 
 // Single block: exclusive scan of the tile sums in place, total to *total.
 __global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ partial, int64_t ntiles,
-                                                         uint64_t* __restrict__ total) {
+                                                         const uint32_t* __restrict__ gate, uint64_t* __restrict__ total) {
     __shared__ uint64_t sm[1024 / kWave + 1];
+    if (gated_off(gate)) return;
     uint64_t carry = 0;
     for (int64_t b = 0; b < ntiles; b += 1024) {
         const int64_t idx = b + threadIdx.x;
@@ -153,9 +210,12 @@ __global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ p
 }
 
 template <class In>
-__global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restrict__ out, int64_t n,
+__global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restrict__ out, DevN dn,
+                                                       const uint32_t* __restrict__ gate,
                                                        const uint64_t* __restrict__ partial) {
     __shared__ uint32_t sm[kBlock / kWave + 1];
+    if (gated_off(gate)) return;
+    const int64_t n = dn.get();
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
     const bool full = base + kScanIPT <= n;
     uint32_t v[kScanIPT];
@@ -180,6 +240,27 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
         for (int q = 0; q < kScanIPT; q++)
             if (base + q < n) out[base + q] = o[q];
     }
+}
+
+// Fill n = dn.get() 32-bit words with `value` (a kernel rather than a memset
+// node, so that a captured pass replays it as a kernel).
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, DevN dn, uint32_t value,
+                                                    const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t n = dn.get();
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = value;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u8(uint8_t* __restrict__ p, DevN dn, uint8_t value) {
+    const int64_t n = dn.get();
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = value;
+}
+
+// off[n] = the low 32 bits of a scan total (the end of the last CSR row).
+__global__ void k_set_row_end(uint32_t* __restrict__ off, DevN dn, const uint64_t* __restrict__ total,
+                              const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[dn.get()] = (uint32_t)*total;
 }
 
 }  // namespace tmc

@@ -23,8 +23,11 @@ __device__ __forceinline__ int32_t he_prev(int32_t h) { return (h & ~3) | ((h + 
 // degenerate simplification) is counted and skipped.
 __device__ __forceinline__ bool vid_ok(int32_t v, int64_t nv) { return v >= 0 && (int64_t)v < nv; }
 
-__global__ void __launch_bounds__(kBlock) k_he_degree(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
-                                                       uint32_t* __restrict__ deg, uint32_t* __restrict__ bad) {
+__global__ void __launch_bounds__(kBlock) k_he_degree(const int32_t* __restrict__ quads, const Sizes* S,
+                                                       uint32_t* __restrict__ deg, uint32_t* __restrict__ bad,
+                                                       const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t nhe = 4 * (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(h, nhe) {
         const int32_t v = quads[h];
         if (vid_ok(v, nv)) atomicAdd(&deg[v], 1u);
@@ -34,9 +37,11 @@ __global__ void __launch_bounds__(kBlock) k_he_degree(const int32_t* __restrict_
 
 // Scatter (halfedge, destination) pairs into the CSR rows; the order
 // inside a row is arbitrary (the group search sorts what it needs).
-__global__ void __launch_bounds__(kBlock) k_he_scatter(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_he_scatter(const int32_t* __restrict__ quads, const Sizes* S,
                                                         const uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-                                                        int2* __restrict__ adj) {
+                                                        int2* __restrict__ adj, const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
         const int32_t v[4] = {q.x, q.y, q.z, q.w};
@@ -130,13 +135,6 @@ __device__ __forceinline__ void twin_rule(const int32_t* __restrict__ quads, int
 // Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669).
 // The group's first member also counts the edge (non-manifold / error) and,
 // for three faces, records the override of he_v and the boundary vertices.
-struct HeStats {
-    unsigned long long nonmanifold;
-    unsigned long long errors;
-    unsigned long long three;    // number of 3-face edges
-    unsigned int bad;            // invalid vertex references
-    unsigned int pad;
-};
 
 // Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669),
 // one thread per halfedge.  The four halfedges of a quadrilateral sit in
@@ -153,14 +151,15 @@ __device__ __forceinline__ int next_in_quad(int x) {
     return __builtin_amdgcn_update_dpp(0, x, 0x39, 0xF, 0xF, false);
 }
 
-__global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, const Sizes* S,
                                                       const uint32_t* __restrict__ off,
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
-                                                      uint8_t* __restrict__ flag, HeStats* __restrict__ st,
+                                                      uint8_t* __restrict__ flag, HeStats* st,
                                                       const int64_t* __restrict__ qb, int nvol,
                                                       unsigned long long* __restrict__ vst,
                                                       uint32_t* __restrict__ val) {
     static_assert(kBlock % 4 == 0, "quads must not straddle blocks");
+    const int64_t nhe = 4 * (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         const int32_t a = quads[h];
@@ -254,7 +253,10 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
         if (tw == -1 && val && vid_ok(b, nv)) atomicAdd(&val[b], 1u);
         if (c > 0 && g[0] == h) {
             if (c == 3 || c == 4) atomicAdd(&st->nonmanifold, 1ull);
-            if (c == 3) atomicAdd(&st->three, 1ull);
+            if (c == 3) {
+                atomicAdd(&st->three, 1ull);
+                st->pad = 1u;   // some 3-face edge: the final build's he_v pass runs
+            }
             if (c > 4) atomicAdd(&st->errors, 1ull);
             // per volume of a batch (rare events): [2v] non-manifold, [2v+1] errors
             if (vst && c >= 3) {
@@ -268,7 +270,8 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
 // he_v: the last halfedge (in face order) leaving each vertex
 // (cpp_dmc/DualMarchingCubes.cpp:561-564).
 __global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ off, const int2* __restrict__ adj,
-                                                  int64_t nv, int32_t* __restrict__ he_v) {
+                                                  const Sizes* S, int32_t* __restrict__ he_v) {
+    const int64_t nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(v, nv) {
         int32_t last = -1;
         for_row(adj, off[v], off[v + 1], [&](int2 r) { last = r.x > last ? r.x : last; });
@@ -288,12 +291,14 @@ struct BndRec {
     int32_t a, b;
 };
 
-__global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__ quads, int64_t nhe, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__ quads, const Sizes* S,
                                                       const uint32_t* __restrict__ off, const int2* __restrict__ adj,
                                                       const uint8_t* __restrict__ flag,
                                                       unsigned long long* __restrict__ ovr, BndRec* __restrict__ recs,
-                                                      unsigned int* __restrict__ nrec, int pass,
+                                                      unsigned int* nrec, uint32_t reccap, uint32_t* ovf, int pass,
                                                       int32_t* __restrict__ he_v) {
+    if (S->he.three == 0) return;   // no 3-face edge in this build
+    const int64_t nhe = 4 * (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         if (flag[h] != 2) continue;   // every member of a 3-face group, and only those
@@ -310,9 +315,13 @@ __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__
         if (pass == 0) {
             atomicMax(&ovr[vl], (unsigned long long)key + 1ull);
             const unsigned int p = atomicAdd(nrec, 1u);
-            recs[p].key = key;
-            recs[p].a = ba;
-            recs[p].b = bb;
+            if (p < reccap) {
+                recs[p].key = key;
+                recs[p].a = ba;
+                recs[p].b = bb;
+            } else {
+                atomicOr(ovf, 8u);
+            }
         } else if (ovr[vl] == (unsigned long long)key + 1ull) {
             he_v[vl] = lone;
         }
@@ -323,8 +332,9 @@ __global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__
 // boundary halfedges ending at the vertex.  The out-degree is the vertex's
 // CSR row length; k_he_twins adds the boundary arrivals.  A vertex has
 // valence 0 exactly when it has no row entries.
-__global__ void __launch_bounds__(kBlock) k_count_zero(const uint32_t* __restrict__ deg, int64_t nv,
-                                                        unsigned int* __restrict__ zero) {
+__global__ void __launch_bounds__(kBlock) k_count_zero(const uint32_t* __restrict__ deg, const Sizes* S,
+                                                        unsigned int* zero) {
+    const int64_t nv = (int64_t)S->nv;
     unsigned int z = 0;
     TMC_GRID_STRIDE(v, nv) z += deg[v] == 0;
     z = wave_sum(z);
@@ -351,9 +361,10 @@ __device__ __forceinline__ void face_neighbors(const int32_t* __restrict__ twin,
 // order inside a class does not matter (its faces are independent).
 constexpr int kColorBlocks = 1024;
 
-__global__ void __launch_bounds__(kBlock) k_color_hist(const int32_t* __restrict__ colors, int64_t nq,
+__global__ void __launch_bounds__(kBlock) k_color_hist(const int32_t* __restrict__ colors, const Sizes* S,
                                                         uint32_t* __restrict__ hist) {
     __shared__ uint32_t sm[24];
+    const int64_t nq = (int64_t)S->nq;
     if (threadIdx.x < 24) sm[threadIdx.x] = 0;
     __syncthreads();
     TMC_GRID_STRIDE(f, nq) {
@@ -385,10 +396,11 @@ __global__ void __launch_bounds__(kColorBlocks) k_color_offsets(uint32_t* __rest
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restrict__ colors, int64_t nq,
+__global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restrict__ colors, const Sizes* S,
                                                            const uint32_t* __restrict__ offs,
                                                            int32_t* __restrict__ list) {
     __shared__ uint32_t cur[24];
+    const int64_t nq = (int64_t)S->nq;
     if (threadIdx.x < 24) cur[threadIdx.x] = offs[blockIdx.x * 24 + threadIdx.x];
     __syncthreads();
     TMC_GRID_STRIDE(f, nq) {
@@ -397,11 +409,14 @@ __global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restr
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restrict__ list, int64_t n,
+// Class `cls`: its count and start in totals (k_color_offsets).
+__global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ totals, int cls,
                                                          const int32_t* __restrict__ twin, int32_t* __restrict__ colors,
                                                          int ncol) {
+    const int64_t n = totals[cls], start = totals[24 + cls];
     TMC_GRID_STRIDE(x, n) {
-        const int32_t f = list[x];
+        const int32_t f = list[start + x];
         int32_t nb[4];
         face_neighbors(twin, f, nb);
         int32_t c[4];
@@ -417,8 +432,9 @@ __global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restric
 }
 
 // 3X3Y pattern (cpp_dmc/DualMarchingCubes.cpp:889-913)
-__global__ void __launch_bounds__(kBlock) k_mark3x3y(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_mark3x3y(const int32_t* __restrict__ quads, const Sizes* S,
                                                       const int32_t* __restrict__ val, uint8_t* __restrict__ p) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
         bool ok = vid_ok(q.x, nv) && vid_ok(q.y, nv) && vid_ok(q.z, nv) && vid_ok(q.w, nv);
@@ -487,13 +503,14 @@ __device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restri
 
 // Pass A: every merging face claims its moved and dropped vertex; the
 // reference loops over faces in order, so the largest face index wins.
-__global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restrict__ quads, const Sizes* S,
                                                          const int32_t* __restrict__ val, const uint8_t* __restrict__ p,
                                                          const int32_t* __restrict__ colors,
                                                          const int32_t* __restrict__ twin,
                                                          const uint8_t* __restrict__ flag, int32_t* __restrict__ lastT,
                                                          int32_t* __restrict__ lastR, uint8_t* __restrict__ em,
                                                          uint8_t* __restrict__ vfirst) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         const int m = merge_decision(f, quads, val, p, colors, twin, flag, nv);
         em[f] = (uint8_t)m;   // 0 kept; 1 / 2 which pair of vertices merges
@@ -511,13 +528,14 @@ __global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restric
 // mean normal (reads only valence >= 5 vertices, which are never written)
 // and the drop map.
 __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restrict__ quads,
-                                                         const int32_t* __restrict__ mlist, int64_t n,
+                                                         const int32_t* __restrict__ mlist, const Sizes* S,
                                                          const uint8_t* __restrict__ em,
                                                          const int32_t* __restrict__ val,
                                                          const int32_t* __restrict__ lastT,
                                                          const int32_t* __restrict__ lastR, double* __restrict__ pos,
                                                          double* __restrict__ nrm, const uint32_t* __restrict__ vslot,
                                                          int32_t* __restrict__ vsecond) {
+    const int64_t n = (int64_t)(S->nq - S->qn);   // the merging faces
     TMC_GRID_STRIDE(i, n) {
         const int64_t f = mlist[i];
         const int m = em[f];
@@ -556,8 +574,9 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
 // The flagged items, ascending, at i - newidx[i] (newidx = exclusive scan of
 // the keep flags): a dense list without atomics.
 __global__ void __launch_bounds__(kBlock) k_list_flagged(const uint8_t* __restrict__ flag,
-                                                          const uint32_t* __restrict__ newidx, int64_t n,
+                                                          const uint32_t* __restrict__ newidx, DevN dn,
                                                           int32_t* __restrict__ list) {
+    const int64_t n = dn.get();
     TMC_GRID_STRIDE(i, n) {
         if (flag[i]) list[i - newidx[i]] = (int32_t)i;
     }
@@ -570,11 +589,12 @@ __global__ void __launch_bounds__(kBlock) k_list_flagged(const uint8_t* __restri
 // Removed vertices are listed densely at v - newidx[v] (the number of
 // removed vertices before v).
 __global__ void __launch_bounds__(kBlock) k_compact_slots(const uint8_t* __restrict__ removed,
-                                                           const uint32_t* __restrict__ newidx, int64_t nv,
+                                                           const uint32_t* __restrict__ newidx, const Sizes* S,
                                                            const uint32_t* __restrict__ vslot,
                                                            uint32_t* __restrict__ vslot2,
                                                            int32_t* __restrict__ vsecond,
                                                            int32_t* __restrict__ rlist) {
+    const int64_t nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(v, nv) {
         if (removed[v]) {
             rlist[v - newidx[v]] = (int32_t)v;
@@ -586,10 +606,11 @@ __global__ void __launch_bounds__(kBlock) k_compact_slots(const uint8_t* __restr
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_gather_vertices(const uint32_t* __restrict__ vslot, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_gather_vertices(const uint32_t* __restrict__ vslot, const Sizes* S,
                                                              const double* __restrict__ pos,
                                                              const double* __restrict__ nrm,
                                                              double* __restrict__ pos2, double* __restrict__ nrm2) {
+    const int64_t nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(v, nv) {
         const int64_t s = vslot[v];
 #pragma unroll
@@ -603,13 +624,13 @@ __global__ void __launch_bounds__(kBlock) k_gather_vertices(const uint32_t* __re
 // removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137)
 // The vertex ids come from vmap (k_vmap3x3y): vm[vm[x]]-style lookups of a
 // removed vertex x, exactly as the reference composes them.
-__global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, int64_t nq,
+__global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, const Sizes* S,
                                                                const uint8_t* __restrict__ em,
                                                                const uint32_t* __restrict__ newq,
-                                                               const int32_t* __restrict__ vmap, int64_t nv,
+                                                               const int32_t* __restrict__ vmap,
                                                                int32_t* __restrict__ quads2,
-                                                               int32_t* __restrict__ qmap,
-                                                               unsigned int* __restrict__ bad) {
+                                                               int32_t* __restrict__ qmap, unsigned int* bad) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         qmap[f] = em[f] ? -1 : (int32_t)newq[f];
         if (em[f]) continue;
@@ -657,11 +678,12 @@ __device__ __forceinline__ bool pattern3333(int64_t f, const int32_t* __restrict
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_mark3333(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_mark3333(const int32_t* __restrict__ quads, const Sizes* S,
                                                       const int32_t* __restrict__ twin, const uint8_t* __restrict__ flag,
                                                       const int32_t* __restrict__ val, int32_t* __restrict__ lastF,
                                                       uint8_t* __restrict__ vflag, uint8_t* __restrict__ p3333,
                                                       uint8_t* __restrict__ rflag) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         int32_t vq[4], vo[4], fn[4];
         const bool m = pattern3333(f, quads, twin, flag, val, nv, vq, vo, fn);
@@ -676,11 +698,12 @@ __global__ void __launch_bounds__(kBlock) k_mark3333(const int32_t* __restrict__
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__ quads, const Sizes* S,
                                                       const int32_t* __restrict__ twin, const uint8_t* __restrict__ flag,
                                                       const int32_t* __restrict__ val,
                                                       const uint8_t* __restrict__ p3333,
                                                       const int32_t* __restrict__ lastF, int32_t* __restrict__ vtwin) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         if (!p3333[f]) continue;
         int32_t vq[4], vo[4], fn[4];
@@ -692,15 +715,15 @@ __global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__
 }
 
 // removeQuadrilaterals3333 (cpp_dmc/DualMarchingCubes.cpp:1320-1360)
-__global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __restrict__ quads, int64_t nq,
+__global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __restrict__ quads, const Sizes* S,
                                                                const uint8_t* __restrict__ rflag,
                                                                const uint32_t* __restrict__ newq,
                                                                const uint8_t* __restrict__ p3333,
                                                                const int32_t* __restrict__ vtwin,
-                                                               const int32_t* __restrict__ vmap, int64_t nv,
+                                                               const int32_t* __restrict__ vmap,
                                                                int32_t* __restrict__ quads2,
-                                                               int32_t* __restrict__ qmap,
-                                                               unsigned int* __restrict__ bad) {
+                                                               int32_t* __restrict__ qmap, unsigned int* bad) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         qmap[f] = rflag[f] ? -1 : (int32_t)(newq[f] | (p3333[f] ? 0x80000000u : 0u));
         if (rflag[f]) continue;
@@ -739,21 +762,22 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
 // through vtwin); vmap[x] = the new id vertex x takes in the quadrilaterals
 // that are not pattern faces (-1 if none).  These are exactly the ids
 // k_compact_quads3x3y / k_compact_quads3333 write.
+// Sizes: nv / nq before the compaction, nn / qn after it (S->nv - S->nn
+// vertices removed, S->nq - S->qn quadrilaterals dropped).
 struct Remap {
     const int32_t* qmap;
     const int32_t* vmap;
     const uint8_t* vrem;        // removed vertices (vfirst / vflag)
     const int32_t* vtwin;       // 3333: replacement of a pattern face's vertex
     const int32_t* rlist;       // the removed vertices, ascending
-    int64_t nv;                 // vertices before
-    int64_t nrem;               // removed vertices
+    const Sizes* S;
 };
 
-__device__ __forceinline__ int32_t remap_vertex(const Remap& m, int32_t qm, int32_t x) {
-    if (!vid_ok(x, m.nv)) return -1;
+__device__ __forceinline__ int32_t remap_vertex(const Remap& m, int64_t nv, int32_t qm, int32_t x) {
+    if (!vid_ok(x, nv)) return -1;
     if (qm < -1) {   // pattern face (bit 31 on a non-negative index)
         x = m.vtwin[x];
-        if (!vid_ok(x, m.nv)) return -1;
+        if (!vid_ok(x, nv)) return -1;
     }
     return m.vmap[x];
 }
@@ -761,8 +785,9 @@ __device__ __forceinline__ int32_t remap_vertex(const Remap& m, int32_t qm, int3
 // 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
 // compaction), vsecond[vsecond[x]] for a removed one (its target's).
 __global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__ vfirst,
-                                                      const int32_t* __restrict__ vsecond, int64_t nv,
+                                                      const int32_t* __restrict__ vsecond, const Sizes* S,
                                                       int32_t* __restrict__ vmap) {
+    const int64_t nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(x, nv) {
         int32_t y = vsecond[x];
         if (vfirst[x]) y = vid_ok(y, nv) ? vsecond[y] : -1;
@@ -772,8 +797,9 @@ __global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__
 
 // 3333: vmap[x] = new id of a kept x, -1 for a removed one.
 __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__ vflag,
-                                                      const uint32_t* __restrict__ newv, int64_t nv,
+                                                      const uint32_t* __restrict__ newv, const Sizes* S,
                                                       int32_t* __restrict__ vmap) {
+    const int64_t nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(x, nv) vmap[x] = vflag[x] ? -1 : (int32_t)newv[x];
 }
 
@@ -781,23 +807,30 @@ __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__
 // streaming pass), less one for each of its corners in a dropped quadrilateral
 // (k_remap_drop, over the list of dropped ones).  The removed vertices then
 // add their surviving entries (k_remap_degree, pass 1).
+// Remap launches run only when the rows they read are complete (*gate:
+// the previous build saw no invalid vertex reference).
 __global__ void __launch_bounds__(kBlock) k_remap_degree_kept(Remap m, const uint32_t* __restrict__ off,
-                                                               uint32_t* __restrict__ deg2) {
-    TMC_GRID_STRIDE(x, m.nv) {
+                                                               uint32_t* __restrict__ deg2,
+                                                               const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t nv = (int64_t)m.S->nv;
+    TMC_GRID_STRIDE(x, nv) {
         if (m.vrem[x]) continue;
         deg2[m.vmap[x]] = off[x + 1] - off[x];
     }
 }
 
 __global__ void __launch_bounds__(kBlock) k_remap_drop(Remap m, const int32_t* __restrict__ quads,
-                                                        const int32_t* __restrict__ dlist, int64_t nd,
-                                                        uint32_t* __restrict__ deg2) {
+                                                        const int32_t* __restrict__ dlist, uint32_t* __restrict__ deg2,
+                                                        const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t nv = (int64_t)m.S->nv, nd = (int64_t)(m.S->nq - m.S->qn);
     TMC_GRID_STRIDE(i, nd) {
         const int4 q = reinterpret_cast<const int4*>(quads)[dlist[i]];
         const int32_t v[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            if (vid_ok(v[c], m.nv) && !m.vrem[v[c]]) atomicSub(&deg2[m.vmap[v[c]]], 1u);
+            if (vid_ok(v[c], nv) && !m.vrem[v[c]]) atomicSub(&deg2[m.vmap[v[c]]], 1u);
     }
 }
 
@@ -806,11 +839,13 @@ __global__ void __launch_bounds__(kBlock) k_remap_drop(Remap m, const int32_t* _
 // Rows are walked four entries at a time with every lookup of a chunk
 // issued before any is used.
 __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t* __restrict__ off,
-                                                          const int2* __restrict__ adj, int64_t nn,
-                                                          uint32_t* __restrict__ deg2, int pass,
-                                                          unsigned int* __restrict__ bad) {
+                                                          const int2* __restrict__ adj, uint32_t* __restrict__ deg2,
+                                                          int pass, unsigned int* bad,
+                                                          const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t nv = (int64_t)m.S->nv, nn = (int64_t)m.S->nn;
     // pass 0 visits every old vertex, pass 1 the list of removed ones
-    TMC_GRID_STRIDE(x64, pass == 0 ? m.nv : m.nrem) {
+    TMC_GRID_STRIDE(x64, pass == 0 ? nv : nv - nn) {
         const int32_t x = pass == 0 ? (int32_t)x64 : m.rlist[x64];
         const bool kept = pass == 0 && !m.vrem[x];
         if (pass == 0 && !kept) continue;
@@ -826,7 +861,7 @@ __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t
                 if (pass == 0) {
                     c++;
                 } else {
-                    const int32_t y = remap_vertex(m, qm[i], x);
+                    const int32_t y = remap_vertex(m, nv, qm[i], x);
                     if (vid_ok(y, nn)) atomicAdd(&deg2[y], 1u);
                     else atomicAdd(bad, 1u);
                 }
@@ -837,11 +872,13 @@ __global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t
 }
 
 __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* __restrict__ off,
-                                                        const int2* __restrict__ adj, int64_t nn,
+                                                        const int2* __restrict__ adj,
                                                         const uint32_t* __restrict__ off2,
                                                         uint32_t* __restrict__ cursor2, int2* __restrict__ adj2,
-                                                        int pass) {
-    TMC_GRID_STRIDE(x64, pass == 0 ? m.nv : m.nrem) {
+                                                        int pass, const uint32_t* __restrict__ gate) {
+    if (gated_off(gate)) return;
+    const int64_t nv = (int64_t)m.S->nv, nn = (int64_t)m.S->nn;
+    TMC_GRID_STRIDE(x64, pass == 0 ? nv : nv - nn) {
         const int32_t x = pass == 0 ? (int32_t)x64 : m.rlist[x64];
         const bool kept = pass == 0 && !m.vrem[x];
         if (pass == 0 && !kept) continue;
@@ -857,7 +894,7 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
 #pragma unroll
             for (int i = 0; i < 4; i++) qm[i] = m.qmap[a[i].x >> 2];
 #pragma unroll
-            for (int i = 0; i < 4; i++) dst[i] = remap_vertex(m, qm[i], a[i].y);
+            for (int i = 0; i < 4; i++) dst[i] = remap_vertex(m, nv, qm[i], a[i].y);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 if (r + (uint32_t)i >= e || qm[i] == -1) continue;
@@ -865,7 +902,7 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
                 if (pass == 0) {
                     adj2[p++] = out;
                 } else {
-                    const int32_t y = remap_vertex(m, qm[i], x);
+                    const int32_t y = remap_vertex(m, nv, qm[i], x);
                     if (!vid_ok(y, nn)) continue;
                     const uint32_t slot = atomicAdd(&cursor2[y], 1u);
                     adj2[off2[y] + slot] = out;
@@ -873,6 +910,38 @@ __global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* 
             }
         }
         if (pass == 0) cursor2[y0] = p - p0;
+    }
+}
+
+// Before a halfedge build: which rows it uses (ready: 1 the cell stage's,
+// 2 a remap's, 0 none) and its statistics, which start from the invalid
+// references the remap met.
+__global__ void k_rows_gate(Sizes* S, int ready) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool remapped = ready == 2 && S->remap_ok;
+        S->generic = ready == 0 ? 1u : ready == 1 ? (S->unresolved != 0u ? 1u : 0u) : (remapped ? 0u : 1u);
+        S->he.nonmanifold = 0;
+        S->he.errors = 0;
+        S->he.three = 0;
+        S->he.bad = remapped ? S->rbad : 0u;
+        S->he.pad = 0;
+    }
+}
+
+// Before a remap: it runs only if the last build saw no invalid vertex
+// reference (its rows then hold every surviving halfedge).
+__global__ void k_remap_gate(Sizes* S) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->remap_ok = S->he.bad == 0u ? 1u : 0u;
+        S->rbad = 0;
+    }
+}
+
+// The compaction is done: the kept counts become the live mesh.
+__global__ void k_commit(Sizes* S) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->nv = S->nn;
+        S->nq = S->qn;
     }
 }
 
@@ -1089,8 +1158,9 @@ __device__ __forceinline__ double tri_min_angle(float a, float b, float c) {
     return (C < m) ? C : m;
 }
 
-__global__ void __launch_bounds__(kBlock) k_triangles(const int32_t* __restrict__ quads, int64_t nq, int64_t nv,
+__global__ void __launch_bounds__(kBlock) k_triangles(const int32_t* __restrict__ quads, const Sizes* S,
                                                        const double* __restrict__ pos, int32_t* __restrict__ tris) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(f, nq) {
         const int4 q = reinterpret_cast<const int4*>(quads)[f];
         int32_t* t = tris + 6 * f;

@@ -37,6 +37,11 @@ using Clock = std::chrono::steady_clock;
 // Upper bound of dmc_server_opts.workers (each worker holds a context's
 // buffers, grown to its largest pass).
 constexpr int kMaxWorkers = 8;
+// Default bound of a pass's grid points (dmc_server_opts.max_batch_points
+// = 0): keeps a pass's union well inside the engine's index limits (2^32
+// active cells, 2^31 vertices, 2^29 quadrilaterals) for any mix of
+// requests, each of which fits alone.
+constexpr int64_t kDefaultBatchPoints = (int64_t)1 << 28;
 
 // One pooled device allocation holding the meshes of one batch.
 struct Block {
@@ -78,6 +83,7 @@ struct Worker {
     dmc_context* ctx = nullptr;
     hipStream_t copy_stream = nullptr;
     std::thread th;
+    size_t pool_bytes = 0;   // its context's buffer pools after its last pass
 };
 
 struct dmc_server {
@@ -89,8 +95,10 @@ struct dmc_server {
     std::deque<Request*> queue;
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     std::vector<Block*> blocks;         // every block (pooled)
-    size_t held_bytes = 0;              // results not yet released
-    size_t inflight_bytes = 0;          // working sets of the passes running
+    size_t held_bytes = 0;              // result blocks holding unreleased results (their full capacity)
+    size_t idle_bytes = 0;              // result blocks free for reuse
+    size_t ctx_bytes = 0;               // buffer pools the worker contexts keep between passes
+    size_t inflight_bytes = 0;          // estimated working sets of the passes running
     uint64_t next_ticket = 1;
     bool stop = false;
     Clock::time_point t0 = Clock::now();
@@ -113,12 +121,15 @@ size_t block_bytes(int64_t V, int64_t Q) {
 }
 
 // A free block of at least `bytes` (smallest fit), or a new one.  Called
-// with the lock held.
+// with the lock held; the block leaves the idle pool.
 Block* get_block(dmc_server* s, size_t bytes) {
     Block* best = nullptr;
     for (Block* b : s->blocks)
         if (b->refs == 0 && b->cap >= bytes && (!best || b->cap < best->cap)) best = b;
-    if (best) return best;
+    if (best) {
+        s->idle_bytes -= best->cap;
+        return best;
+    }
     Block* b = new Block();
     if (hipMalloc(&b->p, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -130,7 +141,42 @@ Block* get_block(dmc_server* s, size_t bytes) {
     return b;
 }
 
-void run_batch(dmc_server* s, Worker* w, std::vector<Request*>& batch) {
+// Free the idle result blocks (lock held).
+void free_idle_blocks(dmc_server* s) {
+    std::vector<Block*> keep;
+    for (Block* b : s->blocks) {
+        if (b->refs == 0) {
+            (void)hipFree(b->p);
+            delete b;
+        } else {
+            keep.push_back(b);
+        }
+    }
+    s->blocks.swap(keep);
+    s->idle_bytes = 0;
+}
+
+// Device memory the server holds or is about to use (lock held): result
+// blocks (held and idle), the worker contexts' pools and the estimated
+// working sets of the passes running.
+size_t committed(const dmc_server* s) { return s->held_bytes + s->idle_bytes + s->ctx_bytes + s->inflight_bytes; }
+
+// Whether `est` more bytes fit the budget, freeing idle blocks if that
+// makes them fit (lock held).
+bool fits(dmc_server* s, size_t est) {
+    const size_t budget = s->opts.budget_bytes;
+    if (est + committed(s) <= budget) return true;
+    if (s->idle_bytes > 0 && est + committed(s) - s->idle_bytes <= budget) {
+        free_idle_blocks(s);
+        return true;
+    }
+    return false;
+}
+
+// One pass over `batch`: its status (every member gets it).  With
+// complete = false a failed pass leaves its members open (the caller
+// retries them one by one).
+dmc_status run_pass(dmc_server* s, Worker* w, std::vector<Request*>& batch, bool complete) {
     const int n = (int)batch.size();
     std::vector<dmc_volume> vols(n);
     for (int i = 0; i < n; i++) vols[i] = batch[i]->vol;
@@ -204,8 +250,24 @@ void run_batch(dmc_server* s, Worker* w, std::vector<Request*>& batch) {
         }
     }
     const double t_done = s->now();
+    size_t cb = 0;
+    (void)dmc_context_bytes(w->ctx, &cb);
     std::lock_guard<std::mutex> lk(s->mu);
     s->batches++;
+    s->ctx_bytes += cb;
+    s->ctx_bytes -= w->pool_bytes;
+    w->pool_bytes = cb;
+    if (st != DMC_OK) {
+        s->err = dmc_last_error(w->ctx);
+        if (blk) {
+            blk->refs = 0;
+            s->idle_bytes += blk->cap;
+        }
+        if (!complete) return st;
+    }
+    // each member is charged its share of the block's whole capacity
+    size_t left = blk ? blk->cap : 0;
+    const int64_t tot = std::max<int64_t>(1, V + Q);
     for (int i = 0; i < n; i++) {
         Request* r = batch[i];
         r->t_start = t_start;
@@ -216,30 +278,60 @@ void run_batch(dmc_server* s, Worker* w, std::vector<Request*>& batch) {
             r->mesh = out[i];
             r->bnd = std::move(bnd[i]);
             r->block = blk;
-            r->held = block_bytes(out[i].nv, out[i].nq);
+            r->held = i + 1 == n ? left : (size_t)((double)blk->cap * (double)(out[i].nv + out[i].nq) / (double)tot);
+            r->held = std::min(r->held, left);
+            left -= r->held;
             s->held_bytes += r->held;
         } else {
             r->mesh = dmc_mesh{};
-            if (blk) blk->refs--;
-            s->err = dmc_last_error(w->ctx);
         }
         r->done = true;
         s->served++;
     }
     s->cv_done.notify_all();
+    return st;
+}
+
+// A pass over the admitted requests.  A batch that fails on a size limit of
+// the union (DMC_ERR_CAPACITY) or on one bad member (DMC_ERR_INVALID_ARG)
+// is run again one request at a time, so that only the requests that fail
+// alone fail.
+void run_batch(dmc_server* s, Worker* w, std::vector<Request*>& batch) {
+    const bool retry = batch.size() > 1;
+    const dmc_status st = run_pass(s, w, batch, !retry);
+    if (st == DMC_OK || !retry) return;
+    if (st != DMC_ERR_CAPACITY && st != DMC_ERR_INVALID_ARG) {
+        // complete every member with the pass's status
+        std::vector<Request*> none;
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (Request* r : batch) {
+            r->t_start = r->t_done = s->now();
+            r->batch_size = (int32_t)batch.size();
+            r->status = st;
+            r->mesh = dmc_mesh{};
+            r->done = true;
+            s->served++;
+        }
+        s->cv_done.notify_all();
+        return;
+    }
+    for (Request* r : batch) {
+        std::vector<Request*> one{r};
+        run_pass(s, w, one, true);
+    }
 }
 
 void worker_main(dmc_server* s, Worker* w) {
     (void)hipSetDevice(s->opts.device);
     const size_t budget = s->opts.budget_bytes;
     // a pass may start when no other is running, or when the oldest request
-    // fits beside the passes in flight and the held results (one that can
-    // never fit is failed at once)
+    // fits beside what the server already holds (one that can never fit is
+    // failed at once)
     auto ready = [&] {
         if (s->stop) return true;
         if (s->queue.empty()) return false;
         const size_t est = s->queue.front()->est;
-        return s->inflight_bytes == 0 || est > budget || est + s->inflight_bytes + s->held_bytes <= budget;
+        return s->inflight_bytes == 0 || est > budget || fits(s, est);
     };
     for (;;) {
         std::vector<Request*> batch;
@@ -248,8 +340,10 @@ void worker_main(dmc_server* s, Worker* w) {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, ready);
             if (s->stop && s->queue.empty()) return;
-            // admission: oldest first, while the working set plus the passes
-            // in flight and the held results fit the budget
+            // admission: oldest first, while the working set fits beside what
+            // the server holds (results, idle blocks, worker pools, passes in
+            // flight)
+            const int64_t max_points = s->opts.max_batch_points > 0 ? s->opts.max_batch_points : kDefaultBatchPoints;
             int64_t points = 0;
             while (!s->queue.empty() && (int)batch.size() < s->opts.max_batch) {
                 Request* r = s->queue.front();
@@ -262,8 +356,8 @@ void worker_main(dmc_server* s, Worker* w) {
                     continue;
                 }
                 const int64_t pts = (int64_t)r->vol.grid.nx * r->vol.grid.ny * r->vol.grid.nz;
-                if (!batch.empty() && used + r->est + s->inflight_bytes + s->held_bytes > budget) break;
-                if (!batch.empty() && s->opts.max_batch_points > 0 && points + pts > s->opts.max_batch_points) break;
+                if (!batch.empty() && !fits(s, used + r->est)) break;
+                if (!batch.empty() && points + pts > max_points) break;
                 batch.push_back(r);
                 used += r->est;
                 points += pts;
@@ -350,9 +444,24 @@ dmc_status dmc_server_destroy(dmc_server* s) {
 
 dmc_status dmc_server_submit(dmc_server* s, const dmc_volume* vol, uint64_t* ticket) {
     if (!s || !vol || !ticket || !vol->field) return DMC_ERR_INVALID_ARG;
+    // the engine's grid rules, checked here so that a request that can
+    // never run fails alone instead of failing the pass it would join
     const dmc_grid& g = vol->grid;
-    if (g.nx < 2 || g.ny < 2 || g.nz < 2) return DMC_ERR_INVALID_ARG;
-    if ((s->opts.flags & DMC_GRADIENTS_SUPPLIED) && !vol->gradients) return DMC_ERR_INVALID_ARG;
+    const char* bad = nullptr;
+    dmc_status st = DMC_ERR_INVALID_ARG;
+    if (g.nx < 2 || g.ny < 2 || g.nz < 2) bad = "grid needs at least 2 points per axis";
+    else if (g.nx > (1 << 24) || g.ny > (1 << 24) || g.nz > (1 << 24)) bad = "an axis above 16777216 points";
+    else if ((uint64_t)(g.nx - 1) * (uint64_t)(g.ny - 1) * (uint64_t)(g.nz - 1) > 0xFFFFFFFFull) {
+        bad = "more than 2^32 cells";
+        st = DMC_ERR_CAPACITY;
+    } else if ((s->opts.flags & DMC_GRADIENTS_SUPPLIED) && !vol->gradients) {
+        bad = "gradients are NULL";
+    }
+    if (bad) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = bad;
+        return st;
+    }
     auto r = std::make_unique<Request>();
     r->vol = *vol;
     r->est = estimate_bytes(*vol, s->opts.field_type, (s->opts.flags & DMC_FIELD_ON_DEVICE) != 0);
@@ -460,7 +569,7 @@ dmc_status dmc_server_release(dmc_server* s, uint64_t ticket) {
     if (it == s->reqs.end()) return DMC_ERR_INVALID_ARG;
     Request* r = it->second.get();
     if (!r->done) return DMC_ERR_STATE;
-    if (r->block) r->block->refs--;
+    if (r->block && --r->block->refs == 0) s->idle_bytes += r->block->cap;
     s->held_bytes -= r->held;
     s->reqs.erase(it);
     return DMC_OK;
@@ -470,7 +579,7 @@ dmc_status dmc_server_stats(dmc_server* s, uint64_t* batches, uint64_t* served, 
                             size_t* pool_bytes) {
     if (!s) return DMC_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    size_t pool = 0;
+    size_t pool = s->ctx_bytes;
     for (Block* b : s->blocks) pool += b->cap;
     if (batches) *batches = s->batches;
     if (served) *served = s->served;
