@@ -134,6 +134,48 @@ typedef struct {
 dmc_status dmc_extract_batch(dmc_context* ctx, const dmc_volume* vols, int32_t nvol, int field_type, uint32_t flags,
                              dmc_mesh* out);
 
+/* ------------------------------------------------------------------------
+ * Extraction that allocates nothing.  No counterpart in the reference
+ * (whose dualMC grows std::vectors and, in dmc/, cudaMallocs its tables on
+ * every call); it is for callers that cannot allocate device memory:
+ * tight serving loops and harnesses that forbid hipMalloc.  It runs the same kernels as dmc_extract on one volume, with
+ * every working buffer carved from a fixed scratch array that the library
+ * holds in device memory on each GPU it runs on (64 MiB, until the process
+ * ends), and writes the mesh into the caller's device buffers.
+ *
+ * Limits: at most 2,097,152 grid points (128^3); the working set of the
+ * field's surface must fit the 64 MiB scratch (a smooth surface on 128^3
+ * does, about 40 MiB; dense noise fits up to about 64^3).  Beyond either,
+ * DMC_ERR_CAPACITY with the reason in dmc_last_error.  The field (and
+ * gradients, DMC_GRADIENTS_SUPPLIED) must be device pointers; the flag
+ * DMC_FIELD_ON_DEVICE is implied and DMC_STAGE_TIMING is ignored.
+ * Calls on one GPU share its scratch array and are serialised inside the
+ * library (each returns with its work finished); the array keeps the last
+ * call's contents, and nothing in it is assumed zero.  The context's own
+ * buffer pools are neither used nor grown; after the call, downloads and
+ * diagnostics of the context return DMC_ERR_STATE until the next
+ * dmc_extract.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    double* positions;   /* device, 3 * cap_v doubles */
+    double* normals;     /* device, 3 * cap_v doubles */
+    int32_t* quads;      /* device, 4 * cap_q */
+    int32_t* tris;       /* device, 6 * cap_q with DMC_TRIANGLES, else may be NULL */
+    int32_t* twins;      /* device, 4 * cap_q: twin of halfedge 4q+k, -1 on the boundary */
+    uint8_t* he_flags;   /* device, 4 * cap_q */
+    int32_t* he_v;       /* device, cap_v */
+    int32_t* bnd;        /* host, cap_bnd entries of getBndVertices(); may be NULL */
+    int64_t cap_v, cap_q, cap_bnd;
+} dmc_out_buffers;
+
+/* As dmc_extract for one device-resident volume, allocating nothing.  `out`
+ * receives the counts and the caller's buffer pointers; when the mesh does
+ * not fit cap_v / cap_q (or the boundary list cap_bnd), DMC_ERR_CAPACITY
+ * with out->nv / out->nq (out->nbnd) set to the sizes required. */
+dmc_status dmc_extract_noalloc(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type,
+                               double iso, uint32_t flags, const double* gradients,
+                               const dmc_out_buffers* out_buffers, dmc_mesh* out);
+
 /* Device bytes a context holds in its buffer pools (grown to its largest
  * extraction and kept for reuse until dmc_destroy).  No counterpart in the
  * reference; the serving loop counts it in its memory budget. */

@@ -44,7 +44,7 @@ EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc
            "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
            "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj",
            "tmc_extract", "tmc_download", "dmc_get_launch", "dmc_set_launch", "dmc_autotune",
-           "dmc_context_bytes"]
+           "dmc_extract_noalloc", "dmc_context_bytes"]
 
 
 class DMCError(RuntimeError):
@@ -96,6 +96,13 @@ class Mesh(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("d_") and f not in self.RUN_FIELDS}
 
 
+class OutBuffers(C.Structure):
+    """dmc_out_buffers: caller-owned device outputs of dmc_extract_noalloc."""
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("quads", C.c_void_p), ("tris", C.c_void_p),
+                ("twins", C.c_void_p), ("he_flags", C.c_void_p), ("he_v", C.c_void_p), ("bnd", C.c_void_p),
+                ("cap_v", C.c_int64), ("cap_q", C.c_int64), ("cap_bnd", C.c_int64)]
+
+
 class Launch(C.Structure):
     _fields_ = [("grid_cap", C.c_int32), ("sweep_waves", C.c_int32)]
 
@@ -131,6 +138,8 @@ def load_library(path=ENGINE_SO):
     L.dmc_extract.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, vp, C.POINTER(Mesh)]
     L.dmc_mesh_download.argtypes = [vp, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     L.dmc_extract_batch.argtypes = [vp, C.POINTER(Volume), C.c_int32, C.c_int, C.c_uint32, C.POINTER(Mesh)]
+    L.dmc_extract_noalloc.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, vp,
+                                      C.POINTER(OutBuffers), C.POINTER(Mesh)]
     L.dmc_context_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.dmc_batch_download.argtypes = [vp, C.c_int32, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     u64 = C.c_uint64
@@ -173,7 +182,7 @@ def load_library(path=ENGINE_SO):
                  "dmc_server_stats", "dmc_mesh_download_f32",
                  "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
                  "dmc_copy_to_host", "dmc_synchronize", "dmc_mesh_edges", "dmc_valence_histogram",
-                 "dmc_element_quality", "dmc_write_obj", "dmc_context_bytes"):
+                 "dmc_element_quality", "dmc_write_obj", "dmc_extract_noalloc", "dmc_context_bytes"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -424,11 +433,40 @@ class Engine:
         self.batch = list(meshes)
         return self.batch
 
+    def extract_noalloc(self, field, iso, grid, outs, flags=SIMPLIFY_ALL | TRIANGLES, dtype=np.float32,
+                        gradients=None, bnd_cap=1 << 16):
+        """dmc_extract_noalloc: `field` is a device pointer (int) or
+        DeviceBuffer holding the volume, `outs` an OutBuffers of device
+        buffers the caller owns.  Returns (Mesh, boundary vertices)."""
+        fptr = field.ptr if isinstance(field, DeviceBuffer) else C.c_void_p(int(field))
+        gptr = None
+        if gradients is not None:
+            gptr = gradients.ptr if isinstance(gradients, DeviceBuffer) else C.c_void_p(int(gradients))
+            flags |= GRADIENTS_SUPPLIED
+        ftype = FIELD_F32 if np.dtype(dtype) == np.float32 else FIELD_F64
+        if not isinstance(grid, Grid):
+            grid = Grid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.dx, grid.dy, grid.dz)
+        bnd = np.empty(bnd_cap, np.int32)
+        outs.bnd = bnd.ctypes.data
+        outs.cap_bnd = bnd_cap
+        m = Mesh()
+        self._check(self.lib.dmc_extract_noalloc(self.ctx, C.byref(grid), fptr, ftype, float(iso), int(flags), gptr,
+                                                 C.byref(outs), C.byref(m)))
+        self.mesh = None
+        self.batch = None
+        return m, bnd[:m.nbnd].copy()
+
     def context_bytes(self):
         """Device bytes held by this context's buffer pools."""
         n = C.c_size_t()
         self._check(self.lib.dmc_context_bytes(self.ctx, C.byref(n)))
         return n.value
+
+    def copy_to_host(self, dst, src_ptr):
+        """Copy dst.nbytes from device address src_ptr into numpy array dst."""
+        if dst.nbytes:
+            self._check(self.lib.dmc_copy_to_host(self.ctx, dst.ctypes.data, C.c_void_p(int(src_ptr)), dst.nbytes))
+        return dst
 
     def download_batch(self, index, halfedges=True):
         """Copy volume `index` of the last extract_batch to host arrays."""

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -52,6 +53,34 @@ struct StageMark {
     const char* name;
     hipEvent_t ev;
 };
+
+// Scratch of dmc_extract_noalloc: one array per device, in the code object,
+// so that the entry point needs no device allocation.  Host code never uses
+// the name `scratch` as an address: each device's address comes from
+// hipGetSymbolAddress, taken once per device at context creation with that
+// device current (scratch_init, kept in g_scratch_ptr).  Calls on one
+// device share the array; g_scratch_mu serialises them (each call leaves
+// the stream drained).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int64_t kNoAllocMaxPoints = (int64_t)1 << 21;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu[kMaxDevices];
+std::mutex g_scratch_init_mu;
+
+// The device's scratch address (device current); nullptr if it cannot be
+// had.
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -136,6 +165,10 @@ struct dmc_context {
     std::vector<hipEvent_t> evpool;   // events reused across extractions
     size_t evused = 0;
     std::vector<std::pair<const char*, float>> stage_ms;
+    // arena mode (dmc_extract_noalloc): buffers are carved from a fixed
+    // device region instead of allocated
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 
     dmc_context() {
         Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &aaux, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
@@ -175,6 +208,19 @@ void fail(dmc_context* ctx, dmc_status s, const char* fmt, ...) {
 template <class T>
 T* ensure(dmc_context* ctx, Buf& b, size_t n) {
     const size_t bytes = std::max<size_t>(n * sizeof(T), 256);
+    if (ctx->arena) {
+        // carve (never free): a buffer that must grow takes a new piece
+        if (b.cap < bytes) {
+            const size_t want = (bytes + 255) & ~(size_t)255;
+            if (ctx->arena_used + want > ctx->arena_cap)
+                fail(ctx, DMC_ERR_CAPACITY, "no-alloc scratch of %zu bytes exhausted (%zu used, %zu more needed)",
+                     ctx->arena_cap, ctx->arena_used, want);
+            b.p = ctx->arena + ctx->arena_used;
+            b.cap = want;
+            ctx->arena_used += want;
+        }
+        return static_cast<T*>(b.p);
+    }
     if (b.cap < bytes) {
         if (ctx->capturing) fail(ctx, DMC_ERR_STATE, "a buffer would grow inside a captured pass");
         ctx->buf_epoch++;
@@ -259,6 +305,17 @@ class HostReads {
     }
     void wait() {
         if (outs_.empty()) return;
+        if (ctx_->pin_cap < used_ && ctx_->arena) {
+            // no-alloc call: the pinned staging is not grown; copy straight
+            // into the (pageable) destinations
+            for (const Out& o : outs_)
+                HIPC(ctx_, hipMemcpyAsync(o.dst, o.src, o.bytes, hipMemcpyDeviceToHost, ctx_->stream));
+            HIPC(ctx_, hipStreamSynchronize(ctx_->stream));
+            ctx_->syncs++;
+            outs_.clear();
+            used_ = 0;
+            return;
+        }
         if (ctx_->pin_cap < used_) {
             HIPC(ctx_, hipStreamSynchronize(ctx_->stream));
             if (ctx_->pin) {
@@ -1224,7 +1281,7 @@ void drop_graphs(dmc_context* ctx) {
 // One pass: eager, captured and launched, or replayed.
 void run_pass(dmc_context* ctx, const SweepPlan& P, int field_type, uint32_t flags) {
     ctx->pass_mode = 0;
-    const bool graphs = ctx->use_graphs && !ctx->timing;
+    const bool graphs = ctx->use_graphs && !ctx->timing && !ctx->arena;
     std::string key;
     if (graphs) {
         // hv is only needed for the key: rebuild it from the volume table
@@ -1355,6 +1412,11 @@ void do_extract(dmc_context* ctx, void* a_) {
     // a pass that outgrew a capacity learned the exact sizes: once more
     bool done = false;
     for (int attempt = 0; attempt < 3 && !done; attempt++) {
+        if (attempt > 0 && ctx->arena) {
+            // no-alloc call: the failed attempt's pieces are dead; carve anew
+            for (Buf* b : ctx->bufs) *b = Buf{};
+            ctx->arena_used = 0;
+        }
         const SweepPlan P = a.field_type == DMC_FIELD_F32 ? plan_sweep<float>(ctx, hv) : plan_sweep<double>(ctx, hv);
         plan_caps(ctx, P.cells);
         run_pass(ctx, P, a.field_type, a.flags);
@@ -1410,8 +1472,9 @@ dmc_status dmc_create(dmc_context** out, int device) {
         delete ctx;
         return DMC_ERR_HIP;
     }
-    // pinned staging of count reads
-    if (hipHostMalloc(&ctx->pin, 64 << 10) != hipSuccess) {
+    // pinned staging of count reads, and this device's no-alloc scratch
+    // address (once per device)
+    if (hipHostMalloc(&ctx->pin, 64 << 10) != hipSuccess || scratch_init(device) != hipSuccess) {
         (void)hipGetLastError();
         dmc_destroy(ctx);
         return DMC_ERR_HIP;
@@ -1448,6 +1511,102 @@ dmc_status dmc_extract_batch(dmc_context* ctx, const dmc_volume* vols, int32_t n
     if (!ctx || !vols || nvol < 1) return DMC_ERR_INVALID_ARG;
     ExtractArgs a{vols, nvol, field_type, flags, out};
     return guard(ctx, do_extract, &a);
+}
+
+struct NoAllocArgs {
+    ExtractArgs x;
+    const dmc_out_buffers* ob;
+};
+
+// dmc_extract with every working buffer carved from this device's scratch
+// array and the mesh copied into the caller's buffers.  The context's own
+// buffers are set aside for the call and put back after it, so its pools
+// are neither used nor grown.
+static void do_extract_noalloc(dmc_context* ctx, void* a_) {
+    NoAllocArgs& a = *static_cast<NoAllocArgs*>(a_);
+    const dmc_grid& g = a.x.vols[0].grid;
+    check_grid(ctx, g);
+    const int64_t npts = (int64_t)g.nx * g.ny * g.nz;
+    if (npts > kNoAllocMaxPoints)
+        fail(ctx, DMC_ERR_CAPACITY, "dmc_extract_noalloc takes at most %lld grid points (got %lld)",
+             (long long)kNoAllocMaxPoints, (long long)npts);
+    const dmc_out_buffers& ob = *a.ob;
+    const bool tris = (a.x.flags & DMC_TRIANGLES) != 0;
+    if (!ob.positions || !ob.normals || !ob.quads || !ob.twins || !ob.he_flags || !ob.he_v || (tris && !ob.tris) ||
+        ob.cap_v < 0 || ob.cap_q < 0)
+        fail(ctx, DMC_ERR_INVALID_ARG, "output buffers missing");
+    void* base = g_scratch_ptr[ctx->device];
+    if (!base) fail(ctx, DMC_ERR_STATE, "no scratch address for device %d", ctx->device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu[ctx->device]);
+    std::vector<Buf> saved;
+    saved.reserve(ctx->bufs.size());
+    for (Buf* b : ctx->bufs) {
+        saved.push_back(*b);
+        *b = Buf{};
+    }
+    ctx->arena = static_cast<char*>(base);
+    ctx->arena_cap = kScratchBytes;
+    ctx->arena_used = 0;
+    auto restore = [&] {
+        // nothing of the call may still run on the scratch when it returns
+        (void)hipStreamSynchronize(ctx->stream);
+        for (size_t i = 0; i < saved.size(); i++) *ctx->bufs[i] = saved[i];
+        ctx->arena = nullptr;
+        ctx->arena_cap = ctx->arena_used = 0;
+        ctx->have_mesh = false;
+        ctx->have_tmesh = false;
+        ctx->vmesh.clear();
+        ctx->vbnd.clear();
+    };
+    try {
+        do_extract(ctx, &a.x);
+        dmc_mesh m = ctx->vmesh[0];
+        if (m.nv > ob.cap_v || m.nq > ob.cap_q) {
+            *a.x.out = m;
+            fail(ctx, DMC_ERR_CAPACITY, "mesh of %lld vertices, %lld quadrilaterals exceeds the output buffers "
+                 "(%lld, %lld)", (long long)m.nv, (long long)m.nq, (long long)ob.cap_v, (long long)ob.cap_q);
+        }
+        auto d2d = [&](void* dst, const void* src, size_t bytes) {
+            if (bytes) HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        };
+        d2d(ob.positions, m.d_positions, 24 * (size_t)m.nv);
+        d2d(ob.normals, m.d_normals, 24 * (size_t)m.nv);
+        d2d(ob.quads, m.d_quads, 16 * (size_t)m.nq);
+        if (tris) d2d(ob.tris, m.d_tris, 12 * (size_t)m.nt);
+        d2d(ob.twins, m.d_twins, 16 * (size_t)m.nq);
+        d2d(ob.he_flags, m.d_he_flags, 4 * (size_t)m.nq);
+        d2d(ob.he_v, m.d_he_v, 4 * (size_t)m.nv);
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        m.d_positions = ob.positions;
+        m.d_normals = ob.normals;
+        m.d_quads = ob.quads;
+        m.d_tris = tris ? ob.tris : nullptr;
+        m.d_twins = ob.twins;
+        m.d_he_flags = ob.he_flags;
+        m.d_he_v = ob.he_v;
+        *a.x.out = m;
+        if (ob.bnd) {
+            if ((int64_t)ctx->vbnd[0].size() > ob.cap_bnd)
+                fail(ctx, DMC_ERR_CAPACITY, "%lld boundary vertices exceed bnd's %lld", (long long)ctx->vbnd[0].size(),
+                     (long long)ob.cap_bnd);
+            if (!ctx->vbnd[0].empty()) memcpy(ob.bnd, ctx->vbnd[0].data(), ctx->vbnd[0].size() * sizeof(int32_t));
+        }
+    } catch (...) {
+        restore();
+        throw;
+    }
+    restore();
+}
+
+dmc_status dmc_extract_noalloc(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type, double iso,
+                               uint32_t flags, const double* gradients, const dmc_out_buffers* out_buffers,
+                               dmc_mesh* out) {
+    if (!ctx || !grid || !out_buffers || !out) return DMC_ERR_INVALID_ARG;
+    const dmc_volume vol{*grid, field, gradients, iso};
+    // device field, no stage events (they would be created)
+    const uint32_t f = (flags | DMC_FIELD_ON_DEVICE) & ~(uint32_t)DMC_STAGE_TIMING;
+    NoAllocArgs a{ExtractArgs{&vol, 1, field_type, f, out}, out_buffers};
+    return guard(ctx, do_extract_noalloc, &a);
 }
 
 dmc_status tmc_extract(dmc_context* ctx, const dmc_grid* grid, const void* field, int field_type, double iso,
