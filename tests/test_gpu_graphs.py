@@ -1,4 +1,4 @@
-"""Captured passes: the second pass of a shape (volume dimensions, element
+"""Captured passes: the third pass of a shape (volume dimensions, element
 type, flags, launch shape, capacities, buffers) is captured into a HIP
 graph and launched; later passes of that shape replay it with their own
 fields and iso-values, which the kernels read from the device volume
@@ -69,10 +69,13 @@ def test_replays_match_eager():
                 assert_same(d[k], e[k], f"pass {i} ({name}, mode {m.pass_mode}): {k}")
             eng.synchronize()
         _say(f"[gpu] pass modes {modes}")
-        # a: eager, captured, then replayed; b (same shape as a) replays a's graph
-        assert modes[0][1] == 0 and modes[1][1] == 1
-        assert modes[3][1] == 2 and modes[4][1] == 2
-        assert sum(1 for _, m in modes if m == 2) >= 5
+        # a's shape (which b shares) is captured within its first two passes
+        # here and replayed from then on, also for b; c is captured too
+        ms = [m for _, m in modes]
+        assert 1 in ms[:2], ms
+        first = ms.index(1)
+        assert all(m == 2 for (n, m) in modes[first + 1:] if n in "ab"), modes
+        assert ms.count(2) >= 6 and ms.count(1) == 2, modes
     finally:
         eng.close()
 

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -43,8 +44,13 @@ constexpr int kGridCap = 16384;
 constexpr int kSweepWaves = 32768;
 constexpr int64_t kGridCapMin = 64, kGridCapMax = 1 << 24;
 constexpr int64_t kSweepWavesMin = 256, kSweepWavesMax = 1 << 24;
-// captured passes kept per context
-constexpr size_t kMaxGraphs = 16;
+// captured passes kept per context, shapes remembered, and the passes of a
+// shape run kernel by kernel before it is captured
+constexpr size_t kMaxGraphs = 64;
+constexpr size_t kMaxSeen = 4096;
+constexpr int kCaptureAfter = 2;
+// capacity floor of a pass: its cells, up to this many
+constexpr int64_t kSmallPassCells = (int64_t)1 << 18;
 // k_vertices blocks: 256 CUs x 2 resident blocks (40 KB of LDS, 256 VGPRs
 // each) x 4 rounds
 constexpr int64_t kVerticesBlocks = 2048;
@@ -139,7 +145,7 @@ struct dmc_context {
         bool slots = false, off_swapped = false, adj_swapped = false;
     };
     std::vector<PassGraph> graphs;
-    std::string last_key;      // shape of the last pass run launch by launch
+    std::unordered_map<std::string, int> seen;   // passes run per shape (bounded)
     uint64_t buf_epoch = 0;    // bumped by every buffer (re)allocation
     uint64_t graph_clock = 0;
     bool use_graphs = true;
@@ -467,6 +473,13 @@ SweepPlan plan_sweep(dmc_context* ctx, const std::vector<HostVol>& hv) {
         D.grads = hv[v].grads;
         D.d = Dims{g.nx, g.ny, g.nz, (int64_t)g.nx - 1, (int64_t)(g.ny - 1) * (g.nz - 1)};
         D.geo = GridGeom{g.x0, g.y0, g.z0, g.dx, g.dy, g.dz};
+        // reciprocals for the corner gradients' divisions (k_vertices
+        // div_by), where the spacing is in a range that keeps them exact
+        const double sp[3] = {g.dx, g.dy, g.dz};
+        for (int c = 0; c < 3; c++) {
+            const double c2 = 2 * sp[c], a2 = std::fabs(c2);
+            D.rdx2[c] = (std::isnormal(c2) && a2 > 0x1p-500 && a2 < 0x1p+500) ? 1.0 / c2 : 0.0;
+        }
         D.iso = hv[v].iso;
         D.iso_t = sizeof(T) == 4 ? (double)smallest_float_not_below(hv[v].iso) : hv[v].iso;
         D.S = (int)((D.d.C + kSeg - 1) / kSeg);
@@ -559,9 +572,13 @@ void plan_caps(dmc_context* ctx, int64_t cells) {
         const double r = std::max(ratio * 1.02, 0.02);
         return cap_step((int64_t)std::ceil((double)cells * r) + 1024);
     };
-    ctx->acap = std::max(cap_step(ctx->acap_min), cap(ctx->ra));
-    ctx->vcap = std::max(cap_step(ctx->vcap_min), cap(ctx->rv));
-    ctx->qcap = std::max(cap_step(ctx->qcap_min), cap(ctx->rq));
+    // small passes: room for every cell (a few MB), so that a dense small
+    // volume never runs twice -- except in the fixed scratch of a no-alloc
+    // call, where a rerun is cheaper than the room
+    const int64_t floor = ctx->arena ? 0 : cap_step(std::min<int64_t>(cells, kSmallPassCells));
+    ctx->acap = std::max({cap_step(ctx->acap_min), cap(ctx->ra), floor});
+    ctx->vcap = std::max({cap_step(ctx->vcap_min), cap(ctx->rv), floor});
+    ctx->qcap = std::max({cap_step(ctx->qcap_min), cap(ctx->rq), floor});
     ctx->acap = std::min<int64_t>(std::min<int64_t>(ctx->acap, cells), 0xFFFFFFFF);
     ctx->vcap = std::min<int64_t>(ctx->vcap, 0x7FFFFFFF);
     ctx->qcap = std::min<int64_t>(ctx->qcap, 0x1FFFFFFF);
@@ -1229,7 +1246,7 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
 // dimensions, the element type and alignment, the flags, the launch shape,
 // the capacities and the buffers (every buffer allocation bumps buf_epoch);
 // the fields, iso-values and geometry are read from the volume table in
-// device memory.  The second time a shape comes, its pass is captured into a
+// device memory.  The third time a shape comes, its pass is captured into a
 // HIP graph; later passes of that shape replay it.  Host-side state the
 // launches leave (which double buffers hold the mesh) is recorded with the
 // graph and restored after a replay.
@@ -1302,12 +1319,12 @@ void run_pass(dmc_context* ctx, const SweepPlan& P, int field_type, uint32_t fla
         }
     }
     upload_vtab(ctx, P);
-    if (!graphs || key != ctx->last_key) {
-        ctx->last_key = key;
+    if (graphs && ctx->seen.size() >= kMaxSeen) ctx->seen.clear();
+    if (!graphs || ctx->seen[key]++ < kCaptureAfter) {
         enqueue_pass(ctx, P, field_type, flags);
         return;
     }
-    // the second pass of this shape: capture it, then launch the graph
+    // a shape that keeps coming: capture its pass, then launch the graph
     const uint64_t epoch = ctx->buf_epoch;
     HIPC(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     ctx->capturing = true;
@@ -1547,10 +1564,17 @@ static void do_extract_noalloc(dmc_context* ctx, void* a_) {
     ctx->arena = static_cast<char*>(base);
     ctx->arena_cap = kScratchBytes;
     ctx->arena_used = 0;
+    // exact sizes a larger earlier pass left would carve the scratch for that
+    // pass: this call plans from the density ratios, and learns its own
+    const int64_t mins[3] = {ctx->acap_min, ctx->vcap_min, ctx->qcap_min};
+    ctx->acap_min = ctx->vcap_min = ctx->qcap_min = 0;
     auto restore = [&] {
         // nothing of the call may still run on the scratch when it returns
         (void)hipStreamSynchronize(ctx->stream);
         for (size_t i = 0; i < saved.size(); i++) *ctx->bufs[i] = saved[i];
+        ctx->acap_min = mins[0];
+        ctx->vcap_min = mins[1];
+        ctx->qcap_min = mins[2];
         ctx->arena = nullptr;
         ctx->arena_cap = ctx->arena_used = 0;
         ctx->have_mesh = false;

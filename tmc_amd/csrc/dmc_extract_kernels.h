@@ -46,6 +46,7 @@ struct VolDesc {
     GridGeom geo;
     double iso;            // iso-value as the reference uses it (double)
     double iso_t;          // comparison threshold, exact in the field type
+    double rdx2[3];        // 1 / (2 dx), 1 / (2 dy), 1 / (2 dz) correctly rounded, or 0 (divide instead)
     int S, ntj, K, ntk;    // sweep tiling: segments per row, row tiles, planes per wave, plane tiles
     int64_t rowBase;       // first global cell row
     int64_t rsBase;        // first global (row, segment)
@@ -549,6 +550,21 @@ __device__ __forceinline__ void load_outer(const VolDesc& V, int i, int j, int k
     }
 }
 
+// x / c, correctly rounded, from rc = 1 / c correctly rounded: q = x rc is
+// within an ulp of x / c, e = x - q c is exact (one FMA), and q + e rc
+// rounds to the correctly rounded quotient (Markstein's theorem) -- three
+// FP64 operations for the dozen of a division.  Zeros, infinities, NaN and
+// quotients near the exponent range's ends take the division itself, as
+// does rc = 0.  tests/cpp/exact_div_check.c checks it against x / c bit for
+// bit.
+__device__ __forceinline__ double div_by(double x, double c, double rc) {
+    const double q = x * rc;
+    const double e = fma(-q, c, x);
+    const double q2 = fma(e, rc, q);
+    const double aq = fabs(q);
+    return (rc != 0.0 && aq > 0x1p-960 && aq < 0x1p+960) ? q2 : x / c;
+}
+
 // Position, normal and interior-edge count of the vertex at local
 // coordinates p of cell (i, j, k) (cpp_dmc/DualMarchingCubes.cpp:160-171,
 // UniformGrid.h:286-305).  The normal is the trilinear blend of the eight
@@ -585,7 +601,8 @@ __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int
                 // select the row (x), column (y) or pillar (z)
                 const int b = q == 0 ? (cy | cz << 1) : q == 1 ? (cx | cz << 1) : (cx | cy << 1);
                 const T* ov = q == 0 ? o.x[b] : q == 1 ? o.y[b] : o.z[b];
-                const double g = ((c & bit) ? (double)ov[1] - Fc(c & ~bit) : Fc(c | bit) - (double)ov[0]) / dx2[q];
+                const double g = div_by((c & bit) ? (double)ov[1] - Fc(c & ~bit) : Fc(c | bit) - (double)ov[0],
+                                        dx2[q], V.rdx2[q]);
                 G[c] = g * s;
             }
         }
@@ -603,9 +620,10 @@ __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int
     vp[1] = geo.y0 + (j + v) * geo.dy;
     vp[2] = geo.z0 + (k + ww) * geo.dz;
     double* vn = vnrm + 3 * (int64_t)out;
-    vn[0] = n[0] / s;
-    vn[1] = n[1] / s;
-    vn[2] = n[2] / s;
+    const double rs = 1.0 / s;
+    vn[0] = div_by(n[0], s, rs);
+    vn[1] = div_by(n[1], s, rs);
+    vn[2] = div_by(n[2], s, rs);
     vdeg[out] = (uint32_t)contour_rank(code >> 8, aux, code & 0xFFu, (unsigned)t, 12);
 }
 
