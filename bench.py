@@ -579,14 +579,14 @@ def main():
     # per-kernel figures: device ms per extraction (HIP events), bytes by
     # each kernel's own model
     R = (grid.ny - 1) * (grid.nz - 1)
-    builds = 4
     models = {
         "k_sweep": (4 * nvox, "4 N (field read once)"),
         "k_rows": (4 * A + 20 * A + 8 * R, "4 A list entries read + 20 A per-cell words + 8 R row totals written"),
         "k_vertices": (24 * V, "24 V (FP32 position + normal; the engine writes FP64, 48 V)"),
         "k_quads": (17 * Q + 32 * Q, "17 Q (int4 quad + colour) + 32 Q (first halfedge rows)"),
-        "k_he_twins": (builds * (16 * Qf + 68 * Qf + 4 * Vf),
-                       "per build 16 Q read + 64 Q + 4 Q + 4 V written (final V, Q; 4 builds)"),
+        "k_he_twins": (16 * Q + 32 * Q + 20 * Q + 4 * V,
+                       "16 Q quads + 32 Q halfedge rows read, 20 Q twins and flags + 4 V he_v written (the one "
+                       "build through rows, on the cell stage's V, Q; later builds are carried over)"),
         "k_triangles": (16 * Qf + 24 * Qf + 24 * 4 * Qf, "16 Q quads + 96 Q positions read, 24 Q written"),
     }
     kernels = []

@@ -113,10 +113,16 @@ struct dmc_context {
     int pcur = 0;  // which positions/normals buffer holds the vertex data
     int scur = 0;  // which slot map is live ...
     bool slots = false;  // ... once a compaction has run (else slots are the identity)
-    int csr_ready = 0;  // off/adj already hold the live mesh's halfedge rows: 1 from the cells, 2 remapped
-    // halfedges / simplification scratch
+    // what the next halfedge build starts from: 1 the cell stage's rows in
+    // off/adj, 2 the build a compaction carried over, 0 nothing
+    int ready = 0;
+    bool rows_live = false;   // off/adj hold the rows of the last extraction's final mesh
+    // halfedges / simplification scratch.  twin, flag, deg (out-degree), hev
+    // and bndrec are the current build's; the ...2 buffers receive the build
+    // a compaction carries over, and the pairs swap.
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        newV, newQ, clist, ccnt, tris, stats, off2, adj2, rbad, qmap, vmap, rlist, mlist, slow;
+        newV, newQ, clist, ccnt, tris, stats, off2, adj2, vmap, mlist, slow, twin2, flag2, deg2, hev2, bndrec2, arr,
+        cflag, aflag, qmap;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     // topologically correct MC: per active cell vertex / triangle counts and
@@ -141,8 +147,9 @@ struct dmc_context {
         std::string key;
         hipGraphExec_t exec = nullptr;
         uint64_t use = 0;
-        int cur = 0, pcur = 0, scur = 0, csr_ready = 0;
-        bool slots = false, off_swapped = false, adj_swapped = false;
+        int cur = 0, pcur = 0, scur = 0, ready = 0;
+        bool slots = false;
+        uint32_t swapped = 0;   // which buffer pairs are in the order opposite to their addresses
     };
     std::vector<PassGraph> graphs;
     std::unordered_map<std::string, int> seen;   // passes run per shape (bounded)
@@ -182,7 +189,8 @@ struct dmc_context {
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
-                      &off2, &adj2, &rbad, &qmap, &vmap, &rlist, &mlist, &slow, &drep, &dnew, &dval,
+                      &off2, &adj2, &vmap, &mlist, &slow, &twin2, &flag2, &deg2, &hev2, &bndrec2, &arr, &cflag,
+                      &aflag, &qmap, &drep, &dnew, &dval,
                       &dhist, &dedges, &dqual, &tcntv, &tcntt, &tvofs, &ttofs, &tamb, &tvb, &tstats, &tpos, &tnrm,
                       &ttris};
         for (Buf* b : all) bufs.push_back(b);
@@ -368,6 +376,36 @@ uint64_t read_u64(dmc_context* ctx, const void* dptr) {
 
 // Device sizes block of the running pass (allocated with the pass).
 Sizes* dsizes(dmc_context* ctx) { return static_cast<Sizes*>(ctx->sizes.p); }
+
+// The double-buffered halfedge arrays swap at every compaction; a pass
+// starts from one order (lower address current) so that passes of one shape
+// launch on the same pointers.
+Buf dmc_context::* const kPairs[][2] = {{&dmc_context::twin, &dmc_context::twin2},
+                                        {&dmc_context::flag, &dmc_context::flag2},
+                                        {&dmc_context::deg, &dmc_context::deg2},
+                                        {&dmc_context::hev, &dmc_context::hev2},
+                                        {&dmc_context::bndrec, &dmc_context::bndrec2}};
+constexpr int kNumPairs = sizeof(kPairs) / sizeof(kPairs[0]);
+
+uint32_t pair_order(dmc_context* ctx) {
+    uint32_t m = 0;
+    for (int i = 0; i < kNumPairs; i++)
+        if ((ctx->*kPairs[i][0]).p > (ctx->*kPairs[i][1]).p) m |= 1u << i;
+    return m;
+}
+
+void set_pair_order(dmc_context* ctx, uint32_t m) {
+    for (int i = 0; i < kNumPairs; i++) {
+        Buf& a = ctx->*kPairs[i][0];
+        Buf& b = ctx->*kPairs[i][1];
+        if ((a.p > b.p) != (((m >> i) & 1u) != 0)) std::swap(a, b);
+    }
+}
+
+void swap_pairs(dmc_context* ctx) {
+    for (int i = 0; i < kNumPairs; i++) std::swap(ctx->*kPairs[i][0], ctx->*kPairs[i][1]);
+}
+
 
 DevN dn(const uint64_t* p, int64_t cap) { return DevN{p, cap}; }
 DevN hn(int64_t n) { return DevN{nullptr, n}; }
@@ -589,8 +627,7 @@ template <class T>
 void run_cells(dmc_context* ctx, const SweepPlan& P) {
     // canonical buffer roles at the start of every pass, so that a pass of
     // the same shape launches on the same pointers (and can be replayed)
-    if (ctx->off.p > ctx->off2.p) std::swap(ctx->off, ctx->off2);
-    if (ctx->adj.p > ctx->adj2.p) std::swap(ctx->adj, ctx->adj2);
+    set_pair_order(ctx, 0);
     const Swept sw = sweep_cells<T>(ctx, P);
     const int n = ctx->nvol;
     const int64_t acap = ctx->acap, vcap = ctx->vcap, qcap = ctx->qcap;
@@ -651,19 +688,26 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
            rowBaseV, rowBaseQ, A, sw.segMask, sw.segBase, quads, colors, off, adj, (uint64_t)(4 * qcap),
            &S->unresolved);
     check_launch(ctx, "k_quads");
-    ctx->csr_ready = 1;
+    ctx->ready = 1;
 }
 
 // ---------------------------------------------------------------------------
 // halfedges
 // ---------------------------------------------------------------------------
-// Halfedges through a CSR of outgoing (halfedge, destination) pairs: rows
-// off[v] .. off[v+1] (off has nv+1 entries).  The rows are those the cell
-// stage or the last remap left (csr_ready 1 / 2) unless the device finds
-// them incomplete (a quadrilateral with an unresolved neighbour cell, or a
-// remap skipped after invalid vertex references); then, and when nothing is
-// ready (0), the build counts them from the quadrilaterals (generic rows).
-void build_halfedges(dmc_context* ctx, bool final_pass) {
+// A halfedge build gives every halfedge its twin and flag, every vertex its
+// out-degree, valence and he_v, and lists the 3-face edges.  The first build
+// pairs through the rows of outgoing (halfedge, destination) pairs the cell
+// stage left (off/adj).  A compaction carries the build over to the mesh it
+// leaves: most groups keep their members, renumbered, and only halfedges at
+// a changed vertex are paired again.  If the device finds the rows or the
+// carry unusable (a quadrilateral with an unresolved neighbour cell, an
+// invalid vertex reference), the build counts generic rows from the
+// quadrilaterals and pairs everything.
+
+// Pairing through full rows, on the device only when the build is not a
+// carried one (S->pair_rows): the cell stage's rows, or generic rows counted
+// from the quadrilaterals (S->generic).
+void pair_rows(dmc_context* ctx, bool final_pass) {
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
     const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
@@ -673,11 +717,11 @@ void build_halfedges(dmc_context* ctx, bool final_pass) {
     int2* adj = ensure<int2>(ctx, ctx->adj, hcap);
     int32_t* twin = ensure<int32_t>(ctx, ctx->twin, hcap);
     uint8_t* flag = ensure<uint8_t>(ctx, ctx->flag, hcap);
-    int32_t* hev = final_pass ? ensure<int32_t>(ctx, ctx->hev, vcap) : nullptr;
-    launch(ctx, "k_rows_gate", k_rows_gate, dim3(1), dim3(64), S, ctx->csr_ready);
-    ctx->csr_ready = 0;
-    // generic rows (only when S->generic)
+    int32_t* hev = ensure<int32_t>(ctx, ctx->hev, vcap);
+    uint32_t* arr = final_pass ? nullptr : ensure<uint32_t>(ctx, ctx->arr, vcap);
+    BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, ctx->reccap);
     const uint32_t* gen = &S->generic;
+    const uint32_t* rows = &S->pair_rows;
     const DevN nv = dn(&S->nv, vcap);
     fill_u32(ctx, deg, nv, 0u, gen);
     fill_u32(ctx, cursor, nv, 0u, gen);
@@ -687,39 +731,47 @@ void build_halfedges(dmc_context* ctx, bool final_pass) {
     launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, nv, &S->tot, gen);
     launch(ctx, "k_he_scatter", k_he_scatter, dim3(grid_for(ctx, qcap)), dim3(kBlock), quads, (const Sizes*)S, off,
            cursor, adj, gen);
+    fill_u32(ctx, hev, nv, 0xFFFFFFFFu, rows);
+    if (arr) fill_u32(ctx, arr, nv, 0u, rows);
     unsigned long long* vst = nullptr;
     if (final_pass && ctx->nvol > 1) {
-        vst = ensure<unsigned long long>(ctx, ctx->vhe, 2 * (size_t)ctx->nvol);
-        fill_u32(ctx, vst, hn(4 * (int64_t)ctx->nvol), 0u);
+        vst = ensure<unsigned long long>(ctx, ctx->vhe, 3 * (size_t)ctx->nvol);
+        fill_u32(ctx, vst, hn(6 * (int64_t)ctx->nvol), 0u, rows);
     }
-    // deg holds every vertex's out-degree here; the twins add boundary
-    // arrivals, turning it into the valence
     launch(ctx, "k_he_twins", k_he_twins, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, off, adj,
-           twin, flag, &S->he, static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst,
-           final_pass ? nullptr : deg);
+           twin, flag, hev, arr, &S->he, recs, &S->nrec, (uint32_t)ctx->reccap, &S->ovf,
+           static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst, rows);
     check_launch(ctx, "k_he_twins");
-    if (!final_pass) return;
-    launch(ctx, "k_he_v", k_he_v, dim3(grid_for(ctx, vcap)), dim3(kBlock), off, adj, (const Sizes*)S, hev);
-    // 3-face edges (when the build found any): he_v overrides and the
-    // boundary-vertex records, sorted on the host at the end
-    unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, vcap);
-    BndRec* recs = ensure<BndRec>(ctx, ctx->bndrec, ctx->reccap);
-    fill_u32(ctx, ovr, dn(&S->nv, vcap).cap > 0 ? DevN{nullptr, 2 * vcap} : hn(0), 0u, &S->he.pad);
-    for (int pass = 0; pass < 2; pass++)
-        launch(ctx, "k_he_three", k_he_three, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, off,
-               adj, flag, ovr, recs, &S->nrec, (uint32_t)ctx->reccap, &S->ovf, pass, hev);
-    check_launch(ctx, "k_he_three");
 }
 
-// Valence of the live mesh after a non-final halfedge build: the degree
-// array, completed by the twins pass (out-degree + boundary arrivals).
-int32_t* valence(dmc_context* ctx) {
-    uint32_t* val = static_cast<uint32_t*>(ctx->deg.p);
+// A halfedge build of the live mesh.  Not final: ends with the valence.
+// Final: the 3-face overrides of he_v and the reference's flag values.
+void build_halfedges(dmc_context* ctx, bool final_pass) {
     Sizes* S = dsizes(ctx);
-    launch(ctx, "k_count_zero", k_count_zero, dim3(grid_for(ctx, ctx->vcap)), dim3(kBlock), val, (const Sizes*)S,
-           &S->zero);
-    check_launch(ctx, "valence");
-    return reinterpret_cast<int32_t*>(val);
+    const int64_t vcap = ctx->vcap;
+    launch(ctx, "k_rows_gate", k_rows_gate, dim3(1), dim3(64), S, ctx->ready);
+    ctx->ready = 0;
+    pair_rows(ctx, final_pass);
+    const int gv = grid_for(ctx, vcap);
+    if (!final_pass) {
+        uint32_t* val = ensure<uint32_t>(ctx, ctx->val, vcap);
+        launch(ctx, "k_valence", k_valence, dim3(gv), dim3(kBlock), static_cast<const uint32_t*>(ctx->deg.p),
+               static_cast<const uint32_t*>(ctx->arr.p), (const Sizes*)S, val, &S->zero);
+        check_launch(ctx, "valence");
+        return;
+    }
+    const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
+    int32_t* hev = static_cast<int32_t*>(ctx->hev.p);
+    const BndRec* recs = static_cast<const BndRec*>(ctx->bndrec.p);
+    unsigned long long* ovr = ensure<unsigned long long>(ctx, ctx->ovr, vcap);
+    fill_u32(ctx, ovr, DevN{nullptr, 2 * vcap}, 0u, &S->nrec);
+    const int gr = grid_for(ctx, ctx->reccap);
+    for (int pass = 0; pass < 2; pass++)
+        launch(ctx, "k_three_apply", k_three_apply, dim3(gr), dim3(kBlock), recs, (const Sizes*)S,
+               (uint32_t)ctx->reccap, quads, ovr, pass, hev);
+    launch(ctx, "k_strip_flags", k_strip_flags, dim3(grid_for(ctx, 4 * ctx->qcap)), dim3(kBlock), (const Sizes*)S,
+           static_cast<uint8_t*>(ctx->flag.p));
+    check_launch(ctx, "final halfedges");
 }
 
 void color_faces(dmc_context* ctx) {
@@ -764,9 +816,8 @@ uint32_t* compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vs
     scan_to(ctx, ScanKeep{removed}, newidx, dn(&S->nv, vcap), &S->nn);
     const uint32_t* slot = ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr;
     uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], vcap);
-    int32_t* rlist = ensure<int32_t>(ctx, ctx->rlist, vcap);
     launch(ctx, "k_compact_slots", k_compact_slots, dim3(grid_for(ctx, vcap)), dim3(kBlock), removed, newidx,
-           (const Sizes*)S, slot, slot2, vsecond, rlist);
+           (const Sizes*)S, slot, slot2, vsecond);
     check_launch(ctx, "k_compact_slots");
     ctx->scur = 1 - ctx->scur;
     ctx->slots = true;
@@ -790,44 +841,81 @@ void gather_vertices(dmc_context* ctx) {
     ctx->slots = false;
 }
 
-// Halfedge rows of the compacted mesh (S->nn vertices, S->qn
-// quadrilaterals) from the current rows, into the spare CSR buffers, which
-// then become the current ones.  On the device this runs only when the
-// last build saw no invalid vertex reference (S->remap_ok); otherwise the
-// next build counts its rows from scratch.
-void remap_rows(dmc_context* ctx, const Remap& m, const int32_t* dlist) {
-    const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
+// The carry of a compaction, first half: fresh counters, the face map, the
+// out-degree and he_v carried from the kept preimages, the changed-vertex
+// flags and per-origin counts cleared, and the buffers of the carried build.
+// S->nn and S->qn must be on the device; `last`: the compaction before the
+// final build (a batch counts per volume).
+Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, const uint8_t* vrem,
+                  const int32_t* vmap, bool last) {
+    const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
-    const uint32_t* off = static_cast<const uint32_t*>(ctx->off.p);
-    const int2* adj = static_cast<const int2*>(ctx->adj.p);
-    uint32_t* deg2 = ensure<uint32_t>(ctx, ctx->deg, vcap);
-    uint32_t* off2 = ensure<uint32_t>(ctx, ctx->off2, vcap + 1);
-    uint32_t* cur2 = ensure<uint32_t>(ctx, ctx->cursor, vcap);
-    int2* adj2 = ensure<int2>(ctx, ctx->adj2, 4 * qcap);
-    launch(ctx, "k_remap_gate", k_remap_gate, dim3(1), dim3(64), S);
-    const uint32_t* ok = &S->remap_ok;
-    const int g = grid_for(ctx, vcap);
-    // kept vertices: old row length less the corners in dropped quads
-    launch(ctx, "k_remap_degree_kept", k_remap_degree_kept, dim3(g), dim3(kBlock), m, off, deg2, ok);
-    launch(ctx, "k_remap_drop", k_remap_drop, dim3(grid_for(ctx, qcap)), dim3(kBlock), m,
-           static_cast<const int32_t*>(ctx->quads[ctx->cur].p), dlist, deg2, ok);
-    launch(ctx, "k_remap_degree", k_remap_degree, dim3(g), dim3(kBlock), m, off, adj, deg2, 1, &S->rbad, ok);
-    check_launch(ctx, "k_remap_degree");
-    scan_to(ctx, ScanCounts{deg2}, off2, dn(&S->nn, vcap), &S->tot, "k_scan", ok);
-    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off2, dn(&S->nn, vcap), &S->tot, ok);
-    for (int pass = 0; pass < 2; pass++)
-        launch(ctx, "k_remap_rows", k_remap_rows, dim3(g), dim3(kBlock), m, off, adj, (const uint32_t*)off2, cur2, adj2,
-               pass, ok);
-    check_launch(ctx, "k_remap_rows");
-    std::swap(ctx->off, ctx->off2);
-    std::swap(ctx->adj, ctx->adj2);
-    ctx->csr_ready = 2;
+    launch(ctx, "k_stage_gate", k_stage_gate, dim3(1), dim3(64), S);
+    const DevN nn = dn(&S->nn, vcap);
+    Carry c{};
+    c.twin = static_cast<const int32_t*>(ctx->twin.p);
+    c.flag = static_cast<const uint8_t*>(ctx->flag.p);
+    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, qcap);
+    c.qmap = qmap;
+    uint8_t* cflag = ensure<uint8_t>(ctx, ctx->cflag, vcap + 4);
+    c.cflag = cflag;
+    c.twin_n = ensure<int32_t>(ctx, ctx->twin2, hcap);
+    c.flag_n = ensure<uint8_t>(ctx, ctx->flag2, hcap);
+    c.deg_n = ensure<uint32_t>(ctx, ctx->deg2, vcap);
+    c.hev_n = ensure<int32_t>(ctx, ctx->hev2, vcap);
+    c.arr = ensure<uint32_t>(ctx, ctx->arr, vcap);
+    c.aflag = ensure<uint8_t>(ctx, ctx->aflag, hcap);
+    c.cnt = ensure<uint32_t>(ctx, ctx->cursor, vcap);
+    c.S = S;
+    c.qb = static_cast<const int64_t*>(ctx->qbound.p);
+    c.nvol = ctx->nvol;
+    if (last && ctx->nvol > 1) {
+        c.vst = ensure<unsigned long long>(ctx, ctx->vhe, 3 * (size_t)ctx->nvol);
+        fill_u32(ctx, c.vst, hn(6 * (int64_t)ctx->nvol), 0u);
+    }
+    // whole words: the marks are set with word atomics
+    fill_u32(ctx, cflag, DevN{nullptr, (vcap + 4) / 4}, 0u);
+    fill_u32(ctx, c.arr, nn, 0u);
+    fill_u32(ctx, c.cnt, nn, 0u);
+    launch(ctx, "k_qmap", k_qmap, dim3(grid_for(ctx, qcap)), dim3(kBlock), drop, newq, (const Sizes*)S, qmap);
+    launch(ctx, "k_carry_vertices", k_carry_vertices, dim3(grid_for(ctx, vcap)), dim3(kBlock), vrem, vmap,
+           (const Sizes*)S, static_cast<const uint32_t*>(ctx->deg.p), static_cast<const int32_t*>(ctx->hev.p),
+           (const int32_t*)qmap, c.deg_n, c.hev_n);
+    return c;
 }
 
-void simplify3x3y(dmc_context* ctx, bool colored) {
+// Second half, after the compaction kernel: the flagged halfedges paired
+// through their rows, the 3-face records carried, the counts committed and
+// the carried build made current.
+void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
+    const int64_t vcap = ctx->vcap, hcap = 4 * ctx->qcap;
+    Sizes* S = dsizes(ctx);
+    const DevN nn = dn(&S->nn, vcap);
+    const int32_t* quads2 = static_cast<const int32_t*>(ctx->quads[1 - ctx->cur].p);
+    uint32_t* aoff = ensure<uint32_t>(ctx, ctx->off2, vcap + 1);
+    int2* aadj = ensure<int2>(ctx, ctx->adj2, hcap);
+    BndRec* recs_n = ensure<BndRec>(ctx, ctx->bndrec2, ctx->reccap);
+    const int gh = grid_for(ctx, hcap);
+    scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan");
+    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), aoff, nn, &S->tot, (const uint32_t*)nullptr);
+    launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(gh), dim3(kBlock), (const uint8_t*)c.aflag, (const Sizes*)S,
+           quads2, (const uint32_t*)aoff, c.cnt, aadj);
+    launch(ctx, "k_aff_twins", k_aff_twins, dim3(gh), dim3(kBlock), (const Sizes*)S, quads2, (const uint32_t*)aoff,
+           (const int2*)aadj, c, recs_n, (uint32_t)ctx->reccap);
+    launch(ctx, "k_carry_records", k_carry_records, dim3(grid_for(ctx, ctx->reccap)), dim3(kBlock),
+           static_cast<const BndRec*>(ctx->bndrec.p), (const Sizes*)S, vmap, c.cflag, c.qmap, recs_n, &S->nrec,
+           (uint32_t)ctx->reccap, &S->ovf);
+    check_launch(ctx, "carry");
+    launch(ctx, "k_commit", k_commit, dim3(1), dim3(64), S);
+    swap_pairs(ctx);
+    ctx->cur = 1 - ctx->cur;
+    ctx->ready = 2;
+}
+
+void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
     build_halfedges(ctx, false);
     if (colored) color_faces(ctx);
-    int32_t* val = valence(ctx);
+    const int32_t* val = static_cast<const int32_t*>(ctx->val.p);
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
     Sizes* S = dsizes(ctx);
     const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
@@ -847,8 +935,8 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
     const int gq = grid_for(ctx, qcap);
-    launch(ctx, "k_mark3x3y", k_mark3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, (const int32_t*)val, p);
-    launch(ctx, "k_merge_claim", k_merge_claim, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, (const int32_t*)val,
+    launch(ctx, "k_mark3x3y", k_mark3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, val, p);
+    launch(ctx, "k_merge_claim", k_merge_claim, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, val,
            (const uint8_t*)p, colors, twin, flag, lastT, lastR, em, vfirst);
     check_launch(ctx, "merge3x3y");
     // the quadrilateral compaction's scan also lists the merging faces
@@ -857,29 +945,29 @@ void simplify3x3y(dmc_context* ctx, bool colored) {
     launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)em, (const uint32_t*)newQ,
            nq, mlist);
     launch(ctx, "k_merge_apply", k_merge_apply, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
-           (const Sizes*)S, (const uint8_t*)em, (const int32_t*)val, (const int32_t*)lastT, (const int32_t*)lastR,
+           (const Sizes*)S, (const uint8_t*)em, val, (const int32_t*)lastT, (const int32_t*)lastR,
            static_cast<double*>(ctx->pos[ctx->pcur].p), static_cast<double*>(ctx->nrm[ctx->pcur].p),
            ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
     check_launch(ctx, "k_merge_apply");
     compact_vertices(ctx, vfirst, vsecond);
-    const int dst = 1 - ctx->cur;
-    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qcap);
-    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, qcap);
+    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
     launch(ctx, "k_vmap3x3y", k_vmap3x3y, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vfirst,
            (const int32_t*)vsecond, (const Sizes*)S, vmap);
-    launch(ctx, "k_compact_quads3x3y", k_compact_quads3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,
-           (const uint8_t*)em, (const uint32_t*)newQ, (const int32_t*)vmap, q2, qmap, &S->cbad);
-    check_launch(ctx, "k_compact_quads3x3y");
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
-    remap_rows(ctx, Remap{qmap, vmap, vfirst, nullptr, static_cast<const int32_t*>(ctx->rlist.p), S}, mlist);
-    launch(ctx, "k_commit", k_commit, dim3(1), dim3(64), S);
-    ctx->cur = dst;
+    Carry c = carry_begin(ctx, em, newQ, vfirst, vmap, last);
+    launch(ctx, "k_changed3x3y", k_changed3x3y, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
+           (const Sizes*)S, (const uint8_t*)em, (const int32_t*)vmap, static_cast<const int32_t*>(ctx->hev.p),
+           const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);
+    launch(ctx, "k_compact_quads3x3y", k_compact_quads3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,
+           (const int32_t*)vmap, q2, c);
+    check_launch(ctx, "k_compact_quads3x3y");
+    carry_end(ctx, c, vmap);
 }
 
-void simplify3333(dmc_context* ctx) {
+void simplify3333(dmc_context* ctx, bool last) {
     build_halfedges(ctx, false);
-    int32_t* val = valence(ctx);
+    const int32_t* val = static_cast<const int32_t*>(ctx->val.p);
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
     Sizes* S = dsizes(ctx);
     const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
@@ -897,32 +985,30 @@ void simplify3333(dmc_context* ctx) {
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int gq = grid_for(ctx, qcap);
-    launch(ctx, "k_mark3333", k_mark3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag,
-           (const int32_t*)val, lastF, vflag, p3333, rflag);
-    launch(ctx, "k_twin3333", k_twin3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag,
-           (const int32_t*)val, (const uint8_t*)p3333, (const int32_t*)lastF, vtwin);
+    launch(ctx, "k_mark3333", k_mark3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag, val, lastF,
+           vflag, p3333, rflag);
+    launch(ctx, "k_twin3333", k_twin3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag, val,
+           (const uint8_t*)p3333, (const int32_t*)lastF, vtwin);
     check_launch(ctx, "mark3333");
     uint32_t* newv = compact_vertices(ctx, vflag, nullptr);
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
     scan_to(ctx, ScanKeep{rflag}, newQ, nq, &S->qn);
-    const int dst = 1 - ctx->cur;
-    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[dst], 4 * qcap);
-    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, qcap);
+    int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
     launch(ctx, "k_vmap3333", k_vmap3333, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vflag,
            (const uint32_t*)newv, (const Sizes*)S, vmap);
-    launch(ctx, "k_compact_quads3333", k_compact_quads3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,
-           (const uint8_t*)rflag, (const uint32_t*)newQ, (const uint8_t*)p3333, (const int32_t*)vtwin,
-           (const int32_t*)vmap, q2, qmap, &S->cbad);
-    check_launch(ctx, "k_compact_quads3333");
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
-    // the dropped quadrilaterals, for the row degrees
+    // the dropped quadrilaterals, whose corners are the changed vertices
     int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
     launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)rflag,
            (const uint32_t*)newQ, nq, dlist);
-    remap_rows(ctx, Remap{qmap, vmap, vflag, vtwin, static_cast<const int32_t*>(ctx->rlist.p), S}, dlist);
-    launch(ctx, "k_commit", k_commit, dim3(1), dim3(64), S);
-    ctx->cur = dst;
+    Carry c = carry_begin(ctx, rflag, newQ, vflag, vmap, last);
+    launch(ctx, "k_changed3333", k_changed3333, dim3(gq), dim3(kBlock), quads, (const int32_t*)dlist,
+           (const Sizes*)S, (const int32_t*)vmap, const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);
+    launch(ctx, "k_compact_quads3333", k_compact_quads3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,
+           (const uint8_t*)p3333, (const int32_t*)vtwin, (const int32_t*)vmap, q2, c);
+    check_launch(ctx, "k_compact_quads3333");
+    carry_end(ctx, c, vmap);
 }
 
 void clear_timing(dmc_context* ctx) {
@@ -1067,17 +1153,18 @@ void enqueue_pass(dmc_context* ctx, const SweepPlan& P, int field_type, uint32_t
     const int n = ctx->nvol;
     if (field_type == DMC_FIELD_F32) run_cells<float>(ctx, P);
     else run_cells<double>(ctx, P);
+    const bool iso = (flags & DMC_SIMPLIFY_3X3Y_ISOLATED) != 0, p3333 = (flags & DMC_SIMPLIFY_3333) != 0;
     if (flags & DMC_SIMPLIFY_3X3Y_COLOR) {
         mark(ctx, "p3x3y_color");
-        simplify3x3y(ctx, true);
+        simplify3x3y(ctx, true, !iso && !p3333);
     }
-    if (flags & DMC_SIMPLIFY_3X3Y_ISOLATED) {
+    if (iso) {
         mark(ctx, "p3x3y_isolated");
-        simplify3x3y(ctx, false);
+        simplify3x3y(ctx, false, !p3333);
     }
-    if (flags & DMC_SIMPLIFY_3333) {
+    if (p3333) {
         mark(ctx, "p3333");
-        simplify3333(ctx);
+        simplify3333(ctx, true);
     }
     mark(ctx, "halfedges");
     build_halfedges(ctx, true);
@@ -1115,7 +1202,7 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
     int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     int32_t* hev = static_cast<int32_t*>(ctx->hev.p);
     // sizes and per-volume bookkeeping back to the host
-    std::vector<int64_t> vb(2 * (n + 1)), qb(2 * (n + 1)), ab(n + 1), vhe(n > 1 ? 2 * n : 0);
+    std::vector<int64_t> vb(2 * (n + 1)), qb(2 * (n + 1)), ab(n + 1), vhe(n > 1 ? 3 * n : 0);
     std::vector<VolStat> vs(n);
     Sizes& hs = ctx->hsz;
     {
@@ -1147,9 +1234,10 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
         ctx->acap_min = std::max<int64_t>(ctx->acap_min, (int64_t)hs.A);
         ctx->vcap_min = std::max<int64_t>(ctx->vcap_min, (int64_t)hs.V);
         ctx->qcap_min = std::max<int64_t>(ctx->qcap_min, (int64_t)hs.Q);
-        if (hs.ovf & 8u) ctx->reccap = std::max<int64_t>(ctx->reccap, (int64_t)hs.nrec);
+        if (hs.ovf & 8u) ctx->reccap = std::max<int64_t>(ctx->reccap, (int64_t)std::max(hs.nrec, hs.recmax));
         return false;
     }
+    ctx->rows_live = hs.pair_rows != 0u;
     if (hs.rows != 4 * hs.Q)
         fail(ctx, DMC_ERR_STATE, "halfedge rows %llu != 4 Q %llu", (unsigned long long)hs.rows,
              (unsigned long long)(4 * hs.Q));
@@ -1172,7 +1260,9 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
     M.nq = nq;
     M.nt = (flags & DMC_TRIANGLES) ? 2 * nq : 0;
     M.d_tris = tris;
-    M.nonmanifold_edges = (int64_t)hs.he.nonmanifold;
+    // the build counts flagged halfedges: 4 per non-manifold edge, 3 per
+    // 3-face edge
+    M.nonmanifold_edges = (int64_t)(hs.he.nm4 / 4 + hs.he.nm3 / 3);
     M.edge_errors = (int64_t)hs.he.errors;
     M.invalid_refs = (int64_t)hs.he.bad + (int64_t)hs.unresolved;
     M.valence_zero = hs.zero;
@@ -1222,9 +1312,9 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
             m.d_twins = twin + 4 * qb[v];
             m.d_he_flags = M.d_he_flags + 4 * qb[v];
             m.d_he_v = hev + vb[v];
-            m.nonmanifold_edges = (int64_t)vhe[2 * v];
+            m.nonmanifold_edges = vhe[3 * v] / 4 + vhe[3 * v + 1] / 3;
             m.nbnd = (int64_t)ctx->vbnd[v].size();
-            m.edge_errors = (int64_t)vhe[2 * v + 1];
+            m.edge_errors = vhe[3 * v + 2];
             m.invalid_refs = M.invalid_refs;   // batch-wide
             m.valence_zero = M.valence_zero;   // batch-wide
             m.failed_projections_l1 = vs[v].fail1;
@@ -1275,9 +1365,8 @@ void save_post(dmc_context* ctx, dmc_context::PassGraph& g) {
     g.pcur = ctx->pcur;
     g.scur = ctx->scur;
     g.slots = ctx->slots;
-    g.csr_ready = ctx->csr_ready;
-    g.off_swapped = ctx->off.p > ctx->off2.p;
-    g.adj_swapped = ctx->adj.p > ctx->adj2.p;
+    g.ready = ctx->ready;
+    g.swapped = pair_order(ctx);
 }
 
 void load_post(dmc_context* ctx, const dmc_context::PassGraph& g) {
@@ -1285,9 +1374,8 @@ void load_post(dmc_context* ctx, const dmc_context::PassGraph& g) {
     ctx->pcur = g.pcur;
     ctx->scur = g.scur;
     ctx->slots = g.slots;
-    ctx->csr_ready = g.csr_ready;
-    if ((ctx->off.p > ctx->off2.p) != g.off_swapped) std::swap(ctx->off, ctx->off2);
-    if ((ctx->adj.p > ctx->adj2.p) != g.adj_swapped) std::swap(ctx->adj, ctx->adj2);
+    ctx->ready = g.ready;
+    set_pair_order(ctx, g.swapped);
 }
 
 void drop_graphs(dmc_context* ctx) {
@@ -1768,6 +1856,31 @@ static void diag_mesh_ok(dmc_context* ctx) {
     if (ctx->vmesh.size() != 1) fail(ctx, DMC_ERR_STATE, "diagnostics need a single-volume extraction");
 }
 
+// The final mesh's halfedge rows (off/adj), counted from the quadrilaterals
+// when the final build was carried over from a compaction and paired
+// without them.
+static void ensure_rows(dmc_context* ctx) {
+    if (ctx->rows_live) return;
+    Sizes* S = dsizes(ctx);
+    const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
+    const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
+    uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg2, vcap);
+    uint32_t* off = ensure<uint32_t>(ctx, ctx->off, vcap + 1);
+    uint32_t* cursor = ensure<uint32_t>(ctx, ctx->cursor, vcap);
+    int2* adj = ensure<int2>(ctx, ctx->adj, hcap);
+    const DevN nv = dn(&S->nv, vcap);
+    fill_u32(ctx, deg, nv, 0u);
+    fill_u32(ctx, cursor, nv, 0u);
+    launch(ctx, "k_he_degree", k_he_degree, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, deg,
+           &S->he.pad, (const uint32_t*)nullptr);
+    scan_to(ctx, ScanCounts{deg}, off, nv, &S->tot, "k_scan");
+    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, nv, &S->tot, (const uint32_t*)nullptr);
+    launch(ctx, "k_he_scatter", k_he_scatter, dim3(grid_for(ctx, qcap)), dim3(kBlock), quads, (const Sizes*)S, off,
+           cursor, adj, (const uint32_t*)nullptr);
+    check_launch(ctx, "rows");
+    ctx->rows_live = true;
+}
+
 struct EdgeArgs {
     int32_t* edges;
     int64_t* count;
@@ -1780,7 +1893,7 @@ static void do_edges(dmc_context* ctx, void* a_) {
     const int64_t nv = m.nv, nhe = 4 * m.nq;
     uint32_t* rep = ensure<uint32_t>(ctx, ctx->drep, nhe);
     uint32_t* nid = ensure<uint32_t>(ctx, ctx->dnew, nhe);
-    // the halfedge rows and twins are those of the final build
+    ensure_rows(ctx);
     if (nhe > 0)
         launch(ctx, "k_edge_reps", k_edge_reps, dim3(grid_for(ctx, nhe)), dim3(kBlock),
                            static_cast<const int32_t*>(m.d_quads), nhe, nv,
@@ -1820,6 +1933,7 @@ static void do_valence_hist(dmc_context* ctx, void* a_) {
     const int64_t nv = m.nv;
     uint32_t* vval = ensure<uint32_t>(ctx, ctx->dval, nv);
     uint32_t* hist = ensure<uint32_t>(ctx, ctx->dhist, kValenceBins + 1);
+    ensure_rows(ctx);
     HIPC(ctx, hipMemsetAsync(hist, 0, (kValenceBins + 1) * sizeof(uint32_t), ctx->stream));
     if (nv > 0) {
         launch(ctx, "k_vertex_valence", k_vertex_valence, dim3(grid_for(ctx, nv)), dim3(kBlock),

@@ -562,7 +562,11 @@ __device__ __forceinline__ double div_by(double x, double c, double rc) {
     const double e = fma(-q, c, x);
     const double q2 = fma(e, rc, q);
     const double aq = fabs(q);
-    return (rc != 0.0 && aq > 0x1p-960 && aq < 0x1p+960) ? q2 : x / c;
+    double r = q2;
+    // a branch, not a select: the wave runs the full division only when one
+    // of its lanes needs it
+    if (__builtin_expect(!(rc != 0.0 && aq > 0x1p-960 && aq < 0x1p+960), 0)) r = x / c;
+    return r;
 }
 
 // Position, normal and interior-edge count of the vertex at local

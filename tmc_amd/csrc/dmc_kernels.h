@@ -46,11 +46,13 @@ struct DevN {
     }
 };
 
-// Statistics of one halfedge build.
+// Statistics of one halfedge build, counted per halfedge so that a build
+// that carries most groups over from the last one can count them too:
+// non-manifold edges = nm4 / 4 + nm3 / 3, 3-face edges = nm3 / 3.
 struct HeStats {
-    unsigned long long nonmanifold;
-    unsigned long long errors;
-    unsigned long long three;    // number of 3-face edges
+    unsigned long long nm4;      // halfedges flagged non-manifold (groups of 4)
+    unsigned long long errors;   // groups of more than 4 halfedges
+    unsigned long long nm3;      // halfedges flagged boundary multiple edge (groups of 3)
     unsigned int bad;            // invalid vertex references
     unsigned int pad;
 };
@@ -66,16 +68,20 @@ struct Sizes {
     uint64_t nn, qn;             // vertices / quadrilaterals a compaction keeps
     uint64_t rows;               // first CSR's row total (4 Q)
     uint64_t tot;                // a scan total no one else reads
+    uint64_t hq;                 // halfedges of the mesh a compaction leaves (4 qn)
     unsigned long long qhead;    // work queue of k_vertices: the next chunk of active cells
     HeStats he;                  // the last halfedge build
     uint32_t ovf;                // capacities exceeded: 1 active cells, 2 vertices, 4 quads, 8 3-face records
     uint32_t unresolved;         // quadrilaterals with an unresolved neighbour cell
     uint32_t zero;               // vertices of valence 0 (all valence passes)
-    uint32_t rbad;               // halfedges a remap left without a valid origin
     uint32_t generic;            // this halfedge build counts its rows from the quadrilaterals
-    uint32_t remap_ok;           // this remap runs (the last build saw no invalid vertex reference)
-    uint32_t cbad;               // invalid references a quadrilateral compaction met
-    uint32_t nrec;               // 3-face edge records
+    uint32_t pair_rows;          // this build pairs halfedges through full rows (cell rows or generic)
+    uint32_t incr;               // this build is the one the last compaction carried over
+    uint32_t incr_ok;            // the build before the compaction saw no invalid vertex reference
+    uint32_t cbad;               // invalid references the last quadrilateral compaction met
+    uint32_t nrec;               // 3-face edge records of the current build
+    uint32_t nrec_prev;          // ... of the build before (carried over by a compaction)
+    uint32_t recmax;             // most 3-face records any build of the pass wrote (capacity planning)
     uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
 };
 
@@ -145,6 +151,16 @@ struct ScanCounts {
             v[4 * q + 2] = t.z;
             v[4 * q + 3] = t.w;
         }
+    }
+};
+struct ScanSet {
+    const uint8_t* p;   // set flags
+    __device__ __forceinline__ uint32_t at(int64_t i) const { return p[i] ? 1u : 0u; }
+    __device__ __forceinline__ void vec16(int64_t base, uint32_t v[kScanIPT]) const {
+        const uint4 t = *reinterpret_cast<const uint4*>(p + base);
+        const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++) v[q] = ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) ? 1u : 0u;
     }
 };
 struct ScanKeep {

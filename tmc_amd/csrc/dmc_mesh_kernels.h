@@ -110,11 +110,20 @@ __device__ __forceinline__ int he_group(const int32_t* __restrict__ quads, const
     return c;
 }
 
+// Flag bytes are the reference's: 0 manifold, 1 non-manifold (4 faces),
+// 2 boundary multiple edge (3 faces).  A group of more than 4 faces (the
+// reference's error case) keeps 0 in the output; inside the engine its
+// members carry kFlagErr (its first member kFlagErrFirst) so that a build
+// that carries the group over can still count it.  Every test of a flag
+// compares with 1 or 2, which the markers never equal; the final build
+// clears them.
+constexpr uint8_t kFlagErr = 0x80, kFlagErrFirst = 0xC0;
+
 // Twin and flag from a sorted group (cpp_dmc/DualMarchingCubes.cpp:574-669).
 __device__ __forceinline__ void twin_rule(const int32_t* __restrict__ quads, int c, const int32_t g[5], int32_t h,
                                           int32_t& tw, uint8_t& fl) {
     tw = -1;
-    fl = 0;
+    fl = c > 4 ? (h == g[0] ? kFlagErrFirst : kFlagErr) : 0;
     if (c == 2) {
         tw = (g[0] == h) ? g[1] : g[0];
     } else if (c == 3) {
@@ -132,9 +141,65 @@ __device__ __forceinline__ void twin_rule(const int32_t* __restrict__ quads, int
     }
 }
 
-// Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669).
-// The group's first member also counts the edge (non-manifold / error) and,
-// for three faces, records the override of he_v and the boundary vertices.
+__device__ __forceinline__ uint64_t he_key(int32_t v0, int32_t v1) {
+    return v0 < v1 ? ((uint64_t)(uint32_t)v0 << 32) | (uint32_t)v1 : ((uint64_t)(uint32_t)v1 << 32) | (uint32_t)v0;
+}
+
+// A 3-face edge (cpp_dmc/DualMarchingCubes.cpp:603-637): its map key, the
+// two boundary vertices (getBndVertices), and the unpaired halfedge, which
+// overrides he_v of its origin -- the map is walked in key order, so the
+// largest key wins.  Written by the group's first member.
+struct BndRec {
+    unsigned long long key;
+    int32_t a, b;
+    int32_t lone, pad;
+};
+
+__device__ __forceinline__ BndRec three_record(const int32_t* __restrict__ quads, const int32_t g[5], int32_t h) {
+    const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
+    BndRec r;
+    if (v0 != v1) { r.a = v0; r.b = v1; r.lone = g[2]; }
+    else if (v0 != v2) { r.a = v0; r.b = v2; r.lone = g[1]; }
+    else { r.a = v1; r.b = v2; r.lone = g[0]; }
+    r.key = he_key(quads[h], he_dest(quads, h));
+    r.pad = 0;
+    return r;
+}
+
+__device__ __forceinline__ void put_record(BndRec* __restrict__ recs, uint32_t* nrec, uint32_t reccap, uint32_t* ovf,
+                                           const BndRec& r) {
+    const uint32_t p = atomicAdd(nrec, 1u);
+    if (p < reccap) recs[p] = r;
+    else atomicOr(ovf, 8u);
+}
+
+// Per-thread counts of flagged halfedges, added to the build's statistics
+// once per wave (every lane of the wave must call flush).
+struct HeCount {
+    uint32_t n4 = 0, n3 = 0, err = 0;
+    __device__ __forceinline__ void add(uint8_t fl) {
+        n4 += fl == 1;
+        n3 += fl == 2;
+        err += fl == kFlagErrFirst;
+    }
+    __device__ __forceinline__ void flush(HeStats* st) const {
+        const uint32_t a = wave_sum(n4), b = wave_sum(n3), e = wave_sum(err);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            if (a) atomicAdd(&st->nm4, (unsigned long long)a);
+            if (b) atomicAdd(&st->nm3, (unsigned long long)b);
+            if (e) atomicAdd(&st->errors, (unsigned long long)e);
+        }
+    }
+};
+
+// Per volume of a batch (rare events): [3v] non-manifold halfedges, [3v+1]
+// 3-face halfedges, [3v+2] groups of more than 4.
+__device__ __forceinline__ void vol_count(unsigned long long* __restrict__ vst, const int64_t* __restrict__ qb,
+                                          int nvol, int32_t h, uint8_t fl) {
+    if (!vst || (fl != 1 && fl != 2 && fl != kFlagErrFirst)) return;
+    const int vol = vol_of(qb, nvol, h >> 2);
+    atomicAdd(&vst[3 * vol + (fl == 1 ? 0 : fl == 2 ? 1 : 2)], 1ull);
+}
 
 // Twin and flag of every halfedge (cpp_dmc/DualMarchingCubes.cpp:574-669),
 // one thread per halfedge.  The four halfedges of a quadrilateral sit in
@@ -142,8 +207,10 @@ __device__ __forceinline__ void twin_rule(const int32_t* __restrict__ quads, int
 // its origin v[k] and of v[k+1] -- the row lane k+1 loads for itself.  So
 // every lane loads only its own row (up to kRowRegs entries in registers)
 // and takes its neighbour's with a DPP quad permutation; longer rows are
-// finished from memory.  The group's first member also counts the edge
-// (non-manifold / error).
+// finished from memory.  Each lane also counts its flag, adds a boundary
+// arrival to its destination's valence, and writes he_v of its origin if
+// it is the row's largest halfedge; the first member of a 3-face group
+// writes the group's record.
 constexpr int kRowRegs = 6;
 
 __device__ __forceinline__ int next_in_quad(int x) {
@@ -154,12 +221,16 @@ __device__ __forceinline__ int next_in_quad(int x) {
 __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__ quads, const Sizes* S,
                                                       const uint32_t* __restrict__ off,
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
-                                                      uint8_t* __restrict__ flag, HeStats* st,
-                                                      const int64_t* __restrict__ qb, int nvol,
+                                                      uint8_t* __restrict__ flag, int32_t* __restrict__ hev,
+                                                      uint32_t* __restrict__ arr, HeStats* st,
+                                                      BndRec* __restrict__ recs, uint32_t* nrec, uint32_t reccap,
+                                                      uint32_t* ovf, const int64_t* __restrict__ qb, int nvol,
                                                       unsigned long long* __restrict__ vst,
-                                                      uint32_t* __restrict__ val) {
+                                                      const uint32_t* __restrict__ gate) {
     static_assert(kBlock % 4 == 0, "quads must not straddle blocks");
+    if (gated_off(gate)) return;
     const int64_t nhe = 4 * (int64_t)S->nq, nv = (int64_t)S->nv;
+    HeCount cnt;
     TMC_GRID_STRIDE(h64, nhe) {
         const int32_t h = (int32_t)h64;
         const int32_t a = quads[h];
@@ -177,6 +248,13 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
 #pragma unroll
             for (int i = 0; i < kRowRegs; i++) r[i] = make_int2(-1, -1);
         }
+        // he_v (cpp_dmc/DualMarchingCubes.cpp:561-564): the largest halfedge
+        // of the row; its own lane writes it
+        int32_t hmax = -1;
+#pragma unroll
+        for (int i = 0; i < kRowRegs; i++) hmax = max(hmax, r[i].x);
+        for (uint32_t x = kRowRegs; x < n; x++) hmax = max(hmax, adj[s + x].x);
+        if (n > 0 && h == hmax) hev[a] = h;
         // the next lane of the quad: its origin is this halfedge's destination
         const int32_t b = next_in_quad(a);
         const uint32_t nb = (uint32_t)next_in_quad((int)n);
@@ -250,93 +328,54 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
         twin[h] = tw;
         flag[h] = fl;
         // valence: a boundary halfedge adds one to the vertex it ends at
-        if (tw == -1 && val && vid_ok(b, nv)) atomicAdd(&val[b], 1u);
-        if (c > 0 && g[0] == h) {
-            if (c == 3 || c == 4) atomicAdd(&st->nonmanifold, 1ull);
-            if (c == 3) {
-                atomicAdd(&st->three, 1ull);
-                st->pad = 1u;   // some 3-face edge: the final build's he_v pass runs
-            }
-            if (c > 4) atomicAdd(&st->errors, 1ull);
-            // per volume of a batch (rare events): [2v] non-manifold, [2v+1] errors
-            if (vst && c >= 3) {
-                const int vol = vol_of(qb, nvol, h >> 2);
-                atomicAdd(&vst[2 * vol + (c > 4)], 1ull);
-            }
-        }
+        if (tw == -1 && arr && vid_ok(b, nv)) atomicAdd(&arr[b], 1u);
+        cnt.add(fl);
+        vol_count(vst, qb, nvol, h, fl);
+        if (c == 3 && g[0] == h) put_record(recs, nrec, reccap, ovf, three_record(quads, g, h));
+    }
+    cnt.flush(st);
+}
+
+// The 3-face overrides of he_v (cpp_dmc/DualMarchingCubes.cpp:603-637) from
+// the final build's records: pass 0 finds each lone vertex's largest key,
+// pass 1 lets that record write.  ovr (2 words per vertex) starts at 0.
+__global__ void __launch_bounds__(kBlock) k_three_apply(const BndRec* __restrict__ recs, const Sizes* S,
+                                                         uint32_t reccap, const int32_t* __restrict__ quads,
+                                                         unsigned long long* __restrict__ ovr, int pass,
+                                                         int32_t* __restrict__ he_v) {
+    const int64_t n = S->nrec < reccap ? S->nrec : reccap, nv = (int64_t)S->nv;
+    TMC_GRID_STRIDE(i, n) {
+        const BndRec r = recs[i];
+        const int32_t vl = quads[r.lone];
+        if (!vid_ok(vl, nv)) continue;
+        if (pass == 0) atomicMax(&ovr[vl], r.key + 1ull);
+        else if (ovr[vl] == r.key + 1ull) he_v[vl] = r.lone;
     }
 }
 
-// he_v: the last halfedge (in face order) leaving each vertex
-// (cpp_dmc/DualMarchingCubes.cpp:561-564).
-__global__ void __launch_bounds__(kBlock) k_he_v(const uint32_t* __restrict__ off, const int2* __restrict__ adj,
-                                                  const Sizes* S, int32_t* __restrict__ he_v) {
-    const int64_t nv = (int64_t)S->nv;
-    TMC_GRID_STRIDE(v, nv) {
-        int32_t last = -1;
-        for_row(adj, off[v], off[v + 1], [&](int2 r) { last = r.x > last ? r.x : last; });
-        he_v[v] = last;
-    }
-}
-
-__device__ __forceinline__ uint64_t he_key(int32_t v0, int32_t v1) {
-    return v0 < v1 ? ((uint64_t)(uint32_t)v0 << 32) | (uint32_t)v1 : ((uint64_t)(uint32_t)v1 << 32) | (uint32_t)v0;
-}
-
-// 3-face edges: the unpaired halfedge overrides he_v of its origin; the map
-// is walked in key order, so the largest key wins.  Records (key, two
-// boundary vertices) for getBndVertices.
-struct BndRec {
-    unsigned long long key;
-    int32_t a, b;
-};
-
-__global__ void __launch_bounds__(kBlock) k_he_three(const int32_t* __restrict__ quads, const Sizes* S,
-                                                      const uint32_t* __restrict__ off, const int2* __restrict__ adj,
-                                                      const uint8_t* __restrict__ flag,
-                                                      unsigned long long* __restrict__ ovr, BndRec* __restrict__ recs,
-                                                      unsigned int* nrec, uint32_t reccap, uint32_t* ovf, int pass,
-                                                      int32_t* __restrict__ he_v) {
-    if (S->he.three == 0) return;   // no 3-face edge in this build
-    const int64_t nhe = 4 * (int64_t)S->nq, nv = (int64_t)S->nv;
-    TMC_GRID_STRIDE(h64, nhe) {
-        const int32_t h = (int32_t)h64;
-        if (flag[h] != 2) continue;   // every member of a 3-face group, and only those
-        int32_t g[5];
-        const int c = he_group(quads, off, adj, h, nv, g);
-        if (c != 3 || g[0] != h) continue;
-        const int32_t v0 = quads[g[0]], v1 = quads[g[1]], v2 = quads[g[2]];
-        int32_t ba, bb, lone;
-        if (v0 != v1) { ba = v0; bb = v1; lone = g[2]; }
-        else if (v0 != v2) { ba = v0; bb = v2; lone = g[1]; }
-        else { ba = v1; bb = v2; lone = g[0]; }
-        const int32_t vl = quads[lone];
-        const uint64_t key = he_key(quads[h], he_dest(quads, h));
-        if (pass == 0) {
-            atomicMax(&ovr[vl], (unsigned long long)key + 1ull);
-            const unsigned int p = atomicAdd(nrec, 1u);
-            if (p < reccap) {
-                recs[p].key = key;
-                recs[p].a = ba;
-                recs[p].b = bb;
-            } else {
-                atomicOr(ovf, 8u);
-            }
-        } else if (ovr[vl] == (unsigned long long)key + 1ull) {
-            he_v[vl] = lone;
-        }
+// The final mesh's flags as the reference leaves them: the markers of
+// groups of more than 4 faces cleared (only when there are any).
+__global__ void __launch_bounds__(kBlock) k_strip_flags(const Sizes* S, uint8_t* __restrict__ flag) {
+    if (S->he.errors == 0) return;
+    const int64_t nhe = 4 * (int64_t)S->nq;
+    TMC_GRID_STRIDE(h, nhe) {
+        if (flag[h] & kFlagErr) flag[h] = 0;
     }
 }
 
 // Valence (cpp_dmc/DualMarchingCubes.cpp:847-871): outgoing halfedges plus
-// boundary halfedges ending at the vertex.  The out-degree is the vertex's
-// CSR row length; k_he_twins adds the boundary arrivals.  A vertex has
-// valence 0 exactly when it has no row entries.
-__global__ void __launch_bounds__(kBlock) k_count_zero(const uint32_t* __restrict__ deg, const Sizes* S,
-                                                        unsigned int* zero) {
+// boundary halfedges ending at the vertex -- the out-degree and the
+// arrivals the pairing counted.  Counts the vertices of valence 0.
+__global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ deg,
+                                                     const uint32_t* __restrict__ arr, const Sizes* S,
+                                                     uint32_t* __restrict__ val, unsigned int* zero) {
     const int64_t nv = (int64_t)S->nv;
     unsigned int z = 0;
-    TMC_GRID_STRIDE(v, nv) z += deg[v] == 0;
+    TMC_GRID_STRIDE(v, nv) {
+        const uint32_t x = deg[v] + arr[v];
+        val[v] = x;
+        z += x == 0;
+    }
     z = wave_sum(z);
     if ((threadIdx.x & (kWave - 1)) == 0 && z) atomicAdd(zero, z);
 }
@@ -586,20 +625,14 @@ __global__ void __launch_bounds__(kBlock) k_list_flagged(const uint8_t* __restri
 // lives at slot vslot[v] (the identity when vslot is null); the compaction
 // only carries the slots along (4 B per vertex instead of 48 B), and one
 // gather at the end lays the survivors out densely.
-// Removed vertices are listed densely at v - newidx[v] (the number of
-// removed vertices before v).
 __global__ void __launch_bounds__(kBlock) k_compact_slots(const uint8_t* __restrict__ removed,
                                                            const uint32_t* __restrict__ newidx, const Sizes* S,
                                                            const uint32_t* __restrict__ vslot,
                                                            uint32_t* __restrict__ vslot2,
-                                                           int32_t* __restrict__ vsecond,
-                                                           int32_t* __restrict__ rlist) {
+                                                           int32_t* __restrict__ vsecond) {
     const int64_t nv = (int64_t)S->nv;
     TMC_GRID_STRIDE(v, nv) {
-        if (removed[v]) {
-            rlist[v - newidx[v]] = (int32_t)v;
-            continue;
-        }
+        if (removed[v]) continue;
         const uint32_t o = newidx[v];
         vslot2[o] = vslot ? vslot[v] : (uint32_t)v;
         if (vsecond) vsecond[v] = (int32_t)o;
@@ -621,30 +654,6 @@ __global__ void __launch_bounds__(kBlock) k_gather_vertices(const uint32_t* __re
     }
 }
 
-// removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137)
-// The vertex ids come from vmap (k_vmap3x3y): vm[vm[x]]-style lookups of a
-// removed vertex x, exactly as the reference composes them.
-__global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, const Sizes* S,
-                                                               const uint8_t* __restrict__ em,
-                                                               const uint32_t* __restrict__ newq,
-                                                               const int32_t* __restrict__ vmap,
-                                                               int32_t* __restrict__ quads2,
-                                                               int32_t* __restrict__ qmap, unsigned int* bad) {
-    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
-    TMC_GRID_STRIDE(f, nq) {
-        qmap[f] = em[f] ? -1 : (int32_t)newq[f];
-        if (em[f]) continue;
-        const int4 q = reinterpret_cast<const int4*>(quads)[f];
-        const int32_t x[4] = {q.x, q.y, q.z, q.w};
-        int32_t r[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) r[k] = vid_ok(x[k], nv) ? vmap[x[k]] : -1;
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (r[k] < 0) atomicAdd(bad, 1u);
-        reinterpret_cast<int4*>(quads2)[newq[f]] = make_int4(r[0], r[1], r[2], r[3]);
-    }
-}
 
 // mark3333 (cpp_dmc/DualMarchingCubes.cpp:1190-1301).  Returns true with
 // the face's four vertices, outer vertices and neighbour faces if f is a
@@ -714,73 +723,6 @@ __global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__
     }
 }
 
-// removeQuadrilaterals3333 (cpp_dmc/DualMarchingCubes.cpp:1320-1360)
-__global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __restrict__ quads, const Sizes* S,
-                                                               const uint8_t* __restrict__ rflag,
-                                                               const uint32_t* __restrict__ newq,
-                                                               const uint8_t* __restrict__ p3333,
-                                                               const int32_t* __restrict__ vtwin,
-                                                               const int32_t* __restrict__ vmap,
-                                                               int32_t* __restrict__ quads2,
-                                                               int32_t* __restrict__ qmap, unsigned int* bad) {
-    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv;
-    TMC_GRID_STRIDE(f, nq) {
-        qmap[f] = rflag[f] ? -1 : (int32_t)(newq[f] | (p3333[f] ? 0x80000000u : 0u));
-        if (rflag[f]) continue;
-        const int4 q = reinterpret_cast<const int4*>(quads)[f];
-        int32_t x[4] = {q.x, q.y, q.z, q.w};
-        if (p3333[f]) {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (vid_ok(x[k], nv)) x[k] = vtwin[x[k]];
-        }
-        int32_t r[4];
-        // vmap[x] = new id of a kept x, -1 for a removed one (k_vmap3333)
-#pragma unroll
-        for (int k = 0; k < 4; k++) r[k] = vid_ok(x[k], nv) ? vmap[x[k]] : -1;
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (r[k] < 0) atomicAdd(bad, 1u);
-        reinterpret_cast<int4*>(quads2)[newq[f]] = make_int4(r[0], r[1], r[2], r[3]);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Halfedge rows of a compacted mesh from the rows of the mesh before it.
-// The compaction keeps quadrilateral f (unless dropped) with each vertex x
-// renamed to remap_vertex(f, x) -- exactly the ids k_compact_quads3x3y /
-// k_compact_quads3333 write -- so halfedge 4f+k (x -> y) becomes
-// 4 newq[f] + k (remap(f, x) -> remap(f, y)) in the row of remap(f, x).
-// A kept vertex is the only kept preimage of its new id and all its
-// surviving halfedges stay with it: its row is written with plain stores
-// (pass 0).  Removed vertices hand their surviving halfedges to other rows
-// with atomics (pass 1, after pass 0).  Row order is irrelevant to every
-// consumer (twins sort their groups; valence counts; he_v takes the max).
-// ---------------------------------------------------------------------------
-// The compaction as two maps: qmap[f] = new index of quadrilateral f, -1
-// if dropped, with bit 31 set for a 3333 pattern face (its vertices go
-// through vtwin); vmap[x] = the new id vertex x takes in the quadrilaterals
-// that are not pattern faces (-1 if none).  These are exactly the ids
-// k_compact_quads3x3y / k_compact_quads3333 write.
-// Sizes: nv / nq before the compaction, nn / qn after it (S->nv - S->nn
-// vertices removed, S->nq - S->qn quadrilaterals dropped).
-struct Remap {
-    const int32_t* qmap;
-    const int32_t* vmap;
-    const uint8_t* vrem;        // removed vertices (vfirst / vflag)
-    const int32_t* vtwin;       // 3333: replacement of a pattern face's vertex
-    const int32_t* rlist;       // the removed vertices, ascending
-    const Sizes* S;
-};
-
-__device__ __forceinline__ int32_t remap_vertex(const Remap& m, int64_t nv, int32_t qm, int32_t x) {
-    if (!vid_ok(x, nv)) return -1;
-    if (qm < -1) {   // pattern face (bit 31 on a non-negative index)
-        x = m.vtwin[x];
-        if (!vid_ok(x, nv)) return -1;
-    }
-    return m.vmap[x];
-}
 
 // 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
 // compaction), vsecond[vsecond[x]] for a removed one (its target's).
@@ -803,137 +745,365 @@ __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__
     TMC_GRID_STRIDE(x, nv) vmap[x] = vflag[x] ? -1 : (int32_t)newv[x];
 }
 
-// Degree of the new row of every kept vertex: its old row length (a
-// streaming pass), less one for each of its corners in a dropped quadrilateral
-// (k_remap_drop, over the list of dropped ones).  The removed vertices then
-// add their surviving entries (k_remap_degree, pass 1).
-// Remap launches run only when the rows they read are complete (*gate:
-// the previous build saw no invalid vertex reference).
-__global__ void __launch_bounds__(kBlock) k_remap_degree_kept(Remap m, const uint32_t* __restrict__ off,
-                                                               uint32_t* __restrict__ deg2,
-                                                               const uint32_t* __restrict__ gate) {
-    if (gated_off(gate)) return;
-    const int64_t nv = (int64_t)m.S->nv;
+// ---------------------------------------------------------------------------
+// The halfedges of a compacted mesh, carried over from the build before it.
+//
+// A compaction keeps face f as face newq[f] (unless dropped) with corner k
+// in place, each vertex renamed to its new id.  Call a new vertex changed
+// (cflag bit 1) if a removed vertex maps to it, or if it is the image of a
+// corner of a dropped face that the compaction moved or merged (3X3Y: the
+// two valence-3 corners of a merging face; 3333: every kept corner of a
+// dropped face, which covers the pattern face's new corners too).  For a
+// new edge {a, b} with neither end changed, a and b each have exactly one
+// preimage x, y (kept vertices), every halfedge on {x, y} sits in a
+// surviving face that is not a pattern face, and every halfedge on {a, b}
+// comes from one of them: the edge's group is the old group renumbered by
+// newq, which keeps its order, so its twins (renumbered) and flags carry
+// over, and so does its 3-face record.  Halfedges with a changed end are
+// paired again through rows of those halfedges only (keyed by origin): a
+// group with a changed end has all its members there.
+//
+// Out-degree and he_v (the largest halfedge leaving the vertex) carry over
+// for a vertex with no corner in a dropped face.  A changed vertex has all
+// its out-halfedges in its row.  A vertex that only lost corners (the
+// valence >= 5 corners of a merging face) loses one out-halfedge per
+// corner, and takes he_v again (with atomics over its surviving
+// out-halfedges) only if its largest one was dropped (cflag bit 4).
+// ---------------------------------------------------------------------------
+struct Carry {
+    const int32_t* twin;          // the build before the compaction
+    const uint8_t* flag;
+    const int32_t* qmap;          // new index of each face, -1 if dropped
+    const uint8_t* cflag;         // changed new vertices (bit 1), vertices whose he_v is taken again (bit 4)
+    int32_t* twin_n;              // the carried build
+    uint8_t* flag_n;
+    uint32_t* deg_n;
+    int32_t* hev_n;
+    uint32_t* arr;                // boundary arrivals per new vertex (valence)
+    uint8_t* aflag;               // new halfedges to pair again
+    uint32_t* cnt;                // ... per origin
+    Sizes* S;
+    unsigned long long* vst;      // per-volume counts (last compaction of a batch)
+    const int64_t* qb;            // first new face of each volume
+    int nvol;
+};
+
+// Mark a new vertex (bits of cflag, one byte per vertex, updated as words).
+__device__ __forceinline__ void mark_vertex(uint8_t* cflag, int32_t y, uint32_t bits) {
+    atomicOr(reinterpret_cast<uint32_t*>(cflag) + (y >> 2), bits << (8 * (y & 3)));
+}
+
+// The compaction as one face map: qmap[f] = new index, -1 if dropped.
+__global__ void __launch_bounds__(kBlock) k_qmap(const uint8_t* __restrict__ drop, const uint32_t* __restrict__ newq,
+                                                  const Sizes* S, int32_t* __restrict__ qmap) {
+    const int64_t nq = (int64_t)S->nq;
+    TMC_GRID_STRIDE(f, nq) qmap[f] = drop[f] ? -1 : (int32_t)newq[f];
+}
+
+// Out-degree and he_v of every new vertex from its kept preimage (the
+// changed vertices and those that lost a corner are corrected next).
+__global__ void __launch_bounds__(kBlock) k_carry_vertices(const uint8_t* __restrict__ vrem,
+                                                            const int32_t* __restrict__ vmap, const Sizes* S,
+                                                            const uint32_t* __restrict__ deg,
+                                                            const int32_t* __restrict__ hev,
+                                                            const int32_t* __restrict__ qmap,
+                                                            uint32_t* __restrict__ deg_n, int32_t* __restrict__ hev_n) {
+    const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
     TMC_GRID_STRIDE(x, nv) {
-        if (m.vrem[x]) continue;
-        deg2[m.vmap[x]] = off[x + 1] - off[x];
+        if (vrem[x]) continue;
+        const int32_t y = vmap[x];
+        if (!vid_ok(y, nn)) continue;
+        deg_n[y] = deg[x];
+        const int32_t e = hev[x];
+        const int32_t m = e < 0 ? -1 : qmap[e >> 2];
+        hev_n[y] = m < 0 ? -1 : 4 * m + (e & 3);
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_remap_drop(Remap m, const int32_t* __restrict__ quads,
-                                                        const int32_t* __restrict__ dlist, uint32_t* __restrict__ deg2,
-                                                        const uint32_t* __restrict__ gate) {
-    if (gated_off(gate)) return;
-    const int64_t nv = (int64_t)m.S->nv, nd = (int64_t)(m.S->nq - m.S->qn);
-    TMC_GRID_STRIDE(i, nd) {
+// A changed vertex: paired again, its out-degree and he_v from its row.
+__device__ __forceinline__ void mark_changed(uint8_t* cflag, uint32_t* deg_n, int32_t* hev_n, int32_t y) {
+    mark_vertex(cflag, y, 1u);
+    deg_n[y] = 0;
+    hev_n[y] = -1;
+}
+
+// 3X3Y: the merging faces (mlist, S->nq - S->qn of them).  The moved and
+// the dropped corner are changed.  The other two (valence >= 5, never moved)
+// each lose the halfedge they start in the face; if that was their largest,
+// their he_v is taken again over what survives (cflag bit 4).
+__global__ void __launch_bounds__(kBlock) k_changed3x3y(const int32_t* __restrict__ quads,
+                                                         const int32_t* __restrict__ mlist, const Sizes* S,
+                                                         const uint8_t* __restrict__ em,
+                                                         const int32_t* __restrict__ vmap,
+                                                         const int32_t* __restrict__ hev, uint8_t* __restrict__ cflag,
+                                                         uint32_t* __restrict__ deg_n, int32_t* __restrict__ hev_n) {
+    const int64_t n = (int64_t)(S->nq - S->qn), nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    TMC_GRID_STRIDE(i, n) {
+        const int32_t f = mlist[i];
+        const int4 q = reinterpret_cast<const int4*>(quads)[f];
+        const bool first = em[f] == 1;
+        const int32_t x[4] = {first ? q.x : q.y, first ? q.z : q.w, first ? q.y : q.x, first ? q.w : q.z};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!vid_ok(x[k], nv)) continue;
+            const int32_t y = vmap[x[k]];
+            if (!vid_ok(y, nn)) continue;
+            if (k < 2) {
+                mark_changed(cflag, deg_n, hev_n, y);
+            } else {
+                atomicSub(&deg_n[y], 1u);
+                if ((hev[x[k]] >> 2) == f) {
+                    mark_vertex(cflag, y, 4u);
+                    hev_n[y] = -1;
+                }
+            }
+        }
+    }
+}
+
+// 3333: the dropped faces (dlist, S->nq - S->qn of them); every kept corner
+// is changed.
+__global__ void __launch_bounds__(kBlock) k_changed3333(const int32_t* __restrict__ quads,
+                                                         const int32_t* __restrict__ dlist, const Sizes* S,
+                                                         const int32_t* __restrict__ vmap, uint8_t* __restrict__ cflag,
+                                                         uint32_t* __restrict__ deg_n, int32_t* __restrict__ hev_n) {
+    const int64_t n = (int64_t)(S->nq - S->qn), nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    TMC_GRID_STRIDE(i, n) {
         const int4 q = reinterpret_cast<const int4*>(quads)[dlist[i]];
-        const int32_t v[4] = {q.x, q.y, q.z, q.w};
+        const int32_t x[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            if (vid_ok(v[c], nv) && !m.vrem[v[c]]) atomicSub(&deg2[m.vmap[v[c]]], 1u);
-    }
-}
-
-// pass 0: degree of every new row from its kept preimage; pass 1: removed
-// vertices add theirs (and count halfedges left without a valid origin).
-// Rows are walked four entries at a time with every lookup of a chunk
-// issued before any is used.
-__global__ void __launch_bounds__(kBlock) k_remap_degree(Remap m, const uint32_t* __restrict__ off,
-                                                          const int2* __restrict__ adj, uint32_t* __restrict__ deg2,
-                                                          int pass, unsigned int* bad,
-                                                          const uint32_t* __restrict__ gate) {
-    if (gated_off(gate)) return;
-    const int64_t nv = (int64_t)m.S->nv, nn = (int64_t)m.S->nn;
-    // pass 0 visits every old vertex, pass 1 the list of removed ones
-    TMC_GRID_STRIDE(x64, pass == 0 ? nv : nv - nn) {
-        const int32_t x = pass == 0 ? (int32_t)x64 : m.rlist[x64];
-        const bool kept = pass == 0 && !m.vrem[x];
-        if (pass == 0 && !kept) continue;
-        const uint32_t s = off[x], e = off[x + 1];
-        uint32_t c = 0;
-        for (uint32_t r = s; r < e; r += 4) {
-            int32_t qm[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) qm[i] = m.qmap[adj[min(r + (uint32_t)i, e - 1)].x >> 2];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                if (r + (uint32_t)i >= e || qm[i] == -1) continue;
-                if (pass == 0) {
-                    c++;
-                } else {
-                    const int32_t y = remap_vertex(m, nv, qm[i], x);
-                    if (vid_ok(y, nn)) atomicAdd(&deg2[y], 1u);
-                    else atomicAdd(bad, 1u);
-                }
-            }
+        for (int k = 0; k < 4; k++) {
+            if (!vid_ok(x[k], nv)) continue;
+            const int32_t y = vmap[x[k]];
+            if (vid_ok(y, nn)) mark_changed(cflag, deg_n, hev_n, y);
         }
-        if (pass == 0) deg2[m.vmap[x]] = c;
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_remap_rows(Remap m, const uint32_t* __restrict__ off,
-                                                        const int2* __restrict__ adj,
-                                                        const uint32_t* __restrict__ off2,
-                                                        uint32_t* __restrict__ cursor2, int2* __restrict__ adj2,
-                                                        int pass, const uint32_t* __restrict__ gate) {
-    if (gated_off(gate)) return;
-    const int64_t nv = (int64_t)m.S->nv, nn = (int64_t)m.S->nn;
-    TMC_GRID_STRIDE(x64, pass == 0 ? nv : nv - nn) {
-        const int32_t x = pass == 0 ? (int32_t)x64 : m.rlist[x64];
-        const bool kept = pass == 0 && !m.vrem[x];
-        if (pass == 0 && !kept) continue;
-        const uint32_t s = off[x], e = off[x + 1];
-        const int32_t y0 = kept ? m.vmap[x] : -1;
-        uint32_t p = kept ? off2[y0] : 0u;
-        const uint32_t p0 = p;
-        for (uint32_t r = s; r < e; r += 4) {
-            int2 a[4];
-            int32_t qm[4], dst[4];
+// The carry of surviving face f (new index nf, new corners r): every
+// halfedge either keeps its group (twin renumbered, flag, arrival) or is
+// flagged for pairing (listed by a scan afterwards).
+__device__ __forceinline__ void carry_face(const Carry& c, bool act, int64_t f, uint32_t nf, const int32_t r[4],
+                                           int64_t nn, HeCount& cnt) {
+    int32_t tw[4] = {-1, -1, -1, -1};
+    uint32_t fl = 0, af = 0;
+    if (act) {
+        const int4 t4 = reinterpret_cast<const int4*>(c.twin)[f];
+        const uint32_t f4 = reinterpret_cast<const uint32_t*>(c.flag)[f];
+        const int32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
+        uint8_t cf[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = adj[min(r + (uint32_t)i, e - 1)];
+        for (int k = 0; k < 4; k++) cf[k] = vid_ok(r[k], nn) ? c.cflag[r[k]] : 0;
 #pragma unroll
-            for (int i = 0; i < 4; i++) qm[i] = m.qmap[a[i].x >> 2];
-#pragma unroll
-            for (int i = 0; i < 4; i++) dst[i] = remap_vertex(m, nv, qm[i], a[i].y);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                if (r + (uint32_t)i >= e || qm[i] == -1) continue;
-                const int2 out = make_int2(4 * (qm[i] & 0x7FFFFFFF) + (a[i].x & 3), dst[i]);
-                if (pass == 0) {
-                    adj2[p++] = out;
-                } else {
-                    const int32_t y = remap_vertex(m, nv, qm[i], x);
-                    if (!vid_ok(y, nn)) continue;
-                    const uint32_t slot = atomicAdd(&cursor2[y], 1u);
-                    adj2[off2[y] + slot] = out;
-                }
+        for (int k = 0; k < 4; k++) {
+            const int32_t a = r[k], b = r[(k + 1) & 3];
+            const int32_t h = (int32_t)(4 * nf) + k;
+            if (!vid_ok(a, nn) || !vid_ok(b, nn)) continue;   // invalid ids: the next build starts over
+            const uint8_t ca = cf[k], cb = cf[(k + 1) & 3];
+            if (ca & 4u) atomicMax(&c.hev_n[a], h);   // its largest halfedge was dropped
+            if ((ca | cb) & 1u) {
+                af |= 1u << (8 * k);
+                atomicAdd(&c.cnt[a], 1u);
+                continue;
             }
+            const int32_t nq = t[k] >= 0 ? c.qmap[t[k] >> 2] : -1;
+            tw[k] = nq < 0 ? -1 : 4 * nq + (t[k] & 3);
+            const uint8_t x = (uint8_t)(f4 >> (8 * k));
+            fl |= (uint32_t)x << (8 * k);
+            if (tw[k] < 0) atomicAdd(&c.arr[b], 1u);
+            cnt.add(x);
+            vol_count(c.vst, c.qb, c.nvol, h, x);
         }
-        if (pass == 0) cursor2[y0] = p - p0;
+        reinterpret_cast<int4*>(c.twin_n)[nf] = make_int4(tw[0], tw[1], tw[2], tw[3]);
+        reinterpret_cast<uint32_t*>(c.flag_n)[nf] = fl;
+        reinterpret_cast<uint32_t*>(c.aflag)[nf] = af;
     }
 }
 
-// Before a halfedge build: which rows it uses (ready: 1 the cell stage's,
-// 2 a remap's, 0 none) and its statistics, which start from the invalid
-// references the remap met.
+// removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137), with
+// the carry of the halfedges.  The vertex ids come from vmap (k_vmap3x3y):
+// vm[vm[x]]-style lookups of a removed vertex x, exactly as the reference
+// composes them.  Blocks stride over the faces in whole blocks, so that
+// every lane of a wave takes part in the carry's list append.
+__global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, const Sizes* S,
+                                                               const int32_t* __restrict__ vmap,
+                                                               int32_t* __restrict__ quads2, Carry c) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    HeCount cnt;
+    for (int64_t f0 = (int64_t)blockIdx.x * kBlock; f0 < nq; f0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t f = f0 + threadIdx.x;
+        const int32_t m = f < nq ? c.qmap[f] : -1;
+        const bool act = m >= 0;
+        int32_t r[4] = {-1, -1, -1, -1};
+        const uint32_t nf = act ? (uint32_t)m : 0u;
+        if (act) {
+            const int4 q = reinterpret_cast<const int4*>(quads)[f];
+            const int32_t x[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) r[k] = vid_ok(x[k], nv) ? vmap[x[k]] : -1;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (r[k] < 0) atomicAdd(&c.S->cbad, 1u);
+            reinterpret_cast<int4*>(quads2)[nf] = make_int4(r[0], r[1], r[2], r[3]);
+        }
+        carry_face(c, act, f, nf, r, nn, cnt);
+    }
+    cnt.flush(&c.S->he);
+}
+
+// removeQuadrilaterals3333 (cpp_dmc/DualMarchingCubes.cpp:1320-1360), with
+// the carry of the halfedges (a pattern face's halfedges all end at
+// changed vertices, so none of them is carried).
+__global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __restrict__ quads, const Sizes* S,
+                                                               const uint8_t* __restrict__ p3333,
+                                                               const int32_t* __restrict__ vtwin,
+                                                               const int32_t* __restrict__ vmap,
+                                                               int32_t* __restrict__ quads2, Carry c) {
+    const int64_t nq = (int64_t)S->nq, nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    HeCount cnt;
+    for (int64_t f0 = (int64_t)blockIdx.x * kBlock; f0 < nq; f0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t f = f0 + threadIdx.x;
+        const int32_t m = f < nq ? c.qmap[f] : -1;
+        const bool act = m >= 0;
+        int32_t r[4] = {-1, -1, -1, -1};
+        const uint32_t nf = act ? (uint32_t)m : 0u;
+        if (act) {
+            const int4 q = reinterpret_cast<const int4*>(quads)[f];
+            int32_t x[4] = {q.x, q.y, q.z, q.w};
+            if (p3333[f]) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (vid_ok(x[k], nv)) x[k] = vtwin[x[k]];
+            }
+            // vmap[x] = new id of a kept x, -1 for a removed one (k_vmap3333)
+#pragma unroll
+            for (int k = 0; k < 4; k++) r[k] = vid_ok(x[k], nv) ? vmap[x[k]] : -1;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (r[k] < 0) atomicAdd(&c.S->cbad, 1u);
+            reinterpret_cast<int4*>(quads2)[nf] = make_int4(r[0], r[1], r[2], r[3]);
+        }
+        carry_face(c, act, f, nf, r, nn, cnt);
+    }
+    cnt.flush(&c.S->he);
+}
+
+// Rows of the flagged halfedges by origin: aoff = exclusive scan of cnt;
+// cnt counts down to 0 as the rows fill.
+__global__ void __launch_bounds__(kBlock) k_aff_scatter(const uint8_t* __restrict__ aflag, const Sizes* S,
+                                                         const int32_t* __restrict__ quads,
+                                                         const uint32_t* __restrict__ aoff, uint32_t* __restrict__ cnt,
+                                                         int2* __restrict__ aadj) {
+    const int64_t n = (int64_t)S->hq;
+    TMC_GRID_STRIDE(i, n) {
+        if (!aflag[i]) continue;
+        const int32_t h = (int32_t)i;
+        const int32_t a = quads[h], b = he_dest(quads, h);
+        const uint32_t p = aoff[a] + atomicSub(&cnt[a], 1u) - 1u;
+        aadj[p] = make_int2(h, b);
+    }
+}
+
+// Pair the flagged halfedges (cpp_dmc/DualMarchingCubes.cpp:574-669) through
+// their rows; a changed origin also takes its out-degree and he_v from its
+// row, which holds all its out-halfedges.
+__global__ void __launch_bounds__(kBlock) k_aff_twins(const Sizes* S, const int32_t* __restrict__ quads,
+                                                       const uint32_t* __restrict__ aoff,
+                                                       const int2* __restrict__ aadj, Carry c, BndRec* __restrict__ recs,
+                                                       uint32_t reccap) {
+    const int64_t n = (int64_t)S->hq, nn = (int64_t)S->nn;
+    HeCount cnt;
+    TMC_GRID_STRIDE(i, n) {
+        if (!c.aflag[i]) continue;
+        const int32_t h = (int32_t)i;
+        const int32_t a = quads[h], b = he_dest(quads, h);
+        int32_t g[5];
+        const int k = he_group(quads, aoff, aadj, h, nn, g);
+        int32_t tw;
+        uint8_t fl;
+        twin_rule(quads, k, g, h, tw, fl);
+        c.twin_n[h] = tw;
+        c.flag_n[h] = fl;
+        if (tw == -1) atomicAdd(&c.arr[b], 1u);
+        cnt.add(fl);
+        vol_count(c.vst, c.qb, c.nvol, h, fl);
+        if (k == 3 && g[0] == h) put_record(recs, &c.S->nrec, reccap, &c.S->ovf, three_record(quads, g, h));
+        if (c.cflag[a] & 1u) {
+            const uint32_t s = aoff[a], e = aoff[a + 1];
+            int32_t hmax = -1;
+            for_row(aadj, s, e, [&](int2 x) { hmax = max(hmax, x.x); });
+            c.deg_n[a] = e - s;
+            if (h == hmax) c.hev_n[a] = h;
+        }
+    }
+    cnt.flush(&c.S->he);
+}
+
+// 3-face records of groups the compaction left alone, renumbered.
+__global__ void __launch_bounds__(kBlock) k_carry_records(const BndRec* __restrict__ recs, const Sizes* S,
+                                                           const int32_t* __restrict__ vmap,
+                                                           const uint8_t* __restrict__ cflag,
+                                                           const int32_t* __restrict__ qmap,
+                                                           BndRec* __restrict__ recs_n, uint32_t* nrec,
+                                                           uint32_t reccap, uint32_t* ovf) {
+    const int64_t n = S->nrec_prev < reccap ? S->nrec_prev : reccap;
+    const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    TMC_GRID_STRIDE(i, n) {
+        const BndRec r = recs[i];
+        const int32_t v0 = (int32_t)(r.key >> 32), v1 = (int32_t)(r.key & 0xFFFFFFFFull);
+        if (!vid_ok(v0, nv) || !vid_ok(v1, nv) || !vid_ok(r.a, nv) || !vid_ok(r.b, nv)) continue;
+        const int32_t y0 = vmap[v0], y1 = vmap[v1];
+        if (!vid_ok(y0, nn) || !vid_ok(y1, nn) || ((cflag[y0] | cflag[y1]) & 1u)) continue;
+        BndRec o;
+        o.key = he_key(y0, y1);
+        o.a = vmap[r.a];
+        o.b = vmap[r.b];
+        const int32_t m = qmap[r.lone >> 2];
+        if (m < 0) continue;
+        o.lone = 4 * m + (r.lone & 3);
+        o.pad = 0;
+        put_record(recs_n, nrec, reccap, ovf, o);
+    }
+}
+
+// Before a halfedge build: which way it pairs (ready: 1 the cell stage left
+// full rows, 2 a compaction carried the build over, 0 nothing) and its
+// statistics.  A carried build stands only if the build before it and the
+// compaction met no invalid vertex reference; otherwise the build counts
+// generic rows and pairs everything again.
 __global__ void k_rows_gate(Sizes* S, int ready) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const bool remapped = ready == 2 && S->remap_ok;
-        S->generic = ready == 0 ? 1u : ready == 1 ? (S->unresolved != 0u ? 1u : 0u) : (remapped ? 0u : 1u);
-        S->he.nonmanifold = 0;
-        S->he.errors = 0;
-        S->he.three = 0;
-        S->he.bad = remapped ? S->rbad : 0u;
-        S->he.pad = 0;
+        const bool incr = ready == 2 && S->incr_ok != 0u && S->cbad == 0u;
+        S->incr = incr ? 1u : 0u;
+        S->pair_rows = incr ? 0u : 1u;
+        S->generic = incr ? 0u : ready == 1 ? (S->unresolved != 0u ? 1u : 0u) : 1u;
+        if (!incr) {
+            S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
+            S->he.nm4 = 0;
+            S->he.errors = 0;
+            S->he.nm3 = 0;
+            S->he.bad = 0;
+            S->he.pad = 0;
+            S->nrec = 0;
+        }
     }
 }
 
-// Before a remap: it runs only if the last build saw no invalid vertex
-// reference (its rows then hold every surviving halfedge).
-__global__ void k_remap_gate(Sizes* S) {
+// Before a compaction's carry: the build it starts from, and fresh counters
+// for the carried one.
+__global__ void k_stage_gate(Sizes* S) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->remap_ok = S->he.bad == 0u ? 1u : 0u;
-        S->rbad = 0;
+        S->incr_ok = S->he.bad == 0u ? 1u : 0u;
+        S->he.nm4 = 0;
+        S->he.errors = 0;
+        S->he.nm3 = 0;
+        S->he.bad = 0;
+        S->he.pad = 0;
+        S->nrec_prev = S->nrec;
+        S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
+        S->nrec = 0;
+        S->hq = 4 * S->qn;
+        S->cbad = 0;
     }
 }
 
