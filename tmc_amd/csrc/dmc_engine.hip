@@ -122,7 +122,7 @@ struct dmc_context {
     // a compaction carries over, and the pairs swap.
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
         newV, newQ, clist, ccnt, tris, stats, off2, adj2, vmap, mlist, slow, twin2, flag2, deg2, hev2, bndrec2, arr,
-        cflag, aflag, qmap;
+        cflag, aflag, qmap, bcount, boff, alist;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     // topologically correct MC: per active cell vertex / triangle counts and
@@ -190,7 +190,7 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &vmap, &mlist, &slow, &twin2, &flag2, &deg2, &hev2, &bndrec2, &arr, &cflag,
-                      &aflag, &qmap, &drep, &dnew, &dval,
+                      &aflag, &qmap, &bcount, &boff, &alist, &drep, &dnew, &dval,
                       &dhist, &dedges, &dqual, &tcntv, &tcntt, &tvofs, &ttofs, &tamb, &tvb, &tstats, &tpos, &tnrm,
                       &ttris};
         for (Buf* b : all) bufs.push_back(b);
@@ -866,6 +866,8 @@ Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, c
     c.arr = ensure<uint32_t>(ctx, ctx->arr, vcap);
     c.aflag = ensure<uint8_t>(ctx, ctx->aflag, hcap);
     c.cnt = ensure<uint32_t>(ctx, ctx->cursor, vcap);
+    const int64_t turns = qcap / kBlock + 1;
+    c.bcount = ensure<uint32_t>(ctx, ctx->bcount, turns);
     c.S = S;
     c.qb = static_cast<const int64_t*>(ctx->qbound.p);
     c.nvol = ctx->nvol;
@@ -877,6 +879,7 @@ Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, c
     fill_u32(ctx, cflag, DevN{nullptr, (vcap + 4) / 4}, 0u);
     fill_u32(ctx, c.arr, nn, 0u);
     fill_u32(ctx, c.cnt, nn, 0u);
+    fill_u32(ctx, c.bcount, hn(turns), 0u);
     launch(ctx, "k_qmap", k_qmap, dim3(grid_for(ctx, qcap)), dim3(kBlock), drop, newq, (const Sizes*)S, qmap);
     launch(ctx, "k_carry_vertices", k_carry_vertices, dim3(grid_for(ctx, vcap)), dim3(kBlock), vrem, vmap,
            (const Sizes*)S, static_cast<const uint32_t*>(ctx->deg.p), static_cast<const int32_t*>(ctx->hev.p),
@@ -896,12 +899,16 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
     int2* aadj = ensure<int2>(ctx, ctx->adj2, hcap);
     BndRec* recs_n = ensure<BndRec>(ctx, ctx->bndrec2, ctx->reccap);
     const int gh = grid_for(ctx, hcap);
+    const int64_t turns = ctx->qcap / kBlock + 1;
+    uint32_t* boff = ensure<uint32_t>(ctx, ctx->boff, turns);
+    int32_t* alist = ensure<int32_t>(ctx, ctx->alist, hcap);
+    scan_to(ctx, ScanCounts{c.bcount}, boff, hn(turns), &S->naff, "k_scan");
     scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan");
     launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), aoff, nn, &S->tot, (const uint32_t*)nullptr);
-    launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(gh), dim3(kBlock), (const uint8_t*)c.aflag, (const Sizes*)S,
-           quads2, (const uint32_t*)aoff, c.cnt, aadj);
-    launch(ctx, "k_aff_twins", k_aff_twins, dim3(gh), dim3(kBlock), (const Sizes*)S, quads2, (const uint32_t*)aoff,
-           (const int2*)aadj, c, recs_n, (uint32_t)ctx->reccap);
+    launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(grid_for(ctx, ctx->qcap)), dim3(kBlock), (const Sizes*)S,
+           c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)boff, quads2, (const uint32_t*)aoff, c.cnt, aadj, alist);
+    launch(ctx, "k_aff_twins", k_aff_twins, dim3(gh), dim3(kBlock), (const int32_t*)alist, (const Sizes*)S, quads2,
+           (const uint32_t*)aoff, (const int2*)aadj, c, recs_n, (uint32_t)ctx->reccap);
     launch(ctx, "k_carry_records", k_carry_records, dim3(grid_for(ctx, ctx->reccap)), dim3(kBlock),
            static_cast<const BndRec*>(ctx->bndrec.p), (const Sizes*)S, vmap, c.cflag, c.qmap, recs_n, &S->nrec,
            (uint32_t)ctx->reccap, &S->ovf);

@@ -69,6 +69,7 @@ struct Sizes {
     uint64_t rows;               // first CSR's row total (4 Q)
     uint64_t tot;                // a scan total no one else reads
     uint64_t hq;                 // halfedges of the mesh a compaction leaves (4 qn)
+    uint64_t naff;               // halfedges a compaction left to pair again
     unsigned long long qhead;    // work queue of k_vertices: the next chunk of active cells
     HeStats he;                  // the last halfedge build
     uint32_t ovf;                // capacities exceeded: 1 active cells, 2 vertices, 4 quads, 8 3-face records

@@ -782,6 +782,7 @@ struct Carry {
     uint32_t* arr;                // boundary arrivals per new vertex (valence)
     uint8_t* aflag;               // new halfedges to pair again
     uint32_t* cnt;                // ... per origin
+    uint32_t* bcount;             // ... per 256 old faces (a block's turn of the compaction loop)
     Sizes* S;
     unsigned long long* vst;      // per-volume counts (last compaction of a batch)
     const int64_t* qb;            // first new face of each volume
@@ -882,9 +883,9 @@ __global__ void __launch_bounds__(kBlock) k_changed3333(const int32_t* __restric
 
 // The carry of surviving face f (new index nf, new corners r): every
 // halfedge either keeps its group (twin renumbered, flag, arrival) or is
-// flagged for pairing (listed by a scan afterwards).
-__device__ __forceinline__ void carry_face(const Carry& c, bool act, int64_t f, uint32_t nf, const int32_t r[4],
-                                           int64_t nn, HeCount& cnt) {
+// flagged for pairing.  Returns the number flagged.
+__device__ __forceinline__ uint32_t carry_face(const Carry& c, bool act, int64_t f, uint32_t nf, const int32_t r[4],
+                                               int64_t nn, HeCount& cnt) {
     int32_t tw[4] = {-1, -1, -1, -1};
     uint32_t fl = 0, af = 0;
     if (act) {
@@ -918,6 +919,22 @@ __device__ __forceinline__ void carry_face(const Carry& c, bool act, int64_t f, 
         reinterpret_cast<uint32_t*>(c.flag_n)[nf] = fl;
         reinterpret_cast<uint32_t*>(c.aflag)[nf] = af;
     }
+    return __popc(af);
+}
+
+// The block's count of flagged halfedges for its turn j of the compaction
+// loop (every thread of the block calls this).
+__device__ __forceinline__ void block_count(uint32_t* __restrict__ bcount, int64_t j, uint32_t k) {
+    __shared__ uint32_t sm[kBlock / kWave];
+    const uint32_t w = wave_sum(k);
+    if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kBlock / kWave; i++) t += sm[i];
+        bcount[j] = t;
+    }
+    __syncthreads();
 }
 
 // removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137), with
@@ -946,7 +963,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __r
                 if (r[k] < 0) atomicAdd(&c.S->cbad, 1u);
             reinterpret_cast<int4*>(quads2)[nf] = make_int4(r[0], r[1], r[2], r[3]);
         }
-        carry_face(c, act, f, nf, r, nn, cnt);
+        block_count(c.bcount, f0 / kBlock, carry_face(c, act, f, nf, r, nn, cnt));
     }
     cnt.flush(&c.S->he);
 }
@@ -983,39 +1000,52 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
                 if (r[k] < 0) atomicAdd(&c.S->cbad, 1u);
             reinterpret_cast<int4*>(quads2)[nf] = make_int4(r[0], r[1], r[2], r[3]);
         }
-        carry_face(c, act, f, nf, r, nn, cnt);
+        block_count(c.bcount, f0 / kBlock, carry_face(c, act, f, nf, r, nn, cnt));
     }
     cnt.flush(&c.S->he);
 }
 
-// Rows of the flagged halfedges by origin: aoff = exclusive scan of cnt;
-// cnt counts down to 0 as the rows fill.
-__global__ void __launch_bounds__(kBlock) k_aff_scatter(const uint8_t* __restrict__ aflag, const Sizes* S,
+// The flagged halfedges as a dense list, and their rows by origin.  Walks
+// the old faces in the compaction's turns of 256: boff = exclusive scan of
+// the turns' counts, ranks inside a turn from a block scan.  aoff =
+// exclusive scan of cnt; cnt counts down to 0 as the rows fill.
+__global__ void __launch_bounds__(kBlock) k_aff_scatter(const Sizes* S, const int32_t* __restrict__ qmap,
+                                                         const uint8_t* __restrict__ aflag,
+                                                         const uint32_t* __restrict__ boff,
                                                          const int32_t* __restrict__ quads,
                                                          const uint32_t* __restrict__ aoff, uint32_t* __restrict__ cnt,
-                                                         int2* __restrict__ aadj) {
-    const int64_t n = (int64_t)S->hq;
-    TMC_GRID_STRIDE(i, n) {
-        if (!aflag[i]) continue;
-        const int32_t h = (int32_t)i;
-        const int32_t a = quads[h], b = he_dest(quads, h);
-        const uint32_t p = aoff[a] + atomicSub(&cnt[a], 1u) - 1u;
-        aadj[p] = make_int2(h, b);
+                                                         int2* __restrict__ aadj, int32_t* __restrict__ alist) {
+    __shared__ uint32_t sm[kBlock / kWave + 1];
+    const int64_t nq = (int64_t)S->nq;
+    for (int64_t f0 = (int64_t)blockIdx.x * kBlock; f0 < nq; f0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t f = f0 + threadIdx.x;
+        const int32_t m = f < nq ? qmap[f] : -1;
+        const uint32_t af = m >= 0 ? reinterpret_cast<const uint32_t*>(aflag)[m] : 0u;
+        uint32_t tot;
+        uint32_t p = boff[f0 / kBlock] + block_exclusive_scan<uint32_t, kBlock>((uint32_t)__popc(af), sm, tot);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!((af >> (8 * k)) & 1u)) continue;
+            const int32_t h = 4 * m + k;
+            alist[p++] = h;
+            const int32_t a = quads[h], b = he_dest(quads, h);
+            aadj[aoff[a] + atomicSub(&cnt[a], 1u) - 1u] = make_int2(h, b);
+        }
     }
 }
 
 // Pair the flagged halfedges (cpp_dmc/DualMarchingCubes.cpp:574-669) through
 // their rows; a changed origin also takes its out-degree and he_v from its
 // row, which holds all its out-halfedges.
-__global__ void __launch_bounds__(kBlock) k_aff_twins(const Sizes* S, const int32_t* __restrict__ quads,
+__global__ void __launch_bounds__(kBlock) k_aff_twins(const int32_t* __restrict__ alist, const Sizes* S,
+                                                       const int32_t* __restrict__ quads,
                                                        const uint32_t* __restrict__ aoff,
                                                        const int2* __restrict__ aadj, Carry c, BndRec* __restrict__ recs,
                                                        uint32_t reccap) {
-    const int64_t n = (int64_t)S->hq, nn = (int64_t)S->nn;
+    const int64_t n = (int64_t)S->naff, nn = (int64_t)S->nn;
     HeCount cnt;
     TMC_GRID_STRIDE(i, n) {
-        if (!c.aflag[i]) continue;
-        const int32_t h = (int32_t)i;
+        const int32_t h = alist[i];
         const int32_t a = quads[h], b = he_dest(quads, h);
         int32_t g[5];
         const int k = he_group(quads, aoff, aadj, h, nn, g);
