@@ -35,10 +35,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per extraction of the cell-stage kernels from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration
-# (profiles/dmc_schwarz_1024_kernel_traffic.md)
-TRAFFIC = {}
+# Bytes per extraction the cell-stage kernels move between L2 and the fabric
+# (HBM and Infinity Cache), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# scaled by the copy-kernel calibration
+# (profiles/dmc_schwarz_1024_kernel_traffic.md): k_sweep 5.74 GB, k_rows
+# 1.15 GB, k_vertices 8.91 GB, k_quads 8.70 GB, the cell scans about 0.1 GB.
+TRAFFIC = {"schwarz_1024": 24_500_000_000}
 CELL_KERNELS = ("k_sweep", "k_scan_cells", "k_vol_abase", "k_rows", "k_vol_bounds", "k_vertices", "k_vertices_slow",
                 "k_quads")
 
