@@ -85,7 +85,10 @@ def _ogrid(field, grid, shape=None):
 def cpu_case(field, grid, iso, reps, label):
     """`reps` CPU runs of the whole field (median), dualMC alone and with
     the gradient."""
-    runs = [_cpu_run(field, _ogrid(field, grid), iso) for _ in range(reps)]
+    runs = []
+    for r in range(reps):
+        runs.append(_cpu_run(field, _ogrid(field, grid), iso))
+        log(f"[cpu] {label}: run {r + 1}/{reps}: gradient {runs[-1][2]:.3f} s, dualMC {runs[-1][3]:.3f} s")
     tg = float(np.median([r[2] for r in runs]))
     td = float(np.median([r[3] for r in runs]))
     n = field.size
@@ -448,6 +451,38 @@ def bench_dropin(eng, field, grid, iso, reps=2):
             "what": "host field in, host v/n/quads/tris out (PCIe both ways), as dualMC's callers see it"}
 
 
+def bench_small(reps=50):
+    """One small volume at a time: the genus-2 field at 128^3 (resident in
+    HBM), extracted again and again, as a served request of one shape is.
+    Wall clock per call, from the call to the mesh's counts on the host (the
+    pass's one wait); from the third call on, the pass replays a HIP
+    graph."""
+    import tmc_amd
+    from tmc_amd import workloads
+    f, g, iso = workloads.genus2(128)
+    eng = tmc_amd.Engine(0)
+    try:
+        buf = eng.alloc(f.nbytes).upload(f)
+        flags = tmc_amd.SIMPLIFY_ALL | tmc_amd.TRIANGLES | tmc_amd.FIELD_ON_DEVICE
+        for _ in range(4):
+            m = eng.extract(buf.ptr.value, iso, g, flags=flags)
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            m = eng.extract(buf.ptr.value, iso, g, flags=flags)
+            ts.append((time.perf_counter() - t) * 1e3)
+        buf.free()
+    finally:
+        eng.close()
+    out = {"workload": "genus2_128", "points": int(f.size), "calls": reps, "p50_ms": round(pct(ts, 50), 4),
+           "p99_ms": round(pct(ts, 99), 4), "min_ms": round(min(ts), 4),
+           "gvoxels_per_s_at_p50": round(f.size / (pct(ts, 50) * 1e-3) / 1e9, 3),
+           "pass": {0: "launched kernel by kernel", 1: "captured", 2: "HIP graph replay"}[m.pass_mode],
+           "host_syncs": m.host_syncs, "V": m.nv, "Q": m.nq}
+    log(f"[bench] small volume {out}")
+    return out
+
+
 def bench_diagnostics(eng, reps=3):
     """Time the mesh diagnostics on the last mesh (wall clock per call,
     results copied to the host included): unique edges, valence
@@ -624,6 +659,7 @@ def main():
         "kernels": kernels,
     }
     out["diagnostics"] = None if args.no_diagnostics else bench_diagnostics(eng)
+    out["small_volume"] = bench_small()
     if not args.no_dropin:
         out["dropin"] = bench_dropin(eng, field, grid, iso)
     if not args.no_tmc:

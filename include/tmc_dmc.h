@@ -203,10 +203,7 @@ dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions
  * pools the worker contexts keep (dmc_context_bytes) and the estimates of
  * the passes running -- and at most max_batch volumes and max_batch_points
  * grid points per pass (0: 2^28, which keeps any pass inside the engine's
- * index limits); the first request of a pass is always admitted.  The
- * oldest request sets a pass's kind: one of at least solo_points points
- * runs alone, else the pass takes the oldest small requests and passes
- * over large ones.  It runs
+ * index limits); the first request of a pass is always admitted.  It runs
  * them as one dmc_extract_batch and completes them together; a pass that
  * fails on a size limit of the union or on one bad member is run again one
  * request at a time, so only a request that fails alone fails.  The working
@@ -227,8 +224,6 @@ typedef struct {
     size_t budget_bytes;     /* admission budget; 0 = 96 GiB */
     int64_t max_batch_points;  /* grid points per pass (bounds a pass's duration); 0 = 2^28 */
     int32_t workers;         /* passes run concurrently, each with its own context and stream; 0 = 1 */
-    int64_t solo_points;     /* requests of at least this many grid points run in a pass of their own, and a
-                                pass of small requests passes over them; 0 = 2^24 (256^3) */
 } dmc_server_opts;
 
 typedef struct {

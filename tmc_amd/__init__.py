@@ -113,8 +113,7 @@ class TuneTrial(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("field_type", C.c_int), ("flags", C.c_uint32), ("max_batch", C.c_int32),
-                ("budget_bytes", C.c_size_t), ("max_batch_points", C.c_int64), ("workers", C.c_int32),
-                ("solo_points", C.c_int64)]
+                ("budget_bytes", C.c_size_t), ("max_batch_points", C.c_int64), ("workers", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -545,13 +544,13 @@ class Server:
     worker serves them one at a time."""
 
     def __init__(self, device=0, max_batch=1 << 20, budget_bytes=0, flags=SIMPLIFY_ALL | TRIANGLES,
-                 dtype=np.float32, max_batch_points=0, workers=1, solo_points=0):
+                 dtype=np.float32, max_batch_points=0, workers=1):
         self.lib = load_library()
         self.srv = C.c_void_p()
         self.dtype = np.dtype(dtype)
         self.flags = int(flags)
         opts = ServerOpts(int(device), FIELD_F32 if self.dtype == np.float32 else FIELD_F64, self.flags,
-                          int(max_batch), int(budget_bytes), int(max_batch_points), int(workers), int(solo_points))
+                          int(max_batch), int(budget_bytes), int(max_batch_points), int(workers))
         s = self.lib.dmc_server_create(C.byref(self.srv), C.byref(opts))
         if s != 0:
             raise DMCError(f"dmc_server_create: {self.lib.dmc_status_str(s).decode()}")

@@ -842,12 +842,11 @@ void gather_vertices(dmc_context* ctx) {
 }
 
 // The carry of a compaction, first half: fresh counters, the face map, the
-// out-degree and he_v carried from the kept preimages, the changed-vertex
-// flags and per-origin counts cleared, and the buffers of the carried build.
-// S->nn and S->qn must be on the device; `last`: the compaction before the
-// final build (a batch counts per volume).
-Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, const uint8_t* vrem,
-                  const int32_t* vmap, bool last) {
+// changed-vertex flags and per-origin counts cleared, and the buffers of the
+// carried build (the vertex map kernel then carries the kept vertices'
+// out-degree and he_v).  S->nn and S->qn must be on the device; `last`: the
+// compaction before the final build (a batch counts per volume).
+Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, bool last) {
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
     launch(ctx, "k_stage_gate", k_stage_gate, dim3(1), dim3(64), S);
@@ -881,9 +880,6 @@ Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, c
     fill_u32(ctx, c.cnt, nn, 0u);
     fill_u32(ctx, c.bcount, hn(turns), 0u);
     launch(ctx, "k_qmap", k_qmap, dim3(grid_for(ctx, qcap)), dim3(kBlock), drop, newq, (const Sizes*)S, qmap);
-    launch(ctx, "k_carry_vertices", k_carry_vertices, dim3(grid_for(ctx, vcap)), dim3(kBlock), vrem, vmap,
-           (const Sizes*)S, static_cast<const uint32_t*>(ctx->deg.p), static_cast<const int32_t*>(ctx->hev.p),
-           (const int32_t*)qmap, c.deg_n, c.hev_n);
     return c;
 }
 
@@ -906,7 +902,7 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
     scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan");
     launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), aoff, nn, &S->tot, (const uint32_t*)nullptr);
     launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(grid_for(ctx, ctx->qcap)), dim3(kBlock), (const Sizes*)S,
-           c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)boff, quads2, (const uint32_t*)aoff, c.cnt, aadj, alist);
+           c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)boff, quads2, (const uint32_t*)aoff, aadj, alist);
     launch(ctx, "k_aff_twins", k_aff_twins, dim3(gh), dim3(kBlock), (const int32_t*)alist, (const Sizes*)S, quads2,
            (const uint32_t*)aoff, (const int2*)aadj, c, recs_n, (uint32_t)ctx->reccap);
     launch(ctx, "k_carry_records", k_carry_records, dim3(grid_for(ctx, ctx->reccap)), dim3(kBlock),
@@ -959,10 +955,11 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
     compact_vertices(ctx, vfirst, vsecond);
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
-    launch(ctx, "k_vmap3x3y", k_vmap3x3y, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vfirst,
-           (const int32_t*)vsecond, (const Sizes*)S, vmap);
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
-    Carry c = carry_begin(ctx, em, newQ, vfirst, vmap, last);
+    Carry c = carry_begin(ctx, em, newQ, last);
+    launch(ctx, "k_vmap3x3y", k_vmap3x3y, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vfirst,
+           (const int32_t*)vsecond, (const Sizes*)S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
+           static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n);
     launch(ctx, "k_changed3x3y", k_changed3x3y, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
            (const Sizes*)S, (const uint8_t*)em, (const int32_t*)vmap, static_cast<const int32_t*>(ctx->hev.p),
            const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);
@@ -1002,14 +999,15 @@ void simplify3333(dmc_context* ctx, bool last) {
     scan_to(ctx, ScanKeep{rflag}, newQ, nq, &S->qn);
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
-    launch(ctx, "k_vmap3333", k_vmap3333, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vflag,
-           (const uint32_t*)newv, (const Sizes*)S, vmap);
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
     // the dropped quadrilaterals, whose corners are the changed vertices
     int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
     launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)rflag,
            (const uint32_t*)newQ, nq, dlist);
-    Carry c = carry_begin(ctx, rflag, newQ, vflag, vmap, last);
+    Carry c = carry_begin(ctx, rflag, newQ, last);
+    launch(ctx, "k_vmap3333", k_vmap3333, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vflag,
+           (const uint32_t*)newv, (const Sizes*)S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
+           static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n);
     launch(ctx, "k_changed3333", k_changed3333, dim3(gq), dim3(kBlock), quads, (const int32_t*)dlist,
            (const Sizes*)S, (const int32_t*)vmap, const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);
     launch(ctx, "k_compact_quads3333", k_compact_quads3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,

@@ -299,11 +299,21 @@ __global__ void __launch_bounds__(kBlock) k_sweep(VolTable vt, int64_t waves, ui
                                    segMask);
 }
 
+// A volume's field as a global-memory pointer.  The descriptor keeps a
+// generic pointer, through which the compiler emits flat loads; those count
+// in the LDS counter too, so an LDS wait would also wait for them.  Global
+// loads leave the LDS traffic of the cell kernels free to proceed.
 template <class T>
-__device__ __forceinline__ void load_corners(const T* __restrict__ field, const Dims& d, int i, int j, int k,
-                                             double F[8]) {
+using GPtr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ GPtr<T> gfield(const void* p) {
+    return (GPtr<T>)p;
+}
+
+template <class T, class P>
+__device__ __forceinline__ void load_corners(P field, const Dims& d, int i, int j, int k, double F[8]) {
     const int64_t sz = (int64_t)d.nx * d.ny;
-    const T* p = field + (int64_t)k * sz + (int64_t)j * d.nx + i;
+    const P p = field + (int64_t)k * sz + (int64_t)j * d.nx + i;
     F[0] = (double)p[0];
     F[1] = (double)p[1];
     F[2] = (double)p[d.nx];
@@ -401,7 +411,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
             cur = v;
         }
         const VolDesc& V = vt.desc[v];
-        const T* __restrict__ field = static_cast<const T*>(V.field);
+        const GPtr<T> field = gfield<T>(V.field);
         const Dims d = V.d;
         const double iso = V.iso;
         const int S = V.S;
@@ -452,7 +462,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                 uint64_t w;
                 if (case_ambiguous(icase)) {
                     double F[8];
-                    load_corners(field, d, i, j, k, F);
+                    load_corners<T>(field, d, i, j, k, F);
                     w = polygons_decider(iso, F);
                 } else {
                     w = c_case_contours[icase];
@@ -508,10 +518,10 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
 template <class T>
 __device__ __forceinline__ uint64_t cell_corners(const VolDesc& V, uint32_t cellword, uint32_t code,
                                                  const CornerRef& Fc, int& i, int& j, int& k, uint32_t& r) {
-    const T* __restrict__ field = static_cast<const T*>(V.field);
+    const GPtr<T> field = gfield<T>(V.field);
     cell_ijk(cellword, V.d, i, j, k, r);
     double F[8];
-    load_corners(field, V.d, i, j, k, F);
+    load_corners<T>(field, V.d, i, j, k, F);
     Fc.store(F);
     return cell_polygons(code & 0xFFu, V.iso, F);
 }
@@ -530,7 +540,7 @@ struct OuterValues {
 template <class T>
 __device__ __forceinline__ void load_outer(const VolDesc& V, int i, int j, int k, OuterValues<T>& o) {
     const Dims& d = V.d;
-    const T* __restrict__ field = static_cast<const T*>(V.field);
+    const GPtr<T> field = gfield<T>(V.field);
     auto idx = [](int x, int m) { return x < 0 ? 0 : (x >= m ? m - x : x); };
     const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
     const int xlo = idx(i - 1, d.nx), xhi = idx(i + 2, d.nx);
@@ -589,7 +599,7 @@ __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int
     for (int q = 0; q < 3; q++) {
         double G[8];
         if (V.grads) {
-            const double* __restrict__ grads = V.grads;
+            const GPtr<double> grads = gfield<double>(V.grads);
 #pragma unroll
             for (int c = 0; c < 8; c++) {
                 const int64_t x = ((int64_t)(k + ((c >> 2) & 1)) * d.ny + (j + ((c >> 1) & 1))) * d.nx + i + (c & 1);

@@ -726,23 +726,53 @@ __global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__
 
 // 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
 // compaction), vsecond[vsecond[x]] for a removed one (its target's).
+// A kept vertex x hands its out-degree and he_v (renumbered by the face
+// map) to its new id y; the changed vertices and those that lost a corner
+// are corrected afterwards (k_changed*).
+__device__ __forceinline__ void carry_vertex(int64_t x, int32_t y, int64_t nn, const uint32_t* __restrict__ deg,
+                                             const int32_t* __restrict__ hev, const int32_t* __restrict__ qmap,
+                                             uint32_t* __restrict__ deg_n, int32_t* __restrict__ hev_n) {
+    if (!vid_ok(y, nn)) return;
+    deg_n[y] = deg[x];
+    const int32_t e = hev[x];
+    const int32_t m = e < 0 ? -1 : qmap[e >> 2];
+    hev_n[y] = m < 0 ? -1 : 4 * m + (e & 3);
+}
+
+// 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
+// compaction), vsecond[vsecond[x]] for a removed one (its target's); and the
+// kept vertices' carry.
 __global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__ vfirst,
                                                       const int32_t* __restrict__ vsecond, const Sizes* S,
-                                                      int32_t* __restrict__ vmap) {
-    const int64_t nv = (int64_t)S->nv;
+                                                      int32_t* __restrict__ vmap, const uint32_t* __restrict__ deg,
+                                                      const int32_t* __restrict__ hev,
+                                                      const int32_t* __restrict__ qmap, uint32_t* __restrict__ deg_n,
+                                                      int32_t* __restrict__ hev_n) {
+    const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
     TMC_GRID_STRIDE(x, nv) {
         int32_t y = vsecond[x];
-        if (vfirst[x]) y = vid_ok(y, nv) ? vsecond[y] : -1;
+        const bool removed = vfirst[x] != 0;
+        if (removed) y = vid_ok(y, nv) ? vsecond[y] : -1;
         vmap[x] = y;
+        if (!removed) carry_vertex(x, y, nn, deg, hev, qmap, deg_n, hev_n);
     }
 }
 
-// 3333: vmap[x] = new id of a kept x, -1 for a removed one.
+// 3333: vmap[x] = new id of a kept x, -1 for a removed one; and the kept
+// vertices' carry.
 __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__ vflag,
                                                       const uint32_t* __restrict__ newv, const Sizes* S,
-                                                      int32_t* __restrict__ vmap) {
-    const int64_t nv = (int64_t)S->nv;
-    TMC_GRID_STRIDE(x, nv) vmap[x] = vflag[x] ? -1 : (int32_t)newv[x];
+                                                      int32_t* __restrict__ vmap, const uint32_t* __restrict__ deg,
+                                                      const int32_t* __restrict__ hev,
+                                                      const int32_t* __restrict__ qmap, uint32_t* __restrict__ deg_n,
+                                                      int32_t* __restrict__ hev_n) {
+    const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    TMC_GRID_STRIDE(x, nv) {
+        const bool removed = vflag[x] != 0;
+        const int32_t y = removed ? -1 : (int32_t)newv[x];
+        vmap[x] = y;
+        if (!removed) carry_vertex(x, y, nn, deg, hev, qmap, deg_n, hev_n);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -780,7 +810,7 @@ struct Carry {
     uint32_t* deg_n;
     int32_t* hev_n;
     uint32_t* arr;                // boundary arrivals per new vertex (valence)
-    uint8_t* aflag;               // new halfedges to pair again
+    uint8_t* aflag;               // new halfedges to pair again: 1 + rank in the origin's row, 0 if carried
     uint32_t* cnt;                // ... per origin
     uint32_t* bcount;             // ... per 256 old faces (a block's turn of the compaction loop)
     Sizes* S;
@@ -799,26 +829,6 @@ __global__ void __launch_bounds__(kBlock) k_qmap(const uint8_t* __restrict__ dro
                                                   const Sizes* S, int32_t* __restrict__ qmap) {
     const int64_t nq = (int64_t)S->nq;
     TMC_GRID_STRIDE(f, nq) qmap[f] = drop[f] ? -1 : (int32_t)newq[f];
-}
-
-// Out-degree and he_v of every new vertex from its kept preimage (the
-// changed vertices and those that lost a corner are corrected next).
-__global__ void __launch_bounds__(kBlock) k_carry_vertices(const uint8_t* __restrict__ vrem,
-                                                            const int32_t* __restrict__ vmap, const Sizes* S,
-                                                            const uint32_t* __restrict__ deg,
-                                                            const int32_t* __restrict__ hev,
-                                                            const int32_t* __restrict__ qmap,
-                                                            uint32_t* __restrict__ deg_n, int32_t* __restrict__ hev_n) {
-    const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
-    TMC_GRID_STRIDE(x, nv) {
-        if (vrem[x]) continue;
-        const int32_t y = vmap[x];
-        if (!vid_ok(y, nn)) continue;
-        deg_n[y] = deg[x];
-        const int32_t e = hev[x];
-        const int32_t m = e < 0 ? -1 : qmap[e >> 2];
-        hev_n[y] = m < 0 ? -1 : 4 * m + (e & 3);
-    }
 }
 
 // A changed vertex: paired again, its out-degree and he_v from its row.
@@ -883,11 +893,13 @@ __global__ void __launch_bounds__(kBlock) k_changed3333(const int32_t* __restric
 
 // The carry of surviving face f (new index nf, new corners r): every
 // halfedge either keeps its group (twin renumbered, flag, arrival) or is
-// flagged for pairing.  Returns the number flagged.
+// flagged for pairing, with its rank in its origin's row (a rank past 253,
+// never met in practice, sends the next build to the generic rows).
+// Returns the number flagged.
 __device__ __forceinline__ uint32_t carry_face(const Carry& c, bool act, int64_t f, uint32_t nf, const int32_t r[4],
                                                int64_t nn, HeCount& cnt) {
     int32_t tw[4] = {-1, -1, -1, -1};
-    uint32_t fl = 0, af = 0;
+    uint32_t fl = 0, af = 0, naff = 0;
     if (act) {
         const int4 t4 = reinterpret_cast<const int4*>(c.twin)[f];
         const uint32_t f4 = reinterpret_cast<const uint32_t*>(c.flag)[f];
@@ -903,8 +915,10 @@ __device__ __forceinline__ uint32_t carry_face(const Carry& c, bool act, int64_t
             const uint8_t ca = cf[k], cb = cf[(k + 1) & 3];
             if (ca & 4u) atomicMax(&c.hev_n[a], h);   // its largest halfedge was dropped
             if ((ca | cb) & 1u) {
-                af |= 1u << (8 * k);
-                atomicAdd(&c.cnt[a], 1u);
+                const uint32_t rk = atomicAdd(&c.cnt[a], 1u);
+                if (rk >= 254u) atomicOr(&c.S->cbad, 0x80000000u);
+                af |= (rk < 254u ? rk + 1u : 255u) << (8 * k);
+                naff++;
                 continue;
             }
             const int32_t nq = t[k] >= 0 ? c.qmap[t[k] >> 2] : -1;
@@ -919,7 +933,7 @@ __device__ __forceinline__ uint32_t carry_face(const Carry& c, bool act, int64_t
         reinterpret_cast<uint32_t*>(c.flag_n)[nf] = fl;
         reinterpret_cast<uint32_t*>(c.aflag)[nf] = af;
     }
-    return __popc(af);
+    return naff;
 }
 
 // The block's count of flagged halfedges for its turn j of the compaction
@@ -940,8 +954,8 @@ __device__ __forceinline__ void block_count(uint32_t* __restrict__ bcount, int64
 // removeQuadrilaterals3X3Y (cpp_dmc/DualMarchingCubes.cpp:1114-1137), with
 // the carry of the halfedges.  The vertex ids come from vmap (k_vmap3x3y):
 // vm[vm[x]]-style lookups of a removed vertex x, exactly as the reference
-// composes them.  Blocks stride over the faces in whole blocks, so that
-// every lane of a wave takes part in the carry's list append.
+// composes them.  Blocks stride over the faces in whole turns of 256, each
+// turn writing its count of flagged halfedges.
 __global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __restrict__ quads, const Sizes* S,
                                                                const int32_t* __restrict__ vmap,
                                                                int32_t* __restrict__ quads2, Carry c) {
@@ -1008,12 +1022,13 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
 // The flagged halfedges as a dense list, and their rows by origin.  Walks
 // the old faces in the compaction's turns of 256: boff = exclusive scan of
 // the turns' counts, ranks inside a turn from a block scan.  aoff =
-// exclusive scan of cnt; cnt counts down to 0 as the rows fill.
+// exclusive scan of the rows' counts; a halfedge's place in its row is the
+// rank its flag holds.
 __global__ void __launch_bounds__(kBlock) k_aff_scatter(const Sizes* S, const int32_t* __restrict__ qmap,
                                                          const uint8_t* __restrict__ aflag,
                                                          const uint32_t* __restrict__ boff,
                                                          const int32_t* __restrict__ quads,
-                                                         const uint32_t* __restrict__ aoff, uint32_t* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ aoff,
                                                          int2* __restrict__ aadj, int32_t* __restrict__ alist) {
     __shared__ uint32_t sm[kBlock / kWave + 1];
     const int64_t nq = (int64_t)S->nq;
@@ -1021,15 +1036,20 @@ __global__ void __launch_bounds__(kBlock) k_aff_scatter(const Sizes* S, const in
         const int64_t f = f0 + threadIdx.x;
         const int32_t m = f < nq ? qmap[f] : -1;
         const uint32_t af = m >= 0 ? reinterpret_cast<const uint32_t*>(aflag)[m] : 0u;
+        uint32_t k4 = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) k4 += ((af >> (8 * k)) & 0xFFu) != 0u;
         uint32_t tot;
-        uint32_t p = boff[f0 / kBlock] + block_exclusive_scan<uint32_t, kBlock>((uint32_t)__popc(af), sm, tot);
+        uint32_t p = boff[f0 / kBlock] + block_exclusive_scan<uint32_t, kBlock>(k4, sm, tot);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            if (!((af >> (8 * k)) & 1u)) continue;
+            const uint32_t code = (af >> (8 * k)) & 0xFFu;
+            if (!code) continue;
             const int32_t h = 4 * m + k;
             alist[p++] = h;
+            if (code == 255u) continue;   // rank unknown: this build is redone from generic rows
             const int32_t a = quads[h], b = he_dest(quads, h);
-            aadj[aoff[a] + atomicSub(&cnt[a], 1u) - 1u] = make_int2(h, b);
+            aadj[aoff[a] + code - 1u] = make_int2(h, b);
         }
     }
 }
@@ -1042,6 +1062,8 @@ __global__ void __launch_bounds__(kBlock) k_aff_twins(const int32_t* __restrict_
                                                        const uint32_t* __restrict__ aoff,
                                                        const int2* __restrict__ aadj, Carry c, BndRec* __restrict__ recs,
                                                        uint32_t reccap) {
+    // rows incomplete (a rank past 253): the next build starts over
+    if (S->cbad & 0x80000000u) return;
     const int64_t n = (int64_t)S->naff, nn = (int64_t)S->nn;
     HeCount cnt;
     TMC_GRID_STRIDE(i, n) {

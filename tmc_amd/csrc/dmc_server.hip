@@ -42,11 +42,6 @@ constexpr int kMaxWorkers = 8;
 // active cells, 2^31 vertices, 2^29 quadrilaterals) for any mix of
 // requests, each of which fits alone.
 constexpr int64_t kDefaultBatchPoints = (int64_t)1 << 28;
-// Default size from which a request runs in a pass of its own
-// (dmc_server_opts.solo_points = 0): 256^3 points.  Batching saves the
-// fixed cost of a pass, which a large volume's own work dwarfs; passing it
-// alone keeps the small requests behind it from waiting for it.
-constexpr int64_t kDefaultSoloPoints = (int64_t)1 << 24;
 
 // One pooled device allocation holding the meshes of one batch.
 struct Block {
@@ -349,44 +344,24 @@ void worker_main(dmc_server* s, Worker* w) {
             // the server holds (results, idle blocks, worker pools, passes in
             // flight)
             const int64_t max_points = s->opts.max_batch_points > 0 ? s->opts.max_batch_points : kDefaultBatchPoints;
-            const int64_t solo = s->opts.solo_points > 0 ? s->opts.solo_points : kDefaultSoloPoints;
-            auto npoints = [](const Request* r) {
-                return (int64_t)r->vol.grid.nx * r->vol.grid.ny * r->vol.grid.nz;
-            };
-            // requests that can never fit fail at once
-            while (!s->queue.empty() && s->queue.front()->est > budget) {
-                Request* r = s->queue.front();
-                s->queue.pop_front();
-                r->status = DMC_ERR_BUDGET;
-                r->done = true;
-                r->t_start = r->t_done = s->now();
-                s->cv_done.notify_all();
-            }
-            // the oldest request sets the pass's kind: a large one runs
-            // alone; otherwise the pass takes the oldest small requests,
-            // passing over large ones (which the next free worker takes
-            // first, as they reach the head)
             int64_t points = 0;
-            if (!s->queue.empty() && npoints(s->queue.front()) >= solo) {
+            while (!s->queue.empty() && (int)batch.size() < s->opts.max_batch) {
                 Request* r = s->queue.front();
-                s->queue.pop_front();
-                batch.push_back(r);
-                used += r->est;
-            }
-            for (auto it = s->queue.begin(); batch.empty() || npoints(batch[0]) < solo;) {
-                if (it == s->queue.end() || (int)batch.size() >= s->opts.max_batch) break;
-                Request* r = *it;
-                const int64_t pts = npoints(r);
-                if (pts >= solo || r->est > budget) {
-                    ++it;
+                if (r->est > budget) {
+                    s->queue.pop_front();
+                    r->status = DMC_ERR_BUDGET;
+                    r->done = true;
+                    r->t_start = r->t_done = s->now();
+                    s->cv_done.notify_all();
                     continue;
                 }
+                const int64_t pts = (int64_t)r->vol.grid.nx * r->vol.grid.ny * r->vol.grid.nz;
                 if (!batch.empty() && !fits(s, used + r->est)) break;
                 if (!batch.empty() && points + pts > max_points) break;
                 batch.push_back(r);
                 used += r->est;
                 points += pts;
-                it = s->queue.erase(it);
+                s->queue.pop_front();
             }
             s->inflight_bytes += used;
             // requests left over are for another idle worker
