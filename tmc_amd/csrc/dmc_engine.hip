@@ -117,6 +117,8 @@ struct dmc_context {
     // off/adj, 2 the build a compaction carried over, 0 nothing
     int ready = 0;
     bool rows_live = false;   // off/adj hold the rows of the last extraction's final mesh
+    bool force_generic = false;   // this extraction's passes pair every build through generic rows
+    bool commit_pending = false;  // a compaction's counts wait for the next build's gate to commit them
     // halfedges / simplification scratch.  twin, flag, deg (out-degree), hev
     // and bndrec are the current build's; the ...2 buffers receive the build
     // a compaction carries over, and the pairs swap.
@@ -421,17 +423,51 @@ void fill_u8(dmc_context* ctx, void* p, DevN n, uint8_t value) {
     launch(ctx, "k_fill", k_fill_u8, dim3(grid_for(ctx, n.cap)), dim3(kBlock), static_cast<uint8_t*>(p), n, value);
 }
 
+// Fills gathered into one launch (k_fill_many).
+struct Fills {
+    dmc_context* ctx;
+    FillSet s{};
+    int n = 0;
+    int64_t cap = 0;
+    explicit Fills(dmc_context* c) : ctx(c) {}
+    Fills& u32(void* p, DevN d, uint32_t value, const uint32_t* gate = nullptr) { return add(p, d, value, gate, 4); }
+    Fills& u8(void* p, DevN d, uint8_t value, const uint32_t* gate = nullptr) { return add(p, d, value, gate, 1); }
+    Fills& add(void* p, DevN d, uint32_t value, const uint32_t* gate, int bytes) {
+        if (d.cap <= 0) return *this;
+        if (n == kMaxFillJobs) run();
+        s.j[n++] = FillJob{p, d, value, gate, bytes};
+        cap = std::max(cap, d.cap);
+        return *this;
+    }
+    void run() {
+        if (n == 0) return;
+        launch(ctx, "k_fill", k_fill_many, dim3(grid_for(ctx, cap), n), dim3(kBlock), s);
+        n = 0;
+        cap = 0;
+    }
+};
+
 // Exclusive scan of n.get() uint32 items into out (launched for n.cap); the
 // 64-bit total goes to *dtotal (device memory).  Nothing runs when *gate is
 // 0.
+// Inputs of at most this many items are scanned by one block in one launch.
+constexpr int64_t kScanSmallItems = (int64_t)1 << 14;
+
+// Exclusive scan of n items into out, the total to *dtotal; row_end: also
+// write the total's low 32 bits to row_end[n] (the end of a CSR's last row).
 template <class In>
 void scan_to(dmc_context* ctx, In in, uint32_t* out, DevN n, uint64_t* dtotal, const char* name = "k_scan",
-             const uint32_t* gate = nullptr) {
-    const int64_t tiles = (std::max<int64_t>(n.cap, 0) + kScanTile - 1) / kScanTile;
-    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, std::max<int64_t>(tiles, 1));
-    if (tiles > 0) launch(ctx, name, k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), in, n, gate, part);
-    launch(ctx, name, k_scan_partials, dim3(1), dim3(1024), part, tiles, gate, dtotal);
-    if (tiles > 0) launch(ctx, name, k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, part);
+             const uint32_t* gate = nullptr, uint32_t* row_end = nullptr) {
+    if (n.cap <= kScanSmallItems) {
+        launch(ctx, name, k_scan_small<In>, dim3(1), dim3(1024), in, out, n, gate, dtotal, row_end);
+        check_launch(ctx, "scan");
+        return;
+    }
+    const int64_t tiles = (n.cap + kScanTile - 1) / kScanTile;
+    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
+    launch(ctx, name, k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), in, n, gate, part);
+    launch(ctx, name, k_scan_partials, dim3(1), dim3(1024), part, tiles, gate, dtotal, row_end, n);
+    launch(ctx, name, k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, part);
     check_launch(ctx, "scan");
 }
 
@@ -525,7 +561,9 @@ SweepPlan plan_sweep(dmc_context* ctx, const std::vector<HostVol>& hv) {
         units += (int64_t)(g.nz - 1) * D.S * D.ntj;
         cells += D.d.C * D.d.R;
     }
-    const int Kg = (int)std::max<int64_t>(8, (units + ctx->sweep_waves - 1) / ctx->sweep_waves);
+    // (at least 2 planes per wave, so that the shared boundary plane costs at
+    // most half again; small volumes get short marches and many waves)
+    const int Kg = (int)std::max<int64_t>(2, (units + ctx->sweep_waves - 1) / ctx->sweep_waves);
     const size_t tab_bytes = (size_t)n * sizeof(VolDesc) + 3 * (size_t)(n + 1) * sizeof(int64_t);
     ctx->vtab_host.assign(tab_bytes, 0);
     int64_t* waveBase = reinterpret_cast<int64_t*>(ctx->vtab_host.data() + (size_t)n * sizeof(VolDesc));
@@ -570,8 +608,7 @@ Swept sweep_cells(dmc_context* ctx, const SweepPlan& P) {
     VolStat* vstat = ensure<VolStat>(ctx, ctx->vstat, n);
     Sizes* S = reinterpret_cast<Sizes*>(ensure<uint8_t>(ctx, ctx->sizes, sizeof(Sizes)));
     static_assert(sizeof(Sizes) % 4 == 0 && sizeof(VolStat) % 4 == 0, "filled as words");
-    fill_u32(ctx, S, hn(sizeof(Sizes) / 4), 0u);
-    fill_u32(ctx, vstat, hn((int64_t)n * sizeof(VolStat) / 4), 0u);
+    Fills(ctx).u32(S, hn(sizeof(Sizes) / 4), 0u).u32(vstat, hn((int64_t)n * sizeof(VolStat) / 4), 0u).run();
 
     uint32_t* segCount = ensure<uint32_t>(ctx, ctx->segCount, RS);
     uint32_t* segBase = ensure<uint32_t>(ctx, ctx->segBase, RS);
@@ -587,8 +624,9 @@ Swept sweep_cells(dmc_context* ctx, const SweepPlan& P) {
     check_launch(ctx, "k_sweep");
     mark(ctx, "rows");
     scan_to(ctx, ScanCounts{segCount}, segBase, hn(RS), &S->A, "k_scan_cells");
-    launch(ctx, "k_vol_abase", k_vol_abase, dim3(grid_for(ctx, n + 1)), dim3(kBlock), vt, segBase, &S->A,
-           vol_ptr(ctx, 2));
+    if (n > 1)
+        launch(ctx, "k_vol_abase", k_vol_abase, dim3(grid_for(ctx, n + 1)), dim3(kBlock), vt, segBase, &S->A,
+               vol_ptr(ctx, 2));
     return Swept{R, P.cells, vt, vstat, lists, segCount, segBase, segMask};
 }
 
@@ -644,7 +682,7 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     uint32_t* rowBaseV = ensure<uint32_t>(ctx, ctx->rowBaseV, R);
     uint32_t* rowBaseQ = ensure<uint32_t>(ctx, ctx->rowBaseQ, R);
     launch(ctx, "k_rows", k_rows<T>, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), vt, R, sw.lists,
-           sw.segCount, sw.segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, sw.vstat, (uint32_t)acap);
+           sw.segCount, sw.segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, sw.vstat, S->cpart, (uint32_t)acap);
     check_launch(ctx, "k_rows");
     scan_to(ctx, ScanCounts{rowV}, rowBaseV, hn(R), &S->V, "k_scan_cells");
     scan_to(ctx, ScanCounts{rowQ}, rowBaseQ, hn(R), &S->Q, "k_scan_cells");
@@ -652,8 +690,9 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     // [0, n]: live bounds; [n+1, 2n+1]: the cell stage's (before simplification)
     int64_t* vb = ensure<int64_t>(ctx, ctx->vbound, 2 * (size_t)(n + 1));
     int64_t* qb = ensure<int64_t>(ctx, ctx->qbound, 2 * (size_t)(n + 1));
-    launch(ctx, "k_vol_bounds", k_vol_bounds, dim3(grid_for(ctx, n + 1)), dim3(kBlock), vt, rowBaseV, rowBaseQ, S,
-           vb, qb);
+    if (n > 1)
+        launch(ctx, "k_vol_bounds", k_vol_bounds, dim3(grid_for(ctx, n + 1)), dim3(kBlock), vt, rowBaseV, rowBaseQ, S,
+               vb, qb);
     ctx->cur = 0;
     ctx->pcur = 0;
     ctx->scur = 0;
@@ -681,9 +720,7 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     check_launch(ctx, "k_vertices_slow");
     mark(ctx, "quads");
     // the row total must be 4 Q (checked with the pass's sizes at the end)
-    scan_to(ctx, ScanCounts{deg}, off, dn(&S->nv, vcap), &S->rows, "k_scan_cells");
-    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, dn(&S->nv, vcap), &S->rows,
-           (const uint32_t*)nullptr);
+    scan_to(ctx, ScanCounts{deg}, off, dn(&S->nv, vcap), &S->rows, "k_scan_cells", nullptr, off);
     launch(ctx, "k_quads", k_quads, dim3(grid_for(ctx, acap)), dim3(kBlock), vt, acell, acode, aaux, avloc, aqloc,
            rowBaseV, rowBaseQ, A, sw.segMask, sw.segBase, quads, colors, off, adj, (uint64_t)(4 * qcap),
            &S->unresolved);
@@ -704,10 +741,10 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
 // invalid vertex reference), the build counts generic rows from the
 // quadrilaterals and pairs everything.
 
-// Pairing through full rows, on the device only when the build is not a
-// carried one (S->pair_rows): the cell stage's rows, or generic rows counted
-// from the quadrilaterals (S->generic).
-void pair_rows(dmc_context* ctx, bool final_pass) {
+// Pairing through full rows (on the device only when S->pair_rows): the
+// cell stage's rows, or (generic: launched only in a pass run again for
+// them) rows counted from the quadrilaterals.
+void pair_rows(dmc_context* ctx, bool final_pass, bool generic) {
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
     const int32_t* quads = static_cast<const int32_t*>(ctx->quads[ctx->cur].p);
@@ -723,21 +760,23 @@ void pair_rows(dmc_context* ctx, bool final_pass) {
     const uint32_t* gen = &S->generic;
     const uint32_t* rows = &S->pair_rows;
     const DevN nv = dn(&S->nv, vcap);
-    fill_u32(ctx, deg, nv, 0u, gen);
-    fill_u32(ctx, cursor, nv, 0u, gen);
-    launch(ctx, "k_he_degree", k_he_degree, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, deg,
-           &S->he.bad, gen);
-    scan_to(ctx, ScanCounts{deg}, off, nv, &S->tot, "k_scan", gen);
-    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, nv, &S->tot, gen);
-    launch(ctx, "k_he_scatter", k_he_scatter, dim3(grid_for(ctx, qcap)), dim3(kBlock), quads, (const Sizes*)S, off,
-           cursor, adj, gen);
-    fill_u32(ctx, hev, nv, 0xFFFFFFFFu, rows);
-    if (arr) fill_u32(ctx, arr, nv, 0u, rows);
+    if (generic) {
+        Fills(ctx).u32(deg, nv, 0u, gen).u32(cursor, nv, 0u, gen).run();
+        launch(ctx, "k_he_degree", k_he_degree, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S,
+               deg, &S->he.bad, gen);
+        scan_to(ctx, ScanCounts{deg}, off, nv, &S->tot, "k_scan", gen, off);
+        launch(ctx, "k_he_scatter", k_he_scatter, dim3(grid_for(ctx, qcap)), dim3(kBlock), quads, (const Sizes*)S,
+               off, cursor, adj, gen);
+    }
+    Fills fl(ctx);
+    fl.u32(hev, nv, 0xFFFFFFFFu, rows);
+    if (arr) fl.u32(arr, nv, 0u, rows);
     unsigned long long* vst = nullptr;
     if (final_pass && ctx->nvol > 1) {
         vst = ensure<unsigned long long>(ctx, ctx->vhe, 3 * (size_t)ctx->nvol);
-        fill_u32(ctx, vst, hn(6 * (int64_t)ctx->nvol), 0u, rows);
+        fl.u32(vst, hn(6 * (int64_t)ctx->nvol), 0u, rows);
     }
+    fl.run();
     launch(ctx, "k_he_twins", k_he_twins, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, off, adj,
            twin, flag, hev, arr, &S->he, recs, &S->nrec, (uint32_t)ctx->reccap, &S->ovf,
            static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst, rows);
@@ -749,9 +788,12 @@ void pair_rows(dmc_context* ctx, bool final_pass) {
 void build_halfedges(dmc_context* ctx, bool final_pass) {
     Sizes* S = dsizes(ctx);
     const int64_t vcap = ctx->vcap;
-    launch(ctx, "k_rows_gate", k_rows_gate, dim3(1), dim3(64), S, ctx->ready);
+    const int ready = ctx->force_generic ? 0 : ctx->ready;
+    launch(ctx, "k_rows_gate", k_rows_gate, dim3(1), dim3(64), S, ready, ctx->commit_pending ? 1 : 0);
+    ctx->commit_pending = false;
     ctx->ready = 0;
-    pair_rows(ctx, final_pass);
+    // a carried build pairs nothing here
+    if (ready != 2) pair_rows(ctx, final_pass, ready == 0);
     const int gv = grid_for(ctx, vcap);
     if (!final_pass) {
         uint32_t* val = ensure<uint32_t>(ctx, ctx->val, vcap);
@@ -774,17 +816,27 @@ void build_halfedges(dmc_context* ctx, bool final_pass) {
     check_launch(ctx, "final halfedges");
 }
 
+// Meshes of at most this many faces are coloured by one block.
+constexpr int64_t kSmallColourFaces = (int64_t)1 << 14;
+
 void color_faces(dmc_context* ctx) {
     const int64_t qcap = ctx->qcap;
     Sizes* S = dsizes(ctx);
     int32_t* colors = static_cast<int32_t*>(ctx->colors.p);
-    const int nb = (int)std::min<int64_t>(kColorBlocks, (qcap + kBlock - 1) / kBlock);
+    // one histogram block per 4096 faces of capacity, at most kColorBlocks
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kColorBlocks, qcap / 4096));
     uint32_t* hist = ensure<uint32_t>(ctx, ctx->ccnt, (size_t)kColorBlocks * 24);
     int32_t* list = ensure<int32_t>(ctx, ctx->clist, qcap);
     launch(ctx, "k_color_hist", k_color_hist, dim3(nb), dim3(kBlock), colors, (const Sizes*)S, hist);
-    launch(ctx, "k_color_offsets", k_color_offsets, dim3(1), dim3(kColorBlocks), hist, nb, S->colors);
+    launch(ctx, "k_color_offsets", k_color_offsets, dim3(1), dim3(1024), hist, nb, S->colors);
     launch(ctx, "k_color_scatter", k_color_scatter, dim3(nb), dim3(kBlock), colors, (const Sizes*)S, hist, list);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    if (qcap <= kSmallColourFaces) {
+        launch(ctx, "k_color_class", k_color_all, dim3(1), dim3(1024), (const int32_t*)list, (const uint32_t*)S->colors,
+               twin, colors);
+        check_launch(ctx, "colour");
+        return;
+    }
     // a class holds at most a quarter of the faces (four parities of one
     // edge direction)
     const int g = grid_for(ctx, qcap / 4 + 1);
@@ -800,6 +852,7 @@ void color_faces(dmc_context* ctx) {
 // compaction (newidx = exclusive scan of the keep flags over *nold items,
 // *nnew kept).
 void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, const uint64_t* nold, const uint64_t* nnew) {
+    if (ctx->nvol == 1) return;   // one volume: its bounds are [0, count], read from the sizes
     launch(ctx, "k_vol_remap", k_vol_remap, dim3(grid_for(ctx, ctx->nvol + 1)), dim3(kBlock),
            static_cast<int64_t*>(bounds.p), ctx->nvol + 1, newidx, nold, nnew);
     check_launch(ctx, "k_vol_remap");
@@ -849,7 +902,6 @@ void gather_vertices(dmc_context* ctx) {
 Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, bool last) {
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
-    launch(ctx, "k_stage_gate", k_stage_gate, dim3(1), dim3(64), S);
     const DevN nn = dn(&S->nn, vcap);
     Carry c{};
     c.twin = static_cast<const int32_t*>(ctx->twin.p);
@@ -870,16 +922,16 @@ Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, b
     c.S = S;
     c.qb = static_cast<const int64_t*>(ctx->qbound.p);
     c.nvol = ctx->nvol;
+    Fills fl(ctx);
     if (last && ctx->nvol > 1) {
         c.vst = ensure<unsigned long long>(ctx, ctx->vhe, 3 * (size_t)ctx->nvol);
-        fill_u32(ctx, c.vst, hn(6 * (int64_t)ctx->nvol), 0u);
+        fl.u32(c.vst, hn(6 * (int64_t)ctx->nvol), 0u);
     }
     // whole words: the marks are set with word atomics
-    fill_u32(ctx, cflag, DevN{nullptr, (vcap + 4) / 4}, 0u);
-    fill_u32(ctx, c.arr, nn, 0u);
-    fill_u32(ctx, c.cnt, nn, 0u);
-    fill_u32(ctx, c.bcount, hn(turns), 0u);
-    launch(ctx, "k_qmap", k_qmap, dim3(grid_for(ctx, qcap)), dim3(kBlock), drop, newq, (const Sizes*)S, qmap);
+    fl.u32(cflag, DevN{nullptr, (vcap + 4) / 4}, 0u).u32(c.arr, nn, 0u).u32(c.cnt, nn, 0u).u32(c.bcount, hn(turns), 0u);
+    fl.run();
+    // (also opens the carry: counters of the carried build)
+    launch(ctx, "k_qmap", k_qmap, dim3(grid_for(ctx, qcap)), dim3(kBlock), drop, newq, S, qmap);
     return c;
 }
 
@@ -899,8 +951,7 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
     uint32_t* boff = ensure<uint32_t>(ctx, ctx->boff, turns);
     int32_t* alist = ensure<int32_t>(ctx, ctx->alist, hcap);
     scan_to(ctx, ScanCounts{c.bcount}, boff, hn(turns), &S->naff, "k_scan");
-    scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan");
-    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), aoff, nn, &S->tot, (const uint32_t*)nullptr);
+    scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan", nullptr, aoff);
     launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(grid_for(ctx, ctx->qcap)), dim3(kBlock), (const Sizes*)S,
            c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)boff, quads2, (const uint32_t*)aoff, aadj, alist);
     launch(ctx, "k_aff_twins", k_aff_twins, dim3(gh), dim3(kBlock), (const int32_t*)alist, (const Sizes*)S, quads2,
@@ -909,7 +960,8 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
            static_cast<const BndRec*>(ctx->bndrec.p), (const Sizes*)S, vmap, c.cflag, c.qmap, recs_n, &S->nrec,
            (uint32_t)ctx->reccap, &S->ovf);
     check_launch(ctx, "carry");
-    launch(ctx, "k_commit", k_commit, dim3(1), dim3(64), S);
+    // the counts are committed by the next build's gate
+    ctx->commit_pending = true;
     swap_pairs(ctx);
     ctx->cur = 1 - ctx->cur;
     ctx->ready = 2;
@@ -929,10 +981,8 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
     int32_t* vsecond = ensure<int32_t>(ctx, ctx->vsecond, vcap);
     int32_t* lastT = ensure<int32_t>(ctx, ctx->lastT, vcap);
     int32_t* lastR = ensure<int32_t>(ctx, ctx->lastR, vcap);
-    fill_u8(ctx, vfirst, nv, 0);
-    fill_u32(ctx, vsecond, nv, 0xFFFFFFFFu);
-    fill_u32(ctx, lastT, nv, 0xFFFFFFFFu);
-    fill_u32(ctx, lastR, nv, 0xFFFFFFFFu);
+    Fills(ctx).u8(vfirst, nv, 0).u32(vsecond, nv, 0xFFFFFFFFu).u32(lastT, nv, 0xFFFFFFFFu)
+        .u32(lastR, nv, 0xFFFFFFFFu).run();
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
@@ -981,11 +1031,8 @@ void simplify3333(dmc_context* ctx, bool last) {
     int32_t* lastF = ensure<int32_t>(ctx, ctx->lastT, vcap);
     uint8_t* p3333 = ensure<uint8_t>(ctx, ctx->p, qcap);
     uint8_t* rflag = ensure<uint8_t>(ctx, ctx->em, qcap);
-    fill_u8(ctx, vflag, nv, 0);
-    fill_u32(ctx, vtwin, nv, 0xFFFFFFFFu);
-    fill_u32(ctx, lastF, nv, 0xFFFFFFFFu);
-    fill_u8(ctx, p3333, nq, 0);
-    fill_u8(ctx, rflag, nq, 0);
+    Fills(ctx).u8(vflag, nv, 0).u32(vtwin, nv, 0xFFFFFFFFu).u32(lastF, nv, 0xFFFFFFFFu).u8(p3333, nq, 0)
+        .u8(rflag, nq, 0).run();
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int gq = grid_for(ctx, qcap);
@@ -1156,6 +1203,7 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
 // The pass's launches, in order; nothing is read back.
 void enqueue_pass(dmc_context* ctx, const SweepPlan& P, int field_type, uint32_t flags) {
     const int n = ctx->nvol;
+    ctx->commit_pending = false;
     if (field_type == DMC_FIELD_F32) run_cells<float>(ctx, P);
     else run_cells<double>(ctx, P);
     const bool iso = (flags & DMC_SIMPLIFY_3X3Y_ISOLATED) != 0, p3333 = (flags & DMC_SIMPLIFY_3333) != 0;
@@ -1233,6 +1281,11 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
         ctx->rv = std::max(ctx->rv, (double)hs.V / (double)cells);
         ctx->rq = std::max(ctx->rq, (double)hs.Q / (double)cells);
     }
+    if (hs.need_generic) {
+        // a build needed generic rows: the same pass again with them
+        ctx->force_generic = true;
+        if (!hs.ovf) return false;
+    }
     if (hs.ovf) {
         // exceeded a capacity: the same pass again with room for the exact
         // sizes (the counts above are exact whatever the capacities)
@@ -1289,6 +1342,8 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
         f2 += x.fail2;
         nc += x.contours;
     }
+    if (n == 1)
+        for (unsigned long long c : hs.cpart) nc += c;
     M.failed_projections_l1 = (uint32_t)f1;
     M.failed_projections_l2 = (uint32_t)f2;
     M.nv_emitted = (int64_t)nc;
@@ -1359,6 +1414,7 @@ std::string pass_key(dmc_context* ctx, const std::vector<HostVol>& hv, const Swe
     put(ctx->vcap);
     put(ctx->qcap);
     put(ctx->reccap);
+    put(ctx->force_generic);
     put((int64_t)ctx->buf_epoch);
     put((int64_t)hv.size());
     for (const HostVol& h : hv) put(((int64_t)h.g.nx << 42) | ((int64_t)h.g.ny << 21) | h.g.nz);
@@ -1519,7 +1575,9 @@ void do_extract(dmc_context* ctx, void* a_) {
         *a.tout = ctx->tmesh;
         return;
     }
-    // a pass that outgrew a capacity learned the exact sizes: once more
+    // a pass that outgrew a capacity learned the exact sizes, and one that
+    // needed generic halfedge rows asks for them: once more
+    ctx->force_generic = false;
     bool done = false;
     for (int attempt = 0; attempt < 3 && !done; attempt++) {
         if (attempt > 0 && ctx->arena) {
@@ -1878,8 +1936,7 @@ static void ensure_rows(dmc_context* ctx) {
     fill_u32(ctx, cursor, nv, 0u);
     launch(ctx, "k_he_degree", k_he_degree, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, deg,
            &S->he.pad, (const uint32_t*)nullptr);
-    scan_to(ctx, ScanCounts{deg}, off, nv, &S->tot, "k_scan");
-    launch(ctx, "k_set_row_end", k_set_row_end, dim3(1), dim3(64), off, nv, &S->tot, (const uint32_t*)nullptr);
+    scan_to(ctx, ScanCounts{deg}, off, nv, &S->tot, "k_scan", nullptr, off);
     launch(ctx, "k_he_scatter", k_he_scatter, dim3(grid_for(ctx, qcap)), dim3(kBlock), quads, (const Sizes*)S, off,
            cursor, adj, (const uint32_t*)nullptr);
     check_launch(ctx, "rows");

@@ -396,9 +396,15 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                                                   uint32_t* __restrict__ aaux,
                                                   uint32_t* __restrict__ avloc, uint32_t* __restrict__ aqloc,
                                                   uint32_t* __restrict__ rowV, uint32_t* __restrict__ rowQ,
-                                                  VolStat* __restrict__ vstat, uint32_t acap) {
+                                                  VolStat* __restrict__ vstat, unsigned long long* __restrict__ cpart,
+                                                  uint32_t acap) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wpb = kBlock / kWave;
+    // one volume: its contour count is spread over 16 counters (summed by
+    // the host), so that the waves' adds do not queue on one address
+    auto contours = [&](int vol) {
+        return vt.n == 1 ? &cpart[(blockIdx.x * wpb + threadIdx.x / kWave) & 15] : &vstat[vol].contours;
+    };
     unsigned long long total = 0;
     int cur = -1;  // volume whose contour count `total` holds (wave-uniform)
     for (int64_t g = (int64_t)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave); g < Rtot;
@@ -406,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
         const int v = vol_of(vt.rowBase, vt.n, g);
         if (v != cur) {
             total = wave_sum(total);
-            if (lane == 0 && total) atomicAdd(&vstat[cur].contours, total);
+            if (lane == 0 && total) atomicAdd(contours(cur), total);
             total = 0;
             cur = v;
         }
@@ -502,7 +508,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
         }
     }
     total = wave_sum(total);
-    if (lane == 0 && total) atomicAdd(&vstat[cur].contours, total);
+    if (lane == 0 && total) atomicAdd(contours(cur), total);
 }
 
 // ---------------------------------------------------------------------------

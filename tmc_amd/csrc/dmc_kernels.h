@@ -76,6 +76,7 @@ struct Sizes {
     uint32_t unresolved;         // quadrilaterals with an unresolved neighbour cell
     uint32_t zero;               // vertices of valence 0 (all valence passes)
     uint32_t generic;            // this halfedge build counts its rows from the quadrilaterals
+    uint32_t need_generic;       // a build needed generic rows the pass did not launch: run it again with them
     uint32_t pair_rows;          // this build pairs halfedges through full rows (cell rows or generic)
     uint32_t incr;               // this build is the one the last compaction carried over
     uint32_t incr_ok;            // the build before the compaction saw no invalid vertex reference
@@ -84,6 +85,7 @@ struct Sizes {
     uint32_t nrec_prev;          // ... of the build before (carried over by a compaction)
     uint32_t recmax;             // most 3-face records any build of the pass wrote (capacity planning)
     uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
+    unsigned long long cpart[16];   // a single volume's contour count, in 16 parts
 };
 
 // A launch whose work is conditional on device data: it runs only when
@@ -209,9 +211,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, DevN dn, const ui
     }
 }
 
-// Single block: exclusive scan of the tile sums in place, total to *total.
+// Single block: exclusive scan of the tile sums in place, total to *total
+// (and, for a CSR, its low 32 bits to row_end[n], the end of the last row).
 __global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ partial, int64_t ntiles,
-                                                         const uint32_t* __restrict__ gate, uint64_t* __restrict__ total) {
+                                                         const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                         uint32_t* __restrict__ row_end, DevN dn) {
     __shared__ uint64_t sm[1024 / kWave + 1];
     if (gated_off(gate)) return;
     uint64_t carry = 0;
@@ -223,7 +227,10 @@ __global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ p
         if (idx < ntiles) partial[idx] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (threadIdx.x == 0) {
+        *total = carry;
+        if (row_end) row_end[dn.get()] = (uint32_t)carry;
+    }
 }
 
 template <class In>
@@ -259,6 +266,40 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
     }
 }
 
+// The whole scan in one block, for short inputs (one launch instead of
+// three): tiles of 1024 x kScanIPT items in turn, carrying the running sum.
+template <class In>
+__global__ void __launch_bounds__(1024) k_scan_small(In in, uint32_t* __restrict__ out, DevN dn,
+                                                      const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                      uint32_t* __restrict__ row_end) {
+    __shared__ uint32_t sm[1024 / kWave + 1];
+    if (gated_off(gate)) return;
+    const int64_t n = dn.get();
+    uint32_t carry = 0;
+    uint64_t carry64 = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += (int64_t)1024 * kScanIPT) {
+        const int64_t base = t0 + (int64_t)threadIdx.x * kScanIPT;
+        uint32_t v[kScanIPT];
+        scan_items(in, n, base, v);
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++) sum += v[q];
+        uint32_t tot;
+        uint32_t run = block_exclusive_scan<uint32_t, 1024>(sum, sm, tot) + carry;
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++) {
+            if (base + q < n) out[base + q] = run;
+            run += v[q];
+        }
+        carry += tot;
+        carry64 += tot;
+    }
+    if (threadIdx.x == 0) {
+        *total = carry64;
+        if (row_end) row_end[n] = carry;
+    }
+}
+
 // Fill n = dn.get() 32-bit words with `value` (a kernel rather than a memset
 // node, so that a captured pass replays it as a kernel).
 __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, DevN dn, uint32_t value,
@@ -271,6 +312,35 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, D
 __global__ void __launch_bounds__(kBlock) k_fill_u8(uint8_t* __restrict__ p, DevN dn, uint8_t value) {
     const int64_t n = dn.get();
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = value;
+}
+
+// Several fills in one launch: fill y of the grid is job y (32-bit words or
+// bytes), each gated on its own.
+struct FillJob {
+    void* p;
+    DevN n;
+    uint32_t value;
+    const uint32_t* gate;
+    int bytes;   // 4 or 1
+};
+constexpr int kMaxFillJobs = 6;
+struct FillSet {
+    FillJob j[kMaxFillJobs];
+};
+
+__global__ void __launch_bounds__(kBlock) k_fill_many(FillSet s) {
+    const FillJob& J = s.j[blockIdx.y];
+    if (gated_off(J.gate)) return;
+    const int64_t n = J.n.get();
+    if (J.bytes == 4) {
+        uint32_t* p = static_cast<uint32_t*>(J.p);
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+            p[i] = J.value;
+    } else {
+        uint8_t* p = static_cast<uint8_t*>(J.p);
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+            p[i] = (uint8_t)J.value;
+    }
 }
 
 // off[n] = the low 32 bits of a scan total (the end of the last CSR row).

@@ -414,24 +414,54 @@ __global__ void __launch_bounds__(kBlock) k_color_hist(const int32_t* __restrict
     if (threadIdx.x < 24) hist[blockIdx.x * 24 + threadIdx.x] = sm[threadIdx.x];
 }
 
-// One block of kColorBlocks threads: totals per colour, class starts, and
-// per-(block, colour) offsets in place of the histogram -- colour by colour,
-// a block scan over the histogram blocks (thread b holds block b).
-__global__ void __launch_bounds__(kColorBlocks) k_color_offsets(uint32_t* __restrict__ hist, int nblocks,
-                                                                 uint32_t* __restrict__ totals) {
-    __shared__ uint32_t sm[kColorBlocks / kWave + 1];
-    const int b = threadIdx.x;
-    uint32_t start = 0;
-    for (int c = 0; c < 24; c++) {
-        const uint32_t h = b < nblocks ? hist[b * 24 + c] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<uint32_t, kColorBlocks>(h, sm, tot);
-        if (b < nblocks) hist[b * 24 + c] = start + ex;
-        if (b == 0) {
-            totals[c] = tot;
-            totals[24 + c] = start;
+// One block of 1024 threads (16 waves): totals per colour, class starts, and
+// per-(block, colour) offsets in place of the histogram.  Wave w takes
+// colours w and w + 16; each lane holds the counts of blocks lane + 64 q
+// (nblocks <= kColorBlocks = 1024), so every load is issued before the
+// wave's scans.
+__global__ void __launch_bounds__(1024) k_color_offsets(uint32_t* __restrict__ hist, int nblocks,
+                                                         uint32_t* __restrict__ totals) {
+    static_assert(kColorBlocks <= 16 * kWave, "16 counts per lane");
+    __shared__ uint32_t tot[24], start[24];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t v[2][16];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int c = w + 16 * h;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int b = lane + kWave * q;
+            v[h][q] = (c < 24 && b < nblocks) ? hist[b * 24 + c] : 0u;
         }
-        start += tot;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) sum += v[h][q];
+        sum = wave_sum(sum);
+        if (c < 24 && lane == 0) tot[c] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int c = 0; c < 24; c++) {
+            start[c] = run;
+            totals[c] = tot[c];
+            totals[24 + c] = run;
+            run += tot[c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int c = w + 16 * h;
+        if (c >= 24) continue;   // wave-uniform
+        uint32_t run = start[c];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int b = lane + kWave * q;
+            const uint32_t inc = wave_inclusive_scan(v[h][q]);
+            if (b < nblocks) hist[b * 24 + c] = run + inc - v[h][q];
+            run += __shfl(inc, kWave - 1, kWave);
+        }
     }
 }
 
@@ -467,6 +497,35 @@ __global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restric
                 break;
             }
         }
+    }
+}
+
+// All classes in one block, in order, for a small mesh (one launch instead
+// of twenty): the same first-free-colour rule, a barrier between classes.
+__global__ void __launch_bounds__(1024) k_color_all(const int32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ totals,
+                                                     const int32_t* __restrict__ twin, int32_t* __restrict__ colors) {
+    for (int step = 0; step < 20; step++) {
+        const int cls = step < 19 ? 5 + step : 4;
+        const int ncol = step < 19 ? 5 : 4;
+        const int64_t n = totals[cls], start = totals[24 + cls];
+        for (int64_t x = threadIdx.x; x < n; x += blockDim.x) {
+            const int32_t f = list[start + x];
+            int32_t nb[4];
+            face_neighbors(twin, f, nb);
+            int32_t c[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) c[k] = nb[k] >= 0 ? colors[nb[k]] : -1;
+            for (int col = 0; col < ncol; col++) {
+                if (c[0] != col && c[1] != col && c[2] != col && c[3] != col) {
+                    colors[f] = col;
+                    break;
+                }
+            }
+        }
+        // the next class reads this one's colours
+        __threadfence_block();
+        __syncthreads();
     }
 }
 
@@ -825,9 +884,25 @@ __device__ __forceinline__ void mark_vertex(uint8_t* cflag, int32_t y, uint32_t 
 }
 
 // The compaction as one face map: qmap[f] = new index, -1 if dropped.
+// Thread 0 also opens the carry: it records whether the build it starts
+// from met invalid vertex references, and starts fresh counters for the
+// carried build (nothing else in this launch reads them).
 __global__ void __launch_bounds__(kBlock) k_qmap(const uint8_t* __restrict__ drop, const uint32_t* __restrict__ newq,
-                                                  const Sizes* S, int32_t* __restrict__ qmap) {
+                                                  Sizes* S, int32_t* __restrict__ qmap) {
     const int64_t nq = (int64_t)S->nq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->incr_ok = S->he.bad == 0u ? 1u : 0u;
+        S->he.nm4 = 0;
+        S->he.errors = 0;
+        S->he.nm3 = 0;
+        S->he.bad = 0;
+        S->he.pad = 0;
+        S->nrec_prev = S->nrec;
+        S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
+        S->nrec = 0;
+        S->hq = 4 * S->qn;
+        S->cbad = 0;
+    }
     TMC_GRID_STRIDE(f, nq) qmap[f] = drop[f] ? -1 : (int32_t)newq[f];
 }
 
@@ -1118,17 +1193,25 @@ __global__ void __launch_bounds__(kBlock) k_carry_records(const BndRec* __restri
     }
 }
 
-// Before a halfedge build: which way it pairs (ready: 1 the cell stage left
-// full rows, 2 a compaction carried the build over, 0 nothing) and its
-// statistics.  A carried build stands only if the build before it and the
-// compaction met no invalid vertex reference; otherwise the build counts
-// generic rows and pairs everything again.
-__global__ void k_rows_gate(Sizes* S, int ready) {
+// Before a halfedge build: which way it pairs.  ready: 0 generic rows
+// (launched only in a pass run again for them), 1 the cell stage's rows, 2
+// the build a compaction carried over.  The cell stage's rows are whole
+// unless a quadrilateral has an unresolved neighbour cell; a carried build
+// stands only if the build before it and the compaction met no invalid
+// vertex reference.  Otherwise the build needs generic rows, which this
+// pass did not launch: it flags the pass to be run again with them.
+__global__ void k_rows_gate(Sizes* S, int ready, int commit) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (commit) {   // the compaction before this build: its counts become the live mesh
+            S->nv = S->nn;
+            S->nq = S->qn;
+        }
         const bool incr = ready == 2 && S->incr_ok != 0u && S->cbad == 0u;
+        const bool need = (ready == 1 && S->unresolved != 0u) || (ready == 2 && !incr);
+        if (need) S->need_generic = 1u;
         S->incr = incr ? 1u : 0u;
-        S->pair_rows = incr ? 0u : 1u;
-        S->generic = incr ? 0u : ready == 1 ? (S->unresolved != 0u ? 1u : 0u) : 1u;
+        S->pair_rows = ready == 0 || (ready == 1 && !need) ? 1u : 0u;
+        S->generic = ready == 0 ? 1u : 0u;
         if (!incr) {
             S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
             S->he.nm4 = 0;
@@ -1141,31 +1224,7 @@ __global__ void k_rows_gate(Sizes* S, int ready) {
     }
 }
 
-// Before a compaction's carry: the build it starts from, and fresh counters
-// for the carried one.
-__global__ void k_stage_gate(Sizes* S) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->incr_ok = S->he.bad == 0u ? 1u : 0u;
-        S->he.nm4 = 0;
-        S->he.errors = 0;
-        S->he.nm3 = 0;
-        S->he.bad = 0;
-        S->he.pad = 0;
-        S->nrec_prev = S->nrec;
-        S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
-        S->nrec = 0;
-        S->hq = 4 * S->qn;
-        S->cbad = 0;
-    }
-}
 
-// The compaction is done: the kept counts become the live mesh.
-__global__ void k_commit(Sizes* S) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->nv = S->nn;
-        S->nq = S->qn;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Diagnostics of a finished mesh
