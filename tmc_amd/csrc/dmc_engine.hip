@@ -119,6 +119,7 @@ struct dmc_context {
     bool rows_live = false;   // off/adj hold the rows of the last extraction's final mesh
     bool force_generic = false;   // this extraction's passes pair every build through generic rows
     bool commit_pending = false;  // a compaction's counts wait for the next build's gate to commit them
+    bool always_generic = false;  // every extraction pairs through generic rows (TMC_GENERIC_ROWS=1)
     // halfedges / simplification scratch.  twin, flag, deg (out-degree), hev
     // and bndrec are the current build's; the ...2 buffers receive the build
     // a compaction carries over, and the pairs swap.
@@ -1577,7 +1578,7 @@ void do_extract(dmc_context* ctx, void* a_) {
     }
     // a pass that outgrew a capacity learned the exact sizes, and one that
     // needed generic halfedge rows asks for them: once more
-    ctx->force_generic = false;
+    ctx->force_generic = ctx->always_generic;
     bool done = false;
     for (int attempt = 0; attempt < 3 && !done; attempt++) {
         if (attempt > 0 && ctx->arena) {
@@ -1632,6 +1633,9 @@ dmc_status dmc_create(dmc_context** out, int device) {
         if (v >= kGridCapMin && v <= kGridCapMax) ctx->grid_cap = (int)v;
     }
     if (const char* gr = getenv("TMC_GRAPHS")) ctx->use_graphs = strtol(gr, nullptr, 10) != 0;
+    // every halfedge build through generic rows (the path a pass takes when
+    // it is run again for them), for checking that path against the others
+    if (const char* g = getenv("TMC_GENERIC_ROWS")) ctx->always_generic = strtol(g, nullptr, 10) != 0;
     if (const char* w = getenv("TMC_SWEEP_WAVES")) {
         const long v = strtol(w, nullptr, 10);
         if (v >= kSweepWavesMin && v <= kSweepWavesMax) ctx->sweep_waves = (int)v;
