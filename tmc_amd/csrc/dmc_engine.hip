@@ -125,7 +125,7 @@ struct dmc_context {
     // a compaction carries over, and the pairs swap.
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
         newV, newQ, clist, ccnt, tris, stats, off2, adj2, vmap, mlist, slow, twin2, flag2, deg2, hev2, bndrec2, arr,
-        cflag, aflag, qmap, bcount, boff, alist;
+        cflag, aflag, bcount, boff, alist;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
     // topologically correct MC: per active cell vertex / triangle counts and
@@ -193,7 +193,7 @@ struct dmc_context {
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &vmap, &mlist, &slow, &twin2, &flag2, &deg2, &hev2, &bndrec2, &arr, &cflag,
-                      &aflag, &qmap, &bcount, &boff, &alist, &drep, &dnew, &dval,
+                      &aflag, &bcount, &boff, &alist, &drep, &dnew, &dval,
                       &dhist, &dedges, &dqual, &tcntv, &tcntt, &tvofs, &ttofs, &tamb, &tvb, &tstats, &tpos, &tnrm,
                       &ttris};
         for (Buf* b : all) bufs.push_back(b);
@@ -900,15 +900,14 @@ void gather_vertices(dmc_context* ctx) {
 // carried build (the vertex map kernel then carries the kept vertices'
 // out-degree and he_v).  S->nn and S->qn must be on the device; `last`: the
 // compaction before the final build (a batch counts per volume).
-Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, bool last) {
+Carry carry_begin(dmc_context* ctx, const uint32_t* newq, bool last) {
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
     const DevN nn = dn(&S->nn, vcap);
     Carry c{};
     c.twin = static_cast<const int32_t*>(ctx->twin.p);
     c.flag = static_cast<const uint8_t*>(ctx->flag.p);
-    int32_t* qmap = ensure<int32_t>(ctx, ctx->qmap, qcap);
-    c.qmap = qmap;
+    c.qmap = reinterpret_cast<const int32_t*>(newq);
     uint8_t* cflag = ensure<uint8_t>(ctx, ctx->cflag, vcap + 4);
     c.cflag = cflag;
     c.twin_n = ensure<int32_t>(ctx, ctx->twin2, hcap);
@@ -931,8 +930,6 @@ Carry carry_begin(dmc_context* ctx, const uint8_t* drop, const uint32_t* newq, b
     // whole words: the marks are set with word atomics
     fl.u32(cflag, DevN{nullptr, (vcap + 4) / 4}, 0u).u32(c.arr, nn, 0u).u32(c.cnt, nn, 0u).u32(c.bcount, hn(turns), 0u);
     fl.run();
-    // (also opens the carry: counters of the carried build)
-    launch(ctx, "k_qmap", k_qmap, dim3(grid_for(ctx, qcap)), dim3(kBlock), drop, newq, S, qmap);
     return c;
 }
 
@@ -954,7 +951,8 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
     scan_to(ctx, ScanCounts{c.bcount}, boff, hn(turns), &S->naff, "k_scan");
     scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan", nullptr, aoff);
     launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(grid_for(ctx, ctx->qcap)), dim3(kBlock), (const Sizes*)S,
-           c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)boff, quads2, (const uint32_t*)aoff, aadj, alist);
+           c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)c.bcount, (const uint32_t*)boff, quads2,
+           (const uint32_t*)aoff, aadj, alist);
     launch(ctx, "k_aff_twins", k_aff_twins, dim3(gh), dim3(kBlock), (const int32_t*)alist, (const Sizes*)S, quads2,
            (const uint32_t*)aoff, (const int2*)aadj, c, recs_n, (uint32_t)ctx->reccap);
     launch(ctx, "k_carry_records", k_carry_records, dim3(grid_for(ctx, ctx->reccap)), dim3(kBlock),
@@ -994,9 +992,9 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
            (const uint8_t*)p, colors, twin, flag, lastT, lastR, em, vfirst);
     check_launch(ctx, "merge3x3y");
     // the quadrilateral compaction's scan also lists the merging faces
-    scan_to(ctx, ScanKeep{em}, newQ, nq, &S->qn);
+    scan_to(ctx, ScanKeepMap{{em}}, newQ, nq, &S->qn);
     int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
-    launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)em, (const uint32_t*)newQ,
+    launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)em, (const int32_t*)newQ,
            nq, mlist);
     launch(ctx, "k_merge_apply", k_merge_apply, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
            (const Sizes*)S, (const uint8_t*)em, val, (const int32_t*)lastT, (const int32_t*)lastR,
@@ -1007,9 +1005,9 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
-    Carry c = carry_begin(ctx, em, newQ, last);
+    Carry c = carry_begin(ctx, newQ, last);
     launch(ctx, "k_vmap3x3y", k_vmap3x3y, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vfirst,
-           (const int32_t*)vsecond, (const Sizes*)S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
+           (const int32_t*)vsecond, S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
            static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n);
     launch(ctx, "k_changed3x3y", k_changed3x3y, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
            (const Sizes*)S, (const uint8_t*)em, (const int32_t*)vmap, static_cast<const int32_t*>(ctx->hev.p),
@@ -1044,17 +1042,17 @@ void simplify3333(dmc_context* ctx, bool last) {
     check_launch(ctx, "mark3333");
     uint32_t* newv = compact_vertices(ctx, vflag, nullptr);
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
-    scan_to(ctx, ScanKeep{rflag}, newQ, nq, &S->qn);
+    scan_to(ctx, ScanKeepMap{{rflag}}, newQ, nq, &S->qn);
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
     // the dropped quadrilaterals, whose corners are the changed vertices
     int32_t* dlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
     launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)rflag,
-           (const uint32_t*)newQ, nq, dlist);
-    Carry c = carry_begin(ctx, rflag, newQ, last);
+           (const int32_t*)newQ, nq, dlist);
+    Carry c = carry_begin(ctx, newQ, last);
     launch(ctx, "k_vmap3333", k_vmap3333, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vflag,
-           (const uint32_t*)newv, (const Sizes*)S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
+           (const uint32_t*)newv, S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
            static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n);
     launch(ctx, "k_changed3333", k_changed3333, dim3(gq), dim3(kBlock), quads, (const int32_t*)dlist,
            (const Sizes*)S, (const int32_t*)vmap, const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);

@@ -945,12 +945,16 @@ __global__ void k_vol_bounds(VolTable vt, const uint32_t* __restrict__ rowBaseV,
 
 // b[v] -> number of kept elements before it (newIdx = exclusive scan of the
 // keep flags over *nOld elements, *nNew kept).
+// newIdx: the exclusive scan of the keep flags, or a compaction map
+// (ScanKeepMap: ~rank among the removed at a removed item), from which the
+// kept count before x is x - ~map[x].
 __global__ void k_vol_remap(int64_t* __restrict__ b, int n1, const uint32_t* __restrict__ newIdx,
                             const uint64_t* nOld, const uint64_t* nNew) {
     const int64_t no = (int64_t)*nOld, nn = (int64_t)*nNew;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n1; v += gridDim.x * blockDim.x) {
         const int64_t x = b[v];
-        b[v] = x < no ? (int64_t)newIdx[x] : nn;
+        const int32_t m = x < no ? (int32_t)newIdx[x] : 0;
+        b[v] = x < no ? (m >= 0 ? (int64_t)m : x - (int64_t)~m) : nn;
     }
 }
 

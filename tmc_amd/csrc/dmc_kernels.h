@@ -143,6 +143,7 @@ constexpr int kScanTile = kBlock * kScanIPT;
 // Scan inputs: uint32 counts, or keep flags (1 where a byte flag is 0).
 // Thread t of a tile holds the 16 consecutive items from base = tile * kScanTile + 16 t.
 struct ScanCounts {
+    static constexpr bool kMap = false;
     const uint32_t* p;
     __device__ __forceinline__ uint32_t at(int64_t i) const { return p[i]; }
     __device__ __forceinline__ void vec16(int64_t base, uint32_t v[kScanIPT]) const {
@@ -157,6 +158,7 @@ struct ScanCounts {
     }
 };
 struct ScanSet {
+    static constexpr bool kMap = false;
     const uint8_t* p;   // set flags
     __device__ __forceinline__ uint32_t at(int64_t i) const { return p[i] ? 1u : 0u; }
     __device__ __forceinline__ void vec16(int64_t base, uint32_t v[kScanIPT]) const {
@@ -167,6 +169,7 @@ struct ScanSet {
     }
 };
 struct ScanKeep {
+    static constexpr bool kMap = false;
     const uint8_t* p;   // removed flags
     __device__ __forceinline__ uint32_t at(int64_t i) const { return p[i] ? 0u : 1u; }
     __device__ __forceinline__ void vec16(int64_t base, uint32_t v[kScanIPT]) const {
@@ -175,6 +178,14 @@ struct ScanKeep {
 #pragma unroll
         for (int q = 0; q < kScanIPT; q++) v[q] = ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) ? 0u : 1u;
     }
+};
+
+// Keep flags whose scan is written as a compaction map: a kept item gets
+// its new index, a removed one ~(its rank among the removed), which is
+// negative as an int32 (so the map reads -1-or-below for "dropped", and the
+// rank is ~map).  The kept count before item i is map[i] or i - ~map[i].
+struct ScanKeepMap : ScanKeep {
+    static constexpr bool kMap = true;
 };
 
 template <class In>
@@ -252,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
     uint32_t o[kScanIPT];
 #pragma unroll
     for (int q = 0; q < kScanIPT; q++) {
-        o[q] = run;
+        o[q] = (In::kMap && v[q] == 0u) ? ~(uint32_t)(base + q - run) : run;
         run += v[q];
     }
     if (full) {
@@ -288,7 +299,7 @@ __global__ void __launch_bounds__(1024) k_scan_small(In in, uint32_t* __restrict
         uint32_t run = block_exclusive_scan<uint32_t, 1024>(sum, sm, tot) + carry;
 #pragma unroll
         for (int q = 0; q < kScanIPT; q++) {
-            if (base + q < n) out[base + q] = run;
+            if (base + q < n) out[base + q] = (In::kMap && v[q] == 0u) ? ~(uint32_t)(base + q - run) : run;
             run += v[q];
         }
         carry += tot;

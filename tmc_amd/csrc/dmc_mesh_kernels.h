@@ -672,11 +672,11 @@ __global__ void __launch_bounds__(kBlock) k_merge_apply(const int32_t* __restric
 // The flagged items, ascending, at i - newidx[i] (newidx = exclusive scan of
 // the keep flags): a dense list without atomics.
 __global__ void __launch_bounds__(kBlock) k_list_flagged(const uint8_t* __restrict__ flag,
-                                                          const uint32_t* __restrict__ newidx, DevN dn,
+                                                          const int32_t* __restrict__ map, DevN dn,
                                                           int32_t* __restrict__ list) {
     const int64_t n = dn.get();
     TMC_GRID_STRIDE(i, n) {
-        if (flag[i]) list[i - newidx[i]] = (int32_t)i;
+        if (flag[i]) list[~map[i]] = (int32_t)i;
     }
 }
 
@@ -785,6 +785,24 @@ __global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__
 
 // 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
 // compaction), vsecond[vsecond[x]] for a removed one (its target's).
+// Opens a compaction's carry (thread 0 of the vertex-map kernel, whose other
+// threads do not read these): whether the build it starts from met invalid
+// vertex references, and fresh counters for the carried build.
+__device__ __forceinline__ void open_carry(Sizes* S) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    S->incr_ok = S->he.bad == 0u ? 1u : 0u;
+    S->he.nm4 = 0;
+    S->he.errors = 0;
+    S->he.nm3 = 0;
+    S->he.bad = 0;
+    S->he.pad = 0;
+    S->nrec_prev = S->nrec;
+    S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
+    S->nrec = 0;
+    S->hq = 4 * S->qn;
+    S->cbad = 0;
+}
+
 // A kept vertex x hands its out-degree and he_v (renumbered by the face
 // map) to its new id y; the changed vertices and those that lost a corner
 // are corrected afterwards (k_changed*).
@@ -802,12 +820,13 @@ __device__ __forceinline__ void carry_vertex(int64_t x, int32_t y, int64_t nn, c
 // compaction), vsecond[vsecond[x]] for a removed one (its target's); and the
 // kept vertices' carry.
 __global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__ vfirst,
-                                                      const int32_t* __restrict__ vsecond, const Sizes* S,
+                                                      const int32_t* __restrict__ vsecond, Sizes* S,
                                                       int32_t* __restrict__ vmap, const uint32_t* __restrict__ deg,
                                                       const int32_t* __restrict__ hev,
                                                       const int32_t* __restrict__ qmap, uint32_t* __restrict__ deg_n,
                                                       int32_t* __restrict__ hev_n) {
     const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    open_carry(S);
     TMC_GRID_STRIDE(x, nv) {
         int32_t y = vsecond[x];
         const bool removed = vfirst[x] != 0;
@@ -820,12 +839,13 @@ __global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__
 // 3333: vmap[x] = new id of a kept x, -1 for a removed one; and the kept
 // vertices' carry.
 __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__ vflag,
-                                                      const uint32_t* __restrict__ newv, const Sizes* S,
+                                                      const uint32_t* __restrict__ newv, Sizes* S,
                                                       int32_t* __restrict__ vmap, const uint32_t* __restrict__ deg,
                                                       const int32_t* __restrict__ hev,
                                                       const int32_t* __restrict__ qmap, uint32_t* __restrict__ deg_n,
                                                       int32_t* __restrict__ hev_n) {
     const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
+    open_carry(S);
     TMC_GRID_STRIDE(x, nv) {
         const bool removed = vflag[x] != 0;
         const int32_t y = removed ? -1 : (int32_t)newv[x];
@@ -862,7 +882,7 @@ __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__
 struct Carry {
     const int32_t* twin;          // the build before the compaction
     const uint8_t* flag;
-    const int32_t* qmap;          // new index of each face, -1 if dropped
+    const int32_t* qmap;          // new index of each face; negative if dropped (ScanKeepMap)
     const uint8_t* cflag;         // changed new vertices (bit 1), vertices whose he_v is taken again (bit 4)
     int32_t* twin_n;              // the carried build
     uint8_t* flag_n;
@@ -883,28 +903,6 @@ __device__ __forceinline__ void mark_vertex(uint8_t* cflag, int32_t y, uint32_t 
     atomicOr(reinterpret_cast<uint32_t*>(cflag) + (y >> 2), bits << (8 * (y & 3)));
 }
 
-// The compaction as one face map: qmap[f] = new index, -1 if dropped.
-// Thread 0 also opens the carry: it records whether the build it starts
-// from met invalid vertex references, and starts fresh counters for the
-// carried build (nothing else in this launch reads them).
-__global__ void __launch_bounds__(kBlock) k_qmap(const uint8_t* __restrict__ drop, const uint32_t* __restrict__ newq,
-                                                  Sizes* S, int32_t* __restrict__ qmap) {
-    const int64_t nq = (int64_t)S->nq;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->incr_ok = S->he.bad == 0u ? 1u : 0u;
-        S->he.nm4 = 0;
-        S->he.errors = 0;
-        S->he.nm3 = 0;
-        S->he.bad = 0;
-        S->he.pad = 0;
-        S->nrec_prev = S->nrec;
-        S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
-        S->nrec = 0;
-        S->hq = 4 * S->qn;
-        S->cbad = 0;
-    }
-    TMC_GRID_STRIDE(f, nq) qmap[f] = drop[f] ? -1 : (int32_t)newq[f];
-}
 
 // A changed vertex: paired again, its out-degree and he_v from its row.
 __device__ __forceinline__ void mark_changed(uint8_t* cflag, uint32_t* deg_n, int32_t* hev_n, int32_t y) {
@@ -1101,6 +1099,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
 // rank its flag holds.
 __global__ void __launch_bounds__(kBlock) k_aff_scatter(const Sizes* S, const int32_t* __restrict__ qmap,
                                                          const uint8_t* __restrict__ aflag,
+                                                         const uint32_t* __restrict__ bcount,
                                                          const uint32_t* __restrict__ boff,
                                                          const int32_t* __restrict__ quads,
                                                          const uint32_t* __restrict__ aoff,
@@ -1108,6 +1107,7 @@ __global__ void __launch_bounds__(kBlock) k_aff_scatter(const Sizes* S, const in
     __shared__ uint32_t sm[kBlock / kWave + 1];
     const int64_t nq = (int64_t)S->nq;
     for (int64_t f0 = (int64_t)blockIdx.x * kBlock; f0 < nq; f0 += (int64_t)gridDim.x * kBlock) {
+        if (bcount[f0 / kBlock] == 0u) continue;   // a turn with nothing flagged (block-uniform)
         const int64_t f = f0 + threadIdx.x;
         const int32_t m = f < nq ? qmap[f] : -1;
         const uint32_t af = m >= 0 ? reinterpret_cast<const uint32_t*>(aflag)[m] : 0u;
