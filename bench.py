@@ -288,7 +288,7 @@ def _alone_ms(eng, pool, flags, reps=3):
     return out
 
 
-def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2, pool_kind="mix"):
+def bench_serve(seed=5, secs=4.0, batch_points=32 << 20, workers=2, pool_kind="mix"):
     """Served workload: Poisson arrivals drawing uniformly from a pool of
     device-resident fields.  pool_kind "mix": the serving mix (70%
     U{32..128}^3, 25% 256^3, 5% 512^3); "small": 256 volumes of the batched
@@ -341,7 +341,7 @@ def bench_serve(seed=5, secs=4.0, batch_points=64 << 20, workers=2, pool_kind="m
                       "batched capacity")["volumes_per_s"]
     best = None
     tried = []
-    for mult in (1.25, 1.5, 1.75, 2, 2.5, 3, 3.5, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32):
+    for mult in (1.1, 1.25, 1.5, 1.75, 2, 2.5, 3, 3.5, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32):
         r = mult * rate
         if r > 1.05 * cap_b:
             break

@@ -139,11 +139,13 @@ struct dmc_context {
     int64_t syncs = 0;         // stream synchronisations of HostReads so far
     int64_t syncs_at_start = 0;   // ... when the current extraction began
     // capacities of the pass's arrays (active cells, vertices,
-    // quadrilaterals, 3-face records), planned from the largest counts per
-    // cell seen (ra, rv, rq) and raised to exact sizes after an overflow
+    // quadrilaterals, 3-face records), planned from the largest densities of
+    // active cells, vertices and quadrilaterals seen in one volume, per cell
+    // (rcell) and per cells^(2/3) (rarea), and raised to exact sizes after an
+    // overflow
     int64_t acap = 0, vcap = 0, qcap = 0, reccap = 0;
     int64_t acap_min = 0, vcap_min = 0, qcap_min = 0;
-    double ra = 0, rv = 0, rq = 0;
+    double rcell[3] = {0, 0, 0}, rarea[3] = {0, 0, 0};
     Sizes hsz{};               // the last pass's sizes block
     // captured passes (replayed when a pass of the same shape comes again)
     struct PassGraph {
@@ -631,12 +633,15 @@ Swept sweep_cells(dmc_context* ctx, const SweepPlan& P) {
     return Swept{R, P.cells, vt, vstat, lists, segCount, segBase, segMask};
 }
 
-// Capacities of the cell-stage arrays for a pass over `cells` cells: the
-// largest count per cell this context has seen (with a margin), at least a
-// default share; a pass that exceeds one is run again with the exact size.
-// Capacities are rounded up to one of 8 steps per power of two, so that
-// passes of one shape keep the same capacities (and buffers) while the
-// context's ratios creep up.
+// Capacities of the cell-stage arrays for a pass over the volumes `hv`: per
+// volume the smaller of two estimates from the densest volumes this context
+// has seen (with a margin) -- count per cell x cells, and count per
+// cells^(2/3) x cells^(2/3), since a surface's cells grow with its area, so
+// that a small dense volume does not size a large one by its density -- and
+// over the pass at least a default share of the cells; a pass that exceeds
+// one is run again with the exact size.  Capacities are rounded up to one of
+// 8 steps per power of two, so that passes of one shape keep the same
+// capacities (and buffers) while the context's densities creep up.
 int64_t cap_step(int64_t x) {
     if (x <= 4096) return 4096;
     int b = 63 - __builtin_clzll((unsigned long long)x);
@@ -644,18 +649,25 @@ int64_t cap_step(int64_t x) {
     return (x + unit - 1) / unit * unit;
 }
 
-void plan_caps(dmc_context* ctx, int64_t cells) {
-    auto cap = [&](double ratio) {
-        const double r = std::max(ratio * 1.02, 0.02);
-        return cap_step((int64_t)std::ceil((double)cells * r) + 1024);
+double area_of(double cells) { return std::cbrt(cells * cells); }
+
+void plan_caps(dmc_context* ctx, const std::vector<HostVol>& hv, int64_t cells) {
+    double est[3] = {0, 0, 0};
+    for (const HostVol& h : hv) {
+        const double c = (double)(h.g.nx - 1) * (double)(h.g.ny - 1) * (double)(h.g.nz - 1);
+        const double a = area_of(c);
+        for (int k = 0; k < 3; k++) est[k] += std::min(ctx->rcell[k] * c, ctx->rarea[k] * a) * 1.02;
+    }
+    auto cap = [&](int k) {
+        return cap_step((int64_t)std::ceil(std::max(est[k], 0.02 * (double)cells)) + 1024);
     };
     // small passes: room for every cell (a few MB), so that a dense small
     // volume never runs twice -- except in the fixed scratch of a no-alloc
     // call, where a rerun is cheaper than the room
     const int64_t floor = ctx->arena ? 0 : cap_step(std::min<int64_t>(cells, kSmallPassCells));
-    ctx->acap = std::max({cap_step(ctx->acap_min), cap(ctx->ra), floor});
-    ctx->vcap = std::max({cap_step(ctx->vcap_min), cap(ctx->rv), floor});
-    ctx->qcap = std::max({cap_step(ctx->qcap_min), cap(ctx->rq), floor});
+    ctx->acap = std::max({cap_step(ctx->acap_min), cap(0), floor});
+    ctx->vcap = std::max({cap_step(ctx->vcap_min), cap(1), floor});
+    ctx->qcap = std::max({cap_step(ctx->qcap_min), cap(2), floor});
     ctx->acap = std::min<int64_t>(std::min<int64_t>(ctx->acap, cells), 0xFFFFFFFF);
     ctx->vcap = std::min<int64_t>(ctx->vcap, 0x7FFFFFFF);
     ctx->qcap = std::min<int64_t>(ctx->qcap, 0x1FFFFFFF);
@@ -1272,13 +1284,19 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
     if (hs.V > 0x7FFFFFFFull || hs.Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)hs.V,
              (unsigned long long)hs.Q);
-    // what this context has seen, for the next pass's capacities
-    int64_t cells = 0;
-    for (const HostVol& h : hv) cells += (int64_t)(h.g.nx - 1) * (h.g.ny - 1) * (h.g.nz - 1);
-    if (cells > 0) {
-        ctx->ra = std::max(ctx->ra, (double)hs.A / (double)cells);
-        ctx->rv = std::max(ctx->rv, (double)hs.V / (double)cells);
-        ctx->rq = std::max(ctx->rq, (double)hs.Q / (double)cells);
+    // what this context has seen, for the next pass's capacities: each
+    // volume's cell-stage counts (exact whatever the capacities)
+    for (int v = 0; v < n; v++) {
+        const dmc_grid& g = hv[v].g;
+        const double c = (double)(g.nx - 1) * (double)(g.ny - 1) * (double)(g.nz - 1);
+        if (c <= 0) continue;
+        const double cnt[3] = {n == 1 ? (double)hs.A : (double)(ab[v + 1] - ab[v]),
+                               n == 1 ? (double)hs.V : (double)(vb[n + 1 + v + 1] - vb[n + 1 + v]),
+                               n == 1 ? (double)hs.Q : (double)(qb[n + 1 + v + 1] - qb[n + 1 + v])};
+        for (int k = 0; k < 3; k++) {
+            ctx->rcell[k] = std::max(ctx->rcell[k], cnt[k] / c);
+            ctx->rarea[k] = std::max(ctx->rarea[k], cnt[k] / area_of(c));
+        }
     }
     if (hs.need_generic) {
         // a build needed generic rows: the same pass again with them
@@ -1585,7 +1603,7 @@ void do_extract(dmc_context* ctx, void* a_) {
             ctx->arena_used = 0;
         }
         const SweepPlan P = a.field_type == DMC_FIELD_F32 ? plan_sweep<float>(ctx, hv) : plan_sweep<double>(ctx, hv);
-        plan_caps(ctx, P.cells);
+        plan_caps(ctx, hv, P.cells);
         run_pass(ctx, P, a.field_type, a.flags);
         done = finish_pass(ctx, hv, a.flags);
     }
@@ -1718,7 +1736,7 @@ static void do_extract_noalloc(dmc_context* ctx, void* a_) {
     ctx->arena_cap = kScratchBytes;
     ctx->arena_used = 0;
     // exact sizes a larger earlier pass left would carve the scratch for that
-    // pass: this call plans from the density ratios, and learns its own
+    // pass: this call plans from the densities, and learns its own
     const int64_t mins[3] = {ctx->acap_min, ctx->vcap_min, ctx->qcap_min};
     ctx->acap_min = ctx->vcap_min = ctx->qcap_min = 0;
     auto restore = [&] {
