@@ -181,6 +181,12 @@ dmc_status dmc_extract_noalloc(dmc_context* ctx, const dmc_grid* grid, const voi
  * reference; the serving loop counts it in its memory budget. */
 dmc_status dmc_context_bytes(dmc_context* ctx, size_t* bytes);
 
+/* Free a context's buffer pools and captured passes (its last mesh goes with
+ * them); the next extraction allocates what it needs.  No counterpart in the
+ * reference; the serving loop calls it when the worker contexts' pools would
+ * keep it over its memory budget. */
+dmc_status dmc_context_trim(dmc_context* ctx);
+
 /* Copy the last mesh to host memory; any pointer may be NULL.
  * positions/normals: 3*nv doubles; quads 4*nq; tris 3*nt;
  * halfedges: 5 ints per halfedge {origin, face, next, twin, flag} as in
@@ -200,8 +206,10 @@ dmc_status dmc_batch_download(dmc_context* ctx, int32_t index, double* positions
  * waiting requests oldest first while their estimated working set fits
  * budget_bytes beside what the server holds -- result blocks (in use or
  * idle; idle ones are freed when that makes a request fit), the buffer
- * pools the worker contexts keep (dmc_context_bytes) and the estimates of
- * the passes running -- and at most max_batch volumes and max_batch_points
+ * pools the worker contexts keep (dmc_context_bytes; a worker frees its
+ * pools with dmc_context_trim before a pass while they keep the server over
+ * the budget) and the estimates of the passes running -- and at most
+ * max_batch volumes and max_batch_points
  * grid points per pass (0: 2^28, which keeps any pass inside the engine's
  * index limits); the first request of a pass is always admitted.  It runs
  * them as one dmc_extract_batch and completes them together; a pass that

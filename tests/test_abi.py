@@ -64,6 +64,7 @@ def test_status_strings_and_null_context(lib):
     assert lib.tmc_extract(None, None, None, 0, 0.0, 0, None) == 1
     assert lib.dmc_extract_noalloc(None, None, None, 0, 0.0, 0, None, None, None) == 1
     assert lib.dmc_context_bytes(None, None) == 1
+    assert lib.dmc_context_trim(None) == 1
 
 
 def test_no_device_is_reported(lib):

@@ -175,3 +175,79 @@ def test_server_bad_requests_fail_alone(eng):
     finally:
         srv.close()
         buf.free()
+
+
+def test_capacities_per_volume_and_trim():
+    """A context sizes a pass's buffers per volume: after the pool's densest
+    small volume (36^3, 0.65 vertices per point) the 512^3 volume takes no
+    more memory than in a fresh context.  dmc_context_trim frees every pool,
+    and the next pass allocates anew and is still exact."""
+    specs = _serve_pool()
+    known = KNOWN["serve_pool"]["volumes"]
+    dense, large, small = 13, 19, 0
+    vols = {i: workloads.make(specs[i]) for i in (dense, large, small)}
+    flags = SIMPLIFY_ALL | TRIANGLES
+
+    def run(e, i):
+        f, g, iso = vols[i]
+        m = e.extract(f, iso, g, flags=flags)
+        assert (m.nv, m.nq) == (known[i]["nv"], known[i]["nq"]), i
+        assert mesh_digest(e.download()) == known[i]["digest"], i
+
+    fresh = tmc_amd.Engine(0)
+    try:
+        run(fresh, large)
+        alone = fresh.context_bytes()
+    finally:
+        fresh.close()
+    e = tmc_amd.Engine(0)
+    try:
+        run(e, dense)
+        run(e, large)
+        after = e.context_bytes()
+        _say(f"[gpu] 512^3 context: {alone / 2**30:.2f} GiB fresh, {after / 2**30:.2f} GiB after the dense volume")
+        assert after <= alone * 5 // 4
+        e.trim()
+        assert e.context_bytes() == 0
+        with pytest.raises(tmc_amd.DMCError):
+            e.download()
+        run(e, small)
+        assert 0 < e.context_bytes() < after // 4
+        run(e, large)
+    finally:
+        e.close()
+
+
+def test_server_trims_worker_pools_over_budget(eng):
+    """With a budget just above a 512^3 request's working-set estimate, the
+    pools a 512^3 pass leaves keep the server over budget when a 256^3
+    request comes: the worker frees them first, so the server's pool
+    shrinks, and every result stays exact."""
+    specs = _serve_pool()
+    known = KNOWN["serve_pool"]["volumes"]
+    idx = (19, 14, 19)
+
+    def estimate(shape):
+        # the server's admission estimate: (row, segment) lists and counts,
+        # 40 bytes per grid point
+        nz, ny, nx = shape
+        return (ny - 1) * (nz - 1) * ((nx - 1 + 255) // 256) * (256 * 4 + 8) + 40 * nx * ny * nz
+
+    vols = [workloads.make(specs[i]) for i in idx]
+    buf, ptrs = _pack(eng, [f for f, _, _ in vols])
+    srv = tmc_amd.Server(0, flags=FLAGS, budget_bytes=estimate(specs[19]["shape"]) + (16 << 20), workers=1)
+    try:
+        pools = []
+        for k, (p, (f, g, iso)) in enumerate(zip(ptrs, vols)):
+            t = srv.submit(p, iso, g)
+            info = srv.wait(t)
+            i = idx[k]
+            assert (info["nv"], info["nq"]) == (known[i]["nv"], known[i]["nq"]), i
+            assert mesh_digest(srv.download(t)) == known[i]["digest"], i
+            srv.release(t)
+            pools.append(srv.stats()["pool_bytes"])
+        _say(f"[gpu] server pool after 512^3, 256^3, 512^3: {[round(x / 2**30, 2) for x in pools]} GiB")
+        assert pools[1] < pools[0] // 2
+    finally:
+        srv.close()
+        buf.free()

@@ -44,7 +44,7 @@ EXPORTS = ["dmc_create", "dmc_destroy", "dmc_extract", "dmc_extract_batch", "dmc
            "dmc_device_free", "dmc_copy_to_device", "dmc_copy_to_host", "dmc_synchronize", "dmc_status_str",
            "dmc_last_error", "dmc_mesh_edges", "dmc_valence_histogram", "dmc_element_quality", "dmc_write_obj",
            "tmc_extract", "tmc_download", "dmc_get_launch", "dmc_set_launch", "dmc_autotune",
-           "dmc_extract_noalloc", "dmc_context_bytes"]
+           "dmc_extract_noalloc", "dmc_context_bytes", "dmc_context_trim"]
 
 
 class DMCError(RuntimeError):
@@ -141,6 +141,7 @@ def load_library(path=ENGINE_SO):
     L.dmc_extract_noalloc.argtypes = [vp, C.POINTER(Grid), vp, C.c_int, C.c_double, C.c_uint32, vp,
                                       C.POINTER(OutBuffers), C.POINTER(Mesh)]
     L.dmc_context_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.dmc_context_trim.argtypes = [vp]
     L.dmc_batch_download.argtypes = [vp, C.c_int32, f64p, f64p, i32p, i32p, i32p, i32p, i32p, i32p]
     u64 = C.c_uint64
     L.dmc_server_create.argtypes = [C.POINTER(vp), C.POINTER(ServerOpts)]
@@ -182,7 +183,8 @@ def load_library(path=ENGINE_SO):
                  "dmc_server_stats", "dmc_mesh_download_f32",
                  "dmc_stage_times", "dmc_device_alloc", "dmc_device_free", "dmc_copy_to_device",
                  "dmc_copy_to_host", "dmc_synchronize", "dmc_mesh_edges", "dmc_valence_histogram",
-                 "dmc_element_quality", "dmc_write_obj", "dmc_extract_noalloc", "dmc_context_bytes"):
+                 "dmc_element_quality", "dmc_write_obj", "dmc_extract_noalloc", "dmc_context_bytes",
+                 "dmc_context_trim"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -461,6 +463,13 @@ class Engine:
         n = C.c_size_t()
         self._check(self.lib.dmc_context_bytes(self.ctx, C.byref(n)))
         return n.value
+
+    def trim(self):
+        """Free this context's buffer pools and captured passes (the last
+        mesh goes with them)."""
+        self._check(self.lib.dmc_context_trim(self.ctx))
+        self.mesh = None
+        self.batch = None
 
     def copy_to_host(self, dst, src_ptr):
         """Copy dst.nbytes from device address src_ptr into numpy array dst."""

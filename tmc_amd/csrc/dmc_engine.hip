@@ -141,8 +141,8 @@ struct dmc_context {
     // capacities of the pass's arrays (active cells, vertices,
     // quadrilaterals, 3-face records), planned from the largest densities of
     // active cells, vertices and quadrilaterals seen in one volume, per cell
-    // (rcell) and per cells^(2/3) (rarea), and raised to exact sizes after an
-    // overflow
+    // (rcell) and per cells^(2/3) (rarea), and in the rerun of a pass that
+    // overflowed, raised to its exact sizes (the *_min, one extraction long)
     int64_t acap = 0, vcap = 0, qcap = 0, reccap = 0;
     int64_t acap_min = 0, vcap_min = 0, qcap_min = 0;
     double rcell[3] = {0, 0, 0}, rarea[3] = {0, 0, 0};
@@ -1595,6 +1595,9 @@ void do_extract(dmc_context* ctx, void* a_) {
     // a pass that outgrew a capacity learned the exact sizes, and one that
     // needed generic halfedge rows asks for them: once more
     ctx->force_generic = ctx->always_generic;
+    // exact sizes learned from an overflow hold for that extraction's rerun
+    // only: later passes plan from the densities, which the overflow raised
+    ctx->acap_min = ctx->vcap_min = ctx->qcap_min = 0;
     bool done = false;
     for (int attempt = 0; attempt < 3 && !done; attempt++) {
         if (attempt > 0 && ctx->arena) {
@@ -1735,17 +1738,10 @@ static void do_extract_noalloc(dmc_context* ctx, void* a_) {
     ctx->arena = static_cast<char*>(base);
     ctx->arena_cap = kScratchBytes;
     ctx->arena_used = 0;
-    // exact sizes a larger earlier pass left would carve the scratch for that
-    // pass: this call plans from the densities, and learns its own
-    const int64_t mins[3] = {ctx->acap_min, ctx->vcap_min, ctx->qcap_min};
-    ctx->acap_min = ctx->vcap_min = ctx->qcap_min = 0;
     auto restore = [&] {
         // nothing of the call may still run on the scratch when it returns
         (void)hipStreamSynchronize(ctx->stream);
         for (size_t i = 0; i < saved.size(); i++) *ctx->bufs[i] = saved[i];
-        ctx->acap_min = mins[0];
-        ctx->vcap_min = mins[1];
-        ctx->qcap_min = mins[2];
         ctx->arena = nullptr;
         ctx->arena_cap = ctx->arena_used = 0;
         ctx->have_mesh = false;
@@ -2145,6 +2141,28 @@ dmc_status dmc_context_bytes(dmc_context* ctx, size_t* bytes) {
     size_t n = 0;
     for (const Buf* b : ctx->bufs) n += b->cap;
     *bytes = n;
+    return DMC_OK;
+}
+
+dmc_status dmc_context_trim(dmc_context* ctx) {
+    if (!ctx) return DMC_ERR_INVALID_ARG;
+    if (ctx->arena) return DMC_ERR_STATE;
+    (void)hipSetDevice(ctx->device);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
+    for (Buf* b : ctx->bufs) {
+        if (b->p) (void)hipFree(b->p);
+        *b = Buf{};
+    }
+    ctx->buf_epoch++;
+    ctx->have_mesh = false;
+    ctx->have_tmesh = false;
+    ctx->vmesh.clear();
+    ctx->vbnd.clear();
+    if (e != hipSuccess) {
+        ctx->err = hipGetErrorString(e);
+        return DMC_ERR_HIP;
+    }
     return DMC_OK;
 }
 

@@ -181,6 +181,19 @@ dmc_status run_pass(dmc_server* s, Worker* w, std::vector<Request*>& batch, bool
     std::vector<dmc_volume> vols(n);
     for (int i = 0; i < n; i++) vols[i] = batch[i]->vol;
     std::vector<dmc_mesh> out(n);
+    // pools that would keep the server over its budget (a context grown by
+    // an earlier, larger pass) are freed first: the pass then allocates only
+    // what it needs
+    bool trim = false;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        trim = w->pool_bytes > 0 && committed(s) > s->opts.budget_bytes;
+    }
+    if (trim && dmc_context_trim(w->ctx) == DMC_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->ctx_bytes -= w->pool_bytes;
+        w->pool_bytes = 0;
+    }
     const double t_start = s->now();
     dmc_status st = dmc_extract_batch(w->ctx, vols.data(), n, s->opts.field_type, s->opts.flags, out.data());
     int64_t V = 0, Q = 0;
