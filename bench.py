@@ -521,6 +521,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson mix) leg")
     ap.add_argument("--no-diagnostics", action="store_true", help="skip the mesh diagnostics timings")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer drop-in call timing")
+    ap.add_argument("--no-small", action="store_true", help="skip the 128^3 single-volume latency leg")
     ap.add_argument("--only", choices=["batch", "serve"], help="run only the batched or served leg")
     ap.add_argument("--tmc-only", action="store_true",
                     help="run only the topologically correct MC leg on the workload (for profiling)")
@@ -659,7 +660,7 @@ def main():
         "kernels": kernels,
     }
     out["diagnostics"] = None if args.no_diagnostics else bench_diagnostics(eng)
-    out["small_volume"] = bench_small()
+    out["small_volume"] = None if args.no_small else bench_small()
     if not args.no_dropin:
         out["dropin"] = bench_dropin(eng, field, grid, iso)
     if not args.no_tmc:
