@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -832,6 +833,56 @@ void build_halfedges(dmc_context* ctx, bool final_pass) {
 // Meshes of at most this many faces are coloured by one block.
 constexpr int64_t kSmallColourFaces = (int64_t)1 << 14;
 
+// Colour classes that run in one launch.  Class c holds the faces of initial
+// colour c = 3 * parity + direction of their grid edge, and a face reads the
+// colours of its edge neighbours.  Two quadrilaterals share a mesh edge only
+// when their grid edges lie on one cell face: the face's other edge of the
+// same direction (lower corners differing in the bit of one of the two other
+// axes), or an edge of another direction b whose lower corner differs by at
+// most the bits of the two axes of the face they span.  A class sees the
+// colours it would see in the sequential order (5..23, then 4) when it runs
+// after every earlier class it touches and before every later one: it runs
+// one level past the highest level of the earlier classes it touches, and
+// the classes of a level share a launch.
+bool classes_touch(int c1, int c2) {
+    const int a = c1 % 3, b = c2 % 3, p = c1 / 3, q = c2 / 3;
+    const int ea = 1 << a, eb = 1 << b;
+    if (a == b) {
+        const int o1 = 1 << ((a + 1) % 3), o2 = 1 << ((a + 2) % 3);
+        return q == (p ^ o1) || q == (p ^ o2);
+    }
+    const int d = p ^ q;
+    return (d & ~(ea | eb)) == 0;
+}
+
+struct ColourLevels {
+    int n = 0;
+    int cls[20][2];
+};
+
+const ColourLevels& colour_levels() {
+    static const ColourLevels L = [] {
+        int order[20], level[24] = {0};
+        for (int i = 0; i < 19; i++) order[i] = 5 + i;
+        order[19] = 4;
+        ColourLevels r;
+        for (auto& x : r.cls) x[0] = x[1] = -1;
+        for (int i = 0; i < 20; i++) {
+            int lv = 0;
+            for (int j = 0; j < i; j++)
+                if (classes_touch(order[i], order[j])) lv = std::max(lv, level[order[j]] + 1);
+            level[order[i]] = lv;
+            r.n = std::max(r.n, lv + 1);
+            int* slot = r.cls[lv];
+            if (slot[0] < 0) slot[0] = order[i];
+            else if (slot[1] < 0) slot[1] = order[i];
+            else throw std::logic_error("three colour classes in one level");
+        }
+        return r;
+    }();
+    return L;
+}
+
 void color_faces(dmc_context* ctx) {
     const int64_t qcap = ctx->qcap;
     Sizes* S = dsizes(ctx);
@@ -851,13 +902,12 @@ void color_faces(dmc_context* ctx) {
         return;
     }
     // a class holds at most a quarter of the faces (four parities of one
-    // edge direction)
-    const int g = grid_for(ctx, qcap / 4 + 1);
-    for (int c = 5; c < 24; c++)
-        launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), list, (const uint32_t*)S->colors, c, twin,
-               colors, 5);
-    launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), list, (const uint32_t*)S->colors, 4, twin,
-           colors, 4);
+    // edge direction), a launch at most two classes
+    const int g = grid_for(ctx, qcap / 2 + 1);
+    const ColourLevels& L = colour_levels();
+    for (int lv = 0; lv < L.n; lv++)
+        launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), list, (const uint32_t*)S->colors,
+               L.cls[lv][0], L.cls[lv][1], twin, colors);
     check_launch(ctx, "colour");
 }
 

@@ -400,10 +400,12 @@ __global__ void __launch_bounds__(kBlock) k_rows(VolTable vt, int64_t Rtot, cons
                                                   uint32_t acap) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wpb = kBlock / kWave;
-    // one volume: its contour count is spread over 16 counters (summed by
-    // the host), so that the waves' adds do not queue on one address
+    // one volume: its contour count is spread over kContourParts counters
+    // (summed by the host), so that the waves' adds do not queue on a few
+    // addresses
     auto contours = [&](int vol) {
-        return vt.n == 1 ? &cpart[(blockIdx.x * wpb + threadIdx.x / kWave) & 15] : &vstat[vol].contours;
+        return vt.n == 1 ? &cpart[(blockIdx.x * wpb + threadIdx.x / kWave) & (kContourParts - 1)]
+                         : &vstat[vol].contours;
     };
     unsigned long long total = 0;
     int cur = -1;  // volume whose contour count `total` holds (wave-uniform)

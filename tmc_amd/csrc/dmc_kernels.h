@@ -57,6 +57,11 @@ struct HeStats {
     unsigned int pad;
 };
 
+// A single volume's contour count is spread over this many counters (one
+// per wave slot, summed by the host), so that the waves' adds at the end of
+// their rows do not queue on a few addresses.
+constexpr int kContourParts = 256;
+
 // The sizes of one extraction pass, kept on the device: every launch reads
 // its item count from here, so a pass needs no count read back between its
 // stages.  The host sizes launches and buffers by capacities and reads this
@@ -85,7 +90,7 @@ struct Sizes {
     uint32_t nrec_prev;          // ... of the build before (carried over by a compaction)
     uint32_t recmax;             // most 3-face records any build of the pass wrote (capacity planning)
     uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
-    unsigned long long cpart[16];   // a single volume's contour count, in 16 parts
+    unsigned long long cpart[kContourParts];   // a single volume's contour count, in parts
 };
 
 // A launch whose work is conditional on device data: it runs only when

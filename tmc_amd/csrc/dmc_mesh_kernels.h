@@ -478,14 +478,18 @@ __global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restr
     }
 }
 
-// Class `cls`: its count and start in totals (k_color_offsets).
+// Classes ca and cb (cb < 0: none) in one launch, two classes with no edge
+// neighbours in common: their counts and starts in totals
+// (k_color_offsets).  Class 4 takes a colour in 0..3, the others in 0..4.
 __global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ totals, int cls,
-                                                         const int32_t* __restrict__ twin, int32_t* __restrict__ colors,
-                                                         int ncol) {
-    const int64_t n = totals[cls], start = totals[24 + cls];
-    TMC_GRID_STRIDE(x, n) {
-        const int32_t f = list[start + x];
+                                                         const uint32_t* __restrict__ totals, int ca, int cb,
+                                                         const int32_t* __restrict__ twin,
+                                                         int32_t* __restrict__ colors) {
+    const int64_t na = totals[ca], nb2 = cb >= 0 ? totals[cb] : 0;
+    TMC_GRID_STRIDE(x, na + nb2) {
+        const bool first = x < na;
+        const int ncol = (first ? ca : cb) == 4 ? 4 : 5;
+        const int32_t f = list[first ? (int64_t)totals[24 + ca] + x : (int64_t)totals[24 + cb] + (x - na)];
         int32_t nb[4];
         face_neighbors(twin, f, nb);
         int32_t c[4];
