@@ -101,7 +101,7 @@ struct dmc_context {
     std::vector<Buf*> bufs;
     // cell sweep
     Buf segCount, segBase, segMask, lists, acell, acode, aaux, avloc, aqloc, rowV, rowQ, rowBaseV, rowBaseQ;
-    Buf field, grads, scanPartial, scanTotal, sizes;
+    Buf field, grads, scanPartial, scanState, scanTotal, sizes;
     // volume table of the last extraction (one volume, or a batch) and the
     // per-volume first active cell / vertex / quadrilateral, kept through
     // compactions
@@ -191,7 +191,7 @@ struct dmc_context {
 
     dmc_context() {
         Buf* all[] = {&segCount, &segBase, &segMask, &lists, &acell, &acode, &aaux, &avloc, &aqloc, &rowV, &rowQ, &rowBaseV, &rowBaseQ,
-                      &field, &grads, &scanPartial, &scanTotal, &sizes, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
+                      &field, &grads, &scanPartial, &scanState, &scanTotal, &sizes, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
                       &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
@@ -451,14 +451,26 @@ struct Fills {
     }
 };
 
-// Exclusive scan of n.get() uint32 items into out (launched for n.cap); the
-// 64-bit total goes to *dtotal (device memory).  Nothing runs when *gate is
-// 0.
 // Inputs of at most this many items are scanned by one block in one launch.
 constexpr int64_t kScanSmallItems = (int64_t)1 << 14;
 
-// Exclusive scan of n items into out, the total to *dtotal; row_end: also
-// write the total's low 32 bits to row_end[n] (the end of a CSR's last row).
+// The state of the single-pass scans: zero when a scan starts, and left zero
+// by every scan, so it is cleared only when its buffer is (re)allocated or
+// carved from the no-alloc scratch.
+unsigned long long* scan_state(dmc_context* ctx, int64_t tiles) {
+    Buf& b = ctx->scanState;
+    const void* p0 = b.p;
+    const size_t c0 = b.cap;
+    unsigned long long* st = ensure<unsigned long long>(ctx, b, kScanStateHead + (size_t)tiles);
+    if (b.p != p0 || b.cap != c0) fill_u32(ctx, reinterpret_cast<uint32_t*>(st), hn((int64_t)(b.cap / 4)), 0u);
+    return st;
+}
+
+// Exclusive scan of n.get() uint32 items into out (launched for n.cap); the
+// 64-bit total goes to *dtotal (device memory); row_end: also the total's low
+// 32 bits to row_end[n] (the end of a CSR's last row).  Nothing runs when
+// *gate is 0.  One block for a few tiles, one launch of look-back tiles for
+// up to kScanOnePassTiles, three launches beyond.
 template <class In>
 void scan_to(dmc_context* ctx, In in, uint32_t* out, DevN n, uint64_t* dtotal, const char* name = "k_scan",
              const uint32_t* gate = nullptr, uint32_t* row_end = nullptr) {
@@ -468,10 +480,15 @@ void scan_to(dmc_context* ctx, In in, uint32_t* out, DevN n, uint64_t* dtotal, c
         return;
     }
     const int64_t tiles = (n.cap + kScanTile - 1) / kScanTile;
-    uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
-    launch(ctx, name, k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), in, n, gate, part);
-    launch(ctx, name, k_scan_partials, dim3(1), dim3(1024), part, tiles, gate, dtotal, row_end, n);
-    launch(ctx, name, k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, part);
+    if (tiles <= kScanOnePassTiles) {
+        launch(ctx, name, k_scan<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, dtotal, row_end,
+               scan_state(ctx, tiles), tiles, &ensure<Sizes>(ctx, ctx->sizes, 1)->stall);
+    } else {
+        uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
+        launch(ctx, name, k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), in, n, gate, part);
+        launch(ctx, name, k_scan_partials, dim3(1), dim3(1024), part, tiles, gate, dtotal, row_end, n);
+        launch(ctx, name, k_scan_down<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, part);
+    }
     check_launch(ctx, "scan");
 }
 
@@ -1330,6 +1347,7 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
         r.wait();
     }
     finish_timing(ctx);
+    if (hs.stall) fail(ctx, DMC_ERR_STATE, "%u scan look-backs ran out of polls", hs.stall);
     if (hs.A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)hs.A);
     if (hs.V > 0x7FFFFFFFull || hs.Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)hs.V,

@@ -89,6 +89,8 @@ struct Sizes {
     uint32_t nrec;               // 3-face edge records of the current build
     uint32_t nrec_prev;          // ... of the build before (carried over by a compaction)
     uint32_t recmax;             // most 3-face records any build of the pass wrote (capacity planning)
+    uint32_t stall;              // scan look-backs that ran out of polls (never expected: the pass fails)
+    uint32_t pad2;
     uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
     unsigned long long cpart[kContourParts];   // a single volume's contour count, in parts
 };
@@ -139,8 +141,8 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T* smem, T& total) {
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan of uint32 counts, or of keep flags read from
-// byte flags (three passes over tiles of kScanTile items).  Totals are accumulated in 64 bits so that overflow of
-// the 32-bit offsets can be detected by the host.
+// byte flags, over tiles of kScanTile items.  Totals are accumulated in 64
+// bits so that overflow of the 32-bit offsets can be detected by the host.
 // ---------------------------------------------------------------------------
 constexpr int kScanIPT = 16;
 constexpr int kScanTile = kBlock * kScanIPT;
@@ -203,8 +205,9 @@ __device__ __forceinline__ void scan_items(const In& in, int64_t n, int64_t base
     }
 }
 
-// Device-wide exclusive scan over n = dn.get() items, launched on
-// ceil(dn.cap / kScanTile) tiles; skipped entirely when gated off.
+// Device-wide exclusive scan over n = dn.get() items in three launches over
+// ceil(dn.cap / kScanTile) tiles (tile sums, their scan, the tiles again);
+// skipped entirely when gated off.  For inputs of many tiles.
 template <class In>
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, DevN dn, const uint32_t* __restrict__ gate,
                                                        uint64_t* __restrict__ partial) {
@@ -284,6 +287,125 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
 
 // The whole scan in one block, for short inputs (one launch instead of
 // three): tiles of 1024 x kScanIPT items in turn, carrying the running sum.
+// The same in one launch of ceil(dn.cap / kScanTile) blocks (decoupled
+// look-back), for inputs of up to kScanOnePassTiles tiles, where a look-back
+// never spans more than one wave's window; skipped entirely when gated off.  Each block takes the next tile from a ticket counter, so a
+// tile only waits on tiles whose blocks already run.  It publishes its sum
+// (kScanAgg), then wave 0 looks back over the predecessors 64 at a time,
+// adding sums until it meets a tile that holds its inclusive prefix
+// (kScanInc), and publishes its own.  The last block to finish its look-back
+// clears the tickets and flags, so the state is zero again for the next
+// scan.  Waits are bounded: a tile that polls kScanSpin times in vain counts
+// a stall in *stall (the host then fails the pass) and goes on, so a lost
+// flag cannot hang the device.
+// state st: [0] tickets, [1] look-backs done, [2..4) unused, [4, 4 + ntiles) tile flags
+constexpr uint64_t kScanAgg = 1ull << 62, kScanInc = 2ull << 62, kScanVal = (1ull << 62) - 1;
+constexpr int kScanSpin = 1 << 22;
+constexpr int kScanStateHead = 4;
+constexpr int64_t kScanOnePassTiles = 64;
+
+__device__ __forceinline__ uint64_t scan_flag_load(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void scan_flag_store(unsigned long long* p, uint64_t v) {
+    __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ out, DevN dn,
+                                                  const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                  uint32_t* __restrict__ row_end, unsigned long long* __restrict__ st,
+                                                  int64_t ntiles, uint32_t* __restrict__ stall) {
+    __shared__ uint64_t sm[kBlock / kWave + 1];
+    __shared__ int64_t s_tile;
+    __shared__ uint64_t s_excl;
+    __shared__ int s_last;
+    if (gated_off(gate)) return;
+    const int64_t n = dn.get();
+    if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&st[0], 1ull);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    const bool full = base + kScanIPT <= n;
+    uint32_t v[kScanIPT];
+    scan_items(in, n, base, v);
+    uint64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanIPT; q++) s += v[q];
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan<uint64_t, kBlock>(s, sm, tot);
+    unsigned long long* flags = st + kScanStateHead;
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        if (lane == 0 && tile > 0) scan_flag_store(&flags[tile], kScanAgg | (tot & kScanVal));
+        uint64_t excl = 0;
+        int64_t p = tile - 1;
+        int spins = 0;
+        while (p >= 0) {
+            const int64_t q = p - lane;
+            // before the first tile: an inclusive prefix of 0
+            const uint64_t w = q >= 0 ? scan_flag_load(&flags[q]) : kScanInc;
+            const uint64_t inc = __ballot((w >> 62) == 2u);
+            const uint64_t notready = __ballot((w >> 62) == 0u);
+            const int fi = inc ? __ffsll((unsigned long long)inc) - 1 : kWave;   // nearest inclusive lane
+            const uint64_t need = fi >= kWave - 1 ? ~0ull : ((2ull << fi) - 1ull);
+            if (notready & need) {
+                if (++spins < kScanSpin) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                if (lane == 0) atomicAdd(stall, 1u);
+                break;
+            }
+            excl += wave_sum(lane <= fi ? (w & kScanVal) : 0ull);
+            if (fi < kWave) break;
+            p -= kWave;
+        }
+        if (lane == 0) {
+            scan_flag_store(&flags[tile], kScanInc | ((excl + tot) & kScanVal));
+            s_excl = excl;
+            if (tile == ntiles - 1) {
+                *total = excl + tot;
+                if (row_end) row_end[n] = (uint32_t)(excl + tot);
+            }
+            // the flag store completes before the count (the clearing block
+            // counts last, so no flag lands after it clears); a plain wait,
+            // not a release, which would write back and invalidate the L2
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const unsigned long long d = atomicAdd(&st[1], 1ull);
+            s_last = d == (unsigned long long)(ntiles - 1) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    uint32_t run = (uint32_t)(s_excl + ex);
+    uint32_t o[kScanIPT];
+#pragma unroll
+    for (int q = 0; q < kScanIPT; q++) {
+        o[q] = (In::kMap && v[q] == 0u) ? ~(uint32_t)(base + q - run) : run;
+        run += v[q];
+    }
+    if (full) {
+#pragma unroll
+        for (int q = 0; q < kScanIPT / 4; q++)
+            reinterpret_cast<uint4*>(out + base)[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kScanIPT; q++)
+            if (base + q < n) out[base + q] = o[q];
+    }
+    // every tile has finished its look-back: clear the state for the next scan
+    if (s_last) {
+        for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) scan_flag_store(&flags[i], 0ull);
+        if (threadIdx.x == 0) {
+            scan_flag_store(&st[0], 0ull);
+            scan_flag_store(&st[1], 0ull);
+        }
+    }
+}
+
+// Single block (1024 threads) for up to a few tiles: no state.
 template <class In>
 __global__ void __launch_bounds__(1024) k_scan_small(In in, uint32_t* __restrict__ out, DevN dn,
                                                       const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
