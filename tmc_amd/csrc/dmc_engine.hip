@@ -121,6 +121,7 @@ struct dmc_context {
     bool force_generic = false;   // this extraction's passes pair every build through generic rows
     bool commit_pending = false;  // a compaction's counts wait for the next build's gate to commit them
     bool always_generic = false;  // every extraction pairs through generic rows (TMC_GENERIC_ROWS=1)
+    int scan_spin = kScanSpin;    // polls a scan look-back waits before it sums the input (TMC_SCAN_SPIN)
     // halfedges / simplification scratch.  twin, flag, deg (out-degree), hev
     // and bndrec are the current build's; the ...2 buffers receive the build
     // a compaction carries over, and the pairs swap.
@@ -481,8 +482,11 @@ void scan_to(dmc_context* ctx, In in, uint32_t* out, DevN n, uint64_t* dtotal, c
     }
     const int64_t tiles = (n.cap + kScanTile - 1) / kScanTile;
     if (tiles <= kScanOnePassTiles) {
+        // a stalled look-back re-reads the input
+        if (static_cast<const void*>(in.p) == static_cast<const void*>(out))
+            fail(ctx, DMC_ERR_STATE, "%s: a scan in place", name);
         launch(ctx, name, k_scan<In>, dim3((unsigned)tiles), dim3(kBlock), in, out, n, gate, dtotal, row_end,
-               scan_state(ctx, tiles), tiles, &ensure<Sizes>(ctx, ctx->sizes, 1)->stall);
+               scan_state(ctx, tiles), tiles, ctx->scan_spin);
     } else {
         uint64_t* part = ensure<uint64_t>(ctx, ctx->scanPartial, tiles);
         launch(ctx, name, k_scan_reduce<In>, dim3((unsigned)tiles), dim3(kBlock), in, n, gate, part);
@@ -1347,7 +1351,6 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
         r.wait();
     }
     finish_timing(ctx);
-    if (hs.stall) fail(ctx, DMC_ERR_STATE, "%u scan look-backs ran out of polls", hs.stall);
     if (hs.A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)hs.A);
     if (hs.V > 0x7FFFFFFFull || hs.Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)hs.V,
@@ -1723,6 +1726,10 @@ dmc_status dmc_create(dmc_context** out, int device) {
     // every halfedge build through generic rows (the path a pass takes when
     // it is run again for them), for checking that path against the others
     if (const char* g = getenv("TMC_GENERIC_ROWS")) ctx->always_generic = strtol(g, nullptr, 10) != 0;
+    if (const char* sp = getenv("TMC_SCAN_SPIN")) {
+        const long v = strtol(sp, nullptr, 10);
+        if (v >= 1 && v <= kScanSpin) ctx->scan_spin = (int)v;
+    }
     if (const char* w = getenv("TMC_SWEEP_WAVES")) {
         const long v = strtol(w, nullptr, 10);
         if (v >= kSweepWavesMin && v <= kSweepWavesMax) ctx->sweep_waves = (int)v;

@@ -89,8 +89,6 @@ struct Sizes {
     uint32_t nrec;               // 3-face edge records of the current build
     uint32_t nrec_prev;          // ... of the build before (carried over by a compaction)
     uint32_t recmax;             // most 3-face records any build of the pass wrote (capacity planning)
-    uint32_t stall;              // scan look-backs that ran out of polls (never expected: the pass fails)
-    uint32_t pad2;
     uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
     unsigned long long cpart[kContourParts];   // a single volume's contour count, in parts
 };
@@ -285,19 +283,20 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
     }
 }
 
-// The whole scan in one block, for short inputs (one launch instead of
-// three): tiles of 1024 x kScanIPT items in turn, carrying the running sum.
 // The same in one launch of ceil(dn.cap / kScanTile) blocks (decoupled
-// look-back), for inputs of up to kScanOnePassTiles tiles, where a look-back
-// never spans more than one wave's window; skipped entirely when gated off.  Each block takes the next tile from a ticket counter, so a
-// tile only waits on tiles whose blocks already run.  It publishes its sum
-// (kScanAgg), then wave 0 looks back over the predecessors 64 at a time,
-// adding sums until it meets a tile that holds its inclusive prefix
-// (kScanInc), and publishes its own.  The last block to finish its look-back
-// clears the tickets and flags, so the state is zero again for the next
-// scan.  Waits are bounded: a tile that polls kScanSpin times in vain counts
-// a stall in *stall (the host then fails the pass) and goes on, so a lost
-// flag cannot hang the device.
+// look-back), for inputs of up to kScanOnePassTiles tiles, so that a
+// look-back never spans more than one wave's window; skipped entirely when
+// gated off.  Each block takes the next tile from a ticket counter, so a tile
+// only waits on tiles whose blocks already run.  It publishes its sum
+// (kScanAgg); wave 0 looks back over the predecessors, adding sums until it
+// meets a tile that holds its inclusive prefix (kScanInc); the tile then
+// publishes its own.  Waits are bounded: a tile whose predecessors are not
+// all published after `spin` polls (at most kScanSpin) sums their items
+// straight from the input, which the scan does not write (the host checks
+// that it is not the output), so its prefix is exact
+// either way, and a late flag can neither hang the device nor corrupt the
+// result.  The last block to finish its look-back clears the tickets and
+// flags, so the state is zero again for the next scan.
 // state st: [0] tickets, [1] look-backs done, [2..4) unused, [4, 4 + ntiles) tile flags
 constexpr uint64_t kScanAgg = 1ull << 62, kScanInc = 2ull << 62, kScanVal = (1ull << 62) - 1;
 constexpr int kScanSpin = 1 << 22;
@@ -315,11 +314,11 @@ template <class In>
 __global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ out, DevN dn,
                                                   const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
                                                   uint32_t* __restrict__ row_end, unsigned long long* __restrict__ st,
-                                                  int64_t ntiles, uint32_t* __restrict__ stall) {
+                                                  int64_t ntiles, int spin) {
     __shared__ uint64_t sm[kBlock / kWave + 1];
     __shared__ int64_t s_tile;
     __shared__ uint64_t s_excl;
-    __shared__ int s_last;
+    __shared__ int s_stalled, s_last;
     if (gated_off(gate)) return;
     const int64_t n = dn.get();
     if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&st[0], 1ull);
@@ -341,6 +340,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ o
         uint64_t excl = 0;
         int64_t p = tile - 1;
         int spins = 0;
+        bool stalled = false;
         while (p >= 0) {
             const int64_t q = p - lane;
             // before the first tile: an inclusive prefix of 0
@@ -350,11 +350,11 @@ __global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ o
             const int fi = inc ? __ffsll((unsigned long long)inc) - 1 : kWave;   // nearest inclusive lane
             const uint64_t need = fi >= kWave - 1 ? ~0ull : ((2ull << fi) - 1ull);
             if (notready & need) {
-                if (++spins < kScanSpin) {
+                if (++spins < spin) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                if (lane == 0) atomicAdd(stall, 1u);
+                stalled = true;
                 break;
             }
             excl += wave_sum(lane <= fi ? (w & kScanVal) : 0ull);
@@ -362,21 +362,36 @@ __global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ o
             p -= kWave;
         }
         if (lane == 0) {
-            scan_flag_store(&flags[tile], kScanInc | ((excl + tot) & kScanVal));
             s_excl = excl;
-            if (tile == ntiles - 1) {
-                *total = excl + tot;
-                if (row_end) row_end[n] = (uint32_t)(excl + tot);
-            }
-            // the flag store completes before the count (the clearing block
-            // counts last, so no flag lands after it clears); a plain wait,
-            // not a release, which would write back and invalidate the L2
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            __builtin_amdgcn_s_waitcnt(0);
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            const unsigned long long d = atomicAdd(&st[1], 1ull);
-            s_last = d == (unsigned long long)(ntiles - 1) ? 1 : 0;
+            s_stalled = stalled ? 1 : 0;
         }
+    }
+    __syncthreads();
+    if (s_stalled) {
+        // the predecessors' items straight from the input
+        const int64_t end = tile * kScanTile < n ? tile * kScanTile : n;
+        uint64_t acc = 0;
+        for (int64_t i = threadIdx.x; i < end; i += kBlock) acc += in.at(i);
+        uint64_t all;
+        (void)block_exclusive_scan<uint64_t, kBlock>(acc, sm, all);
+        if (threadIdx.x == 0) s_excl = all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t excl = s_excl;
+        scan_flag_store(&flags[tile], kScanInc | ((excl + tot) & kScanVal));
+        if (tile == ntiles - 1) {
+            *total = excl + tot;
+            if (row_end) row_end[n] = (uint32_t)(excl + tot);
+        }
+        // the flag store completes before the count (the clearing block
+        // counts last, so no flag lands after it clears); a plain wait, not
+        // a release, which would write back and invalidate the L2
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const unsigned long long d = atomicAdd(&st[1], 1ull);
+        s_last = d == (unsigned long long)(ntiles - 1) ? 1 : 0;
     }
     __syncthreads();
     uint32_t run = (uint32_t)(s_excl + ex);
@@ -405,7 +420,8 @@ __global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ o
     }
 }
 
-// Single block (1024 threads) for up to a few tiles: no state.
+// The whole scan in one block, for short inputs (one launch, no state):
+// tiles of 1024 x kScanIPT items in turn, carrying the running sum.
 template <class In>
 __global__ void __launch_bounds__(1024) k_scan_small(In in, uint32_t* __restrict__ out, DevN dn,
                                                       const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
