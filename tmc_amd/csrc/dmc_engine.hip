@@ -126,7 +126,7 @@ struct dmc_context {
     // and bndrec are the current build's; the ...2 buffers receive the build
     // a compaction carries over, and the pairs swap.
     Buf deg, off, cursor, adj, twin, flag, hev, ovr, bndrec, nbnd, val, p, em, vfirst, vsecond, lastT, lastR,
-        newV, newQ, clist, ccnt, tris, stats, off2, adj2, vmap, mlist, slow, twin2, flag2, deg2, hev2, bndrec2, arr,
+        newV, newQ, clist, cnbr, ccnt, tris, stats, off2, adj2, vmap, mlist, slow, twin2, flag2, deg2, hev2, bndrec2, arr,
         cflag, aflag, bcount, boff, alist;
     // diagnostics of the last mesh
     Buf drep, dnew, dval, dhist, dedges, dqual;
@@ -195,7 +195,7 @@ struct dmc_context {
                       &field, &grads, &scanPartial, &scanState, &scanTotal, &sizes, &pos[0], &pos[1], &nrm[0], &nrm[1], &vslot[0], &vslot[1],
                       &quads[0], &quads[1], &colors, &deg, &off, &cursor, &adj, &twin, &flag, &hev, &ovr,
                       &bndrec, &nbnd, &val, &p, &em, &vfirst, &vsecond, &lastT, &lastR, &newV,
-                      &newQ, &clist, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
+                      &newQ, &clist, &cnbr, &ccnt, &tris, &stats, &vtab, &vstat, &vbound, &qbound, &vhe,
                       &off2, &adj2, &vmap, &mlist, &slow, &twin2, &flag2, &deg2, &hev2, &bndrec2, &arr, &cflag,
                       &aflag, &bcount, &boff, &alist, &drep, &dnew, &dval,
                       &dhist, &dedges, &dqual, &tcntv, &tcntt, &tvofs, &ttofs, &tamb, &tvb, &tstats, &tpos, &tnrm,
@@ -735,7 +735,7 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     double* pos = ensure<double>(ctx, ctx->pos[0], 3 * vcap);
     double* nrm = ensure<double>(ctx, ctx->nrm[0], 3 * vcap);
     int32_t* quads = ensure<int32_t>(ctx, ctx->quads[0], 4 * qcap);
-    int32_t* colors = ensure<int32_t>(ctx, ctx->colors, qcap);
+    uint8_t* colors = ensure<uint8_t>(ctx, ctx->colors, qcap);
     // first halfedge CSR straight from the cells: degree of each vertex from
     // k_vertices, rows filled by k_quads
     uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, vcap);
@@ -907,18 +907,20 @@ const ColourLevels& colour_levels() {
 void color_faces(dmc_context* ctx) {
     const int64_t qcap = ctx->qcap;
     Sizes* S = dsizes(ctx);
-    int32_t* colors = static_cast<int32_t*>(ctx->colors.p);
+    uint8_t* colors = static_cast<uint8_t*>(ctx->colors.p);
     // one histogram block per 4096 faces of capacity, at most kColorBlocks
     const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kColorBlocks, qcap / 4096));
     uint32_t* hist = ensure<uint32_t>(ctx, ctx->ccnt, (size_t)kColorBlocks * 24);
     int32_t* list = ensure<int32_t>(ctx, ctx->clist, qcap);
-    launch(ctx, "k_color_hist", k_color_hist, dim3(nb), dim3(kBlock), colors, (const Sizes*)S, hist);
-    launch(ctx, "k_color_offsets", k_color_offsets, dim3(1), dim3(1024), hist, nb, S->colors);
-    launch(ctx, "k_color_scatter", k_color_scatter, dim3(nb), dim3(kBlock), colors, (const Sizes*)S, hist, list);
+    int4* nbr = ensure<int4>(ctx, ctx->cnbr, qcap);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
+    launch(ctx, "k_color_hist", k_color_hist, dim3(nb), dim3(kBlock), (const uint8_t*)colors, (const Sizes*)S, hist);
+    launch(ctx, "k_color_offsets", k_color_offsets, dim3(1), dim3(1024), hist, nb, S->colors);
+    launch(ctx, "k_color_scatter", k_color_scatter, dim3(nb), dim3(kBlock), (const uint8_t*)colors, (const Sizes*)S,
+           (const uint32_t*)hist, twin, list, nbr);
     if (qcap <= kSmallColourFaces) {
-        launch(ctx, "k_color_class", k_color_all, dim3(1), dim3(1024), (const int32_t*)list, (const uint32_t*)S->colors,
-               twin, colors);
+        launch(ctx, "k_color_class", k_color_all, dim3(1), dim3(1024), (const int32_t*)list, (const int4*)nbr,
+               (const uint32_t*)S->colors, colors);
         check_launch(ctx, "colour");
         return;
     }
@@ -927,8 +929,8 @@ void color_faces(dmc_context* ctx) {
     const int g = grid_for(ctx, qcap / 2 + 1);
     const ColourLevels& L = colour_levels();
     for (int lv = 0; lv < L.n; lv++)
-        launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), list, (const uint32_t*)S->colors,
-               L.cls[lv][0], L.cls[lv][1], twin, colors);
+        launch(ctx, "k_color_class", k_color_class, dim3(g), dim3(kBlock), (const int32_t*)list, (const int4*)nbr,
+               (const uint32_t*)S->colors, L.cls[lv][0], L.cls[lv][1], colors);
     check_launch(ctx, "colour");
 }
 
@@ -1067,7 +1069,7 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
         .u32(lastR, nv, 0xFFFFFFFFu).run();
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
-    const int32_t* colors = colored ? static_cast<int32_t*>(ctx->colors.p) : nullptr;
+    const uint8_t* colors = colored ? static_cast<uint8_t*>(ctx->colors.p) : nullptr;
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
     const int gq = grid_for(ctx, qcap);
     launch(ctx, "k_mark3x3y", k_mark3x3y, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, val, p);

@@ -824,7 +824,7 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
                                                    const uint32_t* __restrict__ rowBaseQ, DevN dA,
                                                    const unsigned long long* __restrict__ segMask,
                                                    const uint32_t* __restrict__ segBase,
-                                                   int32_t* __restrict__ quads, int32_t* __restrict__ colors,
+                                                   int32_t* __restrict__ quads, uint8_t* __restrict__ colors,
                                                    const uint32_t* __restrict__ off, int2* __restrict__ adj,
                                                    uint64_t adjcap, uint32_t* unresolved) {
     const int64_t A = dA.get();
@@ -870,7 +870,7 @@ __global__ void __launch_bounds__(kBlock) k_quads(VolTable vt, const uint32_t* _
                 }
             }
             reinterpret_cast<int4*>(quads)[q] = make_int4(out[0], out[1], out[2], out[3]);
-            colors[q] = 3 * parity + c;
+            colors[q] = (uint8_t)(3 * parity + c);
             q++;
         };
         // x edge: cells (i,j,k) e0, (i,j-1,k) e2, (i,j,k-1) e4, (i,j-1,k-1) e6

@@ -380,35 +380,50 @@ __global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__
     if ((threadIdx.x & (kWave - 1)) == 0 && z) atomicAdd(zero, z);
 }
 
-__device__ __forceinline__ void face_neighbors(const int32_t* __restrict__ twin, int64_t f, int32_t nb[4]) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int32_t t = twin[4 * f + k];
-        nb[k] = t >= 0 ? (t >> 2) : -1;
+// The four edge neighbours of face f (-1 where an edge has no twin).
+__device__ __forceinline__ int4 face_neighbors(const int32_t* __restrict__ twin, int64_t f) {
+    const int4 t = reinterpret_cast<const int4*>(twin)[f];
+    return make_int4(t.x >= 0 ? t.x >> 2 : -1, t.y >= 0 ? t.y >> 2 : -1, t.z >= 0 ? t.z >> 2 : -1,
+                     t.w >= 0 ? t.w >> 2 : -1);
+}
+
+// First colour in 0..ncol-1 that none of the neighbours nb has.
+__device__ __forceinline__ void take_free_colour(const int4 nb, int ncol, int32_t f, uint8_t* __restrict__ colors) {
+    const int c0 = nb.x >= 0 ? colors[nb.x] : -1, c1 = nb.y >= 0 ? colors[nb.y] : -1;
+    const int c2 = nb.z >= 0 ? colors[nb.z] : -1, c3 = nb.w >= 0 ? colors[nb.w] : -1;
+    for (int col = 0; col < ncol; col++) {
+        if (c0 != col && c1 != col && c2 != col && c3 != col) {
+            colors[f] = (uint8_t)col;
+            break;
+        }
     }
 }
 
-// Face colouring (cpp_dmc/DualMarchingCubes.cpp:714-844).  Faces are
-// bucketed by initial colour; classes 5..23 are processed in order, each
-// face taking the first colour in 0..4 unused by its edge neighbours, then
-// the initially colour-4 faces try 0..3.  Faces of one initial colour are
-// never edge neighbours (their grid edges are parallel edges of different
-// parity), so each class is processed in parallel.
+// Face colouring (cpp_dmc/DualMarchingCubes.cpp:714-844).  Colours are
+// bytes (initial colours 0..23, final ones 0..4).  Faces are bucketed by
+// initial colour; classes 5..23 are processed in order, each face taking the
+// first colour in 0..4 unused by its edge neighbours, then the initially
+// colour-4 faces try 0..3.  Faces of one initial colour are never edge
+// neighbours (their grid edges are parallel edges of different parity), so
+// each class is processed in parallel.
 // Bucketing by initial colour without hot global counters: each block keeps
 // an LDS histogram of its grid-stride range, one block turns the histograms
-// into per-(block, colour) offsets, and the same blocks scatter again.  The
-// order inside a class does not matter (its faces are independent).
+// into per-(block, colour) offsets, and the same blocks scatter again,
+// writing each face's id and its neighbours' ids (one int4) in class order,
+// so that a class reads its faces' neighbours contiguously instead of one
+// twin row in every 24.  The order inside a class does not matter (its
+// faces are independent).
 constexpr int kColorBlocks = 1024;
 
-__global__ void __launch_bounds__(kBlock) k_color_hist(const int32_t* __restrict__ colors, const Sizes* S,
+__global__ void __launch_bounds__(kBlock) k_color_hist(const uint8_t* __restrict__ colors, const Sizes* S,
                                                         uint32_t* __restrict__ hist) {
     __shared__ uint32_t sm[24];
     const int64_t nq = (int64_t)S->nq;
     if (threadIdx.x < 24) sm[threadIdx.x] = 0;
     __syncthreads();
     TMC_GRID_STRIDE(f, nq) {
-        const int32_t c = colors[f];
-        if (c >= 0 && c < 24) atomicAdd(&sm[c], 1u);
+        const int c = colors[f];
+        if (c < 24) atomicAdd(&sm[c], 1u);
     }
     __syncthreads();
     if (threadIdx.x < 24) hist[blockIdx.x * 24 + threadIdx.x] = sm[threadIdx.x];
@@ -465,16 +480,20 @@ __global__ void __launch_bounds__(1024) k_color_offsets(uint32_t* __restrict__ h
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restrict__ colors, const Sizes* S,
+__global__ void __launch_bounds__(kBlock) k_color_scatter(const uint8_t* __restrict__ colors, const Sizes* S,
                                                            const uint32_t* __restrict__ offs,
-                                                           int32_t* __restrict__ list) {
+                                                           const int32_t* __restrict__ twin,
+                                                           int32_t* __restrict__ list, int4* __restrict__ nbr) {
     __shared__ uint32_t cur[24];
     const int64_t nq = (int64_t)S->nq;
     if (threadIdx.x < 24) cur[threadIdx.x] = offs[blockIdx.x * 24 + threadIdx.x];
     __syncthreads();
     TMC_GRID_STRIDE(f, nq) {
-        const int32_t c = colors[f];
-        if (c >= 0 && c < 24) list[atomicAdd(&cur[c], 1u)] = (int32_t)f;
+        const int c = colors[f];
+        if (c >= 24) continue;
+        const uint32_t at = atomicAdd(&cur[c], 1u);
+        list[at] = (int32_t)f;
+        if (c >= 4) nbr[at] = face_neighbors(twin, f);   // classes 0..3 are never recoloured
     }
 }
 
@@ -482,51 +501,30 @@ __global__ void __launch_bounds__(kBlock) k_color_scatter(const int32_t* __restr
 // neighbours in common: their counts and starts in totals
 // (k_color_offsets).  Class 4 takes a colour in 0..3, the others in 0..4.
 __global__ void __launch_bounds__(kBlock) k_color_class(const int32_t* __restrict__ list,
+                                                         const int4* __restrict__ nbr,
                                                          const uint32_t* __restrict__ totals, int ca, int cb,
-                                                         const int32_t* __restrict__ twin,
-                                                         int32_t* __restrict__ colors) {
+                                                         uint8_t* __restrict__ colors) {
     const int64_t na = totals[ca], nb2 = cb >= 0 ? totals[cb] : 0;
     TMC_GRID_STRIDE(x, na + nb2) {
         const bool first = x < na;
         const int ncol = (first ? ca : cb) == 4 ? 4 : 5;
-        const int32_t f = list[first ? (int64_t)totals[24 + ca] + x : (int64_t)totals[24 + cb] + (x - na)];
-        int32_t nb[4];
-        face_neighbors(twin, f, nb);
-        int32_t c[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) c[k] = nb[k] >= 0 ? colors[nb[k]] : -1;
-        for (int col = 0; col < ncol; col++) {
-            if (c[0] != col && c[1] != col && c[2] != col && c[3] != col) {
-                colors[f] = col;
-                break;
-            }
-        }
+        const int64_t at = first ? (int64_t)totals[24 + ca] + x : (int64_t)totals[24 + cb] + (x - na);
+        take_free_colour(nbr[at], ncol, list[at], colors);
     }
 }
 
 // All classes in one block, in order, for a small mesh (one launch instead
 // of twenty): the same first-free-colour rule, a barrier between classes.
 __global__ void __launch_bounds__(1024) k_color_all(const int32_t* __restrict__ list,
+                                                     const int4* __restrict__ nbr,
                                                      const uint32_t* __restrict__ totals,
-                                                     const int32_t* __restrict__ twin, int32_t* __restrict__ colors) {
+                                                     uint8_t* __restrict__ colors) {
     for (int step = 0; step < 20; step++) {
         const int cls = step < 19 ? 5 + step : 4;
         const int ncol = step < 19 ? 5 : 4;
         const int64_t n = totals[cls], start = totals[24 + cls];
-        for (int64_t x = threadIdx.x; x < n; x += blockDim.x) {
-            const int32_t f = list[start + x];
-            int32_t nb[4];
-            face_neighbors(twin, f, nb);
-            int32_t c[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) c[k] = nb[k] >= 0 ? colors[nb[k]] : -1;
-            for (int col = 0; col < ncol; col++) {
-                if (c[0] != col && c[1] != col && c[2] != col && c[3] != col) {
-                    colors[f] = col;
-                    break;
-                }
-            }
-        }
+        for (int64_t x = threadIdx.x; x < n; x += blockDim.x)
+            take_free_colour(nbr[start + x], ncol, list[start + x], colors);
         // the next class reads this one's colours
         __threadfence_block();
         __syncthreads();
@@ -564,7 +562,7 @@ __device__ __forceinline__ bool flags_nonmanifold(uint32_t w) {
 // drop v3).  Every lookup of the neighbours is issued before any is tested.
 __device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restrict__ quads,
                                               const int32_t* __restrict__ val, const uint8_t* __restrict__ p,
-                                              const int32_t* __restrict__ colors, const int32_t* __restrict__ twin,
+                                              const uint8_t* __restrict__ colors, const int32_t* __restrict__ twin,
                                               const uint8_t* __restrict__ flag, int64_t nv) {
     if (!p[f]) return 0;
     const int4 tw = reinterpret_cast<const int4*>(twin)[f];
@@ -573,7 +571,7 @@ __device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restri
     const int4 q = reinterpret_cast<const int4*>(quads)[f];
     uint32_t fw[4];
     uint8_t pn[4];
-    int32_t cn[4];
+    int cn[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int32_t g = nb[k] >= 0 ? nb[k] : (int32_t)f;   // any valid face for a missing neighbour
@@ -583,7 +581,7 @@ __device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restri
     }
     const int a = val[q.x], b = val[q.y], c = val[q.z], d = val[q.w];
     if (colors) {
-        const int32_t cf = colors[f];
+        const int cf = colors[f];
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (nb[k] < 0 || flags_nonmanifold(fw[k])) return 0;
@@ -607,7 +605,7 @@ __device__ __forceinline__ int merge_decision(int64_t f, const int32_t* __restri
 // reference loops over faces in order, so the largest face index wins.
 __global__ void __launch_bounds__(kBlock) k_merge_claim(const int32_t* __restrict__ quads, const Sizes* S,
                                                          const int32_t* __restrict__ val, const uint8_t* __restrict__ p,
-                                                         const int32_t* __restrict__ colors,
+                                                         const uint8_t* __restrict__ colors,
                                                          const int32_t* __restrict__ twin,
                                                          const uint8_t* __restrict__ flag, int32_t* __restrict__ lastT,
                                                          int32_t* __restrict__ lastR, uint8_t* __restrict__ em,
