@@ -311,10 +311,13 @@ __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxR
     // first row: about the nearest to p0 (any row is correct)
     const float x = fminf(fmaxf((float)((p[0] - f.umin) / f.du) + 0.5f, 1.f), (float)kIn);
     int cur = (int)x, lo = cur - 1, hi = cur + 1;
-    double best = 100;
-    int bestO = -1;
+    // the best sample as one key: the float distance's bits (a taken
+    // distance is finite and not negative, so its bits order as the values
+    // do) above the sample's index, so that (distance, index) compares in
+    // one unsigned compare; starts at distance 100, the reference's bound
+    const uint64_t start = (uint64_t)__float_as_uint(100.f) << 32;
+    uint64_t bestKey = start;
     float bu = 0.f, bv = 0.f, bw = -1.f;
-    bool found = false;
     while (__any(cur >= 0)) {
         if (cur >= 0) {
             const int i = cur;
@@ -330,17 +333,16 @@ __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxR
                 const float ww = (iso - g1) / (g2 - g1);
                 const double s = ai + bj[j - 1];
                 const float d = s + (p[2] - ww) * (p[2] - ww);
-                const int o = ns * i + j;
-                const bool take = f.wmin <= ww && ww <= f.wmax && ((double)d < best || ((double)d == best && o < bestO));
-                best = take ? (double)d : best;
-                bestO = take ? o : bestO;
+                const uint64_t key = (uint64_t)__float_as_uint(d) << 32 | (uint32_t)(ns * i + j);
+                const bool take = f.wmin <= ww && ww <= f.wmax && key < bestKey;
+                bestKey = take ? key : bestKey;
                 bu = take ? u : bu;
                 bv = take ? v : bv;
                 bw = take ? ww : bw;
-                found = found || take;
             }
             // next row: the side whose next row has the smaller bound, among
             // the sides that can still hold a winner
+            const double best = (double)__uint_as_float((uint32_t)(bestKey >> 32));
             double bl = 0, bh = 0;
             if (lo >= 1) {
                 bl = row_bound(lo);
@@ -354,6 +356,7 @@ __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxR
             cur = (okl && (!okh || bl <= bh)) ? lo-- : okh ? hi++ : -1;
         }
     }
+    const bool found = bestKey != start;
     if (found) {
         p[0] = bu;
         p[1] = bv;
