@@ -452,8 +452,8 @@ struct Fills {
     }
 };
 
-// Inputs of at most this many items are scanned by one block in one launch.
-constexpr int64_t kScanSmallItems = (int64_t)1 << 14;
+// Inputs of at most one tile are scanned by one block in one launch.
+constexpr int64_t kScanSmallItems = kScanTile;
 
 // The state of the single-pass scans: zero when a scan starts, and left zero
 // by every scan, so it is cleared only when its buffer is (re)allocated or
