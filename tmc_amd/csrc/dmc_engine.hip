@@ -721,7 +721,11 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     check_launch(ctx, "k_rows");
     scan_to(ctx, ScanCounts{rowV}, rowBaseV, hn(R), &S->V, "k_scan_cells");
     scan_to(ctx, ScanCounts{rowQ}, rowBaseQ, hn(R), &S->Q, "k_scan_cells");
-    launch(ctx, "k_cell_guard", k_cell_guard, dim3(1), dim3(64), S, (uint64_t)acap, (uint64_t)vcap, (uint64_t)qcap);
+    // cells deferred to the general sampling: slow[0] = count (zeroed by the
+    // guard), then cell ids
+    uint32_t* slow = ensure<uint32_t>(ctx, ctx->slow, acap + 1);
+    launch(ctx, "k_cell_guard", k_cell_guard, dim3(1), dim3(64), S, (uint64_t)acap, (uint64_t)vcap, (uint64_t)qcap,
+           slow);
     // [0, n]: live bounds; [n+1, 2n+1]: the cell stage's (before simplification)
     int64_t* vb = ensure<int64_t>(ctx, ctx->vbound, 2 * (size_t)(n + 1));
     int64_t* qb = ensure<int64_t>(ctx, ctx->qbound, 2 * (size_t)(n + 1));
@@ -741,11 +745,8 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     uint32_t* deg = ensure<uint32_t>(ctx, ctx->deg, vcap);
     uint32_t* off = ensure<uint32_t>(ctx, ctx->off, vcap + 1);
     int2* adj = ensure<int2>(ctx, ctx->adj, 4 * qcap);
-    // cells deferred to the general sampling: slow[0] = count, then cell ids
-    uint32_t* slow = ensure<uint32_t>(ctx, ctx->slow, acap + 1);
     const DevN A = dn(&S->acnt, acap);
     mark(ctx, "vertices");
-    fill_u32(ctx, slow, hn(1), 0u);
     // persistent: a few rounds of the blocks the chip holds at once
     launch(ctx, "k_vertices", k_vertices<T>, dim3(grid_for(ctx, acap, kBlock, kVerticesBlocks)), dim3(kBlock), vt,
            acell, acode, aaux, avloc, rowBaseV, A, pos, nrm, deg, slow, &S->qhead);
@@ -818,9 +819,10 @@ void pair_rows(dmc_context* ctx, bool final_pass, bool generic) {
     check_launch(ctx, "k_he_twins");
 }
 
-// A halfedge build of the live mesh.  Not final: ends with the valence.
-// Final: the 3-face overrides of he_v and the reference's flag values.
-void build_halfedges(dmc_context* ctx, bool final_pass) {
+// A halfedge build of the live mesh.  Not final: ends with the valence,
+// which also clears what the next simplification needs cleared.  Final: the
+// 3-face overrides of he_v and the reference's flag values.
+void build_halfedges(dmc_context* ctx, bool final_pass, const StageClears& clears = StageClears{}) {
     Sizes* S = dsizes(ctx);
     const int64_t vcap = ctx->vcap;
     const int ready = ctx->force_generic ? 0 : ctx->ready;
@@ -833,7 +835,7 @@ void build_halfedges(dmc_context* ctx, bool final_pass) {
     if (!final_pass) {
         uint32_t* val = ensure<uint32_t>(ctx, ctx->val, vcap);
         launch(ctx, "k_valence", k_valence, dim3(gv), dim3(kBlock), static_cast<const uint32_t*>(ctx->deg.p),
-               static_cast<const uint32_t*>(ctx->arr.p), (const Sizes*)S, val, &S->zero);
+               static_cast<const uint32_t*>(ctx->arr.p), (const Sizes*)S, val, &S->zero, clears);
         check_launch(ctx, "valence");
         return;
     }
@@ -1052,21 +1054,27 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
 }
 
 void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
-    build_halfedges(ctx, false);
-    if (colored) color_faces(ctx);
-    const int32_t* val = static_cast<const int32_t*>(ctx->val.p);
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
-    Sizes* S = dsizes(ctx);
-    const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
-    const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
     uint8_t* p = ensure<uint8_t>(ctx, ctx->p, qcap);
     uint8_t* em = ensure<uint8_t>(ctx, ctx->em, qcap);
     uint8_t* vfirst = ensure<uint8_t>(ctx, ctx->vfirst, vcap);
     int32_t* vsecond = ensure<int32_t>(ctx, ctx->vsecond, vcap);
     int32_t* lastT = ensure<int32_t>(ctx, ctx->lastT, vcap);
     int32_t* lastR = ensure<int32_t>(ctx, ctx->lastR, vcap);
-    Fills(ctx).u8(vfirst, nv, 0).u32(vsecond, nv, 0xFFFFFFFFu).u32(lastT, nv, 0xFFFFFFFFu)
-        .u32(lastR, nv, 0xFFFFFFFFu).run();
+    // vfirst = 0, vsecond = lastT = lastR = -1 over the live vertices
+    StageClears cl;
+    cl.v8 = vfirst;
+    cl.v32[0] = reinterpret_cast<uint32_t*>(vsecond);
+    cl.v32[1] = reinterpret_cast<uint32_t*>(lastT);
+    cl.v32[2] = reinterpret_cast<uint32_t*>(lastR);
+    cl.vcap = vcap;
+    cl.qcap = qcap;
+    build_halfedges(ctx, false, cl);
+    if (colored) color_faces(ctx);
+    const int32_t* val = static_cast<const int32_t*>(ctx->val.p);
+    Sizes* S = dsizes(ctx);
+    const DevN nq = dn(&S->nq, qcap);
+    const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const uint8_t* colors = colored ? static_cast<uint8_t*>(ctx->colors.p) : nullptr;
@@ -1104,19 +1112,26 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
 }
 
 void simplify3333(dmc_context* ctx, bool last) {
-    build_halfedges(ctx, false);
-    const int32_t* val = static_cast<const int32_t*>(ctx->val.p);
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap;
-    Sizes* S = dsizes(ctx);
-    const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
-    const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
     uint8_t* vflag = ensure<uint8_t>(ctx, ctx->vfirst, vcap);
     int32_t* vtwin = ensure<int32_t>(ctx, ctx->vsecond, vcap);
     int32_t* lastF = ensure<int32_t>(ctx, ctx->lastT, vcap);
     uint8_t* p3333 = ensure<uint8_t>(ctx, ctx->p, qcap);
     uint8_t* rflag = ensure<uint8_t>(ctx, ctx->em, qcap);
-    Fills(ctx).u8(vflag, nv, 0).u32(vtwin, nv, 0xFFFFFFFFu).u32(lastF, nv, 0xFFFFFFFFu).u8(p3333, nq, 0)
-        .u8(rflag, nq, 0).run();
+    // vflag = 0, vtwin = lastF = -1 over the live vertices, the face flags 0
+    StageClears cl;
+    cl.v8 = vflag;
+    cl.v32[0] = reinterpret_cast<uint32_t*>(vtwin);
+    cl.v32[1] = reinterpret_cast<uint32_t*>(lastF);
+    cl.f8[0] = p3333;
+    cl.f8[1] = rflag;
+    cl.vcap = vcap;
+    cl.qcap = qcap;
+    build_halfedges(ctx, false, cl);
+    const int32_t* val = static_cast<const int32_t*>(ctx->val.p);
+    Sizes* S = dsizes(ctx);
+    const DevN nv = dn(&S->nv, vcap), nq = dn(&S->nq, qcap);
+    const int32_t* quads = static_cast<int32_t*>(ctx->quads[ctx->cur].p);
     const int32_t* twin = static_cast<int32_t*>(ctx->twin.p);
     const uint8_t* flag = static_cast<uint8_t*>(ctx->flag.p);
     const int gq = grid_for(ctx, qcap);

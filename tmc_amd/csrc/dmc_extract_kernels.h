@@ -921,8 +921,9 @@ __global__ void k_vol_abase(VolTable vt, const uint32_t* __restrict__ segBase, c
 // After the row scans: which capacities the cell stage exceeded.  A pass
 // that exceeded one processes an empty mesh from here on (acnt = nv = nq =
 // 0) and is run again by the host with room for the exact sizes.
-__global__ void k_cell_guard(Sizes* S, uint64_t acap, uint64_t vcap, uint64_t qcap) {
+__global__ void k_cell_guard(Sizes* S, uint64_t acap, uint64_t vcap, uint64_t qcap, uint32_t* slow) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        slow[0] = 0;   // k_vertices' count of deferred cells
         const uint32_t o = (S->A > acap ? 1u : 0u) | (S->V > vcap ? 2u : 0u) | (S->Q > qcap ? 4u : 0u);
         S->ovf |= o;
         S->acnt = o ? 0 : S->A;

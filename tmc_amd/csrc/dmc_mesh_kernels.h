@@ -363,12 +363,23 @@ __global__ void __launch_bounds__(kBlock) k_strip_flags(const Sizes* S, uint8_t*
     }
 }
 
+// Arrays the simplification after a halfedge build needs cleared, cleared
+// by the build's valence pass (one launch fewer per stage): bytes to 0 and
+// words to 0xFFFFFFFF per vertex, bytes to 0 per face (null: none).
+struct StageClears {
+    uint8_t* v8 = nullptr;
+    uint32_t* v32[3] = {nullptr, nullptr, nullptr};
+    uint8_t* f8[2] = {nullptr, nullptr};
+    int64_t vcap = 0, qcap = 0;
+};
+
 // Valence (cpp_dmc/DualMarchingCubes.cpp:847-871): outgoing halfedges plus
 // boundary halfedges ending at the vertex -- the out-degree and the
 // arrivals the pairing counted.  Counts the vertices of valence 0.
 __global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__ deg,
                                                      const uint32_t* __restrict__ arr, const Sizes* S,
-                                                     uint32_t* __restrict__ val, unsigned int* zero) {
+                                                     uint32_t* __restrict__ val, unsigned int* zero,
+                                                     StageClears c) {
     const int64_t nv = (int64_t)S->nv;
     unsigned int z = 0;
     TMC_GRID_STRIDE(v, nv) {
@@ -378,6 +389,15 @@ __global__ void __launch_bounds__(kBlock) k_valence(const uint32_t* __restrict__
     }
     z = wave_sum(z);
     if ((threadIdx.x & (kWave - 1)) == 0 && z) atomicAdd(zero, z);
+    const int64_t nc = nv < c.vcap ? nv : c.vcap;
+    if (c.v8) TMC_GRID_STRIDE(v, nc) c.v8[v] = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (c.v32[k]) TMC_GRID_STRIDE(v, nc) c.v32[k][v] = 0xFFFFFFFFu;
+    const int64_t nq = (int64_t)S->nq < c.qcap ? (int64_t)S->nq : c.qcap;
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (c.f8[k]) TMC_GRID_STRIDE(f, nq) c.f8[k][f] = 0;
 }
 
 // The four edge neighbours of face f (-1 where an edge has no twin).
