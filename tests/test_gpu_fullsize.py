@@ -111,6 +111,7 @@ def test_dmc_full_size_properties(eng, which):
     # the same bytes under another launch shape
     keep = {k: d[k] for k in ("v", "n", "quads", "tris", "he")}
     del d, he, quads, v, n, vb
+    before = eng.launch
     eng.launch = {"grid_cap": 4096, "sweep_waves": 8192}
     try:
         m2 = eng.extract(f, iso, g, flags=SIMPLIFY_ALL | TRIANGLES)
@@ -119,7 +120,7 @@ def test_dmc_full_size_properties(eng, which):
         for k, a in keep.items():
             assert_same(d2[k], a, f"{which} under another launch shape: {k}")
     finally:
-        eng.launch = {"grid_cap": 16384, "sweep_waves": 32768}
+        eng.launch = before
 
 
 @pytest.mark.parametrize("which", ["blobs_512", "schwarz_1024"])
