@@ -815,7 +815,7 @@ void pair_rows(dmc_context* ctx, bool final_pass, bool generic) {
     }
     fl.run();
     launch(ctx, "k_he_twins", k_he_twins, dim3(grid_for(ctx, hcap)), dim3(kBlock), quads, (const Sizes*)S, off, adj,
-           twin, flag, hev, arr, &S->he, recs, &S->nrec, (uint32_t)ctx->reccap, &S->ovf,
+           twin, flag, hev, arr, S->hep, recs, &S->nrec, (uint32_t)ctx->reccap, &S->ovf,
            static_cast<const int64_t*>(ctx->qbound.p), ctx->nvol, vst, rows);
     check_launch(ctx, "k_he_twins");
 }
@@ -1370,6 +1370,13 @@ bool finish_pass(dmc_context* ctx, const std::vector<HostVol>& hv, uint32_t flag
     }
     finish_timing(ctx);
     if (hs.A > 0xFFFFFFFFull) fail(ctx, DMC_ERR_CAPACITY, "too many active cells (%llu)", (unsigned long long)hs.A);
+    // the build's flagged-halfedge counts from their slots
+    hs.he.nm4 = hs.he.nm3 = hs.he.errors = 0;
+    for (const auto& part : hs.hep) {
+        hs.he.nm4 += part[0];
+        hs.he.nm3 += part[1];
+        hs.he.errors += part[2];
+    }
     if (hs.V > 0x7FFFFFFFull || hs.Q > 0x1FFFFFFFull)
         fail(ctx, DMC_ERR_CAPACITY, "mesh too large (V=%llu, Q=%llu)", (unsigned long long)hs.V,
              (unsigned long long)hs.Q);

@@ -49,6 +49,9 @@ struct DevN {
 // Statistics of one halfedge build, counted per halfedge so that a build
 // that carries most groups over from the last one can count them too:
 // non-manifold edges = nm4 / 4 + nm3 / 3, 3-face edges = nm3 / 3.
+// Halfedge statistics of a build.  On the device only bad and pad are kept
+// here; the counts of flagged halfedges live in Sizes::hep (spread over
+// kHeParts slots) and the host writes their sums into nm4, errors, nm3.
 struct HeStats {
     unsigned long long nm4;      // halfedges flagged non-manifold (groups of 4)
     unsigned long long errors;   // groups of more than 4 halfedges
@@ -56,6 +59,12 @@ struct HeStats {
     unsigned int bad;            // invalid vertex references
     unsigned int pad;
 };
+
+// The flagged-halfedge counts each block of a build adds are spread over
+// this many slots (block index modulo the count) and summed where they are
+// read, so that a large grid over a noisy mesh does not queue its adds on
+// three addresses.
+constexpr int kHeParts = 64;
 
 // A single volume's contour count is spread over this many counters (one
 // per wave slot, summed by the host), so that the waves' adds at the end of
@@ -91,7 +100,22 @@ struct Sizes {
     uint32_t recmax;             // most 3-face records any build of the pass wrote (capacity planning)
     uint32_t colors[48];         // colour classes: counts [0, 24), starts [24, 48)
     unsigned long long cpart[kContourParts];   // a single volume's contour count, in parts
+    unsigned long long hep[kHeParts][3];      // the build's flagged halfedges per slot: groups of 4, of 3, error groups
 };
+
+// The build's count of error groups (groups of more than 4 halfedges).
+__device__ __forceinline__ unsigned long long he_errors(const Sizes* S) {
+    unsigned long long e = 0;
+    for (int p = 0; p < kHeParts; p++) e += S->hep[p][2];
+    return e;
+}
+
+// Fresh counts for a new build (one thread).
+__device__ __forceinline__ void he_clear(Sizes* S) {
+    for (int p = 0; p < kHeParts; p++) S->hep[p][0] = S->hep[p][1] = S->hep[p][2] = 0;
+    S->he.bad = 0;
+    S->he.pad = 0;
+}
 
 // A launch whose work is conditional on device data: it runs only when
 // *gate != 0 (gate == nullptr: always).

@@ -182,12 +182,24 @@ struct HeCount {
         n3 += fl == 2;
         err += fl == kFlagErrFirst;
     }
-    __device__ __forceinline__ void flush(HeStats* st) const {
+    // One add per block and counter, into the block's slot: the waves' sums
+    // meet in LDS.  Every thread of the block calls this.
+    __device__ __forceinline__ void flush(unsigned long long (*hep)[3]) const {
+        __shared__ uint32_t sm[3];
+        if (threadIdx.x < 3) sm[threadIdx.x] = 0;
+        __syncthreads();
         const uint32_t a = wave_sum(n4), b = wave_sum(n3), e = wave_sum(err);
         if ((threadIdx.x & (kWave - 1)) == 0) {
-            if (a) atomicAdd(&st->nm4, (unsigned long long)a);
-            if (b) atomicAdd(&st->nm3, (unsigned long long)b);
-            if (e) atomicAdd(&st->errors, (unsigned long long)e);
+            if (a) atomicAdd(&sm[0], a);
+            if (b) atomicAdd(&sm[1], b);
+            if (e) atomicAdd(&sm[2], e);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long* slot = hep[blockIdx.x & (kHeParts - 1)];
+            if (sm[0]) atomicAdd(&slot[0], (unsigned long long)sm[0]);
+            if (sm[1]) atomicAdd(&slot[1], (unsigned long long)sm[1]);
+            if (sm[2]) atomicAdd(&slot[2], (unsigned long long)sm[2]);
         }
     }
 };
@@ -222,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
                                                       const uint32_t* __restrict__ off,
                                                       const int2* __restrict__ adj, int32_t* __restrict__ twin,
                                                       uint8_t* __restrict__ flag, int32_t* __restrict__ hev,
-                                                      uint32_t* __restrict__ arr, HeStats* st,
+                                                      uint32_t* __restrict__ arr, unsigned long long (*hep)[3],
                                                       BndRec* __restrict__ recs, uint32_t* nrec, uint32_t reccap,
                                                       uint32_t* ovf, const int64_t* __restrict__ qb, int nvol,
                                                       unsigned long long* __restrict__ vst,
@@ -333,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_he_twins(const int32_t* __restrict__
         vol_count(vst, qb, nvol, h, fl);
         if (c == 3 && g[0] == h) put_record(recs, nrec, reccap, ovf, three_record(quads, g, h));
     }
-    cnt.flush(st);
+    cnt.flush(hep);
 }
 
 // The 3-face overrides of he_v (cpp_dmc/DualMarchingCubes.cpp:603-637) from
@@ -356,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) k_three_apply(const BndRec* __restrict
 // The final mesh's flags as the reference leaves them: the markers of
 // groups of more than 4 faces cleared (only when there are any).
 __global__ void __launch_bounds__(kBlock) k_strip_flags(const Sizes* S, uint8_t* __restrict__ flag) {
-    if (S->he.errors == 0) return;
+    if (he_errors(S) == 0) return;
     const int64_t nhe = 4 * (int64_t)S->nq;
     TMC_GRID_STRIDE(h, nhe) {
         if (flag[h] & kFlagErr) flag[h] = 0;
@@ -813,11 +825,7 @@ __global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__
 __device__ __forceinline__ void open_carry(Sizes* S) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     S->incr_ok = S->he.bad == 0u ? 1u : 0u;
-    S->he.nm4 = 0;
-    S->he.errors = 0;
-    S->he.nm3 = 0;
-    S->he.bad = 0;
-    S->he.pad = 0;
+    he_clear(S);
     S->nrec_prev = S->nrec;
     S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
     S->nrec = 0;
@@ -1074,7 +1082,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3x3y(const int32_t* __r
         }
         block_count(c.bcount, f0 / kBlock, carry_face(c, act, f, nf, r, nn, cnt));
     }
-    cnt.flush(&c.S->he);
+    cnt.flush(c.S->hep);
 }
 
 // removeQuadrilaterals3333 (cpp_dmc/DualMarchingCubes.cpp:1320-1360), with
@@ -1111,7 +1119,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_quads3333(const int32_t* __r
         }
         block_count(c.bcount, f0 / kBlock, carry_face(c, act, f, nf, r, nn, cnt));
     }
-    cnt.flush(&c.S->he);
+    cnt.flush(c.S->hep);
 }
 
 // The flagged halfedges as a dense list, and their rows by origin.  Walks
@@ -1185,7 +1193,7 @@ __global__ void __launch_bounds__(kBlock) k_aff_twins(const int32_t* __restrict_
             if (h == hmax) c.hev_n[a] = h;
         }
     }
-    cnt.flush(&c.S->he);
+    cnt.flush(c.S->hep);
 }
 
 // 3-face records of groups the compaction left alone, renumbered.
@@ -1236,11 +1244,7 @@ __global__ void k_rows_gate(Sizes* S, int ready, int commit) {
         S->generic = ready == 0 ? 1u : 0u;
         if (!incr) {
             S->recmax = S->nrec > S->recmax ? S->nrec : S->recmax;
-            S->he.nm4 = 0;
-            S->he.errors = 0;
-            S->he.nm3 = 0;
-            S->he.bad = 0;
-            S->he.pad = 0;
+            he_clear(S);
             S->nrec = 0;
         }
     }
