@@ -348,7 +348,7 @@ dmc_status tmc_download(dmc_context* ctx, double* positions, double* normals, in
  * dmc/DualMarchingCubes.cu).  Results never depend on the launch shape,
  * only the time does.
  *   grid_cap     blocks (256 threads) of every grid-stride kernel, which
- *                loops over its items: 64 .. 2^24, default 65536 (env
+ *                loops over its items: 64 .. 2^24, default 32768 (env
  *                TMC_GRID_CAP);
  *   sweep_waves  waves the field sweep aims for, one 256-cell x 16-row tile
  *                of K planes each; sets K (never below 8): 256 .. 2^24,

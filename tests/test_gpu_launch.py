@@ -62,7 +62,7 @@ def own_engine():
 def test_launch_shape_does_not_change_results(own_engine):
     eng = own_engine
     default = eng.launch
-    assert default == {"grid_cap": 65536, "sweep_waves": 32768}
+    assert default == {"grid_cap": 32768, "sweep_waves": 32768}
     vols = _volumes()
     _say(f"[gpu] launch shapes: batch of {len(vols)} volumes, default {default}")
     ref = _batch(eng, vols)
@@ -102,7 +102,7 @@ def test_autotune_keeps_fastest_and_same_results(own_engine):
     # four sweep targets, then the four block caps other than the start's
     assert len(trials) == 8
     assert {t["sweep_waves"] for t in trials[:4]} == {8192, 16384, 32768, 65536}
-    assert {t["grid_cap"] for t in trials[4:]} == {4096, 8192, 16384, 32768}
+    assert {t["grid_cap"] for t in trials[4:]} == {4096, 8192, 16384, 65536}
     best = res["best"]
     assert best["ms"] == min(t["ms"] for t in trials)
     assert any(t["grid_cap"] == best["grid_cap"] and t["sweep_waves"] == best["sweep_waves"] for t in trials)
