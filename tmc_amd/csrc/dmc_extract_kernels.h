@@ -89,6 +89,11 @@ constexpr int kSweepRows = TMC_SWEEP_ROWS;  // J
 #ifndef TMC_VERTICES_ATTR
 #define TMC_VERTICES_ATTR
 #endif
+// k_vertices' work-queue take: this many rounds of a block's threads.
+#ifndef TMC_VERTICES_CHUNK
+#define TMC_VERTICES_CHUNK 8
+#endif
+constexpr int kVerticesChunk = TMC_VERTICES_CHUNK;
 
 // Loads of the 4 points i0..i0+3 of a row.  Rows read by one wave only are
 // streamed (non-temporal, so they do not evict reused lines); rows that a
@@ -664,18 +669,22 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
     __shared__ double corners[8 * kCornerStride];
     const BoxRef B{boxes + threadIdx.x};
     const CornerRef Fc{corners + threadIdx.x};
-    // Persistent blocks pull chunks of kBlock active cells from a device
-    // work queue (*qhead, ascending): cells cost unevenly (one to four
-    // contours, searches of different lengths, the volumes of a batch of
-    // different sizes), so no block is held to a fixed share.
+    // Persistent blocks pull chunks of kVerticesChunk x kBlock active cells
+    // from a device work queue (*qhead, ascending): cells cost unevenly (one
+    // to four contours, searches of different lengths, the volumes of a
+    // batch of different sizes), so no block is held to a fixed share.  Each
+    // take is one add on a single address for all blocks, so a take covers
+    // several rounds of the block's threads.
     __shared__ int64_t chunk;
     for (;;) {
-        if (threadIdx.x == 0) chunk = (int64_t)atomicAdd(qhead, (unsigned long long)kBlock);
+        if (threadIdx.x == 0) chunk = (int64_t)atomicAdd(qhead, (unsigned long long)(kVerticesChunk * kBlock));
         __syncthreads();
-        const int64_t a = chunk + threadIdx.x;
-        const bool drained = chunk >= A;
+        const int64_t c0 = chunk;
+        const bool drained = c0 >= A;
         __syncthreads();
         if (drained) break;
+        for (int round = 0; round < kVerticesChunk; round++) {
+        const int64_t a = c0 + (int64_t)round * kBlock + threadIdx.x;
         if (a >= A) continue;
         const int vol = vol_of(vt.aBase, vt.n, a);
         const VolDesc& V = vt.desc[vol];
@@ -714,6 +723,7 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
             base = (uint32_t)__shfl((int)base, leader, kWave);
             const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
             if (defer) slow[1 + base + below] = (uint32_t)a;
+        }
         }
     }
 }
