@@ -764,8 +764,21 @@ __global__ void __launch_bounds__(kBlock) k_vertices_slow(VolTable vt, const uin
         for (int t = 0; t < cnt; t++) {
             double p[3];
             const int fl = contour_representative(Fc, iso, w, cnt, t, B, p);
-            if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
-            if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
+            // failed projections: one add per wave when its lanes share the
+            // volume (always, for one volume)
+            const uint64_t m1 = __ballot(fl >= 1), m2 = __ballot(fl >= 2);
+            if (m1) {
+                const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+                if (__all(vol == __shfl(vol, leader, kWave))) {
+                    if ((int)__lane_id() == leader) {
+                        atomicAdd(&vstat[vol].fail1, (uint32_t)__popcll(m1));
+                        if (m2) atomicAdd(&vstat[vol].fail2, (uint32_t)__popcll(m2));
+                    }
+                } else {
+                    if (fl >= 1) atomicAdd(&vstat[vol].fail1, 1u);
+                    if (fl >= 2) atomicAdd(&vstat[vol].fail2, 1u);
+                }
+            }
             if (!((used >> t) & 1u)) continue;
             store_vertex<T>(V, i, j, k, Fc, o, p, code, aux, t, out, vpos, vnrm, vdeg);
             out++;

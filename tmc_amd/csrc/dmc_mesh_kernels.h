@@ -166,9 +166,16 @@ __device__ __forceinline__ BndRec three_record(const int32_t* __restrict__ quads
     return r;
 }
 
+// A record appended to recs (order free: the host sorts them by key).  The
+// lanes of a wave that append together take their slots with one add.
 __device__ __forceinline__ void put_record(BndRec* __restrict__ recs, uint32_t* nrec, uint32_t reccap, uint32_t* ovf,
                                            const BndRec& r) {
-    const uint32_t p = atomicAdd(nrec, 1u);
+    const uint64_t m = __ballot(1);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(nrec, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader, kWave);
+    const uint32_t p = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
     if (p < reccap) recs[p] = r;
     else atomicOr(ovf, 8u);
 }
