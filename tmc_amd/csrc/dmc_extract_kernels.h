@@ -89,7 +89,7 @@ constexpr int kSweepRows = TMC_SWEEP_ROWS;  // J
 #ifndef TMC_VERTICES_ATTR
 #define TMC_VERTICES_ATTR
 #endif
-// k_vertices' work-queue take: this many rounds of a block's threads.
+// k_vertices' work-queue take: at most this many rounds of a block's threads.
 #ifndef TMC_VERTICES_CHUNK
 #define TMC_VERTICES_CHUNK 8
 #endif
@@ -674,16 +674,19 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
     // to four contours, searches of different lengths, the volumes of a
     // batch of different sizes), so no block is held to a fixed share.  Each
     // take is one add on a single address for all blocks, so a take covers
-    // several rounds of the block's threads.
+    // several rounds of the block's threads: up to kVerticesChunk, fewer
+    // when the cells would make under four takes per block (a small volume
+    // keeps every block busy instead of leaving most of the grid idle).
+    const int rounds = (int)std::min<int64_t>(kVerticesChunk, std::max<int64_t>(1, A / ((int64_t)kBlock * gridDim.x * 4)));
     __shared__ int64_t chunk;
     for (;;) {
-        if (threadIdx.x == 0) chunk = (int64_t)atomicAdd(qhead, (unsigned long long)(kVerticesChunk * kBlock));
+        if (threadIdx.x == 0) chunk = (int64_t)atomicAdd(qhead, (unsigned long long)(rounds * kBlock));
         __syncthreads();
         const int64_t c0 = chunk;
         const bool drained = c0 >= A;
         __syncthreads();
         if (drained) break;
-        for (int round = 0; round < kVerticesChunk; round++) {
+        for (int round = 0; round < rounds; round++) {
         const int64_t a = c0 + (int64_t)round * kBlock + threadIdx.x;
         if (a >= A) continue;
         const int vol = vol_of(vt.aBase, vt.n, a);
