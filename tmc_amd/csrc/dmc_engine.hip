@@ -497,6 +497,46 @@ void scan_to(dmc_context* ctx, In in, uint32_t* out, DevN n, uint64_t* dtotal, c
     check_launch(ctx, "scan");
 }
 
+template <class In>
+ScanJob<In> scan_job(In in, uint32_t* out, DevN n, uint64_t* dtotal, const uint32_t* gate = nullptr,
+                     uint32_t* row_end = nullptr) {
+    ScanJob<In> j{};
+    j.in = in;
+    j.out = out;
+    j.dn = n;
+    j.gate = gate;
+    j.total = dtotal;
+    j.row_end = row_end;
+    j.ntiles = (n.cap + kScanTile - 1) / kScanTile;
+    return j;
+}
+
+// Two independent scans (neither reads what the other writes) in the
+// launches of one: one look-back launch when both fit kScanOnePassTiles,
+// else the three passes for both.
+template <class A, class B>
+void scan_pair(dmc_context* ctx, ScanJob<A> a, ScanJob<B> b, const char* name = "k_scan") {
+    a.ntiles = std::max<int64_t>(a.ntiles, 1);
+    b.ntiles = std::max<int64_t>(b.ntiles, 1);
+    const unsigned blocks = (unsigned)(a.ntiles + b.ntiles);
+    if (a.ntiles <= kScanOnePassTiles && b.ntiles <= kScanOnePassTiles) {
+        if (static_cast<const void*>(a.in.p) == static_cast<const void*>(a.out) ||
+            static_cast<const void*>(b.in.p) == static_cast<const void*>(b.out))
+            fail(ctx, DMC_ERR_STATE, "%s: a scan in place", name);
+        // each job's state: kScanStateHead words, then its tile flags
+        a.st = scan_state(ctx, kScanStateHead + a.ntiles + b.ntiles);
+        b.st = a.st + kScanStateHead + a.ntiles;
+        launch(ctx, name, k_scan_pair<A, B>, dim3(blocks), dim3(kBlock), a, b, ctx->scan_spin);
+    } else {
+        a.partial = ensure<uint64_t>(ctx, ctx->scanPartial, (size_t)blocks);
+        b.partial = a.partial + a.ntiles;
+        launch(ctx, name, k_scan_reduce_pair<A, B>, dim3(blocks), dim3(kBlock), a, b);
+        launch(ctx, name, k_scan_partials_pair<A, B>, dim3(2), dim3(1024), a, b);
+        launch(ctx, name, k_scan_down_pair<A, B>, dim3(blocks), dim3(kBlock), a, b);
+    }
+    check_launch(ctx, "scan");
+}
+
 // Host-counted scan with the total read back (diagnostics and the
 // topologically correct MC, which size their launches on the host).
 template <class In>
@@ -720,8 +760,8 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     launch(ctx, "k_rows", k_rows<T>, dim3(grid_for(ctx, R, kBlock / kWave)), dim3(kBlock), vt, R, sw.lists,
            sw.segCount, sw.segBase, acell, acode, aaux, avloc, aqloc, rowV, rowQ, sw.vstat, S->cpart, (uint32_t)acap);
     check_launch(ctx, "k_rows");
-    scan_to(ctx, ScanCounts{rowV}, rowBaseV, hn(R), &S->V, "k_scan_cells");
-    scan_to(ctx, ScanCounts{rowQ}, rowBaseQ, hn(R), &S->Q, "k_scan_cells");
+    scan_pair(ctx, scan_job(ScanCounts{rowV}, rowBaseV, hn(R), &S->V), scan_job(ScanCounts{rowQ}, rowBaseQ, hn(R), &S->Q),
+              "k_scan_cells");
     // cells deferred to the general sampling: slow[0] = count (zeroed by the
     // guard), then cell ids
     uint32_t* slow = ensure<uint32_t>(ctx, ctx->slow, acap + 1);
@@ -951,11 +991,18 @@ void remap_bounds(dmc_context* ctx, Buf& bounds, const uint32_t* newidx, const u
 // non-null) receives new indices of kept vertices.  Positions and normals
 // stay where they are; the slot map follows the vertices (gather_vertices
 // lays them out at the end).
-uint32_t* compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond) {
+// The scan of the kept vertices into ctx->newV (S->nn of them), the job that
+// compact_vertices(..., scanned = true) expects to have run.
+ScanJob<ScanKeep> kept_vertices_job(dmc_context* ctx, const uint8_t* removed) {
+    Sizes* S = dsizes(ctx);
+    return scan_job(ScanKeep{removed}, ensure<uint32_t>(ctx, ctx->newV, ctx->vcap), dn(&S->nv, ctx->vcap), &S->nn);
+}
+
+uint32_t* compact_vertices(dmc_context* ctx, const uint8_t* removed, int32_t* vsecond, bool scanned = false) {
     const int64_t vcap = ctx->vcap;
     Sizes* S = dsizes(ctx);
     uint32_t* newidx = ensure<uint32_t>(ctx, ctx->newV, vcap);
-    scan_to(ctx, ScanKeep{removed}, newidx, dn(&S->nv, vcap), &S->nn);
+    if (!scanned) scan_to(ctx, ScanKeep{removed}, newidx, dn(&S->nv, vcap), &S->nn);
     const uint32_t* slot = ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr;
     uint32_t* slot2 = ensure<uint32_t>(ctx, ctx->vslot[1 - ctx->scur], vcap);
     launch(ctx, "k_compact_slots", k_compact_slots, dim3(grid_for(ctx, vcap)), dim3(kBlock), removed, newidx,
@@ -1036,8 +1083,8 @@ void carry_end(dmc_context* ctx, const Carry& c, const int32_t* vmap) {
     const int64_t turns = ctx->qcap / kBlock + 1;
     uint32_t* boff = ensure<uint32_t>(ctx, ctx->boff, turns);
     int32_t* alist = ensure<int32_t>(ctx, ctx->alist, hcap);
-    scan_to(ctx, ScanCounts{c.bcount}, boff, hn(turns), &S->naff, "k_scan");
-    scan_to(ctx, ScanCounts{c.cnt}, aoff, nn, &S->tot, "k_scan", nullptr, aoff);
+    scan_pair(ctx, scan_job(ScanCounts{c.bcount}, boff, hn(turns), &S->naff),
+              scan_job(ScanCounts{c.cnt}, aoff, nn, &S->tot, nullptr, aoff));
     launch(ctx, "k_aff_scatter", k_aff_scatter, dim3(grid_for(ctx, ctx->qcap)), dim3(kBlock), (const Sizes*)S,
            c.qmap, (const uint8_t*)c.aflag, (const uint32_t*)c.bcount, (const uint32_t*)boff, quads2,
            (const uint32_t*)aoff, aadj, alist);
@@ -1085,8 +1132,9 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
     launch(ctx, "k_merge_claim", k_merge_claim, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, val,
            (const uint8_t*)p, colors, twin, flag, lastT, lastR, em, vfirst);
     check_launch(ctx, "merge3x3y");
-    // the quadrilateral compaction's scan also lists the merging faces
-    scan_to(ctx, ScanKeepMap{{em}}, newQ, nq, &S->qn);
+    // the quadrilateral compaction's scan (which also lists the merging
+    // faces) beside the vertices' (vfirst is final after the claims)
+    scan_pair(ctx, scan_job(ScanKeepMap{{em}}, newQ, nq, &S->qn), kept_vertices_job(ctx, vfirst));
     int32_t* mlist = ensure<int32_t>(ctx, ctx->mlist, qcap);
     launch(ctx, "k_list_flagged", k_list_flagged, dim3(gq), dim3(kBlock), (const uint8_t*)em, (const int32_t*)newQ,
            nq, mlist);
@@ -1095,7 +1143,7 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
            static_cast<double*>(ctx->pos[ctx->pcur].p), static_cast<double*>(ctx->nrm[ctx->pcur].p),
            ctx->slots ? static_cast<const uint32_t*>(ctx->vslot[ctx->scur].p) : nullptr, vsecond);
     check_launch(ctx, "k_merge_apply");
-    compact_vertices(ctx, vfirst, vsecond);
+    compact_vertices(ctx, vfirst, vsecond, true);
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
@@ -1141,9 +1189,9 @@ void simplify3333(dmc_context* ctx, bool last) {
     launch(ctx, "k_twin3333", k_twin3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S, twin, flag, val,
            (const uint8_t*)p3333, (const int32_t*)lastF, vtwin);
     check_launch(ctx, "mark3333");
-    uint32_t* newv = compact_vertices(ctx, vflag, nullptr);
     uint32_t* newQ = ensure<uint32_t>(ctx, ctx->newQ, qcap);
-    scan_to(ctx, ScanKeepMap{{rflag}}, newQ, nq, &S->qn);
+    scan_pair(ctx, scan_job(ScanKeepMap{{rflag}}, newQ, nq, &S->qn), kept_vertices_job(ctx, vflag));
+    uint32_t* newv = compact_vertices(ctx, vflag, nullptr, true);
     int32_t* q2 = ensure<int32_t>(ctx, ctx->quads[1 - ctx->cur], 4 * qcap);
     int32_t* vmap = ensure<int32_t>(ctx, ctx->vmap, vcap);
     remap_bounds(ctx, ctx->qbound, newQ, &S->nq, &S->qn);
@@ -1247,8 +1295,8 @@ void run_tmc(dmc_context* ctx, const std::vector<HostVol>& hv) {
         check_launch(ctx, "k_tmc_amb_count");
     }
     uint64_t* tot = ensure<uint64_t>(ctx, ctx->scanTotal, 2);
-    scan_to(ctx, ScanCounts{cnt_v}, vofs, hn((int64_t)A), tot);
-    scan_to(ctx, ScanCounts{cnt_t}, tofs, hn((int64_t)A), tot + 1);
+    scan_pair(ctx, scan_job(ScanCounts{cnt_v}, vofs, hn((int64_t)A), tot),
+              scan_job(ScanCounts{cnt_t}, tofs, hn((int64_t)A), tot + 1));
     uint64_t VT[2] = {0, 0};
     {
         HostReads rd(ctx);

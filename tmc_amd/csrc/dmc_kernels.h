@@ -231,13 +231,13 @@ __device__ __forceinline__ void scan_items(const In& in, int64_t n, int64_t base
 // ceil(dn.cap / kScanTile) tiles (tile sums, their scan, the tiles again);
 // skipped entirely when gated off.  For inputs of many tiles.
 template <class In>
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, DevN dn, const uint32_t* __restrict__ gate,
-                                                       uint64_t* __restrict__ partial) {
+__device__ __forceinline__ void scan_reduce_tile(const In& in, DevN dn, const uint32_t* __restrict__ gate,
+                                                 uint64_t* __restrict__ partial, int64_t tile) {
     __shared__ uint64_t sm[kBlock / kWave + 1];
     if (gated_off(gate)) return;
     const int64_t n = dn.get();
     uint32_t v[kScanIPT];
-    scan_items(in, n, (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT, v);
+    scan_items(in, n, tile * kScanTile + (int64_t)threadIdx.x * kScanIPT, v);
     uint64_t s = 0;
 #pragma unroll
     for (int q = 0; q < kScanIPT; q++) s += v[q];
@@ -248,15 +248,21 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, DevN dn, const ui
     if (threadIdx.x == 0) {
         uint64_t t = 0;
         for (int w = 0; w < kBlock / kWave; w++) t += sm[w];
-        partial[blockIdx.x] = t;
+        partial[tile] = t;
     }
+}
+
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(In in, DevN dn, const uint32_t* __restrict__ gate,
+                                                       uint64_t* __restrict__ partial) {
+    scan_reduce_tile(in, dn, gate, partial, (int64_t)blockIdx.x);
 }
 
 // Single block: exclusive scan of the tile sums in place, total to *total
 // (and, for a CSR, its low 32 bits to row_end[n], the end of the last row).
-__global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ partial, int64_t ntiles,
-                                                         const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
-                                                         uint32_t* __restrict__ row_end, DevN dn) {
+__device__ __forceinline__ void scan_partials_run(uint64_t* __restrict__ partial, int64_t ntiles,
+                                                  const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                  uint32_t* __restrict__ row_end, DevN dn) {
     __shared__ uint64_t sm[1024 / kWave + 1];
     if (gated_off(gate)) return;
     uint64_t carry = 0;
@@ -274,14 +280,20 @@ __global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ p
     }
 }
 
+__global__ void __launch_bounds__(1024) k_scan_partials(uint64_t* __restrict__ partial, int64_t ntiles,
+                                                         const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                         uint32_t* __restrict__ row_end, DevN dn) {
+    scan_partials_run(partial, ntiles, gate, total, row_end, dn);
+}
+
 template <class In>
-__global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restrict__ out, DevN dn,
-                                                       const uint32_t* __restrict__ gate,
-                                                       const uint64_t* __restrict__ partial) {
+__device__ __forceinline__ void scan_down_tile(const In& in, uint32_t* __restrict__ out, DevN dn,
+                                               const uint32_t* __restrict__ gate,
+                                               const uint64_t* __restrict__ partial, int64_t tile) {
     __shared__ uint32_t sm[kBlock / kWave + 1];
     if (gated_off(gate)) return;
     const int64_t n = dn.get();
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanIPT;
     const bool full = base + kScanIPT <= n;
     uint32_t v[kScanIPT];
     scan_items(in, n, base, v);
@@ -289,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
 #pragma unroll
     for (int q = 0; q < kScanIPT; q++) s += v[q];
     uint32_t tot;
-    uint32_t run = block_exclusive_scan<uint32_t, kBlock>(s, sm, tot) + (uint32_t)partial[blockIdx.x];
+    uint32_t run = block_exclusive_scan<uint32_t, kBlock>(s, sm, tot) + (uint32_t)partial[tile];
     uint32_t o[kScanIPT];
 #pragma unroll
     for (int q = 0; q < kScanIPT; q++) {
@@ -305,6 +317,13 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restric
         for (int q = 0; q < kScanIPT; q++)
             if (base + q < n) out[base + q] = o[q];
     }
+}
+
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_scan_down(In in, uint32_t* __restrict__ out, DevN dn,
+                                                       const uint32_t* __restrict__ gate,
+                                                       const uint64_t* __restrict__ partial) {
+    scan_down_tile(in, out, dn, gate, partial, (int64_t)blockIdx.x);
 }
 
 // The same in one launch of ceil(dn.cap / kScanTile) blocks (decoupled
@@ -335,10 +354,10 @@ __device__ __forceinline__ void scan_flag_store(unsigned long long* p, uint64_t 
 }
 
 template <class In>
-__global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ out, DevN dn,
-                                                  const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
-                                                  uint32_t* __restrict__ row_end, unsigned long long* __restrict__ st,
-                                                  int64_t ntiles, int spin) {
+__device__ __forceinline__ void scan_onepass_run(const In& in, uint32_t* __restrict__ out, DevN dn,
+                                                 const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                 uint32_t* __restrict__ row_end, unsigned long long* __restrict__ st,
+                                                 int64_t ntiles, int spin) {
     __shared__ uint64_t sm[kBlock / kWave + 1];
     __shared__ int64_t s_tile;
     __shared__ uint64_t s_excl;
@@ -442,6 +461,58 @@ __global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ o
             scan_flag_store(&st[1], 0ull);
         }
     }
+}
+
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_scan(In in, uint32_t* __restrict__ out, DevN dn,
+                                                  const uint32_t* __restrict__ gate, uint64_t* __restrict__ total,
+                                                  uint32_t* __restrict__ row_end, unsigned long long* __restrict__ st,
+                                                  int64_t ntiles, int spin) {
+    scan_onepass_run(in, out, dn, gate, total, row_end, st, ntiles, spin);
+}
+
+// Two independent scans in one launch (a pair of the kernels above): the
+// first a.ntiles blocks run scan a, the rest scan b, each job with its own
+// tickets and flags (one pass) or tile sums (three passes).
+template <class In>
+struct ScanJob {
+    In in;
+    uint32_t* out;
+    DevN dn;
+    const uint32_t* gate;
+    uint64_t* total;
+    uint32_t* row_end;
+    unsigned long long* st;
+    uint64_t* partial;
+    int64_t ntiles;
+};
+
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock) k_scan_pair(ScanJob<A> a, ScanJob<B> b, int spin) {
+    if ((int64_t)blockIdx.x < a.ntiles)
+        scan_onepass_run(a.in, a.out, a.dn, a.gate, a.total, a.row_end, a.st, a.ntiles, spin);
+    else
+        scan_onepass_run(b.in, b.out, b.dn, b.gate, b.total, b.row_end, b.st, b.ntiles, spin);
+}
+
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce_pair(ScanJob<A> a, ScanJob<B> b) {
+    const int64_t t = blockIdx.x;
+    if (t < a.ntiles) scan_reduce_tile(a.in, a.dn, a.gate, a.partial, t);
+    else scan_reduce_tile(b.in, b.dn, b.gate, b.partial, t - a.ntiles);
+}
+
+template <class A, class B>
+__global__ void __launch_bounds__(1024) k_scan_partials_pair(ScanJob<A> a, ScanJob<B> b) {
+    if (blockIdx.x == 0) scan_partials_run(a.partial, a.ntiles, a.gate, a.total, a.row_end, a.dn);
+    else scan_partials_run(b.partial, b.ntiles, b.gate, b.total, b.row_end, b.dn);
+}
+
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock) k_scan_down_pair(ScanJob<A> a, ScanJob<B> b) {
+    const int64_t t = blockIdx.x;
+    if (t < a.ntiles) scan_down_tile(a.in, a.out, a.dn, a.gate, a.partial, t);
+    else scan_down_tile(b.in, b.out, b.dn, b.gate, b.partial, t - a.ntiles);
 }
 
 // The whole scan in one block, for short inputs (one launch, no state):
