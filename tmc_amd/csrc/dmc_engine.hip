@@ -1030,15 +1030,14 @@ void gather_vertices(dmc_context* ctx) {
     ctx->slots = false;
 }
 
-// The carry of a compaction, first half: fresh counters, the face map, the
-// changed-vertex flags and per-origin counts cleared, and the buffers of the
-// carried build (the vertex map kernel then carries the kept vertices'
-// out-degree and he_v).  S->nn and S->qn must be on the device; `last`: the
-// compaction before the final build (a batch counts per volume).
+// The carry of a compaction, first half: the face map and the buffers of the
+// carried build (the vertex map kernel then opens the carry's counters,
+// clears its flags and counts -- carry_clears -- and carries the kept
+// vertices' out-degree and he_v).  S->nn and S->qn must be on the device;
+// `last`: the compaction before the final build (a batch counts per volume).
 Carry carry_begin(dmc_context* ctx, const uint32_t* newq, bool last) {
     const int64_t vcap = ctx->vcap, qcap = ctx->qcap, hcap = 4 * qcap;
     Sizes* S = dsizes(ctx);
-    const DevN nn = dn(&S->nn, vcap);
     Carry c{};
     c.twin = static_cast<const int32_t*>(ctx->twin.p);
     c.flag = static_cast<const uint8_t*>(ctx->flag.p);
@@ -1062,10 +1061,14 @@ Carry carry_begin(dmc_context* ctx, const uint32_t* newq, bool last) {
         c.vst = ensure<unsigned long long>(ctx, ctx->vhe, 3 * (size_t)ctx->nvol);
         fl.u32(c.vst, hn(6 * (int64_t)ctx->nvol), 0u);
     }
-    // whole words: the marks are set with word atomics
-    fl.u32(cflag, DevN{nullptr, (vcap + 4) / 4}, 0u).u32(c.arr, nn, 0u).u32(c.cnt, nn, 0u).u32(c.bcount, hn(turns), 0u);
     fl.run();
     return c;
+}
+
+// What k_vmap* clears for the carry begun by carry_begin.
+CarryClears carry_clears(dmc_context* ctx, const Carry& c) {
+    return CarryClears{reinterpret_cast<uint32_t*>(const_cast<uint8_t*>(c.cflag)), (ctx->vcap + 4) / 4, c.arr, c.cnt,
+                       c.bcount, ctx->qcap / kBlock + 1};
 }
 
 // Second half, after the compaction kernel: the flagged halfedges paired
@@ -1150,7 +1153,7 @@ void simplify3x3y(dmc_context* ctx, bool colored, bool last) {
     Carry c = carry_begin(ctx, newQ, last);
     launch(ctx, "k_vmap3x3y", k_vmap3x3y, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vfirst,
            (const int32_t*)vsecond, S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
-           static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n);
+           static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n, carry_clears(ctx, c));
     launch(ctx, "k_changed3x3y", k_changed3x3y, dim3(gq), dim3(kBlock), quads, (const int32_t*)mlist,
            (const Sizes*)S, (const uint8_t*)em, (const int32_t*)vmap, static_cast<const int32_t*>(ctx->hev.p),
            const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);
@@ -1202,7 +1205,7 @@ void simplify3333(dmc_context* ctx, bool last) {
     Carry c = carry_begin(ctx, newQ, last);
     launch(ctx, "k_vmap3333", k_vmap3333, dim3(grid_for(ctx, vcap)), dim3(kBlock), (const uint8_t*)vflag,
            (const uint32_t*)newv, S, vmap, static_cast<const uint32_t*>(ctx->deg.p),
-           static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n);
+           static_cast<const int32_t*>(ctx->hev.p), c.qmap, c.deg_n, c.hev_n, carry_clears(ctx, c));
     launch(ctx, "k_changed3333", k_changed3333, dim3(gq), dim3(kBlock), quads, (const int32_t*)dlist,
            (const Sizes*)S, (const int32_t*)vmap, const_cast<uint8_t*>(c.cflag), c.deg_n, c.hev_n);
     launch(ctx, "k_compact_quads3333", k_compact_quads3333, dim3(gq), dim3(kBlock), quads, (const Sizes*)S,

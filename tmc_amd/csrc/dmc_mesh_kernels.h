@@ -823,9 +823,6 @@ __global__ void __launch_bounds__(kBlock) k_twin3333(const int32_t* __restrict__
     }
 }
 
-
-// 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
-// compaction), vsecond[vsecond[x]] for a removed one (its target's).
 // Opens a compaction's carry (thread 0 of the vertex-map kernel, whose other
 // threads do not read these): whether the build it starts from met invalid
 // vertex references, and fresh counters for the carried build.
@@ -853,6 +850,30 @@ __device__ __forceinline__ void carry_vertex(int64_t x, int32_t y, int64_t nn, c
     hev_n[y] = m < 0 ? -1 : 4 * m + (e & 3);
 }
 
+// The zeroed state of a carry, cleared by the vertex map kernel that starts
+// it (which reads none of it): the changed-vertex marks (whole words, set
+// with word atomics), the new vertices' arrivals and per-origin counts, and
+// the per-turn counts of the compaction.
+struct CarryClears {
+    uint32_t* cflag;
+    int64_t cwords;
+    uint32_t* arr;
+    uint32_t* cnt;
+    uint32_t* bcount;
+    int64_t turns;
+};
+
+__device__ __forceinline__ void carry_clear(const CarryClears& z, int64_t nn) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t i = t0; i < z.cwords; i += stride) z.cflag[i] = 0u;
+    for (int64_t i = t0; i < nn; i += stride) {
+        z.arr[i] = 0u;
+        z.cnt[i] = 0u;
+    }
+    for (int64_t i = t0; i < z.turns; i += stride) z.bcount[i] = 0u;
+}
+
 // 3X3Y: vmap[x] = vsecond[x] for a kept x (its new id after the vertex
 // compaction), vsecond[vsecond[x]] for a removed one (its target's); and the
 // kept vertices' carry.
@@ -861,9 +882,10 @@ __global__ void __launch_bounds__(kBlock) k_vmap3x3y(const uint8_t* __restrict__
                                                       int32_t* __restrict__ vmap, const uint32_t* __restrict__ deg,
                                                       const int32_t* __restrict__ hev,
                                                       const int32_t* __restrict__ qmap, uint32_t* __restrict__ deg_n,
-                                                      int32_t* __restrict__ hev_n) {
+                                                      int32_t* __restrict__ hev_n, CarryClears z) {
     const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
     open_carry(S);
+    carry_clear(z, nn);
     TMC_GRID_STRIDE(x, nv) {
         int32_t y = vsecond[x];
         const bool removed = vfirst[x] != 0;
@@ -880,9 +902,10 @@ __global__ void __launch_bounds__(kBlock) k_vmap3333(const uint8_t* __restrict__
                                                       int32_t* __restrict__ vmap, const uint32_t* __restrict__ deg,
                                                       const int32_t* __restrict__ hev,
                                                       const int32_t* __restrict__ qmap, uint32_t* __restrict__ deg_n,
-                                                      int32_t* __restrict__ hev_n) {
+                                                      int32_t* __restrict__ hev_n, CarryClears z) {
     const int64_t nv = (int64_t)S->nv, nn = (int64_t)S->nn;
     open_carry(S);
+    carry_clear(z, nn);
     TMC_GRID_STRIDE(x, nv) {
         const bool removed = vflag[x] != 0;
         const int32_t y = removed ? -1 : (int32_t)newv[x];
