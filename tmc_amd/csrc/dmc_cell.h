@@ -288,6 +288,9 @@ __device__ __forceinline__ void proj_row(const CornerRef& Fc, uint32_t cpk, floa
 // is not negative), and a_i grows away from p on each side.  A sample with
 // g1 == g2 gives w = +-inf or NaN, which no [wmin, wmax] test accepts,
 // exactly as the reference's skip of that sample.
+// kUnroll: how many of a row's 7 samples the compiler interleaves (7: the
+// most instruction-level parallelism, 1: the fewest registers).
+template <int kUnroll>
 __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
                                    double iso, int t, double p[3]) {
     constexpr int ns = 9;
@@ -325,7 +328,7 @@ __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxR
             const double ai = (p[0] - u) * (p[0] - u);
             double u01, u23, u45, u67;
             proj_row(Fc, cpk, u, u01, u23, u45, u67);
-#pragma unroll
+#pragma unroll kUnroll
             for (int j = 1; j <= kIn; j++) {
                 const float v = vj[j - 1];
                 const float g1 = (1 - v) * u01 + v * u23;
@@ -506,11 +509,12 @@ __device__ __forceinline__ bool cell_needs_box_test(uint64_t w, int cnt) {
 // Representative of contour t of a cell without the box test, from the 9 x 9
 // sampling; false when that sampling finds no point (the caller then takes
 // the general path, which also samples at 28 x 28).
+template <int kUnroll>
 __device__ inline bool contour_representative9(const CornerRef& Fc, double iso, uint64_t w, int t, const BoxRef& B,
                                                double p[3]) {
     double pt[3];
     const ContourFrame f = contour_frame(Fc, iso, w, t, B, p, pt);
-    if (!projection9(Fc, f.cpk, B, f.axa, f.axb, f.axc, iso, t, pt)) return false;
+    if (!projection9<kUnroll>(Fc, f.cpk, B, f.axa, f.axb, f.axc, iso, t, pt)) return false;
     contour_unframe(f, pt, p);
     return true;
 }

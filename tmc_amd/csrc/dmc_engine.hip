@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -788,9 +789,16 @@ void run_cells(dmc_context* ctx, const SweepPlan& P) {
     int2* adj = ensure<int2>(ctx, ctx->adj, 4 * qcap);
     const DevN A = dn(&S->acnt, acap);
     mark(ctx, "vertices");
-    // persistent: a few rounds of the blocks the chip holds at once
-    launch(ctx, "k_vertices", k_vertices<T>, dim3(grid_for(ctx, acap, kBlock, kVerticesBlocks)), dim3(kBlock), vt,
-           acell, acode, aaux, avloc, rowBaseV, A, pos, nrm, deg, slow, &S->qhead);
+    // persistent: a few rounds of the blocks the chip holds at once; the
+    // 3-wave form once there are more cells than one round of the grid's
+    // threads (F32 fields: the F64 form would spill at 3 waves)
+    const dim3 gv(grid_for(ctx, acap, kBlock, kVerticesBlocks));
+    if (std::is_same<T, float>::value && acap > kVerticesBlocks * kBlock)
+        launch(ctx, "k_vertices", k_vertices_wide<T>, gv, dim3(kBlock), vt, acell, acode, aaux, avloc, rowBaseV, A,
+               pos, nrm, deg, slow, &S->qhead);
+    else
+        launch(ctx, "k_vertices", k_vertices<T>, gv, dim3(kBlock), vt, acell, acode, aaux, avloc, rowBaseV, A, pos,
+               nrm, deg, slow, &S->qhead);
     check_launch(ctx, "k_vertices");
     launch(ctx, "k_vertices_slow", k_vertices_slow<T>, dim3(grid_for(ctx, acap, kBlock, 1024)), dim3(kBlock), vt,
            acell, acode, aaux, avloc, rowBaseV, slow, pos, nrm, deg, sw.vstat);

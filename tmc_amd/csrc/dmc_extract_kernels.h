@@ -654,15 +654,14 @@ __device__ __forceinline__ void store_vertex(const VolDesc& V, int i, int j, int
     vdeg[out] = (uint32_t)contour_rank(code >> 8, aux, code & 0xFFu, (unsigned)t, 12);
 }
 
-template <class T>
-__global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
-                                                      const uint32_t* __restrict__ acode,
-                                                      const uint32_t* __restrict__ aaux,
-                                                      const uint32_t* __restrict__ avloc,
-                                                      const uint32_t* __restrict__ rowBaseV, DevN dA,
-                                                      double* __restrict__ vpos, double* __restrict__ vnrm,
-                                                      uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow,
-                                                      unsigned long long* qhead) {
+template <class T, int kUnroll>
+__device__ __forceinline__ void vertices_run(const VolTable& vt, const uint32_t* __restrict__ acell,
+                                             const uint32_t* __restrict__ acode, const uint32_t* __restrict__ aaux,
+                                             const uint32_t* __restrict__ avloc,
+                                             const uint32_t* __restrict__ rowBaseV, DevN dA,
+                                             double* __restrict__ vpos, double* __restrict__ vnrm,
+                                             uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow,
+                                             unsigned long long* qhead) {
     static_assert(kBlock == kBoxStride && kBlock == kCornerStride, "LDS columns are one per thread");
     const int64_t A = dA.get();
     __shared__ float boxes[kBoxFloats * kBoxStride];
@@ -708,7 +707,7 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
             uint32_t out = rowBaseV[V.rowBase + r] + avloc[a];
             for (int t = 0; t < cnt; t++) {
                 double p[3];
-                if (!contour_representative9(Fc, iso, w, t, B, p)) {
+                if (!contour_representative9<kUnroll>(Fc, iso, w, t, B, p)) {
                     defer = true;
                     break;
                 }
@@ -729,6 +728,36 @@ __global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable 
         }
         }
     }
+}
+
+// K3 for passes of up to one round of the persistent grid's threads per
+// active cell: a row's samples interleaved (the latency of one cell matters
+// most), 2 waves per SIMD.
+template <class T>
+__global__ void __launch_bounds__(kBlock) TMC_VERTICES_ATTR k_vertices(VolTable vt, const uint32_t* __restrict__ acell,
+                                                      const uint32_t* __restrict__ acode,
+                                                      const uint32_t* __restrict__ aaux,
+                                                      const uint32_t* __restrict__ avloc,
+                                                      const uint32_t* __restrict__ rowBaseV, DevN dA,
+                                                      double* __restrict__ vpos, double* __restrict__ vnrm,
+                                                      uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow,
+                                                      unsigned long long* qhead) {
+    vertices_run<T, 7>(vt, acell, acode, aaux, avloc, rowBaseV, dA, vpos, vnrm, vdeg, slow, qhead);
+}
+
+// K3 for larger passes: samples one at a time, which fits 3 waves per SIMD
+// (166 VGPRs for F32 fields, no scratch), so more cells hide each other's
+// memory waits (1024^3 Schwarz-P: 20.25 -> 19.85 ms per extraction; one
+// 128^3 volume: 0.401 -> 0.407 ms, hence the two forms).  The same
+// arithmetic in the same order as k_vertices.
+template <class T>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_vertices_wide(VolTable vt, const uint32_t* __restrict__ acell, const uint32_t* __restrict__ acode,
+                const uint32_t* __restrict__ aaux, const uint32_t* __restrict__ avloc,
+                const uint32_t* __restrict__ rowBaseV, DevN dA, double* __restrict__ vpos,
+                double* __restrict__ vnrm, uint32_t* __restrict__ vdeg, uint32_t* __restrict__ slow,
+                unsigned long long* qhead) {
+    vertices_run<T, 1>(vt, acell, acode, aaux, avloc, rowBaseV, dA, vpos, vnrm, vdeg, slow, qhead);
 }
 
 // General path of K3 for the cells k_vertices deferred: every contour is
