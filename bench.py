@@ -39,8 +39,9 @@ HBM_PEAK_GBS = 8000.0
 # (HBM and Infinity Cache), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # scaled by the copy-kernel calibration
 # (profiles/dmc_schwarz_1024_kernel_traffic.md): k_sweep 5.74 GB, k_rows
-# 1.15 GB, k_vertices 8.91 GB, k_quads 8.70 GB, the cell scans about 0.1 GB.
-TRAFFIC = {"schwarz_1024": 24_500_000_000}
+# 1.14 GB, k_vertices (its 3-wave form) 11.46 GB, k_quads 8.27 GB, the cell
+# scans 0.47 GB.
+TRAFFIC = {"schwarz_1024": 27_100_000_000}
 CELL_KERNELS = ("k_sweep", "k_scan_cells", "k_vol_abase", "k_rows", "k_vol_bounds", "k_vertices", "k_vertices_slow",
                 "k_quads")
 
