@@ -368,26 +368,49 @@ __device__ inline bool projection9(const CornerRef& Fc, uint32_t cpk, const BoxR
     return found;
 }
 
-// General sampling with ns x ns samples and the neighbour-box test.
+// General sampling with ns x ns samples and the neighbour-box test, in the
+// order of projection9: rows nearest to p first, outwards on both sides, a
+// side stopping at the first row that cannot hold a winner, and the best
+// sample kept as one (distance bits, index) key so that ties resolve as in
+// (i, j) order.  The box test only removes samples, so float(a_i + min_j
+// b_j) stays a lower bound of every distance row i can contribute.  The
+// distance is the reference's expression, ((p0 - u)^2 + (p1 - v)^2) +
+// (p2 - w)^2 in double, rounded to float.
 __device__ inline bool projection(const CornerRef& Fc, uint32_t cpk, const BoxRef& B, int axa, int axb, int axc,
                                   double iso, int t, unsigned nbmask, double p[3], int ns) {
     const ProjFrame f = proj_frame(B, axa, axb, axc, t, ns);
-    double minDistance = 100;
-    float bu = 0.f, bv = 0.f, bw = -1.f;
-    bool found = false;
+    const int kIn = ns - 2;   // interior samples per axis: i, j = 1 .. ns - 2
+    double bmin = __builtin_huge_val();
 #pragma unroll 1
-    for (int i = 1; i < (ns - 1); i++) {
+    for (int j = 1; j <= kIn; j++) {
+        const float v = f.vmin + j * f.dv;
+        const double b = (p[1] - v) * (p[1] - v);
+        bmin = b < bmin ? b : bmin;
+    }
+    auto row_bound = [&](int i) {
         const float u = f.umin + i * f.du;
-        double u01, u23, u45, u67;
-        proj_row(Fc, cpk, u, u01, u23, u45, u67);
+        return (double)(float)((p[0] - u) * (p[0] - u) + bmin);
+    };
+    const float x = fminf(fmaxf((float)((p[0] - f.umin) / f.du) + 0.5f, 1.f), (float)kIn);
+    int cur = (int)x, lo = cur - 1, hi = cur + 1;
+    const uint64_t start = (uint64_t)__float_as_uint(100.f) << 32;
+    uint64_t bestKey = start;
+    float bu = 0.f, bv = 0.f, bw = -1.f;
+    while (__any(cur >= 0)) {
+        if (cur >= 0) {
+            const int i = cur;
+            const float u = f.umin + i * f.du;
+            const double ai = (p[0] - u) * (p[0] - u);
+            double u01, u23, u45, u67;
+            proj_row(Fc, cpk, u, u01, u23, u45, u67);
 #pragma unroll 1
-        for (int j = 1; j < (ns - 1); j++) {
-            const float v = f.vmin + j * f.dv;
-            const float g1 = (1 - v) * u01 + v * u23;
-            const float g2 = (1 - v) * u45 + v * u67;
-            if (g1 == g2) continue;
-            const float ww = (iso - g1) / (g2 - g1);
-            if (f.wmin <= ww && ww <= f.wmax) {
+            for (int j = 1; j <= kIn; j++) {
+                const float v = f.vmin + j * f.dv;
+                const float g1 = (1 - v) * u01 + v * u23;
+                const float g2 = (1 - v) * u45 + v * u67;
+                // g1 == g2: w is +-inf or NaN, which the range test rejects
+                const float ww = (iso - g1) / (g2 - g1);
+                if (!(f.wmin <= ww && ww <= f.wmax)) continue;
                 bool inside = false;
                 if (nbmask) {
                     const double du_ = u, dv_ = v, dw_ = ww;
@@ -399,19 +422,32 @@ __device__ inline bool projection(const CornerRef& Fc, uint32_t cpk, const BoxRe
                         inside = true;
                     }
                 }
-                if (!inside) {
-                    const float d = (p[0] - u) * (p[0] - u) + (p[1] - v) * (p[1] - v) + (p[2] - ww) * (p[2] - ww);
-                    if (minDistance > d) {
-                        minDistance = d;
-                        bu = u;
-                        bv = v;
-                        bw = ww;
-                        found = true;
-                    }
+                if (inside) continue;
+                const double s = ai + (p[1] - v) * (p[1] - v);
+                const float d = s + (p[2] - ww) * (p[2] - ww);
+                const uint64_t key = (uint64_t)__float_as_uint(d) << 32 | (uint32_t)(ns * i + j);
+                if (key < bestKey) {
+                    bestKey = key;
+                    bu = u;
+                    bv = v;
+                    bw = ww;
                 }
             }
+            const double best = (double)__uint_as_float((uint32_t)(bestKey >> 32));
+            double bl = 0, bh = 0;
+            if (lo >= 1) {
+                bl = row_bound(lo);
+                if (bl > best) lo = 0;
+            }
+            if (hi <= kIn) {
+                bh = row_bound(hi);
+                if (bh > best) hi = kIn + 1;
+            }
+            const bool okl = lo >= 1, okh = hi <= kIn;
+            cur = (okl && (!okh || bl <= bh)) ? lo-- : okh ? hi++ : -1;
         }
     }
+    const bool found = bestKey != start;
     if (found) {
         p[0] = bu;
         p[1] = bv;
